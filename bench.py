@@ -1,0 +1,248 @@
+"""Forward+adjoint evaluations per second at 2M cells, mixed precision vs FP64.
+
+One step = one gradient evaluation of the inverse-problem objective on one
+GPU, every array resident in HBM before the timed region starts:
+
+  logK (2M cells) -> K_e = hm(exp(logK_a), exp(logK_b))          (6M edges)
+  -> steady solve h (assembly, numeric AMG refresh, AMG-PCG to 1e-10)
+  -> J = sum_obs (h - h_obs)^2 over 200 cells, delta = dJ/dh
+  -> adjoint solve through the same hierarchy -> dJ/dK_e -> dJ/dlogK.
+
+Workload: 200x200x50 cells over 100x100x25 m, layered Matern(nu=1) logK with
+correlation length 50 m, sigma 1, 4 injectors of 1e-4 m^3/s, Dirichlet h = 0
+on the top face; observed heads from a second field plus N(0, 1e-3^2) noise.
+Steps cycle through 4 pre-generated logK fields so no step reuses an earlier
+result.  The FP64 path is timed on the same steps for the ratio.
+
+Prints one JSON line (the driver's contract).  ``--gpus N`` with N > 1 exits:
+this build runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
+    ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU (asked for --gpus {args.gpus}); nothing in this build "
+              "uses more than one GPU.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from __graft_entry__ import load_package
+    P = load_package()
+    from dpfehm_jl_amd import _lib, api
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    ns = tuple(args.nx)
+    maxs = (100.0 * (ns[0] - 1) / 199.0, 100.0 * (ns[1] - 1) / 199.0, 25.0 * (ns[2] - 1) / 49.0)
+    t0 = time.time()
+    w = P.workloads.steady_2m(ns=ns, maxs=maxs)
+    log(f"[bench] problem n={w.n} E={w.nedges} built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    topo_s = time.time() - t0
+    log(f"[bench] topology: nfree={topo.nfree} nnz={topo.nnz} levels={topo.levels} "
+        f"coarse={topo.coarse_rows} opcx={topo.operator_complexity:.2f} "
+        f"host symbolic setup {topo.symbolic_setup_ms:.0f} ms (create {topo_s:.1f}s)")
+
+    stream = torch.cuda.current_stream(dev)
+    sptr = ctypes.c_void_p(stream.cuda_stream)
+    L = _lib.lib()
+    sp = grid_sp = P.grid.grid_spacing(w.mins, w.maxs, w.ns)
+    fields = [w.logK] + [P.fields.layered_logK(w.ns, sp, -9.0, 50.0, 1.0, s) for s in (10, 11, 12)]
+    logK_d = [torch.from_numpy(f).to(dev) for f in fields]
+    dh_d = torch.from_numpy(w.dirichleths).to(dev)
+    Q_d = torch.from_numpy(w.Qs).to(dev)
+    obs_d = torch.from_numpy(w.obs - 1).to(dev)
+    Ke_d = torch.empty(w.nedges, dtype=torch.float64, device=dev)
+    h_d = torch.empty(w.n, dtype=torch.float64, device=dev)
+    delta_d = torch.zeros(w.n, dtype=torch.float64, device=dev)
+    dK_d = torch.empty(w.nedges, dtype=torch.float64, device=dev)
+    dlogK_d = torch.empty(w.n, dtype=torch.float64, device=dev)
+
+    def opts(precision):
+        return api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
+                         stream=sptr)
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def forward(logK, o, keep=True):
+        _lib.check(L.dpf_hm_forward(topo.handle, p(logK), p(Ke_d), ctypes.byref(o)))
+        st = _lib.Stats()
+        sh = ctypes.c_void_p()
+        _lib.check(L.dpf_steady_forward(topo.handle, p(Ke_d), p(dh_d), p(Q_d), p(h_d),
+                                        ctypes.byref(o), ctypes.byref(st),
+                                        ctypes.byref(sh) if keep else None))
+        return st, sh
+
+    # observed heads: FP64 solve on the observation field + noise (set-up, untimed)
+    lk_obs = torch.from_numpy(P.workloads.observation_logK(w, seed=4)).to(dev)
+    forward(lk_obs, opts("fp64"), keep=False)
+    h_obs_d = h_d[obs_d] + torch.from_numpy(P.workloads.observation_noise(w.obs.size)).to(dev)
+
+    def evaluate(k, o, stats):
+        logK = logK_d[k % len(logK_d)]
+        st, sh = forward(logK, o)
+        r = h_d[obs_d] - h_obs_d
+        J = torch.sum(r * r)
+        delta_d.zero_()
+        delta_d[obs_d] = 2.0 * r
+        st2 = _lib.Stats()
+        code = L.dpf_steady_pullback(sh, p(delta_d), p(dK_d), None, None, ctypes.byref(o),
+                                     ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        _lib.check(L.dpf_hm_pullback(topo.handle, p(logK), p(Ke_d), p(dK_d), p(dlogK_d),
+                                     ctypes.byref(o)))
+        stats.append((st.as_dict(), st2.as_dict()))
+        return J
+
+    results = {}
+    for precision in ("fp64", "mixed"):
+        o = opts(precision)
+        for k in range(args.warmup):
+            evaluate(k, o, [])
+        torch.cuda.synchronize()
+        stats = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            evaluate(k, o, stats)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        results[precision] = dict(seconds=dt, stats=stats)
+        its = [(a["iterations"], b["iterations"]) for a, b in stats]
+        log(f"[bench] {precision}: {args.steps / dt:.2f} evals/s ({1e3 * dt / args.steps:.1f} ms/eval) "
+            f"iterations fwd/adj {its[:4]} setup {stats[-1][0]['setup_ms']:.2f} ms "
+            f"relres {stats[-1][0]['relres']:.2e}/{stats[-1][1]['relres']:.2e}")
+
+    # accuracy of the mixed path against FP64 on the same field (untimed)
+    acc = {}
+    for precision in ("fp64", "mixed"):
+        evaluate(0, opts(precision), [])
+        torch.cuda.synchronize()
+        acc[precision] = (h_d.cpu().numpy().copy(), dlogK_d.cpu().numpy().copy())
+    head_err = float(np.linalg.norm(acc["mixed"][0] - acc["fp64"][0]) / np.linalg.norm(acc["fp64"][0]))
+    grad_err = float(np.linalg.norm(acc["mixed"][1] - acc["fp64"][1]) / np.linalg.norm(acc["fp64"][1]))
+    log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}")
+
+    # roofline of the dominant kernel: finest-level FP32 Chebyshev sweep of the
+    # mixed path, HIP events around back-to-back launches on the bench stream
+    prof = {}
+    for precision, pc in (("mixed", _lib.DPF_MIXED), ("fp64", _lib.DPF_FP64)):
+        if precision == "fp64":
+            evaluate(1, opts("fp64"), [])
+        else:
+            evaluate(1, opts("mixed"), [])
+        for kern in (0, 1):
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            o = opts(precision)
+            _lib.check(L.dpf_profile_kernel(topo.handle, pc, kern, 50, ctypes.byref(ms),
+                                            ctypes.byref(by), ctypes.byref(o)))
+            prof[(precision, kern)] = (ms.value, by.value)
+            log(f"[bench] kernel {kern} {precision}: {ms.value * 1e3:.1f} us/launch, "
+                f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e9:.0f} GB/s")
+    ms, by = prof[("mixed", 0)]
+    achieved = by / (ms * 1e-3) / 1e9
+
+    value = args.steps / results["mixed"]["seconds"]
+    fp64_rate = args.steps / results["fp64"]["seconds"]
+
+    cpu = None
+    if args.cpu_baseline == "auto":
+        cpu = cpu_baseline(w, fields[0], h_obs_d.cpu().numpy(), args.cpu_seconds)
+
+    out = {
+        "metric": "forward+adjoint evals/s @2M cells, mixed vs FP64 GPU, <=1e-8 rel head err",
+        "value": round(value, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * results["mixed"]["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": f"steady groundwater forward+adjoint, {ns[0]}x{ns[1]}x{ns[2]} cells "
+                        f"({w.n} cells, {w.nedges} edges), harmonic-mean K from layered "
+                        "Matern(nu=1, l=50 m, sigma=1) logK, top-face Dirichlet, 4 injectors, "
+                        "200 observations, AMG-PCG to 1e-10, mixed precision",
+            "cells": w.n, "edges": w.nedges, "nfree": topo.nfree, "nnz": topo.nnz,
+            "amg_levels": topo.levels, "reltol": 1e-10, "inner_reltol": 1e-5,
+        },
+        "fp64_evals_per_s": round(fp64_rate, 3),
+        "mixed_over_fp64": round(value / fp64_rate, 3),
+        "head_rel_err_mixed_vs_fp64": head_err,
+        "grad_rel_err_mixed_vs_fp64": grad_err,
+        "iterations_mixed_fwd_adj": [results["mixed"]["stats"][0][0]["iterations"],
+                                     results["mixed"]["stats"][0][1]["iterations"]],
+        "iterations_fp64_fwd_adj": [results["fp64"]["stats"][0][0]["iterations"],
+                                    results["fp64"]["stats"][0][1]["iterations"]],
+        "topology_setup_s": round(topo_s, 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",
+            "achieved": round(achieved, 1),
+            "peak": 8000.0,
+            "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4),
+            "traffic": None,
+            "us_per_launch": round(ms * 1e3, 2),
+            "bytes_per_launch": by,
+        },
+        "kernels": {f"{pr}/{k}": {"us": round(v[0] * 1e3, 2), "GBps": round(v[1] / v[0] / 1e6, 1)}
+                    for (pr, k), v in prof.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(w, logK, h_obs, budget_s):
+    """The FP64 oracle (oracle/gw_oracle.c: IC(0)-PCG, one core) on the same
+    workload.  Sample: the K_e map and one full forward solve are timed; the
+    adjoint solve has the same matrix, tolerance and cost, so one
+    forward+adjoint evaluation is taken as map + 2 x forward solve."""
+    from oracle import gw_oracle as O
+    t0 = time.perf_counter()
+    K = O.hm_forward(logK, w.neighbors)
+    t1 = time.perf_counter()
+    h, hf, info = O.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                            w.dirichleths, w.Qs, reltol=1e-10, return_info=True)
+    t2 = time.perf_counter()
+    ev = (t1 - t0) + 2.0 * (t2 - t1)
+    log(f"[bench] cpu oracle: K map {t1 - t0:.2f}s, forward solve {t2 - t1:.1f}s "
+        f"({info['iterations']} IC(0)-PCG iterations) -> {ev:.1f}s per forward+adjoint eval")
+    return {"value": round(1.0 / ev, 5), "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": "K_e map plus one full forward solve of the same 2M-cell workload by the "
+                      "FP64 C oracle (IC(0)-PCG to 1e-10, one thread); the adjoint solve is "
+                      "counted as a second forward solve of equal cost"}
+
+
+if __name__ == "__main__":
+    main()

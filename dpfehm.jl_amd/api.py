@@ -1,0 +1,275 @@
+"""DPFEHM's groundwater interface on the GPU (host side, Python).
+
+Same names, argument meaning and error behaviour as the reference's Julia
+functions, with 1-based ``neighbors`` (an ``(E, 2)`` array of pairs) and
+``dirichletnodes``:
+
+* ``groundwater_steadystate``  -- src/groundwater.jl:75-83
+* ``rrule_groundwater_steadystate`` -- the ChainRulesCore.rrule of
+  src/groundwater.jl:85-109: returns ``(h, pullback)``, ``pullback(delta)``
+  returns the 7-tuple ``(NoTangent, dKs, NoTangent, NoTangent, NoTangent,
+  ddirichleths, dQs)``
+* ``groundwater_residuals``    -- src/groundwater.jl:1-38
+* ``hm``, ``logKs2Ks_harmonic`` and its pullback -- DPFEHM.hm
+  (src/richards.jl:50-52) applied per edge as the configs' logK -> K map
+
+Every call goes through libdpfgpu.so (HIP kernels on the GPU).  The topology
+(free-cell maps, Jacobian pattern, AMG symbolic setup) is cached per
+``(n, neighbors, areasoverlengths, dirichletnodes)`` content, as the reference
+recomputes it in every call.
+
+Keyword arguments: ``reltol`` (default 1e-10), ``abstol``, ``maxiter``,
+``precision`` ("fp64" or "mixed"), ``inner_reltol``, ``max_outer``,
+``cheb_degree``.  ``linear_solver`` is accepted and ignored, as the
+reference's rrule ignores it (src/groundwater.jl:90).  Unknown keywords raise
+TypeError.  Non-convergence raises ``NotConverged`` instead of returning
+silently.
+"""
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _lib
+from ._lib import DpfError, NotConverged  # noqa: F401
+
+try:
+    import xxhash
+
+    def _digest(*arrays):
+        h = xxhash.xxh3_128()
+        for a in arrays:
+            h.update(np.ascontiguousarray(a).view(np.uint8))
+        return h.hexdigest()
+except ImportError:  # pragma: no cover
+    import hashlib
+
+    def _digest(*arrays):
+        h = hashlib.blake2b(digest_size=16)
+        for a in arrays:
+            h.update(np.ascontiguousarray(a).view(np.uint8))
+        return h.hexdigest()
+
+
+class _NoTangent:
+    def __repr__(self):
+        return "NoTangent()"
+
+
+NoTangent = _NoTangent()
+
+_OPT_KEYS = {"reltol", "abstol", "maxiter", "precision", "inner_reltol", "max_outer",
+             "cheb_degree"}
+
+
+def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
+    kw = dict(kwargs)
+    kw.pop("linear_solver", None)
+    bad = set(kw) - _OPT_KEYS
+    if bad:
+        raise TypeError(f"unexpected keyword argument(s): {sorted(bad)}")
+    o = _lib.default_opts(**kw)
+    o.memory = memory
+    o.stream = stream
+    return o
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Topology:
+    """A mesh + Dirichlet set cached on the GPU (dpf_topology)."""
+
+    def __init__(self, n, neighbors, areasoverlengths, dirichletnodes):
+        nb = np.ascontiguousarray(np.asarray(neighbors, dtype=np.int64).reshape(-1, 2))
+        ne = nb.shape[0]
+        aol = _f64(np.asarray(areasoverlengths, dtype=np.float64)[:ne])
+        if aol.shape[0] != ne:
+            raise ValueError("areasoverlengths is shorter than neighbors")
+        dn = np.ascontiguousarray(np.asarray(dirichletnodes, dtype=np.int64).ravel())
+        self.n, self.ne = int(n), int(ne)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().dpf_topology_create(self.n, self.ne, _ptr(nb), _ptr(aol),
+                                                  dn.size, _ptr(dn), ctypes.byref(h)))
+        self._h = h
+        info = np.zeros(8, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 8)
+        self.nfree = int(info[2])
+        self.nnz = int(info[3])
+        self.levels = int(info[4])
+        self.coarse_rows = int(info[5])
+        self.operator_complexity = info[6] / 1000.0
+        self.symbolic_setup_ms = info[7] / 1000.0
+        isfree = np.ones(self.n, dtype=bool)
+        isfree[dn - 1] = False
+        self.isfree = isfree
+        self.lock = threading.Lock()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().dpf_topology_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_topo_cache = {}
+_topo_lock = threading.Lock()
+
+
+def get_topology(n, neighbors, areasoverlengths, dirichletnodes):
+    nb = np.asarray(neighbors, dtype=np.int64).reshape(-1, 2)
+    aol = _f64(np.asarray(areasoverlengths, dtype=np.float64)[:nb.shape[0]])
+    dn = np.asarray(dirichletnodes, dtype=np.int64).ravel()
+    key = (int(n), nb.shape[0], dn.size, _digest(nb, aol, dn))
+    with _topo_lock:
+        t = _topo_cache.get(key)
+        if t is None:
+            if len(_topo_cache) >= 8:
+                _topo_cache.pop(next(iter(_topo_cache))).close()
+            t = Topology(n, nb, aol, dn)
+            _topo_cache[key] = t
+        return t
+
+
+def clear_cache():
+    with _topo_lock:
+        for t in _topo_cache.values():
+            t.close()
+        _topo_cache.clear()
+
+
+class State:
+    """What a pullback needs from its forward solve (dpf_state)."""
+
+    def __init__(self, topo, handle):
+        self.topo = topo
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().dpf_state_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                            linear_solver=None, stats=None, **kwargs):
+    """src/groundwater.jl:75-83.  Returns h (n,)."""
+    h, _ = _steady(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs, False,
+                   stats, kwargs)
+    return h
+
+
+def _steady(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs, keep, stats,
+            kwargs):
+    o = _opts(kwargs)
+    Qs = _f64(Qs)
+    n = Qs.shape[0]
+    topo = get_topology(n, neighbors, areasoverlengths, dirichletnodes)
+    Ks, dh = _f64(Ks), _f64(dirichleths)
+    if Ks.shape[0] != topo.ne or dh.shape[0] != n:
+        raise ValueError("Ks must have one entry per edge and dirichleths one per cell")
+    h = np.empty(n)
+    st = _lib.Stats()
+    sh = ctypes.c_void_p()
+    code = _lib.lib().dpf_steady_forward(topo.handle, _ptr(Ks), _ptr(dh), _ptr(Qs), _ptr(h),
+                                         ctypes.byref(o), ctypes.byref(st),
+                                         ctypes.byref(sh) if keep else None)
+    if stats is not None:
+        stats.update(st.as_dict())
+    _lib.check(code)
+    return h, (State(topo, sh) if keep else None)
+
+
+def rrule_groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
+                                  Qs, stats=None, **kwargs):
+    """src/groundwater.jl:85-109.  Returns (h, pullback)."""
+    h, state = _steady(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs, True,
+                       stats, kwargs)
+    o = _opts(kwargs)
+    n = h.shape[0]
+    ne = state.topo.ne
+
+    def pullback(delta, stats=None):
+        delta = _f64(delta)
+        if delta.shape[0] != n:
+            raise ValueError("delta must have one entry per cell")
+        dK = np.empty(ne)
+        ddh = np.empty(n)
+        dQ = np.empty(n)
+        st = _lib.Stats()
+        code = _lib.lib().dpf_steady_pullback(state._h, _ptr(delta), _ptr(dK), _ptr(ddh),
+                                              _ptr(dQ), ctypes.byref(o), ctypes.byref(st))
+        if stats is not None:
+            stats.update(st.as_dict())
+        _lib.check(code)
+        return (NoTangent, dK, NoTangent, NoTangent, NoTangent, ddh, dQ)
+
+    pullback.state = state
+    return h, pullback
+
+
+def groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                          specificstorage, volumes):
+    """src/groundwater.jl:1-38.  Returns the residuals at the free cells."""
+    Qs = _f64(Qs)
+    n = Qs.shape[0]
+    topo = get_topology(n, neighbors, areasoverlengths, dirichletnodes)
+    h = _f64(h)
+    if h.shape[0] != topo.nfree:
+        raise ValueError("h must have one entry per free cell")
+    r = np.empty(topo.nfree)
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_residuals(topo.handle, _ptr(h), _ptr(_f64(Ks)),
+                                        _ptr(_f64(dirichleths)), _ptr(Qs),
+                                        _ptr(_f64(specificstorage)), _ptr(_f64(volumes)),
+                                        _ptr(r), ctypes.byref(o)))
+    return r
+
+
+def hm(x, y):
+    """DPFEHM.hm (src/richards.jl:50-52), elementwise on host arrays."""
+    return 2.0 / (1.0 / np.asarray(x) + 1.0 / np.asarray(y))
+
+
+def logKs2Ks_harmonic(logKs, neighbors, dirichletnodes=(), areasoverlengths=None):
+    """K_e = hm(exp(logK_a), exp(logK_b)) computed on the GPU."""
+    logKs = _f64(logKs)
+    nb = np.asarray(neighbors, dtype=np.int64).reshape(-1, 2)
+    aol = np.ones(nb.shape[0]) if areasoverlengths is None else areasoverlengths
+    topo = get_topology(logKs.shape[0], nb, aol, dirichletnodes)
+    Ke = np.empty(topo.ne)
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_hm_forward(topo.handle, _ptr(logKs), _ptr(Ke), ctypes.byref(o)))
+    return Ke
+
+
+def logKs2Ks_harmonic_pullback(logKs, Ke, dKe, neighbors, dirichletnodes=(), areasoverlengths=None):
+    logKs = _f64(logKs)
+    nb = np.asarray(neighbors, dtype=np.int64).reshape(-1, 2)
+    aol = np.ones(nb.shape[0]) if areasoverlengths is None else areasoverlengths
+    topo = get_topology(logKs.shape[0], nb, aol, dirichletnodes)
+    out = np.empty(logKs.shape[0])
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_hm_pullback(topo.handle, _ptr(logKs), _ptr(_f64(Ke)),
+                                          _ptr(_f64(dKe)), _ptr(out), ctypes.byref(o)))
+    return out

@@ -1,0 +1,1102 @@
+// libdpfgpu: C-ABI driver of the groundwater solve and adjoint on one MI355X.
+//
+// Per topology (dpf_topology_create): host symbolic setup (dpf_host.h), then
+// every index array is uploaded once.  Per solve: assembly of A and b from
+// the edge conductivities, numeric refresh of the smoothed-aggregation
+// hierarchy (prolongator smoothing, R = P', Galerkin products, dense inverse
+// of the coarsest operator), then CG preconditioned by one V-cycle with
+// l1-Jacobi Chebyshev smoothing.  In mixed precision the hierarchy and the
+// inner CG run in FP32 inside FP64 iterative refinement; the refinement
+// residual, the update, every CG scalar and the coarsest solve stay FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dpfgpu.h"
+#include "dpf_host.h"
+#include "kernels.hpp"
+
+namespace dpf {
+
+static thread_local std::string g_err;
+
+struct Err {
+  int code;
+  std::string msg;
+};
+
+#define HIPTRY(x)                                                                       \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      throw Err{DPF_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)};              \
+  } while (0)
+
+static inline unsigned nblocks(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+// ------------------------------------------------------------------ device buffers
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p && bytes >= b) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) b = 8;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Err{DPF_ENOMEM, "hipMalloc(" + std::to_string(b) + " bytes) failed: " +
+                                hipGetErrorString(e)};
+    }
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+template <typename T>
+static T* upload(DBuf& d, const std::vector<T>& v) {
+  d.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPTRY(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d.as<T>();
+}
+
+struct SellDev {
+  int64_t nrows = 0, ncols = 0, entries = 0;
+  DBuf sptr, col, rowlen;
+  void up(const SellHost& h) {
+    nrows = h.nrows;
+    ncols = h.ncols;
+    entries = h.entries();
+    upload(sptr, h.sptr);
+    upload(col, h.col);
+    upload(rowlen, h.rowlen);
+  }
+  template <typename T>
+  SellD<T> view(T* val) const {
+    return SellD<T>{nrows, sptr.as<int64_t>(), col.as<int32_t>(), val};
+  }
+};
+
+struct LevelDev {
+  SellDev A;
+  DBuf diag_slot;
+  bool transfer = false;
+  SellDev P, R;
+  DBuf p_slot, p_self, r_src, ap_rp, ap_col, ap_pp, ap_map, rap_pp, rap_map;
+  int64_t ap_nnz = 0;
+};
+
+// Numeric values and work vectors of a hierarchy in precision T.
+template <typename T>
+struct LevelNum {
+  DBuf val, invd, pval, rval, apval;
+  DBuf x, xs, b, r, d;
+};
+
+template <typename T>
+struct Hier {
+  bool allocated = false;
+  std::vector<LevelNum<T>> lev;
+  DBuf W;  // dense FP64 inverse of the coarsest operator
+  DBuf status;
+  // CG vectors on the fine level (x, r, p ping-pong, q); z is the V-cycle's
+  // fine-level output.
+  DBuf cx, cr, cp0, cp1, cq;
+  uint64_t version = 0;  // numeric refresh this hierarchy holds
+};
+
+// Chebyshev coefficients for the interval [lmax/ratio, lmax] of the
+// l1-Jacobi-preconditioned operator (lmax = 1 bounds its spectrum).
+struct Cheb {
+  double c0;               // zero-guess first step: d = c0 invd r
+  std::vector<double> c1;  // step k >= 1: d = c1[k] d + c2[k] invd r
+  std::vector<double> c2;
+  void make(int degree, double lmax, double ratio) {
+    double lmin = lmax / ratio;
+    double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    double sigma = theta / delta;
+    c0 = 1.0 / theta;
+    c1.assign(degree, 0.0);
+    c2.assign(degree, 0.0);
+    c2[0] = c0;
+    double rho = 1.0 / sigma;
+    for (int k = 1; k < degree; k++) {
+      double rn = 1.0 / (2.0 * sigma - rho);
+      c1[k] = rn * rho;
+      c2[k] = 2.0 * rn / delta;
+      rho = rn;
+    }
+  }
+};
+
+}  // namespace dpf
+
+using namespace dpf;
+
+struct dpf_topology {
+  TopologyHost H;
+  std::mutex mu;
+  hipStream_t own_stream = nullptr;
+  int device = 0;
+  // topology arrays on the device
+  DBuf edges, aol, n2f, f2n, inc_ptr, inc_edge, inc_other, inc_slot;
+  std::vector<std::unique_ptr<LevelDev>> lev;
+  int64_t max_grid = 0;
+  // fine FP64 operator (values of levels[0] pattern), its l1 diagonal, b
+  DBuf A64, invd64, b64;
+  // inputs staged from the host / kept for the pullback
+  DBuf Ke, dh, Q, stage_n, stage_n2, stage_e, stage_f, stage_f2, stage_f3;
+  // refinement vectors
+  DBuf x64, r64, b32;
+  Hier<double> h64;
+  Hier<float> h32;
+  // reductions and solve scalars
+  DBuf partial, counter, scal, zero_flag;
+  SolveScalars* scal_host = nullptr;
+  uint64_t version = 0;        // bumped at every numeric refresh
+  uint64_t assembled_for = 0;  // version whose K the fine operator holds
+  int cheb_degree = 2;
+  double cheb_ratio = 30.0;
+  int last_iters[2] = {0, 0};  // iterations of the last FP64 / FP32 solve
+  int levels() const { return (int)H.levels.size(); }
+};
+
+struct dpf_state {
+  dpf_topology* topo;
+  uint64_t version;  // topo->version of the forward solve
+  DBuf Ke, dh, hfree, lam, delta, Q;
+  int precision;
+};
+
+namespace {
+
+void set_err(const std::string& m) { g_err = m; }
+
+hipStream_t pick_stream(dpf_topology* t, const dpf_opts* o) {
+  if (o && o->memory == DPF_DEVICE && o->stream) return (hipStream_t)o->stream;
+  return t->own_stream;
+}
+
+template <typename T>
+void ensure_hier(dpf_topology* t, Hier<T>& h) {
+  if (h.allocated) return;
+  int L = t->levels();
+  h.lev.clear();
+  h.lev = std::vector<LevelNum<T>>(L);
+  for (int l = 0; l < L; l++) {
+    LevelDev& d = *t->lev[l];
+    LevelNum<T>& n = h.lev[l];
+    size_t nr = (size_t)d.A.nrows;
+    if (!(l == 0 && sizeof(T) == sizeof(double))) n.val.alloc(d.A.entries * sizeof(T));
+    if (!(l == 0 && sizeof(T) == sizeof(double))) n.invd.alloc(nr * sizeof(T));
+    if (d.transfer) {
+      n.pval.alloc(d.P.entries * sizeof(T));
+      n.rval.alloc(d.R.entries * sizeof(T));
+      n.apval.alloc(d.ap_nnz * sizeof(T));
+    }
+    n.x.alloc(nr * sizeof(T));
+    n.xs.alloc(nr * sizeof(T));
+    n.d.alloc(nr * sizeof(T));
+    n.r.alloc(nr * sizeof(T));
+    if (l > 0) n.b.alloc(nr * sizeof(T));
+  }
+  int64_t nc = t->lev[L - 1]->A.nrows;
+  if (t->H.coarse_dense) h.W.alloc((size_t)nc * nc * sizeof(double));
+  h.status.alloc(sizeof(int));
+  size_t nf = (size_t)t->H.nf;
+  h.cx.alloc(nf * sizeof(T));
+  h.cr.alloc(nf * sizeof(T));
+  h.cp0.alloc(nf * sizeof(T));
+  h.cp1.alloc(nf * sizeof(T));
+  h.cq.alloc(nf * sizeof(T));
+  h.allocated = true;
+}
+
+template <typename T>
+T* level_val(dpf_topology* t, Hier<T>& h, int l) {
+  if (l == 0 && sizeof(T) == sizeof(double)) return (T*)t->A64.p;
+  return h.lev[l].val.template as<T>();
+}
+template <typename T>
+T* level_invd(dpf_topology* t, Hier<T>& h, int l) {
+  if (l == 0 && sizeof(T) == sizeof(double)) return (T*)t->invd64.p;
+  return h.lev[l].invd.template as<T>();
+}
+
+// Fine assembly from t->Ke (+ Q, dh): A64 (+ FP32 copy into h32 level 0), b64.
+void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool want_b) {
+  LevelDev& L0 = *t->lev[0];
+  float* v32 = nullptr;
+  float* i32 = nullptr;
+  if (want32) {
+    ensure_hier(t, t->h32);
+    v32 = t->h32.lev[0].val.as<float>();
+    i32 = t->h32.lev[0].invd.as<float>();
+  }
+  hipLaunchKernelGGL(k_assemble, dim3(nblocks(t->H.nf)), dim3(BLK), 0, s, (int64_t)t->H.nf,
+                     t->f2n.as<int32_t>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                     t->inc_other.as<int32_t>(), t->inc_slot.as<uint16_t>(),
+                     L0.diag_slot.as<uint16_t>(), t->Ke.as<double>(), t->aol.as<double>(),
+                     t->dh.as<double>(), Q, 1.0, (const double*)nullptr,
+                     L0.A.view<double>(t->A64.as<double>()), v32, t->invd64.as<double>(), i32,
+                     want_b ? t->b64.as<double>() : (double*)nullptr);
+  HIPTRY(hipGetLastError());
+}
+
+// Numeric refresh of levels 1.. and the coarsest inverse, from level 0 values.
+template <typename T>
+void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
+  ensure_hier(t, h);
+  int L = t->levels();
+  for (int l = 0; l + 1 < L; l++) {
+    LevelDev& d = *t->lev[l];
+    LevelDev& dn = *t->lev[l + 1];
+    SellD<T> A = d.A.view<T>(level_val(t, h, l));
+    SellD<T> P = d.P.view<T>(h.lev[l].pval.template as<T>());
+    SellD<T> R = d.R.view<T>(h.lev[l].rval.template as<T>());
+    hipLaunchKernelGGL(k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
+                       (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(),
+                       d.p_self.as<uint16_t>(), P);
+    hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(d.R.entries)), dim3(BLK), 0, s, d.R.entries,
+                       d.r_src.as<int32_t>(), (const T*)P.val, R.val);
+    hipLaunchKernelGGL(k_ap<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A, P,
+                       d.ap_rp.as<int64_t>(), d.ap_pp.as<int64_t>(), d.ap_map.as<uint16_t>(),
+                       d.A.rowlen.as<int32_t>(), d.P.rowlen.as<int32_t>(),
+                       h.lev[l].apval.template as<T>());
+    hipLaunchKernelGGL(k_rap<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, R,
+                       d.R.rowlen.as<int32_t>(), d.ap_rp.as<int64_t>(),
+                       (const T*)h.lev[l].apval.template as<T>(), d.rap_pp.as<int64_t>(),
+                       d.rap_map.as<uint16_t>(), dn.A.view<T>(level_val(t, h, l + 1)),
+                       dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
+  }
+  if (t->H.coarse_dense) {
+    LevelDev& dc = *t->lev[L - 1];
+    hipLaunchKernelGGL(k_dense_inverse<T>, dim3(1), dim3(1024), 0, s,
+                       dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
+                       h.W.template as<double>(), h.status.template as<int>());
+  }
+  HIPTRY(hipGetLastError());
+}
+
+template <typename T>
+auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
+  return level == 0 ? &k_cheb<T, true> : &k_cheb<T, false>;
+}
+
+RedWS redws(dpf_topology* t) {
+  return RedWS{t->partial.as<double>(), t->counter.as<unsigned int>()};
+}
+
+// One V-cycle on levels l..: input b at level l (for l = 0 the CG residual),
+// level x/d already hold the zero-guess first Chebyshev step.  Returns the
+// buffer holding x_l on exit through *xout.
+template <typename T>
+void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* xl, T* xsl,
+            T** xout, const Cheb& ch, int* done, bool fuse_dot_at_fine) {
+  int L = t->levels();
+  LevelDev& d = *t->lev[l];
+  LevelNum<T>& n = h.lev[l];
+  SellD<T> A = d.A.view<T>(level_val(t, h, l));
+  const T* invd = level_invd(t, h, l);
+  T* dd = n.d.template as<T>();
+  RedWS ws = redws(t);
+  SolveScalars* sc = t->scal.as<SolveScalars>();
+  unsigned g = nblocks(d.A.nrows);
+  int m = (int)ch.c1.size();
+  T* cur = xl;
+  T* oth = xsl;
+  if (l == L - 1) {
+    if (t->H.coarse_dense) {
+      int nc = (int)d.A.nrows;
+      hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), nc * sizeof(double), s, nc,
+                         (const double*)h.W.template as<double>(), bl, cur, (const int*)done);
+    } else {
+      // no dense solve: a longer smoothing sweep (degree 2m) stands in
+      Cheb c2;
+      c2.make(2 * m, 1.0, t->cheb_ratio);
+      for (int k = 1; k < 2 * m; k++) {
+        hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd,
+                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, 0, ws, sc);
+        std::swap(cur, oth);
+      }
+    }
+    *xout = cur;
+    return;
+  }
+  // pre-smoothing (step 0 done by the producer of b)
+  for (int k = 1; k < m; k++) {
+    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
+                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, ws, sc);
+    std::swap(cur, oth);
+  }
+  T* rl = n.r.template as<T>();
+  hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)cur, bl, rl,
+                     (const int*)done);
+  LevelDev& dn = *t->lev[l + 1];
+  LevelNum<T>& nn = h.lev[l + 1];
+  bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
+  SellD<T> R = d.R.view<T>(n.rval.template as<T>());
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, R,
+                     (const T*)rl, nn.b.template as<T>(), (const T*)level_invd(t, h, l + 1),
+                     nn.d.template as<T>(), nn.x.template as<T>(), (T)ch.c0,
+                     coarse_next ? 0 : 1, (const int*)done);
+  T* xc = nullptr;
+  vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
+            nn.xs.template as<T>(), &xc, ch, done, false);
+  SellD<T> P = d.P.view<T>(n.pval.template as<T>());
+  hipLaunchKernelGGL(k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur,
+                     (const int*)done);
+  for (int k = 0; k < m; k++) {
+    double c1 = k == 0 ? 0.0 : ch.c1[k];
+    double c2 = k == 0 ? ch.c0 : ch.c2[k];
+    int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
+    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
+                       (T)c1, (T)c2, (const int*)done, fuse, ws, sc);
+    std::swap(cur, oth);
+  }
+  *xout = cur;
+}
+
+struct SolveResult {
+  int status;  // SolveScalars.done
+  int iters;
+  double rr, rr0;
+};
+
+// PCG on the fine level in precision T: solves A x = b where b (length nf,
+// device) is h.cr's source; x ends in h.cx.  Blocking (polls the device
+// convergence flag in chunks of iterations).
+template <typename T>
+SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double reltol,
+                double abstol, int maxiter, const Cheb& ch, int chunk_hint) {
+  LevelDev& d0 = *t->lev[0];
+  LevelNum<T>& n0 = h.lev[0];
+  int64_t nf = t->H.nf;
+  unsigned g = nblocks(nf);
+  RedWS ws = redws(t);
+  SolveScalars* sc = t->scal.as<SolveScalars>();
+  int* done = &sc->done;
+  T* x = h.cx.template as<T>();
+  T* r = h.cr.template as<T>();
+  T* q = h.cq.template as<T>();
+  T* pb[2] = {h.cp0.template as<T>(), h.cp1.template as<T>()};
+  const T* invd = level_invd(t, h, 0);
+  SellD<T> A = d0.A.view<T>(level_val(t, h, 0));
+  T* dd = n0.d.template as<T>();
+  T* z0 = n0.x.template as<T>();
+  T* zs = n0.xs.template as<T>();
+  hipLaunchKernelGGL(k_cg_start<T>, dim3(g), dim3(BLK), 0, s, nf, b, r, x, invd, dd, z0,
+                     (T)ch.c0, reltol * reltol, abstol * abstol, maxiter, ws, sc);
+  T* z = nullptr;
+  vcycle<T>(t, h, s, 0, r, z0, zs, &z, ch, done, true);
+  int par = 0;
+  SolveResult res{0, 0, 0, 0};
+  int pidx = sizeof(T) == sizeof(double) ? 0 : 1;
+  int predicted = t->last_iters[pidx];
+  // launch up to the iteration count of the previous solve, then poll every
+  // iteration, so that no launch is wasted past convergence
+  int chunk = predicted > 2 ? predicted - 2 : std::max(1, chunk_hint);
+  int launched = 0;
+  while (true) {
+    for (int c = 0; c < chunk; c++) {
+      hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)z,
+                         (const T*)pb[par], pb[par ^ 1], q, (const int*)done, ws, sc);
+      par ^= 1;
+      hipLaunchKernelGGL(k_cg_xr<T>, dim3(g), dim3(BLK), 0, s, nf, x, r, (const T*)pb[par],
+                         (const T*)q, invd, dd, z0, (T)ch.c0, (const int*)done, ws, sc);
+      T* zz = nullptr;
+      vcycle<T>(t, h, s, 0, r, z0, zs, &zz, ch, done, true);
+      z = zz;
+      launched++;
+    }
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    SolveScalars hs = *t->scal_host;
+    if (hs.done != 0 || hs.iters >= maxiter) {
+      res.status = hs.done == 0 ? 2 : hs.done;
+      res.iters = hs.iters;
+      res.rr = hs.rr;
+      break;
+    }
+    // the device counts iterations; if it stops counting, stop launching
+    if (launched > maxiter + 64 || hs.iters + 2 * chunk < launched)
+      throw Err{DPF_EHIP, "Krylov loop made no progress on the device"};
+    chunk = 1;
+  }
+  t->last_iters[pidx] = res.iters;
+  return res;
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+const char* dpf_last_error(void) { return g_err.c_str(); }
+
+void dpf_opts_init(dpf_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->reltol = 1e-10;
+  o->abstol = 0.0;
+  o->maxiter = 2000;
+  o->precision = DPF_FP64;
+  o->inner_reltol = 1e-5;
+  o->max_outer = 20;
+  o->cheb_degree = 2;
+  o->memory = DPF_HOST;
+  o->stream = nullptr;
+}
+
+int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
+                        const double* areasoverlengths, int64_t ndirichlet,
+                        const int64_t* dirichletnodes, dpf_topology** out) {
+  if (!out) {
+    set_err("out is NULL");
+    return DPF_EINVAL;
+  }
+  *out = nullptr;
+  if ((nedges > 0 && (!neighbors || !areasoverlengths)) || (ndirichlet > 0 && !dirichletnodes)) {
+    set_err("NULL array argument");
+    return DPF_EINVAL;
+  }
+  auto t = std::make_unique<dpf_topology>();
+  try {
+    std::string e = build_topology(t->H, n, nedges, neighbors, areasoverlengths, ndirichlet,
+                                   dirichletnodes);
+    if (!e.empty()) {
+      set_err(e);
+      return e.find("singular") != std::string::npos ? DPF_ESINGULAR : DPF_EINVAL;
+    }
+    e = build_hierarchy(t->H, 25, 128, 512);
+    if (!e.empty()) {
+      set_err(e);
+      return DPF_EINVAL;
+    }
+    HIPTRY(hipGetDevice(&t->device));
+    HIPTRY(hipStreamCreateWithFlags(&t->own_stream, hipStreamNonBlocking));
+    TopologyHost& H = t->H;
+    upload(t->edges, H.edges);
+    upload(t->aol, H.aol);
+    upload(t->n2f, H.n2f);
+    upload(t->f2n, H.f2n);
+    upload(t->inc_ptr, H.inc_ptr);
+    upload(t->inc_edge, H.inc_edge);
+    upload(t->inc_other, H.inc_other);
+    upload(t->inc_slot, H.inc_slot);
+    int64_t maxrows = 0;
+    for (size_t l = 0; l < H.levels.size(); l++) {
+      auto d = std::make_unique<LevelDev>();
+      LevelHost& lh = H.levels[l];
+      d->A.up(lh.A);
+      upload(d->diag_slot, lh.diag_slot);
+      maxrows = std::max(maxrows, lh.A.nrows);
+      if (!lh.agg.empty()) {
+        d->transfer = true;
+        d->P.up(lh.P);
+        d->R.up(lh.R);
+        upload(d->p_slot, lh.p_slot);
+        upload(d->p_self, lh.p_self);
+        upload(d->r_src, lh.r_src);
+        upload(d->ap_rp, lh.ap_rp);
+        upload(d->ap_col, lh.ap_col);
+        upload(d->ap_pp, lh.ap_pp);
+        upload(d->ap_map, lh.ap_map);
+        upload(d->rap_pp, lh.rap_pp);
+        upload(d->rap_map, lh.rap_map);
+        d->ap_nnz = lh.ap_rp.back();
+        maxrows = std::max(maxrows, lh.R.entries());
+      }
+      t->lev.push_back(std::move(d));
+    }
+    maxrows = std::max<int64_t>(maxrows, std::max(H.n, H.ne));
+    t->max_grid = nblocks(maxrows) + 8;
+    t->partial.alloc(t->max_grid * sizeof(double));
+    t->counter.alloc(sizeof(unsigned int));
+    HIPTRY(hipMemset(t->counter.p, 0, sizeof(unsigned int)));
+    t->scal.alloc(sizeof(SolveScalars));
+    HIPTRY(hipMemset(t->scal.p, 0, sizeof(SolveScalars)));
+    t->zero_flag.alloc(sizeof(int));
+    HIPTRY(hipMemset(t->zero_flag.p, 0, sizeof(int)));
+    HIPTRY(hipHostMalloc((void**)&t->scal_host, sizeof(SolveScalars), hipHostMallocDefault));
+    t->A64.alloc(t->lev[0]->A.entries * sizeof(double));
+    t->invd64.alloc(H.nf * sizeof(double));
+    t->b64.alloc(H.nf * sizeof(double));
+    t->Ke.alloc(H.ne * sizeof(double));
+    t->dh.alloc(H.n * sizeof(double));
+    t->Q.alloc(H.n * sizeof(double));
+    HIPTRY(hipDeviceSynchronize());
+  } catch (const Err& e) {
+    set_err(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err("host allocation failed");
+    return DPF_ENOMEM;
+  }
+  *out = t.release();
+  return DPF_OK;
+}
+
+int dpf_topology_destroy(dpf_topology* t) {
+  if (!t) return DPF_OK;
+  {
+    std::lock_guard<std::mutex> g(t->mu);
+    if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
+    if (t->scal_host) (void)hipHostFree(t->scal_host);
+    if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
+  }
+  delete t;
+  return DPF_OK;
+}
+
+int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
+  if (!t || !info) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  const TopologyHost& H = t->H;
+  int64_t nnz_all = 0;
+  for (auto& l : H.levels) nnz_all += l.A.nnz;
+  int64_t v[8] = {H.n,
+                  H.ne,
+                  H.nf,
+                  H.levels[0].A.nnz,
+                  (int64_t)H.levels.size(),
+                  H.levels.back().A.nrows,
+                  (int64_t)(1000.0 * (double)nnz_all / (double)H.levels[0].A.nnz),
+                  (int64_t)(H.setup_ms * 1000.0)};
+  for (int i = 0; i < ninfo && i < 8; i++) info[i] = v[i];
+  return DPF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Copy-in helper: host pointer -> staging device buffer, or pass the device pointer.
+const double* stage_in(dpf_topology* t, const dpf_opts* o, DBuf& buf, const double* p,
+                       int64_t count, hipStream_t s) {
+  if (!p) return nullptr;
+  if (o->memory == DPF_DEVICE) return p;
+  buf.alloc(count * sizeof(double));
+  HIPTRY(hipMemcpyAsync(buf.p, p, count * sizeof(double), hipMemcpyHostToDevice, s));
+  return buf.as<double>();
+}
+
+void copy_dev(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  HIPTRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// Solve A x = b64 (device, FP64, nf) with the current hierarchy; x -> xout (device FP64).
+// Returns 0 / DPF_ENOCONV / DPF_EBREAKDOWN and fills stats.
+int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
+                 const dpf_opts* o, dpf_stats* st) {
+  int64_t nf = t->H.nf;
+  int deg = o->cheb_degree > 0 ? o->cheb_degree : t->cheb_degree;
+  Cheb ch;
+  ch.make(deg, 1.0, t->cheb_ratio);
+  RedWS ws = redws(t);
+  SolveScalars* sc = t->scal.as<SolveScalars>();
+  LevelDev& d0 = *t->lev[0];
+  SellD<double> A64 = d0.A.view<double>(t->A64.as<double>());
+  int status = DPF_OK;
+  int total_it = 0, outer = 0;
+  double relres = 0;
+  // ||b||
+  t->x64.alloc(nf * sizeof(double));
+  t->r64.alloc(nf * sizeof(double));
+  hipLaunchKernelGGL(k_zero_f64, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->x64.as<double>());
+  hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
+                     (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
+  HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  double bnorm = t->scal_host->outer_s;
+  double target = std::max(o->reltol * bnorm, o->abstol);
+  if (o->precision == DPF_FP64) {
+    SolveResult r = pcg<double>(t, t->h64, s, b, o->reltol, o->abstol, o->maxiter, ch, 16);
+    total_it = r.iters;
+    outer = 1;
+    copy_dev(xout, t->h64.cx.p, nf * sizeof(double), s);
+    hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
+                       (const double*)xout, b, t->r64.as<double>(), ws, sc);
+    HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    double rn = t->scal_host->outer_s;
+    relres = bnorm > 0 ? rn / bnorm : rn;
+    if (r.status == 3)
+      status = DPF_EBREAKDOWN;
+    else if (r.status == 2 && rn > target)
+      status = DPF_ENOCONV;
+  } else {
+    ensure_hier(t, t->h32);
+    t->b32.alloc(nf * sizeof(float));
+    double rn = bnorm;
+    int budget = o->maxiter;
+    while (true) {
+      if (rn <= target) break;
+      if (outer >= o->max_outer || budget <= 0) {
+        status = DPF_ENOCONV;
+        break;
+      }
+      hipLaunchKernelGGL(k_scale_to_f32, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                         (const double*)t->r64.as<double>(), t->b32.as<float>(),
+                         (const SolveScalars*)sc);
+      SolveResult r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget,
+                                 ch, 8);
+      total_it += r.iters;
+      budget -= r.iters;
+      outer++;
+      if (r.status == 3) {
+        status = DPF_EBREAKDOWN;
+        break;
+      }
+      hipLaunchKernelGGL(k_refine, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->x64.as<double>(),
+                         (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc);
+      hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
+                         (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
+      HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+      HIPTRY(hipStreamSynchronize(s));
+      double rn_new = t->scal_host->outer_s;
+      if (!(rn_new == rn_new)) {
+        status = DPF_EBREAKDOWN;
+        rn = rn_new;
+        break;
+      }
+      rn = rn_new;
+    }
+    relres = bnorm > 0 ? rn / bnorm : rn;
+    copy_dev(xout, t->x64.p, nf * sizeof(double), s);
+  }
+  if (st) {
+    st->iterations = total_it;
+    st->outer = outer;
+    st->converged = (status == DPF_OK);
+    st->levels = t->levels();
+    st->relres = relres;
+  }
+  return status;
+}
+
+// Assemble + numeric refresh for the conductivities already in t->Ke.
+void prepare(dpf_topology* t, hipStream_t s, int precision, const double* Q, bool want_b) {
+  assemble(t, s, precision == DPF_MIXED, Q, want_b);
+  t->version++;
+  if (precision == DPF_MIXED) {
+    refresh<float>(t, t->h32, s);
+    t->h32.version = t->version;
+  } else {
+    ensure_hier(t, t->h64);
+    refresh<double>(t, t->h64, s);
+    t->h64.version = t->version;
+  }
+  t->assembled_for = t->version;
+}
+
+int check_opts(const dpf_opts* o) {
+  if (!o) return DPF_OK;
+  if (o->precision != DPF_FP64 && o->precision != DPF_MIXED) return DPF_EINVAL;
+  if (o->memory != DPF_HOST && o->memory != DPF_DEVICE) return DPF_EINVAL;
+  if (!(o->reltol >= 0) || !(o->abstol >= 0) || o->maxiter <= 0) return DPF_EINVAL;
+  if (o->precision == DPF_MIXED && (!(o->inner_reltol > 0 && o->inner_reltol < 1) || o->max_outer <= 0))
+    return DPF_EINVAL;
+  if (o->cheb_degree < 0 || o->cheb_degree > 16) return DPF_EINVAL;
+  return DPF_OK;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Err& e) {
+    set_err(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err("host allocation failed");
+    return DPF_ENOMEM;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpf_residuals(dpf_topology* t, const double* hfree, const double* Ks,
+                  const double* dirichleths, const double* Qs, const double* Ss,
+                  const double* V, double* r_out, const dpf_opts* opts) {
+  if (!t || !hfree || !Ks || !dirichleths || !Qs || !Ss || !V || !r_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, nf = t->H.nf, ne = t->H.ne;
+    const double* dh = stage_in(t, o, t->stage_n, dirichleths, n, s);
+    const double* Q = stage_in(t, o, t->stage_n2, Qs, n, s);
+    const double* K = stage_in(t, o, t->stage_e, Ks, ne, s);
+    const double* h = stage_in(t, o, t->stage_f, hfree, nf, s);
+    DBuf ssb, vb;
+    const double* ss = stage_in(t, o, ssb, Ss, n, s);
+    const double* vv = stage_in(t, o, vb, V, n, s);
+    double* r = r_out;
+    if (o->memory == DPF_HOST) {
+      t->stage_f2.alloc(nf * sizeof(double));
+      r = t->stage_f2.as<double>();
+    }
+    hipLaunchKernelGGL(k_residuals, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                       t->inc_other.as<int32_t>(), t->n2f.as<int32_t>(), K,
+                       t->aol.as<double>(), h, dh, Q, ss, vv, r);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(r_out, r, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return DPF_OK;
+  });
+}
+
+int dpf_hm_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_opts* opts) {
+  if (!t || !logK || !Ke) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    const double* lk = stage_in(t, o, t->stage_n, logK, t->H.n, s);
+    double* out = Ke;
+    if (o->memory == DPF_HOST) {
+      t->stage_e.alloc(t->H.ne * sizeof(double));
+      out = t->stage_e.as<double>();
+    }
+    hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
+                       t->edges.as<int2>(), lk, out);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST) {
+      HIPTRY(hipMemcpyAsync(Ke, out, t->H.ne * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPTRY(hipStreamSynchronize(s));
+    }
+    return DPF_OK;
+  });
+}
+
+int dpf_hm_pullback(dpf_topology* t, const double* logK, const double* Ke, const double* dKe,
+                    double* dlogK, const dpf_opts* opts) {
+  if (!t || !logK || !Ke || !dKe || !dlogK) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    const double* lk = stage_in(t, o, t->stage_n, logK, t->H.n, s);
+    const double* ke = stage_in(t, o, t->stage_e, Ke, t->H.ne, s);
+    DBuf dkb;
+    const double* dk = stage_in(t, o, dkb, dKe, t->H.ne, s);
+    double* out = dlogK;
+    if (o->memory == DPF_HOST) {
+      t->stage_n2.alloc(t->H.n * sizeof(double));
+      out = t->stage_n2.as<double>();
+    }
+    hipLaunchKernelGGL(k_hm_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
+                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                       t->inc_other.as<int32_t>(), lk, ke, dk, out);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST) {
+      HIPTRY(hipMemcpyAsync(dlogK, out, t->H.n * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    return DPF_OK;
+  });
+}
+
+int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichleths,
+                       const double* Qs, double* h_out, const dpf_opts* opts, dpf_stats* stats,
+                       dpf_state** state_out) {
+  if (state_out) *state_out = nullptr;
+  if (!t || !Ks || !dirichleths || !Qs || !h_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    // inputs -> topology-owned device buffers
+    if (o->memory == DPF_DEVICE) {
+      copy_dev(t->Ke.p, Ks, ne * sizeof(double), s);
+      copy_dev(t->dh.p, dirichleths, n * sizeof(double), s);
+      copy_dev(t->Q.p, Qs, n * sizeof(double), s);
+    } else {
+      HIPTRY(hipMemcpyAsync(t->Ke.p, Ks, ne * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPTRY(hipMemcpyAsync(t->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    prepare(t, s, o->precision, t->Q.as<double>(), true);
+    HIPTRY(hipStreamSynchronize(s));
+    double t1 = now_ms();
+    std::unique_ptr<dpf_state> st;
+    double* hfree;
+    DBuf tmp;
+    if (state_out) {
+      st = std::make_unique<dpf_state>();
+      st->topo = t;
+      st->hfree.alloc(nf * sizeof(double));
+      hfree = st->hfree.as<double>();
+    } else {
+      tmp.alloc(nf * sizeof(double));
+      hfree = tmp.as<double>();
+    }
+    int status = solve_system(t, s, t->b64.as<double>(), hfree, o, stats);
+    double t2 = now_ms();
+    double* hdev = h_out;
+    if (o->memory == DPF_HOST) {
+      t->stage_n2.alloc(n * sizeof(double));
+      hdev = t->stage_n2.as<double>();
+    }
+    hipLaunchKernelGGL(k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                       (const double*)hfree, t->dh.as<double>(), hdev);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(h_out, hdev, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (st) {
+      st->version = t->version;
+      st->precision = o->precision;
+      st->Ke.alloc(ne * sizeof(double));
+      st->dh.alloc(n * sizeof(double));
+      st->Q.alloc(n * sizeof(double));
+      copy_dev(st->Ke.p, t->Ke.p, ne * sizeof(double), s);
+      copy_dev(st->dh.p, t->dh.p, n * sizeof(double), s);
+      copy_dev(st->Q.p, t->Q.p, n * sizeof(double), s);
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      stats->setup_ms = t1 - t0;
+      stats->solve_ms = t2 - t1;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status == DPF_ENOCONV) set_err("AMG-PCG did not reach the tolerance within maxiter");
+    if (status == DPF_EBREAKDOWN) set_err("AMG-PCG breakdown (non-SPD operator or NaN input)");
+    if (state_out && status == DPF_OK) *state_out = st.release();
+    return status;
+  });
+}
+
+int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double* ddh,
+                        double* dQs, const dpf_opts* opts, dpf_stats* stats) {
+  if (!st || !st->topo || !delta) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  dpf_topology* t = st->topo;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    // the hierarchy must hold this state's conductivities in the requested precision
+    bool fresh = (t->version == st->version) &&
+                 ((o->precision == DPF_MIXED) ? t->h32.version == st->version
+                                              : t->h64.version == st->version);
+    if (!fresh) {
+      copy_dev(t->Ke.p, st->Ke.p, ne * sizeof(double), s);
+      copy_dev(t->dh.p, st->dh.p, n * sizeof(double), s);
+      prepare(t, s, o->precision, nullptr, false);
+      st->version = t->version;
+    }
+    const double* dl = stage_in(t, o, t->stage_n, delta, n, s);
+    st->lam.alloc(nf * sizeof(double));
+    t->stage_f3.alloc(nf * sizeof(double));
+    double* rhs = t->stage_f3.as<double>();
+    hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                       t->f2n.as<int32_t>(), dl, rhs);
+    double t1 = now_ms();
+    int status = solve_system(t, s, rhs, st->lam.as<double>(), o, stats);
+    double t2 = now_ms();
+    const double* lam = st->lam.as<double>();
+    double* dK = dKs;
+    double* dd = ddh;
+    double* dq = dQs;
+    DBuf bk, bd, bq;
+    if (o->memory == DPF_HOST) {
+      if (dKs) { bk.alloc(ne * sizeof(double)); dK = bk.as<double>(); }
+      if (ddh) { bd.alloc(n * sizeof(double)); dd = bd.as<double>(); }
+      if (dQs) { bq.alloc(n * sizeof(double)); dq = bq.as<double>(); }
+    }
+    if (dK)
+      hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
+                         t->n2f.as<int32_t>(), t->aol.as<double>(),
+                         (const double*)st->hfree.as<double>(), (const double*)st->dh.as<double>(),
+                         lam, dK);
+    if (dd || dq)
+      hipLaunchKernelGGL(k_ddh_dq, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                         t->inc_other.as<int32_t>(), (const double*)st->Ke.as<double>(),
+                         t->aol.as<double>(), lam, dl, dd, dq);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST) {
+      if (dKs) HIPTRY(hipMemcpyAsync(dKs, dK, ne * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (ddh) HIPTRY(hipMemcpyAsync(ddh, dd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (dQs) HIPTRY(hipMemcpyAsync(dQs, dq, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      stats->setup_ms = t1 - t0;
+      stats->solve_ms = t2 - t1;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status == DPF_ENOCONV) set_err("adjoint AMG-PCG did not reach the tolerance within maxiter");
+    if (status == DPF_EBREAKDOWN) set_err("adjoint AMG-PCG breakdown");
+    return status;
+  });
+}
+
+int dpf_state_destroy(dpf_state* st) {
+  if (!st) return DPF_OK;
+  if (st->topo) {
+    std::lock_guard<std::mutex> g(st->topo->mu);
+    (void)hipStreamSynchronize(st->topo->own_stream);
+  }
+  delete st;
+  return DPF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int reps,
+                   double* bytes) {
+  LevelDev& d0 = *t->lev[0];
+  LevelNum<T>& n0 = h.lev[0];
+  int64_t nf = t->H.nf, nnz = t->H.levels[0].A.nnz;
+  unsigned g = nblocks(nf);
+  const int* nodone = t->zero_flag.as<int>();
+  RedWS ws = redws(t);
+  SolveScalars* sc = t->scal.as<SolveScalars>();
+  SellD<T> A = d0.A.view<T>(level_val(t, h, 0));
+  double w = (double)sizeof(T);
+  Cheb ch;
+  ch.make(std::max(2, t->cheb_degree), 1.0, t->cheb_ratio);
+  hipEvent_t e0, e1;
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  auto launch = [&](int k) {
+    T* xa = (k & 1) ? n0.xs.template as<T>() : n0.x.template as<T>();
+    T* xb = (k & 1) ? n0.x.template as<T>() : n0.xs.template as<T>();
+    if (kernel == 0) {
+      hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa, xb,
+                         (const T*)h.cr.template as<T>(), n0.d.template as<T>(),
+                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0, ws,
+                         sc);
+    } else if (kernel == 1) {
+      hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
+                         (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
+                         h.cq.template as<T>(), nodone, ws, sc);
+    } else {
+      hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
+                         d0.A.view<double>(t->A64.as<double>()), (const double*)t->x64.as<double>(),
+                         (const double*)t->b64.as<double>(), t->r64.as<double>(), nodone);
+    }
+  };
+  launch(0);  // warm
+  HIPTRY(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; k++) launch(k);
+  HIPTRY(hipEventRecord(e1, s));
+  HIPTRY(hipGetLastError());
+  HIPTRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (kernel == 0)
+    *bytes = (double)nnz * (w + 4.0) + 6.0 * (double)nf * w;
+  else if (kernel == 1)
+    *bytes = (double)nnz * (w + 4.0) + 4.0 * (double)nf * w;
+  else
+    *bytes = (double)nnz * 12.0 + 3.0 * (double)nf * 8.0;
+  return (double)ms / reps;
+}
+
+}  // namespace
+
+extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t kernel,
+                                  int32_t reps, double* avg_ms, double* bytes,
+                                  const dpf_opts* opts) {
+  if (!t || !avg_ms || !bytes || reps <= 0 || kernel < 0 || kernel > 2) {
+    set_err("invalid argument");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    if (precision == DPF_MIXED) {
+      if (!t->h32.allocated || t->h32.version == 0) throw Err{DPF_EINVAL, "no FP32 solve yet"};
+      *avg_ms = profile_one<float>(t, t->h32, s, kernel, reps, bytes);
+    } else {
+      if (!t->h64.allocated || t->h64.version == 0) throw Err{DPF_EINVAL, "no FP64 solve yet"};
+      if (kernel == 2 && (!t->x64.p || !t->r64.p)) throw Err{DPF_EINVAL, "no solve yet"};
+      *avg_ms = profile_one<double>(t, t->h64, s, kernel, reps, bytes);
+    }
+    return DPF_OK;
+  });
+}

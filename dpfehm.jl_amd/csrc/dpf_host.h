@@ -1,0 +1,85 @@
+// Host-side topology cache and smoothed-aggregation (SA) symbolic setup.
+//
+// Everything here depends only on the mesh topology (neighbors,
+// areasoverlengths, dirichletnodes), never on the conductivities, so it is
+// built once per topology and reused for every solve: the per-solve work
+// (assembly, prolongator smoothing, Galerkin products) runs on the GPU
+// through the index maps built here.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dpf {
+
+// Sliced ELLPACK with 64-row slices (one row per lane of a 64-wide wavefront),
+// column-major inside a slice: entry s of row r lives at
+//   sptr[r / 64] + s * 64 + r % 64.
+// Rows shorter than their slice's width are padded with value 0 and a valid
+// column (the row itself for square matrices, 0 otherwise).
+constexpr int SELL_C = 64;
+constexpr uint16_t NO_SLOT = 0xFFFF;
+
+struct SellHost {
+  int64_t nrows = 0, ncols = 0;
+  std::vector<int64_t> sptr;    // nslices + 1
+  std::vector<int32_t> col;     // sptr.back() entries
+  std::vector<int32_t> rowlen;  // nrows
+  int64_t nnz = 0;
+  int max_rowlen = 0;
+  int64_t entries() const { return sptr.empty() ? 0 : sptr.back(); }
+  int64_t nslices() const { return sptr.empty() ? 0 : (int64_t)sptr.size() - 1; }
+  int64_t idx(int64_t r, int s) const {
+    return sptr[r / SELL_C] + (int64_t)s * SELL_C + (r % SELL_C);
+  }
+  int32_t at(int64_t r, int s) const { return col[idx(r, s)]; }
+  // rp/ci: row lists with sorted unique columns.
+  void build(int64_t nrows, int64_t ncols, const std::vector<int64_t>& rp,
+             const std::vector<int32_t>& ci, bool square);
+};
+
+// One level of the AMG hierarchy.  A is the level operator; the transfer
+// data (agg .. rap_map) connect it to the next coarser level and are empty
+// on the coarsest level.
+struct LevelHost {
+  SellHost A;
+  std::vector<uint16_t> diag_slot;  // per row: slot of the diagonal entry
+  std::vector<int32_t> agg;         // per row: aggregate (= coarse row)
+  SellHost P;                       // prolongator, nrows x ncoarse
+  std::vector<uint16_t> p_slot;     // per A entry: slot of agg(col) in P's row
+  std::vector<uint16_t> p_self;     // per row: slot of agg(row) in P's row
+  SellHost R;                       // restriction = P', ncoarse x nrows
+  std::vector<int32_t> r_src;       // per R entry: index of the P entry, -1 = padding
+  std::vector<int64_t> ap_rp;       // CSR of A*P
+  std::vector<int32_t> ap_col;
+  std::vector<int64_t> ap_pp;       // product offsets of A*P per row
+  std::vector<uint16_t> ap_map;     // per product: slot in the A*P row
+  std::vector<int64_t> rap_pp;      // product offsets of R*(A*P) per coarse row
+  std::vector<uint16_t> rap_map;    // per product: slot in the next level's A row
+};
+
+struct TopologyHost {
+  int64_t n = 0, ne = 0, nd = 0, nf = 0;
+  std::vector<int32_t> edges;       // 2*ne, 0-based
+  std::vector<double> aol;
+  std::vector<uint8_t> isfree;
+  std::vector<int32_t> n2f;         // -1 on Dirichlet cells
+  std::vector<int32_t> f2n;
+  // cell -> incident edges (all cells, edges in list order; self-edges,
+  // which contribute nothing, are left out)
+  std::vector<int64_t> inc_ptr;     // n + 1
+  std::vector<int32_t> inc_edge;
+  std::vector<int32_t> inc_other;   // the edge's other cell
+  std::vector<uint16_t> inc_slot;   // free cell, free other: slot of the
+                                    // other's column in this cell's A row
+  std::vector<LevelHost> levels;    // levels[0].A has the fine pattern
+  bool coarse_dense = false;        // coarsest level solved by a dense inverse
+  double setup_ms = 0;              // host time of the symbolic setup
+};
+
+// Both return "" on success or an error message.
+std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
+                           const double* aol, int64_t nd, const int64_t* dn);
+std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max);
+
+}  // namespace dpf

@@ -1,0 +1,372 @@
+// Host-side topology cache and smoothed-aggregation symbolic setup.
+//
+// Topology: free-cell maps of getfreenodes (src/utilities.jl:10-24), the
+// cell -> edge incidence in edge-list order (the order the residual of
+// src/groundwater.jl:11-20 visits edges in), and the pattern of the free-cell
+// Jacobian dr/dh with a slot map from every (cell, incident edge) to the
+// nonzero it feeds, so that assembly on the GPU is a deterministic row
+// gather with no atomics.
+//
+// Hierarchy: smoothed aggregation on the matrix graph (all couplings strong),
+// the pattern of P = (I - w D^-1 A) T, R = P', A*P and R*A*P, with product
+// slot maps so the GPU recomputes every level's values for new
+// conductivities without any symbolic work.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+#include "dpf_host.h"
+
+namespace dpf {
+
+void SellHost::build(int64_t nr, int64_t nc, const std::vector<int64_t>& rp,
+                     const std::vector<int32_t>& ci, bool square) {
+  nrows = nr;
+  ncols = nc;
+  int64_t ns = (nr + SELL_C - 1) / SELL_C;
+  sptr.assign(ns + 1, 0);
+  rowlen.assign(nr, 0);
+  max_rowlen = 0;
+  nnz = 0;
+  for (int64_t s = 0; s < ns; s++) {
+    int w = 0;
+    for (int64_t r = s * SELL_C; r < std::min<int64_t>(nr, (s + 1) * SELL_C); r++) {
+      int len = (int)(rp[r + 1] - rp[r]);
+      rowlen[r] = len;
+      nnz += len;
+      w = std::max(w, len);
+    }
+    max_rowlen = std::max(max_rowlen, w);
+    sptr[s + 1] = sptr[s] + (int64_t)w * SELL_C;
+  }
+  col.assign(sptr[ns], 0);
+  for (int64_t s = 0; s < ns; s++) {
+    int w = (int)((sptr[s + 1] - sptr[s]) / SELL_C);
+    for (int l = 0; l < SELL_C; l++) {
+      int64_t r = s * SELL_C + l;
+      int32_t pad = (square && r < nr) ? (int32_t)r : 0;
+      for (int k = 0; k < w; k++) {
+        int64_t e = sptr[s] + (int64_t)k * SELL_C + l;
+        col[e] = (r < nr && k < rowlen[r]) ? ci[rp[r] + k] : pad;
+      }
+    }
+  }
+}
+
+std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
+                           const double* aol, int64_t nd, const int64_t* dn) {
+  if (n <= 0) return "n must be positive";
+  if (n >= INT32_MAX || ne >= INT32_MAX) return "too many cells or edges for 32-bit indices";
+  if (ne < 0 || nd < 0) return "negative size";
+  T.n = n;
+  T.ne = ne;
+  T.edges.resize(2 * ne);
+  T.aol.assign(aol, aol + ne);
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t a = nb[2 * e], b = nb[2 * e + 1];
+    if (a < 1 || a > n || b < 1 || b > n)
+      return "neighbors[" + std::to_string(e + 1) + "] refers to a cell outside 1.." +
+             std::to_string(n);
+    T.edges[2 * e] = (int32_t)(a - 1);
+    T.edges[2 * e + 1] = (int32_t)(b - 1);
+  }
+  T.isfree.assign(n, 1);
+  for (int64_t k = 0; k < nd; k++) {
+    if (dn[k] < 1 || dn[k] > n)
+      return "dirichletnodes[" + std::to_string(k + 1) + "] outside 1.." + std::to_string(n);
+    T.isfree[dn[k] - 1] = 0;
+  }
+  T.n2f.assign(n, -1);
+  T.f2n.clear();
+  for (int64_t i = 0; i < n; i++)
+    if (T.isfree[i]) {
+      T.n2f[i] = (int32_t)T.f2n.size();
+      T.f2n.push_back((int32_t)i);
+    }
+  T.nf = (int64_t)T.f2n.size();
+  T.nd = n - T.nf;
+  if (T.nf == 0) return "every cell is a Dirichlet cell";
+
+  // incidence, edges in list order per cell
+  T.inc_ptr.assign(n + 1, 0);
+  for (int64_t e = 0; e < ne; e++) {
+    int32_t a = T.edges[2 * e], b = T.edges[2 * e + 1];
+    if (a == b) continue;
+    T.inc_ptr[a + 1]++;
+    T.inc_ptr[b + 1]++;
+  }
+  for (int64_t i = 0; i < n; i++) T.inc_ptr[i + 1] += T.inc_ptr[i];
+  int64_t ninc = T.inc_ptr[n];
+  T.inc_edge.resize(ninc);
+  T.inc_other.resize(ninc);
+  T.inc_slot.assign(ninc, NO_SLOT);
+  std::vector<int64_t> pos(T.inc_ptr.begin(), T.inc_ptr.end() - 1);
+  for (int64_t e = 0; e < ne; e++) {
+    int32_t a = T.edges[2 * e], b = T.edges[2 * e + 1];
+    if (a == b) continue;
+    T.inc_edge[pos[a]] = (int32_t)e;
+    T.inc_other[pos[a]++] = b;
+    T.inc_edge[pos[b]] = (int32_t)e;
+    T.inc_other[pos[b]++] = a;
+  }
+
+  // fine pattern: row j = {j} U {free neighbours}, sorted unique
+  int64_t nf = T.nf;
+  std::vector<int64_t> rp(nf + 1, 0);
+  std::vector<int32_t> ci;
+  ci.reserve(ninc + nf);
+  std::vector<int32_t> row;
+  for (int64_t j = 0; j < nf; j++) {
+    int32_t i = T.f2n[j];
+    row.clear();
+    row.push_back((int32_t)j);
+    for (int64_t p = T.inc_ptr[i]; p < T.inc_ptr[i + 1]; p++) {
+      int32_t o = T.inc_other[p];
+      if (T.isfree[o]) row.push_back(T.n2f[o]);
+    }
+    std::sort(row.begin(), row.end());
+    row.erase(std::unique(row.begin(), row.end()), row.end());
+    if (row.size() >= NO_SLOT) return "a cell has too many distinct neighbours";
+    ci.insert(ci.end(), row.begin(), row.end());
+    rp[j + 1] = (int64_t)ci.size();
+  }
+  for (int64_t j = 0; j < nf; j++) {
+    int32_t i = T.f2n[j];
+    if (T.inc_ptr[i + 1] == T.inc_ptr[i])
+      return "free cell " + std::to_string(i + 1) + " has no edges: the system is singular";
+  }
+  T.levels.assign(1, LevelHost());
+  LevelHost& L0 = T.levels[0];
+  L0.A.build(nf, nf, rp, ci, true);
+  L0.diag_slot.resize(nf);
+  for (int64_t j = 0; j < nf; j++) {
+    int64_t s = rp[j], t = rp[j + 1];
+    for (int64_t k = s; k < t; k++)
+      if (ci[k] == j) L0.diag_slot[j] = (uint16_t)(k - s);
+    int32_t i = T.f2n[j];
+    for (int64_t p = T.inc_ptr[i]; p < T.inc_ptr[i + 1]; p++) {
+      int32_t o = T.inc_other[p];
+      if (!T.isfree[o]) continue;
+      int32_t c = T.n2f[o];
+      auto it = std::lower_bound(ci.begin() + s, ci.begin() + t, c);
+      T.inc_slot[p] = (uint16_t)(it - (ci.begin() + s));
+    }
+  }
+  return "";
+}
+
+namespace {
+
+// Standard SA aggregation on the graph of A (pattern only): roots whose
+// whole neighbourhood is free start an aggregate; leftovers join a
+// neighbouring phase-1 aggregate; the rest form new aggregates with their
+// unaggregated neighbours.
+int64_t aggregate(const SellHost& A, std::vector<int32_t>& agg) {
+  int64_t n = A.nrows;
+  agg.assign(n, -1);
+  int64_t nc = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (agg[i] != -1) continue;
+    bool ok = true;
+    for (int s = 0; s < A.rowlen[i] && ok; s++) {
+      int32_t j = A.at(i, s);
+      if (j != i && agg[j] != -1) ok = false;
+    }
+    if (!ok) continue;
+    for (int s = 0; s < A.rowlen[i]; s++) agg[A.at(i, s)] = (int32_t)nc;
+    agg[i] = (int32_t)nc;
+    nc++;
+  }
+  std::vector<int32_t> agg1(agg);
+  for (int64_t i = 0; i < n; i++) {
+    if (agg[i] != -1) continue;
+    for (int s = 0; s < A.rowlen[i]; s++) {
+      int32_t j = A.at(i, s);
+      if (agg1[j] != -1) {
+        agg[i] = agg1[j];
+        break;
+      }
+    }
+  }
+  for (int64_t i = 0; i < n; i++) {
+    if (agg[i] != -1) continue;
+    agg[i] = (int32_t)nc;
+    for (int s = 0; s < A.rowlen[i]; s++) {
+      int32_t j = A.at(i, s);
+      if (agg[j] == -1) agg[j] = (int32_t)nc;
+    }
+    nc++;
+  }
+  return nc;
+}
+
+// Row-wise unique-column accumulator keyed by a stamp array.
+struct RowSet {
+  std::vector<int64_t> stamp;
+  std::vector<int32_t> slot;
+  std::vector<int32_t> cols;
+  explicit RowSet(int64_t ncols) : stamp(ncols, -1), slot(ncols, 0) {}
+  void begin() { cols.clear(); }
+  void add(int32_t c, int64_t row) {
+    if (stamp[c] != row) {
+      stamp[c] = row;
+      cols.push_back(c);
+    }
+  }
+  // sort and record the slot of every column; the stamp stays set for lookups
+  void finish() {
+    std::sort(cols.begin(), cols.end());
+    for (size_t k = 0; k < cols.size(); k++) slot[cols[k]] = (int32_t)k;
+  }
+};
+
+std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
+  const SellHost& A = L.A;
+  int64_t n = A.nrows;
+  // P pattern: aggregates seen from each row (including the row's own)
+  {
+    RowSet rs(nc);
+    std::vector<int64_t> rp(n + 1, 0);
+    std::vector<int32_t> ci;
+    ci.reserve(A.nnz);
+    L.p_slot.assign(A.entries(), NO_SLOT);
+    L.p_self.resize(n);
+    for (int64_t i = 0; i < n; i++) {
+      rs.begin();
+      rs.add(L.agg[i], i);
+      for (int s = 0; s < A.rowlen[i]; s++) rs.add(L.agg[A.at(i, s)], i);
+      rs.finish();
+      if (rs.cols.size() >= NO_SLOT) return "prolongator row too long";
+      ci.insert(ci.end(), rs.cols.begin(), rs.cols.end());
+      rp[i + 1] = (int64_t)ci.size();
+      for (int s = 0; s < A.rowlen[i]; s++) L.p_slot[A.idx(i, s)] = (uint16_t)rs.slot[L.agg[A.at(i, s)]];
+      L.p_self[i] = (uint16_t)rs.slot[L.agg[i]];
+    }
+    L.P.build(n, nc, rp, ci, false);
+  }
+  const SellHost& P = L.P;
+  // R = P' (columns of R = fine rows, sorted ascending by construction)
+  {
+    std::vector<int64_t> rp(nc + 1, 0);
+    for (int64_t i = 0; i < n; i++)
+      for (int s = 0; s < P.rowlen[i]; s++) rp[P.at(i, s) + 1]++;
+    for (int64_t I = 0; I < nc; I++) rp[I + 1] += rp[I];
+    std::vector<int32_t> ci(rp[nc]);
+    std::vector<int64_t> src(rp[nc]);
+    std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
+    for (int64_t i = 0; i < n; i++)
+      for (int s = 0; s < P.rowlen[i]; s++) {
+        int32_t I = P.at(i, s);
+        ci[pos[I]] = (int32_t)i;
+        src[pos[I]++] = P.idx(i, s);
+      }
+    for (int64_t I = 0; I < nc; I++)
+      if (rp[I + 1] - rp[I] >= NO_SLOT) return "restriction row too long";
+    L.R.build(nc, n, rp, ci, false);
+    L.r_src.assign(L.R.entries(), -1);
+    for (int64_t I = 0; I < nc; I++)
+      for (int t = 0; t < L.R.rowlen[I]; t++) L.r_src[L.R.idx(I, t)] = (int32_t)src[rp[I] + t];
+  }
+  const SellHost& R = L.R;
+  // A*P pattern and product map
+  {
+    RowSet rs(nc);
+    L.ap_rp.assign(n + 1, 0);
+    L.ap_col.clear();
+    L.ap_pp.assign(n + 1, 0);
+    int64_t nprod = 0;
+    for (int64_t i = 0; i < n; i++) {
+      for (int s = 0; s < A.rowlen[i]; s++) nprod += P.rowlen[A.at(i, s)];
+      L.ap_pp[i + 1] = nprod;
+    }
+    L.ap_map.resize(nprod);
+    int64_t q = 0;
+    for (int64_t i = 0; i < n; i++) {
+      rs.begin();
+      for (int s = 0; s < A.rowlen[i]; s++) {
+        int32_t k = A.at(i, s);
+        for (int m = 0; m < P.rowlen[k]; m++) rs.add(P.at(k, m), i);
+      }
+      rs.finish();
+      if (rs.cols.size() >= NO_SLOT) return "A*P row too long";
+      L.ap_col.insert(L.ap_col.end(), rs.cols.begin(), rs.cols.end());
+      L.ap_rp[i + 1] = (int64_t)L.ap_col.size();
+      for (int s = 0; s < A.rowlen[i]; s++) {
+        int32_t k = A.at(i, s);
+        for (int m = 0; m < P.rowlen[k]; m++) L.ap_map[q++] = (uint16_t)rs.slot[P.at(k, m)];
+      }
+    }
+  }
+  // R*(A*P) pattern (the next level's A) and product map
+  {
+    RowSet rs(nc);
+    std::vector<int64_t> rp(nc + 1, 0);
+    std::vector<int32_t> ci;
+    L.rap_pp.assign(nc + 1, 0);
+    int64_t nprod = 0;
+    for (int64_t I = 0; I < nc; I++) {
+      for (int t = 0; t < R.rowlen[I]; t++) {
+        int32_t i = R.at(I, t);
+        nprod += L.ap_rp[i + 1] - L.ap_rp[i];
+      }
+      L.rap_pp[I + 1] = nprod;
+    }
+    L.rap_map.resize(nprod);
+    std::vector<std::vector<int32_t>> rowcols(1);
+    int64_t q = 0;
+    std::vector<int32_t> slots_scratch;
+    std::vector<int64_t> rows_first(nc + 1, 0);
+    // first pass: patterns
+    for (int64_t I = 0; I < nc; I++) {
+      rs.begin();
+      rs.add((int32_t)I, I);
+      for (int t = 0; t < R.rowlen[I]; t++) {
+        int32_t i = R.at(I, t);
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) rs.add(L.ap_col[a], I);
+      }
+      rs.finish();
+      if (rs.cols.size() >= NO_SLOT) return "coarse operator row too long";
+      ci.insert(ci.end(), rs.cols.begin(), rs.cols.end());
+      rp[I + 1] = (int64_t)ci.size();
+      for (int t = 0; t < R.rowlen[I]; t++) {
+        int32_t i = R.at(I, t);
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
+          L.rap_map[q++] = (uint16_t)rs.slot[L.ap_col[a]];
+      }
+    }
+    next = LevelHost();
+    next.A.build(nc, nc, rp, ci, true);
+    next.diag_slot.resize(nc);
+    for (int64_t I = 0; I < nc; I++) {
+      auto b = ci.begin() + rp[I], e = ci.begin() + rp[I + 1];
+      next.diag_slot[I] = (uint16_t)(std::lower_bound(b, e, (int32_t)I) - b);
+    }
+  }
+  return "";
+}
+
+}  // namespace
+
+std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max) {
+  auto t0 = std::chrono::steady_clock::now();
+  T.levels.resize(1);
+  while ((int)T.levels.size() < max_levels && T.levels.back().A.nrows > coarse_max) {
+    LevelHost& L = T.levels.back();
+    int64_t nc = aggregate(L.A, L.agg);
+    if (nc <= 0 || (double)nc > 0.85 * (double)L.A.nrows) {
+      L.agg.clear();
+      break;
+    }
+    LevelHost next;
+    std::string err = build_transfer(L, next, nc);
+    if (!err.empty()) return err;
+    T.levels.push_back(std::move(next));
+  }
+  T.coarse_dense = T.levels.back().A.nrows <= dense_max;
+  T.setup_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return "";
+}
+
+}  // namespace dpf

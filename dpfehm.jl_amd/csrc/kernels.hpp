@@ -1,0 +1,704 @@
+// Device kernels of the groundwater solve and adjoint (gfx950, wave64).
+//
+// All kernels are memory-bound row or edge loops; none uses MFMA.
+//  * Matrices are SELL-64: one matrix row per lane, the 64 rows of a slice
+//    share a wavefront, entries column-major inside the slice, so every
+//    value/column load of a wavefront is one contiguous 256/512-byte burst.
+//  * Row-parallel grids remap the hardware block index so that each of the
+//    8 XCDs (own L2 each) processes one contiguous band of rows: the x-vector
+//    gathers of a band then stay inside that XCD's L2.
+//  * Nothing scatters: assembly, transfer operators and gradients are row
+//    or cell gathers through index maps built on the host, so results are
+//    bitwise reproducible.  Reductions are per-block shuffles plus a
+//    last-block-done pass over the block partials in fixed order (integer
+//    ticket atomics only, no floating-point atomics).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dpf {
+
+constexpr int BLK = 256;
+constexpr uint16_t NO_SLOT_D = 0xFFFF;
+
+// Device-side scalars of one Krylov solve (all FP64, also in mixed mode).
+struct SolveScalars {
+  double rr;       // current ||r||^2
+  double rr_stop;  // stop when rr <= rr_stop
+  double rz, rz_old;
+  double pq, alpha;
+  double outer_rr;  // refinement: ||b - A x||^2 in FP64
+  double outer_s;   // refinement: scale of the inner right-hand side
+  int iters, maxiter;
+  int done;  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
+  int first;
+};
+
+struct RedWS {
+  double* partial;
+  unsigned int* counter;
+};
+
+template <typename T>
+struct SellD {
+  int64_t nrows;
+  const int64_t* sptr;
+  const int32_t* col;
+  T* val;
+};
+
+__device__ __forceinline__ int64_t logical_row() {
+  // Hardware dispatches block b to XCD b % 8.  Give XCD x the contiguous
+  // logical blocks [start_x, start_x + count_x).
+  int b = blockIdx.x, nb = gridDim.x;
+  int q = nb >> 3, r = nb & 7;
+  int x = b & 7, p = b >> 3;
+  int lb = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + p;
+  return (int64_t)lb * BLK + threadIdx.x;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over the block; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return s;
+}
+
+// Grid-wide deterministic sum.  Returns true (in every thread of the block)
+// only in the last block to finish, which holds the total in *total.
+__device__ __forceinline__ bool grid_sum(double v, RedWS ws, double* total) {
+  __shared__ double sh[4];
+  __shared__ int last;
+  __shared__ double tot;
+  double bs = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    ws.partial[blockIdx.x] = bs;
+    __threadfence();
+    unsigned t = atomicAdd(ws.counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double s = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += BLK)
+    s += __hip_atomic_load(ws.partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    tot = s;
+    atomicExch(ws.counter, 0u);
+  }
+  __syncthreads();
+  *total = tot;
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void sell_row(const SellD<T>& A, int64_t r, int64_t& base, int& w) {
+  int64_t s = r >> 6;
+  int64_t p0 = A.sptr[s];
+  base = p0 + (r & 63);
+  w = (int)((A.sptr[s + 1] - p0) >> 6);
+}
+
+// ------------------------------------------------------------------ edge maps
+
+// K_e = hm(exp(logK_a), exp(logK_b)), hm(x, y) = 2 / (1/x + 1/y).
+__global__ void k_hm_forward(int64_t ne, const int2* __restrict__ edges,
+                             const double* __restrict__ logK, double* __restrict__ Ke) {
+  int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ne) return;
+  int2 ab = edges[e];
+  double ka = exp(logK[ab.x]), kb = exp(logK[ab.y]);
+  Ke[e] = 2.0 / (1.0 / ka + 1.0 / kb);
+}
+
+// dlogK_c = sum_{e incident to c} dK_e K_e k_other / (k_c + k_other), in
+// edge-list order per cell.
+__global__ void k_hm_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
+                              const int32_t* __restrict__ inc_edge,
+                              const int32_t* __restrict__ inc_other,
+                              const double* __restrict__ logK, const double* __restrict__ Ke,
+                              const double* __restrict__ dKe, double* __restrict__ dlogK) {
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double kc = exp(logK[c]);
+  double s = 0.0;
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+    int32_t e = inc_edge[p];
+    double ko = exp(logK[inc_other[p]]);
+    s += dKe[e] * Ke[e] * ko / (kc + ko);
+  }
+  dlogK[c] = s;
+}
+
+// ------------------------------------------------------------------ assembly
+
+// Row j of  M = tscale * dr/dh (+ diag(svec) if given), and optionally
+// b_j = Q_i + sum over Dirichlet neighbours of K_e aol_e dh_o, for free cell
+// i = f2n[j].  Values are written in FP64 and, if val32 is given, also in FP32.
+// invd = 1 / (m_jj + sum_k |m_jk|) (l1-Jacobi diagonal).
+__global__ void k_assemble(int64_t nf, const int32_t* __restrict__ f2n,
+                           const int64_t* __restrict__ inc_ptr,
+                           const int32_t* __restrict__ inc_edge,
+                           const int32_t* __restrict__ inc_other,
+                           const uint16_t* __restrict__ inc_slot,
+                           const uint16_t* __restrict__ diag_slot,
+                           const double* __restrict__ Ke, const double* __restrict__ aol,
+                           const double* __restrict__ dh, const double* __restrict__ Q,
+                           double tscale, const double* __restrict__ svec, SellD<double> A,
+                           float* __restrict__ val32, double* __restrict__ invd64,
+                           float* __restrict__ invd32, double* __restrict__ b) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  int64_t base;
+  int w;
+  sell_row(A, j, base, w);
+  for (int k = 0; k < w; k++) A.val[base + (int64_t)k * 64] = 0.0;
+  int32_t i = f2n[j];
+  double dsum = 0.0, bj = Q ? Q[i] : 0.0;
+  for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
+    int32_t e = inc_edge[p];
+    double t = Ke[e] * aol[e];
+    dsum += t;
+    uint16_t sl = inc_slot[p];
+    if (sl != NO_SLOT_D)
+      A.val[base + (int64_t)sl * 64] -= tscale * t;
+    else
+      bj += t * dh[inc_other[p]];
+  }
+  int ds = diag_slot[j];
+  double dv = tscale * dsum + (svec ? svec[j] : 0.0);
+  A.val[base + (int64_t)ds * 64] = dv;
+  double l1 = dv;
+  for (int k = 0; k < w; k++)
+    if (k != ds) l1 += fabs(A.val[base + (int64_t)k * 64]);
+  double inv = 1.0 / l1;
+  if (invd64) invd64[j] = inv;
+  if (invd32) invd32[j] = (float)inv;
+  if (val32)
+    for (int k = 0; k < w; k++) val32[base + (int64_t)k * 64] = (float)A.val[base + (int64_t)k * 64];
+  if (b) b[j] = bj;
+}
+
+// groundwater_residuals: r_j = -Q_i/s_i + sum_e K_e (h_i - h~_o) aol_e / s_i,
+// term order of src/groundwater.jl:4-20 (source first, then edges in order).
+__global__ void k_residuals(int64_t nf, const int32_t* __restrict__ f2n,
+                            const int64_t* __restrict__ inc_ptr,
+                            const int32_t* __restrict__ inc_edge,
+                            const int32_t* __restrict__ inc_other,
+                            const int32_t* __restrict__ n2f, const double* __restrict__ Ke,
+                            const double* __restrict__ aol, const double* __restrict__ h,
+                            const double* __restrict__ dh, const double* __restrict__ Q,
+                            const double* __restrict__ Ss, const double* __restrict__ V,
+                            double* __restrict__ r) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  int32_t i = f2n[j];
+  double s = Ss[i] * V[i];
+  double acc = -Q[i] / s;
+  double hi = h[j];
+  for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
+    int32_t e = inc_edge[p], o = inc_other[p];
+    int32_t fo = n2f[o];
+    double ho = fo >= 0 ? h[fo] : dh[o];
+    acc += Ke[e] * (hi - ho) * aol[e] / s;
+  }
+  r[j] = acc;
+}
+
+// ------------------------------------------------------------------ AMG numeric setup
+
+// P(i,:) = e_agg(i) - w invd_i sum_k a_ik e_agg(k), w = 4/3 (l1 diagonal,
+// whose spectral radius bound is 1).
+template <typename T>
+__global__ void k_psmooth(SellD<T> A, const T* __restrict__ invd,
+                          const uint16_t* __restrict__ p_slot,
+                          const uint16_t* __restrict__ p_self, SellD<T> P) {
+  int64_t i = logical_row();
+  if (i >= A.nrows) return;
+  int64_t pb, ab;
+  int pw, aw;
+  sell_row(P, i, pb, pw);
+  sell_row(A, i, ab, aw);
+  for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = (T)0;
+  T f = (T)(4.0 / 3.0) * invd[i];
+  for (int k = 0; k < aw; k++) {
+    int64_t e = ab + (int64_t)k * 64;
+    uint16_t sl = p_slot[e];
+    if (sl != NO_SLOT_D) P.val[pb + (int64_t)sl * 64] -= f * A.val[e];
+  }
+  P.val[pb + (int64_t)p_self[i] * 64] += (T)1;
+}
+
+// R = P': every R entry copies the P entry it mirrors.
+template <typename T>
+__global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
+                          const T* __restrict__ pval, T* __restrict__ rval) {
+  int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (e >= nent) return;
+  int32_t s = r_src[e];
+  rval[e] = s >= 0 ? pval[s] : (T)0;
+}
+
+// (A P)(i,:) = sum_k a_ik P(k,:), products in the host's enumeration order.
+template <typename T>
+__global__ void k_ap(SellD<T> A, SellD<T> P, const int64_t* __restrict__ ap_rp,
+                     const int64_t* __restrict__ ap_pp, const uint16_t* __restrict__ ap_map,
+                     const int32_t* __restrict__ rowlenA, const int32_t* __restrict__ rowlenP,
+                     T* __restrict__ apval) {
+  int64_t i = logical_row();
+  if (i >= A.nrows) return;
+  int64_t r0 = ap_rp[i], r1 = ap_rp[i + 1];
+  for (int64_t q = r0; q < r1; q++) apval[q] = (T)0;
+  int64_t ab;
+  int aw;
+  sell_row(A, i, ab, aw);
+  int la = rowlenA[i];
+  int64_t q = ap_pp[i];
+  for (int k = 0; k < la; k++) {
+    int64_t e = ab + (int64_t)k * 64;
+    T a = A.val[e];
+    int32_t c = A.col[e];
+    int64_t pb;
+    int pw;
+    sell_row(P, c, pb, pw);
+    int lp = rowlenP[c];
+    for (int m = 0; m < lp; m++) apval[r0 + ap_map[q++]] += a * P.val[pb + (int64_t)m * 64];
+  }
+}
+
+// A_c(I,:) = sum_i R(I,i) (A P)(i,:); also the next level's l1 diagonal.
+template <typename T>
+__global__ void k_rap(SellD<T> R, const int32_t* __restrict__ rowlenR,
+                      const int64_t* __restrict__ ap_rp, const T* __restrict__ apval,
+                      const int64_t* __restrict__ rap_pp, const uint16_t* __restrict__ rap_map,
+                      SellD<T> Ac, const uint16_t* __restrict__ diag_slot_c,
+                      T* __restrict__ invd_c) {
+  int64_t I = logical_row();
+  if (I >= Ac.nrows) return;
+  int64_t cb, rb;
+  int cw, rw;
+  sell_row(Ac, I, cb, cw);
+  sell_row(R, I, rb, rw);
+  for (int k = 0; k < cw; k++) Ac.val[cb + (int64_t)k * 64] = (T)0;
+  int lr = rowlenR[I];
+  int64_t q = rap_pp[I];
+  for (int t = 0; t < lr; t++) {
+    int64_t e = rb + (int64_t)t * 64;
+    T rv = R.val[e];
+    int32_t i = R.col[e];
+    for (int64_t a = ap_rp[i]; a < ap_rp[i + 1]; a++)
+      Ac.val[cb + (int64_t)rap_map[q++] * 64] += rv * apval[a];
+  }
+  int ds = diag_slot_c[I];
+  T l1 = Ac.val[cb + (int64_t)ds * 64];
+  for (int k = 0; k < cw; k++)
+    if (k != ds) l1 += fabs(Ac.val[cb + (int64_t)k * 64]);
+  invd_c[I] = (T)1 / l1;
+}
+
+// In-place Gauss-Jordan inverse of the (SPD) coarsest operator, in FP64, by
+// one workgroup.  W is nc*nc, row-major.  status[0] = 1 on a non-positive pivot.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32_t* __restrict__ rowlen,
+                                                        double* __restrict__ W, int* status) {
+  int nc = (int)Ac.nrows;
+  int tid = threadIdx.x, nt = blockDim.x;
+  for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) W[t] = 0.0;
+  __syncthreads();
+  for (int I = tid; I < nc; I += nt) {
+    int64_t cb;
+    int cw;
+    sell_row(Ac, I, cb, cw);
+    for (int k = 0; k < rowlen[I]; k++)
+      W[(int64_t)I * nc + Ac.col[cb + (int64_t)k * 64]] += (double)Ac.val[cb + (int64_t)k * 64];
+  }
+  __syncthreads();
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int k = 0; k < nc; k++) {
+    double piv = W[(int64_t)k * nc + k];
+    if (!(piv > 0.0)) {
+      if (tid == 0) bad = 1;
+      piv = 1.0;
+    }
+    double ip = 1.0 / piv;
+    for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) {
+      int i = (int)(t / nc), j = (int)(t % nc);
+      if (i != k && j != k) W[t] -= W[(int64_t)i * nc + k] * W[(int64_t)k * nc + j] * ip;
+    }
+    __syncthreads();
+    for (int t = tid; t < nc; t += nt) {
+      if (t != k) {
+        W[(int64_t)k * nc + t] *= ip;
+        W[(int64_t)t * nc + k] *= -ip;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) W[(int64_t)k * nc + k] = ip;
+    __syncthreads();
+  }
+  if (tid == 0) status[0] = bad;
+}
+
+// ------------------------------------------------------------------ V-cycle
+
+// One Chebyshev step of the l1-Jacobi-preconditioned level operator:
+//   d <- c1 d + c2 invd (b - A xin);  xout = xin + d.
+// With dot != nullptr, also sums b . xout over the grid into sc->rz (the
+// fine-level sweep that completes z = M^-1 r, where b is the CG residual).
+// FINE marks the finest level's instantiation (a separate symbol in profiles).
+template <typename T, bool FINE>
+__global__ void k_cheb(SellD<T> A, const T* __restrict__ xin, T* __restrict__ xout,
+                       const T* __restrict__ b, T* __restrict__ d, const T* __restrict__ invd,
+                       T c1, T c2, const int* __restrict__ done, int fuse_dot, RedWS ws,
+                       SolveScalars* sc) {
+  if (*done) return;
+  int64_t r = logical_row();
+  double local = 0.0;
+  if (r < A.nrows) {
+    int64_t base;
+    int w;
+    sell_row(A, r, base, w);
+    T ax = (T)0;
+    for (int k = 0; k < w; k++) {
+      int64_t e = base + (int64_t)k * 64;
+      ax += A.val[e] * xin[A.col[e]];
+    }
+    T br = b[r];
+    T dn = c1 * d[r] + c2 * invd[r] * (br - ax);
+    d[r] = dn;
+    T xo = xin[r] + dn;
+    xout[r] = xo;
+    local = (double)br * (double)xo;
+  }
+  if (fuse_dot) {
+    double tot;
+    if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+      sc->rz_old = sc->rz;
+      sc->rz = tot;
+      if (!(tot > 0.0)) sc->done = 3;
+    }
+  }
+}
+
+// r = b - A x
+template <typename T>
+__global__ void k_residual(SellD<T> A, const T* __restrict__ x, const T* __restrict__ b,
+                           T* __restrict__ r, const int* __restrict__ done) {
+  if (*done) return;
+  int64_t i = logical_row();
+  if (i >= A.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(A, i, base, w);
+  T ax = (T)0;
+  for (int k = 0; k < w; k++) {
+    int64_t e = base + (int64_t)k * 64;
+    ax += A.val[e] * x[A.col[e]];
+  }
+  r[i] = b[i] - ax;
+}
+
+// bc = R rf; if init, also the zero-guess first Chebyshev step on the coarse
+// level: dc = xc = c0 invdc bc.
+template <typename T>
+__global__ void k_restrict(SellD<T> R, const T* __restrict__ rf, T* __restrict__ bc,
+                           const T* __restrict__ invdc, T* __restrict__ dc, T* __restrict__ xc,
+                           T c0, int init, const int* __restrict__ done) {
+  if (*done) return;
+  int64_t I = logical_row();
+  if (I >= R.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(R, I, base, w);
+  T s = (T)0;
+  for (int k = 0; k < w; k++) {
+    int64_t e = base + (int64_t)k * 64;
+    s += R.val[e] * rf[R.col[e]];
+  }
+  bc[I] = s;
+  if (init) {
+    T dd = c0 * invdc[I] * s;
+    dc[I] = dd;
+    xc[I] = dd;
+  }
+}
+
+// xf += P xc
+template <typename T>
+__global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ xf,
+                          const int* __restrict__ done) {
+  if (*done) return;
+  int64_t i = logical_row();
+  if (i >= P.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(P, i, base, w);
+  T s = (T)0;
+  for (int k = 0; k < w; k++) {
+    int64_t e = base + (int64_t)k * 64;
+    s += P.val[e] * xc[P.col[e]];
+  }
+  xf[i] += s;
+}
+
+// Coarsest level: x = W b with the FP64 inverse W (symmetric, so column j
+// of W is row j and the loads are coalesced).
+template <typename T>
+__global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __restrict__ b,
+                               T* __restrict__ x, const int* __restrict__ done) {
+  if (*done) return;
+  extern __shared__ double bs[];
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) bs[j] = (double)b[j];
+  __syncthreads();
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < nc; j++) s += W[(int64_t)j * nc + i] * bs[j];
+    x[i] = (T)s;
+  }
+}
+
+// ------------------------------------------------------------------ CG
+
+// x = 0, r = b, rr = b.b, stop threshold; first smoothing step of the
+// V-cycle on r (d0 = z0 = c0 invd r).
+template <typename T>
+__global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r,
+                           T* __restrict__ x, const T* __restrict__ invd, T* __restrict__ d0,
+                           T* __restrict__ z0, T c0, double reltol2, double abstol2,
+                           int maxiter, RedWS ws, SolveScalars* sc) {
+  int64_t i = logical_row();
+  double local = 0.0;
+  if (i < n) {
+    T bv = b[i];
+    r[i] = bv;
+    x[i] = (T)0;
+    T dd = c0 * invd[i] * bv;
+    d0[i] = dd;
+    z0[i] = dd;
+    local = (double)bv * (double)bv;
+  }
+  double tot;
+  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+    sc->rr = tot;
+    double stop = reltol2 * tot;
+    sc->rr_stop = stop > abstol2 ? stop : abstol2;
+    sc->iters = 0;
+    sc->maxiter = maxiter;
+    sc->first = 1;
+    sc->rz = 0.0;
+    sc->rz_old = 0.0;
+    sc->done = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
+  }
+}
+
+// p_new = z + beta p_old,  q = A p_new (formed from z and p_old at the
+// neighbours, so p is never read while it is rewritten),  alpha = rz / p.q.
+template <typename T>
+__global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+                        T* __restrict__ pnew, T* __restrict__ q, const int* __restrict__ done,
+                        RedWS ws, SolveScalars* sc) {
+  if (*done) return;
+  int first = sc->first;
+  T beta = first ? (T)0 : (T)(sc->rz / sc->rz_old);
+  int64_t i = logical_row();
+  double local = 0.0;
+  if (i < A.nrows) {
+    int64_t base;
+    int w;
+    sell_row(A, i, base, w);
+    T s = (T)0, pn;
+    if (first) {
+      for (int k = 0; k < w; k++) {
+        int64_t e = base + (int64_t)k * 64;
+        s += A.val[e] * z[A.col[e]];
+      }
+      pn = z[i];
+    } else {
+      for (int k = 0; k < w; k++) {
+        int64_t e = base + (int64_t)k * 64;
+        int32_t c = A.col[e];
+        s += A.val[e] * (z[c] + beta * pold[c]);
+      }
+      pn = z[i] + beta * pold[i];
+    }
+    pnew[i] = pn;
+    q[i] = s;
+    local = (double)pn * (double)s;
+  }
+  double tot;
+  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+    sc->pq = tot;
+    sc->first = 0;
+    if (!(tot > 0.0))
+      sc->done = 3;
+    else
+      sc->alpha = sc->rz / tot;
+  }
+}
+
+// x += alpha p, r -= alpha q, rr = r.r, convergence test; first smoothing
+// step of the next V-cycle on r.
+template <typename T>
+__global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
+                        const T* __restrict__ p, const T* __restrict__ q,
+                        const T* __restrict__ invd, T* __restrict__ d0, T* __restrict__ z0, T c0,
+                        const int* __restrict__ done, RedWS ws, SolveScalars* sc) {
+  if (*done) return;
+  T alpha = (T)sc->alpha;
+  int64_t i = logical_row();
+  double local = 0.0;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    T rv = r[i] - alpha * q[i];
+    r[i] = rv;
+    T dd = c0 * invd[i] * rv;
+    d0[i] = dd;
+    z0[i] = dd;
+    local = (double)rv * (double)rv;
+  }
+  double tot;
+  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+    sc->rr = tot;
+    int it = sc->iters + 1;
+    sc->iters = it;
+    if (tot <= sc->rr_stop)
+      sc->done = 1;
+    else if (!(tot == tot))
+      sc->done = 3;
+    else if (it >= sc->maxiter)
+      sc->done = 2;
+  }
+}
+
+// ------------------------------------------------------------------ FP64 outer residual / refinement
+
+// r = b - A x in FP64 with ||r||^2 -> sc->outer_rr, sqrt -> sc->outer_s.
+__global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
+                                 const double* __restrict__ b, double* __restrict__ r,
+                                 RedWS ws, SolveScalars* sc) {
+  int64_t i = logical_row();
+  double local = 0.0;
+  if (i < A.nrows) {
+    int64_t base;
+    int w;
+    sell_row(A, i, base, w);
+    double ax = 0.0;
+    for (int k = 0; k < w; k++) {
+      int64_t e = base + (int64_t)k * 64;
+      ax += A.val[e] * x[A.col[e]];
+    }
+    double rv = b[i] - ax;
+    r[i] = rv;
+    local = rv * rv;
+  }
+  double tot;
+  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+    sc->outer_rr = tot;
+    sc->outer_s = sqrt(tot);
+  }
+}
+
+// inner right-hand side in FP32: b32 = r64 / s
+__global__ void k_scale_to_f32(int64_t n, const double* __restrict__ r, float* __restrict__ b32,
+                               const SolveScalars* __restrict__ sc) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  double s = sc->outer_s;
+  b32[i] = (float)(r[i] / (s > 0.0 ? s : 1.0));
+}
+
+// x64 += s * x32
+__global__ void k_refine(int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                         const SolveScalars* __restrict__ sc) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  x[i] += sc->outer_s * (double)d[i];
+}
+
+__global__ void k_zero_f64(int64_t n, double* __restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ scatter / gather / adjoint
+
+// h_i = hfree_{j(i)} on free cells, dirichleths_i on Dirichlet cells
+__global__ void k_scatter_h(int64_t n, const int32_t* __restrict__ n2f,
+                            const double* __restrict__ hfree, const double* __restrict__ dh,
+                            double* __restrict__ h) {
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  int32_t j = n2f[i];
+  h[i] = j >= 0 ? hfree[j] : dh[i];
+}
+
+// out_j = v_{f2n[j]}
+__global__ void k_gather_free(int64_t nf, const int32_t* __restrict__ f2n,
+                              const double* __restrict__ v, double* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (j >= nf) return;
+  out[j] = v[f2n[j]];
+}
+
+// dK_e = -aol_e (h~_a - h~_b)(lambda~_a - lambda~_b), lambda~ = 0 on Dirichlet cells
+__global__ void k_edge_grad(int64_t ne, const int2* __restrict__ edges,
+                            const int32_t* __restrict__ n2f, const double* __restrict__ aol,
+                            const double* __restrict__ hfree, const double* __restrict__ dh,
+                            const double* __restrict__ lam, double* __restrict__ dK) {
+  int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ne) return;
+  int2 ab = edges[e];
+  int32_t fa = n2f[ab.x], fb = n2f[ab.y];
+  double ha = fa >= 0 ? hfree[fa] : dh[ab.x];
+  double hb = fb >= 0 ? hfree[fb] : dh[ab.y];
+  double la = fa >= 0 ? lam[fa] : 0.0;
+  double lb = fb >= 0 ? lam[fb] : 0.0;
+  dK[e] = -aol[e] * (ha - hb) * (la - lb);
+}
+
+// dQ_i = lambda_{j(i)} (free), 0 (Dirichlet);
+// ddh_i = 0 (free), delta_i + sum_{e: other free} K_e aol_e lambda_other (Dirichlet)
+__global__ void k_ddh_dq(int64_t n, const int32_t* __restrict__ n2f,
+                         const int64_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_edge,
+                         const int32_t* __restrict__ inc_other, const double* __restrict__ Ke,
+                         const double* __restrict__ aol, const double* __restrict__ lam,
+                         const double* __restrict__ delta, double* __restrict__ ddh,
+                         double* __restrict__ dQ) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  int32_t j = n2f[i];
+  if (j >= 0) {
+    if (dQ) dQ[i] = lam[j];
+    if (ddh) ddh[i] = 0.0;
+    return;
+  }
+  if (dQ) dQ[i] = 0.0;
+  if (ddh) {
+    double s = delta[i];
+    for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
+      int32_t fo = n2f[inc_other[p]];
+      if (fo >= 0) {
+        int32_t e = inc_edge[p];
+        s += Ke[e] * aol[e] * lam[fo];
+      }
+    }
+    ddh[i] = s;
+  }
+}
+
+}  // namespace dpf

@@ -1,0 +1,147 @@
+/*
+ * dpfgpu.h -- C-ABI of libdpfgpu.so, the MI355X (gfx950) groundwater solve
+ * and adjoint behind DPFEHM's groundwater interface.
+ *
+ * Plain pointers and sizes only.  Index arguments follow the reference's
+ * 1-based convention (Julia Pair{Int,Int} neighbors reinterpret as 2*E
+ * int64 without copying).  Every call returns DPF_OK or an error code;
+ * dpf_last_error() gives the message of the calling thread's last failure.
+ *
+ * Array arguments of compute calls are host pointers when opts->memory is
+ * DPF_HOST (the library stages them through device buffers) and device
+ * pointers when it is DPF_DEVICE (work is queued on opts->stream, or on the
+ * topology's own stream when that is NULL).  Calls on one topology are
+ * serialised by the library.
+ */
+#ifndef DPFGPU_H
+#define DPFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dpf_topology dpf_topology; /* mesh + Dirichlet set + AMG symbolic setup */
+typedef struct dpf_state dpf_state;       /* what a pullback needs from its forward call */
+
+enum {
+  DPF_OK = 0,
+  DPF_EINVAL = 1,      /* bad argument (the reference would throw) */
+  DPF_EHIP = 2,        /* HIP runtime failure */
+  DPF_ENOCONV = 3,     /* iteration cap reached before the tolerance */
+  DPF_EBREAKDOWN = 4,  /* CG breakdown: matrix or preconditioner not SPD, or NaN */
+  DPF_ESINGULAR = 5,   /* structurally singular system (a free cell without edges) */
+  DPF_ENOMEM = 6
+};
+enum { DPF_FP64 = 0, DPF_MIXED = 1 };
+enum { DPF_HOST = 0, DPF_DEVICE = 1 };
+
+typedef struct {
+  double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
+  double abstol;
+  int32_t maxiter;     /* PCG iteration cap, summed over refinement steps */
+  int32_t precision;   /* DPF_FP64, or DPF_MIXED: FP32 AMG-PCG inside FP64 refinement */
+  double inner_reltol; /* DPF_MIXED: relative tolerance of each FP32 inner solve */
+  int32_t max_outer;   /* DPF_MIXED: refinement step cap */
+  int32_t cheb_degree; /* Chebyshev smoother degree per pre/post sweep (0: default 2) */
+  int32_t memory;      /* DPF_HOST or DPF_DEVICE */
+  int32_t flags;       /* reserved, 0 */
+  void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = topology stream */
+} dpf_opts;
+
+typedef struct {
+  int32_t iterations;  /* PCG iterations, all refinement steps */
+  int32_t outer;       /* refinement steps (1 for DPF_FP64) */
+  int32_t converged;   /* 1 if the FP64 residual met the tolerance */
+  int32_t levels;      /* AMG levels */
+  double relres;       /* final ||b - A h|| / ||b|| in FP64 */
+  double setup_ms;     /* assembly + numeric AMG refresh (GPU) */
+  double solve_ms;     /* Krylov solve(s) */
+  double total_ms;     /* the whole call, wall clock */
+} dpf_stats;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *dpf_last_error(void);
+
+/* Defaults: reltol 1e-10, abstol 0, maxiter 2000, DPF_FP64, inner_reltol 1e-5,
+ * max_outer 20, cheb_degree 2, DPF_HOST, stream NULL. */
+void dpf_opts_init(dpf_opts *opts);
+
+/* Topology cache.  Replaces the per-call work the reference repeats inside
+ * every generated function: getfreenodes (src/utilities.jl:10-24, effective
+ * src/twophase.jl:15-20), the COO -> sparse() pattern build of the
+ * NonlinearEquations-generated Jacobians (src/groundwater.jl:1,79) and the
+ * AMG setup of amg_solver (src/groundwater.jl:45-49) -- here the
+ * conductivity-independent part of a smoothed-aggregation hierarchy.
+ *   neighbors: 2*nedges int64, pairs (a, b), 1-based;
+ *   areasoverlengths: nedges;  dirichletnodes: ndirichlet, 1-based. */
+int dpf_topology_create(int64_t n, int64_t nedges, const int64_t *neighbors,
+                        const double *areasoverlengths, int64_t ndirichlet,
+                        const int64_t *dirichletnodes, dpf_topology **out);
+int dpf_topology_destroy(dpf_topology *topo);
+
+/* info[0..7] = n, nedges, nfree, nnz(A), levels, coarse rows, operator-complexity*1000,
+ * host symbolic setup time in microseconds. */
+int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
+
+/* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
+ * dirichleths, Qs, specificstorage, volumes), src/groundwater.jl:1-38.
+ *   hfree: nfree;  Ks: nedges;  dirichleths, Qs, specificstorage, volumes: n;
+ *   r_out: nfree. */
+int dpf_residuals(dpf_topology *topo, const double *hfree, const double *Ks,
+                  const double *dirichleths, const double *Qs, const double *specificstorage,
+                  const double *volumes, double *r_out, const dpf_opts *opts);
+
+/* Edge conductivity from cell log-conductivity, K_e = hm(exp(logK_a),
+ * exp(logK_b)) with DPFEHM.hm (src/richards.jl:50-52), and its pullback
+ * dlogK = (dK_e/dlogK)' dK (the scatter Zygote performs for the user-side
+ * map of examples/gw_steadystate_3d/inputdeck.jl:51), as a cell gather.
+ *   logK: n;  Ke, dKe: nedges;  dlogK: n. */
+int dpf_hm_forward(dpf_topology *topo, const double *logK, double *Ke, const dpf_opts *opts);
+int dpf_hm_pullback(dpf_topology *topo, const double *logK, const double *Ke,
+                    const double *dKe, double *dlogK, const dpf_opts *opts);
+
+/* groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes,
+ * dirichleths, Qs; kwargs...) (src/groundwater.jl:75-83) and the forward
+ * half of its rrule (src/groundwater.jl:85-92): assemble A = dr/dh and
+ * b = -r(0) at Ss = V = 1, solve with AMG-preconditioned CG, scatter.
+ *   Ks: nedges;  dirichleths, Qs: n;  h_out: n.
+ * If state_out is non-NULL it receives a state for dpf_steady_pullback
+ * (release with dpf_state_destroy).  stats may be NULL. */
+int dpf_steady_forward(dpf_topology *topo, const double *Ks, const double *dirichleths,
+                       const double *Qs, double *h_out, const dpf_opts *opts,
+                       dpf_stats *stats, dpf_state **state_out);
+
+/* The rrule pullback (src/groundwater.jl:93-107): lambda = A'^-1 delta[free]
+ * through the same hierarchy (A is symmetric), then
+ *   dKs = -(dr/dKs)' lambda,  ddirichleths = -(dr/ddh)' lambda + delta .* !free,
+ *   dQs = -(dr/dQs)' lambda.
+ *   delta: n;  dKs: nedges;  ddirichleths, dQs: n.  Output pointers may be
+ * NULL to skip that tangent. */
+int dpf_steady_pullback(dpf_state *state, const double *delta, double *dKs,
+                        double *ddirichleths, double *dQs, const dpf_opts *opts,
+                        dpf_stats *stats);
+
+int dpf_state_destroy(dpf_state *state);
+
+/* Kernel timing for roofline reporting (no counterpart in the reference).
+ * Launches one kernel of the current hierarchy `reps` times back to back
+ * between two hipEvents on opts->stream (or the topology stream) and returns
+ * the mean duration per launch and the algorithmic bytes one launch moves:
+ *   kernel 0: finest-level Chebyshev sweep  d = c1 d + c2 D^-1 (b - A x), x' = x + d
+ *             bytes = nnz (w + 4) + 6 nfree w      (w = 8 FP64, 4 FP32)
+ *   kernel 1: CG direction + SpMV  p' = z + beta p, q = A p'
+ *             bytes = nnz (w + 4) + 4 nfree w
+ *   kernel 2: FP64 refinement residual  r = b - A x
+ *             bytes = nnz 12 + 3 nfree 8
+ * precision selects the FP64 or the FP32 (mixed) hierarchy; it must have been
+ * used by a solve on this topology. */
+int dpf_profile_kernel(dpf_topology *topo, int32_t precision, int32_t kernel, int32_t reps,
+                       double *avg_ms, double *bytes, const dpf_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPFGPU_H */

@@ -1,0 +1,41 @@
+/*
+ * dpfgpu_inspect.h -- read-only view of libdpfgpu's host symbolic setup.
+ *
+ * No counterpart in the reference.  It builds the same topology cache and
+ * AMG patterns as dpf_topology_create, on the host only (no GPU needed), and
+ * exposes the index arrays the GPU kernels consume, so that they can be
+ * checked against independent sparse algebra on a machine without a GPU.
+ *
+ * Array names and element types (level-independent ones ignore `level`):
+ *   n2f, f2n, inc_edge, inc_other: int32   inc_ptr: int64   inc_slot: uint16
+ *   A.sptr, P.sptr, R.sptr, ap_rp, ap_pp, rap_pp: int64
+ *   A.col, A.rowlen, P.col, P.rowlen, R.col, R.rowlen, agg, r_src, ap_col: int32
+ *   diag_slot, p_slot, p_self, ap_map, rap_map: uint16
+ */
+#ifndef DPFGPU_INSPECT_H
+#define DPFGPU_INSPECT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dpf_inspect dpf_inspect;
+
+const char *dpf_inspect_last_error(void);
+int dpf_inspect_create(int64_t n, int64_t nedges, const int64_t *neighbors,
+                       const double *areasoverlengths, int64_t ndirichlet,
+                       const int64_t *dirichletnodes, int32_t max_levels, int32_t coarse_max,
+                       dpf_inspect **out);
+void dpf_inspect_destroy(dpf_inspect *h);
+int32_t dpf_inspect_levels(const dpf_inspect *h);
+int64_t dpf_inspect_size(const dpf_inspect *h, int32_t level, const char *name);
+int dpf_inspect_get(const dpf_inspect *h, int32_t level, const char *name, void *out,
+                    int64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPFGPU_INSPECT_H */

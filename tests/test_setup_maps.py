@@ -1,0 +1,210 @@
+"""The host symbolic setup (topology maps and AMG patterns) that the GPU
+kernels consume, checked without a GPU: the numeric steps of the kernels are
+replayed in numpy through the same maps and compared with independent sparse
+algebra (scipy) -- the fine Jacobian against the oracle, P against
+(I - 4/3 D_l1^-1 A) T, R against P', A*P and R*A*P against scipy products."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+NO_SLOT = 0xFFFF
+DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.int32,
+      "inc_ptr": np.int64, "inc_slot": np.uint16, "A.sptr": np.int64, "P.sptr": np.int64,
+      "R.sptr": np.int64, "ap_rp": np.int64, "ap_pp": np.int64, "rap_pp": np.int64,
+      "A.col": np.int32, "A.rowlen": np.int32, "P.col": np.int32, "P.rowlen": np.int32,
+      "R.col": np.int32, "R.rowlen": np.int32, "agg": np.int32, "r_src": np.int32,
+      "ap_col": np.int32, "diag_slot": np.uint16, "p_slot": np.uint16, "p_self": np.uint16,
+      "ap_map": np.uint16, "rap_map": np.uint16}
+
+
+class Inspect:
+    def __init__(self, n, nb, aol, dn, max_levels=25, coarse_max=16):
+        from dpfehm_jl_amd import _lib
+        self.L = _lib.lib()
+        nb = np.ascontiguousarray(nb, dtype=np.int64)
+        aol = np.ascontiguousarray(aol[:nb.shape[0]], dtype=np.float64)
+        dn = np.ascontiguousarray(dn, dtype=np.int64)
+        h = ctypes.c_void_p()
+        rc = self.L.dpf_inspect_create(n, nb.shape[0], nb.ctypes.data, aol.ctypes.data, dn.size,
+                                       dn.ctypes.data, max_levels, coarse_max, ctypes.byref(h))
+        assert rc == 0, self.L.dpf_inspect_last_error()
+        self.h = h
+        self.levels = self.L.dpf_inspect_levels(h)
+
+    def get(self, name, level=0):
+        sz = self.L.dpf_inspect_size(self.h, level, name.encode())
+        assert sz >= 0, name
+        out = np.empty(sz, dtype=DT[name])
+        assert self.L.dpf_inspect_get(self.h, level, name.encode(), out.ctypes.data, sz) == 0
+        return out
+
+    def close(self):
+        self.L.dpf_inspect_destroy(self.h)
+
+
+def sell_idx(sptr, r, s):
+    return sptr[r // 64] + s * 64 + (r % 64)
+
+
+def sell_to_csr(sptr, col, rowlen, val, nrows, ncols):
+    rows, cols, vals = [], [], []
+    for r in range(nrows):
+        for s in range(rowlen[r]):
+            e = sell_idx(sptr, r, s)
+            rows.append(r); cols.append(col[e]); vals.append(val[e])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(nrows, ncols))
+
+
+def replay(I, Ke, aol, dh, Q):
+    """numpy replay of k_assemble, k_psmooth, k_rgather, k_ap, k_rap."""
+    f2n, inc_ptr, inc_edge = I.get("f2n"), I.get("inc_ptr"), I.get("inc_edge")
+    inc_other, inc_slot = I.get("inc_other"), I.get("inc_slot")
+    sptr, col, rowlen, dsl = I.get("A.sptr"), I.get("A.col"), I.get("A.rowlen"), I.get("diag_slot")
+    nf = f2n.size
+    val = np.zeros(sptr[-1])
+    b = np.zeros(nf)
+    invd = np.zeros(nf)
+    for j in range(nf):
+        i = f2n[j]
+        dsum, bj = 0.0, Q[i]
+        for p in range(inc_ptr[i], inc_ptr[i + 1]):
+            t = Ke[inc_edge[p]] * aol[inc_edge[p]]
+            dsum += t
+            if inc_slot[p] != NO_SLOT:
+                val[sell_idx(sptr, j, int(inc_slot[p]))] -= t
+            else:
+                bj += t * dh[inc_other[p]]
+        val[sell_idx(sptr, j, int(dsl[j]))] = dsum
+        w = (sptr[j // 64 + 1] - sptr[j // 64]) // 64
+        row = [val[sell_idx(sptr, j, s)] for s in range(w)]
+        invd[j] = 1.0 / (dsum + sum(abs(v) for k, v in enumerate(row) if k != dsl[j]))
+        b[j] = bj
+    levels = [dict(sptr=sptr, col=col, rowlen=rowlen, val=val, invd=invd, n=nf)]
+    for l in range(I.levels - 1):
+        A = levels[-1]
+        n = A["n"]
+        Ps, Pc, Pl = I.get("P.sptr", l), I.get("P.col", l), I.get("P.rowlen", l)
+        ps, pself = I.get("p_slot", l), I.get("p_self", l)
+        pv = np.zeros(Ps[-1])
+        for i in range(n):
+            f = 4.0 / 3.0 * A["invd"][i]
+            for s in range(A["rowlen"][i]):
+                e = sell_idx(A["sptr"], i, s)
+                pv[sell_idx(Ps, i, int(ps[e]))] -= f * A["val"][e]
+            pv[sell_idx(Ps, i, int(pself[i]))] += 1.0
+        rsrc = I.get("r_src", l)
+        rv = np.where(rsrc >= 0, pv[np.maximum(rsrc, 0)], 0.0)
+        ap_rp, ap_pp, ap_map = I.get("ap_rp", l), I.get("ap_pp", l), I.get("ap_map", l)
+        apv = np.zeros(ap_rp[-1])
+        for i in range(n):
+            q = ap_pp[i]
+            for s in range(A["rowlen"][i]):
+                e = sell_idx(A["sptr"], i, s)
+                k = A["col"][e]
+                for m in range(Pl[k]):
+                    apv[ap_rp[i] + ap_map[q]] += A["val"][e] * pv[sell_idx(Ps, k, m)]
+                    q += 1
+            assert q == ap_pp[i + 1]
+        Rs, Rc, Rl = I.get("R.sptr", l), I.get("R.col", l), I.get("R.rowlen", l)
+        rap_pp, rap_map = I.get("rap_pp", l), I.get("rap_map", l)
+        Cs, Cc, Cl, Cd = (I.get("A.sptr", l + 1), I.get("A.col", l + 1), I.get("A.rowlen", l + 1),
+                          I.get("diag_slot", l + 1))
+        nc = Cl.size
+        cv = np.zeros(Cs[-1])
+        cinv = np.zeros(nc)
+        for Ir in range(nc):
+            q = rap_pp[Ir]
+            for t in range(Rl[Ir]):
+                e = sell_idx(Rs, Ir, t)
+                i = Rc[e]
+                for a in range(ap_rp[i], ap_rp[i + 1]):
+                    cv[sell_idx(Cs, Ir, int(rap_map[q]))] += rv[e] * apv[a]
+                    q += 1
+            w = (Cs[Ir // 64 + 1] - Cs[Ir // 64]) // 64
+            row = [cv[sell_idx(Cs, Ir, s)] for s in range(w)]
+            cinv[Ir] = 1.0 / (row[Cd[Ir]] + sum(abs(v) for k, v in enumerate(row) if k != Cd[Ir]))
+        A["P"] = sell_to_csr(Ps, Pc, Pl, pv, n, nc)
+        A["R"] = sell_to_csr(Rs, Rc, Rl, rv, nc, n)
+        A["AP"] = sp.csr_matrix((apv, I.get("ap_col", l), ap_rp), shape=(n, nc))
+        A["agg"] = I.get("agg", l)
+        levels.append(dict(sptr=Cs, col=Cc, rowlen=Cl, val=cv, invd=cinv, n=nc))
+    for L in levels:
+        L["A"] = sell_to_csr(L["sptr"], L["col"], L["rowlen"], L["val"], L["n"], L["n"])
+    return levels, b
+
+
+@pytest.mark.parametrize("shape,dirichlet", [((9, 8, 6), "top"), ((12, 7, 5), "last")])
+def test_replayed_setup_matches_sparse_algebra(pkg, oracle, shape, dirichlet):
+    w = pkg.workloads.box_problem("t", shape, (0, 0, 0), (4.0, 3.5, 2.5), lam=3.0, sigma=2.0,
+                                  field_seed=3, dirichlet=dirichlet, sources=((1, 1e-4),),
+                                  n_injectors=2, n_obs=4)
+    rng = np.random.default_rng(0)
+    dh = w.dirichleths + rng.normal(size=w.n)
+    Ke = w.Ks()
+    I = Inspect(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes, coarse_max=16)
+    try:
+        assert I.levels >= 3
+        levels, b = replay(I, Ke, w.areasoverlengths, dh, w.Qs)
+    finally:
+        I.close()
+    one = np.ones(w.n)
+    nf = w.n - w.dirichletnodes.size
+    args = (Ke, w.neighbors, w.areasoverlengths, w.dirichletnodes, dh, w.Qs, one, one)
+    A_ref = oracle.groundwater_h(np.zeros(nf), *args)
+    b_ref = -oracle.groundwater_residuals(np.zeros(nf), *args)
+    assert abs(levels[0]["A"] - A_ref).max() <= 1e-14 * abs(A_ref).max()
+    assert np.allclose(b, b_ref, rtol=1e-14, atol=1e-20)
+    for l in range(len(levels) - 1):
+        L = levels[l]
+        A = L["A"]
+        n, nc = A.shape[0], levels[l + 1]["n"]
+        T = sp.csr_matrix((np.ones(n), (np.arange(n), L["agg"])), shape=(n, nc))
+        d_l1 = A.diagonal() + (abs(A).sum(axis=1).A1 - abs(A.diagonal()))
+        assert np.allclose(L["invd"], 1.0 / d_l1, rtol=1e-13)
+        P_ref = T - sp.diags(4.0 / 3.0 / d_l1) @ A @ T
+        assert abs(L["P"] - P_ref).max() <= 1e-13
+        assert abs(L["R"] - L["P"].T).max() == 0.0
+        assert abs(L["AP"] - A @ L["P"]).max() <= 1e-13 * abs(A).max()
+        Ac = levels[l + 1]["A"]
+        Ac_ref = L["P"].T @ A @ L["P"]
+        assert abs(Ac - Ac_ref).max() <= 1e-12 * abs(Ac_ref).max()
+        # Galerkin operators stay symmetric and every aggregate is non-empty
+        assert abs(Ac - Ac.T).max() <= 1e-12 * abs(Ac).max()
+        assert np.bincount(L["agg"], minlength=nc).min() >= 1
+    assert levels[-1]["n"] <= 16
+
+
+def test_topology_maps(pkg, oracle):
+    w = pkg.workloads.steady_2m(ns=(6, 5, 4), maxs=(2.5, 2.0, 1.5))
+    I = Inspect(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    try:
+        isfree, n2f, f2n = oracle.getfreenodes(w.n, w.dirichletnodes)
+        assert np.array_equal(I.get("n2f"), np.where(n2f > 0, n2f - 1, -1))
+        assert np.array_equal(I.get("f2n"), f2n - 1)
+        ptr, edge, other = I.get("inc_ptr"), I.get("inc_edge"), I.get("inc_other")
+        nb = w.neighbors - 1
+        for c in range(w.n):
+            es = [e for e in range(nb.shape[0]) if c in nb[e]]
+            assert list(edge[ptr[c]:ptr[c + 1]]) == es
+            assert all(other[p] == (nb[edge[p], 1] if nb[edge[p], 0] == c else nb[edge[p], 0])
+                       for p in range(ptr[c], ptr[c + 1]))
+    finally:
+        I.close()
+
+
+def test_topology_rejects_bad_input(pkg):
+    from dpfehm_jl_amd import _lib
+    L = _lib.lib()
+    nb = np.array([[1, 2], [2, 5]], dtype=np.int64)
+    aol = np.ones(2)
+    dn = np.array([1], dtype=np.int64)
+    h = ctypes.c_void_p()
+    assert L.dpf_inspect_create(3, 2, nb.ctypes.data, aol.ctypes.data, 1, dn.ctypes.data, 25, 16,
+                                ctypes.byref(h)) != 0
+    assert b"outside" in L.dpf_inspect_last_error()
+    nb = np.array([[1, 2]], dtype=np.int64)
+    assert L.dpf_inspect_create(3, 1, nb.ctypes.data, aol.ctypes.data, 1, dn.ctypes.data, 25, 16,
+                                ctypes.byref(h)) != 0
+    assert b"singular" in L.dpf_inspect_last_error()
