@@ -164,7 +164,7 @@ def main():
                                             ctypes.byref(by), ctypes.byref(o)))
             prof[(precision, kern)] = (ms.value, by.value)
             log(f"[bench] kernel {kern} {precision}: {ms.value * 1e3:.1f} us/launch, "
-                f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e9:.0f} GB/s")
+                f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e6:.0f} GB/s")
     ms, by = prof[("mixed", 0)]
     achieved = by / (ms * 1e-3) / 1e9
 

@@ -329,13 +329,18 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       int nc = (int)d.A.nrows;
       hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), nc * sizeof(double), s, nc,
                          (const double*)h.W.template as<double>(), bl, cur, (const int*)done);
+      // a one-level hierarchy: the CG scalar r.z still has to be formed
+      if (fuse_dot_at_fine)
+        hipLaunchKernelGGL(k_rz_dot<T>, dim3(g), dim3(BLK), 0, s, d.A.nrows, bl, (const T*)cur,
+                           (const int*)done, ws, sc);
     } else {
       // no dense solve: a longer smoothing sweep (degree 2m) stands in
       Cheb c2;
       c2.make(2 * m, 1.0, t->cheb_ratio);
       for (int k = 1; k < 2 * m; k++) {
+        int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
         hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd,
-                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, 0, ws, sc);
+                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, ws, sc);
         std::swap(cur, oth);
       }
     }

@@ -472,6 +472,22 @@ __global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __
   }
 }
 
+// rz = b . z over the grid (used when the V-cycle ends without a fine-level
+// smoothing sweep to fuse it into).
+template <typename T>
+__global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict__ z,
+                         const int* __restrict__ done, RedWS ws, SolveScalars* sc) {
+  if (*done) return;
+  int64_t i = logical_row();
+  double local = i < n ? (double)b[i] * (double)z[i] : 0.0;
+  double tot;
+  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
+    sc->rz_old = sc->rz;
+    sc->rz = tot;
+    if (!(tot > 0.0)) sc->done = 3;
+  }
+}
+
 // ------------------------------------------------------------------ CG
 
 // x = 0, r = b, rr = b.b, stop threshold; first smoothing step of the
