@@ -76,6 +76,11 @@ SIGNATURES = {
     "dpf_steady_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts),
                                            ctypes.POINTER(Stats)]),
     "dpf_state_destroy": (ctypes.c_int, [_p]),
+    "dpf_transient_forward": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i32, ctypes.c_double,
+                                             _p, _i32, _p, _p, ctypes.POINTER(Opts),
+                                             ctypes.POINTER(Stats), ctypes.POINTER(_p)]),
+    "dpf_transient_pullback": (ctypes.c_int, [_p, _p, _p, _p, ctypes.POINTER(Opts),
+                                              ctypes.POINTER(Stats)]),
     "dpf_profile_kernel": (ctypes.c_int, [_p, _i32, _i32, _i32, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_inspect_last_error": (ctypes.c_char_p, []),
     "dpf_inspect_create": (ctypes.c_int, [_i64, _i64, _p, _p, _i64, _p, _i32, _i32,

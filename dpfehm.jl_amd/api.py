@@ -273,3 +273,72 @@ def logKs2Ks_harmonic_pullback(logKs, Ke, dKe, neighbors, dirichletnodes=(), are
     _lib.check(_lib.lib().dpf_hm_pullback(topo.handle, _ptr(logKs), _ptr(_f64(Ke)),
                                           _ptr(_f64(dKe)), _ptr(out), ctypes.byref(o)))
     return out
+
+
+def _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+               specificstorage, volumes, u0, nsteps, dt, qscale, checkpoint_every, keep,
+               return_states, stats, kwargs):
+    o = _opts(kwargs)
+    Qs = _f64(Qs)
+    n = Qs.shape[0]
+    topo = get_topology(n, neighbors, areasoverlengths, dirichletnodes)
+    u0 = _f64(u0)
+    if u0.shape[0] != topo.nfree:
+        raise ValueError("u0 must have one entry per free cell")
+    qscale = _f64(np.ones(nsteps + 1) if qscale is None else qscale)
+    if qscale.shape[0] != nsteps + 1:
+        raise ValueError("qscale must have nsteps + 1 entries")
+    uN = np.empty(topo.nfree)
+    states = np.empty((nsteps + 1, topo.nfree)) if return_states else None
+    st = _lib.Stats()
+    sh = ctypes.c_void_p()
+    code = _lib.lib().dpf_transient_forward(
+        topo.handle, _ptr(_f64(Ks)), _ptr(_f64(dirichleths)), _ptr(Qs),
+        _ptr(_f64(specificstorage)), _ptr(_f64(volumes)), _ptr(u0), int(nsteps), float(dt),
+        _ptr(qscale), int(checkpoint_every), _ptr(uN),
+        _ptr(states) if return_states else None, ctypes.byref(o), ctypes.byref(st),
+        ctypes.byref(sh) if keep else None)
+    if stats is not None:
+        stats.update(st.as_dict())
+    _lib.check(code)
+    return uN, states, (State(topo, sh) if keep else None), o
+
+
+def groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                          specificstorage, volumes, u0, nsteps, dt, qscale=None,
+                          return_states=False, stats=None, **kwargs):
+    """Fixed-step backward Euler of du/dt = -groundwater_residuals(u, ...), the
+    system DifferentiableBackwardEuler.steps integrates in
+    examples/gw_transient_3d/inputdeck.jl:60-92.  ``u0`` and the result are on
+    the free cells; ``qscale[n]`` scales Qs at t_n = n dt.  Returns u_N, or the
+    (nsteps + 1, nfree) matrix of states with ``return_states``."""
+    uN, states, _, _ = _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
+                                  Qs, specificstorage, volumes, u0, nsteps, dt, qscale, 0, False,
+                                  return_states, stats, kwargs)
+    return states if return_states else uN
+
+
+def rrule_groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                                specificstorage, volumes, u0, nsteps, dt, qscale=None,
+                                checkpoint_every=20, stats=None, **kwargs):
+    """Returns (u_N, pullback); pullback(dJ/du_N) -> (dJ/dKs, dJ/dQs), the
+    states between checkpoints recomputed during the reverse sweep."""
+    uN, _, state, o = _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
+                                 Qs, specificstorage, volumes, u0, nsteps, dt, qscale,
+                                 checkpoint_every, True, False, stats, kwargs)
+    ne, n = state.topo.ne, state.topo.n
+
+    def pullback(dJdu, stats=None):
+        g = _f64(dJdu)
+        dK = np.empty(ne)
+        dQ = np.empty(n)
+        st = _lib.Stats()
+        code = _lib.lib().dpf_transient_pullback(state._h, _ptr(g), _ptr(dK), _ptr(dQ),
+                                                 ctypes.byref(o), ctypes.byref(st))
+        if stats is not None:
+            stats.update(st.as_dict())
+        _lib.check(code)
+        return dK, dQ
+
+    pullback.state = state
+    return uN, pullback

@@ -185,6 +185,13 @@ struct dpf_state {
   uint64_t version;  // topo->version of the forward solve
   DBuf Ke, dh, hfree, lam, delta, Q;
   int precision;
+  // transient runs
+  int kind = 0;  // 0 steady, 1 backward Euler
+  int nsteps = 0, ck = 0;
+  double dt = 0;
+  std::vector<double> qs;  // Q scale per time level (nsteps + 1)
+  DBuf svec;               // S = Ss V on free cells
+  DBuf ckpt;               // checkpointed states u_0, u_ck, u_2ck, ...
 };
 
 namespace {
@@ -243,7 +250,8 @@ T* level_invd(dpf_topology* t, Hier<T>& h, int l) {
 }
 
 // Fine assembly from t->Ke (+ Q, dh): A64 (+ FP32 copy into h32 level 0), b64.
-void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool want_b) {
+void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool want_b,
+              double tscale = 1.0, const double* svec = nullptr) {
   LevelDev& L0 = *t->lev[0];
   float* v32 = nullptr;
   float* i32 = nullptr;
@@ -256,7 +264,7 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
                      t->f2n.as<int32_t>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
                      t->inc_other.as<int32_t>(), t->inc_slot.as<uint16_t>(),
                      L0.diag_slot.as<uint16_t>(), t->Ke.as<double>(), t->aol.as<double>(),
-                     t->dh.as<double>(), Q, 1.0, (const double*)nullptr,
+                     t->dh.as<double>(), Q, tscale, svec,
                      L0.A.view<double>(t->A64.as<double>()), v32, t->invd64.as<double>(), i32,
                      want_b ? t->b64.as<double>() : (double*)nullptr);
   HIPTRY(hipGetLastError());
@@ -622,7 +630,7 @@ double now_ms() {
 // Solve A x = b64 (device, FP64, nf) with the current hierarchy; x -> xout (device FP64).
 // Returns 0 / DPF_ENOCONV / DPF_EBREAKDOWN and fills stats.
 int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
-                 const dpf_opts* o, dpf_stats* st) {
+                 const dpf_opts* o, dpf_stats* st, const double* x0 = nullptr) {
   int64_t nf = t->H.nf;
   int deg = o->cheb_degree > 0 ? o->cheb_degree : t->cheb_degree;
   Cheb ch;
@@ -631,74 +639,73 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   SolveScalars* sc = t->scal.as<SolveScalars>();
   LevelDev& d0 = *t->lev[0];
   SellD<double> A64 = d0.A.view<double>(t->A64.as<double>());
+  unsigned g = nblocks(nf);
   int status = DPF_OK;
   int total_it = 0, outer = 0;
-  double relres = 0;
-  // ||b||
   t->x64.alloc(nf * sizeof(double));
   t->r64.alloc(nf * sizeof(double));
-  hipLaunchKernelGGL(k_zero_f64, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->x64.as<double>());
-  hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
-                     (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
-  HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
-  HIPTRY(hipStreamSynchronize(s));
-  double bnorm = t->scal_host->outer_s;
-  double target = std::max(o->reltol * bnorm, o->abstol);
-  if (o->precision == DPF_FP64) {
-    SolveResult r = pcg<double>(t, t->h64, s, b, o->reltol, o->abstol, o->maxiter, ch, 16);
-    total_it = r.iters;
-    outer = 1;
-    copy_dev(xout, t->h64.cx.p, nf * sizeof(double), s);
-    hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
-                       (const double*)xout, b, t->r64.as<double>(), ws, sc);
+  auto residual = [&]() -> double {  // r64 = b - A x64, returns ||r64||
+    hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
+                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
     HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
-    double rn = t->scal_host->outer_s;
-    relres = bnorm > 0 ? rn / bnorm : rn;
-    if (r.status == 3)
-      status = DPF_EBREAKDOWN;
-    else if (r.status == 2 && rn > target)
-      status = DPF_ENOCONV;
-  } else {
+    return t->scal_host->outer_s;
+  };
+  // ||b||, then the residual of the initial guess
+  hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>());
+  double bnorm = residual();
+  double target = std::max(o->reltol * bnorm, o->abstol);
+  double rn = bnorm;
+  if (x0) {
+    copy_dev(t->x64.p, x0, nf * sizeof(double), s);
+    rn = residual();
+  }
+  // Iterative refinement on x64: each step solves A d = r for the current FP64
+  // residual (FP64 CG in DPF_FP64 mode, one step to the full tolerance; FP32
+  // CG to inner_reltol in DPF_MIXED mode), then x64 += d.
+  int budget = o->maxiter;
+  // FP64: one CG to the full tolerance; a restart only if rounding left the
+  // true residual just above it
+  int max_outer = o->precision == DPF_FP64 ? 3 : o->max_outer;
+  if (o->precision == DPF_MIXED) {
     ensure_hier(t, t->h32);
     t->b32.alloc(nf * sizeof(float));
-    double rn = bnorm;
-    int budget = o->maxiter;
-    while (true) {
-      if (rn <= target) break;
-      if (outer >= o->max_outer || budget <= 0) {
-        status = DPF_ENOCONV;
-        break;
-      }
-      hipLaunchKernelGGL(k_scale_to_f32, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+  }
+  while (rn > target) {
+    if (outer >= max_outer || budget <= 0) {
+      status = DPF_ENOCONV;
+      break;
+    }
+    SolveResult r;
+    if (o->precision == DPF_FP64) {
+      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, ch, 16);
+      hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
+                         (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
+    } else {
+      hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(BLK), 0, s, nf,
                          (const double*)t->r64.as<double>(), t->b32.as<float>(),
                          (const SolveScalars*)sc);
-      SolveResult r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget,
-                                 ch, 8);
-      total_it += r.iters;
-      budget -= r.iters;
-      outer++;
-      if (r.status == 3) {
-        status = DPF_EBREAKDOWN;
-        break;
-      }
-      hipLaunchKernelGGL(k_refine, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->x64.as<double>(),
+      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, ch, 8);
+      hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
                          (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc);
-      hipLaunchKernelGGL(k_outer_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, A64,
-                         (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
-      HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
-      HIPTRY(hipStreamSynchronize(s));
-      double rn_new = t->scal_host->outer_s;
-      if (!(rn_new == rn_new)) {
-        status = DPF_EBREAKDOWN;
-        rn = rn_new;
-        break;
-      }
-      rn = rn_new;
     }
-    relres = bnorm > 0 ? rn / bnorm : rn;
-    copy_dev(xout, t->x64.p, nf * sizeof(double), s);
+    total_it += r.iters;
+    budget -= r.iters;
+    outer++;
+    if (r.status == 3) {
+      status = DPF_EBREAKDOWN;
+      break;
+    }
+    double rn_new = residual();
+    if (!(rn_new == rn_new)) {
+      status = DPF_EBREAKDOWN;
+      rn = rn_new;
+      break;
+    }
+    rn = rn_new;
   }
+  double relres = bnorm > 0 ? rn / bnorm : rn;
+  copy_dev(xout, t->x64.p, nf * sizeof(double), s);
   if (st) {
     st->iterations = total_it;
     st->outer = outer;
@@ -710,8 +717,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
 }
 
 // Assemble + numeric refresh for the conductivities already in t->Ke.
-void prepare(dpf_topology* t, hipStream_t s, int precision, const double* Q, bool want_b) {
-  assemble(t, s, precision == DPF_MIXED, Q, want_b);
+void prepare(dpf_topology* t, hipStream_t s, int precision, const double* Q, bool want_b,
+             double tscale = 1.0, const double* svec = nullptr) {
+  assemble(t, s, precision == DPF_MIXED, Q, want_b, tscale, svec);
   t->version++;
   if (precision == DPF_MIXED) {
     refresh<float>(t, t->h32, s);
@@ -987,7 +995,7 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
       hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
                          t->n2f.as<int32_t>(), t->aol.as<double>(),
                          (const double*)st->hfree.as<double>(), (const double*)st->dh.as<double>(),
-                         lam, dK);
+                         lam, 1.0, 0, dK);
     if (dd || dq)
       hipLaunchKernelGGL(k_ddh_dq, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
                          t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
@@ -1103,5 +1111,255 @@ extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t ke
       *avg_ms = profile_one<double>(t, t->h64, s, kernel, reps, bytes);
     }
     return DPF_OK;
+  });
+}
+
+// ====================================================================== transient
+
+namespace {
+
+// (S + dt A) and its hierarchy for the state's K, S and dt (bD -> t->b64)
+void prepare_be(dpf_topology* t, hipStream_t s, int precision, dpf_state* st) {
+  copy_dev(t->Ke.p, st->Ke.p, t->H.ne * sizeof(double), s);
+  copy_dev(t->dh.p, st->dh.p, t->H.n * sizeof(double), s);
+  prepare(t, s, precision, nullptr, true, st->dt, st->svec.as<double>());
+  st->version = t->version;
+}
+
+bool hier_fresh(dpf_topology* t, dpf_state* st, int precision) {
+  return t->version == st->version &&
+         (precision == DPF_MIXED ? t->h32.version == st->version : t->h64.version == st->version);
+}
+
+// u_{n} from u_{n-1}: one backward-Euler step (warm-started at u_{n-1})
+int be_step(dpf_topology* t, hipStream_t s, dpf_state* st, int n, const double* uprev,
+            double* unext, const dpf_opts* o, int& iters) {
+  int64_t nf = t->H.nf;
+  t->stage_f3.alloc(nf * sizeof(double));
+  double* rhs = t->stage_f3.as<double>();
+  hipLaunchKernelGGL(k_be_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                     (const double*)st->svec.as<double>(), uprev,
+                     (const double*)st->Q.as<double>(), st->qs[n], st->dt,
+                     (const double*)t->b64.as<double>(), rhs);
+  dpf_stats ss;
+  std::memset(&ss, 0, sizeof(ss));
+  int code = solve_system(t, s, rhs, unext, o, &ss, uprev);
+  iters += ss.iterations;
+  return code;
+}
+
+}  // namespace
+
+extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const double* dirichleths,
+                                     const double* Qs, const double* Ss, const double* V,
+                                     const double* u0, int32_t nsteps, double dt,
+                                     const double* qscale, int32_t checkpoint_every,
+                                     double* u_final, double* states_out, const dpf_opts* opts,
+                                     dpf_stats* stats, dpf_state** state_out) {
+  if (state_out) *state_out = nullptr;
+  if (!t || !Ks || !dirichleths || !Qs || !Ss || !V || !u0 || !qscale || !u_final ||
+      nsteps <= 0 || !(dt > 0)) {
+    set_err("NULL argument or non-positive nsteps/dt");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    auto st = std::make_unique<dpf_state>();
+    st->topo = t;
+    st->kind = 1;
+    st->precision = o->precision;
+    st->nsteps = nsteps;
+    st->dt = dt;
+    st->ck = checkpoint_every > 0 ? std::min<int>(checkpoint_every, nsteps) : nsteps;
+    st->Ke.alloc(ne * sizeof(double));
+    st->dh.alloc(n * sizeof(double));
+    st->Q.alloc(n * sizeof(double));
+    st->svec.alloc(nf * sizeof(double));
+    DBuf ssb, vb, u0b;
+    const double *ss_d, *v_d, *u0_d;
+    st->qs.resize(nsteps + 1);
+    if (o->memory == DPF_DEVICE) {
+      copy_dev(st->Ke.p, Ks, ne * sizeof(double), s);
+      copy_dev(st->dh.p, dirichleths, n * sizeof(double), s);
+      copy_dev(st->Q.p, Qs, n * sizeof(double), s);
+      ss_d = Ss;
+      v_d = V;
+      u0_d = u0;
+      HIPTRY(hipMemcpyAsync(st->qs.data(), qscale, (nsteps + 1) * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    } else {
+      HIPTRY(hipMemcpyAsync(st->Ke.p, Ks, ne * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPTRY(hipMemcpyAsync(st->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPTRY(hipMemcpyAsync(st->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
+      ss_d = stage_in(t, o, ssb, Ss, n, s);
+      v_d = stage_in(t, o, vb, V, n, s);
+      u0_d = stage_in(t, o, u0b, u0, nf, s);
+      std::memcpy(st->qs.data(), qscale, (nsteps + 1) * sizeof(double));
+    }
+    hipLaunchKernelGGL(k_storage, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                       ss_d, v_d, st->svec.as<double>());
+    HIPTRY(hipStreamSynchronize(s));
+    prepare_be(t, s, o->precision, st.get());
+    double t1 = now_ms();
+    int nck = nsteps / st->ck + 1;
+    st->ckpt.alloc((size_t)nck * nf * sizeof(double));
+    double* ck = st->ckpt.as<double>();
+    copy_dev(ck, u0_d, nf * sizeof(double), s);
+    DBuf ua, ub;
+    ua.alloc(nf * sizeof(double));
+    ub.alloc(nf * sizeof(double));
+    copy_dev(ua.p, u0_d, nf * sizeof(double), s);
+    if (states_out) {
+      if (o->memory == DPF_DEVICE)
+        copy_dev(states_out, u0_d, nf * sizeof(double), s);
+      else
+        HIPTRY(hipMemcpyAsync(states_out, u0_d, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    double* cur = ua.as<double>();
+    double* nxt = ub.as<double>();
+    int iters = 0, status = DPF_OK;
+    for (int k = 1; k <= nsteps; k++) {
+      int code = be_step(t, s, st.get(), k, cur, nxt, o, iters);
+      if (code != DPF_OK) {
+        status = code;
+        break;
+      }
+      std::swap(cur, nxt);
+      if (k % st->ck == 0) copy_dev(ck + (size_t)(k / st->ck) * nf, cur, nf * sizeof(double), s);
+      if (states_out) {
+        double* dst = states_out + (size_t)k * nf;
+        if (o->memory == DPF_DEVICE)
+          copy_dev(dst, cur, nf * sizeof(double), s);
+        else
+          HIPTRY(hipMemcpyAsync(dst, cur, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+      }
+    }
+    if (o->memory == DPF_DEVICE)
+      copy_dev(u_final, cur, nf * sizeof(double), s);
+    else
+      HIPTRY(hipMemcpyAsync(u_final, cur, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = iters;
+      stats->outer = nsteps;
+      stats->converged = status == DPF_OK;
+      stats->levels = t->levels();
+      stats->relres = 0;
+      stats->setup_ms = t1 - t0;
+      stats->solve_ms = now_ms() - t1;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status != DPF_OK) {
+      set_err("backward-Euler step did not converge");
+      return status;
+    }
+    if (state_out) *state_out = st.release();
+    return (int)DPF_OK;
+  });
+}
+
+extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, double* dKs,
+                                      double* dQs, const dpf_opts* opts, dpf_stats* stats) {
+  if (!st || !st->topo || st->kind != 1 || !dJ_du_final || !dKs) {
+    set_err("invalid argument (state must come from dpf_transient_forward)");
+    return DPF_EINVAL;
+  }
+  dpf_topology* t = st->topo;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    if (!hier_fresh(t, st, o->precision)) prepare_be(t, s, o->precision, st);
+    const double* gN = stage_in(t, o, t->stage_f, dJ_du_final, nf, s);
+    double* dK = dKs;
+    double* dQ = dQs;
+    DBuf bk, bq;
+    if (o->memory == DPF_HOST) {
+      bk.alloc(ne * sizeof(double));
+      dK = bk.as<double>();
+      if (dQs) {
+        bq.alloc(n * sizeof(double));
+        dQ = bq.as<double>();
+      }
+    }
+    int N = st->nsteps, C = st->ck;
+    DBuf seg, mua, mub, rhsb;
+    seg.alloc((size_t)(C + 1) * nf * sizeof(double));
+    mua.alloc(nf * sizeof(double));
+    mub.alloc(nf * sizeof(double));
+    rhsb.alloc(nf * sizeof(double));
+    double* segp = seg.as<double>();
+    double* mu_next = mua.as<double>();
+    double* mu = mub.as<double>();
+    double* rhs = rhsb.as<double>();
+    bool have_next = false, first_acc = true;
+    int iters = 0, status = DPF_OK;
+    for (int c = (N - 1) / C; c >= 0 && status == DPF_OK; c--) {
+      int a = c * C, b = std::min(N, (c + 1) * C);
+      // recompute u_{a..b} from checkpoint a into the segment buffer
+      copy_dev(segp, st->ckpt.as<double>() + (size_t)c * nf, nf * sizeof(double), s);
+      for (int k = a + 1; k <= b && status == DPF_OK; k++)
+        status = be_step(t, s, st, k, segp + (size_t)(k - a - 1) * nf,
+                         segp + (size_t)(k - a) * nf, o, iters);
+      for (int k = b; k > a && status == DPF_OK; k--) {
+        hipLaunchKernelGGL(k_be_adj_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                           (const double*)st->svec.as<double>(),
+                           have_next ? (const double*)mu_next : (const double*)nullptr,
+                           k == N ? gN : (const double*)nullptr, rhs);
+        dpf_stats ss;
+        std::memset(&ss, 0, sizeof(ss));
+        status = solve_system(t, s, rhs, mu, o, &ss, have_next ? mu_next : nullptr);
+        iters += ss.iterations;
+        if (status != DPF_OK) break;
+        const double* uk = segp + (size_t)(k - a) * nf;
+        hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                           t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), uk,
+                           (const double*)st->dh.as<double>(), (const double*)mu, st->dt,
+                           first_acc ? 0 : 1, dK);
+        if (dQ)
+          hipLaunchKernelGGL(k_dq_acc, dim3(nblocks(n)), dim3(BLK), 0, s, n,
+                             t->n2f.as<int32_t>(), (const double*)mu, st->dt * st->qs[k],
+                             first_acc ? 0 : 1, dQ);
+        first_acc = false;
+        std::swap(mu, mu_next);
+        have_next = true;
+      }
+    }
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST) {
+      HIPTRY(hipMemcpyAsync(dKs, dK, ne * sizeof(double), hipMemcpyDeviceToHost, s));
+      if (dQs) HIPTRY(hipMemcpyAsync(dQs, dQ, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = iters;
+      stats->outer = N;
+      stats->converged = status == DPF_OK;
+      stats->levels = t->levels();
+      stats->relres = 0;
+      stats->setup_ms = 0;
+      stats->solve_ms = now_ms() - t0;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status != DPF_OK) set_err("adjoint backward-Euler step did not converge");
+    return status;
   });
 }

@@ -646,6 +646,13 @@ __global__ void k_refine(int64_t n, double* __restrict__ x, const float* __restr
   x[i] += sc->outer_s * (double)d[i];
 }
 
+// y += a x
+__global__ void k_axpy_f64(int64_t n, double a, const double* __restrict__ x,
+                           double* __restrict__ y) {
+  int64_t i = logical_row();
+  if (i < n) y[i] += a * x[i];
+}
+
 __global__ void k_zero_f64(int64_t n, double* __restrict__ x) {
   int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i < n) x[i] = 0.0;
@@ -671,11 +678,13 @@ __global__ void k_gather_free(int64_t nf, const int32_t* __restrict__ f2n,
   out[j] = v[f2n[j]];
 }
 
-// dK_e = -aol_e (h~_a - h~_b)(lambda~_a - lambda~_b), lambda~ = 0 on Dirichlet cells
+// dK_e = -c aol_e (h~_a - h~_b)(lambda~_a - lambda~_b), lambda~ = 0 on Dirichlet
+// cells; added to dK when accumulate is set (the transient sum over steps).
 __global__ void k_edge_grad(int64_t ne, const int2* __restrict__ edges,
                             const int32_t* __restrict__ n2f, const double* __restrict__ aol,
                             const double* __restrict__ hfree, const double* __restrict__ dh,
-                            const double* __restrict__ lam, double* __restrict__ dK) {
+                            const double* __restrict__ lam, double c, int accumulate,
+                            double* __restrict__ dK) {
   int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (e >= ne) return;
   int2 ab = edges[e];
@@ -684,7 +693,50 @@ __global__ void k_edge_grad(int64_t ne, const int2* __restrict__ edges,
   double hb = fb >= 0 ? hfree[fb] : dh[ab.y];
   double la = fa >= 0 ? lam[fa] : 0.0;
   double lb = fb >= 0 ? lam[fb] : 0.0;
-  dK[e] = -aol[e] * (ha - hb) * (la - lb);
+  double g = -c * aol[e] * (ha - hb) * (la - lb);
+  dK[e] = accumulate ? dK[e] + g : g;
+}
+
+// ------------------------------------------------------------------ backward Euler
+
+// s_j = Ss_i V_i for free cell i = f2n[j]
+__global__ void k_storage(int64_t nf, const int32_t* __restrict__ f2n,
+                          const double* __restrict__ Ss, const double* __restrict__ V,
+                          double* __restrict__ svec) {
+  int64_t j = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (j >= nf) return;
+  int32_t i = f2n[j];
+  svec[j] = Ss[i] * V[i];
+}
+
+// rhs_j = s_j u_j + dt (q Q_i + bD_j): the right-hand side of (S + dt A) u_n = S u_{n-1} + dt b(t_n)
+__global__ void k_be_rhs(int64_t nf, const int32_t* __restrict__ f2n,
+                         const double* __restrict__ svec, const double* __restrict__ u,
+                         const double* __restrict__ Q, double q, double dt,
+                         const double* __restrict__ bD, double* __restrict__ rhs) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  rhs[j] = svec[j] * u[j] + dt * (q * Q[f2n[j]] + bD[j]);
+}
+
+// adjoint right-hand side: rhs_j = s_j mu_j (if mu) + g_j (if g)
+__global__ void k_be_adj_rhs(int64_t nf, const double* __restrict__ svec,
+                             const double* __restrict__ mu, const double* __restrict__ g,
+                             double* __restrict__ rhs) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  double v = mu ? svec[j] * mu[j] : 0.0;
+  rhs[j] = g ? v + g[j] : v;
+}
+
+// dQ_i (+)= c mu_{j(i)} on free cells, 0 on Dirichlet cells
+__global__ void k_dq_acc(int64_t n, const int32_t* __restrict__ n2f, const double* __restrict__ mu,
+                         double c, int accumulate, double* __restrict__ dQ) {
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  int32_t j = n2f[i];
+  double v = j >= 0 ? c * mu[j] : 0.0;
+  dQ[i] = accumulate ? dQ[i] + v : v;
 }
 
 // dQ_i = lambda_{j(i)} (free), 0 (Dirichlet);

@@ -125,6 +125,33 @@ int dpf_steady_pullback(dpf_state *state, const double *delta, double *dKs,
 
 int dpf_state_destroy(dpf_state *state);
 
+/* Transient groundwater flow by fixed-step backward Euler: the
+ * reference integrates du/dt = f(u) = -groundwater_residuals(u, Ks, ...,
+ * specificstorage, volumes) with DifferentiableBackwardEuler.steps
+ * (examples/gw_transient_3d/inputdeck.jl:60-92, test/theis.jl:61-106).  Each
+ * step solves (S + dt A) u_n = S u_{n-1} + dt b(t_n), S = diag(Ss V) on free
+ * cells, with Q(t_n) = qscale[n] Qs, by AMG-PCG warm-started at u_{n-1}.
+ *   Ks: nedges;  dirichleths, Qs, specificstorage, volumes: n;  u0, u_final:
+ *   nfree;  qscale: nsteps + 1;  states_out: NULL or (nsteps + 1) * nfree
+ *   (row k = u_k, like the columns of the reference's steps matrix).
+ * The state keeps every checkpoint_every-th state (0 = only u_0) for the
+ * pullback, which recomputes the states in between. */
+int dpf_transient_forward(dpf_topology *topo, const double *Ks, const double *dirichleths,
+                          const double *Qs, const double *specificstorage,
+                          const double *volumes, const double *u0, int32_t nsteps, double dt,
+                          const double *qscale, int32_t checkpoint_every, double *u_final,
+                          double *states_out, const dpf_opts *opts, dpf_stats *stats,
+                          dpf_state **state_out);
+
+/* Discrete adjoint of dpf_transient_forward for an objective of the final
+ * state (the rrule DifferentiableBackwardEuler provides):
+ *   (S + dt A) mu_N = dJ/du_N,  (S + dt A) mu_n = S mu_{n+1},
+ *   dKs_e = -sum_n dt aol_e (u~n_a - u~n_b)(mu~n_a - mu~n_b),
+ *   dQs_i = sum_n dt qscale[n] mu_n[i]  (0 on Dirichlet cells).
+ *   dJ_du_final: nfree;  dKs: nedges;  dQs: n or NULL. */
+int dpf_transient_pullback(dpf_state *state, const double *dJ_du_final, double *dKs,
+                           double *dQs, const dpf_opts *opts, dpf_stats *stats);
+
 /* Kernel timing for roofline reporting (no counterpart in the reference).
  * Launches one kernel of the current hierarchy `reps` times back to back
  * between two hipEvents on opts->stream (or the topology stream) and returns

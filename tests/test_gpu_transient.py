@@ -1,0 +1,65 @@
+"""GPU parity of the fixed-step backward-Euler transient solve and its
+checkpointed adjoint against the FP64 oracle (oracle/gw_oracle.c)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _problem(pkg, ns=(20, 18, 8), sigma=1.5):
+    w = pkg.workloads.steady_2m(sigma=sigma, ns=ns, maxs=(ns[0] * 0.5, ns[1] * 0.5, ns[2] * 0.5))
+    Ss = np.full(w.n, 1e-5)
+    return w, Ss
+
+
+@pytest.mark.parametrize("precision,ck", [("fp64", 7), ("fp64", 0), ("mixed", 5)])
+def test_transient_forward_adjoint_vs_oracle(pkg, oracle, precision, ck):
+    from dpfehm_jl_amd import api
+    print("transient", precision, ck, flush=True)
+    w, Ss = _problem(pkg)
+    K = w.Ks()
+    nf = w.n - w.dirichletnodes.size
+    N, dt = 24, 3600.0
+    qs = pkg.workloads.transient_schedule(N, dt, ramp_steps=10)
+    u0 = np.zeros(nf)
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs, Ss,
+            w.volumes)
+    ref = oracle.transient(*args, u0, N, dt, qs, reltol=1e-14)
+    st = {}
+    uN, pb = api.rrule_groundwater_transient(*args, u0, N, dt, qs, checkpoint_every=ck,
+                                             reltol=1e-12, precision=precision, stats=st)
+    print(st, flush=True)
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    assert rel(uN, ref[-1]) < tol
+    rng = np.random.default_rng(3)
+    g = np.zeros(nf)
+    g[rng.choice(nf, 30, replace=False)] = rng.normal(size=30)
+    G = np.zeros_like(ref)
+    G[-1] = g
+    dK_ref, dQ_ref = oracle.transient_adjoint(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                              w.dirichleths, Ss, w.volumes, N, dt, qs, ref, G,
+                                              reltol=1e-14)
+    dK, dQ = pb(g)
+    assert rel(dK, dK_ref) < 1e-7
+    assert rel(dQ, dQ_ref) < 1e-7
+
+
+def test_transient_states_and_steady_limit(pkg):
+    from dpfehm_jl_amd import api
+    w, Ss = _problem(pkg, ns=(16, 14, 6))
+    K = w.Ks()
+    nf = w.n - w.dirichletnodes.size
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs, Ss,
+            w.volumes)
+    states = api.groundwater_transient(*args, np.zeros(nf), 5, 1e9, return_states=True,
+                                       reltol=1e-13)
+    assert states.shape == (6, nf) and np.all(states[0] == 0)
+    h = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                    w.dirichleths, w.Qs, reltol=1e-13)
+    isfree = np.ones(w.n, dtype=bool)
+    isfree[w.dirichletnodes - 1] = False
+    assert rel(states[-1], h[isfree]) < 1e-8
