@@ -100,8 +100,10 @@ struct LevelDev {
   SellDev A;
   DBuf diag_slot;
   bool transfer = false;
-  SellDev P, R;
-  DBuf p_slot, p_self, r_src, ap_rp, ap_col, ap_pp, ap_map, rap_pp, rap_map;
+  SellDev P, Rs;
+  int rs_lg = 0;
+  int64_t nc = 0;  // rows of the next level
+  DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_p, rap_ap;
   int64_t ap_nnz = 0;
 };
 
@@ -170,7 +172,7 @@ struct dpf_topology {
   Hier<double> h64;
   Hier<float> h32;
   // reductions and solve scalars
-  DBuf partial, counter, scal, zero_flag;
+  DBuf partial, scal, zero_flag;
   SolveScalars* scal_host = nullptr;
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
@@ -217,7 +219,7 @@ void ensure_hier(dpf_topology* t, Hier<T>& h) {
     if (!(l == 0 && sizeof(T) == sizeof(double))) n.invd.alloc(nr * sizeof(T));
     if (d.transfer) {
       n.pval.alloc(d.P.entries * sizeof(T));
-      n.rval.alloc(d.R.entries * sizeof(T));
+      n.rval.alloc(d.Rs.entries * sizeof(T));
       n.apval.alloc(d.ap_nnz * sizeof(T));
     }
     n.x.alloc(nr * sizeof(T));
@@ -280,21 +282,22 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
     LevelDev& dn = *t->lev[l + 1];
     SellD<T> A = d.A.view<T>(level_val(t, h, l));
     SellD<T> P = d.P.view<T>(h.lev[l].pval.template as<T>());
-    SellD<T> R = d.R.view<T>(h.lev[l].rval.template as<T>());
+    T* apv = h.lev[l].apval.template as<T>();
+    T* cval = level_val(t, h, l + 1);
     hipLaunchKernelGGL(k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
                        (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(),
                        d.p_self.as<uint16_t>(), P);
-    hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(d.R.entries)), dim3(BLK), 0, s, d.R.entries,
-                       d.r_src.as<int32_t>(), (const T*)P.val, R.val);
-    hipLaunchKernelGGL(k_ap<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A, P,
-                       d.ap_rp.as<int64_t>(), d.ap_pp.as<int64_t>(), d.ap_map.as<uint16_t>(),
-                       d.A.rowlen.as<int32_t>(), d.P.rowlen.as<int32_t>(),
-                       h.lev[l].apval.template as<T>());
-    hipLaunchKernelGGL(k_rap<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, R,
-                       d.R.rowlen.as<int32_t>(), d.ap_rp.as<int64_t>(),
-                       (const T*)h.lev[l].apval.template as<T>(), d.rap_pp.as<int64_t>(),
-                       d.rap_map.as<uint16_t>(), dn.A.view<T>(level_val(t, h, l + 1)),
-                       dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
+    hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries,
+                       d.rs_src.as<int32_t>(), (const T*)P.val,
+                       h.lev[l].rval.template as<T>());
+    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.ap_nnz)), dim3(BLK), 0, s, d.ap_nnz,
+                       d.ap_off.as<int64_t>(), d.ap_a.as<int32_t>(), d.ap_p.as<int32_t>(),
+                       (const T*)A.val, (const T*)P.val, apv);
+    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(dn.A.entries)), dim3(BLK), 0, s,
+                       dn.A.entries, d.rap_off.as<int64_t>(), d.rap_p.as<int32_t>(),
+                       d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
+    hipLaunchKernelGGL(k_rowdiag<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s,
+                       dn.A.view<T>(cval), dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
   }
   if (t->H.coarse_dense) {
     LevelDev& dc = *t->lev[L - 1];
@@ -310,8 +313,12 @@ auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
   return level == 0 ? &k_cheb<T, true> : &k_cheb<T, false>;
 }
 
-RedWS redws(dpf_topology* t) {
-  return RedWS{t->partial.as<double>(), t->counter.as<unsigned int>()};
+// Finishes the grid sum the previous kernel (nparts blocks) left in t->partial.
+void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol2 = 0,
+            double abstol2 = 0, int maxiter = 0) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, (int)nparts,
+                     (const double*)t->partial.as<double>(), t->scal.as<SolveScalars>(), reltol2,
+                     abstol2, maxiter);
 }
 
 // One V-cycle on levels l..: input b at level l (for l = 0 the CG residual),
@@ -326,8 +333,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   SellD<T> A = d.A.view<T>(level_val(t, h, l));
   const T* invd = level_invd(t, h, l);
   T* dd = n.d.template as<T>();
-  RedWS ws = redws(t);
-  SolveScalars* sc = t->scal.as<SolveScalars>();
+  double* part = t->partial.as<double>();
   unsigned g = nblocks(d.A.nrows);
   int m = (int)ch.c1.size();
   T* cur = xl;
@@ -338,9 +344,11 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), nc * sizeof(double), s, nc,
                          (const double*)h.W.template as<double>(), bl, cur, (const int*)done);
       // a one-level hierarchy: the CG scalar r.z still has to be formed
-      if (fuse_dot_at_fine)
+      if (fuse_dot_at_fine) {
         hipLaunchKernelGGL(k_rz_dot<T>, dim3(g), dim3(BLK), 0, s, d.A.nrows, bl, (const T*)cur,
-                           (const int*)done, ws, sc);
+                           (const int*)done, part);
+        finish(t, s, FIN_RZ, g);
+      }
     } else {
       // no dense solve: a longer smoothing sweep (degree 2m) stands in
       Cheb c2;
@@ -348,7 +356,8 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
         hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd,
-                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, ws, sc);
+                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, part);
+        if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
       }
     }
@@ -358,7 +367,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   // pre-smoothing (step 0 done by the producer of b)
   for (int k = 1; k < m; k++) {
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
-                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, ws, sc);
+                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, part);
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
@@ -367,9 +376,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   LevelDev& dn = *t->lev[l + 1];
   LevelNum<T>& nn = h.lev[l + 1];
   bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
-  SellD<T> R = d.R.view<T>(n.rval.template as<T>());
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, R,
-                     (const T*)rl, nn.b.template as<T>(), (const T*)level_invd(t, h, l + 1),
+  SellD<T> Rs = d.Rs.view<T>(n.rval.template as<T>());
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg,
+                     dn.A.nrows, (const T*)rl, nn.b.template as<T>(),
+                     (const T*)level_invd(t, h, l + 1),
                      nn.d.template as<T>(), nn.x.template as<T>(), (T)ch.c0,
                      coarse_next ? 0 : 1, (const int*)done);
   T* xc = nullptr;
@@ -383,7 +393,8 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
-                       (T)c1, (T)c2, (const int*)done, fuse, ws, sc);
+                       (T)c1, (T)c2, (const int*)done, fuse, part);
+    if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
   }
   *xout = cur;
@@ -405,7 +416,7 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   LevelNum<T>& n0 = h.lev[0];
   int64_t nf = t->H.nf;
   unsigned g = nblocks(nf);
-  RedWS ws = redws(t);
+  double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   int* done = &sc->done;
   T* x = h.cx.template as<T>();
@@ -418,7 +429,8 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   T* z0 = n0.x.template as<T>();
   T* zs = n0.xs.template as<T>();
   hipLaunchKernelGGL(k_cg_start<T>, dim3(g), dim3(BLK), 0, s, nf, b, r, x, invd, dd, z0,
-                     (T)ch.c0, reltol * reltol, abstol * abstol, maxiter, ws, sc);
+                     (T)ch.c0, part);
+  finish(t, s, FIN_START, g, reltol * reltol, abstol * abstol, maxiter);
   T* z = nullptr;
   vcycle<T>(t, h, s, 0, r, z0, zs, &z, ch, done, true);
   int par = 0;
@@ -432,10 +444,14 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   while (true) {
     for (int c = 0; c < chunk; c++) {
       hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)z,
-                         (const T*)pb[par], pb[par ^ 1], q, (const int*)done, ws, sc);
+                         (const T*)pb[par], pb[par ^ 1], q, (const int*)done,
+                         (const SolveScalars*)sc, part);
+      finish(t, s, FIN_PQ, g);
       par ^= 1;
       hipLaunchKernelGGL(k_cg_xr<T>, dim3(g), dim3(BLK), 0, s, nf, x, r, (const T*)pb[par],
-                         (const T*)q, invd, dd, z0, (T)ch.c0, (const int*)done, ws, sc);
+                         (const T*)q, invd, dd, z0, (T)ch.c0, (const int*)done,
+                         (const SolveScalars*)sc, part);
+      finish(t, s, FIN_XR, g);
       T* zz = nullptr;
       vcycle<T>(t, h, s, 0, r, z0, zs, &zz, ch, done, true);
       z = zz;
@@ -528,26 +544,26 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
       if (!lh.agg.empty()) {
         d->transfer = true;
         d->P.up(lh.P);
-        d->R.up(lh.R);
+        d->Rs.up(lh.Rs);
+        d->rs_lg = lh.rs_lg;
+        d->nc = lh.R.nrows;
         upload(d->p_slot, lh.p_slot);
         upload(d->p_self, lh.p_self);
-        upload(d->r_src, lh.r_src);
-        upload(d->ap_rp, lh.ap_rp);
-        upload(d->ap_col, lh.ap_col);
-        upload(d->ap_pp, lh.ap_pp);
-        upload(d->ap_map, lh.ap_map);
-        upload(d->rap_pp, lh.rap_pp);
-        upload(d->rap_map, lh.rap_map);
+        upload(d->rs_src, lh.rs_src);
+        upload(d->ap_off, lh.ap_off);
+        upload(d->ap_a, lh.ap_a);
+        upload(d->ap_p, lh.ap_p);
+        upload(d->rap_off, lh.rap_off);
+        upload(d->rap_p, lh.rap_p);
+        upload(d->rap_ap, lh.rap_ap);
         d->ap_nnz = lh.ap_rp.back();
-        maxrows = std::max(maxrows, lh.R.entries());
+        maxrows = std::max(maxrows, lh.Rs.nrows);
       }
       t->lev.push_back(std::move(d));
     }
     maxrows = std::max<int64_t>(maxrows, std::max(H.n, H.ne));
     t->max_grid = nblocks(maxrows) + 8;
     t->partial.alloc(t->max_grid * sizeof(double));
-    t->counter.alloc(sizeof(unsigned int));
-    HIPTRY(hipMemset(t->counter.p, 0, sizeof(unsigned int)));
     t->scal.alloc(sizeof(SolveScalars));
     HIPTRY(hipMemset(t->scal.p, 0, sizeof(SolveScalars)));
     t->zero_flag.alloc(sizeof(int));
@@ -635,7 +651,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int deg = o->cheb_degree > 0 ? o->cheb_degree : t->cheb_degree;
   Cheb ch;
   ch.make(deg, 1.0, t->cheb_ratio);
-  RedWS ws = redws(t);
+  double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   LevelDev& d0 = *t->lev[0];
   SellD<double> A64 = d0.A.view<double>(t->A64.as<double>());
@@ -646,7 +662,8 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   t->r64.alloc(nf * sizeof(double));
   auto residual = [&]() -> double {  // r64 = b - A x64, returns ||r64||
     hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
-                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), ws, sc);
+                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), part);
+    finish(t, s, FIN_OUTER, g);
     HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     return t->scal_host->outer_s;
@@ -670,6 +687,11 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   if (o->precision == DPF_MIXED) {
     ensure_hier(t, t->h32);
     t->b32.alloc(nf * sizeof(float));
+  }
+  if (!std::isfinite(bnorm) || !std::isfinite(rn)) {
+    if (st) std::memset(st, 0, sizeof(*st));
+    throw Err{DPF_EINVAL, "non-finite values in the system (NaN or Inf in K, Q, dirichleths or "
+                          "the initial guess)"};
   }
   while (rn > target) {
     if (outer >= max_outer || budget <= 0) {
@@ -704,6 +726,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     }
     rn = rn_new;
   }
+  if (!std::isfinite(rn)) status = DPF_EBREAKDOWN;
   double relres = bnorm > 0 ? rn / bnorm : rn;
   copy_dev(xout, t->x64.p, nf * sizeof(double), s);
   if (st) {
@@ -1041,7 +1064,7 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
   int64_t nf = t->H.nf, nnz = t->H.levels[0].A.nnz;
   unsigned g = nblocks(nf);
   const int* nodone = t->zero_flag.as<int>();
-  RedWS ws = redws(t);
+  double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   SellD<T> A = d0.A.view<T>(level_val(t, h, 0));
   double w = (double)sizeof(T);
@@ -1056,12 +1079,11 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
     if (kernel == 0) {
       hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa, xb,
                          (const T*)h.cr.template as<T>(), n0.d.template as<T>(),
-                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0, ws,
-                         sc);
+                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0, part);
     } else if (kernel == 1) {
       hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
-                         h.cq.template as<T>(), nodone, ws, sc);
+                         h.cq.template as<T>(), nodone, (const SolveScalars*)sc, part);
     } else {
       hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
                          d0.A.view<double>(t->A64.as<double>()), (const double*)t->x64.as<double>(),

@@ -50,12 +50,21 @@ struct LevelHost {
   std::vector<uint16_t> p_self;     // per row: slot of agg(row) in P's row
   SellHost R;                       // restriction = P', ncoarse x nrows
   std::vector<int32_t> r_src;       // per R entry: index of the P entry, -1 = padding
-  std::vector<int64_t> ap_rp;       // CSR of A*P
+  // R with each row split over G lanes (sub-row I*G+g holds the entries
+  // t = g, g+G, ... of row I), for the V-cycle's restriction
+  int rs_lg = 0;                    // log2 G
+  SellHost Rs;
+  std::vector<int32_t> rs_src;      // per Rs entry: index of the P entry, -1 = padding
+  // A*P (CSR) and its products grouped by destination nonzero:
+  //   AP[d] = sum_{q in [ap_off[d], ap_off[d+1])} A.val[ap_a[q]] * P.val[ap_p[q]]
+  std::vector<int64_t> ap_rp;
   std::vector<int32_t> ap_col;
-  std::vector<int64_t> ap_pp;       // product offsets of A*P per row
-  std::vector<uint16_t> ap_map;     // per product: slot in the A*P row
-  std::vector<int64_t> rap_pp;      // product offsets of R*(A*P) per coarse row
-  std::vector<uint16_t> rap_map;    // per product: slot in the next level's A row
+  std::vector<int64_t> ap_off;
+  std::vector<int32_t> ap_a, ap_p;
+  // next level's A, grouped by destination SELL entry (padding: empty):
+  //   Ac[e] = sum_{q in [rap_off[e], rap_off[e+1])} P.val[rap_p[q]] * AP[rap_ap[q]]
+  std::vector<int64_t> rap_off;
+  std::vector<int32_t> rap_p, rap_ap;
 };
 
 struct TopologyHost {

@@ -269,19 +269,40 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       for (int t = 0; t < L.R.rowlen[I]; t++) L.r_src[L.R.idx(I, t)] = (int32_t)src[rp[I] + t];
   }
   const SellHost& R = L.R;
-  // A*P pattern and product map
+  // R split over G lanes per row
+  {
+    double avg = nc > 0 ? (double)R.nnz / (double)nc : 1.0;
+    int lg = 0;
+    while ((1 << lg) * 4 < avg && lg < 6) lg++;
+    L.rs_lg = lg;
+    int G = 1 << lg;
+    int64_t ns = nc * G;
+    std::vector<int64_t> rp(ns + 1, 0);
+    std::vector<int32_t> ci;
+    std::vector<int32_t> src;
+    ci.reserve(R.nnz);
+    src.reserve(R.nnz);
+    for (int64_t I = 0; I < nc; I++)
+      for (int g = 0; g < G; g++) {
+        for (int t = g; t < R.rowlen[I]; t += G) {
+          ci.push_back(R.at(I, t));
+          src.push_back(L.r_src[R.idx(I, t)]);
+        }
+        rp[I * G + g + 1] = (int64_t)ci.size();
+      }
+    L.Rs.build(ns, n, rp, ci, false);
+    L.rs_src.assign(L.Rs.entries(), -1);
+    for (int64_t r = 0; r < ns; r++)
+      for (int k = 0; k < L.Rs.rowlen[r]; k++) L.rs_src[L.Rs.idx(r, k)] = src[rp[r] + k];
+  }
+  // A*P pattern; products grouped by destination
   {
     RowSet rs(nc);
     L.ap_rp.assign(n + 1, 0);
     L.ap_col.clear();
-    L.ap_pp.assign(n + 1, 0);
-    int64_t nprod = 0;
-    for (int64_t i = 0; i < n; i++) {
-      for (int s = 0; s < A.rowlen[i]; s++) nprod += P.rowlen[A.at(i, s)];
-      L.ap_pp[i + 1] = nprod;
-    }
-    L.ap_map.resize(nprod);
-    int64_t q = 0;
+    std::vector<int32_t> cnt;
+    std::vector<int64_t> pos;
+    // pass 1: pattern
     for (int64_t i = 0; i < n; i++) {
       rs.begin();
       for (int s = 0; s < A.rowlen[i]; s++) {
@@ -292,32 +313,43 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       if (rs.cols.size() >= NO_SLOT) return "A*P row too long";
       L.ap_col.insert(L.ap_col.end(), rs.cols.begin(), rs.cols.end());
       L.ap_rp[i + 1] = (int64_t)L.ap_col.size();
+    }
+    int64_t nap = L.ap_rp[n];
+    if (nap >= INT32_MAX) return "A*P too large for 32-bit indices";
+    // pass 2: count products per destination, then fill in enumeration order
+    L.ap_off.assign(nap + 1, 0);
+    std::vector<int32_t> slot_of(nc, 0);
+    for (int64_t i = 0; i < n; i++) {
+      for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
       for (int s = 0; s < A.rowlen[i]; s++) {
         int32_t k = A.at(i, s);
-        for (int m = 0; m < P.rowlen[k]; m++) L.ap_map[q++] = (uint16_t)rs.slot[P.at(k, m)];
+        for (int m = 0; m < P.rowlen[k]; m++) L.ap_off[L.ap_rp[i] + slot_of[P.at(k, m)] + 1]++;
+      }
+    }
+    for (int64_t d = 0; d < nap; d++) L.ap_off[d + 1] += L.ap_off[d];
+    int64_t nprod = L.ap_off[nap];
+    if (nprod >= INT32_MAX) return "too many A*P products";
+    L.ap_a.resize(nprod);
+    L.ap_p.resize(nprod);
+    std::vector<int64_t> fill(L.ap_off.begin(), L.ap_off.end() - 1);
+    for (int64_t i = 0; i < n; i++) {
+      for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
+      for (int s = 0; s < A.rowlen[i]; s++) {
+        int32_t k = A.at(i, s);
+        int64_t ea = A.idx(i, s);
+        for (int m = 0; m < P.rowlen[k]; m++) {
+          int64_t q = fill[L.ap_rp[i] + slot_of[P.at(k, m)]]++;
+          L.ap_a[q] = (int32_t)ea;
+          L.ap_p[q] = (int32_t)P.idx(k, m);
+        }
       }
     }
   }
-  // R*(A*P) pattern (the next level's A) and product map
+  // R*(A*P): the next level's pattern, then products grouped by its SELL entries
   {
     RowSet rs(nc);
     std::vector<int64_t> rp(nc + 1, 0);
     std::vector<int32_t> ci;
-    L.rap_pp.assign(nc + 1, 0);
-    int64_t nprod = 0;
-    for (int64_t I = 0; I < nc; I++) {
-      for (int t = 0; t < R.rowlen[I]; t++) {
-        int32_t i = R.at(I, t);
-        nprod += L.ap_rp[i + 1] - L.ap_rp[i];
-      }
-      L.rap_pp[I + 1] = nprod;
-    }
-    L.rap_map.resize(nprod);
-    std::vector<std::vector<int32_t>> rowcols(1);
-    int64_t q = 0;
-    std::vector<int32_t> slots_scratch;
-    std::vector<int64_t> rows_first(nc + 1, 0);
-    // first pass: patterns
     for (int64_t I = 0; I < nc; I++) {
       rs.begin();
       rs.add((int32_t)I, I);
@@ -329,11 +361,6 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       if (rs.cols.size() >= NO_SLOT) return "coarse operator row too long";
       ci.insert(ci.end(), rs.cols.begin(), rs.cols.end());
       rp[I + 1] = (int64_t)ci.size();
-      for (int t = 0; t < R.rowlen[I]; t++) {
-        int32_t i = R.at(I, t);
-        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
-          L.rap_map[q++] = (uint16_t)rs.slot[L.ap_col[a]];
-      }
     }
     next = LevelHost();
     next.A.build(nc, nc, rp, ci, true);
@@ -342,6 +369,33 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       auto b = ci.begin() + rp[I], e = ci.begin() + rp[I + 1];
       next.diag_slot[I] = (uint16_t)(std::lower_bound(b, e, (int32_t)I) - b);
     }
+    const SellHost& C = next.A;
+    int64_t nent = C.entries();
+    L.rap_off.assign(nent + 1, 0);
+    std::vector<int32_t> slot_of(nc, 0);
+    auto each = [&](auto&& fn) {
+      for (int64_t I = 0; I < nc; I++) {
+        for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]);
+        for (int t = 0; t < R.rowlen[I]; t++) {
+          int32_t i = R.at(I, t);
+          int32_t pe = L.r_src[R.idx(I, t)];
+          for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
+            fn(C.idx(I, slot_of[L.ap_col[a]]), pe, a);
+        }
+      }
+    };
+    each([&](int64_t e, int32_t, int64_t) { L.rap_off[e + 1]++; });
+    for (int64_t e = 0; e < nent; e++) L.rap_off[e + 1] += L.rap_off[e];
+    int64_t nprod = L.rap_off[nent];
+    if (nprod >= INT32_MAX) return "too many R*A*P products";
+    L.rap_p.resize(nprod);
+    L.rap_ap.resize(nprod);
+    std::vector<int64_t> fill(L.rap_off.begin(), L.rap_off.end() - 1);
+    each([&](int64_t e, int32_t pe, int64_t a) {
+      int64_t q = fill[e]++;
+      L.rap_p[q] = pe;
+      L.rap_ap[q] = (int32_t)a;
+    });
   }
   return "";
 }

@@ -50,74 +50,74 @@ void dpf_inspect_destroy(dpf_inspect* h) { delete h; }
 
 int32_t dpf_inspect_levels(const dpf_inspect* h) { return (int32_t)h->H.levels.size(); }
 
-// Sizes (element counts) of the arrays dpf_inspect_get can return.
-int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) {
-  const TopologyHost& T = h->H;
-  std::string s(name);
-  if (s == "n2f") return (int64_t)T.n2f.size();
-  if (s == "f2n") return (int64_t)T.f2n.size();
-  if (s == "inc_ptr") return (int64_t)T.inc_ptr.size();
-  if (s == "inc_edge") return (int64_t)T.inc_edge.size();
-  if (s == "inc_other") return (int64_t)T.inc_other.size();
-  if (s == "inc_slot") return (int64_t)T.inc_slot.size();
+}  // extern "C"
+
+namespace {
+
+// name -> (size, copy) over the topology / level arrays
+template <typename F>
+int with_array(const TopologyHost& T, int32_t level, const std::string& s, F&& f) {
+  if (s == "n2f") return f(T.n2f);
+  if (s == "f2n") return f(T.f2n);
+  if (s == "inc_ptr") return f(T.inc_ptr);
+  if (s == "inc_edge") return f(T.inc_edge);
+  if (s == "inc_other") return f(T.inc_other);
+  if (s == "inc_slot") return f(T.inc_slot);
   if (level < 0 || level >= (int32_t)T.levels.size()) return -1;
   const LevelHost& L = T.levels[level];
-  if (s == "A.sptr") return (int64_t)L.A.sptr.size();
-  if (s == "A.col") return (int64_t)L.A.col.size();
-  if (s == "A.rowlen") return (int64_t)L.A.rowlen.size();
-  if (s == "diag_slot") return (int64_t)L.diag_slot.size();
-  if (s == "agg") return (int64_t)L.agg.size();
-  if (s == "P.sptr") return (int64_t)L.P.sptr.size();
-  if (s == "P.col") return (int64_t)L.P.col.size();
-  if (s == "P.rowlen") return (int64_t)L.P.rowlen.size();
-  if (s == "p_slot") return (int64_t)L.p_slot.size();
-  if (s == "p_self") return (int64_t)L.p_self.size();
-  if (s == "R.sptr") return (int64_t)L.R.sptr.size();
-  if (s == "R.col") return (int64_t)L.R.col.size();
-  if (s == "R.rowlen") return (int64_t)L.R.rowlen.size();
-  if (s == "r_src") return (int64_t)L.r_src.size();
-  if (s == "ap_rp") return (int64_t)L.ap_rp.size();
-  if (s == "ap_col") return (int64_t)L.ap_col.size();
-  if (s == "ap_pp") return (int64_t)L.ap_pp.size();
-  if (s == "ap_map") return (int64_t)L.ap_map.size();
-  if (s == "rap_pp") return (int64_t)L.rap_pp.size();
-  if (s == "rap_map") return (int64_t)L.rap_map.size();
+  if (s == "A.sptr") return f(L.A.sptr);
+  if (s == "A.col") return f(L.A.col);
+  if (s == "A.rowlen") return f(L.A.rowlen);
+  if (s == "diag_slot") return f(L.diag_slot);
+  if (s == "agg") return f(L.agg);
+  if (s == "P.sptr") return f(L.P.sptr);
+  if (s == "P.col") return f(L.P.col);
+  if (s == "P.rowlen") return f(L.P.rowlen);
+  if (s == "p_slot") return f(L.p_slot);
+  if (s == "p_self") return f(L.p_self);
+  if (s == "R.sptr") return f(L.R.sptr);
+  if (s == "R.col") return f(L.R.col);
+  if (s == "R.rowlen") return f(L.R.rowlen);
+  if (s == "r_src") return f(L.r_src);
+  if (s == "Rs.sptr") return f(L.Rs.sptr);
+  if (s == "Rs.col") return f(L.Rs.col);
+  if (s == "Rs.rowlen") return f(L.Rs.rowlen);
+  if (s == "rs_src") return f(L.rs_src);
+  if (s == "ap_rp") return f(L.ap_rp);
+  if (s == "ap_col") return f(L.ap_col);
+  if (s == "ap_off") return f(L.ap_off);
+  if (s == "ap_a") return f(L.ap_a);
+  if (s == "ap_p") return f(L.ap_p);
+  if (s == "rap_off") return f(L.rap_off);
+  if (s == "rap_p") return f(L.rap_p);
+  if (s == "rap_ap") return f(L.rap_ap);
   return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Sizes (element counts) of the arrays dpf_inspect_get can return; the
+// pseudo-name "rs_lg" returns log2 of the restriction's lanes per row.
+int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) {
+  std::string s(name);
+  if (s == "rs_lg") {
+    if (level < 0 || level >= (int32_t)h->H.levels.size()) return -1;
+    return h->H.levels[level].rs_lg;
+  }
+  int64_t out = -1;
+  with_array(h->H, level, s, [&](const auto& v) {
+    out = (int64_t)v.size();
+    return 0;
+  });
+  return out;
 }
 
 int dpf_inspect_get(const dpf_inspect* h, int32_t level, const char* name, void* out,
                     int64_t cap) {
-  const TopologyHost& T = h->H;
-  std::string s(name);
-  if (s == "n2f") return put(T.n2f, out, cap);
-  if (s == "f2n") return put(T.f2n, out, cap);
-  if (s == "inc_ptr") return put(T.inc_ptr, out, cap);
-  if (s == "inc_edge") return put(T.inc_edge, out, cap);
-  if (s == "inc_other") return put(T.inc_other, out, cap);
-  if (s == "inc_slot") return put(T.inc_slot, out, cap);
-  if (level < 0 || level >= (int32_t)T.levels.size()) return -1;
-  const LevelHost& L = T.levels[level];
-  if (s == "A.sptr") return put(L.A.sptr, out, cap);
-  if (s == "A.col") return put(L.A.col, out, cap);
-  if (s == "A.rowlen") return put(L.A.rowlen, out, cap);
-  if (s == "diag_slot") return put(L.diag_slot, out, cap);
-  if (s == "agg") return put(L.agg, out, cap);
-  if (s == "P.sptr") return put(L.P.sptr, out, cap);
-  if (s == "P.col") return put(L.P.col, out, cap);
-  if (s == "P.rowlen") return put(L.P.rowlen, out, cap);
-  if (s == "p_slot") return put(L.p_slot, out, cap);
-  if (s == "p_self") return put(L.p_self, out, cap);
-  if (s == "R.sptr") return put(L.R.sptr, out, cap);
-  if (s == "R.col") return put(L.R.col, out, cap);
-  if (s == "R.rowlen") return put(L.R.rowlen, out, cap);
-  if (s == "r_src") return put(L.r_src, out, cap);
-  if (s == "ap_rp") return put(L.ap_rp, out, cap);
-  if (s == "ap_col") return put(L.ap_col, out, cap);
-  if (s == "ap_pp") return put(L.ap_pp, out, cap);
-  if (s == "ap_map") return put(L.ap_map, out, cap);
-  if (s == "rap_pp") return put(L.rap_pp, out, cap);
-  if (s == "rap_map") return put(L.rap_map, out, cap);
-  return -1;
+  return with_array(h->H, level, std::string(name),
+                    [&](const auto& v) { return put(v, out, cap); });
 }
 
 }  // extern "C"

@@ -9,9 +9,9 @@
 //    gathers of a band then stay inside that XCD's L2.
 //  * Nothing scatters: assembly, transfer operators and gradients are row
 //    or cell gathers through index maps built on the host, so results are
-//    bitwise reproducible.  Reductions are per-block shuffles plus a
-//    last-block-done pass over the block partials in fixed order (integer
-//    ticket atomics only, no floating-point atomics).
+//    bitwise reproducible.  Reductions are per-block shuffles writing one
+//    partial per block, summed in fixed order by a one-workgroup finish
+//    kernel (no atomics of any kind).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -35,10 +35,6 @@ struct SolveScalars {
   int first;
 };
 
-struct RedWS {
-  double* partial;
-  unsigned int* counter;
-};
 
 template <typename T>
 struct SellD {
@@ -75,33 +71,81 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
-// Grid-wide deterministic sum.  Returns true (in every thread of the block)
-// only in the last block to finish, which holds the total in *total.
-__device__ __forceinline__ bool grid_sum(double v, RedWS ws, double* total) {
+// Block partial of a grid-wide sum: thread 0 of block b writes partial[b].
+// The sum over blocks is finished by k_finish in the next launch on the
+// stream; the kernel boundary makes the partials visible without any
+// device-scope fence or atomic (on a multi-XCD part a release fence per block
+// writes back the XCD's L2 and costs far more than the kernel's own traffic).
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial) {
   __shared__ double sh[4];
-  __shared__ int last;
-  __shared__ double tot;
   double bs = block_sum(v, sh);
-  if (threadIdx.x == 0) {
-    ws.partial[blockIdx.x] = bs;
-    __threadfence();
-    unsigned t = atomicAdd(ws.counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER = 4 };
+
+// Sums the block partials of the previous kernel in a fixed order (one
+// workgroup) and applies the scalar step of the Krylov solve:
+//   FIN_START  rr, stop threshold, counters;   FIN_PQ  alpha = rz / p.q
+//   FIN_XR     rr, iteration count, convergence / cap / breakdown
+//   FIN_RZ     rz_old = rz, rz = r.z;           FIN_OUTER  FP64 residual norm
+__global__ void __launch_bounds__(1024) k_finish(int mode, int nparts,
+                                                 const double* __restrict__ partial,
+                                                 SolveScalars* sc, double reltol2,
+                                                 double abstol2, int maxiter) {
+  if (mode != FIN_START && mode != FIN_OUTER && sc->done) return;
+  __shared__ double sh[16];
   double s = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BLK)
-    s += __hip_atomic_load(ws.partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) {
-    tot = s;
-    atomicExch(ws.counter, 0u);
-  }
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  *total = tot;
-  return true;
+  if (threadIdx.x != 0) return;
+  int nw = (int)(blockDim.x >> 6);
+  double tot = 0.0;
+  for (int k = 0; k < nw; k++) tot += sh[k];
+  switch (mode) {
+    case FIN_START: {
+      sc->rr = tot;
+      double stop = reltol2 * tot;
+      sc->rr_stop = stop > abstol2 ? stop : abstol2;
+      sc->iters = 0;
+      sc->maxiter = maxiter;
+      sc->first = 1;
+      sc->rz = 0.0;
+      sc->rz_old = 0.0;
+      sc->done = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
+      break;
+    }
+    case FIN_PQ:
+      sc->pq = tot;
+      sc->first = 0;
+      if (!(tot > 0.0))
+        sc->done = 3;
+      else
+        sc->alpha = sc->rz / tot;
+      break;
+    case FIN_XR: {
+      sc->rr = tot;
+      int it = sc->iters + 1;
+      sc->iters = it;
+      if (tot <= sc->rr_stop)
+        sc->done = 1;
+      else if (!(tot == tot))
+        sc->done = 3;
+      else if (it >= sc->maxiter)
+        sc->done = 2;
+      break;
+    }
+    case FIN_RZ:
+      sc->rz_old = sc->rz;
+      sc->rz = tot;
+      if (!(tot > 0.0)) sc->done = 3;
+      break;
+    default:
+      sc->outer_rr = tot;
+      sc->outer_s = sqrt(tot);
+  }
 }
 
 template <typename T>
@@ -252,61 +296,37 @@ __global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
   rval[e] = s >= 0 ? pval[s] : (T)0;
 }
 
-// (A P)(i,:) = sum_k a_ik P(k,:), products in the host's enumeration order.
+// Galerkin products grouped by destination: out[d] = sum over the
+// destination's contribution list of X[s1[q]] * Y[s2[q]], in the host's fixed
+// enumeration order.  Used for A*P (X = A, Y = P) and for the next level's
+// operator R*(A*P) (X = P, since R(I,i) = P(i,I); Y = A*P).  Destinations with
+// an empty list (SELL padding) become 0.
 template <typename T>
-__global__ void k_ap(SellD<T> A, SellD<T> P, const int64_t* __restrict__ ap_rp,
-                     const int64_t* __restrict__ ap_pp, const uint16_t* __restrict__ ap_map,
-                     const int32_t* __restrict__ rowlenA, const int32_t* __restrict__ rowlenP,
-                     T* __restrict__ apval) {
-  int64_t i = logical_row();
-  if (i >= A.nrows) return;
-  int64_t r0 = ap_rp[i], r1 = ap_rp[i + 1];
-  for (int64_t q = r0; q < r1; q++) apval[q] = (T)0;
-  int64_t ab;
-  int aw;
-  sell_row(A, i, ab, aw);
-  int la = rowlenA[i];
-  int64_t q = ap_pp[i];
-  for (int k = 0; k < la; k++) {
-    int64_t e = ab + (int64_t)k * 64;
-    T a = A.val[e];
-    int32_t c = A.col[e];
-    int64_t pb;
-    int pw;
-    sell_row(P, c, pb, pw);
-    int lp = rowlenP[c];
-    for (int m = 0; m < lp; m++) apval[r0 + ap_map[q++]] += a * P.val[pb + (int64_t)m * 64];
-  }
+__global__ void k_prod_gather(int64_t ndest, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ s1, const int32_t* __restrict__ s2,
+                              const T* __restrict__ X, const T* __restrict__ Y,
+                              T* __restrict__ out) {
+  int64_t d = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (d >= ndest) return;
+  T acc = (T)0;
+  for (int64_t q = off[d]; q < off[d + 1]; q++) acc += X[s1[q]] * Y[s2[q]];
+  out[d] = acc;
 }
 
-// A_c(I,:) = sum_i R(I,i) (A P)(i,:); also the next level's l1 diagonal.
+// invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator
 template <typename T>
-__global__ void k_rap(SellD<T> R, const int32_t* __restrict__ rowlenR,
-                      const int64_t* __restrict__ ap_rp, const T* __restrict__ apval,
-                      const int64_t* __restrict__ rap_pp, const uint16_t* __restrict__ rap_map,
-                      SellD<T> Ac, const uint16_t* __restrict__ diag_slot_c,
-                      T* __restrict__ invd_c) {
-  int64_t I = logical_row();
-  if (I >= Ac.nrows) return;
-  int64_t cb, rb;
-  int cw, rw;
-  sell_row(Ac, I, cb, cw);
-  sell_row(R, I, rb, rw);
-  for (int k = 0; k < cw; k++) Ac.val[cb + (int64_t)k * 64] = (T)0;
-  int lr = rowlenR[I];
-  int64_t q = rap_pp[I];
-  for (int t = 0; t < lr; t++) {
-    int64_t e = rb + (int64_t)t * 64;
-    T rv = R.val[e];
-    int32_t i = R.col[e];
-    for (int64_t a = ap_rp[i]; a < ap_rp[i + 1]; a++)
-      Ac.val[cb + (int64_t)rap_map[q++] * 64] += rv * apval[a];
-  }
-  int ds = diag_slot_c[I];
-  T l1 = Ac.val[cb + (int64_t)ds * 64];
-  for (int k = 0; k < cw; k++)
-    if (k != ds) l1 += fabs(Ac.val[cb + (int64_t)k * 64]);
-  invd_c[I] = (T)1 / l1;
+__global__ void k_rowdiag(SellD<T> A, const uint16_t* __restrict__ diag_slot,
+                          T* __restrict__ invd) {
+  int64_t i = logical_row();
+  if (i >= A.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(A, i, base, w);
+  int ds = diag_slot[i];
+  T l1 = A.val[base + (int64_t)ds * 64];
+  for (int k = 0; k < w; k++)
+    if (k != ds) l1 += fabs(A.val[base + (int64_t)k * 64]);
+  invd[i] = (T)1 / l1;
 }
 
 // In-place Gauss-Jordan inverse of the (SPD) coarsest operator, in FP64, by
@@ -364,8 +384,8 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 template <typename T, bool FINE>
 __global__ void k_cheb(SellD<T> A, const T* __restrict__ xin, T* __restrict__ xout,
                        const T* __restrict__ b, T* __restrict__ d, const T* __restrict__ invd,
-                       T c1, T c2, const int* __restrict__ done, int fuse_dot, RedWS ws,
-                       SolveScalars* sc) {
+                       T c1, T c2, const int* __restrict__ done, int fuse_dot,
+                       double* __restrict__ partial) {
   if (*done) return;
   int64_t r = logical_row();
   double local = 0.0;
@@ -385,14 +405,7 @@ __global__ void k_cheb(SellD<T> A, const T* __restrict__ xin, T* __restrict__ xo
     xout[r] = xo;
     local = (double)br * (double)xo;
   }
-  if (fuse_dot) {
-    double tot;
-    if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-      sc->rz_old = sc->rz;
-      sc->rz = tot;
-      if (!(tot > 0.0)) sc->done = 3;
-    }
-  }
+  if (fuse_dot) block_partial(local, partial);
 }
 
 // r = b - A x
@@ -413,28 +426,36 @@ __global__ void k_residual(SellD<T> A, const T* __restrict__ x, const T* __restr
   r[i] = b[i] - ax;
 }
 
-// bc = R rf; if init, also the zero-guess first Chebyshev step on the coarse
-// level: dc = xc = c0 invdc bc.
+// bc = R rf with every row of R split over G = 2^lg consecutive lanes
+// (sub-row I*G + g); the lane partials are combined by wave shuffles in a
+// fixed pattern.  If init, also the zero-guess first Chebyshev step on the
+// coarse level: dc = xc = c0 invdc bc.
 template <typename T>
-__global__ void k_restrict(SellD<T> R, const T* __restrict__ rf, T* __restrict__ bc,
-                           const T* __restrict__ invdc, T* __restrict__ dc, T* __restrict__ xc,
-                           T c0, int init, const int* __restrict__ done) {
+__global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict__ rf,
+                           T* __restrict__ bc, const T* __restrict__ invdc,
+                           T* __restrict__ dc, T* __restrict__ xc, T c0, int init,
+                           const int* __restrict__ done) {
   if (*done) return;
-  int64_t I = logical_row();
-  if (I >= R.nrows) return;
-  int64_t base;
-  int w;
-  sell_row(R, I, base, w);
+  int64_t sr = logical_row();
   T s = (T)0;
-  for (int k = 0; k < w; k++) {
-    int64_t e = base + (int64_t)k * 64;
-    s += R.val[e] * rf[R.col[e]];
+  if (sr < Rs.nrows) {
+    int64_t base;
+    int w;
+    sell_row(Rs, sr, base, w);
+    for (int k = 0; k < w; k++) {
+      int64_t e = base + (int64_t)k * 64;
+      s += Rs.val[e] * rf[Rs.col[e]];
+    }
   }
-  bc[I] = s;
-  if (init) {
-    T dd = c0 * invdc[I] * s;
-    dc[I] = dd;
-    xc[I] = dd;
+  for (int o = (1 << lg) >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  int64_t I = sr >> lg;
+  if ((sr & ((1 << lg) - 1)) == 0 && I < nc) {
+    bc[I] = s;
+    if (init) {
+      T dd = c0 * invdc[I] * s;
+      dc[I] = dd;
+      xc[I] = dd;
+    }
   }
 }
 
@@ -476,16 +497,11 @@ __global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __
 // smoothing sweep to fuse it into).
 template <typename T>
 __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict__ z,
-                         const int* __restrict__ done, RedWS ws, SolveScalars* sc) {
+                         const int* __restrict__ done, double* __restrict__ partial) {
   if (*done) return;
   int64_t i = logical_row();
   double local = i < n ? (double)b[i] * (double)z[i] : 0.0;
-  double tot;
-  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-    sc->rz_old = sc->rz;
-    sc->rz = tot;
-    if (!(tot > 0.0)) sc->done = 3;
-  }
+  block_partial(local, partial);
 }
 
 // ------------------------------------------------------------------ CG
@@ -495,8 +511,7 @@ __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict
 template <typename T>
 __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r,
                            T* __restrict__ x, const T* __restrict__ invd, T* __restrict__ d0,
-                           T* __restrict__ z0, T c0, double reltol2, double abstol2,
-                           int maxiter, RedWS ws, SolveScalars* sc) {
+                           T* __restrict__ z0, T c0, double* __restrict__ partial) {
   int64_t i = logical_row();
   double local = 0.0;
   if (i < n) {
@@ -508,18 +523,7 @@ __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r
     z0[i] = dd;
     local = (double)bv * (double)bv;
   }
-  double tot;
-  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-    sc->rr = tot;
-    double stop = reltol2 * tot;
-    sc->rr_stop = stop > abstol2 ? stop : abstol2;
-    sc->iters = 0;
-    sc->maxiter = maxiter;
-    sc->first = 1;
-    sc->rz = 0.0;
-    sc->rz_old = 0.0;
-    sc->done = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
-  }
+  block_partial(local, partial);
 }
 
 // p_new = z + beta p_old,  q = A p_new (formed from z and p_old at the
@@ -527,7 +531,7 @@ __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r
 template <typename T>
 __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict__ pold,
                         T* __restrict__ pnew, T* __restrict__ q, const int* __restrict__ done,
-                        RedWS ws, SolveScalars* sc) {
+                        const SolveScalars* __restrict__ sc, double* __restrict__ partial) {
   if (*done) return;
   int first = sc->first;
   T beta = first ? (T)0 : (T)(sc->rz / sc->rz_old);
@@ -556,15 +560,7 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
     q[i] = s;
     local = (double)pn * (double)s;
   }
-  double tot;
-  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-    sc->pq = tot;
-    sc->first = 0;
-    if (!(tot > 0.0))
-      sc->done = 3;
-    else
-      sc->alpha = sc->rz / tot;
-  }
+  block_partial(local, partial);
 }
 
 // x += alpha p, r -= alpha q, rr = r.r, convergence test; first smoothing
@@ -573,7 +569,8 @@ template <typename T>
 __global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
                         const T* __restrict__ p, const T* __restrict__ q,
                         const T* __restrict__ invd, T* __restrict__ d0, T* __restrict__ z0, T c0,
-                        const int* __restrict__ done, RedWS ws, SolveScalars* sc) {
+                        const int* __restrict__ done, const SolveScalars* __restrict__ sc,
+                        double* __restrict__ partial) {
   if (*done) return;
   T alpha = (T)sc->alpha;
   int64_t i = logical_row();
@@ -587,18 +584,7 @@ __global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
     z0[i] = dd;
     local = (double)rv * (double)rv;
   }
-  double tot;
-  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-    sc->rr = tot;
-    int it = sc->iters + 1;
-    sc->iters = it;
-    if (tot <= sc->rr_stop)
-      sc->done = 1;
-    else if (!(tot == tot))
-      sc->done = 3;
-    else if (it >= sc->maxiter)
-      sc->done = 2;
-  }
+  block_partial(local, partial);
 }
 
 // ------------------------------------------------------------------ FP64 outer residual / refinement
@@ -606,7 +592,7 @@ __global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
 // r = b - A x in FP64 with ||r||^2 -> sc->outer_rr, sqrt -> sc->outer_s.
 __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
                                  const double* __restrict__ b, double* __restrict__ r,
-                                 RedWS ws, SolveScalars* sc) {
+                                 double* __restrict__ partial) {
   int64_t i = logical_row();
   double local = 0.0;
   if (i < A.nrows) {
@@ -622,11 +608,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
     r[i] = rv;
     local = rv * rv;
   }
-  double tot;
-  if (grid_sum(local, ws, &tot) && threadIdx.x == 0) {
-    sc->outer_rr = tot;
-    sc->outer_s = sqrt(tot);
-  }
+  block_partial(local, partial);
 }
 
 // inner right-hand side in FP32: b32 = r64 / s
