@@ -100,8 +100,9 @@ struct LevelDev {
   SellDev A;
   DBuf diag_slot;
   bool transfer = false;
-  SellDev P, Rs;
-  int rs_lg = 0;
+  SellDev P, Rs, As;
+  int rs_lg = 0, as_lg = 0;
+  DBuf as_src;
   int64_t nc = 0;  // rows of the next level
   DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_p, rap_ap;
   int64_t ap_nnz = 0;
@@ -110,7 +111,7 @@ struct LevelDev {
 // Numeric values and work vectors of a hierarchy in precision T.
 template <typename T>
 struct LevelNum {
-  DBuf val, invd, pval, rval, apval;
+  DBuf val, invd, pval, rval, apval, asval;
   DBuf x, xs, b, r, d;
 };
 
@@ -222,6 +223,7 @@ void ensure_hier(dpf_topology* t, Hier<T>& h) {
       n.rval.alloc(d.Rs.entries * sizeof(T));
       n.apval.alloc(d.ap_nnz * sizeof(T));
     }
+    if (d.as_lg > 0) n.asval.alloc(d.As.entries * sizeof(T));
     n.x.alloc(nr * sizeof(T));
     n.xs.alloc(nr * sizeof(T));
     n.d.alloc(nr * sizeof(T));
@@ -298,6 +300,10 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
                        d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
     hipLaunchKernelGGL(k_rowdiag<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s,
                        dn.A.view<T>(cval), dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
+    if (dn.as_lg > 0)
+      hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(dn.As.entries)), dim3(BLK), 0, s,
+                         dn.As.entries, dn.as_src.as<int32_t>(), (const T*)cval,
+                         h.lev[l + 1].asval.template as<T>());
   }
   if (t->H.coarse_dense) {
     LevelDev& dc = *t->lev[L - 1];
@@ -336,6 +342,11 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   double* part = t->partial.as<double>();
   unsigned g = nblocks(d.A.nrows);
   int m = (int)ch.c1.size();
+  // the smoothing sweeps and the residual use the row-split copy on coarse levels
+  int lg = d.as_lg;
+  SellD<T> Asm = lg > 0 ? d.As.view<T>(n.asval.template as<T>()) : A;
+  unsigned gs = nblocks(Asm.nrows);
+  int64_t nr = d.A.nrows;
   T* cur = xl;
   T* oth = xsl;
   if (l == L - 1) {
@@ -355,7 +366,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       c2.make(2 * m, 1.0, t->cheb_ratio);
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
-        hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd,
+        hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd,
                            invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, part);
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
@@ -366,12 +377,12 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   }
   // pre-smoothing (step 0 done by the producer of b)
   for (int k = 1; k < m; k++) {
-    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
+    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
                        (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, part);
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
-  hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)cur, bl, rl,
+  hipLaunchKernelGGL(k_residual<T>, dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, rl,
                      (const int*)done);
   LevelDev& dn = *t->lev[l + 1];
   LevelNum<T>& nn = h.lev[l + 1];
@@ -392,7 +403,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     double c1 = k == 0 ? 0.0 : ch.c1[k];
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
-    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(g), dim3(BLK), 0, s, A, (const T*)cur, oth, bl, dd, invd,
+    hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
                        (T)c1, (T)c2, (const int*)done, fuse, part);
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
@@ -540,6 +551,12 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
       LevelHost& lh = H.levels[l];
       d->A.up(lh.A);
       upload(d->diag_slot, lh.diag_slot);
+      d->as_lg = lh.as_lg;
+      if (lh.as_lg > 0) {
+        d->As.up(lh.As);
+        upload(d->as_src, lh.as_src);
+        maxrows = std::max(maxrows, lh.As.nrows);
+      }
       maxrows = std::max(maxrows, lh.A.nrows);
       if (!lh.agg.empty()) {
         d->transfer = true;
@@ -1077,7 +1094,7 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
     T* xa = (k & 1) ? n0.xs.template as<T>() : n0.x.template as<T>();
     T* xb = (k & 1) ? n0.x.template as<T>() : n0.xs.template as<T>();
     if (kernel == 0) {
-      hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa, xb,
+      hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, 0, nf, (const T*)xa, xb,
                          (const T*)h.cr.template as<T>(), n0.d.template as<T>(),
                          (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0, part);
     } else if (kernel == 1) {
@@ -1086,7 +1103,8 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
                          h.cq.template as<T>(), nodone, (const SolveScalars*)sc, part);
     } else {
       hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
-                         d0.A.view<double>(t->A64.as<double>()), (const double*)t->x64.as<double>(),
+                         d0.A.view<double>(t->A64.as<double>()), 0, nf,
+                         (const double*)t->x64.as<double>(),
                          (const double*)t->b64.as<double>(), t->r64.as<double>(), nodone);
     }
   };

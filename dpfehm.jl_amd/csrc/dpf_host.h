@@ -44,6 +44,11 @@ struct SellHost {
 struct LevelHost {
   SellHost A;
   std::vector<uint16_t> diag_slot;  // per row: slot of the diagonal entry
+  // A with each row split over 2^as_lg lanes for the smoothing sweeps
+  // (coarse levels, whose rows are long); as_lg = 0: the sweeps use A itself
+  int as_lg = 0;
+  SellHost As;
+  std::vector<int32_t> as_src;      // per As entry: index of the A entry, -1 = padding
   std::vector<int32_t> agg;         // per row: aggregate (= coarse row)
   SellHost P;                       // prolongator, nrows x ncoarse
   std::vector<uint16_t> p_slot;     // per A entry: slot of agg(col) in P's row
@@ -90,5 +95,6 @@ struct TopologyHost {
 std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
                            const double* aol, int64_t nd, const int64_t* dn);
 std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max);
+void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src);
 
 }  // namespace dpf

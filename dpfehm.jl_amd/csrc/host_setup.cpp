@@ -402,6 +402,32 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
 
 }  // namespace
 
+// Row-split copy of a SELL matrix: sub-row r*G + g holds the entries
+// g, g + G, ... of row r, G = 2^lg chosen so that a lane sees ~4 entries.
+void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src) {
+  double avg = A.nrows > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
+  lg = 0;
+  while ((1 << lg) * 4 < avg && lg < 6) lg++;
+  int G = 1 << lg;
+  int64_t ns = A.nrows * G;
+  std::vector<int64_t> rp(ns + 1, 0);
+  std::vector<int32_t> ci, si;
+  ci.reserve(A.nnz);
+  si.reserve(A.nnz);
+  for (int64_t r = 0; r < A.nrows; r++)
+    for (int g = 0; g < G; g++) {
+      for (int t = g; t < A.rowlen[r]; t += G) {
+        ci.push_back(A.at(r, t));
+        si.push_back((int32_t)A.idx(r, t));
+      }
+      rp[r * G + g + 1] = (int64_t)ci.size();
+    }
+  S.build(ns, A.ncols, rp, ci, false);
+  src.assign(S.entries(), -1);
+  for (int64_t r = 0; r < ns; r++)
+    for (int k = 0; k < S.rowlen[r]; k++) src[S.idx(r, k)] = si[rp[r] + k];
+}
+
 std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max) {
   auto t0 = std::chrono::steady_clock::now();
   T.levels.resize(1);
@@ -418,6 +444,8 @@ std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int
     T.levels.push_back(std::move(next));
   }
   T.coarse_dense = T.levels.back().A.nrows <= dense_max;
+  for (size_t l = 1; l < T.levels.size(); l++) split_rows(T.levels[l].A, T.levels[l].as_lg,
+                                                          T.levels[l].As, T.levels[l].as_src);
   T.setup_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return "";

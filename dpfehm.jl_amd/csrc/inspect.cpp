@@ -83,6 +83,10 @@ int with_array(const TopologyHost& T, int32_t level, const std::string& s, F&& f
   if (s == "Rs.col") return f(L.Rs.col);
   if (s == "Rs.rowlen") return f(L.Rs.rowlen);
   if (s == "rs_src") return f(L.rs_src);
+  if (s == "As.sptr") return f(L.As.sptr);
+  if (s == "As.col") return f(L.As.col);
+  if (s == "As.rowlen") return f(L.As.rowlen);
+  if (s == "as_src") return f(L.as_src);
   if (s == "ap_rp") return f(L.ap_rp);
   if (s == "ap_col") return f(L.ap_col);
   if (s == "ap_off") return f(L.ap_off);
@@ -99,12 +103,13 @@ int with_array(const TopologyHost& T, int32_t level, const std::string& s, F&& f
 extern "C" {
 
 // Sizes (element counts) of the arrays dpf_inspect_get can return; the
-// pseudo-name "rs_lg" returns log2 of the restriction's lanes per row.
+// pseudo-names "rs_lg" / "as_lg" return log2 of the lanes per row of the
+// split restriction / split level operator.
 int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) {
   std::string s(name);
-  if (s == "rs_lg") {
+  if (s == "rs_lg" || s == "as_lg") {
     if (level < 0 || level >= (int32_t)h->H.levels.size()) return -1;
-    return h->H.levels[level].rs_lg;
+    return s == "rs_lg" ? h->H.levels[level].rs_lg : h->H.levels[level].as_lg;
   }
   int64_t out = -1;
   with_array(h->H, level, s, [&](const auto& v) {

@@ -376,28 +376,43 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 
 // ------------------------------------------------------------------ V-cycle
 
-// One Chebyshev step of the l1-Jacobi-preconditioned level operator:
-//   d <- c1 d + c2 invd (b - A xin);  xout = xin + d.
-// With dot != nullptr, also sums b . xout over the grid into sc->rz (the
-// fine-level sweep that completes z = M^-1 r, where b is the CG residual).
-// FINE marks the finest level's instantiation (a separate symbol in profiles).
-template <typename T, bool FINE>
-__global__ void k_cheb(SellD<T> A, const T* __restrict__ xin, T* __restrict__ xout,
-                       const T* __restrict__ b, T* __restrict__ d, const T* __restrict__ invd,
-                       T c1, T c2, const int* __restrict__ done, int fuse_dot,
-                       double* __restrict__ partial) {
-  if (*done) return;
-  int64_t r = logical_row();
-  double local = 0.0;
-  if (r < A.nrows) {
+// (A x)_r with the rows of A split over G = 2^lg consecutive lanes: every
+// lane of the group returns the full row sum (fixed shuffle pattern).  The
+// caller's lanes must all reach this call (no early exit before it).
+template <typename T>
+__device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int lg,
+                                           const T* __restrict__ x) {
+  T ax = (T)0;
+  if (sub < A.nrows) {
     int64_t base;
     int w;
-    sell_row(A, r, base, w);
-    T ax = (T)0;
+    sell_row(A, sub, base, w);
     for (int k = 0; k < w; k++) {
       int64_t e = base + (int64_t)k * 64;
-      ax += A.val[e] * xin[A.col[e]];
+      ax += A.val[e] * x[A.col[e]];
     }
+  }
+  for (int o = (1 << lg) >> 1; o > 0; o >>= 1) ax += __shfl_xor(ax, o, 64);
+  return ax;
+}
+
+// One Chebyshev step of the l1-Jacobi-preconditioned level operator:
+//   d <- c1 d + c2 invd (b - A xin);  xout = xin + d,
+// A given with its rows split over 2^lg lanes (lg = 0 on the finest level).
+// With fuse_dot, also writes the block partials of b . xout (the fine-level
+// sweep that completes z = M^-1 r, where b is the CG residual).
+// FINE marks the finest level's instantiation (a separate symbol in profiles).
+template <typename T, bool FINE>
+__global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
+                       T* __restrict__ xout, const T* __restrict__ b, T* __restrict__ d,
+                       const T* __restrict__ invd, T c1, T c2, const int* __restrict__ done,
+                       int fuse_dot, double* __restrict__ partial) {
+  if (*done) return;
+  int64_t sub = logical_row();
+  T ax = split_row_dot(A, sub, lg, xin);
+  int64_t r = sub >> lg;
+  double local = 0.0;
+  if ((sub & ((1 << lg) - 1)) == 0 && r < n) {
     T br = b[r];
     T dn = c1 * d[r] + c2 * invd[r] * (br - ax);
     d[r] = dn;
@@ -408,22 +423,16 @@ __global__ void k_cheb(SellD<T> A, const T* __restrict__ xin, T* __restrict__ xo
   if (fuse_dot) block_partial(local, partial);
 }
 
-// r = b - A x
+// r = b - A x  (rows of A split over 2^lg lanes)
 template <typename T>
-__global__ void k_residual(SellD<T> A, const T* __restrict__ x, const T* __restrict__ b,
-                           T* __restrict__ r, const int* __restrict__ done) {
+__global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ x,
+                           const T* __restrict__ b, T* __restrict__ r,
+                           const int* __restrict__ done) {
   if (*done) return;
-  int64_t i = logical_row();
-  if (i >= A.nrows) return;
-  int64_t base;
-  int w;
-  sell_row(A, i, base, w);
-  T ax = (T)0;
-  for (int k = 0; k < w; k++) {
-    int64_t e = base + (int64_t)k * 64;
-    ax += A.val[e] * x[A.col[e]];
-  }
-  r[i] = b[i] - ax;
+  int64_t sub = logical_row();
+  T ax = split_row_dot(A, sub, lg, x);
+  int64_t i = sub >> lg;
+  if ((sub & ((1 << lg) - 1)) == 0 && i < n) r[i] = b[i] - ax;
 }
 
 // bc = R rf with every row of R split over G = 2^lg consecutive lanes

@@ -18,7 +18,8 @@ DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.in
       "ap_col": np.int32, "diag_slot": np.uint16, "p_slot": np.uint16, "p_self": np.uint16,
       "Rs.sptr": np.int64, "Rs.col": np.int32, "Rs.rowlen": np.int32, "rs_src": np.int32,
       "ap_off": np.int64, "ap_a": np.int32, "ap_p": np.int32, "rap_off": np.int64,
-      "rap_p": np.int32, "rap_ap": np.int32}
+      "rap_p": np.int32, "rap_ap": np.int32, "As.sptr": np.int64, "As.col": np.int32,
+      "As.rowlen": np.int32, "as_src": np.int32}
 
 
 class Inspect:
@@ -127,7 +128,17 @@ def replay(I, Ke, aol, dh, Q):
         A["P"] = sell_to_csr(Ps, Pc, Pl, pv, n, nc)
         A["AP"] = sp.csr_matrix((apv, I.get("ap_col", l), ap_rp), shape=(n, nc))
         A["agg"] = I.get("agg", l)
-        levels.append(dict(sptr=Cs, col=Cc, rowlen=Cl, val=cv, invd=cinv, n=nc))
+        nxt = dict(sptr=Cs, col=Cc, rowlen=Cl, val=cv, invd=cinv, n=nc)
+        alg = I.L.dpf_inspect_size(I.h, l + 1, b"as_lg")
+        if alg > 0:  # row-split copy used by the smoothing sweeps
+            asrc = I.get("as_src", l + 1)
+            asv = np.where(asrc >= 0, cv[np.maximum(asrc, 0)], 0.0)
+            Asplit = sell_to_csr(I.get("As.sptr", l + 1), I.get("As.col", l + 1),
+                                 I.get("As.rowlen", l + 1), asv, nc << alg, nc)
+            rows = sp.csr_matrix((np.ones(nc << alg), (np.arange(nc << alg) >> alg,
+                                                       np.arange(nc << alg))), shape=(nc, nc << alg))
+            nxt["A_split"] = (rows @ Asplit).tocsr()
+        levels.append(nxt)
     for L in levels:
         L["A"] = sell_to_csr(L["sptr"], L["col"], L["rowlen"], L["val"], L["n"], L["n"])
     return levels, b
@@ -168,6 +179,8 @@ def test_replayed_setup_matches_sparse_algebra(pkg, oracle, shape, dirichlet):
         Ac = levels[l + 1]["A"]
         Ac_ref = L["P"].T @ A @ L["P"]
         assert abs(Ac - Ac_ref).max() <= 1e-12 * abs(Ac_ref).max()
+        if "A_split" in levels[l + 1]:
+            assert abs(levels[l + 1]["A_split"] - Ac).max() <= 1e-15 * abs(Ac).max()
         # Galerkin operators stay symmetric and every aggregate is non-empty
         assert abs(Ac - Ac.T).max() <= 1e-12 * abs(Ac).max()
         assert np.bincount(L["agg"], minlength=nc).min() >= 1
