@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU (asked for --gpus {args.gpus}); nothing in this build "
@@ -82,9 +85,12 @@ def main():
     dK_d = torch.empty(w.nedges, dtype=torch.float64, device=dev)
     dlogK_d = torch.empty(w.n, dtype=torch.float64, device=dev)
 
-    def opts(precision):
-        return api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
-                         stream=sptr)
+    def opts(precision, timing=False):
+        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
+                      stream=sptr)
+        if timing:
+            o.flags = _lib.DPF_FLAG_SWEEP_TIMING
+        return o
 
     def p(t):
         return ctypes.c_void_p(t.data_ptr())
@@ -120,6 +126,14 @@ def main():
         stats.append((st.as_dict(), st2.as_dict()))
         return J
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 26, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 3 x 256 MiB read + 256 MiB written")
+
     results = {}
     for precision in ("fp64", "mixed"):
         o = opts(precision)
@@ -148,8 +162,23 @@ def main():
     grad_err = float(np.linalg.norm(acc["mixed"][1] - acc["fp64"][1]) / np.linalg.norm(acc["fp64"][1]))
     log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}")
 
-    # roofline of the dominant kernel: finest-level FP32 Chebyshev sweep of the
-    # mixed path, HIP events around back-to-back launches on the bench stream
+    # roofline of the dominant kernel, the finest-level FP32 Chebyshev sweep of
+    # the mixed path: the same timed loop again with a hipEvent pair around
+    # every launch of that kernel (on the stream it runs on), averaged
+    live = {}
+    for precision in ("mixed", "fp64"):
+        o = opts(precision, timing=True)
+        stats = []
+        for k in range(args.steps):
+            evaluate(k, o, stats)
+        torch.cuda.synchronize()
+        ms_tot = sum(a["sweep_ms"] + b["sweep_ms"] for a, b in stats)
+        n_tot = sum(a["sweep_launches"] + b["sweep_launches"] for a, b in stats)
+        live[precision] = (ms_tot / max(n_tot, 1), n_tot)
+        log(f"[bench] live finest-level sweep {precision}: {n_tot} launches, "
+            f"{1e3 * ms_tot / max(n_tot, 1):.1f} us mean")
+
+    # back-to-back launches of single kernels, for reference
     prof = {}
     for precision, pc in (("mixed", _lib.DPF_MIXED), ("fp64", _lib.DPF_FP64)):
         if precision == "fp64":
@@ -165,7 +194,8 @@ def main():
             prof[(precision, kern)] = (ms.value, by.value)
             log(f"[bench] kernel {kern} {precision}: {ms.value * 1e3:.1f} us/launch, "
                 f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e6:.0f} GB/s")
-    ms, by = prof[("mixed", 0)]
+    by = prof[("mixed", 0)][1]
+    ms = live["mixed"][0]
     achieved = by / (ms * 1e-3) / 1e9
 
     value = args.steps / results["mixed"]["seconds"]
@@ -212,15 +242,35 @@ def main():
             "peak": 8000.0,
             "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
-            "traffic": None,
+            "traffic": traffic_per_launch(ns),
             "us_per_launch": round(ms * 1e3, 2),
+            "launches_timed": live["mixed"][1],
             "bytes_per_launch": by,
+            "fp64_us_per_launch": round(live["fp64"][0] * 1e3, 2),
+            "fp64_GBps": round(prof[("fp64", 0)][1] / (live["fp64"][0] * 1e-3) / 1e9, 1),
         },
-        "kernels": {f"{pr}/{k}": {"us": round(v[0] * 1e3, 2), "GBps": round(v[1] / v[0] / 1e6, 1)}
-                    for (pr, k), v in prof.items()},
+        "kernels_back_to_back": {f"{pr}/{k}": {"us": round(v[0] * 1e3, 2),
+                                           "GBps": round(v[1] / v[0] / 1e6, 1)}
+                             for (pr, k), v in prof.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_per_launch(ns):
+    """HBM bytes per launch of the finest-level FP32 Chebyshev sweep, from the
+    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE measurement of this
+    workload (scaled by the same counters on a plain copy of known size), or
+    None if no measurement for this grid is committed."""
+    path = os.path.join(ROOT, "profiles", "k_cheb_fine_fp32_2m_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if tuple(t.get("grid", ())) != tuple(ns):
+        return None
+    return t.get("bytes_per_launch")
 
 
 def cpu_baseline(w, logK, h_obs, budget_s):

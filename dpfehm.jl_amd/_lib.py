@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libdpfgpu.so")
 DPF_OK, DPF_EINVAL, DPF_EHIP, DPF_ENOCONV, DPF_EBREAKDOWN, DPF_ESINGULAR, DPF_ENOMEM = range(7)
 DPF_FP64, DPF_MIXED = 0, 1
 DPF_HOST, DPF_DEVICE = 0, 1
+DPF_FLAG_SWEEP_TIMING = 1
 
 _ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOCONV", 4: "EBREAKDOWN", 5: "ESINGULAR", 6: "ENOMEM"}
 
@@ -51,6 +52,9 @@ class Stats(ctypes.Structure):
         ("setup_ms", ctypes.c_double),
         ("solve_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("sweep_ms", ctypes.c_double),
+        ("sweep_launches", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):

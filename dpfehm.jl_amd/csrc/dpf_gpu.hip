@@ -179,7 +179,16 @@ struct dpf_topology {
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
   int cheb_degree = 2;
   double cheb_ratio = 30.0;
-  int last_iters[2] = {0, 0};  // iterations of the last FP64 / FP32 solve
+  int last_iters[2][8] = {};  // iterations of the last FP64 / FP32 solve per refinement step
+  // optional event timing of every finest-level smoothing sweep (opts->flags)
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  double sweep_ms = 0;
+  int sweep_n = 0;
+  ~dpf_topology() {
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+  }
   int levels() const { return (int)H.levels.size(); }
 };
 
@@ -327,6 +336,27 @@ void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double re
                      abstol2, maxiter);
 }
 
+// Event pair around one finest-level sweep (only when timing is on).
+hipEvent_t timing_event(dpf_topology* t) {
+  if (t->evused == t->evpool.size()) {
+    hipEvent_t e;
+    HIPTRY(hipEventCreate(&e));
+    t->evpool.push_back(e);
+  }
+  return t->evpool[t->evused++];
+}
+
+// Adds the recorded sweep durations (the stream has been synchronised).
+void harvest_timing(dpf_topology* t) {
+  for (size_t i = 0; i + 1 < t->evused; i += 2) {
+    float ms = 0;
+    HIPTRY(hipEventElapsedTime(&ms, t->evpool[i], t->evpool[i + 1]));
+    t->sweep_ms += ms;
+    t->sweep_n++;
+  }
+  t->evused = 0;
+}
+
 // One V-cycle on levels l..: input b at level l (for l = 0 the CG residual),
 // level x/d already hold the zero-guess first Chebyshev step.  Returns the
 // buffer holding x_l on exit through *xout.
@@ -366,8 +396,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       c2.make(2 * m, 1.0, t->cheb_ratio);
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
+        if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
         hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd,
                            invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, part);
+        if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
       }
@@ -377,8 +409,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   }
   // pre-smoothing (step 0 done by the producer of b)
   for (int k = 1; k < m; k++) {
+    if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
                        (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, part);
+    if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
@@ -403,8 +437,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     double c1 = k == 0 ? 0.0 : ch.c1[k];
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
+    if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
                        (T)c1, (T)c2, (const int*)done, fuse, part);
+    if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
   }
@@ -422,7 +458,7 @@ struct SolveResult {
 // convergence flag in chunks of iterations).
 template <typename T>
 SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double reltol,
-                double abstol, int maxiter, const Cheb& ch, int chunk_hint) {
+                double abstol, int maxiter, const Cheb& ch, int chunk_hint, int step) {
   LevelDev& d0 = *t->lev[0];
   LevelNum<T>& n0 = h.lev[0];
   int64_t nf = t->H.nf;
@@ -447,7 +483,8 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   int par = 0;
   SolveResult res{0, 0, 0, 0};
   int pidx = sizeof(T) == sizeof(double) ? 0 : 1;
-  int predicted = t->last_iters[pidx];
+  int sidx = std::min(step, 7);
+  int predicted = t->last_iters[pidx][sidx];
   // launch up to the iteration count of the previous solve, then poll every
   // iteration, so that no launch is wasted past convergence
   int chunk = predicted > 2 ? predicted - 2 : std::max(1, chunk_hint);
@@ -483,7 +520,8 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
       throw Err{DPF_EHIP, "Krylov loop made no progress on the device"};
     chunk = 1;
   }
-  t->last_iters[pidx] = res.iters;
+  t->last_iters[pidx][sidx] = res.iters;
+  harvest_timing(t);
   return res;
 }
 
@@ -665,6 +703,9 @@ double now_ms() {
 int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
                  const dpf_opts* o, dpf_stats* st, const double* x0 = nullptr) {
   int64_t nf = t->H.nf;
+  t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
+  t->sweep_ms = 0;
+  t->sweep_n = 0;
   int deg = o->cheb_degree > 0 ? o->cheb_degree : t->cheb_degree;
   Cheb ch;
   ch.make(deg, 1.0, t->cheb_ratio);
@@ -717,14 +758,14 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     }
     SolveResult r;
     if (o->precision == DPF_FP64) {
-      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, ch, 16);
+      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, ch, 16, outer);
       hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
                          (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
     } else {
       hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(BLK), 0, s, nf,
                          (const double*)t->r64.as<double>(), t->b32.as<float>(),
                          (const SolveScalars*)sc);
-      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, ch, 8);
+      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, ch, 8, outer);
       hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
                          (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc);
     }
@@ -752,7 +793,10 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     st->converged = (status == DPF_OK);
     st->levels = t->levels();
     st->relres = relres;
+    st->sweep_ms = t->sweep_ms;
+    st->sweep_launches = t->sweep_n;
   }
+  t->timing = false;
   return status;
 }
 

@@ -36,6 +36,10 @@ enum {
 };
 enum { DPF_FP64 = 0, DPF_MIXED = 1 };
 enum { DPF_HOST = 0, DPF_DEVICE = 1 };
+/* flags: time every finest-level smoothing sweep with a hipEvent pair and
+ * report the sum in dpf_stats.sweep_ms / sweep_launches (adds two event
+ * records per sweep) */
+enum { DPF_FLAG_SWEEP_TIMING = 1 };
 
 typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
@@ -46,7 +50,7 @@ typedef struct {
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
   int32_t cheb_degree; /* Chebyshev smoother degree per pre/post sweep (0: default 2) */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
-  int32_t flags;       /* reserved, 0 */
+  int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
   void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = topology stream */
 } dpf_opts;
 
@@ -59,6 +63,9 @@ typedef struct {
   double setup_ms;     /* assembly + numeric AMG refresh (GPU) */
   double solve_ms;     /* Krylov solve(s) */
   double total_ms;     /* the whole call, wall clock */
+  double sweep_ms;     /* with DPF_FLAG_SWEEP_TIMING: summed finest-level sweep time */
+  int32_t sweep_launches;
+  int32_t reserved;
 } dpf_stats;
 
 /* Message of the last failing call on this thread ("" if none). */
