@@ -41,6 +41,12 @@ struct Err {
 
 static inline unsigned nblocks(int64_t n) { return (unsigned)((n + BLK - 1) / BLK); }
 
+static double now_wall_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
 // ------------------------------------------------------------------ device buffers
 
 struct DBuf {
@@ -175,11 +181,13 @@ struct dpf_topology {
   // reductions and solve scalars
   DBuf partial, scal, zero_flag;
   SolveScalars* scal_host = nullptr;
+  HostMirror* hm_host = nullptr;  // pinned coherent host memory the finish kernel writes
+  HostMirror* hm_dev = nullptr;   // the same memory as seen by the device
+  int seq = 0;                    // solve / residual sequence numbers
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
   int cheb_degree = 2;
   double cheb_ratio = 30.0;
-  int last_iters[2][8] = {};  // iterations of the last FP64 / FP32 solve per refinement step
   // optional event timing of every finest-level smoothing sweep (opts->flags)
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -330,10 +338,35 @@ auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
 
 // Finishes the grid sum the previous kernel (nparts blocks) left in t->partial.
 void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol2 = 0,
-            double abstol2 = 0, int maxiter = 0) {
+            double abstol2 = 0, int maxiter = 0, int seq = 0) {
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, (int)nparts,
                      (const double*)t->partial.as<double>(), t->scal.as<SolveScalars>(), reltol2,
-                     abstol2, maxiter);
+                     abstol2, maxiter, seq, t->hm_dev);
+}
+
+int mirror_load(const int* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+double mirror_load(const double* p) {
+  double v;
+  __atomic_load(p, &v, __ATOMIC_ACQUIRE);
+  return v;
+}
+
+// Waits (spinning) until the mirror's field at `seqp` equals `want`.  Fails
+// if the stream drains or errs without it, or after a generous time limit.
+void wait_published(dpf_topology* t, hipStream_t s, const int* seqp, int want) {
+  double t0 = now_wall_ms();
+  unsigned spins = 0;
+  while (mirror_load(seqp) != want) {
+    if ((++spins & 1023) == 0) {
+      hipError_t q = hipStreamQuery(s);
+      if (q != hipSuccess && q != hipErrorNotReady)
+        throw Err{DPF_EHIP, std::string("stream error while waiting: ") + hipGetErrorString(q)};
+      if (q == hipSuccess && mirror_load(seqp) != want)
+        throw Err{DPF_EHIP, "stream drained without publishing the solve state"};
+      if (now_wall_ms() - t0 > 300000.0) throw Err{DPF_EHIP, "timed out waiting for the GPU"};
+    }
+    __builtin_ia32_pause();
+  }
 }
 
 // Event pair around one finest-level sweep (only when timing is on).
@@ -348,6 +381,7 @@ hipEvent_t timing_event(dpf_topology* t) {
 
 // Adds the recorded sweep durations (the stream has been synchronised).
 void harvest_timing(dpf_topology* t) {
+  if (t->evused >= 2) HIPTRY(hipEventSynchronize(t->evpool[t->evused - 1]));
   for (size_t i = 0; i + 1 < t->evused; i += 2) {
     float ms = 0;
     HIPTRY(hipEventElapsedTime(&ms, t->evpool[i], t->evpool[i + 1]));
@@ -475,22 +509,37 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   T* dd = n0.d.template as<T>();
   T* z0 = n0.x.template as<T>();
   T* zs = n0.xs.template as<T>();
+  int seq = ++t->seq;
   hipLaunchKernelGGL(k_cg_start<T>, dim3(g), dim3(BLK), 0, s, nf, b, r, x, invd, dd, z0,
                      (T)ch.c0, part);
-  finish(t, s, FIN_START, g, reltol * reltol, abstol * abstol, maxiter);
+  finish(t, s, FIN_START, g, reltol * reltol, abstol * abstol, maxiter, seq);
   T* z = nullptr;
   vcycle<T>(t, h, s, 0, r, z0, zs, &z, ch, done, true);
   int par = 0;
   SolveResult res{0, 0, 0, 0};
-  int pidx = sizeof(T) == sizeof(double) ? 0 : 1;
-  int sidx = std::min(step, 7);
-  int predicted = t->last_iters[pidx][sidx];
-  // launch up to the iteration count of the previous solve, then poll every
-  // iteration, so that no launch is wasted past convergence
-  int chunk = predicted > 2 ? predicted - 2 : std::max(1, chunk_hint);
+  (void)chunk_hint;
+  (void)step;
+  // Keep one iteration queued beyond the last one whose convergence test the
+  // device has published: the host enqueues iteration k+1 while the GPU runs
+  // iteration k's V-cycle, no launch outlives convergence, and the stream is
+  // never drained to poll.
+  HostMirror* m = t->hm_host;
   int launched = 0;
+  double t0 = now_wall_ms();
+  unsigned spins = 0;
   while (true) {
-    for (int c = 0; c < chunk; c++) {
+    int its = 0, dn = 0;
+    if (mirror_load(&m->seq) == seq) {
+      dn = mirror_load(&m->done);
+      its = mirror_load(&m->iters);
+    }
+    if (dn != 0) {
+      res.status = dn;
+      res.iters = its;
+      res.rr = mirror_load(&m->rr);
+      break;
+    }
+    if (launched < maxiter && launched <= its) {
       hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)z,
                          (const T*)pb[par], pb[par ^ 1], q, (const int*)done,
                          (const SolveScalars*)sc, part);
@@ -504,23 +553,20 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
       vcycle<T>(t, h, s, 0, r, z0, zs, &zz, ch, done, true);
       z = zz;
       launched++;
+      HIPTRY(hipGetLastError());
+      continue;
     }
-    HIPTRY(hipGetLastError());
-    HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipStreamSynchronize(s));
-    SolveScalars hs = *t->scal_host;
-    if (hs.done != 0 || hs.iters >= maxiter) {
-      res.status = hs.done == 0 ? 2 : hs.done;
-      res.iters = hs.iters;
-      res.rr = hs.rr;
-      break;
+    if ((++spins & 1023) == 0) {
+      hipError_t qs = hipStreamQuery(s);
+      if (qs != hipSuccess && qs != hipErrorNotReady)
+        throw Err{DPF_EHIP, std::string("stream error in the Krylov loop: ") + hipGetErrorString(qs)};
+      if (qs == hipSuccess && mirror_load(&m->seq) == seq && mirror_load(&m->done) == 0 &&
+          mirror_load(&m->iters) < launched)
+        throw Err{DPF_EHIP, "Krylov loop made no progress on the device"};
+      if (now_wall_ms() - t0 > 300000.0) throw Err{DPF_EHIP, "timed out waiting for the GPU"};
     }
-    // the device counts iterations; if it stops counting, stop launching
-    if (launched > maxiter + 64 || hs.iters + 2 * chunk < launched)
-      throw Err{DPF_EHIP, "Krylov loop made no progress on the device"};
-    chunk = 1;
+    __builtin_ia32_pause();
   }
-  t->last_iters[pidx][sidx] = res.iters;
   harvest_timing(t);
   return res;
 }
@@ -624,6 +670,11 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
     t->zero_flag.alloc(sizeof(int));
     HIPTRY(hipMemset(t->zero_flag.p, 0, sizeof(int)));
     HIPTRY(hipHostMalloc((void**)&t->scal_host, sizeof(SolveScalars), hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void**)&t->hm_host, sizeof(HostMirror), hipHostMallocCoherent));
+    std::memset(t->hm_host, 0, sizeof(HostMirror));
+    t->hm_host->seq = -1;
+    t->hm_host->outer_seq = -1;
+    HIPTRY(hipHostGetDevicePointer((void**)&t->hm_dev, t->hm_host, 0));
     t->A64.alloc(t->lev[0]->A.entries * sizeof(double));
     t->invd64.alloc(H.nf * sizeof(double));
     t->b64.alloc(H.nf * sizeof(double));
@@ -648,6 +699,7 @@ int dpf_topology_destroy(dpf_topology* t) {
     std::lock_guard<std::mutex> g(t->mu);
     if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
     if (t->scal_host) (void)hipHostFree(t->scal_host);
+    if (t->hm_host) (void)hipHostFree(t->hm_host);
     if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
   }
   delete t;
@@ -721,10 +773,11 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   auto residual = [&]() -> double {  // r64 = b - A x64, returns ||r64||
     hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
                        (const double*)t->x64.as<double>(), b, t->r64.as<double>(), part);
-    finish(t, s, FIN_OUTER, g);
-    HIPTRY(hipMemcpyAsync(t->scal_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipStreamSynchronize(s));
-    return t->scal_host->outer_s;
+    int oseq = ++t->seq;
+    finish(t, s, FIN_OUTER, g, 0, 0, 0, oseq);
+    HIPTRY(hipGetLastError());
+    wait_published(t, s, &t->hm_host->outer_seq, oseq);
+    return mirror_load(&t->hm_host->outer_s);
   };
   // ||b||, then the residual of the initial guess
   hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>());

@@ -23,6 +23,28 @@ constexpr int BLK = 256;
 constexpr uint16_t NO_SLOT_D = 0xFFFF;
 
 // Device-side scalars of one Krylov solve (all FP64, also in mixed mode).
+// Host-visible mirror of the solve's progress, in pinned coherent host
+// memory: the finish kernel writes it with system-scope stores, so the host
+// follows convergence without synchronising the stream or copying.
+struct HostMirror {
+  int seq;        // solve sequence number of the fields below
+  int iters;
+  int done;
+  int outer_seq;  // sequence number of outer_s
+  double rr;
+  double outer_s;
+};
+
+__device__ __forceinline__ void mput(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void mput(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void mpublish(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 struct SolveScalars {
   double rr;       // current ||r||^2
   double rr_stop;  // stop when rr <= rr_stop
@@ -92,7 +114,8 @@ enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER =
 __global__ void __launch_bounds__(1024) k_finish(int mode, int nparts,
                                                  const double* __restrict__ partial,
                                                  SolveScalars* sc, double reltol2,
-                                                 double abstol2, int maxiter) {
+                                                 double abstol2, int maxiter, int seq,
+                                                 HostMirror* hm) {
   if (mode != FIN_START && mode != FIN_OUTER && sc->done) return;
   __shared__ double sh[16];
   double s = 0.0;
@@ -114,37 +137,54 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nparts,
       sc->first = 1;
       sc->rz = 0.0;
       sc->rz_old = 0.0;
-      sc->done = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
+      int dn = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
+      sc->done = dn;
+      mput(&hm->rr, tot);
+      mput(&hm->iters, 0);
+      mput(&hm->done, dn);
+      mpublish(&hm->seq, seq);
       break;
     }
     case FIN_PQ:
       sc->pq = tot;
       sc->first = 0;
-      if (!(tot > 0.0))
+      if (!(tot > 0.0)) {
         sc->done = 3;
-      else
+        mput(&hm->done, 3);
+      } else {
         sc->alpha = sc->rz / tot;
+      }
       break;
     case FIN_XR: {
       sc->rr = tot;
       int it = sc->iters + 1;
       sc->iters = it;
+      int dn = 0;
       if (tot <= sc->rr_stop)
-        sc->done = 1;
+        dn = 1;
       else if (!(tot == tot))
-        sc->done = 3;
+        dn = 3;
       else if (it >= sc->maxiter)
-        sc->done = 2;
+        dn = 2;
+      sc->done = dn;
+      mput(&hm->rr, tot);
+      mput(&hm->done, dn);
+      mput(&hm->iters, it);
       break;
     }
     case FIN_RZ:
       sc->rz_old = sc->rz;
       sc->rz = tot;
-      if (!(tot > 0.0)) sc->done = 3;
+      if (!(tot > 0.0)) {
+        sc->done = 3;
+        mput(&hm->done, 3);
+      }
       break;
     default:
       sc->outer_rr = tot;
       sc->outer_s = sqrt(tot);
+      mput(&hm->outer_s, sqrt(tot));
+      mpublish(&hm->outer_seq, seq);
   }
 }
 
