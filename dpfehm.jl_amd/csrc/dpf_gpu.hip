@@ -110,7 +110,8 @@ struct LevelDev {
   int rs_lg = 0, as_lg = 0;
   DBuf as_src;
   int64_t nc = 0;  // rows of the next level
-  DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_p, rap_ap;
+  DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_dest, rap_p, rap_ap;
+  int ap_lg = 0, rap_lg = 0;
   int64_t ap_nnz = 0;
 };
 
@@ -309,12 +310,14 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
     hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries,
                        d.rs_src.as<int32_t>(), (const T*)P.val,
                        h.lev[l].rval.template as<T>());
-    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.ap_nnz)), dim3(BLK), 0, s, d.ap_nnz,
-                       d.ap_off.as<int64_t>(), d.ap_a.as<int32_t>(), d.ap_p.as<int32_t>(),
-                       (const T*)A.val, (const T*)P.val, apv);
-    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(dn.A.entries)), dim3(BLK), 0, s,
-                       dn.A.entries, d.rap_off.as<int64_t>(), d.rap_p.as<int32_t>(),
-                       d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
+    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s,
+                       d.ap_nnz, d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr,
+                       d.ap_a.as<int32_t>(), d.ap_p.as<int32_t>(), (const T*)A.val,
+                       (const T*)P.val, apv);
+    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(dn.A.entries << d.rap_lg)), dim3(BLK), 0,
+                       s, dn.A.entries, d.rap_lg, d.rap_off.as<int64_t>(),
+                       d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(), d.rap_ap.as<int32_t>(),
+                       (const T*)P.val, (const T*)apv, cval);
     hipLaunchKernelGGL(k_rowdiag<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s,
                        dn.A.view<T>(cval), dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
     if (dn.as_lg > 0)
@@ -655,6 +658,9 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
         upload(d->ap_a, lh.ap_a);
         upload(d->ap_p, lh.ap_p);
         upload(d->rap_off, lh.rap_off);
+        upload(d->rap_dest, lh.rap_dest);
+        d->ap_lg = lh.ap_lg;
+        d->rap_lg = lh.rap_lg;
         upload(d->rap_p, lh.rap_p);
         upload(d->rap_ap, lh.rap_ap);
         d->ap_nnz = lh.ap_rp.back();

@@ -66,10 +66,13 @@ struct LevelHost {
   std::vector<int32_t> ap_col;
   std::vector<int64_t> ap_off;
   std::vector<int32_t> ap_a, ap_p;
-  // next level's A, grouped by destination SELL entry (padding: empty):
-  //   Ac[e] = sum_{q in [rap_off[e], rap_off[e+1])} P.val[rap_p[q]] * AP[rap_ap[q]]
+  // next level's A, one contribution list per SELL entry, lists in row-major
+  // order of (row, slot) so that neighbouring threads share fine rows:
+  //   Ac[rap_dest[k]] = sum_{q in [rap_off[k], rap_off[k+1])} P.val[rap_p[q]] * AP[rap_ap[q]]
+  // (padding entries have empty lists)
   std::vector<int64_t> rap_off;
-  std::vector<int32_t> rap_p, rap_ap;
+  std::vector<int32_t> rap_dest, rap_p, rap_ap;
+  int ap_lg = 0, rap_lg = 0;  // log2 lanes per contribution list on the GPU
 };
 
 struct TopologyHost {

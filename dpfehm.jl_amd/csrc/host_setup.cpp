@@ -371,6 +371,18 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
     }
     const SellHost& C = next.A;
     int64_t nent = C.entries();
+    // row-major list numbering over all SELL slots, padding included (also
+    // the lanes of the last slice beyond the last row)
+    int64_t nrm = C.nslices() * SELL_C;
+    std::vector<int64_t> rm_base(nrm + 1, 0);
+    for (int64_t I = 0; I < nrm; I++) {
+      int64_t sl = I / SELL_C;
+      rm_base[I + 1] = rm_base[I] + (C.sptr[sl + 1] - C.sptr[sl]) / SELL_C;
+    }
+    L.rap_dest.resize(nent);
+    for (int64_t I = 0; I < nrm; I++)
+      for (int64_t k = 0; k < rm_base[I + 1] - rm_base[I]; k++)
+        L.rap_dest[rm_base[I] + k] = (int32_t)C.idx(I, (int)k);
     L.rap_off.assign(nent + 1, 0);
     std::vector<int32_t> slot_of(nc, 0);
     auto each = [&](auto&& fn) {
@@ -380,7 +392,7 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
           int32_t i = R.at(I, t);
           int32_t pe = L.r_src[R.idx(I, t)];
           for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
-            fn(C.idx(I, slot_of[L.ap_col[a]]), pe, a);
+            fn(rm_base[I] + slot_of[L.ap_col[a]], pe, a);
         }
       }
     };
@@ -396,6 +408,15 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       L.rap_p[q] = pe;
       L.rap_ap[q] = (int32_t)a;
     });
+    // lanes per list: about 4 contributions per lane on average
+    auto lanes = [](int64_t nprod_, int64_t nlists) {
+      double avg = nlists > 0 ? (double)nprod_ / (double)nlists : 1.0;
+      int lg = 0;
+      while ((1 << lg) * 4 < avg && lg < 5) lg++;
+      return lg;
+    };
+    L.rap_lg = lanes(nprod, C.nnz);
+    L.ap_lg = lanes(L.ap_off.back(), (int64_t)L.ap_col.size());
   }
   return "";
 }

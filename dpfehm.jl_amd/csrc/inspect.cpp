@@ -94,6 +94,7 @@ int with_array(const TopologyHost& T, int32_t level, const std::string& s, F&& f
   if (s == "ap_p") return f(L.ap_p);
   if (s == "rap_off") return f(L.rap_off);
   if (s == "rap_p") return f(L.rap_p);
+  if (s == "rap_dest") return f(L.rap_dest);
   if (s == "rap_ap") return f(L.rap_ap);
   return -1;
 }

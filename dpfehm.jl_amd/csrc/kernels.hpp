@@ -336,21 +336,29 @@ __global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
   rval[e] = s >= 0 ? pval[s] : (T)0;
 }
 
-// Galerkin products grouped by destination: out[d] = sum over the
-// destination's contribution list of X[s1[q]] * Y[s2[q]], in the host's fixed
-// enumeration order.  Used for A*P (X = A, Y = P) and for the next level's
-// operator R*(A*P) (X = P, since R(I,i) = P(i,I); Y = A*P).  Destinations with
-// an empty list (SELL padding) become 0.
+// Galerkin products grouped by destination: list k holds the contributions
+// of destination dest[k] (k itself when dest is null):
+//   out[dest[k]] = sum_{q in [off[k], off[k+1])} X[s1[q]] * Y[s2[q]].
+// Each list is spread over G = 2^lg consecutive lanes (lane g takes q = off[k]
+// + g, g + G, ...) and combined by wave shuffles in a fixed pattern, so the
+// result is reproducible.  Used for A*P (X = A, Y = P) and for the next
+// level's operator R*(A*P) (X = P, since R(I,i) = P(i,I); Y = A*P); empty
+// lists (SELL padding) give 0.
 template <typename T>
-__global__ void k_prod_gather(int64_t ndest, const int64_t* __restrict__ off,
-                              const int32_t* __restrict__ s1, const int32_t* __restrict__ s2,
-                              const T* __restrict__ X, const T* __restrict__ Y,
-                              T* __restrict__ out) {
-  int64_t d = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (d >= ndest) return;
+__global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ dest, const int32_t* __restrict__ s1,
+                              const int32_t* __restrict__ s2, const T* __restrict__ X,
+                              const T* __restrict__ Y, T* __restrict__ out) {
+  int64_t sub = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int64_t k = sub >> lg;
+  int G = 1 << lg;
   T acc = (T)0;
-  for (int64_t q = off[d]; q < off[d + 1]; q++) acc += X[s1[q]] * Y[s2[q]];
-  out[d] = acc;
+  if (k < nlists) {
+    int64_t q1 = off[k + 1];
+    for (int64_t q = off[k] + (sub & (G - 1)); q < q1; q += G) acc += X[s1[q]] * Y[s2[q]];
+  }
+  for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((sub & (G - 1)) == 0 && k < nlists) out[dest ? dest[k] : k] = acc;
 }
 
 // invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator

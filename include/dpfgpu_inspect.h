@@ -10,7 +10,7 @@
  *   n2f, f2n, inc_edge, inc_other: int32   inc_ptr: int64   inc_slot: uint16
  *   A.sptr, P.sptr, R.sptr, Rs.sptr, As.sptr, ap_rp, ap_off, rap_off: int64
  *   A.col, A.rowlen, P.col, P.rowlen, R.col, R.rowlen, Rs.col, Rs.rowlen, As.col,
- *   As.rowlen, agg, r_src, rs_src, as_src, ap_col, ap_a, ap_p, rap_p, rap_ap: int32
+ *   As.rowlen, agg, r_src, rs_src, as_src, ap_col, ap_a, ap_p, rap_dest, rap_p, rap_ap: int32
  *   diag_slot, p_slot, p_self: uint16
  * dpf_inspect_size(h, level, "rs_lg" | "as_lg") returns log2 of the lanes per
  * row of the split restriction / split level operator (no array).

@@ -19,7 +19,7 @@ DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.in
       "Rs.sptr": np.int64, "Rs.col": np.int32, "Rs.rowlen": np.int32, "rs_src": np.int32,
       "ap_off": np.int64, "ap_a": np.int32, "ap_p": np.int32, "rap_off": np.int64,
       "rap_p": np.int32, "rap_ap": np.int32, "As.sptr": np.int64, "As.col": np.int32,
-      "As.rowlen": np.int32, "as_src": np.int32}
+      "As.rowlen": np.int32, "as_src": np.int32, "rap_dest": np.int32}
 
 
 class Inspect:
@@ -107,9 +107,13 @@ def replay(I, Ke, aol, dh, Q):
                           I.get("diag_slot", l + 1))
         nc = Cl.size
         rap_off, rap_p, rap_ap = I.get("rap_off", l), I.get("rap_p", l), I.get("rap_ap", l)
+        rap_dest = I.get("rap_dest", l)
         assert rap_off.size == Cs[-1] + 1
-        cv = np.array([np.dot(pv[rap_p[rap_off[e]:rap_off[e + 1]]],
-                              apv[rap_ap[rap_off[e]:rap_off[e + 1]]]) for e in range(Cs[-1])])
+        assert np.array_equal(np.sort(rap_dest), np.arange(Cs[-1]))  # every entry exactly once
+        cv = np.zeros(Cs[-1])
+        for k in range(Cs[-1]):
+            cv[rap_dest[k]] = np.dot(pv[rap_p[rap_off[k]:rap_off[k + 1]]],
+                                     apv[rap_ap[rap_off[k]:rap_off[k + 1]]])
         cinv = np.zeros(nc)
         for Ir in range(nc):
             w = (Cs[Ir // 64 + 1] - Cs[Ir // 64]) // 64
