@@ -198,6 +198,8 @@ def main():
     ms = live["mixed"][0]
     achieved = by / (ms * 1e-3) / 1e9
 
+    tuned = topo.tuned_degrees()
+    log(f"[bench] tuned Chebyshev degrees (finest, coarse): {tuned}")
     value = args.steps / results["mixed"]["seconds"]
     fp64_rate = args.steps / results["fp64"]["seconds"]
 
@@ -235,6 +237,7 @@ def main():
         "iterations_fp64_fwd_adj": [results["fp64"]["stats"][0][0]["iterations"],
                                     results["fp64"]["stats"][0][1]["iterations"]],
         "topology_setup_s": round(topo_s, 3),
+        "cheb_degrees_finest_coarse": tuned,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",

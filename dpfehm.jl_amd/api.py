@@ -97,8 +97,7 @@ class Topology:
         _lib.check(_lib.lib().dpf_topology_create(self.n, self.ne, _ptr(nb), _ptr(aol),
                                                   dn.size, _ptr(dn), ctypes.byref(h)))
         self._h = h
-        info = np.zeros(8, dtype=np.int64)
-        _lib.lib().dpf_topology_info(self._h, _ptr(info), 8)
+        info = self.info()
         self.nfree = int(info[2])
         self.nnz = int(info[3])
         self.levels = int(info[4])
@@ -109,6 +108,21 @@ class Topology:
         isfree[dn - 1] = False
         self.isfree = isfree
         self.lock = threading.Lock()
+
+    def info(self):
+        info = np.zeros(12, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 12)
+        return info
+
+    def tuned_degrees(self):
+        """{'fp64': (finest, coarse), 'mixed': (finest, coarse)} once tuned."""
+        i = self.info()
+        out = {}
+        if i[8]:
+            out["fp64"] = (int(i[8]), int(i[9]))
+        if i[10]:
+            out["mixed"] = (int(i[10]), int(i[11]))
+        return out
 
     @property
     def handle(self):

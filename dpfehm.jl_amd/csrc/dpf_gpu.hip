@@ -187,8 +187,13 @@ struct dpf_topology {
   int seq = 0;                    // solve / residual sequence numbers
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
-  int cheb_degree = 2;
   double cheb_ratio = 30.0;
+  // Chebyshev degrees of the finest / coarser levels per precision (FP64,
+  // FP32), default 2/2 until tuned, and the tuner's record
+  int deg_fine[2] = {2, 2}, deg_coarse[2] = {2, 2};
+  bool tuned[2] = {false, false};
+  double tune_ms[2][5] = {};
+  int tune_iters[2][5] = {};
   // optional event timing of every finest-level smoothing sweep (opts->flags)
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -399,7 +404,8 @@ void harvest_timing(dpf_topology* t) {
 // buffer holding x_l on exit through *xout.
 template <typename T>
 void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* xl, T* xsl,
-            T** xout, const Cheb& ch, int* done, bool fuse_dot_at_fine) {
+            T** xout, const std::vector<Cheb>& chs, int* done, bool fuse_dot_at_fine) {
+  const Cheb& ch = chs[l];
   int L = t->levels();
   LevelDev& d = *t->lev[l];
   LevelNum<T>& n = h.lev[l];
@@ -462,11 +468,11 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg,
                      dn.A.nrows, (const T*)rl, nn.b.template as<T>(),
                      (const T*)level_invd(t, h, l + 1),
-                     nn.d.template as<T>(), nn.x.template as<T>(), (T)ch.c0,
+                     nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0,
                      coarse_next ? 0 : 1, (const int*)done);
   T* xc = nullptr;
   vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
-            nn.xs.template as<T>(), &xc, ch, done, false);
+            nn.xs.template as<T>(), &xc, chs, done, false);
   SellD<T> P = d.P.view<T>(n.pval.template as<T>());
   hipLaunchKernelGGL(k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur,
                      (const int*)done);
@@ -495,7 +501,8 @@ struct SolveResult {
 // convergence flag in chunks of iterations).
 template <typename T>
 SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double reltol,
-                double abstol, int maxiter, const Cheb& ch, int chunk_hint, int step) {
+                double abstol, int maxiter, const std::vector<Cheb>& chs) {
+  const Cheb& ch = chs[0];
   LevelDev& d0 = *t->lev[0];
   LevelNum<T>& n0 = h.lev[0];
   int64_t nf = t->H.nf;
@@ -517,11 +524,9 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
                      (T)ch.c0, part);
   finish(t, s, FIN_START, g, reltol * reltol, abstol * abstol, maxiter, seq);
   T* z = nullptr;
-  vcycle<T>(t, h, s, 0, r, z0, zs, &z, ch, done, true);
+  vcycle<T>(t, h, s, 0, r, z0, zs, &z, chs, done, true);
   int par = 0;
   SolveResult res{0, 0, 0, 0};
-  (void)chunk_hint;
-  (void)step;
   // Keep one iteration queued beyond the last one whose convergence test the
   // device has published: the host enqueues iteration k+1 while the GPU runs
   // iteration k's V-cycle, no launch outlives convergence, and the stream is
@@ -553,7 +558,7 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
                          (const SolveScalars*)sc, part);
       finish(t, s, FIN_XR, g);
       T* zz = nullptr;
-      vcycle<T>(t, h, s, 0, r, z0, zs, &zz, ch, done, true);
+      vcycle<T>(t, h, s, 0, r, z0, zs, &zz, chs, done, true);
       z = zz;
       launched++;
       HIPTRY(hipGetLastError());
@@ -591,7 +596,7 @@ void dpf_opts_init(dpf_opts* o) {
   o->precision = DPF_FP64;
   o->inner_reltol = 1e-5;
   o->max_outer = 20;
-  o->cheb_degree = 2;
+  o->cheb_degree = 0;
   o->memory = DPF_HOST;
   o->stream = nullptr;
 }
@@ -720,15 +725,19 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   const TopologyHost& H = t->H;
   int64_t nnz_all = 0;
   for (auto& l : H.levels) nnz_all += l.A.nnz;
-  int64_t v[8] = {H.n,
+  int64_t v[12] = {H.n,
                   H.ne,
                   H.nf,
                   H.levels[0].A.nnz,
                   (int64_t)H.levels.size(),
                   H.levels.back().A.nrows,
                   (int64_t)(1000.0 * (double)nnz_all / (double)H.levels[0].A.nnz),
-                  (int64_t)(H.setup_ms * 1000.0)};
-  for (int i = 0; i < ninfo && i < 8; i++) info[i] = v[i];
+                  (int64_t)(H.setup_ms * 1000.0),
+                  t->tuned[0] ? t->deg_fine[0] : 0,
+                  t->tuned[0] ? t->deg_coarse[0] : 0,
+                  t->tuned[1] ? t->deg_fine[1] : 0,
+                  t->tuned[1] ? t->deg_coarse[1] : 0};
+  for (int i = 0; i < ninfo && i < 12; i++) info[i] = v[i];
   return DPF_OK;
 }
 
@@ -764,9 +773,19 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
   t->sweep_ms = 0;
   t->sweep_n = 0;
-  int deg = o->cheb_degree > 0 ? o->cheb_degree : t->cheb_degree;
-  Cheb ch;
-  ch.make(deg, 1.0, t->cheb_ratio);
+  int pidx = o->precision == DPF_MIXED ? 1 : 0;
+  int L = t->levels();
+  auto make_chs = [&](int m0, int mc) {
+    std::vector<Cheb> v(L);
+    for (int l = 0; l < L; l++) v[l].make(l == 0 ? m0 : mc, 1.0, t->cheb_ratio);
+    return v;
+  };
+  // smoother degrees: explicit (every level), or tuned on this topology's
+  // first solve in this precision
+  bool tune = o->cheb_degree == 0 && !t->tuned[pidx];
+  std::vector<Cheb> chs = o->cheb_degree > 0
+                              ? make_chs(o->cheb_degree, o->cheb_degree)
+                              : make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
   double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   LevelDev& d0 = *t->lev[0];
@@ -816,15 +835,45 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       break;
     }
     SolveResult r;
-    if (o->precision == DPF_FP64) {
-      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, ch, 16, outer);
-      hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
-                         (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
-    } else {
+    if (o->precision == DPF_MIXED)
       hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(BLK), 0, s, nf,
                          (const double*)t->r64.as<double>(), t->b32.as<float>(),
                          (const SolveScalars*)sc);
-      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, ch, 8, outer);
+    if (tune) {
+      // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
+      // very system and keep the fastest; the result is fixed for the life
+      // of the topology, so its solves stay bitwise reproducible.
+      static const int cand[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
+      double best = 1e300;
+      for (int c = 0; c < 5; c++) {
+        std::vector<Cheb> cc = make_chs(cand[c][0], cand[c][1]);
+        HIPTRY(hipStreamSynchronize(s));
+        double ta = now_ms();
+        SolveResult tr = o->precision == DPF_FP64
+                             ? pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0,
+                                           budget, cc)
+                             : pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0,
+                                          budget, cc);
+        HIPTRY(hipStreamSynchronize(s));
+        double el = now_ms() - ta;
+        t->tune_ms[pidx][c] = tr.status == 1 ? el : -1.0;
+        t->tune_iters[pidx][c] = tr.iters;
+        if (tr.status == 1 && el < best) {
+          best = el;
+          t->deg_fine[pidx] = cand[c][0];
+          t->deg_coarse[pidx] = cand[c][1];
+        }
+      }
+      t->tuned[pidx] = true;
+      tune = false;
+      chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
+    }
+    if (o->precision == DPF_FP64) {
+      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, chs);
+      hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
+                         (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
+    } else {
+      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, chs);
       hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
                          (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc);
     }
@@ -1189,7 +1238,7 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
   SellD<T> A = d0.A.view<T>(level_val(t, h, 0));
   double w = (double)sizeof(T);
   Cheb ch;
-  ch.make(std::max(2, t->cheb_degree), 1.0, t->cheb_ratio);
+  ch.make(2, 1.0, t->cheb_ratio);
   hipEvent_t e0, e1;
   HIPTRY(hipEventCreate(&e0));
   HIPTRY(hipEventCreate(&e1));

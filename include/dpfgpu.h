@@ -48,7 +48,9 @@ typedef struct {
   int32_t precision;   /* DPF_FP64, or DPF_MIXED: FP32 AMG-PCG inside FP64 refinement */
   double inner_reltol; /* DPF_MIXED: relative tolerance of each FP32 inner solve */
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
-  int32_t cheb_degree; /* Chebyshev smoother degree per pre/post sweep (0: default 2) */
+  int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = autotuned:
+                          the first solve on a topology (per precision) times
+                          several (finest, coarse) degree pairs and keeps the fastest */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
   int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
   void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = topology stream */
@@ -72,7 +74,7 @@ typedef struct {
 const char *dpf_last_error(void);
 
 /* Defaults: reltol 1e-10, abstol 0, maxiter 2000, DPF_FP64, inner_reltol 1e-5,
- * max_outer 20, cheb_degree 2, DPF_HOST, stream NULL. */
+ * max_outer 20, cheb_degree 0 (autotuned), DPF_HOST, stream NULL, flags 0. */
 void dpf_opts_init(dpf_opts *opts);
 
 /* Topology cache.  Replaces the per-call work the reference repeats inside
@@ -88,8 +90,10 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t *neighbors,
                         const int64_t *dirichletnodes, dpf_topology **out);
 int dpf_topology_destroy(dpf_topology *topo);
 
-/* info[0..7] = n, nedges, nfree, nnz(A), levels, coarse rows, operator-complexity*1000,
- * host symbolic setup time in microseconds. */
+/* info[0..11] = n, nedges, nfree, nnz(A), levels, coarse rows,
+ * operator-complexity*1000, host symbolic setup time in microseconds, then the
+ * tuned Chebyshev degrees (finest, coarse) for FP64 and for mixed (0 = not
+ * tuned yet). */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,

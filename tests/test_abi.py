@@ -37,7 +37,7 @@ def test_opts_defaults(pkg):
     from dpfehm_jl_amd import _lib
     o = _lib.default_opts()
     assert o.reltol == 1e-10 and o.maxiter == 2000 and o.precision == _lib.DPF_FP64
-    assert o.inner_reltol == 1e-5 and o.cheb_degree == 2 and o.memory == _lib.DPF_HOST
+    assert o.inner_reltol == 1e-5 and o.cheb_degree == 0 and o.memory == _lib.DPF_HOST
     assert ctypes.sizeof(_lib.Opts) == 56
 
 
