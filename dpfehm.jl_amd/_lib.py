@@ -71,6 +71,8 @@ SIGNATURES = {
     "dpf_opts_init": (None, [ctypes.POINTER(Opts)]),
     "dpf_topology_create": (ctypes.c_int, [_i64, _i64, _p, _p, _i64, _p, ctypes.POINTER(_p)]),
     "dpf_topology_destroy": (ctypes.c_int, [_p]),
+    "dpf_batch_topology_create": (ctypes.c_int, [_i32, _p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
+    "dpf_member_stats": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_info": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),

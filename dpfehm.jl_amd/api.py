@@ -356,3 +356,116 @@ def rrule_groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes,
 
     pullback.state = state
     return uN, pullback
+
+
+class BatchTopology(Topology):
+    """Several meshes packed into one block-diagonal topology
+    (dpf_batch_topology_create).  ``members`` is a list of
+    ``(n, neighbors, areasoverlengths, dirichletnodes)`` with member-local
+    1-based indices.  The packed arrays (cells, edges, free cells) are the
+    members' arrays concatenated; ``cell0``/``edge0``/``free0`` give the
+    offsets."""
+
+    def __init__(self, members):  # noqa: D107 - Topology.__init__ is not reused
+        ns = np.array([int(m[0]) for m in members], dtype=np.int64)
+        nbs = [np.asarray(m[1], dtype=np.int64).reshape(-1, 2) for m in members]
+        es = np.array([nb.shape[0] for nb in nbs], dtype=np.int64)
+        aols = [np.asarray(m[2], dtype=np.float64)[:e] for m, e in zip(members, es)]
+        dns = [np.asarray(m[3], dtype=np.int64).ravel() for m in members]
+        nds = np.array([d.size for d in dns], dtype=np.int64)
+        nb = np.ascontiguousarray(np.concatenate(nbs))
+        aol = _f64(np.concatenate(aols))
+        dn = np.ascontiguousarray(np.concatenate(dns)) if nds.sum() else np.zeros(0, np.int64)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().dpf_batch_topology_create(
+            len(members), _ptr(ns), _ptr(es), _ptr(nb), _ptr(aol), _ptr(nds), _ptr(dn),
+            ctypes.byref(h)))
+        self._h = h
+        self.nmembers = len(members)
+        self.cell0 = np.concatenate([[0], np.cumsum(ns)])
+        self.edge0 = np.concatenate([[0], np.cumsum(es)])
+        self.n, self.ne = int(self.cell0[-1]), int(self.edge0[-1])
+        isfree = np.ones(self.n, dtype=bool)
+        isfree[(np.concatenate([d - 1 + c for d, c in zip(dns, self.cell0[:-1])])
+                if nds.sum() else np.zeros(0, np.int64))] = False
+        self.isfree = isfree
+        self.free0 = np.concatenate([[0], np.cumsum([isfree[a:b].sum() for a, b in
+                                                     zip(self.cell0[:-1], self.cell0[1:])])])
+        info = self.info()
+        self.nfree = int(info[2])
+        self.nnz = int(info[3])
+        self.levels = int(info[4])
+        self.coarse_rows = int(info[5])
+        self.operator_complexity = info[6] / 1000.0
+        self.symbolic_setup_ms = info[7] / 1000.0
+        self.lock = threading.Lock()
+
+    def member_relres(self):
+        out = np.empty(self.nmembers)
+        _lib.check(_lib.lib().dpf_member_stats(self._h, _ptr(out), self.nmembers))
+        return out
+
+
+def batch_steadystate(topo, Ks, dirichleths, Qs, stats=None, **kwargs):
+    """Steady solves of every member of a BatchTopology in one set of launches.
+    Ks, dirichleths, Qs: the members' arrays concatenated.  Returns h (packed)."""
+    o = _opts(kwargs)
+    Ks, dh, Qs = _f64(Ks), _f64(dirichleths), _f64(Qs)
+    if Ks.shape[0] != topo.ne or dh.shape[0] != topo.n or Qs.shape[0] != topo.n:
+        raise ValueError("packed arrays do not match the batch topology")
+    h = np.empty(topo.n)
+    st = _lib.Stats()
+    code = _lib.lib().dpf_steady_forward(topo.handle, _ptr(Ks), _ptr(dh), _ptr(Qs), _ptr(h),
+                                         ctypes.byref(o), ctypes.byref(st), None)
+    if stats is not None:
+        stats.update(st.as_dict())
+    _lib.check(code)
+    return h
+
+
+def rrule_batch_steadystate(topo, Ks, dirichleths, Qs, stats=None, **kwargs):
+    """Batched forward + pullback: pullback(delta_packed) -> (dKs, ddirichleths, dQs)."""
+    o = _opts(kwargs)
+    Ks, dh, Qs = _f64(Ks), _f64(dirichleths), _f64(Qs)
+    h = np.empty(topo.n)
+    st = _lib.Stats()
+    sh = ctypes.c_void_p()
+    code = _lib.lib().dpf_steady_forward(topo.handle, _ptr(Ks), _ptr(dh), _ptr(Qs), _ptr(h),
+                                         ctypes.byref(o), ctypes.byref(st), ctypes.byref(sh))
+    if stats is not None:
+        stats.update(st.as_dict())
+    _lib.check(code)
+    state = State(topo, sh)
+
+    def pullback(delta, stats=None):
+        delta = _f64(delta)
+        dK = np.empty(topo.ne)
+        ddh = np.empty(topo.n)
+        dQ = np.empty(topo.n)
+        st2 = _lib.Stats()
+        code = _lib.lib().dpf_steady_pullback(state._h, _ptr(delta), _ptr(dK), _ptr(ddh), _ptr(dQ),
+                                              ctypes.byref(o), ctypes.byref(st2))
+        if stats is not None:
+            stats.update(st2.as_dict())
+        _lib.check(code)
+        return dK, ddh, dQ
+
+    pullback.state = state
+    return h, pullback
+
+
+def hm_forward(topo, logK):
+    """K_e = hm(exp(logK_a), exp(logK_b)) on a Topology / BatchTopology."""
+    logK = _f64(logK)
+    Ke = np.empty(topo.ne)
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_hm_forward(topo.handle, _ptr(logK), _ptr(Ke), ctypes.byref(o)))
+    return Ke
+
+
+def hm_pullback(topo, logK, Ke, dKe):
+    out = np.empty(topo.n)
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_hm_pullback(topo.handle, _ptr(_f64(logK)), _ptr(_f64(Ke)),
+                                          _ptr(_f64(dKe)), _ptr(out), ctypes.byref(o)))
+    return out

@@ -180,7 +180,16 @@ struct dpf_topology {
   Hier<double> h64;
   Hier<float> h32;
   // reductions and solve scalars
-  DBuf partial, scal, zero_flag;
+  DBuf partial, scal, gdone, zero_flag;
+  // members of a batch (one for a single system): fine-row ranges and, per
+  // logical block of the fine grid, the block's first member and the row
+  // where the next member starts
+  int nm = 1;
+  std::vector<int64_t> m_cell0, m_row0, m_edge0;  // nm + 1 offsets each
+  DBuf bm, mblk;
+  double* hm_outer_host = nullptr;  // per-member refinement residual norms (pinned)
+  std::vector<double> m_relres;     // per-member relative residual of the last solve
+  double* hm_outer_dev = nullptr;
   SolveScalars* scal_host = nullptr;
   HostMirror* hm_host = nullptr;  // pinned coherent host memory the finish kernel writes
   HostMirror* hm_dev = nullptr;   // the same memory as seen by the device
@@ -345,11 +354,39 @@ auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
 }
 
 // Finishes the grid sum the previous kernel (nparts blocks) left in t->partial.
-void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol2 = 0,
-            double abstol2 = 0, int maxiter = 0, int seq = 0) {
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, (int)nparts,
-                     (const double*)t->partial.as<double>(), t->scal.as<SolveScalars>(), reltol2,
-                     abstol2, maxiter, seq, t->hm_dev);
+const int2* bm_ptr(dpf_topology* t) { return t->nm > 1 ? t->bm.as<int2>() : nullptr; }
+
+// Member maps of the fine grid (logical blocks of BLK rows).  A single system
+// is one member spanning every block.
+void setup_members(dpf_topology* t) {
+  int64_t nf = t->H.nf;
+  int64_t nblk = (nf + BLK - 1) / BLK;
+  if (t->m_row0.empty()) t->m_row0 = {0, nf};
+  int nm = t->nm;
+  std::vector<int> mblk(2 * nm);
+  std::vector<int2> bm(nblk);
+  for (int m = 0; m < nm; m++) {
+    mblk[2 * m] = (int)(t->m_row0[m] / BLK);
+    mblk[2 * m + 1] = (int)((t->m_row0[m + 1] - 1) / BLK);
+  }
+  int m = 0;
+  for (int64_t b = 0; b < nblk; b++) {
+    int64_t r0 = b * BLK;
+    while (m + 1 < nm && t->m_row0[m + 1] <= r0) m++;
+    int64_t nxt = m + 1 < nm ? t->m_row0[m + 1] : INT32_MAX;
+    bm[b] = make_int2(m, (int)std::min<int64_t>(nxt, INT32_MAX));
+  }
+  upload(t->mblk, mblk);
+  upload(t->bm, bm);
+}
+
+void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol = 0,
+            double abstol = 0, int maxiter = 0, int abs_target = 0, int seq = 0) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, t->nm, (int)nparts,
+                     (const double*)t->partial.as<double>(), bm_ptr(t),
+                     (const int*)t->mblk.as<int>(), t->scal.as<SolveScalars>(),
+                     t->gdone.as<int>(), reltol, abstol, maxiter, abs_target, seq, t->hm_dev,
+                     t->hm_outer_dev);
 }
 
 int mirror_load(const int* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
@@ -430,7 +467,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       // a one-level hierarchy: the CG scalar r.z still has to be formed
       if (fuse_dot_at_fine) {
         hipLaunchKernelGGL(k_rz_dot<T>, dim3(g), dim3(BLK), 0, s, d.A.nrows, bl, (const T*)cur,
-                           (const int*)done, part);
+                           (const int*)done, bm_ptr(t), part);
         finish(t, s, FIN_RZ, g);
       }
     } else {
@@ -441,7 +478,8 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
         if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
         hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd,
-                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, part);
+                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, bm_ptr(t),
+                           part);
         if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
@@ -454,7 +492,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   for (int k = 1; k < m; k++) {
     if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
-                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, part);
+                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, bm_ptr(t), part);
     if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     std::swap(cur, oth);
   }
@@ -482,7 +520,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
     if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     hipLaunchKernelGGL(cheb_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
-                       (T)c1, (T)c2, (const int*)done, fuse, part);
+                       (T)c1, (T)c2, (const int*)done, fuse, bm_ptr(t), part);
     if (t->timing && l == 0) HIPTRY(hipEventRecord(timing_event(t), s));
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
@@ -501,7 +539,7 @@ struct SolveResult {
 // convergence flag in chunks of iterations).
 template <typename T>
 SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double reltol,
-                double abstol, int maxiter, const std::vector<Cheb>& chs) {
+                double abstol, int maxiter, const std::vector<Cheb>& chs, int abs_target = 0) {
   const Cheb& ch = chs[0];
   LevelDev& d0 = *t->lev[0];
   LevelNum<T>& n0 = h.lev[0];
@@ -509,7 +547,7 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   unsigned g = nblocks(nf);
   double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
-  int* done = &sc->done;
+  int* done = t->gdone.as<int>();
   T* x = h.cx.template as<T>();
   T* r = h.cr.template as<T>();
   T* q = h.cq.template as<T>();
@@ -521,8 +559,8 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
   T* zs = n0.xs.template as<T>();
   int seq = ++t->seq;
   hipLaunchKernelGGL(k_cg_start<T>, dim3(g), dim3(BLK), 0, s, nf, b, r, x, invd, dd, z0,
-                     (T)ch.c0, part);
-  finish(t, s, FIN_START, g, reltol * reltol, abstol * abstol, maxiter, seq);
+                     (T)ch.c0, bm_ptr(t), part);
+  finish(t, s, FIN_START, g, reltol, abstol, maxiter, abs_target, seq);
   T* z = nullptr;
   vcycle<T>(t, h, s, 0, r, z0, zs, &z, chs, done, true);
   int par = 0;
@@ -550,12 +588,12 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
     if (launched < maxiter && launched <= its) {
       hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)z,
                          (const T*)pb[par], pb[par ^ 1], q, (const int*)done,
-                         (const SolveScalars*)sc, part);
+                         (const SolveScalars*)sc, bm_ptr(t), part);
       finish(t, s, FIN_PQ, g);
       par ^= 1;
       hipLaunchKernelGGL(k_cg_xr<T>, dim3(g), dim3(BLK), 0, s, nf, x, r, (const T*)pb[par],
                          (const T*)q, invd, dd, z0, (T)ch.c0, (const int*)done,
-                         (const SolveScalars*)sc, part);
+                         (const SolveScalars*)sc, bm_ptr(t), part);
       finish(t, s, FIN_XR, g);
       T* zz = nullptr;
       vcycle<T>(t, h, s, 0, r, z0, zs, &zz, chs, done, true);
@@ -601,18 +639,16 @@ void dpf_opts_init(dpf_opts* o) {
   o->stream = nullptr;
 }
 
-int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
-                        const double* areasoverlengths, int64_t ndirichlet,
-                        const int64_t* dirichletnodes, dpf_topology** out) {
-  if (!out) {
-    set_err("out is NULL");
-    return DPF_EINVAL;
-  }
-  *out = nullptr;
-  if ((nedges > 0 && (!neighbors || !areasoverlengths)) || (ndirichlet > 0 && !dirichletnodes)) {
-    set_err("NULL array argument");
-    return DPF_EINVAL;
-  }
+}  // extern "C"
+
+namespace {
+
+// Shared by dpf_topology_create and dpf_batch_topology_create.  members:
+// cell offsets of the packed members (empty for a single system).
+int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
+                const double* areasoverlengths, int64_t ndirichlet,
+                const int64_t* dirichletnodes, const std::vector<int64_t>& cell0,
+                dpf_topology** out) {
   auto t = std::make_unique<dpf_topology>();
   try {
     std::string e = build_topology(t->H, n, nedges, neighbors, areasoverlengths, ndirichlet,
@@ -620,6 +656,23 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
     if (!e.empty()) {
       set_err(e);
       return e.find("singular") != std::string::npos ? DPF_ESINGULAR : DPF_EINVAL;
+    }
+    if (!cell0.empty()) {
+      // members: free rows are numbered in cell order, so each member owns a
+      // contiguous range of them
+      t->nm = (int)cell0.size() - 1;
+      t->m_cell0 = cell0;
+      t->m_row0.assign(t->nm + 1, 0);
+      for (int m = 0; m < t->nm; m++) {
+        int64_t cnt = 0;
+        for (int64_t i = cell0[m]; i < cell0[m + 1]; i++) cnt += t->H.isfree[i];
+        t->m_row0[m + 1] = t->m_row0[m] + cnt;
+        if (cnt < BLK) {
+          set_err("batch member " + std::to_string(m + 1) + " has fewer than " +
+                  std::to_string(BLK) + " free cells");
+          return DPF_EINVAL;
+        }
+      }
     }
     e = build_hierarchy(t->H, 25, 128, 512);
     if (!e.empty()) {
@@ -675,9 +728,12 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
     }
     maxrows = std::max<int64_t>(maxrows, std::max(H.n, H.ne));
     t->max_grid = nblocks(maxrows) + 8;
-    t->partial.alloc(t->max_grid * sizeof(double));
-    t->scal.alloc(sizeof(SolveScalars));
-    HIPTRY(hipMemset(t->scal.p, 0, sizeof(SolveScalars)));
+    t->partial.alloc(2 * t->max_grid * sizeof(double));
+    setup_members(t.get());
+    t->scal.alloc(t->nm * sizeof(SolveScalars));
+    HIPTRY(hipMemset(t->scal.p, 0, t->nm * sizeof(SolveScalars)));
+    t->gdone.alloc(sizeof(int));
+    HIPTRY(hipMemset(t->gdone.p, 0, sizeof(int)));
     t->zero_flag.alloc(sizeof(int));
     HIPTRY(hipMemset(t->zero_flag.p, 0, sizeof(int)));
     HIPTRY(hipHostMalloc((void**)&t->scal_host, sizeof(SolveScalars), hipHostMallocDefault));
@@ -686,6 +742,10 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
     t->hm_host->seq = -1;
     t->hm_host->outer_seq = -1;
     HIPTRY(hipHostGetDevicePointer((void**)&t->hm_dev, t->hm_host, 0));
+    HIPTRY(hipHostMalloc((void**)&t->hm_outer_host, t->nm * sizeof(double),
+                         hipHostMallocCoherent));
+    std::memset(t->hm_outer_host, 0, t->nm * sizeof(double));
+    HIPTRY(hipHostGetDevicePointer((void**)&t->hm_outer_dev, t->hm_outer_host, 0));
     t->A64.alloc(t->lev[0]->A.entries * sizeof(double));
     t->invd64.alloc(H.nf * sizeof(double));
     t->b64.alloc(H.nf * sizeof(double));
@@ -704,6 +764,97 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
   return DPF_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int dpf_topology_create(int64_t n, int64_t nedges, const int64_t* neighbors,
+                        const double* areasoverlengths, int64_t ndirichlet,
+                        const int64_t* dirichletnodes, dpf_topology** out) {
+  if (!out) {
+    set_err("out is NULL");
+    return DPF_EINVAL;
+  }
+  *out = nullptr;
+  if ((nedges > 0 && (!neighbors || !areasoverlengths)) || (ndirichlet > 0 && !dirichletnodes)) {
+    set_err("NULL array argument");
+    return DPF_EINVAL;
+  }
+  return create_impl(n, nedges, neighbors, areasoverlengths, ndirichlet, dirichletnodes, {}, out);
+}
+
+int dpf_batch_topology_create(int32_t nmembers, const int64_t* ncells, const int64_t* nedges,
+                              const int64_t* neighbors, const double* areasoverlengths,
+                              const int64_t* ndirichlet, const int64_t* dirichletnodes,
+                              dpf_topology** out) {
+  if (!out) {
+    set_err("out is NULL");
+    return DPF_EINVAL;
+  }
+  *out = nullptr;
+  if (nmembers <= 0 || !ncells || !nedges || !ndirichlet) {
+    set_err("invalid member description");
+    return DPF_EINVAL;
+  }
+  try {
+    std::vector<int64_t> cell0(nmembers + 1, 0), edge0(nmembers + 1, 0), d0(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; m++) {
+      if (ncells[m] <= 0 || nedges[m] < 0 || ndirichlet[m] < 0) {
+        set_err("member " + std::to_string(m + 1) + " has a negative or empty size");
+        return DPF_EINVAL;
+      }
+      cell0[m + 1] = cell0[m] + ncells[m];
+      edge0[m + 1] = edge0[m] + nedges[m];
+      d0[m + 1] = d0[m] + ndirichlet[m];
+    }
+    if ((edge0[nmembers] > 0 && (!neighbors || !areasoverlengths)) ||
+        (d0[nmembers] > 0 && !dirichletnodes)) {
+      set_err("NULL array argument");
+      return DPF_EINVAL;
+    }
+    // pack: member-local 1-based indices -> global 1-based indices
+    std::vector<int64_t> nb(2 * edge0[nmembers]), dn(d0[nmembers]);
+    for (int m = 0; m < nmembers; m++) {
+      for (int64_t e = edge0[m]; e < edge0[m + 1]; e++)
+        for (int k = 0; k < 2; k++) {
+          int64_t v = neighbors[2 * e + k];
+          if (v < 1 || v > ncells[m]) {
+            set_err("member " + std::to_string(m + 1) + ": neighbors refer to a cell outside 1.." +
+                    std::to_string(ncells[m]));
+            return DPF_EINVAL;
+          }
+          nb[2 * e + k] = v + cell0[m];
+        }
+      for (int64_t k = d0[m]; k < d0[m + 1]; k++) {
+        int64_t v = dirichletnodes[k];
+        if (v < 1 || v > ncells[m]) {
+          set_err("member " + std::to_string(m + 1) + ": dirichletnodes outside 1.." +
+                  std::to_string(ncells[m]));
+          return DPF_EINVAL;
+        }
+        dn[k] = v + cell0[m];
+      }
+    }
+    int rc = create_impl(cell0[nmembers], edge0[nmembers], nb.data(), areasoverlengths,
+                         d0[nmembers], dn.data(), cell0, out);
+    if (rc == DPF_OK) (*out)->m_edge0 = edge0;
+    return rc;
+  } catch (const std::bad_alloc&) {
+    set_err("host allocation failed");
+    return DPF_ENOMEM;
+  }
+}
+
+int dpf_member_stats(const dpf_topology* t, double* relres, int32_t nmembers) {
+  if (!t || !relres || nmembers != t->nm) {
+    set_err("invalid argument (nmembers must equal the topology's member count)");
+    return DPF_EINVAL;
+  }
+  for (int m = 0; m < t->nm; m++)
+    relres[m] = m < (int)t->m_relres.size() ? t->m_relres[m] : -1.0;
+  return DPF_OK;
+}
+
 int dpf_topology_destroy(dpf_topology* t) {
   if (!t) return DPF_OK;
   {
@@ -711,6 +862,7 @@ int dpf_topology_destroy(dpf_topology* t) {
     if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
     if (t->scal_host) (void)hipHostFree(t->scal_host);
     if (t->hm_host) (void)hipHostFree(t->hm_host);
+    if (t->hm_outer_host) (void)hipHostFree(t->hm_outer_host);
     if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
   }
   delete t;
@@ -795,27 +947,43 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int total_it = 0, outer = 0;
   t->x64.alloc(nf * sizeof(double));
   t->r64.alloc(nf * sizeof(double));
-  auto residual = [&]() -> double {  // r64 = b - A x64, returns ||r64||
+  // r64 = b - A x64; every member's ||r64|| lands in the pinned mirror.
+  // first: also take it as the member's ||b|| and set its refinement target.
+  // Returns the number of members whose residual is above their target.
+  auto residual = [&](int first) -> int {
     hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
-                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), part);
+                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), bm_ptr(t),
+                       part);
     int oseq = ++t->seq;
-    finish(t, s, FIN_OUTER, g, 0, 0, 0, oseq);
+    finish(t, s, FIN_OUTER, g, o->reltol, o->abstol, 0, first, oseq);
     HIPTRY(hipGetLastError());
     wait_published(t, s, &t->hm_host->outer_seq, oseq);
-    return mirror_load(&t->hm_host->outer_s);
+    return mirror_load(&t->hm_host->active);
   };
-  // ||b||, then the residual of the initial guess
+  auto all_finite = [&]() {
+    for (int m = 0; m < t->nm; m++)
+      if (!std::isfinite(mirror_load(&t->hm_outer_host[m]))) return false;
+    return true;
+  };
   hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>());
-  double bnorm = residual();
-  double target = std::max(o->reltol * bnorm, o->abstol);
-  double rn = bnorm;
-  if (x0) {
+  int active = residual(1);
+  std::vector<double> bnorm(t->nm);
+  for (int m = 0; m < t->nm; m++) bnorm[m] = mirror_load(&t->hm_outer_host[m]);
+  bool finite = all_finite();
+  if (finite && x0) {
     copy_dev(t->x64.p, x0, nf * sizeof(double), s);
-    rn = residual();
+    active = residual(0);
+    finite = all_finite();
+  }
+  if (!finite) {
+    if (st) std::memset(st, 0, sizeof(*st));
+    throw Err{DPF_EINVAL, "non-finite values in the system (NaN or Inf in K, Q, dirichleths or "
+                          "the initial guess)"};
   }
   // Iterative refinement on x64: each step solves A d = r for the current FP64
-  // residual (FP64 CG in DPF_FP64 mode, one step to the full tolerance; FP32
-  // CG to inner_reltol in DPF_MIXED mode), then x64 += d.
+  // residual of every member still above its target (FP64 CG straight to the
+  // target in DPF_FP64 mode; FP32 CG to inner_reltol in DPF_MIXED mode), then
+  // x64 += d.
   int budget = o->maxiter;
   // FP64: one CG to the full tolerance; a restart only if rounding left the
   // true residual just above it
@@ -824,12 +992,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     ensure_hier(t, t->h32);
     t->b32.alloc(nf * sizeof(float));
   }
-  if (!std::isfinite(bnorm) || !std::isfinite(rn)) {
-    if (st) std::memset(st, 0, sizeof(*st));
-    throw Err{DPF_EINVAL, "non-finite values in the system (NaN or Inf in K, Q, dirichleths or "
-                          "the initial guess)"};
-  }
-  while (rn > target) {
+  while (active > 0) {
     if (outer >= max_outer || budget <= 0) {
       status = DPF_ENOCONV;
       break;
@@ -838,7 +1001,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     if (o->precision == DPF_MIXED)
       hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(BLK), 0, s, nf,
                          (const double*)t->r64.as<double>(), t->b32.as<float>(),
-                         (const SolveScalars*)sc);
+                         (const SolveScalars*)sc, bm_ptr(t));
     if (tune) {
       // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
       // very system and keep the fastest; the result is fixed for the life
@@ -850,10 +1013,10 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
         HIPTRY(hipStreamSynchronize(s));
         double ta = now_ms();
         SolveResult tr = o->precision == DPF_FP64
-                             ? pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0,
-                                           budget, cc)
+                             ? pcg<double>(t, t->h64, s, t->r64.as<double>(), 0.0, 0.0, budget,
+                                           cc, 1)
                              : pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0,
-                                          budget, cc);
+                                          budget, cc, 0);
         HIPTRY(hipStreamSynchronize(s));
         double el = now_ms() - ta;
         t->tune_ms[pidx][c] = tr.status == 1 ? el : -1.0;
@@ -869,13 +1032,13 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
     }
     if (o->precision == DPF_FP64) {
-      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), target / rn, 0.0, budget, chs);
+      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), 0.0, 0.0, budget, chs, 1);
       hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
                          (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
     } else {
-      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, chs);
+      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, chs, 0);
       hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
-                         (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc);
+                         (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc, bm_ptr(t));
     }
     total_it += r.iters;
     budget -= r.iters;
@@ -884,16 +1047,21 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       status = DPF_EBREAKDOWN;
       break;
     }
-    double rn_new = residual();
-    if (!(rn_new == rn_new)) {
+    active = residual(0);
+    if (!all_finite()) {
       status = DPF_EBREAKDOWN;
-      rn = rn_new;
       break;
     }
-    rn = rn_new;
   }
-  if (!std::isfinite(rn)) status = DPF_EBREAKDOWN;
-  double relres = bnorm > 0 ? rn / bnorm : rn;
+  double relres = 0.0;
+  t->m_relres.resize(t->nm);
+  for (int m = 0; m < t->nm; m++) {
+    double rm = mirror_load(&t->hm_outer_host[m]);
+    t->m_relres[m] = bnorm[m] > 0 ? rm / bnorm[m] : rm;
+    relres = std::max(relres, t->m_relres[m]);
+  }
+  if (!(relres == relres)) relres = INFINITY;
+
   copy_dev(xout, t->x64.p, nf * sizeof(double), s);
   if (st) {
     st->iterations = total_it;
@@ -1248,11 +1416,12 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
     if (kernel == 0) {
       hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, 0, nf, (const T*)xa, xb,
                          (const T*)h.cr.template as<T>(), n0.d.template as<T>(),
-                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0, part);
+                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0,
+                         bm_ptr(t), part);
     } else if (kernel == 1) {
       hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
-                         h.cq.template as<T>(), nodone, (const SolveScalars*)sc, part);
+                         h.cq.template as<T>(), nodone, (const SolveScalars*)sc, bm_ptr(t), part);
     } else {
       hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
                          d0.A.view<double>(t->A64.as<double>()), 0, nf,

@@ -22,17 +22,17 @@ namespace dpf {
 constexpr int BLK = 256;
 constexpr uint16_t NO_SLOT_D = 0xFFFF;
 
-// Device-side scalars of one Krylov solve (all FP64, also in mixed mode).
 // Host-visible mirror of the solve's progress, in pinned coherent host
 // memory: the finish kernel writes it with system-scope stores, so the host
 // follows convergence without synchronising the stream or copying.
 struct HostMirror {
   int seq;        // solve sequence number of the fields below
-  int iters;
-  int done;
-  int outer_seq;  // sequence number of outer_s
+  int iters;      // largest iteration count over the members
+  int done;       // 0 while any member runs; else 3 if any broke down, 2 if any hit the cap, 1
+  int outer_seq;  // sequence number of the refinement residuals
+  int active;     // members whose refinement residual is above their target
+  int pad;
   double rr;
-  double outer_s;
 };
 
 __device__ __forceinline__ void mput(int* p, int v) {
@@ -45,18 +45,20 @@ __device__ __forceinline__ void mpublish(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Device-side scalars of one Krylov solve, one record per member of a batch
+// (a single system is a batch of one).  All FP64, also in mixed mode.
 struct SolveScalars {
   double rr;       // current ||r||^2
   double rr_stop;  // stop when rr <= rr_stop
   double rz, rz_old;
   double pq, alpha;
-  double outer_rr;  // refinement: ||b - A x||^2 in FP64
-  double outer_s;   // refinement: scale of the inner right-hand side
+  double outer_s;  // refinement: ||b - A x|| in FP64
+  double bnorm;    // refinement: ||b||
+  double target;   // refinement: stop when outer_s <= target
   int iters, maxiter;
   int done;  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
   int first;
 };
-
 
 template <typename T>
 struct SellD {
@@ -93,99 +95,169 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
-// Block partial of a grid-wide sum: thread 0 of block b writes partial[b].
-// The sum over blocks is finished by k_finish in the next launch on the
+// Batched solves pack the members' fine rows contiguously, each member with at
+// least BLK rows, so a logical block of the fine grid holds rows of at most
+// two members.  bm[lb] = {first member of block lb, first row of the next
+// member (or INT32_MAX)}; bm == nullptr for a single system.
+__device__ __forceinline__ int member_of(const int2* __restrict__ bm, int64_t row, bool& second) {
+  if (!bm) {
+    second = false;
+    return 0;
+  }
+  int2 e = bm[row / BLK];
+  second = row >= e.y;
+  return second ? e.x + 1 : e.x;
+}
+
+// Block partials of a grid-wide sum, two per logical block: slot 0 for the
+// block's first member, slot 1 for its second (single systems use slot 0).
+// The sums over blocks are finished by k_finish in the next launch on the
 // stream; the kernel boundary makes the partials visible without any
 // device-scope fence or atomic (on a multi-XCD part a release fence per block
 // writes back the XCD's L2 and costs far more than the kernel's own traffic).
-__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial) {
+__device__ __forceinline__ void block_partial2(double v, bool second,
+                                               double* __restrict__ partial) {
   __shared__ double sh[4];
-  double bs = block_sum(v, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  double a = block_sum(second ? 0.0 : v, sh);
+  double b = block_sum(second ? v : 0.0, sh);
+  if (threadIdx.x == 0) {
+    int64_t lb = logical_row() / BLK;
+    partial[2 * lb] = a;
+    partial[2 * lb + 1] = b;
+  }
 }
 
 enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER = 4 };
 
-// Sums the block partials of the previous kernel in a fixed order (one
-// workgroup) and applies the scalar step of the Krylov solve:
-//   FIN_START  rr, stop threshold, counters;   FIN_PQ  alpha = rz / p.q
-//   FIN_XR     rr, iteration count, convergence / cap / breakdown
-//   FIN_RZ     rz_old = rz, rz = r.z;           FIN_OUTER  FP64 residual norm
-__global__ void __launch_bounds__(1024) k_finish(int mode, int nparts,
-                                                 const double* __restrict__ partial,
-                                                 SolveScalars* sc, double reltol2,
-                                                 double abstol2, int maxiter, int seq,
-                                                 HostMirror* hm) {
-  if (mode != FIN_START && mode != FIN_OUTER && sc->done) return;
-  __shared__ double sh[16];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  int nw = (int)(blockDim.x >> 6);
-  double tot = 0.0;
-  for (int k = 0; k < nw; k++) tot += sh[k];
+// Per member: the scalar step of the Krylov solve after the block partials.
+//   FIN_START  rr, stop threshold (relative, or the member's absolute
+//              refinement target when abs_target), counters
+//   FIN_PQ     alpha = rz / p.q          FIN_XR  rr, iteration count, convergence
+//   FIN_RZ     rz_old = rz, rz = r.z     FIN_OUTER  FP64 residual norm (and, when
+//              abs_target, the member's ||b|| and target: the first residual)
+__device__ void finish_member(int mode, double tot, SolveScalars& m, double reltol,
+                              double abstol, int maxiter, int abs_target) {
   switch (mode) {
     case FIN_START: {
-      sc->rr = tot;
-      double stop = reltol2 * tot;
-      sc->rr_stop = stop > abstol2 ? stop : abstol2;
-      sc->iters = 0;
-      sc->maxiter = maxiter;
-      sc->first = 1;
-      sc->rz = 0.0;
-      sc->rz_old = 0.0;
-      int dn = (tot <= sc->rr_stop) ? 1 : (tot == tot ? 0 : 3);
-      sc->done = dn;
-      mput(&hm->rr, tot);
-      mput(&hm->iters, 0);
-      mput(&hm->done, dn);
-      mpublish(&hm->seq, seq);
+      m.rr = tot;
+      double stop;
+      if (abs_target) {
+        stop = m.target * m.target;
+      } else {
+        stop = reltol * reltol * tot;
+        if (stop < abstol * abstol) stop = abstol * abstol;
+      }
+      m.rr_stop = stop;
+      m.iters = 0;
+      m.maxiter = maxiter;
+      m.first = 1;
+      m.rz = 0.0;
+      m.rz_old = 0.0;
+      m.done = (tot <= stop) ? 1 : (tot == tot ? 0 : 3);
       break;
     }
     case FIN_PQ:
-      sc->pq = tot;
-      sc->first = 0;
-      if (!(tot > 0.0)) {
-        sc->done = 3;
-        mput(&hm->done, 3);
-      } else {
-        sc->alpha = sc->rz / tot;
-      }
+      if (m.done) break;
+      m.pq = tot;
+      m.first = 0;
+      if (!(tot > 0.0))
+        m.done = 3;
+      else
+        m.alpha = m.rz / tot;
       break;
     case FIN_XR: {
-      sc->rr = tot;
-      int it = sc->iters + 1;
-      sc->iters = it;
-      int dn = 0;
-      if (tot <= sc->rr_stop)
-        dn = 1;
+      if (m.done) break;
+      m.rr = tot;
+      int it = m.iters + 1;
+      m.iters = it;
+      if (tot <= m.rr_stop)
+        m.done = 1;
       else if (!(tot == tot))
-        dn = 3;
-      else if (it >= sc->maxiter)
-        dn = 2;
-      sc->done = dn;
-      mput(&hm->rr, tot);
-      mput(&hm->done, dn);
-      mput(&hm->iters, it);
+        m.done = 3;
+      else if (it >= m.maxiter)
+        m.done = 2;
       break;
     }
     case FIN_RZ:
-      sc->rz_old = sc->rz;
-      sc->rz = tot;
-      if (!(tot > 0.0)) {
-        sc->done = 3;
-        mput(&hm->done, 3);
-      }
+      if (m.done) break;
+      m.rz_old = m.rz;
+      m.rz = tot;
+      if (!(tot > 0.0)) m.done = 3;
       break;
-    default:
-      sc->outer_rr = tot;
-      sc->outer_s = sqrt(tot);
-      mput(&hm->outer_s, sqrt(tot));
-      mpublish(&hm->outer_seq, seq);
+    default: {
+      double sn = sqrt(tot);
+      m.outer_s = sn;
+      if (abs_target) {
+        m.bnorm = sn;
+        double tg = reltol * sn;
+        m.target = tg > abstol ? tg : abstol;
+      }
+    }
   }
+}
+
+// Sums the block partials of the previous kernel per member, in a fixed
+// order, applies finish_member, then folds the members' states into the
+// global early-exit flag *gdone and the host mirror.  One workgroup; with
+// several members each wave takes members m = wave, wave + 16, ...
+__global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
+                                                 const double* __restrict__ partial,
+                                                 const int2* __restrict__ bm,
+                                                 const int* __restrict__ mblk,
+                                                 SolveScalars* sc, int* gdone, double reltol,
+                                                 double abstol, int maxiter, int abs_target,
+                                                 int seq, HostMirror* hm,
+                                                 double* __restrict__ hm_outer) {
+  if (mode != FIN_START && mode != FIN_OUTER && *gdone) return;
+  __shared__ double sh[16];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (nm == 1) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[2 * b];
+    s = wave_sum(s);
+    if (lane == 0) sh[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot += sh[k];
+      finish_member(mode, tot, sc[0], reltol, abstol, maxiter, abs_target);
+    }
+  } else {
+    for (int m = wave; m < nm; m += (int)(blockDim.x >> 6)) {
+      double s = 0.0;
+      for (int b = mblk[2 * m] + lane; b <= mblk[2 * m + 1]; b += 64)
+        s += partial[2 * b + (bm[b].x == m ? 0 : 1)];
+      s = wave_sum(s);
+      if (lane == 0) finish_member(mode, s, sc[m], reltol, abstol, maxiter, abs_target);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int running = 0, capped = 0, broke = 0, maxit = 0, active = 0;
+  double rr = 0.0;
+  for (int m = 0; m < nm; m++) {
+    int d = sc[m].done;
+    running |= (d == 0);
+    capped |= (d == 2);
+    broke |= (d == 3);
+    if (sc[m].iters > maxit) maxit = sc[m].iters;
+    if (sc[m].rr > rr) rr = sc[m].rr;
+    if (mode == FIN_OUTER) {
+      mput(&hm_outer[m], sc[m].outer_s);
+      active += sc[m].outer_s > sc[m].target ? 1 : 0;
+    }
+  }
+  if (mode == FIN_OUTER) {
+    mput(&hm->active, active);
+    mpublish(&hm->outer_seq, seq);
+    return;
+  }
+  int dn = running ? 0 : (broke ? 3 : (capped ? 2 : 1));
+  *gdone = dn;
+  mput(&hm->rr, rr);
+  mput(&hm->iters, maxit);
+  mput(&hm->done, dn);
+  if (mode == FIN_START) mpublish(&hm->seq, seq);
 }
 
 template <typename T>
@@ -454,7 +526,7 @@ template <typename T, bool FINE>
 __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
                        T* __restrict__ xout, const T* __restrict__ b, T* __restrict__ d,
                        const T* __restrict__ invd, T c1, T c2, const int* __restrict__ done,
-                       int fuse_dot, double* __restrict__ partial) {
+                       int fuse_dot, const int2* __restrict__ bm, double* __restrict__ partial) {
   if (*done) return;
   int64_t sub = logical_row();
   T ax = split_row_dot(A, sub, lg, xin);
@@ -468,7 +540,11 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
     xout[r] = xo;
     local = (double)br * (double)xo;
   }
-  if (fuse_dot) block_partial(local, partial);
+  if (fuse_dot) {  // finest level only (lg == 0): row == sub
+    bool second;
+    member_of(bm, sub, second);
+    block_partial2(local, second, partial);
+  }
 }
 
 // r = b - A x  (rows of A split over 2^lg lanes)
@@ -554,22 +630,31 @@ __global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __
 // smoothing sweep to fuse it into).
 template <typename T>
 __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict__ z,
-                         const int* __restrict__ done, double* __restrict__ partial) {
+                         const int* __restrict__ done, const int2* __restrict__ bm,
+                         double* __restrict__ partial) {
   if (*done) return;
   int64_t i = logical_row();
+  bool second;
+  member_of(bm, i, second);
   double local = i < n ? (double)b[i] * (double)z[i] : 0.0;
-  block_partial(local, partial);
+  block_partial2(local, second, partial);
 }
 
 // ------------------------------------------------------------------ CG
+// Every vector kernel reads the scalars of its row's member; a member whose
+// solve has finished is frozen (its x and r no longer change) while the
+// others iterate.
 
-// x = 0, r = b, rr = b.b, stop threshold; first smoothing step of the
-// V-cycle on r (d0 = z0 = c0 invd r).
+// x = 0, r = b, rr = b.b (partials); first smoothing step of the V-cycle on r
+// (d0 = z0 = c0 invd r).
 template <typename T>
 __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r,
                            T* __restrict__ x, const T* __restrict__ invd, T* __restrict__ d0,
-                           T* __restrict__ z0, T c0, double* __restrict__ partial) {
+                           T* __restrict__ z0, T c0, const int2* __restrict__ bm,
+                           double* __restrict__ partial) {
   int64_t i = logical_row();
+  bool second;
+  member_of(bm, i, second);
   double local = 0.0;
   if (i < n) {
     T bv = b[i];
@@ -580,21 +665,26 @@ __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r
     z0[i] = dd;
     local = (double)bv * (double)bv;
   }
-  block_partial(local, partial);
+  block_partial2(local, second, partial);
 }
 
 // p_new = z + beta p_old,  q = A p_new (formed from z and p_old at the
-// neighbours, so p is never read while it is rewritten),  alpha = rz / p.q.
+// neighbours, so p is never read while it is rewritten), partials of p.q.
+// Neighbours belong to the same member (the operator is block diagonal).
 template <typename T>
 __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict__ pold,
                         T* __restrict__ pnew, T* __restrict__ q, const int* __restrict__ done,
-                        const SolveScalars* __restrict__ sc, double* __restrict__ partial) {
+                        const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
+                        double* __restrict__ partial) {
   if (*done) return;
-  int first = sc->first;
-  T beta = first ? (T)0 : (T)(sc->rz / sc->rz_old);
   int64_t i = logical_row();
+  bool second;
+  int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < A.nrows) {
+    const SolveScalars& m = sc[mi];
+    int first = m.first;
+    T beta = first ? (T)0 : (T)(m.rz / m.rz_old);
     int64_t base;
     int w;
     sell_row(A, i, base, w);
@@ -617,40 +707,47 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
     q[i] = s;
     local = (double)pn * (double)s;
   }
-  block_partial(local, partial);
+  block_partial2(local, second, partial);
 }
 
-// x += alpha p, r -= alpha q, rr = r.r, convergence test; first smoothing
-// step of the next V-cycle on r.
+// x += alpha p, r -= alpha q (members still running), partials of r.r; first
+// smoothing step of the next V-cycle on r.
 template <typename T>
 __global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
                         const T* __restrict__ p, const T* __restrict__ q,
                         const T* __restrict__ invd, T* __restrict__ d0, T* __restrict__ z0, T c0,
                         const int* __restrict__ done, const SolveScalars* __restrict__ sc,
-                        double* __restrict__ partial) {
+                        const int2* __restrict__ bm, double* __restrict__ partial) {
   if (*done) return;
-  T alpha = (T)sc->alpha;
   int64_t i = logical_row();
+  bool second;
+  int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < n) {
-    x[i] += alpha * p[i];
-    T rv = r[i] - alpha * q[i];
-    r[i] = rv;
+    T rv = r[i];
+    if (!sc[mi].done) {
+      T alpha = (T)sc[mi].alpha;
+      x[i] += alpha * p[i];
+      rv = rv - alpha * q[i];
+      r[i] = rv;
+    }
     T dd = c0 * invd[i] * rv;
     d0[i] = dd;
     z0[i] = dd;
     local = (double)rv * (double)rv;
   }
-  block_partial(local, partial);
+  block_partial2(local, second, partial);
 }
 
 // ------------------------------------------------------------------ FP64 outer residual / refinement
 
-// r = b - A x in FP64 with ||r||^2 -> sc->outer_rr, sqrt -> sc->outer_s.
+// r = b - A x in FP64, partials of ||r||^2 per member.
 __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
                                  const double* __restrict__ b, double* __restrict__ r,
-                                 double* __restrict__ partial) {
+                                 const int2* __restrict__ bm, double* __restrict__ partial) {
   int64_t i = logical_row();
+  bool second;
+  member_of(bm, i, second);
   double local = 0.0;
   if (i < A.nrows) {
     int64_t base;
@@ -665,24 +762,30 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
     r[i] = rv;
     local = rv * rv;
   }
-  block_partial(local, partial);
+  block_partial2(local, second, partial);
 }
 
-// inner right-hand side in FP32: b32 = r64 / s
+// inner right-hand side in FP32: b32 = r64 / s for members above their
+// refinement target, 0 for the others (whose inner solve then ends at once)
 __global__ void k_scale_to_f32(int64_t n, const double* __restrict__ r, float* __restrict__ b32,
-                               const SolveScalars* __restrict__ sc) {
+                               const SolveScalars* __restrict__ sc,
+                               const int2* __restrict__ bm) {
   int64_t i = logical_row();
   if (i >= n) return;
-  double s = sc->outer_s;
-  b32[i] = (float)(r[i] / (s > 0.0 ? s : 1.0));
+  bool second;
+  const SolveScalars& m = sc[member_of(bm, i, second)];
+  double s = m.outer_s;
+  b32[i] = (s > m.target && s > 0.0) ? (float)(r[i] / s) : 0.0f;
 }
 
-// x64 += s * x32
+// x64 += s * x32 for members above their refinement target
 __global__ void k_refine(int64_t n, double* __restrict__ x, const float* __restrict__ d,
-                         const SolveScalars* __restrict__ sc) {
+                         const SolveScalars* __restrict__ sc, const int2* __restrict__ bm) {
   int64_t i = logical_row();
   if (i >= n) return;
-  x[i] += sc->outer_s * (double)d[i];
+  bool second;
+  const SolveScalars& m = sc[member_of(bm, i, second)];
+  if (m.outer_s > m.target) x[i] += m.outer_s * (double)d[i];
 }
 
 // y += a x

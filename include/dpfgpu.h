@@ -90,6 +90,26 @@ int dpf_topology_create(int64_t n, int64_t nedges, const int64_t *neighbors,
                         const int64_t *dirichletnodes, dpf_topology **out);
 int dpf_topology_destroy(dpf_topology *topo);
 
+/* Batched ensembles (no counterpart in the reference, whose ensembles are
+ * independent calls, e.g. pmap in examples/scott_example/ex.jl:10): the
+ * members' meshes packed into one block-diagonal topology.  Member m has
+ * ncells[m] cells, nedges[m] edges and ndirichlet[m] Dirichlet cells; the
+ * neighbors (2 per edge), areasoverlengths and dirichletnodes arrays hold the
+ * members one after another, with 1-based indices local to the member.
+ * Every other entry point then works on the packed arrays (cells, edges and
+ * free cells concatenated member by member) in one set of launches, while
+ * each member keeps its own CG scalars and its own convergence test against
+ * reltol.  Every member needs at least 256 free cells. */
+int dpf_batch_topology_create(int32_t nmembers, const int64_t *ncells, const int64_t *nedges,
+                              const int64_t *neighbors, const double *areasoverlengths,
+                              const int64_t *ndirichlet, const int64_t *dirichletnodes,
+                              dpf_topology **out);
+
+/* Relative FP64 residual ||b - A h|| / ||b|| of every member after the last
+ * solve on the topology (nmembers must equal its member count; 1 for a
+ * single system). */
+int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers);
+
 /* info[0..11] = n, nedges, nfree, nnz(A), levels, coarse rows,
  * operator-complexity*1000, host symbolic setup time in microseconds, then the
  * tuned Chebyshev degrees (finest, coarse) for FP64 and for mixed (0 = not

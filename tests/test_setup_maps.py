@@ -223,3 +223,29 @@ def test_topology_rejects_bad_input(pkg):
     assert L.dpf_inspect_create(3, 1, nb.ctypes.data, aol.ctypes.data, 1, dn.ctypes.data, 25, 16,
                                 ctypes.byref(h)) != 0
     assert b"singular" in L.dpf_inspect_last_error()
+
+
+def test_batch_packing_keeps_members_apart(pkg):
+    """The packed hierarchy never aggregates rows of different members."""
+    W = pkg.workloads
+    ms = [W.box_problem("a", (9, 8, 6), (0, 0, 0), (4.0, 3.5, 2.5), lam=3.0, sigma=1.0,
+                        field_seed=1, dirichlet="top", sources=(), n_injectors=2, n_obs=4),
+          W.box_problem("b", (7, 7, 7), (0, 0, 0), (3.0, 3.0, 3.0), lam=3.0, sigma=1.0,
+                        field_seed=2, dirichlet="last", sources=((1, 1e-4),), n_obs=4)]
+    c0 = np.cumsum([0] + [w.n for w in ms])
+    nb = np.concatenate([w.neighbors + c for w, c in zip(ms, c0)])
+    aol = np.concatenate([w.areasoverlengths for w in ms])
+    dn = np.concatenate([w.dirichletnodes + c for w, c in zip(ms, c0)])
+    I = Inspect(int(c0[-1]), nb, aol, dn, coarse_max=8)
+    try:
+        f2n = I.get("f2n")
+        member = np.searchsorted(c0, f2n, side="right") - 1
+        for l in range(I.levels - 1):
+            agg = I.get("agg", l)
+            owner = -np.ones(agg.max() + 1, dtype=np.int64)
+            for r, a in enumerate(agg):
+                assert owner[a] in (-1, member[r])
+                owner[a] = member[r]
+            member = owner
+    finally:
+        I.close()
