@@ -42,6 +42,12 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient"],
+                    default="steady2m",
+                    help="steady2m (default, the headline line); ensemble: 96 packed "
+                         "realizations on 32x64^3 + 32x96x96x24 + 32x128x128x16 grids; "
+                         "transient: 200 backward-Euler steps on the 2M grid with the "
+                         "checkpointed adjoint")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -55,6 +61,11 @@ def main():
     from __graft_entry__ import load_package
     P = load_package()
     from dpfehm_jl_amd import _lib, api
+
+    if args.workload == "ensemble":
+        return bench_ensemble(args, P, api, _lib, torch)
+    if args.workload == "transient":
+        return bench_transient(args, P, api, _lib, torch)
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -258,6 +269,176 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def ensemble_members(P):
+    """96 realizations: 32 each of 64^3, 96x96x24 and 128x128x16 cells at 0.5 m
+    node spacing, logK = -9 + layered Matern(nu=1) field with sigma 1.5 and a
+    correlation length drawn from U(20, 80) m (seeds 100..195), top-face
+    Dirichlet h = 0, 4 injectors of 1e-4 m^3/s and 50 observation cells per
+    member (the 2M-cell pattern scaled)."""
+    shapes = [(64, 64, 64)] * 32 + [(96, 96, 24)] * 32 + [(128, 128, 16)] * 32
+    grids = {}
+    out = []
+    for m, ns in enumerate(shapes):
+        seed = 100 + m
+        lam = float(np.random.Generator(np.random.Philox(seed)).uniform(20.0, 80.0))
+        maxs = tuple(0.5 * (k - 1) for k in ns)
+        if ns not in grids:
+            grids[ns] = P.grid.regulargrid3d((0.0, 0.0, 0.0), maxs, ns)
+        coords, nb, aol, vol = grids[ns]
+        n = coords.shape[1]
+        logK = P.fields.layered_logK(ns, (0.5, 0.5, 0.5), -9.0, lam, 1.5, seed)
+        dn = np.arange(ns[2], n + 1, ns[2], dtype=np.int64)
+        free = np.ones(n, dtype=bool)
+        free[dn - 1] = False
+        fc = np.nonzero(free)[0] + 1
+        rng = np.random.Generator(np.random.Philox(seed + 1000))
+        inj = rng.choice(fc, 4, replace=False)
+        obs = np.sort(rng.choice(fc, 50, replace=False))
+        Q = np.zeros(n)
+        Q[inj - 1] = 1e-4
+        out.append(dict(n=n, nb=nb, aol=aol, dn=dn, logK=logK, Q=Q, obs=obs))
+    return out
+
+
+def bench_ensemble(args, P, api, _lib, torch):
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    ms = ensemble_members(P)
+    log(f"[bench] ensemble: {len(ms)} members, {sum(m['n'] for m in ms)} cells, "
+        f"generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    topo = api.BatchTopology([(m["n"], m["nb"], m["aol"], m["dn"]) for m in ms])
+    topo_s = time.time() - t0
+    log(f"[bench] batch topology: nfree={topo.nfree} nnz={topo.nnz} levels={topo.levels} "
+        f"coarse={topo.coarse_rows} host symbolic {topo.symbolic_setup_ms:.0f} ms, create {topo_s:.1f}s")
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    logK = torch.from_numpy(np.concatenate([m["logK"] for m in ms])).to(dev)
+    dh = torch.zeros(topo.n, dtype=torch.float64, device=dev)
+    Q = torch.from_numpy(np.concatenate([m["Q"] for m in ms])).to(dev)
+    obs = torch.from_numpy(np.concatenate([m["obs"] - 1 + c for m, c in
+                                           zip(ms, topo.cell0[:-1])])).to(dev)
+    Ke = torch.empty(topo.ne, dtype=torch.float64, device=dev)
+    h = torch.empty(topo.n, dtype=torch.float64, device=dev)
+    delta = torch.zeros(topo.n, dtype=torch.float64, device=dev)
+    dK = torch.empty(topo.ne, dtype=torch.float64, device=dev)
+    dlogK = torch.empty(topo.n, dtype=torch.float64, device=dev)
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def evaluate(o):
+        _lib.check(L.dpf_hm_forward(topo.handle, p(logK), p(Ke), ctypes.byref(o)))
+        st, sh = _lib.Stats(), ctypes.c_void_p()
+        _lib.check(L.dpf_steady_forward(topo.handle, p(Ke), p(dh), p(Q), p(h), ctypes.byref(o),
+                                        ctypes.byref(st), ctypes.byref(sh)))
+        delta.zero_()
+        delta[obs] = 2.0 * h[obs]
+        st2 = _lib.Stats()
+        code = L.dpf_steady_pullback(sh, p(delta), p(dK), None, None, ctypes.byref(o),
+                                     ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        _lib.check(L.dpf_hm_pullback(topo.handle, p(logK), p(Ke), p(dK), p(dlogK),
+                                     ctypes.byref(o)))
+        return st.as_dict(), st2.as_dict()
+
+    res = {}
+    for precision in ("fp64", "mixed"):
+        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
+                      stream=sptr)
+        for _ in range(args.warmup):
+            evaluate(o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [evaluate(o) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        rr = topo.member_relres()
+        res[precision] = (dt, stats, float(rr.max()))
+        log(f"[bench] ensemble {precision}: {args.steps / dt:.2f} batch evals/s = "
+            f"{len(ms) * args.steps / dt:.1f} member forward+adjoint evals/s; iterations "
+            f"{stats[-1][0]['iterations']}/{stats[-1][1]['iterations']}, worst member relres {rr.max():.2e}")
+    value = len(ms) * args.steps / res["mixed"][0]
+    out = {
+        "metric": "member forward+adjoint evals/s, 96-realization packed ensemble, mixed",
+        "value": round(value, 2), "unit": "member evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * res["mixed"][0] / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+        "config": {"workload": "96 packed realizations: 32x64^3 + 32x96x96x24 + 32x128x128x16 "
+                               "cells, sigma 1.5, correlation length U(20,80) m, "
+                               "forward+adjoint to 1e-10 per member",
+                   "cells": topo.n, "nfree": topo.nfree, "amg_levels": topo.levels},
+        "fp64_member_evals_per_s": round(len(ms) * args.steps / res["fp64"][0], 2),
+        "worst_member_relres": {k: v[2] for k, v in res.items()},
+        "topology_setup_s": round(topo_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_transient(args, P, api, _lib, torch):
+    """200 backward-Euler steps of 3600 s on the 2M-cell grid (Ss = 1e-5 1/m,
+    injection ramp over the first 50 steps), objective on the final heads at
+    the 200 observation cells, checkpointed adjoint (every 20 steps)."""
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    w = P.workloads.steady_2m()
+    topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    nsteps, dt = 200, 3600.0
+    qs = torch.from_numpy(P.workloads.transient_schedule(nsteps, dt, 50)).to(dev)
+    K = torch.from_numpy(w.Ks()).to(dev)
+    dh = torch.from_numpy(w.dirichleths).to(dev)
+    Q = torch.from_numpy(w.Qs).to(dev)
+    Ss = torch.full((w.n,), 1e-5, dtype=torch.float64, device=dev)
+    V = torch.from_numpy(w.volumes).to(dev)
+    u0 = torch.zeros(topo.nfree, dtype=torch.float64, device=dev)
+    uN = torch.empty(topo.nfree, dtype=torch.float64, device=dev)
+    free_idx = np.cumsum(topo.isfree) - 1
+    obs_f = torch.from_numpy(free_idx[w.obs - 1]).to(dev)
+    g = torch.zeros(topo.nfree, dtype=torch.float64, device=dev)
+    dK = torch.empty(topo.ne, dtype=torch.float64, device=dev)
+    dQ = torch.empty(topo.n, dtype=torch.float64, device=dev)
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    out = {}
+    for precision in ("fp64", "mixed"):
+        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
+                      stream=sptr)
+        st, sh = _lib.Stats(), ctypes.c_void_p()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.dpf_transient_forward(topo.handle, p(K), p(dh), p(Q), p(Ss), p(V), p(u0),
+                                           nsteps, dt, p(qs), 20, p(uN), None, ctypes.byref(o),
+                                           ctypes.byref(st), ctypes.byref(sh)))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g.zero_()
+        g[obs_f] = 2.0 * uN[obs_f]
+        st2 = _lib.Stats()
+        code = L.dpf_transient_pullback(sh, p(g), p(dK), p(dQ), ctypes.byref(o), ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[precision] = dict(forward_s=round(t1 - t0, 3), adjoint_s=round(t2 - t1, 3),
+                              forward_iterations=st.iterations, adjoint_iterations=st2.iterations)
+        log(f"[bench] transient {precision}: forward {t1 - t0:.2f}s ({st.iterations} PCG "
+            f"iterations over {nsteps} steps), checkpointed adjoint {t2 - t1:.2f}s "
+            f"({st2.iterations} iterations incl. recomputation)")
+    res = {"metric": "transient forward+adjoint wall time, 2M cells x 200 backward-Euler steps",
+           "value": out["mixed"]["forward_s"] + out["mixed"]["adjoint_s"], "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+           "dtype": "f32 inner / f64 refinement", "config": {"workload": "transient 2M, 200 steps, "
+           "checkpoint every 20"}, "detail": out}
+    print(json.dumps(res), flush=True)
 
 
 def traffic_per_launch(ns):

@@ -37,7 +37,7 @@ def build(verbose=False):
         if device:
             cmd += ["--offload-arch=" + ARCH] + DEVICE_FLAGS
         else:
-            cmd = ["g++"] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = ["g++"] + COMMON + ["-fopenmp", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -53,7 +53,8 @@ def build(verbose=False):
         msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
         raise RuntimeError("libdpfgpu build failed:\n" + msg)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    # host setup uses OpenMP through libgomp (the runtime PyTorch also loads)
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lgomp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

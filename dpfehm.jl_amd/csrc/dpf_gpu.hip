@@ -674,7 +674,7 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
         }
       }
     }
-    e = build_hierarchy(t->H, 25, 128, 512);
+    e = build_hierarchy(t->H, 25, 128, 512, t->nm > 1 ? t->m_row0 : std::vector<int64_t>());
     if (!e.empty()) {
       set_err(e);
       return DPF_EINVAL;

@@ -97,7 +97,11 @@ struct TopologyHost {
 // Both return "" on success or an error message.
 std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
                            const double* aol, int64_t nd, const int64_t* dn);
-std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max);
+// member_row0: fine-row offsets of the members of a batch (empty: one
+// system); members are aggregated independently so that they never share an
+// aggregate and each owns a contiguous row range on every level.
+std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max,
+                            const std::vector<int64_t>& member_row0 = {});
 void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src);
 
 }  // namespace dpf

@@ -27,22 +27,27 @@ void SellHost::build(int64_t nr, int64_t nc, const std::vector<int64_t>& rp,
   int64_t ns = (nr + SELL_C - 1) / SELL_C;
   sptr.assign(ns + 1, 0);
   rowlen.assign(nr, 0);
-  max_rowlen = 0;
-  nnz = 0;
+  std::vector<int> width(ns, 0);
+#pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < ns; s++) {
     int w = 0;
     for (int64_t r = s * SELL_C; r < std::min<int64_t>(nr, (s + 1) * SELL_C); r++) {
       int len = (int)(rp[r + 1] - rp[r]);
       rowlen[r] = len;
-      nnz += len;
       w = std::max(w, len);
     }
-    max_rowlen = std::max(max_rowlen, w);
-    sptr[s + 1] = sptr[s] + (int64_t)w * SELL_C;
+    width[s] = w;
   }
-  col.assign(sptr[ns], 0);
+  max_rowlen = 0;
   for (int64_t s = 0; s < ns; s++) {
-    int w = (int)((sptr[s + 1] - sptr[s]) / SELL_C);
+    max_rowlen = std::max(max_rowlen, width[s]);
+    sptr[s + 1] = sptr[s] + (int64_t)width[s] * SELL_C;
+  }
+  nnz = rp[nr] - rp[0];
+  col.assign(sptr[ns], 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < ns; s++) {
+    int w = width[s];
     for (int l = 0; l < SELL_C; l++) {
       int64_t r = s * SELL_C + l;
       int32_t pad = (square && r < nr) ? (int32_t)r : 0;
@@ -158,15 +163,17 @@ std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t
 
 namespace {
 
-// Standard SA aggregation on the graph of A (pattern only): roots whose
-// whole neighbourhood is free start an aggregate; leftovers join a
-// neighbouring phase-1 aggregate; the rest form new aggregates with their
-// unaggregated neighbours.
-int64_t aggregate(const SellHost& A, std::vector<int32_t>& agg) {
-  int64_t n = A.nrows;
-  agg.assign(n, -1);
+// Standard SA aggregation on the graph of A (pattern only), over the rows
+// [r0, r1) (a range closed under the graph: a whole system or one batch
+// member): roots whose whole neighbourhood is free start an aggregate;
+// leftovers join a neighbouring phase-1 aggregate; the rest form new
+// aggregates with their unaggregated neighbours.  Aggregates are numbered
+// from `base`.  Returns how many were made.
+int64_t aggregate_range(const SellHost& A, int64_t r0, int64_t r1, int64_t base,
+                        std::vector<int32_t>& agg) {
   int64_t nc = 0;
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = r0; i < r1; i++) agg[i] = -1;
+  for (int64_t i = r0; i < r1; i++) {
     if (agg[i] != -1) continue;
     bool ok = true;
     for (int s = 0; s < A.rowlen[i] && ok; s++) {
@@ -174,31 +181,50 @@ int64_t aggregate(const SellHost& A, std::vector<int32_t>& agg) {
       if (j != i && agg[j] != -1) ok = false;
     }
     if (!ok) continue;
-    for (int s = 0; s < A.rowlen[i]; s++) agg[A.at(i, s)] = (int32_t)nc;
-    agg[i] = (int32_t)nc;
+    for (int s = 0; s < A.rowlen[i]; s++) agg[A.at(i, s)] = (int32_t)(base + nc);
+    agg[i] = (int32_t)(base + nc);
     nc++;
   }
-  std::vector<int32_t> agg1(agg);
-  for (int64_t i = 0; i < n; i++) {
+  std::vector<int32_t> agg1(agg.begin() + r0, agg.begin() + r1);
+  for (int64_t i = r0; i < r1; i++) {
     if (agg[i] != -1) continue;
     for (int s = 0; s < A.rowlen[i]; s++) {
       int32_t j = A.at(i, s);
-      if (agg1[j] != -1) {
-        agg[i] = agg1[j];
+      if (agg1[j - r0] != -1) {
+        agg[i] = agg1[j - r0];
         break;
       }
     }
   }
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = r0; i < r1; i++) {
     if (agg[i] != -1) continue;
-    agg[i] = (int32_t)nc;
+    agg[i] = (int32_t)(base + nc);
     for (int s = 0; s < A.rowlen[i]; s++) {
       int32_t j = A.at(i, s);
-      if (agg[j] == -1) agg[j] = (int32_t)nc;
+      if (agg[j] == -1) agg[j] = (int32_t)(base + nc);
     }
     nc++;
   }
   return nc;
+}
+
+// Aggregates every member range independently (in parallel); member m's
+// aggregates are numbered after those of members < m, so each member also
+// owns a contiguous row range on the next level (returned in next_row0).
+int64_t aggregate(const SellHost& A, const std::vector<int64_t>& row0, std::vector<int32_t>& agg,
+                  std::vector<int64_t>& next_row0) {
+  int64_t nm = (int64_t)row0.size() - 1;
+  agg.assign(A.nrows, -1);
+  std::vector<int64_t> cnt(nm, 0);
+  // aggregate each member from 0, then shift by the members before it
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t m = 0; m < nm; m++) cnt[m] = aggregate_range(A, row0[m], row0[m + 1], 0, agg);
+  next_row0.assign(nm + 1, 0);
+  for (int64_t m = 0; m < nm; m++) next_row0[m + 1] = next_row0[m] + cnt[m];
+#pragma omp parallel for schedule(static)
+  for (int64_t m = 0; m < nm; m++)
+    for (int64_t i = row0[m]; i < row0[m + 1]; i++) agg[i] += (int32_t)next_row0[m];
+  return next_row0[nm];
 }
 
 // Row-wise unique-column accumulator keyed by a stamp array.
@@ -221,37 +247,79 @@ struct RowSet {
   }
 };
 
+// Builds a sorted, duplicate-free row pattern in parallel.  gen(i, rs)
+// adds row i's candidate columns to rs (rs.add(c, i)); after(i, rs) may use
+// rs.slot for the row's columns.  Returns false if a row exceeds maxlen.
+template <typename Gen, typename After>
+bool par_pattern(int64_t nrows, int64_t ncols, Gen gen, After after, std::vector<int64_t>& rp,
+                 std::vector<int32_t>& ci, size_t maxlen) {
+  rp.assign(nrows + 1, 0);
+  bool bad = false;
+#pragma omp parallel
+  {
+    RowSet rs(ncols);
+#pragma omp for schedule(dynamic, 2048)
+    for (int64_t i = 0; i < nrows; i++) {
+      rs.begin();
+      gen(i, rs);
+      rp[i + 1] = (int64_t)rs.cols.size();
+      if (rs.cols.size() >= maxlen) bad = true;
+    }
+  }
+  if (bad) return false;
+  for (int64_t i = 0; i < nrows; i++) rp[i + 1] += rp[i];
+  ci.resize(rp[nrows]);
+#pragma omp parallel
+  {
+    RowSet rs(ncols);
+#pragma omp for schedule(dynamic, 2048)
+    for (int64_t i = 0; i < nrows; i++) {
+      rs.begin();
+      gen(i, rs);
+      rs.finish();
+      std::copy(rs.cols.begin(), rs.cols.end(), ci.begin() + rp[i]);
+      after(i, rs);
+    }
+  }
+  return true;
+}
+
+// Exclusive prefix sum of counts stored at [1..n] of v (v[0] = 0).
+void prefix(std::vector<int64_t>& v) {
+  for (size_t i = 1; i < v.size(); i++) v[i] += v[i - 1];
+}
+
 std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
   const SellHost& A = L.A;
   int64_t n = A.nrows;
   // P pattern: aggregates seen from each row (including the row's own)
   {
-    RowSet rs(nc);
-    std::vector<int64_t> rp(n + 1, 0);
+    std::vector<int64_t> rp;
     std::vector<int32_t> ci;
-    ci.reserve(A.nnz);
     L.p_slot.assign(A.entries(), NO_SLOT);
     L.p_self.resize(n);
-    for (int64_t i = 0; i < n; i++) {
-      rs.begin();
-      rs.add(L.agg[i], i);
-      for (int s = 0; s < A.rowlen[i]; s++) rs.add(L.agg[A.at(i, s)], i);
-      rs.finish();
-      if (rs.cols.size() >= NO_SLOT) return "prolongator row too long";
-      ci.insert(ci.end(), rs.cols.begin(), rs.cols.end());
-      rp[i + 1] = (int64_t)ci.size();
-      for (int s = 0; s < A.rowlen[i]; s++) L.p_slot[A.idx(i, s)] = (uint16_t)rs.slot[L.agg[A.at(i, s)]];
-      L.p_self[i] = (uint16_t)rs.slot[L.agg[i]];
-    }
+    bool ok = par_pattern(
+        n, nc,
+        [&](int64_t i, RowSet& rs) {
+          rs.add(L.agg[i], i);
+          for (int s = 0; s < A.rowlen[i]; s++) rs.add(L.agg[A.at(i, s)], i);
+        },
+        [&](int64_t i, RowSet& rs) {
+          for (int s = 0; s < A.rowlen[i]; s++)
+            L.p_slot[A.idx(i, s)] = (uint16_t)rs.slot[L.agg[A.at(i, s)]];
+          L.p_self[i] = (uint16_t)rs.slot[L.agg[i]];
+        },
+        rp, ci, NO_SLOT);
+    if (!ok) return "prolongator row too long";
     L.P.build(n, nc, rp, ci, false);
   }
   const SellHost& P = L.P;
-  // R = P' (columns of R = fine rows, sorted ascending by construction)
+  // R = P' (columns of R = fine rows, ascending by construction)
   {
     std::vector<int64_t> rp(nc + 1, 0);
     for (int64_t i = 0; i < n; i++)
       for (int s = 0; s < P.rowlen[i]; s++) rp[P.at(i, s) + 1]++;
-    for (int64_t I = 0; I < nc; I++) rp[I + 1] += rp[I];
+    prefix(rp);
     std::vector<int32_t> ci(rp[nc]);
     std::vector<int64_t> src(rp[nc]);
     std::vector<int64_t> pos(rp.begin(), rp.end() - 1);
@@ -265,11 +333,12 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       if (rp[I + 1] - rp[I] >= NO_SLOT) return "restriction row too long";
     L.R.build(nc, n, rp, ci, false);
     L.r_src.assign(L.R.entries(), -1);
+#pragma omp parallel for schedule(static)
     for (int64_t I = 0; I < nc; I++)
       for (int t = 0; t < L.R.rowlen[I]; t++) L.r_src[L.R.idx(I, t)] = (int32_t)src[rp[I] + t];
   }
   const SellHost& R = L.R;
-  // R split over G lanes per row
+  // R split over G lanes per row (sub-row I*G+g: entries g, g+G, ...)
   {
     double avg = nc > 0 ? (double)R.nnz / (double)nc : 1.0;
     int lg = 0;
@@ -278,101 +347,107 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
     int G = 1 << lg;
     int64_t ns = nc * G;
     std::vector<int64_t> rp(ns + 1, 0);
-    std::vector<int32_t> ci;
-    std::vector<int32_t> src;
-    ci.reserve(R.nnz);
-    src.reserve(R.nnz);
+    for (int64_t I = 0; I < nc; I++)
+      for (int g = 0; g < G; g++) rp[I * G + g + 1] = (R.rowlen[I] - g + G - 1) / G;
+    prefix(rp);
+    std::vector<int32_t> ci(rp[ns]);
+    std::vector<int32_t> src(rp[ns]);
+#pragma omp parallel for schedule(static)
     for (int64_t I = 0; I < nc; I++)
       for (int g = 0; g < G; g++) {
-        for (int t = g; t < R.rowlen[I]; t += G) {
-          ci.push_back(R.at(I, t));
-          src.push_back(L.r_src[R.idx(I, t)]);
+        int64_t q = rp[I * G + g];
+        for (int t = g; t < R.rowlen[I]; t += G, q++) {
+          ci[q] = R.at(I, t);
+          src[q] = L.r_src[R.idx(I, t)];
         }
-        rp[I * G + g + 1] = (int64_t)ci.size();
       }
     L.Rs.build(ns, n, rp, ci, false);
     L.rs_src.assign(L.Rs.entries(), -1);
+#pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < ns; r++)
       for (int k = 0; k < L.Rs.rowlen[r]; k++) L.rs_src[L.Rs.idx(r, k)] = src[rp[r] + k];
   }
-  // A*P pattern; products grouped by destination
+  // A*P pattern; products grouped by destination nonzero
   {
-    RowSet rs(nc);
-    L.ap_rp.assign(n + 1, 0);
-    L.ap_col.clear();
-    std::vector<int32_t> cnt;
-    std::vector<int64_t> pos;
-    // pass 1: pattern
-    for (int64_t i = 0; i < n; i++) {
-      rs.begin();
-      for (int s = 0; s < A.rowlen[i]; s++) {
-        int32_t k = A.at(i, s);
-        for (int m = 0; m < P.rowlen[k]; m++) rs.add(P.at(k, m), i);
-      }
-      rs.finish();
-      if (rs.cols.size() >= NO_SLOT) return "A*P row too long";
-      L.ap_col.insert(L.ap_col.end(), rs.cols.begin(), rs.cols.end());
-      L.ap_rp[i + 1] = (int64_t)L.ap_col.size();
-    }
+    bool ok = par_pattern(
+        n, nc,
+        [&](int64_t i, RowSet& rs) {
+          for (int s = 0; s < A.rowlen[i]; s++) {
+            int32_t k = A.at(i, s);
+            for (int m = 0; m < P.rowlen[k]; m++) rs.add(P.at(k, m), i);
+          }
+        },
+        [&](int64_t, RowSet&) {}, L.ap_rp, L.ap_col, NO_SLOT);
+    if (!ok) return "A*P row too long";
     int64_t nap = L.ap_rp[n];
     if (nap >= INT32_MAX) return "A*P too large for 32-bit indices";
-    // pass 2: count products per destination, then fill in enumeration order
     L.ap_off.assign(nap + 1, 0);
-    std::vector<int32_t> slot_of(nc, 0);
-    for (int64_t i = 0; i < n; i++) {
-      for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
-      for (int s = 0; s < A.rowlen[i]; s++) {
-        int32_t k = A.at(i, s);
-        for (int m = 0; m < P.rowlen[k]; m++) L.ap_off[L.ap_rp[i] + slot_of[P.at(k, m)] + 1]++;
+    // products of row i in enumeration order: (s, m); counts per destination
+#pragma omp parallel
+    {
+      std::vector<int32_t> slot_of(nc, 0);
+#pragma omp for schedule(dynamic, 2048)
+      for (int64_t i = 0; i < n; i++) {
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
+          slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
+        for (int s = 0; s < A.rowlen[i]; s++) {
+          int32_t k = A.at(i, s);
+          for (int m = 0; m < P.rowlen[k]; m++) L.ap_off[L.ap_rp[i] + slot_of[P.at(k, m)] + 1]++;
+        }
       }
     }
-    for (int64_t d = 0; d < nap; d++) L.ap_off[d + 1] += L.ap_off[d];
+    prefix(L.ap_off);
     int64_t nprod = L.ap_off[nap];
     if (nprod >= INT32_MAX) return "too many A*P products";
     L.ap_a.resize(nprod);
     L.ap_p.resize(nprod);
-    std::vector<int64_t> fill(L.ap_off.begin(), L.ap_off.end() - 1);
-    for (int64_t i = 0; i < n; i++) {
-      for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
-      for (int s = 0; s < A.rowlen[i]; s++) {
-        int32_t k = A.at(i, s);
-        int64_t ea = A.idx(i, s);
-        for (int m = 0; m < P.rowlen[k]; m++) {
-          int64_t q = fill[L.ap_rp[i] + slot_of[P.at(k, m)]]++;
-          L.ap_a[q] = (int32_t)ea;
-          L.ap_p[q] = (int32_t)P.idx(k, m);
+#pragma omp parallel
+    {
+      std::vector<int32_t> slot_of(nc, 0);
+      std::vector<int64_t> fill;
+#pragma omp for schedule(dynamic, 2048)
+      for (int64_t i = 0; i < n; i++) {
+        int64_t a0 = L.ap_rp[i], a1 = L.ap_rp[i + 1];
+        fill.assign(L.ap_off.begin() + a0, L.ap_off.begin() + a1);
+        for (int64_t a = a0; a < a1; a++) slot_of[L.ap_col[a]] = (int32_t)(a - a0);
+        for (int s = 0; s < A.rowlen[i]; s++) {
+          int32_t k = A.at(i, s);
+          int64_t ea = A.idx(i, s);
+          for (int m = 0; m < P.rowlen[k]; m++) {
+            int64_t q = fill[slot_of[P.at(k, m)]]++;
+            L.ap_a[q] = (int32_t)ea;
+            L.ap_p[q] = (int32_t)P.idx(k, m);
+          }
         }
       }
     }
   }
-  // R*(A*P): the next level's pattern, then products grouped by its SELL entries
+  // R*(A*P): the next level's pattern, then products grouped by its SELL
+  // entries (lists numbered row-major over every slot, padding included)
   {
-    RowSet rs(nc);
-    std::vector<int64_t> rp(nc + 1, 0);
+    std::vector<int64_t> rp;
     std::vector<int32_t> ci;
-    for (int64_t I = 0; I < nc; I++) {
-      rs.begin();
-      rs.add((int32_t)I, I);
-      for (int t = 0; t < R.rowlen[I]; t++) {
-        int32_t i = R.at(I, t);
-        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) rs.add(L.ap_col[a], I);
-      }
-      rs.finish();
-      if (rs.cols.size() >= NO_SLOT) return "coarse operator row too long";
-      ci.insert(ci.end(), rs.cols.begin(), rs.cols.end());
-      rp[I + 1] = (int64_t)ci.size();
-    }
+    bool ok = par_pattern(
+        nc, nc,
+        [&](int64_t I, RowSet& rs) {
+          rs.add((int32_t)I, I);
+          for (int t = 0; t < R.rowlen[I]; t++) {
+            int32_t i = R.at(I, t);
+            for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) rs.add(L.ap_col[a], I);
+          }
+        },
+        [&](int64_t, RowSet&) {}, rp, ci, NO_SLOT);
+    if (!ok) return "coarse operator row too long";
     next = LevelHost();
     next.A.build(nc, nc, rp, ci, true);
     next.diag_slot.resize(nc);
+#pragma omp parallel for schedule(static)
     for (int64_t I = 0; I < nc; I++) {
       auto b = ci.begin() + rp[I], e = ci.begin() + rp[I + 1];
       next.diag_slot[I] = (uint16_t)(std::lower_bound(b, e, (int32_t)I) - b);
     }
     const SellHost& C = next.A;
     int64_t nent = C.entries();
-    // row-major list numbering over all SELL slots, padding included (also
-    // the lanes of the last slice beyond the last row)
     int64_t nrm = C.nslices() * SELL_C;
     std::vector<int64_t> rm_base(nrm + 1, 0);
     for (int64_t I = 0; I < nrm; I++) {
@@ -380,35 +455,49 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       rm_base[I + 1] = rm_base[I] + (C.sptr[sl + 1] - C.sptr[sl]) / SELL_C;
     }
     L.rap_dest.resize(nent);
+#pragma omp parallel for schedule(static)
     for (int64_t I = 0; I < nrm; I++)
       for (int64_t k = 0; k < rm_base[I + 1] - rm_base[I]; k++)
         L.rap_dest[rm_base[I] + k] = (int32_t)C.idx(I, (int)k);
     L.rap_off.assign(nent + 1, 0);
-    std::vector<int32_t> slot_of(nc, 0);
-    auto each = [&](auto&& fn) {
+#pragma omp parallel
+    {
+      std::vector<int32_t> slot_of(nc, 0);
+#pragma omp for schedule(dynamic, 512)
       for (int64_t I = 0; I < nc; I++) {
         for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]);
         for (int t = 0; t < R.rowlen[I]; t++) {
           int32_t i = R.at(I, t);
-          int32_t pe = L.r_src[R.idx(I, t)];
           for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
-            fn(rm_base[I] + slot_of[L.ap_col[a]], pe, a);
+            L.rap_off[rm_base[I] + slot_of[L.ap_col[a]] + 1]++;
         }
       }
-    };
-    each([&](int64_t e, int32_t, int64_t) { L.rap_off[e + 1]++; });
-    for (int64_t e = 0; e < nent; e++) L.rap_off[e + 1] += L.rap_off[e];
+    }
+    prefix(L.rap_off);
     int64_t nprod = L.rap_off[nent];
     if (nprod >= INT32_MAX) return "too many R*A*P products";
     L.rap_p.resize(nprod);
     L.rap_ap.resize(nprod);
-    std::vector<int64_t> fill(L.rap_off.begin(), L.rap_off.end() - 1);
-    each([&](int64_t e, int32_t pe, int64_t a) {
-      int64_t q = fill[e]++;
-      L.rap_p[q] = pe;
-      L.rap_ap[q] = (int32_t)a;
-    });
-    // lanes per list: about 4 contributions per lane on average
+#pragma omp parallel
+    {
+      std::vector<int32_t> slot_of(nc, 0);
+      std::vector<int64_t> fill;
+#pragma omp for schedule(dynamic, 512)
+      for (int64_t I = 0; I < nc; I++) {
+        int64_t b0 = rm_base[I], b1 = rm_base[I + 1];
+        fill.assign(L.rap_off.begin() + b0, L.rap_off.begin() + b1);
+        for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]);
+        for (int t = 0; t < R.rowlen[I]; t++) {
+          int32_t i = R.at(I, t);
+          int32_t pe = L.r_src[R.idx(I, t)];
+          for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) {
+            int64_t q = fill[slot_of[L.ap_col[a]]]++;
+            L.rap_p[q] = pe;
+            L.rap_ap[q] = (int32_t)a;
+          }
+        }
+      }
+    }
     auto lanes = [](int64_t nprod_, int64_t nlists) {
       double avg = nlists > 0 ? (double)nprod_ / (double)nlists : 1.0;
       int lg = 0;
@@ -449,12 +538,17 @@ void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& s
     for (int k = 0; k < S.rowlen[r]; k++) src[S.idx(r, k)] = si[rp[r] + k];
 }
 
-std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max) {
+std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max,
+                            const std::vector<int64_t>& member_row0) {
   auto t0 = std::chrono::steady_clock::now();
   T.levels.resize(1);
+  std::vector<int64_t> row0 = member_row0;
+  if (row0.size() < 2) row0 = {0, T.levels[0].A.nrows};
   while ((int)T.levels.size() < max_levels && T.levels.back().A.nrows > coarse_max) {
     LevelHost& L = T.levels.back();
-    int64_t nc = aggregate(L.A, L.agg);
+    std::vector<int64_t> next_row0;
+    int64_t nc = aggregate(L.A, row0, L.agg, next_row0);
+    row0 = next_row0;
     if (nc <= 0 || (double)nc > 0.85 * (double)L.A.nrows) {
       L.agg.clear();
       break;
@@ -465,8 +559,9 @@ std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int
     T.levels.push_back(std::move(next));
   }
   T.coarse_dense = T.levels.back().A.nrows <= dense_max;
-  for (size_t l = 1; l < T.levels.size(); l++) split_rows(T.levels[l].A, T.levels[l].as_lg,
-                                                          T.levels[l].As, T.levels[l].as_src);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t l = 1; l < T.levels.size(); l++)
+    split_rows(T.levels[l].A, T.levels[l].as_lg, T.levels[l].As, T.levels[l].as_src);
   T.setup_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return "";
