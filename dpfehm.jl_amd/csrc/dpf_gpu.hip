@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -952,8 +953,8 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   // Returns the number of members whose residual is above their target.
   auto residual = [&](int first) -> int {
     hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
-                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(), bm_ptr(t),
-                       part);
+                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(),
+                       (const SolveScalars*)sc, bm_ptr(t), part);
     int oseq = ++t->seq;
     finish(t, s, FIN_OUTER, g, o->reltol, o->abstol, 0, first, oseq);
     HIPTRY(hipGetLastError());
@@ -965,13 +966,20 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       if (!std::isfinite(mirror_load(&t->hm_outer_host[m]))) return false;
     return true;
   };
+  // every member works on b 2^-e, max|b| = f 2^e with f in [1/2, 1): exact,
+  // and keeps the FP64 norms of b and of the residuals clear of underflow
+  // and overflow (an adjoint that decays over many time steps reaches
+  // |b| ~ 1e-200 long before it reaches zero)
+  hipLaunchKernelGGL(k_amax, dim3(g), dim3(BLK), 0, s, nf, b, bm_ptr(t), part);
+  finish(t, s, FIN_AMAX, g);
   hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>());
   int active = residual(1);
   std::vector<double> bnorm(t->nm);
   for (int m = 0; m < t->nm; m++) bnorm[m] = mirror_load(&t->hm_outer_host[m]);
   bool finite = all_finite();
   if (finite && x0) {
-    copy_dev(t->x64.p, x0, nf * sizeof(double), s);
+    hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(BLK), 0, s, nf, x0, t->x64.as<double>(),
+                       (const SolveScalars*)sc, bm_ptr(t), 0);
     active = residual(0);
     finite = all_finite();
   }
@@ -1062,7 +1070,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   }
   if (!(relres == relres)) relres = INFINITY;
 
-  copy_dev(xout, t->x64.p, nf * sizeof(double), s);
+  hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(BLK), 0, s, nf,
+                     (const double*)t->x64.as<double>(), xout, (const SolveScalars*)sc, bm_ptr(t),
+                     1);
   if (st) {
     st->iterations = total_it;
     st->outer = outer;
@@ -1492,6 +1502,16 @@ bool hier_fresh(dpf_topology* t, dpf_state* st, int precision) {
          (precision == DPF_MIXED ? t->h32.version == st->version : t->h64.version == st->version);
 }
 
+// Error text for a failed time step: which step, which phase, how far it got.
+void step_failure(const char* what, int step, int code, const dpf_stats& ss) {
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s %d did not converge (%s after %d iterations in %d "
+                "refinement steps, relative residual %.3e)", what, step,
+                code == DPF_EBREAKDOWN ? "breakdown" : "iteration limit", ss.iterations, ss.outer,
+                ss.relres);
+  set_err(buf);
+}
+
 // u_{n} from u_{n-1}: one backward-Euler step (warm-started at u_{n-1})
 int be_step(dpf_topology* t, hipStream_t s, dpf_state* st, int n, const double* uprev,
             double* unext, const dpf_opts* o, int& iters) {
@@ -1506,6 +1526,7 @@ int be_step(dpf_topology* t, hipStream_t s, dpf_state* st, int n, const double* 
   std::memset(&ss, 0, sizeof(ss));
   int code = solve_system(t, s, rhs, unext, o, &ss, uprev);
   iters += ss.iterations;
+  if (code != DPF_OK) step_failure("backward-Euler step", n, code, ss);
   return code;
 }
 
@@ -1620,10 +1641,7 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
       stats->solve_ms = now_ms() - t1;
       stats->total_ms = now_ms() - t0;
     }
-    if (status != DPF_OK) {
-      set_err("backward-Euler step did not converge");
-      return status;
-    }
+    if (status != DPF_OK) return status;
     if (state_out) *state_out = st.release();
     return (int)DPF_OK;
   });
@@ -1689,7 +1707,10 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
         std::memset(&ss, 0, sizeof(ss));
         status = solve_system(t, s, rhs, mu, o, &ss, have_next ? mu_next : nullptr);
         iters += ss.iterations;
-        if (status != DPF_OK) break;
+        if (status != DPF_OK) {
+          step_failure("adjoint backward-Euler step", k, status, ss);
+          break;
+        }
         const double* uk = segp + (size_t)(k - a) * nf;
         hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
                            t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), uk,
@@ -1720,7 +1741,6 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
       stats->solve_ms = now_ms() - t0;
       stats->total_ms = now_ms() - t0;
     }
-    if (status != DPF_OK) set_err("adjoint backward-Euler step did not converge");
     return status;
   });
 }

@@ -55,6 +55,7 @@ struct SolveScalars {
   double outer_s;  // refinement: ||b - A x|| in FP64
   double bnorm;    // refinement: ||b||
   double target;   // refinement: stop when outer_s <= target
+  double bscale;   // refinement: power of two that brings max|b| into [1/2, 1)
   int iters, maxiter;
   int done;  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
   int first;
@@ -81,6 +82,12 @@ __device__ __forceinline__ int64_t logical_row() {
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
 
@@ -127,7 +134,25 @@ __device__ __forceinline__ void block_partial2(double v, bool second,
   }
 }
 
-enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER = 4 };
+// Block maxima of non-negative values, laid out like block_partial2.
+__device__ __forceinline__ void block_partial2_max(double v, bool second,
+                                                   double* __restrict__ partial) {
+  __shared__ double sh[2][4];
+  double a = wave_max(second ? 0.0 : v);
+  double b = wave_max(second ? v : 0.0);
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t lb = logical_row() / BLK;
+    partial[2 * lb] = fmax(fmax(sh[0][0], sh[0][1]), fmax(sh[0][2], sh[0][3]));
+    partial[2 * lb + 1] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+  }
+}
+
+enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER = 4, FIN_AMAX = 5 };
 
 // Per member: the scalar step of the Krylov solve after the block partials.
 //   FIN_START  rr, stop threshold (relative, or the member's absolute
@@ -135,6 +160,12 @@ enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER =
 //   FIN_PQ     alpha = rz / p.q          FIN_XR  rr, iteration count, convergence
 //   FIN_RZ     rz_old = rz, rz = r.z     FIN_OUTER  FP64 residual norm (and, when
 //              abs_target, the member's ||b|| and target: the first residual)
+//   FIN_AMAX   tot = max|b| over the member: bscale = 2^-e with max|b| = f 2^e,
+//              f in [1/2, 1) (1 for a zero or non-finite b), so that the
+//              refinement works on b bscale, whose squared norms neither
+//              underflow nor overflow in FP64; scaling by a power of two is
+//              exact, so the iterates are those of the unscaled system times
+//              bscale wherever the unscaled ones are representable
 __device__ void finish_member(int mode, double tot, SolveScalars& m, double reltol,
                               double abstol, int maxiter, int abs_target) {
   switch (mode) {
@@ -184,13 +215,19 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
       m.rz = tot;
       if (!(tot > 0.0)) m.done = 3;
       break;
+    case FIN_AMAX: {
+      int e = (tot > 0.0 && tot <= 1.79769313486231570e308) ? ilogb(tot) + 1 : 0;
+      e = e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
+      m.bscale = ldexp(1.0, -e);
+      break;
+    }
     default: {
       double sn = sqrt(tot);
       m.outer_s = sn;
       if (abs_target) {
         m.bnorm = sn;
-        double tg = reltol * sn;
-        m.target = tg > abstol ? tg : abstol;
+        double tg = reltol * sn, ta = abstol * m.bscale;
+        m.target = tg > ta ? tg : ta;
       }
     }
   }
@@ -208,31 +245,35 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
                                                  double abstol, int maxiter, int abs_target,
                                                  int seq, HostMirror* hm,
                                                  double* __restrict__ hm_outer) {
-  if (mode != FIN_START && mode != FIN_OUTER && *gdone) return;
+  if (mode != FIN_START && mode != FIN_OUTER && mode != FIN_AMAX && *gdone) return;
   __shared__ double sh[16];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool mx = mode == FIN_AMAX;
   if (nm == 1) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[2 * b];
-    s = wave_sum(s);
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+      s = mx ? fmax(s, partial[2 * b]) : s + partial[2 * b];
+    s = mx ? wave_max(s) : wave_sum(s);
     if (lane == 0) sh[wave] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
       double tot = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot += sh[k];
+      for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot = mx ? fmax(tot, sh[k]) : tot + sh[k];
       finish_member(mode, tot, sc[0], reltol, abstol, maxiter, abs_target);
     }
   } else {
     for (int m = wave; m < nm; m += (int)(blockDim.x >> 6)) {
       double s = 0.0;
-      for (int b = mblk[2 * m] + lane; b <= mblk[2 * m + 1]; b += 64)
-        s += partial[2 * b + (bm[b].x == m ? 0 : 1)];
-      s = wave_sum(s);
+      for (int b = mblk[2 * m] + lane; b <= mblk[2 * m + 1]; b += 64) {
+        double v = partial[2 * b + (bm[b].x == m ? 0 : 1)];
+        s = mx ? fmax(s, v) : s + v;
+      }
+      s = mx ? wave_max(s) : wave_sum(s);
       if (lane == 0) finish_member(mode, s, sc[m], reltol, abstol, maxiter, abs_target);
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || mx) return;
   int running = 0, capped = 0, broke = 0, maxit = 0, active = 0;
   double rr = 0.0;
   for (int m = 0; m < nm; m++) {
@@ -741,13 +782,37 @@ __global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
 
 // ------------------------------------------------------------------ FP64 outer residual / refinement
 
-// r = b - A x in FP64, partials of ||r||^2 per member.
-__global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
-                                 const double* __restrict__ b, double* __restrict__ r,
-                                 const int2* __restrict__ bm, double* __restrict__ partial) {
+// Partials of max|b| per member (FIN_AMAX).
+__global__ void k_amax(int64_t n, const double* __restrict__ b, const int2* __restrict__ bm,
+                       double* __restrict__ partial) {
   int64_t i = logical_row();
   bool second;
   member_of(bm, i, second);
+  // NaN propagates as +inf, which the finish maps to scale 1
+  double v = i < n ? (b[i] == b[i] ? fabs(b[i]) : INFINITY) : 0.0;
+  block_partial2_max(v, second, partial);
+}
+
+// y = x bscale (inverse = 0) or x / bscale (inverse = 1), per member
+__global__ void k_scale_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y,
+                             const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
+                             int inverse) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  bool second;
+  double f = sc[member_of(bm, i, second)].bscale;
+  y[i] = inverse ? x[i] / f : x[i] * f;
+}
+
+// r = b bscale - A x in FP64 (bscale = 1 when sc is null), partials of
+// ||r||^2 per member.
+__global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
+                                 const double* __restrict__ b, double* __restrict__ r,
+                                 const SolveScalars* __restrict__ sc,
+                                 const int2* __restrict__ bm, double* __restrict__ partial) {
+  int64_t i = logical_row();
+  bool second;
+  int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < A.nrows) {
     int64_t base;
@@ -758,7 +823,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
       int64_t e = base + (int64_t)k * 64;
       ax += A.val[e] * x[A.col[e]];
     }
-    double rv = b[i] - ax;
+    double rv = (sc ? b[i] * sc[mi].bscale : b[i]) - ax;
     r[i] = rv;
     local = rv * rv;
   }

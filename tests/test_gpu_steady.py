@@ -124,6 +124,32 @@ def test_deterministic_bitwise(pkg, api):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_power_of_two_scale_invariance(pkg, api, precision):
+    """The solve is linear in (dirichleths, Qs) and the pullback linear in
+    delta: scaling by 2^-600 or 2^600 (where FP64 sums of squares of the
+    unscaled solver would underflow or overflow) must scale the results by
+    exactly that power of two, bit for bit."""
+    _say("scale invariance", precision)
+    w = pkg.workloads.steady_2m(ns=(24, 20, 10), maxs=(12.0, 10.0, 5.0))
+    K = w.Ks()
+    dh = w.dirichleths + np.where(np.arange(w.n) % 3 == 0, 0.5, 0.0)
+    delta = np.where(np.arange(w.n) % 11 == 0, 1.0, 0.0)
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    h1, pb1 = api.rrule_groundwater_steadystate(*args, dh, w.Qs, precision=precision)
+    g1 = pb1(delta)
+    for e in (-600, 600):
+        s = 2.0 ** e
+        hs, pbs = api.rrule_groundwater_steadystate(*args, dh * s, w.Qs * s, precision=precision)
+        assert np.array_equal(hs, h1 * s)
+        gs = pbs(delta)        # lambda as before, dK scaled through h
+        assert np.array_equal(gs[1], g1[1] * s)
+        assert np.array_equal(gs[5], g1[5]) and np.array_equal(gs[6], g1[6])
+        gd = pb1(delta * s)    # lambda scaled
+        assert np.array_equal(gd[1], g1[1] * s)
+        assert np.array_equal(gd[5], g1[5] * s) and np.array_equal(gd[6], g1[6] * s)
+
+
 def test_gradient_central_fd_logK(pkg, api):
     """J = sum_obs (h - h_obs)^2 through hm, the solve and the pullback; central
     finite differences (step 1e-6) on seeded cells."""

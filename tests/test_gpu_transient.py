@@ -48,6 +48,44 @@ def test_transient_forward_adjoint_vs_oracle(pkg, oracle, precision, ck):
     assert rel(dQ, dQ_ref) < 1e-7
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stiff_adjoint_long_horizon(pkg, oracle, precision):
+    """dt = 3600 s on a 4 m deep box with K/Ss ~ 12 m^2/s: every step damps
+    the slowest mode by ~1e-4, so the adjoint falls below 1e-160 within 40
+    steps of the reverse sweep and squared FP64 norms of it underflow.  The
+    states reach the steady state after a few steps, so with a constant Q the
+    gradient over 100 steps equals the gradient over the last 20 (earlier
+    steps add terms ~1e-80 smaller), which the oracle gives."""
+    from dpfehm_jl_amd import api
+    print("stiff adjoint", precision, flush=True)
+    w, Ss = _problem(pkg)
+    K = w.Ks()
+    nf = w.n - w.dirichletnodes.size
+    dt = 3600.0
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs, Ss,
+            w.volumes)
+    rng = np.random.default_rng(4)
+    g = np.zeros(nf)
+    g[rng.choice(nf, 30, replace=False)] = rng.normal(size=30)
+    out = {}
+    for N in (20, 100):
+        uN, pb = api.rrule_groundwater_transient(*args, np.zeros(nf), N, dt, np.ones(N + 1),
+                                                 checkpoint_every=25, reltol=1e-12,
+                                                 precision=precision)
+        out[N] = (uN,) + tuple(pb(g))
+    ref = oracle.transient(*args, np.zeros(nf), 20, dt, np.ones(21), reltol=1e-14)
+    G = np.zeros_like(ref)
+    G[-1] = g
+    dK_ref, dQ_ref = oracle.transient_adjoint(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                              w.dirichleths, Ss, w.volumes, 20, dt, np.ones(21),
+                                              ref, G, reltol=1e-14)
+    assert rel(out[20][0], ref[-1]) < 1e-9
+    assert rel(out[100][0], ref[-1]) < 1e-9
+    for N in (20, 100):
+        assert rel(out[N][1], dK_ref) < 1e-7
+        assert rel(out[N][2], dQ_ref) < 1e-7
+
+
 def test_transient_states_and_steady_limit(pkg):
     from dpfehm_jl_amd import api
     w, Ss = _problem(pkg, ns=(16, 14, 6))
