@@ -181,7 +181,7 @@ struct dpf_topology {
   Hier<double> h64;
   Hier<float> h32;
   // reductions and solve scalars
-  DBuf partial, scal, gdone, zero_flag;
+  DBuf partial, partial2, scal, gdone, zero_flag;
   // members of a batch (one for a single system): fine-row ranges and, per
   // logical block of the fine grid, the block's first member and the row
   // where the next member starts
@@ -382,9 +382,10 @@ void setup_members(dpf_topology* t) {
 }
 
 void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol = 0,
-            double abstol = 0, int maxiter = 0, int abs_target = 0, int seq = 0) {
+            double abstol = 0, int maxiter = 0, int abs_target = 0, int seq = 0,
+            const double* partial = nullptr) {
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, t->nm, (int)nparts,
-                     (const double*)t->partial.as<double>(), bm_ptr(t),
+                     partial ? partial : (const double*)t->partial.as<double>(), bm_ptr(t),
                      (const int*)t->mblk.as<int>(), t->scal.as<SolveScalars>(),
                      t->gdone.as<int>(), reltol, abstol, maxiter, abs_target, seq, t->hm_dev,
                      t->hm_outer_dev);
@@ -730,6 +731,7 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     maxrows = std::max<int64_t>(maxrows, std::max(H.n, H.ne));
     t->max_grid = nblocks(maxrows) + 8;
     t->partial.alloc(2 * t->max_grid * sizeof(double));
+    t->partial2.alloc(2 * t->max_grid * sizeof(double));
     setup_members(t.get());
     t->scal.alloc(t->nm * sizeof(SolveScalars));
     HIPTRY(hipMemset(t->scal.p, 0, t->nm * sizeof(SolveScalars)));
@@ -948,13 +950,25 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int total_it = 0, outer = 0;
   t->x64.alloc(nf * sizeof(double));
   t->r64.alloc(nf * sizeof(double));
-  // r64 = b - A x64; every member's ||r64|| lands in the pinned mirror.
-  // first: also take it as the member's ||b|| and set its refinement target.
-  // Returns the number of members whose residual is above their target.
-  auto residual = [&](int first) -> int {
+  // r64 = b - A x64 (b alone when first: x64 = 0); every member's ||r64||
+  // lands in the pinned mirror.  first: also take it as the member's ||b||
+  // and set its refinement target.  warm: x64 holds the initial guess x0,
+  // which is first replaced by its A-norm-optimal multiple.  Returns the
+  // number of members whose residual is above their target.
+  auto residual = [&](int first, bool warm = false) -> int {
     hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
-                       (const double*)t->x64.as<double>(), b, t->r64.as<double>(),
-                       (const SolveScalars*)sc, bm_ptr(t), part);
+                       first ? (const double*)nullptr : (const double*)t->x64.as<double>(), b,
+                       t->r64.as<double>(), (const SolveScalars*)sc, bm_ptr(t), part);
+    if (warm) {
+      double* part2 = t->partial2.as<double>();
+      hipLaunchKernelGGL(k_warm_dots, dim3(g), dim3(BLK), 0, s, nf,
+                         (const double*)t->x64.as<double>(), b, (const double*)t->r64.as<double>(),
+                         (const SolveScalars*)sc, bm_ptr(t), part, part2);
+      finish(t, s, FIN_WXB, g);
+      finish(t, s, FIN_WXR, g, 0, 0, 0, 0, 0, part2);
+      hipLaunchKernelGGL(k_warm_apply, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(), b,
+                         t->r64.as<double>(), (const SolveScalars*)sc, bm_ptr(t), part);
+    }
     int oseq = ++t->seq;
     finish(t, s, FIN_OUTER, g, o->reltol, o->abstol, 0, first, oseq);
     HIPTRY(hipGetLastError());
@@ -980,7 +994,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   if (finite && x0) {
     hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(BLK), 0, s, nf, x0, t->x64.as<double>(),
                        (const SolveScalars*)sc, bm_ptr(t), 0);
-    active = residual(0);
+    active = residual(0, true);
     finite = all_finite();
   }
   if (!finite) {

@@ -49,16 +49,21 @@ __device__ __forceinline__ void mpublish(int* p, int v) {
 // (a single system is a batch of one).  All FP64, also in mixed mode.
 struct SolveScalars {
   double rr;       // current ||r||^2
-  double rr_stop;  // stop when rr <= rr_stop
+  double rr_stop;  // converged when rr <= rr_stop
+  double rr_freeze;  // stop updating the member when rr <= rr_freeze (<= rr_stop)
   double rz, rz_old;
   double pq, alpha;
   double outer_s;  // refinement: ||b - A x|| in FP64
   double bnorm;    // refinement: ||b||
   double target;   // refinement: stop when outer_s <= target
   double bscale;   // refinement: power of two that brings max|b| into [1/2, 1)
+  double xb;       // warm start: x0 . b
+  double cwarm;    // warm start: x0 is replaced by cwarm x0
   int iters, maxiter;
   int done;  // 0 running, 1 converged, 2 iteration cap, 3 breakdown
   int first;
+  int conv;  // rr has reached rr_stop (the member may still be iterating)
+  int pad;
 };
 
 template <typename T>
@@ -152,7 +157,16 @@ __device__ __forceinline__ void block_partial2_max(double v, bool second,
   }
 }
 
-enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER = 4, FIN_AMAX = 5 };
+enum FinishMode {
+  FIN_START = 0,
+  FIN_PQ = 1,
+  FIN_XR = 2,
+  FIN_RZ = 3,
+  FIN_OUTER = 4,
+  FIN_AMAX = 5,
+  FIN_WXB = 6,
+  FIN_WXR = 7
+};
 
 // Per member: the scalar step of the Krylov solve after the block partials.
 //   FIN_START  rr, stop threshold (relative, or the member's absolute
@@ -166,8 +180,15 @@ enum FinishMode { FIN_START = 0, FIN_PQ = 1, FIN_XR = 2, FIN_RZ = 3, FIN_OUTER =
 //              underflow nor overflow in FP64; scaling by a power of two is
 //              exact, so the iterates are those of the unscaled system times
 //              bscale wherever the unscaled ones are representable
+//   FIN_WXB    xb = x0 . b           FIN_WXR  tot = x0 . r0: cwarm = x0.b / x0.A x0,
+//              the multiple of the initial guess closest to the solution in
+//              the A-norm (x0.A x0 = x0.b - x0.r0; 1 when x0 = 0)
+// deep: a batch's relative (inner) solve keeps iterating a converged member
+// while other members still run (its updates cost nothing extra: the batched
+// kernels sweep every row anyway) until its residual is 100 times below the
+// target, so that the next refinement round starts deeper for it.
 __device__ void finish_member(int mode, double tot, SolveScalars& m, double reltol,
-                              double abstol, int maxiter, int abs_target) {
+                              double abstol, int maxiter, int abs_target, int deep) {
   switch (mode) {
     case FIN_START: {
       m.rr = tot;
@@ -179,12 +200,14 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
         if (stop < abstol * abstol) stop = abstol * abstol;
       }
       m.rr_stop = stop;
+      m.rr_freeze = deep && !abs_target ? 1e-4 * stop : stop;
       m.iters = 0;
       m.maxiter = maxiter;
       m.first = 1;
       m.rz = 0.0;
       m.rz_old = 0.0;
-      m.done = (tot <= stop) ? 1 : (tot == tot ? 0 : 3);
+      m.conv = tot <= stop;
+      m.done = (tot <= m.rr_freeze) ? 1 : (tot == tot ? 0 : 3);
       break;
     }
     case FIN_PQ:
@@ -192,7 +215,7 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
       m.pq = tot;
       m.first = 0;
       if (!(tot > 0.0))
-        m.done = 3;
+        m.done = m.conv ? 1 : 3;
       else
         m.alpha = m.rz / tot;
       break;
@@ -201,20 +224,29 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
       m.rr = tot;
       int it = m.iters + 1;
       m.iters = it;
-      if (tot <= m.rr_stop)
+      if (tot <= m.rr_stop) m.conv = 1;
+      if (tot <= m.rr_freeze)
         m.done = 1;
       else if (!(tot == tot))
         m.done = 3;
       else if (it >= m.maxiter)
-        m.done = 2;
+        m.done = m.conv ? 1 : 2;
       break;
     }
     case FIN_RZ:
       if (m.done) break;
       m.rz_old = m.rz;
       m.rz = tot;
-      if (!(tot > 0.0)) m.done = 3;
+      if (!(tot > 0.0)) m.done = m.conv ? 1 : 3;
       break;
+    case FIN_WXB:
+      m.xb = tot;
+      break;
+    case FIN_WXR: {
+      double xax = m.xb - tot;
+      m.cwarm = (xax > 0.0 && m.xb == m.xb) ? m.xb / xax : 1.0;
+      break;
+    }
     case FIN_AMAX: {
       int e = (tot > 0.0 && tot <= 1.79769313486231570e308) ? ilogb(tot) + 1 : 0;
       e = e > 1000 ? 1000 : (e < -1000 ? -1000 : e);
@@ -236,7 +268,9 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
 // Sums the block partials of the previous kernel per member, in a fixed
 // order, applies finish_member, then folds the members' states into the
 // global early-exit flag *gdone and the host mirror.  One workgroup; with
-// several members each wave takes members m = wave, wave + 16, ...
+// several members each wave takes members m = wave, wave + 16, ... and folds
+// them in registers, and thread 0 combines the 16 wave results.  The solve
+// stops once no member is below its target and still unconverged.
 __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
                                                  const double* __restrict__ partial,
                                                  const int2* __restrict__ bm,
@@ -245,10 +279,23 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
                                                  double abstol, int maxiter, int abs_target,
                                                  int seq, HostMirror* hm,
                                                  double* __restrict__ hm_outer) {
-  if (mode != FIN_START && mode != FIN_OUTER && mode != FIN_AMAX && *gdone) return;
-  __shared__ double sh[16];
+  if (mode != FIN_START && mode < FIN_OUTER && *gdone) return;
+  constexpr int NW = 16;  // waves of the 1024-thread workgroup
+  __shared__ double sh[NW], sh_rr[NW];
+  __shared__ int sh_flags[NW], sh_maxit[NW], sh_active[NW];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool mx = mode == FIN_AMAX;
+  // this wave's fold of its members: running (below target, unconverged),
+  // capped, broken down; largest iteration count and rr; members above their
+  // refinement target (FIN_OUTER)
+  int flags = 0, maxit = 0, active = 0;
+  double rr = 0.0;
+  auto fold = [&](const SolveScalars& q) {
+    flags |= (q.done == 0 && !q.conv ? 1 : 0) | (q.done == 2 ? 2 : 0) | (q.done == 3 ? 4 : 0);
+    maxit = q.iters > maxit ? q.iters : maxit;
+    rr = q.rr > rr ? q.rr : rr;
+    active += q.outer_s > q.target ? 1 : 0;
+  };
   if (nm == 1) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x)
@@ -256,38 +303,58 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     s = mx ? wave_max(s) : wave_sum(s);
     if (lane == 0) sh[wave] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double tot = 0.0;
-      for (int k = 0; k < (int)(blockDim.x >> 6); k++) tot = mx ? fmax(tot, sh[k]) : tot + sh[k];
-      finish_member(mode, tot, sc[0], reltol, abstol, maxiter, abs_target);
-    }
+    if (threadIdx.x != 0) return;
+    double tot = 0.0;
+    for (int k = 0; k < NW; k++) tot = mx ? fmax(tot, sh[k]) : tot + sh[k];
+    SolveScalars q = sc[0];
+    finish_member(mode, tot, q, reltol, abstol, maxiter, abs_target, 0);
+    sc[0] = q;
+    if (mode > FIN_OUTER) return;
+    fold(q);
+    if (mode == FIN_OUTER) mput(&hm_outer[0], q.outer_s);
   } else {
-    for (int m = wave; m < nm; m += (int)(blockDim.x >> 6)) {
+    for (int m = wave; m < nm; m += NW) {
+      // every block of the member's range but the first starts with the
+      // member (slot 0); the first may start with the previous member
+      int b0 = mblk[2 * m], b1 = mblk[2 * m + 1];
       double s = 0.0;
-      for (int b = mblk[2 * m] + lane; b <= mblk[2 * m + 1]; b += 64) {
-        double v = partial[2 * b + (bm[b].x == m ? 0 : 1)];
+      for (int b = b0 + 1 + lane; b <= b1; b += 64) s = mx ? fmax(s, partial[2 * b]) : s + partial[2 * b];
+      if (lane == 0) {
+        double v = partial[2 * b0 + (bm[b0].x == m ? 0 : 1)];
         s = mx ? fmax(s, v) : s + v;
       }
       s = mx ? wave_max(s) : wave_sum(s);
-      if (lane == 0) finish_member(mode, s, sc[m], reltol, abstol, maxiter, abs_target);
+      if (lane == 0) {
+        SolveScalars q = sc[m];
+        finish_member(mode, s, q, reltol, abstol, maxiter, abs_target, 1);
+        sc[m] = q;
+        fold(q);
+        if (mode == FIN_OUTER) mput(&hm_outer[m], q.outer_s);
+      }
+    }
+    if (mode > FIN_OUTER) return;
+    if (lane == 0) {
+      sh_flags[wave] = flags;
+      sh_maxit[wave] = maxit;
+      sh_active[wave] = active;
+      sh_rr[wave] = rr;
+    }
+    // the barrier also completes every wave's mirror stores before thread 0
+    // publishes below
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    flags = 0;
+    maxit = 0;
+    active = 0;
+    rr = 0.0;
+    for (int k = 0; k < NW; k++) {
+      flags |= sh_flags[k];
+      maxit = sh_maxit[k] > maxit ? sh_maxit[k] : maxit;
+      active += sh_active[k];
+      rr = sh_rr[k] > rr ? sh_rr[k] : rr;
     }
   }
-  __syncthreads();
-  if (threadIdx.x != 0 || mx) return;
-  int running = 0, capped = 0, broke = 0, maxit = 0, active = 0;
-  double rr = 0.0;
-  for (int m = 0; m < nm; m++) {
-    int d = sc[m].done;
-    running |= (d == 0);
-    capped |= (d == 2);
-    broke |= (d == 3);
-    if (sc[m].iters > maxit) maxit = sc[m].iters;
-    if (sc[m].rr > rr) rr = sc[m].rr;
-    if (mode == FIN_OUTER) {
-      mput(&hm_outer[m], sc[m].outer_s);
-      active += sc[m].outer_s > sc[m].target ? 1 : 0;
-    }
-  }
+  int running = flags & 1, capped = flags & 2, broke = flags & 4;
   if (mode == FIN_OUTER) {
     mput(&hm->active, active);
     mpublish(&hm->outer_seq, seq);
@@ -804,8 +871,45 @@ __global__ void k_scale_copy(int64_t n, const double* __restrict__ x, double* __
   y[i] = inverse ? x[i] / f : x[i] * f;
 }
 
-// r = b bscale - A x in FP64 (bscale = 1 when sc is null), partials of
-// ||r||^2 per member.
+// Partials of x0 . (b bscale) and x0 . r0 per member (warm start projection).
+__global__ void k_warm_dots(int64_t n, const double* __restrict__ x, const double* __restrict__ b,
+                            const double* __restrict__ r, const SolveScalars* __restrict__ sc,
+                            const int2* __restrict__ bm, double* __restrict__ pxb,
+                            double* __restrict__ pxr) {
+  int64_t i = logical_row();
+  bool second;
+  int mi = member_of(bm, i, second);
+  double xb = 0.0, xr = 0.0;
+  if (i < n) {
+    double xi = x[i];
+    xb = xi * (b[i] * sc[mi].bscale);
+    xr = xi * r[i];
+  }
+  block_partial2(xb, second, pxb);
+  block_partial2(xr, second, pxr);
+}
+
+// x = c x0, r = b bscale - c A x0 = (1 - c) b bscale + c r0, partials of ||r||^2
+__global__ void k_warm_apply(int64_t n, double* __restrict__ x, const double* __restrict__ b,
+                             double* __restrict__ r, const SolveScalars* __restrict__ sc,
+                             const int2* __restrict__ bm, double* __restrict__ partial) {
+  int64_t i = logical_row();
+  bool second;
+  int mi = member_of(bm, i, second);
+  double local = 0.0;
+  if (i < n) {
+    const SolveScalars& m = sc[mi];
+    double c = m.cwarm;
+    x[i] *= c;
+    double rv = (1.0 - c) * (b[i] * m.bscale) + c * r[i];
+    r[i] = rv;
+    local = rv * rv;
+  }
+  block_partial2(local, second, partial);
+}
+
+// r = b bscale - A x in FP64 (bscale = 1 when sc is null; x = null stands
+// for x = 0), partials of ||r||^2 per member.
 __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
                                  const double* __restrict__ b, double* __restrict__ r,
                                  const SolveScalars* __restrict__ sc,
@@ -815,13 +919,15 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
   int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < A.nrows) {
-    int64_t base;
-    int w;
-    sell_row(A, i, base, w);
     double ax = 0.0;
-    for (int k = 0; k < w; k++) {
-      int64_t e = base + (int64_t)k * 64;
-      ax += A.val[e] * x[A.col[e]];
+    if (x) {
+      int64_t base;
+      int w;
+      sell_row(A, i, base, w);
+      for (int k = 0; k < w; k++) {
+        int64_t e = base + (int64_t)k * 64;
+        ax += A.val[e] * x[A.col[e]];
+      }
     }
     double rv = (sc ? b[i] * sc[mi].bscale : b[i]) - ax;
     r[i] = rv;
