@@ -297,9 +297,18 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     active += q.outer_s > q.target ? 1 : 0;
   };
   if (nm == 1) {
+    // eight independent loads per thread in flight at a time
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-      s = mx ? fmax(s, partial[2 * b]) : s + partial[2 * b];
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += 8 * 1024) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        int b = b0 + k * 1024;
+        v[k] = b < nblk ? partial[2 * b] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) s = mx ? fmax(s, v[k]) : s + v[k];
+    }
     s = mx ? wave_max(s) : wave_sum(s);
     if (lane == 0) sh[wave] = s;
     __syncthreads();
@@ -318,7 +327,16 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
       // member (slot 0); the first may start with the previous member
       int b0 = mblk[2 * m], b1 = mblk[2 * m + 1];
       double s = 0.0;
-      for (int b = b0 + 1 + lane; b <= b1; b += 64) s = mx ? fmax(s, partial[2 * b]) : s + partial[2 * b];
+      for (int bb = b0 + 1 + lane; bb <= b1; bb += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          int b = bb + k * 64;
+          v[k] = b <= b1 ? partial[2 * b] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) s = mx ? fmax(s, v[k]) : s + v[k];
+      }
       if (lane == 0) {
         double v = partial[2 * b0 + (bm[b0].x == m ? 0 : 1)];
         s = mx ? fmax(s, v) : s + v;
@@ -607,6 +625,66 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 // (A x)_r with the rows of A split over G = 2^lg consecutive lanes: every
 // lane of the group returns the full row sum (fixed shuffle pattern).  The
 // caller's lanes must all reach this call (no early exit before it).
+// sum_k A.val[e_k] x[A.col[e_k]] over one SELL row (e_k = base + 64 k, k < w),
+// added in ascending k.  The loads go out in groups of 8 entries (values and
+// columns first, then the gathers), so a lane has up to 8 independent gather
+// chains in flight instead of one: the sweeps are latency-bound otherwise.
+template <typename T, typename TX>
+__device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t base, int w,
+                                      const TX* __restrict__ x) {
+  constexpr int U = 8;
+  T s = (T)0;
+  for (int k0 = 0; k0 < w; k0 += U) {
+    T v[U];
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = k0 + u < w;
+      int64_t e = base + (int64_t)(k0 + u) * 64;
+      v[u] = ok ? A.val[e] : (T)0;
+      c[u] = ok ? A.col[e] : 0;
+    }
+    T xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = k0 + u < w ? (T)x[c[u]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < w) s += v[u] * xv[u];
+  }
+  return s;
+}
+
+// sum_k A.val[e_k] (z[c_k] + beta p[c_k]), same grouping and order as sell_dot
+template <typename T>
+__device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t base, int w,
+                                         const T* __restrict__ z, const T* __restrict__ p,
+                                         T beta) {
+  constexpr int U = 8;
+  T s = (T)0;
+  for (int k0 = 0; k0 < w; k0 += U) {
+    T v[U];
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = k0 + u < w;
+      int64_t e = base + (int64_t)(k0 + u) * 64;
+      v[u] = ok ? A.val[e] : (T)0;
+      c[u] = ok ? A.col[e] : 0;
+    }
+    T zv[U], pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = k0 + u < w;
+      zv[u] = ok ? z[c[u]] : (T)0;
+      pv[u] = ok ? p[c[u]] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < w) s += v[u] * (zv[u] + beta * pv[u]);
+  }
+  return s;
+}
+
 template <typename T>
 __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int lg,
                                            const T* __restrict__ x) {
@@ -615,10 +693,7 @@ __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int l
     int64_t base;
     int w;
     sell_row(A, sub, base, w);
-    for (int k = 0; k < w; k++) {
-      int64_t e = base + (int64_t)k * 64;
-      ax += A.val[e] * x[A.col[e]];
-    }
+    ax = sell_dot(A, base, w, x);
   }
   for (int o = (1 << lg) >> 1; o > 0; o >>= 1) ax += __shfl_xor(ax, o, 64);
   return ax;
@@ -683,10 +758,7 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
     int64_t base;
     int w;
     sell_row(Rs, sr, base, w);
-    for (int k = 0; k < w; k++) {
-      int64_t e = base + (int64_t)k * 64;
-      s += Rs.val[e] * rf[Rs.col[e]];
-    }
+    s = sell_dot(Rs, base, w, rf);
   }
   for (int o = (1 << lg) >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   int64_t I = sr >> lg;
@@ -710,12 +782,7 @@ __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ 
   int64_t base;
   int w;
   sell_row(P, i, base, w);
-  T s = (T)0;
-  for (int k = 0; k < w; k++) {
-    int64_t e = base + (int64_t)k * 64;
-    s += P.val[e] * xc[P.col[e]];
-  }
-  xf[i] += s;
+  xf[i] += sell_dot(P, base, w, xc);
 }
 
 // Coarsest level: x = W b with the FP64 inverse W (symmetric, so column j
@@ -796,19 +863,12 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
     int64_t base;
     int w;
     sell_row(A, i, base, w);
-    T s = (T)0, pn;
+    T s, pn;
     if (first) {
-      for (int k = 0; k < w; k++) {
-        int64_t e = base + (int64_t)k * 64;
-        s += A.val[e] * z[A.col[e]];
-      }
+      s = sell_dot(A, base, w, z);
       pn = z[i];
     } else {
-      for (int k = 0; k < w; k++) {
-        int64_t e = base + (int64_t)k * 64;
-        int32_t c = A.col[e];
-        s += A.val[e] * (z[c] + beta * pold[c]);
-      }
+      s = sell_dot_zp(A, base, w, z, pold, beta);
       pn = z[i] + beta * pold[i];
     }
     pnew[i] = pn;
@@ -924,10 +984,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
       int64_t base;
       int w;
       sell_row(A, i, base, w);
-      for (int k = 0; k < w; k++) {
-        int64_t e = base + (int64_t)k * 64;
-        ax += A.val[e] * x[A.col[e]];
-      }
+      ax = sell_dot(A, base, w, x);
     }
     double rv = (sc ? b[i] * sc[mi].bscale : b[i]) - ax;
     r[i] = rv;
