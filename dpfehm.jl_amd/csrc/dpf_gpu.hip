@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -50,21 +51,76 @@ static double now_wall_ms() {
 
 // ------------------------------------------------------------------ device buffers
 
+// Device buffers released by forward states and per-call temporaries, kept
+// for reuse: hipFree synchronises the whole device and hipMalloc of large
+// buffers costs hundreds of microseconds, both per evaluation otherwise.
+// Shared (reference-counted) by a topology and its states, so that a state
+// may outlive its topology's handle.  Every call that uses a pooled buffer
+// synchronises its stream before returning, so a released buffer has no
+// work pending when it is handed out again.
+struct BufPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free;
+  size_t cached = 0, limit;
+  explicit BufPool(size_t lim) : limit(lim) {}
+  BufPool(const BufPool&) = delete;
+  BufPool& operator=(const BufPool&) = delete;
+  ~BufPool() {
+    for (auto& kv : free) (void)hipFree(kv.second);
+  }
+  // a cached buffer of at least b bytes (and at most 2b), or null
+  void* take(size_t b, size_t& got) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = free.lower_bound(b);
+    if (it == free.end() || it->first > 2 * b) return nullptr;
+    void* p = it->second;
+    got = it->first;
+    cached -= got;
+    free.erase(it);
+    return p;
+  }
+  void give(void* p, size_t b) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      if (cached + b <= limit) {
+        free.emplace(b, p);
+        cached += b;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+};
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  BufPool* pool = nullptr;  // where the memory goes back to (null: hipFree)
   DBuf() = default;
+  explicit DBuf(BufPool* bp) : pool(bp) {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() {
-    if (p) (void)hipFree(p);
+  ~DBuf() { release(); }
+  void release() {
+    if (!p) return;
+    if (pool)
+      pool->give(p, bytes);
+    else
+      (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
   }
   void alloc(size_t b) {
     if (p && bytes >= b) return;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
+    release();
     if (b == 0) b = 8;
+    if (pool) {
+      size_t got = 0;
+      if ((p = pool->take(b, got))) {
+        bytes = got;
+        return;
+      }
+    }
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       p = nullptr;
@@ -88,7 +144,8 @@ static T* upload(DBuf& d, const std::vector<T>& v) {
 
 struct SellDev {
   int64_t nrows = 0, ncols = 0, entries = 0;
-  DBuf sptr, col, rowlen;
+  DBuf sptr, col, rowlen, soff;
+  bool offsets = false;
   void up(const SellHost& h) {
     nrows = h.nrows;
     ncols = h.ncols;
@@ -97,9 +154,21 @@ struct SellDev {
     upload(col, h.col);
     upload(rowlen, h.rowlen);
   }
+  // the fine operator in its offset-slice layout (rowlen is not uploaded:
+  // a row's entries are not the first slots of an offset slice)
+  void up_fine(const SellHost& h, const FineLayout& F) {
+    nrows = h.nrows;
+    ncols = h.ncols;
+    entries = F.sptr.back();
+    upload(sptr, F.sptr);
+    upload(col, F.col);
+    upload(soff, F.soff);
+    offsets = true;
+  }
   template <typename T>
   SellD<T> view(T* val) const {
-    return SellD<T>{nrows, sptr.as<int64_t>(), col.as<int32_t>(), val};
+    return SellD<T>{nrows, sptr.as<int64_t>(), col.as<int32_t>(), val,
+                    offsets ? soff.as<int32_t>() : nullptr};
   }
 };
 
@@ -168,6 +237,8 @@ struct dpf_topology {
   std::mutex mu;
   hipStream_t own_stream = nullptr;
   int device = 0;
+  std::shared_ptr<BufPool> pool;       // reused state and temporary buffers
+  hipEvent_t ev_setup[2] = {nullptr, nullptr};  // GPU time of a call's numeric setup
   // topology arrays on the device
   DBuf edges, aol, n2f, f2n, inc_ptr, inc_edge, inc_other, inc_slot;
   std::vector<std::unique_ptr<LevelDev>> lev;
@@ -197,6 +268,8 @@ struct dpf_topology {
   int seq = 0;                    // solve / residual sequence numbers
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
+  // fine operator layout: offset slices, and nonzeros whose column is stored
+  int64_t fine_offset_slices = 0, fine_explicit_nnz = 0;
   double cheb_ratio = 30.0;
   // Chebyshev degrees of the finest / coarser levels per precision (FP64,
   // FP32), default 2/2 until tuned, and the tuner's record
@@ -218,6 +291,7 @@ struct dpf_topology {
 
 struct dpf_state {
   dpf_topology* topo;
+  std::shared_ptr<BufPool> pool;  // declared before the buffers: outlives them
   uint64_t version;  // topo->version of the forward solve
   DBuf Ke, dh, hfree, lam, delta, Q;
   int precision;
@@ -228,7 +302,12 @@ struct dpf_state {
   std::vector<double> qs;  // Q scale per time level (nsteps + 1)
   DBuf svec;               // S = Ss V on free cells
   DBuf ckpt;               // checkpointed states u_0, u_ck, u_2ck, ...
+  explicit dpf_state(dpf_topology* t);
 };
+
+dpf_state::dpf_state(dpf_topology* t) : topo(t), pool(t->pool), version(0), precision(0) {
+  for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt}) b->pool = pool.get();
+}
 
 namespace {
 
@@ -683,6 +762,11 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     }
     HIPTRY(hipGetDevice(&t->device));
     HIPTRY(hipStreamCreateWithFlags(&t->own_stream, hipStreamNonBlocking));
+    for (auto& ev : t->ev_setup) HIPTRY(hipEventCreate(&ev));
+    // room for a few forward states (K_e, dh, Q, heads, multipliers) and the
+    // per-call temporaries of the host-memory interface
+    size_t state_bytes = (size_t)(t->H.ne + 3 * t->H.n + 2 * t->H.nf) * sizeof(double);
+    t->pool = std::make_shared<BufPool>(std::max<size_t>(size_t(1) << 30, 4 * state_bytes));
     TopologyHost& H = t->H;
     upload(t->edges, H.edges);
     upload(t->aol, H.aol);
@@ -691,13 +775,29 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     upload(t->inc_ptr, H.inc_ptr);
     upload(t->inc_edge, H.inc_edge);
     upload(t->inc_other, H.inc_other);
-    upload(t->inc_slot, H.inc_slot);
+    // the fine operator goes to the GPU in the offset-slice layout when the
+    // hierarchy has coarse levels (a one-level hierarchy inverts it densely,
+    // which walks rows through rowlen)
+    FineLayout F;
+    bool fine_offsets = H.levels.size() >= 2;
+    if (fine_offsets) {
+      build_fine_layout(H, F);
+      t->fine_offset_slices = F.offset_slices;
+      t->fine_explicit_nnz = F.explicit_nnz;
+    } else {
+      t->fine_explicit_nnz = H.levels[0].A.nnz;
+    }
+    upload(t->inc_slot, fine_offsets ? F.inc_slot : H.inc_slot);
     int64_t maxrows = 0;
     for (size_t l = 0; l < H.levels.size(); l++) {
       auto d = std::make_unique<LevelDev>();
       LevelHost& lh = H.levels[l];
-      d->A.up(lh.A);
-      upload(d->diag_slot, lh.diag_slot);
+      bool fl = l == 0 && fine_offsets;
+      if (fl)
+        d->A.up_fine(lh.A, F);
+      else
+        d->A.up(lh.A);
+      upload(d->diag_slot, fl ? F.diag_slot : lh.diag_slot);
       d->as_lg = lh.as_lg;
       if (lh.as_lg > 0) {
         d->As.up(lh.As);
@@ -711,11 +811,11 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
         d->Rs.up(lh.Rs);
         d->rs_lg = lh.rs_lg;
         d->nc = lh.R.nrows;
-        upload(d->p_slot, lh.p_slot);
+        upload(d->p_slot, fl ? F.p_slot : lh.p_slot);
         upload(d->p_self, lh.p_self);
         upload(d->rs_src, lh.rs_src);
         upload(d->ap_off, lh.ap_off);
-        upload(d->ap_a, lh.ap_a);
+        upload(d->ap_a, fl ? F.ap_a : lh.ap_a);
         upload(d->ap_p, lh.ap_p);
         upload(d->rap_off, lh.rap_off);
         upload(d->rap_dest, lh.rap_dest);
@@ -863,6 +963,8 @@ int dpf_topology_destroy(dpf_topology* t) {
   {
     std::lock_guard<std::mutex> g(t->mu);
     if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
+    for (auto& ev : t->ev_setup)
+      if (ev) (void)hipEventDestroy(ev);
     if (t->scal_host) (void)hipHostFree(t->scal_host);
     if (t->hm_host) (void)hipHostFree(t->hm_host);
     if (t->hm_outer_host) (void)hipHostFree(t->hm_outer_host);
@@ -1277,15 +1379,14 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
       HIPTRY(hipMemcpyAsync(t->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
     }
+    HIPTRY(hipEventRecord(t->ev_setup[0], s));
     prepare(t, s, o->precision, t->Q.as<double>(), true);
-    HIPTRY(hipStreamSynchronize(s));
-    double t1 = now_ms();
+    HIPTRY(hipEventRecord(t->ev_setup[1], s));
     std::unique_ptr<dpf_state> st;
     double* hfree;
-    DBuf tmp;
+    DBuf tmp(t->pool.get());
     if (state_out) {
-      st = std::make_unique<dpf_state>();
-      st->topo = t;
+      st = std::make_unique<dpf_state>(t);
       st->hfree.alloc(nf * sizeof(double));
       hfree = st->hfree.as<double>();
     } else {
@@ -1293,7 +1394,6 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       hfree = tmp.as<double>();
     }
     int status = solve_system(t, s, t->b64.as<double>(), hfree, o, stats);
-    double t2 = now_ms();
     double* hdev = h_out;
     if (o->memory == DPF_HOST) {
       t->stage_n2.alloc(n * sizeof(double));
@@ -1316,9 +1416,11 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
     }
     HIPTRY(hipStreamSynchronize(s));
     if (stats) {
-      stats->setup_ms = t1 - t0;
-      stats->solve_ms = t2 - t1;
+      float setup = 0.f;
+      HIPTRY(hipEventElapsedTime(&setup, t->ev_setup[0], t->ev_setup[1]));
       stats->total_ms = now_ms() - t0;
+      stats->setup_ms = setup;
+      stats->solve_ms = stats->total_ms - setup;
     }
     if (status == DPF_ENOCONV) set_err("AMG-PCG did not reach the tolerance within maxiter");
     if (status == DPF_EBREAKDOWN) set_err("AMG-PCG breakdown (non-SPD operator or NaN input)");
@@ -1369,7 +1471,7 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
     double* dK = dKs;
     double* dd = ddh;
     double* dq = dQs;
-    DBuf bk, bd, bq;
+    DBuf bk(t->pool.get()), bd(t->pool.get()), bq(t->pool.get());
     if (o->memory == DPF_HOST) {
       if (dKs) { bk.alloc(ne * sizeof(double)); dK = bk.as<double>(); }
       if (ddh) { bd.alloc(n * sizeof(double)); dd = bd.as<double>(); }
@@ -1423,6 +1525,8 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
   LevelDev& d0 = *t->lev[0];
   LevelNum<T>& n0 = h.lev[0];
   int64_t nf = t->H.nf, nnz = t->H.levels[0].A.nnz;
+  // column bytes only where the layout stores columns
+  double cb = 4.0 * (double)t->fine_explicit_nnz;
   unsigned g = nblocks(nf);
   const int* nodone = t->zero_flag.as<int>();
   double* part = t->partial.as<double>();
@@ -1464,11 +1568,11 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (kernel == 0)
-    *bytes = (double)nnz * (w + 4.0) + 6.0 * (double)nf * w;
+    *bytes = (double)nnz * w + cb + 6.0 * (double)nf * w;
   else if (kernel == 1)
-    *bytes = (double)nnz * (w + 4.0) + 4.0 * (double)nf * w;
+    *bytes = (double)nnz * w + cb + 4.0 * (double)nf * w;
   else
-    *bytes = (double)nnz * 12.0 + 3.0 * (double)nf * 8.0;
+    *bytes = (double)nnz * 8.0 + cb + 3.0 * (double)nf * 8.0;
   return (double)ms / reps;
 }
 
@@ -1570,8 +1674,7 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
     double t0 = now_ms();
     hipStream_t s = pick_stream(t, o);
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
-    auto st = std::make_unique<dpf_state>();
-    st->topo = t;
+    auto st = std::make_unique<dpf_state>(t);
     st->kind = 1;
     st->precision = o->precision;
     st->nsteps = nsteps;
@@ -1581,7 +1684,7 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
     st->dh.alloc(n * sizeof(double));
     st->Q.alloc(n * sizeof(double));
     st->svec.alloc(nf * sizeof(double));
-    DBuf ssb, vb, u0b;
+    DBuf ssb(t->pool.get()), vb(t->pool.get()), u0b(t->pool.get());
     const double *ss_d, *v_d, *u0_d;
     st->qs.resize(nsteps + 1);
     if (o->memory == DPF_DEVICE) {
@@ -1611,7 +1714,7 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
     st->ckpt.alloc((size_t)nck * nf * sizeof(double));
     double* ck = st->ckpt.as<double>();
     copy_dev(ck, u0_d, nf * sizeof(double), s);
-    DBuf ua, ub;
+    DBuf ua(t->pool.get()), ub(t->pool.get());
     ua.alloc(nf * sizeof(double));
     ub.alloc(nf * sizeof(double));
     copy_dev(ua.p, u0_d, nf * sizeof(double), s);
@@ -1684,7 +1787,7 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     const double* gN = stage_in(t, o, t->stage_f, dJ_du_final, nf, s);
     double* dK = dKs;
     double* dQ = dQs;
-    DBuf bk, bq;
+    DBuf bk(t->pool.get()), bq(t->pool.get());
     if (o->memory == DPF_HOST) {
       bk.alloc(ne * sizeof(double));
       dK = bk.as<double>();
@@ -1694,7 +1797,8 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
       }
     }
     int N = st->nsteps, C = st->ck;
-    DBuf seg, mua, mub, rhsb;
+    BufPool* bp = t->pool.get();
+    DBuf seg(bp), mua(bp), mub(bp), rhsb(bp);
     seg.alloc((size_t)(C + 1) * nf * sizeof(double));
     mua.alloc(nf * sizeof(double));
     mub.alloc(nf * sizeof(double));

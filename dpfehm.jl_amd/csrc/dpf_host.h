@@ -94,6 +94,26 @@ struct TopologyHost {
   double setup_ms = 0;              // host time of the symbolic setup
 };
 
+// GPU layout of the fine operator.  A 64-row slice whose nonzeros all sit at
+// no more than 8 distinct offsets o = col - row (with row + o inside the
+// matrix for every lane) is an offset slice: slot k of every lane holds the
+// entry at offset soff[9s + 1 + k] (0 where the row has none), and no column
+// is stored; soff[9s] = the slice's width (0: an explicit slice, stored as in
+// the host layout).  Regular grids make almost every slice an offset slice,
+// which halves the FP32 operator's bytes.  Entries keep their ascending
+// column order inside a row, so every accumulation over a row's entries runs
+// in the same order as on the host layout.  The maps that address fine
+// entries (inc_slot, diag_slot, p_slot, ap_a) are translated to the layout.
+struct FineLayout {
+  std::vector<int64_t> sptr;      // nslices + 1
+  std::vector<int32_t> col;       // explicit slices' columns (offset slices: unused)
+  std::vector<int32_t> soff;      // 9 per slice
+  std::vector<uint16_t> inc_slot, diag_slot, p_slot;
+  std::vector<int32_t> ap_a;
+  int64_t offset_slices = 0, explicit_nnz = 0;
+};
+void build_fine_layout(const TopologyHost& T, FineLayout& F);
+
 // Both return "" on success or an error message.
 std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
                            const double* aol, int64_t nd, const int64_t* dn);

@@ -538,6 +538,104 @@ void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& s
     for (int k = 0; k < S.rowlen[r]; k++) src[S.idx(r, k)] = si[rp[r] + k];
 }
 
+void build_fine_layout(const TopologyHost& T, FineLayout& F) {
+  const LevelHost& L = T.levels[0];
+  const SellHost& A = L.A;
+  const int64_t nr = A.nrows, ns = A.nslices();
+  // slice widths and offset sets
+  std::vector<int> width(ns, 0);
+  F.soff.assign(9 * ns, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < ns; s++) {
+    int64_t r0 = s * SELL_C, r1 = std::min<int64_t>(nr, r0 + SELL_C);
+    int ws = (int)((A.sptr[s + 1] - A.sptr[s]) / SELL_C);
+    width[s] = ws;
+    if (r1 - r0 < SELL_C) continue;
+    std::vector<int32_t> offs;
+    bool ok = true;
+    for (int64_t r = r0; r < r1 && ok; r++)
+      for (int k = 0; k < A.rowlen[r] && ok; k++) {
+        int32_t o = (int32_t)(A.at(r, k) - r);
+        if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+          offs.push_back(o);
+          ok = offs.size() <= 8;
+        }
+      }
+    if (!ok || offs.empty()) continue;
+    std::sort(offs.begin(), offs.end());
+    for (int32_t o : offs)
+      if (r0 + o < 0 || r1 - 1 + o >= A.ncols) ok = false;
+    if (!ok) continue;
+    F.soff[9 * s] = (int32_t)offs.size();
+    for (size_t k = 0; k < offs.size(); k++) F.soff[9 * s + 1 + k] = offs[k];
+    width[s] = (int)offs.size();
+  }
+  F.sptr.assign(ns + 1, 0);
+  F.offset_slices = 0;
+  F.explicit_nnz = 0;
+  for (int64_t s = 0; s < ns; s++) {
+    F.sptr[s + 1] = F.sptr[s] + (int64_t)width[s] * SELL_C;
+    if (F.soff[9 * s]) {
+      F.offset_slices++;
+    } else {
+      for (int64_t r = s * SELL_C; r < std::min<int64_t>(nr, (s + 1) * SELL_C); r++)
+        F.explicit_nnz += A.rowlen[r];
+    }
+  }
+  // host entry -> device entry (-1 for host padding), and the slot map per row
+  std::vector<int64_t> emap(A.entries(), -1);
+  F.col.assign(F.sptr[ns], 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < ns; s++) {
+    const int32_t* so = &F.soff[9 * s];
+    for (int64_t r = s * SELL_C; r < std::min<int64_t>(nr, (s + 1) * SELL_C); r++) {
+      int lane = (int)(r % SELL_C);
+      if (so[0]) {
+        int k2 = 0;
+        for (int k = 0; k < A.rowlen[r]; k++) {
+          int32_t o = (int32_t)(A.at(r, k) - r);
+          while (so[1 + k2] != o) k2++;  // ascending in both
+          emap[A.idx(r, k)] = F.sptr[s] + (int64_t)k2 * SELL_C + lane;
+        }
+      } else {
+        int ws = (int)((F.sptr[s + 1] - F.sptr[s]) / SELL_C);
+        for (int k = 0; k < ws; k++) {
+          int64_t e = F.sptr[s] + (int64_t)k * SELL_C + lane;
+          F.col[e] = A.at(r, k);
+          if (k < A.rowlen[r]) emap[A.idx(r, k)] = e;
+        }
+      }
+    }
+    if (!so[0])  // rows past the end of the matrix (last slice): padding columns
+      for (int64_t r = nr; r < (s + 1) * SELL_C; r++) {
+        int ws = (int)((F.sptr[s + 1] - F.sptr[s]) / SELL_C);
+        for (int k = 0; k < ws; k++)
+          F.col[F.sptr[s] + (int64_t)k * SELL_C + (r % SELL_C)] = A.at(r, k);
+      }
+  }
+  auto dslot = [&](int64_t r, int k) -> uint16_t {
+    if (k == NO_SLOT) return NO_SLOT;
+    int64_t e = emap[A.idx(r, k)];
+    return (uint16_t)((e - F.sptr[r / SELL_C] - r % SELL_C) / SELL_C);
+  };
+  F.diag_slot.resize(nr);
+  for (int64_t r = 0; r < nr; r++) F.diag_slot[r] = dslot(r, L.diag_slot[r]);
+  F.inc_slot.assign(T.inc_slot.size(), NO_SLOT);
+  for (int64_t i = 0; i < T.n; i++) {
+    int32_t j = T.n2f[i];
+    if (j < 0) continue;
+    for (int64_t p = T.inc_ptr[i]; p < T.inc_ptr[i + 1]; p++) F.inc_slot[p] = dslot(j, T.inc_slot[p]);
+  }
+  if (!L.p_slot.empty()) {
+    F.p_slot.assign(F.sptr[ns], NO_SLOT);
+    for (int64_t e = 0; e < A.entries(); e++)
+      if (emap[e] >= 0) F.p_slot[emap[e]] = L.p_slot[e];
+  }
+  F.ap_a.resize(L.ap_a.size());
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < (int64_t)L.ap_a.size(); q++) F.ap_a[q] = (int32_t)emap[L.ap_a[q]];
+}
+
 std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max,
                             const std::vector<int64_t>& member_row0) {
   auto t0 = std::chrono::steady_clock::now();

@@ -11,6 +11,7 @@ using namespace dpf;
 
 struct dpf_inspect {
   TopologyHost H;
+  FineLayout F;  // the fine operator's GPU layout (hierarchies of two or more levels)
 };
 
 static thread_local std::string g_ierr;
@@ -37,6 +38,7 @@ int dpf_inspect_create(int64_t n, int64_t nedges, const int64_t* neighbors,
   std::string e = build_topology(h->H, n, nedges, neighbors, areasoverlengths, ndirichlet,
                                  dirichletnodes);
   if (e.empty()) e = build_hierarchy(h->H, max_levels, coarse_max, 512);
+  if (e.empty() && h->H.levels.size() >= 2) build_fine_layout(h->H, h->F);
   if (!e.empty()) {
     g_ierr = e;
     delete h;
@@ -55,8 +57,17 @@ int32_t dpf_inspect_levels(const dpf_inspect* h) { return (int32_t)h->H.levels.s
 namespace {
 
 // name -> (size, copy) over the topology / level arrays
-template <typename F>
-int with_array(const TopologyHost& T, int32_t level, const std::string& s, F&& f) {
+template <typename Fn>
+int with_array(const dpf_inspect* h, int32_t level, const std::string& s, Fn&& f) {
+  const TopologyHost& T = h->H;
+  // the fine operator's GPU layout ("F." names; empty for one-level hierarchies)
+  if (s == "F.sptr") return f(h->F.sptr);
+  if (s == "F.col") return f(h->F.col);
+  if (s == "F.soff") return f(h->F.soff);
+  if (s == "F.inc_slot") return f(h->F.inc_slot);
+  if (s == "F.diag_slot") return f(h->F.diag_slot);
+  if (s == "F.p_slot") return f(h->F.p_slot);
+  if (s == "F.ap_a") return f(h->F.ap_a);
   if (s == "n2f") return f(T.n2f);
   if (s == "f2n") return f(T.f2n);
   if (s == "inc_ptr") return f(T.inc_ptr);
@@ -113,7 +124,7 @@ int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) 
     return s == "rs_lg" ? h->H.levels[level].rs_lg : h->H.levels[level].as_lg;
   }
   int64_t out = -1;
-  with_array(h->H, level, s, [&](const auto& v) {
+  with_array(h, level, s, [&](const auto& v) {
     out = (int64_t)v.size();
     return 0;
   });
@@ -122,7 +133,7 @@ int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) 
 
 int dpf_inspect_get(const dpf_inspect* h, int32_t level, const char* name, void* out,
                     int64_t cap) {
-  return with_array(h->H, level, std::string(name),
+  return with_array(h, level, std::string(name),
                     [&](const auto& v) { return put(v, out, cap); });
 }
 

@@ -66,13 +66,27 @@ struct SolveScalars {
   int pad;
 };
 
+// A SELL-64 operator on the device.  soff (fine operator only, else null):
+// per slice 9 ints, [0] = width of an offset slice (0: explicit columns),
+// [1..8] the offsets o_k, so that slot k of row r has column r + o_k.
 template <typename T>
 struct SellD {
   int64_t nrows;
   const int64_t* sptr;
   const int32_t* col;
   T* val;
+  const int32_t* soff = nullptr;
 };
+
+// The offsets of row r's slice if it is an offset slice, else null.  Every
+// lane of a wave is in the same slice, so the lookup is a scalar load.
+template <typename T>
+__device__ __forceinline__ const int32_t* slice_offsets(const SellD<T>& A, int64_t r) {
+  if (!A.soff) return nullptr;
+  int sl = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+  const int32_t* q = A.soff + 9 * (int64_t)sl;
+  return q[0] ? q + 1 : nullptr;
+}
 
 __device__ __forceinline__ int64_t logical_row() {
   // Hardware dispatches block b to XCD b % 8.  Give XCD x the contiguous
@@ -630,9 +644,10 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 // columns first, then the gathers), so a lane has up to 8 independent gather
 // chains in flight instead of one: the sweeps are latency-bound otherwise.
 template <typename T, typename TX>
-__device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t base, int w,
+__device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t r, int64_t base, int w,
                                       const TX* __restrict__ x) {
   constexpr int U = 8;
+  const int32_t* so = slice_offsets(A, r);
   T s = (T)0;
   for (int k0 = 0; k0 < w; k0 += U) {
     T v[U];
@@ -642,7 +657,7 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t base, int w,
       bool ok = k0 + u < w;
       int64_t e = base + (int64_t)(k0 + u) * 64;
       v[u] = ok ? A.val[e] : (T)0;
-      c[u] = ok ? A.col[e] : 0;
+      c[u] = ok ? (so ? (int32_t)r + so[k0 + u] : A.col[e]) : 0;
     }
     T xv[U];
 #pragma unroll
@@ -656,10 +671,11 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t base, int w,
 
 // sum_k A.val[e_k] (z[c_k] + beta p[c_k]), same grouping and order as sell_dot
 template <typename T>
-__device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t base, int w,
+__device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t r, int64_t base, int w,
                                          const T* __restrict__ z, const T* __restrict__ p,
                                          T beta) {
   constexpr int U = 8;
+  const int32_t* so = slice_offsets(A, r);
   T s = (T)0;
   for (int k0 = 0; k0 < w; k0 += U) {
     T v[U];
@@ -669,7 +685,7 @@ __device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t base, int w,
       bool ok = k0 + u < w;
       int64_t e = base + (int64_t)(k0 + u) * 64;
       v[u] = ok ? A.val[e] : (T)0;
-      c[u] = ok ? A.col[e] : 0;
+      c[u] = ok ? (so ? (int32_t)r + so[k0 + u] : A.col[e]) : 0;
     }
     T zv[U], pv[U];
 #pragma unroll
@@ -693,7 +709,7 @@ __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int l
     int64_t base;
     int w;
     sell_row(A, sub, base, w);
-    ax = sell_dot(A, base, w, x);
+    ax = sell_dot(A, sub, base, w, x);
   }
   for (int o = (1 << lg) >> 1; o > 0; o >>= 1) ax += __shfl_xor(ax, o, 64);
   return ax;
@@ -758,7 +774,7 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
     int64_t base;
     int w;
     sell_row(Rs, sr, base, w);
-    s = sell_dot(Rs, base, w, rf);
+    s = sell_dot(Rs, sr, base, w, rf);
   }
   for (int o = (1 << lg) >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   int64_t I = sr >> lg;
@@ -782,7 +798,7 @@ __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ 
   int64_t base;
   int w;
   sell_row(P, i, base, w);
-  xf[i] += sell_dot(P, base, w, xc);
+  xf[i] += sell_dot(P, i, base, w, xc);
 }
 
 // Coarsest level: x = W b with the FP64 inverse W (symmetric, so column j
@@ -865,10 +881,10 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
     sell_row(A, i, base, w);
     T s, pn;
     if (first) {
-      s = sell_dot(A, base, w, z);
+      s = sell_dot(A, i, base, w, z);
       pn = z[i];
     } else {
-      s = sell_dot_zp(A, base, w, z, pold, beta);
+      s = sell_dot_zp(A, i, base, w, z, pold, beta);
       pn = z[i] + beta * pold[i];
     }
     pnew[i] = pn;
@@ -984,7 +1000,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
       int64_t base;
       int w;
       sell_row(A, i, base, w);
-      ax = sell_dot(A, base, w, x);
+      ax = sell_dot(A, i, base, w, x);
     }
     double rv = (sc ? b[i] * sc[mi].bscale : b[i]) - ax;
     r[i] = rv;

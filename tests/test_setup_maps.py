@@ -249,3 +249,99 @@ def test_batch_packing_keeps_members_apart(pkg):
             member = owner
     finally:
         I.close()
+
+
+def _device_entries(F_sptr, F_col, F_soff, nrows):
+    """(row, col) of every entry of the fine operator's GPU layout."""
+    rows, cols, ents, slots, offs = [], [], [], [], []
+    for s in range(F_sptr.size - 1):
+        w = (F_sptr[s + 1] - F_sptr[s]) // 64
+        lanes = np.arange(64)
+        r = s * 64 + lanes
+        for k in range(w):
+            e = F_sptr[s] + k * 64 + lanes
+            c = r + F_soff[9 * s + 1 + k] if F_soff[9 * s] else F_col[e]
+            keep = r < nrows
+            rows.append(r[keep]); cols.append(c[keep]); ents.append(e[keep])
+            slots.append(np.full(keep.sum(), k)); offs.append(np.full(keep.sum(), F_soff[9 * s] > 0))
+    return tuple(np.concatenate(a) for a in (rows, cols, ents, slots, offs))
+
+
+@pytest.mark.parametrize("shape,dirichlet", [((20, 18, 10), "top"), ((12, 7, 5), "last")])
+def test_fine_offset_layout(pkg, oracle, shape, dirichlet):
+    """The fine operator's GPU layout (offset slices where every entry of a
+    64-row slice sits at one of <= 8 offsets col - row): the assembly replayed
+    through the translated slot maps gives the oracle's Jacobian, and the
+    translated p_slot / ap_a address the same (row, col) entries as on the host
+    layout."""
+    w = pkg.workloads.box_problem("t", shape, (0, 0, 0), (9.5, 8.5, 4.5), lam=3.0, sigma=1.0,
+                                  field_seed=5, dirichlet=dirichlet, sources=((1, 1e-4),),
+                                  n_injectors=2, n_obs=4)
+    Ke = w.Ks()
+    I = Inspect(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes, coarse_max=16)
+    try:
+        assert I.levels >= 2
+        g = lambda name, dt: (lambda a: a)(
+            (lambda sz: (lambda out: (I.L.dpf_inspect_get(I.h, 0, name.encode(),
+                                                           out.ctypes.data, sz), out)[1])(
+                np.empty(sz, dtype=dt)))(I.L.dpf_inspect_size(I.h, 0, name.encode())))
+        Fs, Fc, Fo = g("F.sptr", np.int64), g("F.col", np.int32), g("F.soff", np.int32)
+        Finc, Fdiag = g("F.inc_slot", np.uint16), g("F.diag_slot", np.uint16)
+        Fp, Fap = g("F.p_slot", np.uint16), g("F.ap_a", np.int32)
+        A_sptr, A_col, A_len = I.get("A.sptr"), I.get("A.col"), I.get("A.rowlen")
+        p_slot, ap_a = I.get("p_slot"), I.get("ap_a")
+        f2n, inc_ptr = I.get("f2n"), I.get("inc_ptr")
+        inc_edge = I.get("inc_edge")
+    finally:
+        I.close()
+    nf = f2n.size
+    nsl = Fs.size - 1
+    noff = int((Fo[0::9] > 0).sum())
+    if dirichlet == "top":
+        assert noff >= 0.8 * nsl, (noff, nsl)
+    # (row, col) of the device entries; of the host entries
+    dr, dc, de, dk, doff = _device_entries(Fs, Fc, Fo, nf)
+    dev_rc = np.full((Fs[-1], 2), -1, dtype=np.int64)
+    dev_rc[de, 0], dev_rc[de, 1] = dr, dc
+    host_rc = np.full((A_sptr[-1], 2), -1, dtype=np.int64)
+    for r in range(nf):
+        for k in range(A_len[r]):
+            e = sell_idx(A_sptr, r, k)
+            host_rc[e] = (r, A_col[e])
+    # assembly replayed in the GPU layout
+    val = np.zeros(Fs[-1])
+    aol = w.areasoverlengths
+    for j in range(nf):
+        i = f2n[j]
+        dsum = 0.0
+        for p in range(inc_ptr[i], inc_ptr[i + 1]):
+            t = Ke[inc_edge[p]] * aol[inc_edge[p]]
+            dsum += t
+            if Finc[p] != NO_SLOT:
+                val[Fs[j // 64] + int(Finc[p]) * 64 + j % 64] -= t
+        val[Fs[j // 64] + int(Fdiag[j]) * 64 + j % 64] = dsum
+    assert (dev_rc[np.nonzero(val)[0], 0] >= 0).all()
+    A_dev = sp.csr_matrix((val[de], (dr, dc)), shape=(nf, nf))
+    one = np.ones(w.n)
+    A_ref = oracle.groundwater_h(np.zeros(nf), Ke, w.neighbors, w.areasoverlengths,
+                                 w.dirichletnodes, w.dirichleths, w.Qs, one, one)
+    assert abs(A_dev - A_ref).max() <= 1e-14 * abs(A_ref).max()
+    # every host nonzero appears once in the GPU layout, at the same (row, col)
+    key = lambda rc: rc[:, 0] * nf + rc[:, 1]
+    real_host = host_rc[:, 0] >= 0
+    hk = key(host_rc[real_host])
+    dkeys = key(dev_rc[de])
+    # real device entries: the first rowlen slots of an explicit slice; the
+    # host's nonzeros among an offset slice's (row, row + offset) pairs
+    real_dev = np.where(doff, np.isin(dkeys, hk), dk < A_len[dr])
+    assert np.unique(dkeys[real_dev]).size == real_dev.sum() == hk.size
+    assert np.array_equal(np.sort(dkeys[real_dev]), np.sort(hk))
+    assert (val[de[~real_dev]] == 0).all()
+    pos = {int(k): int(e) for k, e in zip(dkeys[real_dev], de[real_dev])}
+    emap = np.full(host_rc.shape[0], -1, dtype=np.int64)
+    emap[np.nonzero(real_host)[0]] = [pos[int(k)] for k in hk]
+    assert np.array_equal(dev_rc[Fap], host_rc[ap_a])
+    assert np.array_equal(Fp[emap[real_host]], p_slot[real_host])
+    holes = np.ones(Fs[-1], dtype=bool)
+    holes[emap[real_host]] = False
+    assert (Fp[holes] == NO_SLOT).all()
