@@ -20,6 +20,8 @@
 namespace dpf {
 
 constexpr int BLK = 256;
+// coarsest levels up to this many rows are inverted in LDS (nc^2 doubles <= 64 KiB)
+constexpr int DENSE_LDS_MAX = 90;
 constexpr uint16_t NO_SLOT_D = 0xFFFF;
 
 // Host-visible mirror of the solve's progress, in pinned coherent host
@@ -519,23 +521,49 @@ __global__ void k_residuals(int64_t nf, const int32_t* __restrict__ f2n,
 // P(i,:) = e_agg(i) - w invd_i sum_k a_ik e_agg(k), w = 4/3 (l1 diagonal,
 // whose spectral radius bound is 1).
 template <typename T>
-__global__ void k_psmooth(SellD<T> A, const T* __restrict__ invd,
-                          const uint16_t* __restrict__ p_slot,
-                          const uint16_t* __restrict__ p_self, SellD<T> P) {
+__global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict__ invd,
+                                                 const uint16_t* __restrict__ p_slot,
+                                                 const uint16_t* __restrict__ p_self, SellD<T> P) {
+  // the lane's P row accumulates in LDS ([slot][lane]: no bank conflicts)
+  // instead of through read-modify-writes of global memory
+  constexpr int PW = 16;
+  __shared__ T acc[PW][BLK];
   int64_t i = logical_row();
   if (i >= A.nrows) return;
   int64_t pb, ab;
   int pw, aw;
   sell_row(P, i, pb, pw);
   sell_row(A, i, ab, aw);
-  for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = (T)0;
   T f = (T)(4.0 / 3.0) * invd[i];
-  for (int k = 0; k < aw; k++) {
-    int64_t e = ab + (int64_t)k * 64;
-    uint16_t sl = p_slot[e];
-    if (sl != NO_SLOT_D) P.val[pb + (int64_t)sl * 64] -= f * A.val[e];
+  int tid = threadIdx.x;
+  if (pw <= PW) {
+    for (int k = 0; k < pw; k++) acc[k][tid] = (T)0;
+    constexpr int U = 8;
+    for (int k0 = 0; k0 < aw; k0 += U) {
+      uint16_t sl[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        int64_t e = ab + (int64_t)(k0 + u) * 64;
+        bool ok = k0 + u < aw;
+        sl[u] = ok ? p_slot[e] : NO_SLOT_D;
+        v[u] = ok ? A.val[e] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (sl[u] != NO_SLOT_D) acc[sl[u]][tid] -= f * v[u];
+    }
+    acc[p_self[i]][tid] += (T)1;
+    for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = acc[k][tid];
+  } else {
+    for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = (T)0;
+    for (int k = 0; k < aw; k++) {
+      int64_t e = ab + (int64_t)k * 64;
+      uint16_t sl = p_slot[e];
+      if (sl != NO_SLOT_D) P.val[pb + (int64_t)sl * 64] -= f * A.val[e];
+    }
+    P.val[pb + (int64_t)p_self[i] * 64] += (T)1;
   }
-  P.val[pb + (int64_t)p_self[i] * 64] += (T)1;
 }
 
 // R = P': every R entry copies the P entry it mirrors.
@@ -566,8 +594,28 @@ __global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict_
   int G = 1 << lg;
   T acc = (T)0;
   if (k < nlists) {
+    // this lane's products q = off[k] + g, + G, ... in order, loads in groups of 8
+    constexpr int U = 8;
     int64_t q1 = off[k + 1];
-    for (int64_t q = off[k] + (sub & (G - 1)); q < q1; q += G) acc += X[s1[q]] * Y[s2[q]];
+    for (int64_t q0 = off[k] + (sub & (G - 1)); q0 < q1; q0 += (int64_t)U * G) {
+      int32_t a[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        int64_t q = q0 + (int64_t)u * G;
+        a[u] = q < q1 ? s1[q] : 0;
+        b[u] = q < q1 ? s2[q] : 0;
+      }
+      T xa[U], yb[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        bool ok = q0 + (int64_t)u * G < q1;
+        xa[u] = ok ? X[a[u]] : (T)0;
+        yb[u] = ok ? Y[b[u]] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q0 + (int64_t)u * G < q1) acc += xa[u] * yb[u];
+    }
   }
   for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((sub & (G - 1)) == 0 && k < nlists) out[dest ? dest[k] : k] = acc;
@@ -584,8 +632,15 @@ __global__ void k_rowdiag(SellD<T> A, const uint16_t* __restrict__ diag_slot,
   sell_row(A, i, base, w);
   int ds = diag_slot[i];
   T l1 = A.val[base + (int64_t)ds * 64];
-  for (int k = 0; k < w; k++)
-    if (k != ds) l1 += fabs(A.val[base + (int64_t)k * 64]);
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < w; k0 += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = k0 + u < w ? A.val[base + (int64_t)(k0 + u) * 64] : (T)0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u < w && k0 + u != ds) l1 += fabs(v[u]);
+  }
   invd[i] = (T)1 / l1;
 }
 
@@ -596,41 +651,61 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
                                                         double* __restrict__ W, int* status) {
   int nc = (int)Ac.nrows;
   int tid = threadIdx.x, nt = blockDim.x;
-  for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) W[t] = 0.0;
+  // up to DENSE_LDS_MAX rows the elimination runs in LDS and W is written once
+  extern __shared__ double wl[];
+  double* Wk = nc <= DENSE_LDS_MAX ? wl : W;
+  for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) Wk[t] = 0.0;
   __syncthreads();
   for (int I = tid; I < nc; I += nt) {
     int64_t cb;
     int cw;
     sell_row(Ac, I, cb, cw);
     for (int k = 0; k < rowlen[I]; k++)
-      W[(int64_t)I * nc + Ac.col[cb + (int64_t)k * 64]] += (double)Ac.val[cb + (int64_t)k * 64];
+      Wk[(int64_t)I * nc + Ac.col[cb + (int64_t)k * 64]] += (double)Ac.val[cb + (int64_t)k * 64];
   }
   __syncthreads();
   __shared__ int bad;
   if (tid == 0) bad = 0;
   __syncthreads();
+  // on the LDS path each thread owns at most 8 fixed positions of W
+  const bool small = Wk != W;
+  int pi[8], pj[8], np = 0;
+  if (small)
+    for (int t = tid; t < nc * nc && np < 8; t += nt, np++) {
+      pi[np] = t / nc;
+      pj[np] = t - pi[np] * nc;
+    }
   for (int k = 0; k < nc; k++) {
-    double piv = W[(int64_t)k * nc + k];
+    double piv = Wk[(int64_t)k * nc + k];
     if (!(piv > 0.0)) {
       if (tid == 0) bad = 1;
       piv = 1.0;
     }
     double ip = 1.0 / piv;
-    for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) {
-      int i = (int)(t / nc), j = (int)(t % nc);
-      if (i != k && j != k) W[t] -= W[(int64_t)i * nc + k] * W[(int64_t)k * nc + j] * ip;
+    if (small) {
+      for (int q = 0; q < np; q++) {
+        int i = pi[q], j = pj[q];
+        if (i != k && j != k) Wk[i * nc + j] -= Wk[i * nc + k] * Wk[k * nc + j] * ip;
+      }
+    } else {
+      for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) {
+        int i = (int)(t / nc), j = (int)(t % nc);
+        if (i != k && j != k) Wk[t] -= Wk[(int64_t)i * nc + k] * Wk[(int64_t)k * nc + j] * ip;
+      }
     }
     __syncthreads();
     for (int t = tid; t < nc; t += nt) {
       if (t != k) {
-        W[(int64_t)k * nc + t] *= ip;
-        W[(int64_t)t * nc + k] *= -ip;
+        Wk[(int64_t)k * nc + t] *= ip;
+        Wk[(int64_t)t * nc + k] *= -ip;
       }
     }
     __syncthreads();
-    if (tid == 0) W[(int64_t)k * nc + k] = ip;
+    if (tid == 0) Wk[(int64_t)k * nc + k] = ip;
     __syncthreads();
   }
+  if (Wk != W)
+    for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) W[t] = Wk[t];
   if (tid == 0) status[0] = bad;
 }
 
