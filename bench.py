@@ -210,7 +210,7 @@ def main():
     achieved = by / (ms * 1e-3) / 1e9
 
     tuned = topo.tuned_degrees()
-    log(f"[bench] tuned Chebyshev degrees (finest, coarse): {tuned}")
+    log(f"[bench] tuned Chebyshev degrees (finest, coarse): {tuned}; layout {topo.layout()}")
     value = args.steps / results["mixed"]["seconds"]
     fp64_rate = args.steps / results["fp64"]["seconds"]
 

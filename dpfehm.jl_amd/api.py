@@ -110,9 +110,14 @@ class Topology:
         self.lock = threading.Lock()
 
     def info(self):
-        info = np.zeros(12, dtype=np.int64)
-        _lib.lib().dpf_topology_info(self._h, _ptr(info), 12)
+        info = np.zeros(13, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 13)
         return info
+
+    def layout(self):
+        """{'fine_offset_slices': k}: how many of the fine operator's 64-row
+        slices store column offsets instead of column indices."""
+        return {"fine_offset_slices": int(self.info()[12])}
 
     def tuned_degrees(self):
         """{'fp64': (finest, coarse), 'mixed': (finest, coarse)} once tuned."""

@@ -984,7 +984,7 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   const TopologyHost& H = t->H;
   int64_t nnz_all = 0;
   for (auto& l : H.levels) nnz_all += l.A.nnz;
-  int64_t v[12] = {H.n,
+  int64_t v[13] = {H.n,
                   H.ne,
                   H.nf,
                   H.levels[0].A.nnz,
@@ -995,8 +995,9 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
                   t->tuned[0] ? t->deg_fine[0] : 0,
                   t->tuned[0] ? t->deg_coarse[0] : 0,
                   t->tuned[1] ? t->deg_fine[1] : 0,
-                  t->tuned[1] ? t->deg_coarse[1] : 0};
-  for (int i = 0; i < ninfo && i < 12; i++) info[i] = v[i];
+                  t->tuned[1] ? t->deg_coarse[1] : 0,
+                  t->fine_offset_slices};
+  for (int i = 0; i < ninfo && i < 13; i++) info[i] = v[i];
   return DPF_OK;
 }
 

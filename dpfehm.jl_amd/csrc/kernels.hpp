@@ -402,9 +402,12 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
   if (mode == FIN_START) mpublish(&hm->seq, seq);
 }
 
+// Start and width of row r's SELL entries.  Every caller runs the 64 rows of
+// one slice in one wavefront, so the slice index is wave-uniform and its
+// pointer loads are scalar loads.
 template <typename T>
 __device__ __forceinline__ void sell_row(const SellD<T>& A, int64_t r, int64_t& base, int& w) {
-  int64_t s = r >> 6;
+  int s = __builtin_amdgcn_readfirstlane((int)(r >> 6));
   int64_t p0 = A.sptr[s];
   base = p0 + (r & 63);
   w = (int)((A.sptr[s + 1] - p0) >> 6);

@@ -110,10 +110,11 @@ int dpf_batch_topology_create(int32_t nmembers, const int64_t *ncells, const int
  * single system). */
 int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers);
 
-/* info[0..11] = n, nedges, nfree, nnz(A), levels, coarse rows,
- * operator-complexity*1000, host symbolic setup time in microseconds, then the
+/* info[0..12] = n, nedges, nfree, nnz(A), levels, coarse rows,
+ * operator-complexity*1000, host symbolic setup time in microseconds, the
  * tuned Chebyshev degrees (finest, coarse) for FP64 and for mixed (0 = not
- * tuned yet). */
+ * tuned yet), and the fine operator's 64-row slices stored as column offsets
+ * (no column indices). */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
