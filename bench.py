@@ -146,7 +146,7 @@ def main():
         log("[bench] calibration copy: 3 x 256 MiB read + 256 MiB written")
 
     results = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed_pc"):
         o = opts(precision)
         for k in range(args.warmup):
             evaluate(k, o, [])
@@ -165,13 +165,18 @@ def main():
 
     # accuracy of the mixed path against FP64 on the same field (untimed)
     acc = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed_pc"):
         evaluate(0, opts(precision), [])
         torch.cuda.synchronize()
         acc[precision] = (h_d.cpu().numpy().copy(), dlogK_d.cpu().numpy().copy())
-    head_err = float(np.linalg.norm(acc["mixed"][0] - acc["fp64"][0]) / np.linalg.norm(acc["fp64"][0]))
-    grad_err = float(np.linalg.norm(acc["mixed"][1] - acc["fp64"][1]) / np.linalg.norm(acc["fp64"][1]))
-    log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}")
+
+    def rel_err(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    head_err = rel_err(acc["mixed"][0], acc["fp64"][0])
+    grad_err = rel_err(acc["mixed"][1], acc["fp64"][1])
+    pc_err = (rel_err(acc["mixed_pc"][0], acc["fp64"][0]), rel_err(acc["mixed_pc"][1], acc["fp64"][1]))
+    log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}; "
+        f"mixed_pc vs fp64: {pc_err[0]:.2e}, {pc_err[1]:.2e}")
 
     # roofline of the dominant kernel, the finest-level FP32 Chebyshev sweep of
     # the mixed path: the same timed loop again with a hipEvent pair around
@@ -241,6 +246,8 @@ def main():
         },
         "fp64_evals_per_s": round(fp64_rate, 3),
         "mixed_over_fp64": round(value / fp64_rate, 3),
+        "mixed_pc_evals_per_s": round(args.steps / results["mixed_pc"]["seconds"], 3),
+        "mixed_pc_rel_err_head_grad": pc_err,
         "head_rel_err_mixed_vs_fp64": head_err,
         "grad_rel_err_mixed_vs_fp64": grad_err,
         "iterations_mixed_fwd_adj": [results["mixed"]["stats"][0][0]["iterations"],
@@ -347,7 +354,7 @@ def bench_ensemble(args, P, api, _lib, torch):
         return st.as_dict(), st2.as_dict()
 
     res = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed_pc"):
         o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
                       stream=sptr)
         for _ in range(args.warmup):
@@ -374,6 +381,7 @@ def bench_ensemble(args, P, api, _lib, torch):
                                "forward+adjoint to 1e-10 per member",
                    "cells": topo.n, "nfree": topo.nfree, "amg_levels": topo.levels},
         "fp64_member_evals_per_s": round(len(ms) * args.steps / res["fp64"][0], 2),
+        "mixed_pc_member_evals_per_s": round(len(ms) * args.steps / res["mixed_pc"][0], 2),
         "worst_member_relres": {k: v[2] for k, v in res.items()},
         "topology_setup_s": round(topo_s, 2),
     }
@@ -409,7 +417,7 @@ def bench_transient(args, P, api, _lib, torch):
         return ctypes.c_void_p(t.data_ptr())
 
     out = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed_pc"):
         o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
                       stream=sptr)
         st, sh = _lib.Stats(), ctypes.c_void_p()

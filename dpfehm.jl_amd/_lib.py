@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdpfgpu.so")
 
 DPF_OK, DPF_EINVAL, DPF_EHIP, DPF_ENOCONV, DPF_EBREAKDOWN, DPF_ESINGULAR, DPF_ENOMEM = range(7)
-DPF_FP64, DPF_MIXED = 0, 1
+DPF_FP64, DPF_MIXED, DPF_MIXED_PC = 0, 1, 2
 DPF_HOST, DPF_DEVICE = 0, 1
 DPF_FLAG_SWEEP_TIMING = 1
 
@@ -127,6 +127,6 @@ def default_opts(**kw):
     lib().dpf_opts_init(ctypes.byref(o))
     for k, v in kw.items():
         if k == "precision" and isinstance(v, str):
-            v = {"fp64": DPF_FP64, "mixed": DPF_MIXED}[v]
+            v = {"fp64": DPF_FP64, "mixed": DPF_MIXED, "mixed_pc": DPF_MIXED_PC}[v]
         setattr(o, k, v)
     return o

@@ -110,8 +110,8 @@ class Topology:
         self.lock = threading.Lock()
 
     def info(self):
-        info = np.zeros(13, dtype=np.int64)
-        _lib.lib().dpf_topology_info(self._h, _ptr(info), 13)
+        info = np.zeros(15, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 15)
         return info
 
     def layout(self):
@@ -127,6 +127,8 @@ class Topology:
             out["fp64"] = (int(i[8]), int(i[9]))
         if i[10]:
             out["mixed"] = (int(i[10]), int(i[11]))
+        if i[13]:
+            out["mixed_pc"] = (int(i[13]), int(i[14]))
         return out
 
     @property

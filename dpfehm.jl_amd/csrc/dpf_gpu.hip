@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/dpfgpu.h"
@@ -274,10 +275,11 @@ struct dpf_topology {
   double cheb_ratio = 30.0;
   // Chebyshev degrees of the finest / coarser levels per precision (FP64,
   // FP32), default 2/2 until tuned, and the tuner's record
-  int deg_fine[2] = {2, 2}, deg_coarse[2] = {2, 2};
-  bool tuned[2] = {false, false};
-  double tune_ms[2][5] = {};
-  int tune_iters[2][5] = {};
+  // (index = precision: FP64, mixed refinement, FP64 Krylov with FP32 V-cycle)
+  int deg_fine[3] = {2, 2, 2}, deg_coarse[3] = {2, 2, 2};
+  bool tuned[3] = {false, false, false};
+  double tune_ms[3][5] = {};
+  int tune_iters[3][5] = {};
   // optional event timing of every finest-level smoothing sweep (opts->flags)
   bool timing = false;
   std::vector<hipEvent_t> evpool;
@@ -627,35 +629,43 @@ struct SolveResult {
   double rr, rr0;
 };
 
-// PCG on the fine level in precision T: solves A x = b where b (length nf,
-// device) is h.cr's source; x ends in h.cx.  Blocking (polls the device
-// convergence flag in chunks of iterations).
-template <typename T>
-SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double reltol,
-                double abstol, int maxiter, const std::vector<Cheb>& chs, int abs_target = 0) {
+// PCG on the fine level: solves A x = b (b length nf, device); x ends in
+// hk.cx.  The Krylov vectors, the operator A and its SpMV are in TK (from
+// hk); the V-cycle preconditioner runs in TP on hp's hierarchy.  TK == TP
+// (hk and hp the same hierarchy) is the plain FP64 or FP32 solve; TK =
+// double, TP = float is the FP64 Krylov method with an FP32 V-cycle, whose
+// input is the FP64 residual rounded to FP32 (in hp.cr).  Blocking (polls the
+// device convergence flag through the host mirror).
+template <typename TK, typename TP>
+SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, const TK* b,
+                double reltol, double abstol, int maxiter, const std::vector<Cheb>& chs,
+                int abs_target = 0) {
+  constexpr bool split = !std::is_same<TK, TP>::value;
   const Cheb& ch = chs[0];
   LevelDev& d0 = *t->lev[0];
-  LevelNum<T>& n0 = h.lev[0];
+  LevelNum<TP>& n0 = hp.lev[0];
   int64_t nf = t->H.nf;
   unsigned g = nblocks(nf);
   double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   int* done = t->gdone.as<int>();
-  T* x = h.cx.template as<T>();
-  T* r = h.cr.template as<T>();
-  T* q = h.cq.template as<T>();
-  T* pb[2] = {h.cp0.template as<T>(), h.cp1.template as<T>()};
-  const T* invd = level_invd(t, h, 0);
-  SellD<T> A = d0.A.view<T>(level_val(t, h, 0));
-  T* dd = n0.d.template as<T>();
-  T* z0 = n0.x.template as<T>();
-  T* zs = n0.xs.template as<T>();
+  TK* x = hk.cx.template as<TK>();
+  TK* r = hk.cr.template as<TK>();
+  TK* q = hk.cq.template as<TK>();
+  TK* pb[2] = {hk.cp0.template as<TK>(), hk.cp1.template as<TK>()};
+  TP* rp = split ? hp.cr.template as<TP>() : nullptr;  // V-cycle input when split
+  const TP* vb = split ? (const TP*)rp : (const TP*)(void*)r;
+  const TP* invd = level_invd(t, hp, 0);
+  SellD<TK> A = d0.A.view<TK>(level_val(t, hk, 0));
+  TP* dd = n0.d.template as<TP>();
+  TP* z0 = n0.x.template as<TP>();
+  TP* zs = n0.xs.template as<TP>();
   int seq = ++t->seq;
-  hipLaunchKernelGGL(k_cg_start<T>, dim3(g), dim3(BLK), 0, s, nf, b, r, x, invd, dd, z0,
-                     (T)ch.c0, bm_ptr(t), part);
+  hipLaunchKernelGGL((k_cg_start<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, b, r, x, rp, invd, dd,
+                     z0, (TP)ch.c0, bm_ptr(t), part);
   finish(t, s, FIN_START, g, reltol, abstol, maxiter, abs_target, seq);
-  T* z = nullptr;
-  vcycle<T>(t, h, s, 0, r, z0, zs, &z, chs, done, true);
+  TP* z = nullptr;
+  vcycle<TP>(t, hp, s, 0, vb, z0, zs, &z, chs, done, true);
   int par = 0;
   SolveResult res{0, 0, 0, 0};
   // Keep one iteration queued beyond the last one whose convergence test the
@@ -679,17 +689,17 @@ SolveResult pcg(dpf_topology* t, Hier<T>& h, hipStream_t s, const T* b, double r
       break;
     }
     if (launched < maxiter && launched <= its) {
-      hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)z,
-                         (const T*)pb[par], pb[par ^ 1], q, (const int*)done,
+      hipLaunchKernelGGL((k_cg_pq<TK, TP>), dim3(g), dim3(BLK), 0, s, A, (const TP*)z,
+                         (const TK*)pb[par], pb[par ^ 1], q, (const int*)done,
                          (const SolveScalars*)sc, bm_ptr(t), part);
       finish(t, s, FIN_PQ, g);
       par ^= 1;
-      hipLaunchKernelGGL(k_cg_xr<T>, dim3(g), dim3(BLK), 0, s, nf, x, r, (const T*)pb[par],
-                         (const T*)q, invd, dd, z0, (T)ch.c0, (const int*)done,
-                         (const SolveScalars*)sc, bm_ptr(t), part);
+      hipLaunchKernelGGL((k_cg_xr<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, x, r,
+                         (const TK*)pb[par], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0,
+                         (const int*)done, (const SolveScalars*)sc, bm_ptr(t), part);
       finish(t, s, FIN_XR, g);
-      T* zz = nullptr;
-      vcycle<T>(t, h, s, 0, r, z0, zs, &zz, chs, done, true);
+      TP* zz = nullptr;
+      vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
       z = zz;
       launched++;
       HIPTRY(hipGetLastError());
@@ -994,7 +1004,7 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   const TopologyHost& H = t->H;
   int64_t nnz_all = 0;
   for (auto& l : H.levels) nnz_all += l.A.nnz;
-  int64_t v[13] = {H.n,
+  int64_t v[15] = {H.n,
                   H.ne,
                   H.nf,
                   H.levels[0].A.nnz,
@@ -1006,8 +1016,10 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
                   t->tuned[0] ? t->deg_coarse[0] : 0,
                   t->tuned[1] ? t->deg_fine[1] : 0,
                   t->tuned[1] ? t->deg_coarse[1] : 0,
-                  t->fine_offset_slices};
-  for (int i = 0; i < ninfo && i < 13; i++) info[i] = v[i];
+                  t->fine_offset_slices,
+                  t->tuned[2] ? t->deg_fine[2] : 0,
+                  t->tuned[2] ? t->deg_coarse[2] : 0};
+  for (int i = 0; i < ninfo && i < 15; i++) info[i] = v[i];
   return DPF_OK;
 }
 
@@ -1043,7 +1055,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
   t->sweep_ms = 0;
   t->sweep_n = 0;
-  int pidx = o->precision == DPF_MIXED ? 1 : 0;
+  int pidx = o->precision;
   int L = t->levels();
   auto make_chs = [&](int m0, int mc) {
     std::vector<Cheb> v(L);
@@ -1119,16 +1131,31 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   }
   // Iterative refinement on x64: each step solves A d = r for the current FP64
   // residual of every member still above its target (FP64 CG straight to the
-  // target in DPF_FP64 mode; FP32 CG to inner_reltol in DPF_MIXED mode), then
-  // x64 += d.
+  // target in DPF_FP64 and DPF_MIXED_PC modes; FP32 CG to inner_reltol in
+  // DPF_MIXED mode), then x64 += d.
   int budget = o->maxiter;
-  // FP64: one CG to the full tolerance; a restart only if rounding left the
-  // true residual just above it
-  int max_outer = o->precision == DPF_FP64 ? 3 : o->max_outer;
+  // FP64 Krylov: one CG to the full tolerance; a restart only if rounding
+  // left the true residual just above it
+  int max_outer = o->precision == DPF_MIXED ? o->max_outer : 3;
   if (o->precision == DPF_MIXED) {
     ensure_hier(t, t->h32);
     t->b32.alloc(nf * sizeof(float));
   }
+  if (o->precision == DPF_MIXED_PC) {
+    ensure_hier(t, t->h32);
+    ensure_hier(t, t->h64);  // its fine-level Krylov vectors
+  }
+  // one Krylov solve of the current residual with the given smoothers
+  auto krylov = [&](const std::vector<Cheb>& cc) -> SolveResult {
+    if (o->precision == DPF_FP64)
+      return pcg<double, double>(t, t->h64, t->h64, s, t->r64.as<double>(), 0.0, 0.0, budget,
+                                 cc, 1);
+    if (o->precision == DPF_MIXED_PC)
+      return pcg<double, float>(t, t->h64, t->h32, s, t->r64.as<double>(), 0.0, 0.0, budget,
+                                cc, 1);
+    return pcg<float, float>(t, t->h32, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0,
+                             budget, cc, 0);
+  };
   while (active > 0) {
     if (outer >= max_outer || budget <= 0) {
       status = DPF_ENOCONV;
@@ -1149,11 +1176,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
         std::vector<Cheb> cc = make_chs(cand[c][0], cand[c][1]);
         HIPTRY(hipStreamSynchronize(s));
         double ta = now_ms();
-        SolveResult tr = o->precision == DPF_FP64
-                             ? pcg<double>(t, t->h64, s, t->r64.as<double>(), 0.0, 0.0, budget,
-                                           cc, 1)
-                             : pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0,
-                                          budget, cc, 0);
+        SolveResult tr = krylov(cc);
         HIPTRY(hipStreamSynchronize(s));
         double el = now_ms() - ta;
         t->tune_ms[pidx][c] = tr.status == 1 ? el : -1.0;
@@ -1168,12 +1191,11 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       tune = false;
       chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
     }
-    if (o->precision == DPF_FP64) {
-      r = pcg<double>(t, t->h64, s, t->r64.as<double>(), 0.0, 0.0, budget, chs, 1);
+    r = krylov(chs);
+    if (o->precision != DPF_MIXED) {
       hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
                          (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
     } else {
-      r = pcg<float>(t, t->h32, s, t->b32.as<float>(), o->inner_reltol, 0.0, budget, chs, 0);
       hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
                          (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc, bm_ptr(t));
     }
@@ -1218,9 +1240,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
 // Assemble + numeric refresh for the conductivities already in t->Ke.
 void prepare(dpf_topology* t, hipStream_t s, int precision, const double* Q, bool want_b,
              double tscale = 1.0, const double* svec = nullptr) {
-  assemble(t, s, precision == DPF_MIXED, Q, want_b, tscale, svec);
+  assemble(t, s, precision != DPF_FP64, Q, want_b, tscale, svec);
   t->version++;
-  if (precision == DPF_MIXED) {
+  if (precision != DPF_FP64) {
     refresh<float>(t, t->h32, s);
     t->h32.version = t->version;
   } else {
@@ -1233,7 +1255,8 @@ void prepare(dpf_topology* t, hipStream_t s, int precision, const double* Q, boo
 
 int check_opts(const dpf_opts* o) {
   if (!o) return DPF_OK;
-  if (o->precision != DPF_FP64 && o->precision != DPF_MIXED) return DPF_EINVAL;
+  if (o->precision != DPF_FP64 && o->precision != DPF_MIXED && o->precision != DPF_MIXED_PC)
+    return DPF_EINVAL;
   if (o->memory != DPF_HOST && o->memory != DPF_DEVICE) return DPF_EINVAL;
   if (!(o->reltol >= 0) || !(o->abstol >= 0) || o->maxiter <= 0) return DPF_EINVAL;
   if (o->precision == DPF_MIXED && (!(o->inner_reltol > 0 && o->inner_reltol < 1) || o->max_outer <= 0))
@@ -1463,7 +1486,7 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     // the hierarchy must hold this state's conductivities in the requested precision
     bool fresh = (t->version == st->version) &&
-                 ((o->precision == DPF_MIXED) ? t->h32.version == st->version
+                 ((o->precision != DPF_FP64) ? t->h32.version == st->version
                                               : t->h64.version == st->version);
     if (!fresh) {
       copy_dev(t->Ke.p, st->Ke.p, ne * sizeof(double), s);
@@ -1560,7 +1583,7 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
                          (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0,
                          bm_ptr(t), part);
     } else if (kernel == 1) {
-      hipLaunchKernelGGL(k_cg_pq<T>, dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
+      hipLaunchKernelGGL((k_cg_pq<T, T>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
                          h.cq.template as<T>(), nodone, (const SolveScalars*)sc, bm_ptr(t), part);
     } else {
@@ -1604,7 +1627,7 @@ extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t ke
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     hipStream_t s = pick_stream(t, o);
-    if (precision == DPF_MIXED) {
+    if (precision != DPF_FP64) {
       if (!t->h32.allocated || t->h32.version == 0) throw Err{DPF_EINVAL, "no FP32 solve yet"};
       *avg_ms = profile_one<float>(t, t->h32, s, kernel, reps, bytes);
     } else {
@@ -1630,7 +1653,7 @@ void prepare_be(dpf_topology* t, hipStream_t s, int precision, dpf_state* st) {
 
 bool hier_fresh(dpf_topology* t, dpf_state* st, int precision) {
   return t->version == st->version &&
-         (precision == DPF_MIXED ? t->h32.version == st->version : t->h64.version == st->version);
+         (precision != DPF_FP64 ? t->h32.version == st->version : t->h64.version == st->version);
 }
 
 // Error text for a failed time step: which step, which phase, how far it got.

@@ -748,9 +748,9 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t r, int64_t base
 }
 
 // sum_k A.val[e_k] (z[c_k] + beta p[c_k]), same grouping and order as sell_dot
-template <typename T>
+template <typename T, typename TZ>
 __device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t r, int64_t base, int w,
-                                         const T* __restrict__ z, const T* __restrict__ p,
+                                         const TZ* __restrict__ z, const T* __restrict__ p,
                                          T beta) {
   constexpr int U = 8;
   const int32_t* so = slice_offsets(A, r);
@@ -769,7 +769,7 @@ __device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t r, int64_t b
 #pragma unroll
     for (int u = 0; u < U; u++) {
       bool ok = k0 + u < w;
-      zv[u] = ok ? z[c[u]] : (T)0;
+      zv[u] = ok ? (T)z[c[u]] : (T)0;
       pv[u] = ok ? p[c[u]] : (T)0;
     }
 #pragma unroll
@@ -916,20 +916,26 @@ __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict
 
 // x = 0, r = b, rr = b.b (partials); first smoothing step of the V-cycle on r
 // (d0 = z0 = c0 invd r).
-template <typename T>
-__global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r,
-                           T* __restrict__ x, const T* __restrict__ invd, T* __restrict__ d0,
-                           T* __restrict__ z0, T c0, const int2* __restrict__ bm,
-                           double* __restrict__ partial) {
+// r = b, x = 0, partials of b.b, and the V-cycle's zero-guess first step
+// d0 = z0 = c0 D^-1 r.  The Krylov vectors are in TK and the V-cycle in TP;
+// when they differ, rp receives r rounded to TP, the V-cycle's input (rp is
+// null when TK == TP: the V-cycle reads r itself).
+template <typename TK, typename TP>
+__global__ void k_cg_start(int64_t n, const TK* __restrict__ b, TK* __restrict__ r,
+                           TK* __restrict__ x, TP* __restrict__ rp, const TP* __restrict__ invd,
+                           TP* __restrict__ d0, TP* __restrict__ z0, TP c0,
+                           const int2* __restrict__ bm, double* __restrict__ partial) {
   int64_t i = logical_row();
   bool second;
   member_of(bm, i, second);
   double local = 0.0;
   if (i < n) {
-    T bv = b[i];
+    TK bv = b[i];
     r[i] = bv;
-    x[i] = (T)0;
-    T dd = c0 * invd[i] * bv;
+    x[i] = (TK)0;
+    TP rv = (TP)bv;
+    if (rp) rp[i] = rv;
+    TP dd = c0 * invd[i] * rv;
     d0[i] = dd;
     z0[i] = dd;
     local = (double)bv * (double)bv;
@@ -940,8 +946,9 @@ __global__ void k_cg_start(int64_t n, const T* __restrict__ b, T* __restrict__ r
 // p_new = z + beta p_old,  q = A p_new (formed from z and p_old at the
 // neighbours, so p is never read while it is rewritten), partials of p.q.
 // Neighbours belong to the same member (the operator is block diagonal).
-template <typename T>
-__global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict__ pold,
+// (z is the V-cycle's output, in its precision TZ; p, q and A are in T)
+template <typename T, typename TZ>
+__global__ void k_cg_pq(SellD<T> A, const TZ* __restrict__ z, const T* __restrict__ pold,
                         T* __restrict__ pnew, T* __restrict__ q, const int* __restrict__ done,
                         const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
                         double* __restrict__ partial) {
@@ -960,10 +967,10 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
     T s, pn;
     if (first) {
       s = sell_dot(A, i, base, w, z);
-      pn = z[i];
+      pn = (T)z[i];
     } else {
       s = sell_dot_zp(A, i, base, w, z, pold, beta);
-      pn = z[i] + beta * pold[i];
+      pn = (T)z[i] + beta * pold[i];
     }
     pnew[i] = pn;
     q[i] = s;
@@ -974,26 +981,29 @@ __global__ void k_cg_pq(SellD<T> A, const T* __restrict__ z, const T* __restrict
 
 // x += alpha p, r -= alpha q (members still running), partials of r.r; first
 // smoothing step of the next V-cycle on r.
-template <typename T>
-__global__ void k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
-                        const T* __restrict__ p, const T* __restrict__ q,
-                        const T* __restrict__ invd, T* __restrict__ d0, T* __restrict__ z0, T c0,
-                        const int* __restrict__ done, const SolveScalars* __restrict__ sc,
-                        const int2* __restrict__ bm, double* __restrict__ partial) {
+template <typename TK, typename TP>
+__global__ void k_cg_xr(int64_t n, TK* __restrict__ x, TK* __restrict__ r,
+                        const TK* __restrict__ p, const TK* __restrict__ q,
+                        TP* __restrict__ rp, const TP* __restrict__ invd, TP* __restrict__ d0,
+                        TP* __restrict__ z0, TP c0, const int* __restrict__ done,
+                        const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
+                        double* __restrict__ partial) {
   if (*done) return;
   int64_t i = logical_row();
   bool second;
   int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < n) {
-    T rv = r[i];
+    TK rv = r[i];
     if (!sc[mi].done) {
-      T alpha = (T)sc[mi].alpha;
+      TK alpha = (TK)sc[mi].alpha;
       x[i] += alpha * p[i];
       rv = rv - alpha * q[i];
       r[i] = rv;
     }
-    T dd = c0 * invd[i] * rv;
+    TP rpv = (TP)rv;
+    if (rp) rp[i] = rpv;
+    TP dd = c0 * invd[i] * rpv;
     d0[i] = dd;
     z0[i] = dd;
     local = (double)rv * (double)rv;

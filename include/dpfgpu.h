@@ -34,7 +34,11 @@ enum {
   DPF_ESINGULAR = 5,   /* structurally singular system (a free cell without edges) */
   DPF_ENOMEM = 6
 };
-enum { DPF_FP64 = 0, DPF_MIXED = 1 };
+/* DPF_FP64: FP64 AMG-PCG.  DPF_MIXED: FP32 AMG-PCG inner solves inside FP64
+ * iterative refinement.  DPF_MIXED_PC: FP64 CG (vectors, SpMV, scalars)
+ * preconditioned by the FP32 V-cycle -- no refinement restarts, so the FP64
+ * iteration count, with the V-cycle's traffic halved. */
+enum { DPF_FP64 = 0, DPF_MIXED = 1, DPF_MIXED_PC = 2 };
 enum { DPF_HOST = 0, DPF_DEVICE = 1 };
 /* flags: time every finest-level smoothing sweep with a hipEvent pair and
  * report the sum in dpf_stats.sweep_ms / sweep_launches (adds two event
@@ -45,7 +49,7 @@ typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
   double abstol;
   int32_t maxiter;     /* PCG iteration cap, summed over refinement steps */
-  int32_t precision;   /* DPF_FP64, or DPF_MIXED: FP32 AMG-PCG inside FP64 refinement */
+  int32_t precision;   /* DPF_FP64, DPF_MIXED or DPF_MIXED_PC (above) */
   double inner_reltol; /* DPF_MIXED: relative tolerance of each FP32 inner solve */
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
   int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = autotuned:
@@ -110,11 +114,11 @@ int dpf_batch_topology_create(int32_t nmembers, const int64_t *ncells, const int
  * single system). */
 int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers);
 
-/* info[0..12] = n, nedges, nfree, nnz(A), levels, coarse rows,
+/* info[0..14] = n, nedges, nfree, nnz(A), levels, coarse rows,
  * operator-complexity*1000, host symbolic setup time in microseconds, the
- * tuned Chebyshev degrees (finest, coarse) for FP64 and for mixed (0 = not
- * tuned yet), and the fine operator's 64-row slices stored as column offsets
- * (no column indices). */
+ * tuned Chebyshev degrees (finest, coarse) for DPF_FP64 and for DPF_MIXED
+ * (0 = not tuned yet), the fine operator's 64-row slices stored as column
+ * offsets (no column indices), then the tuned degrees for DPF_MIXED_PC. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,

@@ -26,7 +26,8 @@ def _members(pkg):
     ]
 
 
-@pytest.mark.parametrize("precision,tol_h", [("fp64", 1e-10), ("mixed", 1e-8)])
+@pytest.mark.parametrize("precision,tol_h", [("fp64", 1e-10), ("mixed", 1e-8),
+                                           ("mixed_pc", 1e-8)])
 def test_batch_matches_members_solved_alone(pkg, oracle, precision, tol_h):
     from dpfehm_jl_amd import api
     print("batch", precision, flush=True)

@@ -29,7 +29,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_pc"])
 def test_golden_forward_and_pullback(api, name, precision):
     _say("golden", name, precision)
     g = load_case(name)
@@ -124,7 +124,7 @@ def test_deterministic_bitwise(pkg, api):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_pc"])
 def test_power_of_two_scale_invariance(pkg, api, precision):
     """The solve is linear in (dirichleths, Qs) and the pullback linear in
     delta: scaling by 2^-600 or 2^600 (where FP64 sums of squares of the
@@ -231,7 +231,7 @@ def test_2m_properties(pkg, api):
     w = pkg.workloads.steady_2m()
     K = w.Ks()
     res = {}
-    for precision in ("fp64", "mixed"):
+    for precision in ("fp64", "mixed", "mixed_pc"):
         st = {}
         h = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
                                         w.dirichleths, w.Qs, precision=precision, stats=st)
@@ -248,3 +248,4 @@ def test_2m_properties(pkg, api):
                                    w.dirichletnodes, w.dirichleths, w.Qs, one, one)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.5e-10
     assert rel(res["mixed"], res["fp64"]) < 1e-8
+    assert rel(res["mixed_pc"], res["fp64"]) < 1e-8

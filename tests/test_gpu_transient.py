@@ -16,7 +16,8 @@ def _problem(pkg, ns=(20, 18, 8), sigma=1.5):
     return w, Ss
 
 
-@pytest.mark.parametrize("precision,ck", [("fp64", 7), ("fp64", 0), ("mixed", 5)])
+@pytest.mark.parametrize("precision,ck", [("fp64", 7), ("fp64", 0), ("mixed", 5),
+                                          ("mixed_pc", 4)])
 def test_transient_forward_adjoint_vs_oracle(pkg, oracle, precision, ck):
     from dpfehm_jl_amd import api
     print("transient", precision, ck, flush=True)
@@ -48,7 +49,7 @@ def test_transient_forward_adjoint_vs_oracle(pkg, oracle, precision, ck):
     assert rel(dQ, dQ_ref) < 1e-7
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_pc"])
 def test_stiff_adjoint_long_horizon(pkg, oracle, precision):
     """dt = 3600 s on a 4 m deep box with K/Ss ~ 12 m^2/s: every step damps
     the slowest mode by ~1e-4, so the adjoint falls below 1e-160 within 40
