@@ -255,7 +255,7 @@ def main():
         "iterations_fp64_fwd_adj": [results["fp64"]["stats"][0][0]["iterations"],
                                     results["fp64"]["stats"][0][1]["iterations"]],
         "topology_setup_s": round(topo_s, 3),
-        "cheb_degrees_finest_coarse": tuned,
+        "cheb_tuned_finest_coarse_ratio": tuned,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",

@@ -110,8 +110,8 @@ class Topology:
         self.lock = threading.Lock()
 
     def info(self):
-        info = np.zeros(15, dtype=np.int64)
-        _lib.lib().dpf_topology_info(self._h, _ptr(info), 15)
+        info = np.zeros(18, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 18)
         return info
 
     def layout(self):
@@ -120,15 +120,16 @@ class Topology:
         return {"fine_offset_slices": int(self.info()[12])}
 
     def tuned_degrees(self):
-        """{'fp64': (finest, coarse), 'mixed': (finest, coarse)} once tuned."""
+        """{precision: (finest degree, coarse degree, interval ratio)} for the
+        precisions tuned so far."""
         i = self.info()
         out = {}
         if i[8]:
-            out["fp64"] = (int(i[8]), int(i[9]))
+            out["fp64"] = (int(i[8]), int(i[9]), int(i[15]))
         if i[10]:
-            out["mixed"] = (int(i[10]), int(i[11]))
+            out["mixed"] = (int(i[10]), int(i[11]), int(i[16]))
         if i[13]:
-            out["mixed_pc"] = (int(i[13]), int(i[14]))
+            out["mixed_pc"] = (int(i[13]), int(i[14]), int(i[17]))
         return out
 
     @property

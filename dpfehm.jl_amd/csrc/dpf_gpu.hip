@@ -212,7 +212,9 @@ struct Cheb {
   double c0;               // zero-guess first step: d = c0 invd r
   std::vector<double> c1;  // step k >= 1: d = c1[k] d + c2[k] invd r
   std::vector<double> c2;
-  void make(int degree, double lmax, double ratio) {
+  double ratio = 15.0;     // interval [lmax / ratio, lmax]
+  void make(int degree, double lmax, double ratio_) {
+    ratio = ratio_;
     double lmin = lmax / ratio;
     double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     double sigma = theta / delta;
@@ -272,7 +274,9 @@ struct dpf_topology {
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
   // fine operator layout: offset slices, and nonzeros whose column is stored
   int64_t fine_offset_slices = 0, fine_explicit_nnz = 0;
-  double cheb_ratio = 30.0;
+  // Chebyshev interval [1/ratio, 1] of D_l1^-1 A: default, and tuned per precision
+  double cheb_ratio = 15.0;
+  double ratio_tuned[3] = {15.0, 15.0, 15.0};
   // Chebyshev degrees of the finest / coarser levels per precision (FP64,
   // FP32), default 2/2 until tuned, and the tuner's record
   // (index = precision: FP64, mixed refinement, FP64 Krylov with FP32 V-cycle)
@@ -574,7 +578,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     } else {
       // no dense solve: a longer smoothing sweep (degree 2m) stands in
       Cheb c2;
-      c2.make(2 * m, 1.0, t->cheb_ratio);
+      c2.make(2 * m, 1.0, ch.ratio);
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
         sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl, dd,
@@ -1004,7 +1008,7 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   const TopologyHost& H = t->H;
   int64_t nnz_all = 0;
   for (auto& l : H.levels) nnz_all += l.A.nnz;
-  int64_t v[15] = {H.n,
+  int64_t v[18] = {H.n,
                   H.ne,
                   H.nf,
                   H.levels[0].A.nnz,
@@ -1018,8 +1022,11 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
                   t->tuned[1] ? t->deg_coarse[1] : 0,
                   t->fine_offset_slices,
                   t->tuned[2] ? t->deg_fine[2] : 0,
-                  t->tuned[2] ? t->deg_coarse[2] : 0};
-  for (int i = 0; i < ninfo && i < 15; i++) info[i] = v[i];
+                  t->tuned[2] ? t->deg_coarse[2] : 0,
+                  t->tuned[0] ? (int64_t)t->ratio_tuned[0] : 0,
+                  t->tuned[1] ? (int64_t)t->ratio_tuned[1] : 0,
+                  t->tuned[2] ? (int64_t)t->ratio_tuned[2] : 0};
+  for (int i = 0; i < ninfo && i < 18; i++) info[i] = v[i];
   return DPF_OK;
 }
 
@@ -1057,17 +1064,18 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   t->sweep_n = 0;
   int pidx = o->precision;
   int L = t->levels();
-  auto make_chs = [&](int m0, int mc) {
+  auto make_chs = [&](int m0, int mc, double ratio) {
     std::vector<Cheb> v(L);
-    for (int l = 0; l < L; l++) v[l].make(l == 0 ? m0 : mc, 1.0, t->cheb_ratio);
+    for (int l = 0; l < L; l++) v[l].make(l == 0 ? m0 : mc, 1.0, ratio);
     return v;
   };
-  // smoother degrees: explicit (every level), or tuned on this topology's
-  // first solve in this precision
+  // smoothers: explicit degree (every level, default interval), or the
+  // degrees and interval tuned on this topology's first solve in this precision
   bool tune = o->cheb_degree == 0 && !t->tuned[pidx];
-  std::vector<Cheb> chs = o->cheb_degree > 0
-                              ? make_chs(o->cheb_degree, o->cheb_degree)
-                              : make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
+  std::vector<Cheb> chs =
+      o->cheb_degree > 0
+          ? make_chs(o->cheb_degree, o->cheb_degree, t->cheb_ratio)
+          : make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
   double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   LevelDev& d0 = *t->lev[0];
@@ -1168,28 +1176,36 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
                          (const SolveScalars*)sc, bm_ptr(t));
     if (tune) {
       // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
-      // very system and keep the fastest; the result is fixed for the life
-      // of the topology, so its solves stay bitwise reproducible.
+      // very system at the default interval, then two other intervals at the
+      // best pair, and keep the fastest; the result is fixed for the life of
+      // the topology, so its solves stay bitwise reproducible.
       static const int cand[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
+      static const double ratios[] = {30.0, 8.0};
       double best = 1e300;
-      for (int c = 0; c < 5; c++) {
-        std::vector<Cheb> cc = make_chs(cand[c][0], cand[c][1]);
+      auto trial = [&](int m0, int mc, double ratio, int slot) {
+        std::vector<Cheb> cc = make_chs(m0, mc, ratio);
         HIPTRY(hipStreamSynchronize(s));
         double ta = now_ms();
         SolveResult tr = krylov(cc);
         HIPTRY(hipStreamSynchronize(s));
         double el = now_ms() - ta;
-        t->tune_ms[pidx][c] = tr.status == 1 ? el : -1.0;
-        t->tune_iters[pidx][c] = tr.iters;
+        if (slot >= 0) {
+          t->tune_ms[pidx][slot] = tr.status == 1 ? el : -1.0;
+          t->tune_iters[pidx][slot] = tr.iters;
+        }
         if (tr.status == 1 && el < best) {
           best = el;
-          t->deg_fine[pidx] = cand[c][0];
-          t->deg_coarse[pidx] = cand[c][1];
+          t->deg_fine[pidx] = m0;
+          t->deg_coarse[pidx] = mc;
+          t->ratio_tuned[pidx] = ratio;
         }
-      }
+      };
+      for (int c = 0; c < 5; c++) trial(cand[c][0], cand[c][1], t->cheb_ratio, c);
+      int m0 = t->deg_fine[pidx], mc = t->deg_coarse[pidx];
+      for (double ratio : ratios) trial(m0, mc, ratio, -1);
       t->tuned[pidx] = true;
       tune = false;
-      chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx]);
+      chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
     }
     r = krylov(chs);
     if (o->precision != DPF_MIXED) {

@@ -114,11 +114,13 @@ int dpf_batch_topology_create(int32_t nmembers, const int64_t *ncells, const int
  * single system). */
 int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers);
 
-/* info[0..14] = n, nedges, nfree, nnz(A), levels, coarse rows,
+/* info[0..17] = n, nedges, nfree, nnz(A), levels, coarse rows,
  * operator-complexity*1000, host symbolic setup time in microseconds, the
  * tuned Chebyshev degrees (finest, coarse) for DPF_FP64 and for DPF_MIXED
  * (0 = not tuned yet), the fine operator's 64-row slices stored as column
- * offsets (no column indices), then the tuned degrees for DPF_MIXED_PC. */
+ * offsets (no column indices), the tuned degrees for DPF_MIXED_PC, then the
+ * tuned Chebyshev interval ratio r (interval [lmax/r, lmax]) for DPF_FP64,
+ * DPF_MIXED and DPF_MIXED_PC. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
