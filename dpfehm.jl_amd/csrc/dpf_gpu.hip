@@ -661,7 +661,11 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   const TP* vb = split ? (const TP*)rp : (const TP*)(void*)r;
   const TP* invd = level_invd(t, hp, 0);
   SellD<TK> A = d0.A.view<TK>(level_val(t, hk, 0));
-  TP* dd = n0.d.template as<TP>();
+  // the zero-guess step's d is read only by further pre-smoothing sweeps, or
+  // by the longer sweep that stands in for a dense solve on a one-level
+  // hierarchy
+  bool need_d = ch.c1.size() > 1 || (t->levels() == 1 && !t->H.coarse_dense);
+  TP* dd = need_d ? n0.d.template as<TP>() : nullptr;
   TP* z0 = n0.x.template as<TP>();
   TP* zs = n0.xs.template as<TP>();
   int seq = ++t->seq;

@@ -513,11 +513,12 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
 }  // namespace
 
 // Row-split copy of a SELL matrix: sub-row r*G + g holds the entries
-// g, g + G, ... of row r, G = 2^lg chosen so that a lane sees ~4 entries.
+// g, g + G, ... of row r, G = 2^lg chosen so that a lane sees ~8 entries.
 void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src) {
   double avg = A.nrows > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
   lg = 0;
-  while ((1 << lg) * 4 < avg && lg < 6) lg++;
+  // ~8 entries a lane: one batch of eight loads in flight per lane
+  while ((1 << lg) * 8 < avg && lg < 6) lg++;
   int G = 1 << lg;
   int64_t ns = A.nrows * G;
   std::vector<int64_t> rp(ns + 1, 0);

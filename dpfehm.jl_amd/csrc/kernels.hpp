@@ -450,7 +450,7 @@ __global__ void k_hm_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
 // b_j = Q_i + sum over Dirichlet neighbours of K_e aol_e dh_o, for free cell
 // i = f2n[j].  Values are written in FP64 and, if val32 is given, also in FP32.
 // invd = 1 / (m_jj + sum_k |m_jk|) (l1-Jacobi diagonal).
-__global__ void k_assemble(int64_t nf, const int32_t* __restrict__ f2n,
+__global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __restrict__ f2n,
                            const int64_t* __restrict__ inc_ptr,
                            const int32_t* __restrict__ inc_edge,
                            const int32_t* __restrict__ inc_other,
@@ -461,35 +461,59 @@ __global__ void k_assemble(int64_t nf, const int32_t* __restrict__ f2n,
                            double tscale, const double* __restrict__ svec, SellD<double> A,
                            float* __restrict__ val32, double* __restrict__ invd64,
                            float* __restrict__ invd32, double* __restrict__ b) {
+  // rows up to AW slots accumulate in LDS ([slot][lane]) and are written once;
+  // wider rows (unstructured meshes) accumulate in place
+  constexpr int AW = 8;
+  __shared__ double acc[AW][BLK];
   int64_t j = logical_row();
   if (j >= nf) return;
   int64_t base;
   int w;
   sell_row(A, j, base, w);
-  for (int k = 0; k < w; k++) A.val[base + (int64_t)k * 64] = 0.0;
+  const int tid = threadIdx.x;
+  const bool lds = w <= AW;
+  auto slot = [&](int k) -> double& { return lds ? acc[k][tid] : A.val[base + (int64_t)k * 64]; };
+  for (int k = 0; k < w; k++) slot(k) = 0.0;
   int32_t i = f2n[j];
   double dsum = 0.0, bj = Q ? Q[i] : 0.0;
-  for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
-    int32_t e = inc_edge[p];
-    double t = Ke[e] * aol[e];
-    dsum += t;
-    uint16_t sl = inc_slot[p];
-    if (sl != NO_SLOT_D)
-      A.val[base + (int64_t)sl * 64] -= tscale * t;
-    else
-      bj += t * dh[inc_other[p]];
+  int64_t p1 = inc_ptr[i + 1];
+  constexpr int U = 8;
+  for (int64_t p0 = inc_ptr[i]; p0 < p1; p0 += U) {
+    int32_t e[U];
+    uint16_t sl[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = p0 + u < p1;
+      e[u] = ok ? inc_edge[p0 + u] : 0;
+      sl[u] = ok ? inc_slot[p0 + u] : NO_SLOT_D;
+    }
+    double t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u] = p0 + u < p1 ? Ke[e[u]] * aol[e[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (p0 + u >= p1) break;
+      dsum += t[u];
+      if (sl[u] != NO_SLOT_D)
+        slot(sl[u]) -= tscale * t[u];
+      else
+        bj += t[u] * dh[inc_other[p0 + u]];
+    }
   }
   int ds = diag_slot[j];
   double dv = tscale * dsum + (svec ? svec[j] : 0.0);
-  A.val[base + (int64_t)ds * 64] = dv;
+  slot(ds) = dv;
   double l1 = dv;
   for (int k = 0; k < w; k++)
-    if (k != ds) l1 += fabs(A.val[base + (int64_t)k * 64]);
+    if (k != ds) l1 += fabs(slot(k));
   double inv = 1.0 / l1;
   if (invd64) invd64[j] = inv;
   if (invd32) invd32[j] = (float)inv;
-  if (val32)
-    for (int k = 0; k < w; k++) val32[base + (int64_t)k * 64] = (float)A.val[base + (int64_t)k * 64];
+  for (int k = 0; k < w; k++) {
+    double v = slot(k);
+    if (lds) A.val[base + (int64_t)k * 64] = v;
+    if (val32) val32[base + (int64_t)k * 64] = (float)v;
+  }
   if (b) b[j] = bj;
 }
 
@@ -811,7 +835,9 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   double local = 0.0;
   if ((sub & ((1 << lg) - 1)) == 0 && r < n) {
     T br = b[r];
-    T dn = c1 * d[r] + c2 * invd[r] * (br - ax);
+    // c1 = 0 on the first sweep after a coarse correction: d is not read
+    T dn = c2 * invd[r] * (br - ax);
+    if (c1 != (T)0) dn = c1 * d[r] + dn;
     d[r] = dn;
     T xo = xin[r] + dn;
     xout[r] = xo;
@@ -936,7 +962,7 @@ __global__ void k_cg_start(int64_t n, const TK* __restrict__ b, TK* __restrict__
     TP rv = (TP)bv;
     if (rp) rp[i] = rv;
     TP dd = c0 * invd[i] * rv;
-    d0[i] = dd;
+    if (d0) d0[i] = dd;  // null when no sweep reads it (finest degree 1)
     z0[i] = dd;
     local = (double)bv * (double)bv;
   }
@@ -1004,7 +1030,7 @@ __global__ void k_cg_xr(int64_t n, TK* __restrict__ x, TK* __restrict__ r,
     TP rpv = (TP)rv;
     if (rp) rp[i] = rpv;
     TP dd = c0 * invd[i] * rpv;
-    d0[i] = dd;
+    if (d0) d0[i] = dd;  // null when no sweep reads it (finest degree 1)
     z0[i] = dd;
     local = (double)rv * (double)rv;
   }
