@@ -77,6 +77,8 @@ SIGNATURES = {
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
+    "dpf_geo_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
+    "dpf_geo_pullback": (ctypes.c_int, [_p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_steady_forward": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts),
                                           ctypes.POINTER(Stats), ctypes.POINTER(_p)]),
     "dpf_steady_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts),

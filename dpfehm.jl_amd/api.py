@@ -297,6 +297,32 @@ def logKs2Ks_harmonic_pullback(logKs, Ke, dKe, neighbors, dirichletnodes=(), are
     return out
 
 
+def logKs2Ks_geometric(logKs, neighbors, dirichletnodes=(), areasoverlengths=None):
+    """K_e = exp(0.5 (logK_a + logK_b)), the map of
+    examples/gw_steadystate_3d/inputdeck.jl:51, computed on the GPU."""
+    logKs = _f64(logKs)
+    nb = np.asarray(neighbors, dtype=np.int64).reshape(-1, 2)
+    aol = np.ones(nb.shape[0]) if areasoverlengths is None else areasoverlengths
+    topo = get_topology(logKs.shape[0], nb, aol, dirichletnodes)
+    Ke = np.empty(topo.ne)
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_geo_forward(topo.handle, _ptr(logKs), _ptr(Ke), ctypes.byref(o)))
+    return Ke
+
+
+def logKs2Ks_geometric_pullback(logKs, Ke, dKe, neighbors, dirichletnodes=(),
+                                areasoverlengths=None):
+    logKs = _f64(logKs)
+    nb = np.asarray(neighbors, dtype=np.int64).reshape(-1, 2)
+    aol = np.ones(nb.shape[0]) if areasoverlengths is None else areasoverlengths
+    topo = get_topology(logKs.shape[0], nb, aol, dirichletnodes)
+    out = np.empty(logKs.shape[0])
+    o = _opts({})
+    _lib.check(_lib.lib().dpf_geo_pullback(topo.handle, _ptr(_f64(Ke)), _ptr(_f64(dKe)),
+                                           _ptr(out), ctypes.byref(o)))
+    return out
+
+
 def _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
                specificstorage, volumes, u0, nsteps, dt, qscale, checkpoint_every, keep,
                return_states, stats, kwargs):

@@ -1344,7 +1344,13 @@ int dpf_residuals(dpf_topology* t, const double* hfree, const double* Ks,
   });
 }
 
-int dpf_hm_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_opts* opts) {
+}  // extern "C"
+
+namespace {
+
+// logK -> K_e (geometric = false: harmonic mean, true: geometric mean)
+int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke,
+                 const dpf_opts* opts) {
   if (!t || !logK || !Ke) {
     set_err("NULL argument");
     return DPF_EINVAL;
@@ -1361,8 +1367,12 @@ int dpf_hm_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_op
       t->stage_e.alloc(t->H.ne * sizeof(double));
       out = t->stage_e.as<double>();
     }
-    hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
-                       t->edges.as<int2>(), lk, out);
+    if (geometric)
+      hipLaunchKernelGGL(k_geo_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
+                         t->edges.as<int2>(), lk, out);
+    else
+      hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
+                         t->edges.as<int2>(), lk, out);
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST) {
       HIPTRY(hipMemcpyAsync(Ke, out, t->H.ne * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1372,9 +1382,9 @@ int dpf_hm_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_op
   });
 }
 
-int dpf_hm_pullback(dpf_topology* t, const double* logK, const double* Ke, const double* dKe,
-                    double* dlogK, const dpf_opts* opts) {
-  if (!t || !logK || !Ke || !dKe || !dlogK) {
+int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const double* Ke,
+                  const double* dKe, double* dlogK, const dpf_opts* opts) {
+  if (!t || (!geometric && !logK) || !Ke || !dKe || !dlogK) {
     set_err("NULL argument");
     return DPF_EINVAL;
   }
@@ -1384,25 +1394,50 @@ int dpf_hm_pullback(dpf_topology* t, const double* logK, const double* Ke, const
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     hipStream_t s = pick_stream(t, o);
-    const double* lk = stage_in(t, o, t->stage_n, logK, t->H.n, s);
+    const double* lk = geometric ? nullptr : stage_in(t, o, t->stage_n, logK, t->H.n, s);
     const double* ke = stage_in(t, o, t->stage_e, Ke, t->H.ne, s);
-    DBuf dkb;
+    DBuf dkb(t->pool.get());
     const double* dk = stage_in(t, o, dkb, dKe, t->H.ne, s);
     double* out = dlogK;
     if (o->memory == DPF_HOST) {
       t->stage_n2.alloc(t->H.n * sizeof(double));
       out = t->stage_n2.as<double>();
     }
-    hipLaunchKernelGGL(k_hm_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
-                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                       t->inc_other.as<int32_t>(), lk, ke, dk, out);
+    if (geometric)
+      hipLaunchKernelGGL(k_geo_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
+                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(), ke, dk, out);
+    else
+      hipLaunchKernelGGL(k_hm_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
+                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                         t->inc_other.as<int32_t>(), lk, ke, dk, out);
     HIPTRY(hipGetLastError());
-    if (o->memory == DPF_HOST) {
+    if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(dlogK, out, t->H.n * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
     HIPTRY(hipStreamSynchronize(s));
     return DPF_OK;
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpf_hm_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_opts* opts) {
+  return kmap_forward(t, false, logK, Ke, opts);
+}
+
+int dpf_hm_pullback(dpf_topology* t, const double* logK, const double* Ke, const double* dKe,
+                    double* dlogK, const dpf_opts* opts) {
+  return kmap_pullback(t, false, logK, Ke, dKe, dlogK, opts);
+}
+
+int dpf_geo_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_opts* opts) {
+  return kmap_forward(t, true, logK, Ke, opts);
+}
+
+int dpf_geo_pullback(dpf_topology* t, const double* Ke, const double* dKe, double* dlogK,
+                     const dpf_opts* opts) {
+  return kmap_pullback(t, true, nullptr, Ke, dKe, dlogK, opts);
 }
 
 int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichleths,

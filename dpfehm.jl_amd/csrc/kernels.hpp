@@ -425,6 +425,30 @@ __global__ void k_hm_forward(int64_t ne, const int2* __restrict__ edges,
   Ke[e] = 2.0 / (1.0 / ka + 1.0 / kb);
 }
 
+// K_e = exp((logK_a + logK_b) / 2), the geometric mean of exp(logK).
+__global__ void k_geo_forward(int64_t ne, const int2* __restrict__ edges,
+                              const double* __restrict__ logK, double* __restrict__ Ke) {
+  int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (e >= ne) return;
+  int2 ab = edges[e];
+  Ke[e] = exp(0.5 * (logK[ab.x] + logK[ab.y]));
+}
+
+// dlogK_c = sum_{e incident to c} dK_e K_e / 2, in edge-list order per cell.
+__global__ void k_geo_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
+                               const int32_t* __restrict__ inc_edge,
+                               const double* __restrict__ Ke, const double* __restrict__ dKe,
+                               double* __restrict__ dlogK) {
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double s = 0.0;
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+    int32_t e = inc_edge[p];
+    s += dKe[e] * 0.5 * Ke[e];
+  }
+  dlogK[c] = s;
+}
+
 // dlogK_c = sum_{e incident to c} dK_e K_e k_other / (k_c + k_other), in
 // edge-list order per cell.
 __global__ void k_hm_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,

@@ -140,6 +140,17 @@ int dpf_hm_forward(dpf_topology *topo, const double *logK, double *Ke, const dpf
 int dpf_hm_pullback(dpf_topology *topo, const double *logK, const double *Ke,
                     const double *dKe, double *dlogK, const dpf_opts *opts);
 
+/* The geometric-mean map of examples/gw_steadystate_3d/inputdeck.jl:51,
+ * K_e = exp(0.5 (logK_a + logK_b)), and its pullback
+ * dlogK_c = sum over the edges of c of dK_e K_e / 2 (a cell gather).
+ * Self-edges (a == b) are not incident to their cell and contribute nothing
+ * to dlogK (nor, for the same reason, to dpf_hm_pullback); the residual gives
+ * them zero flux.
+ *   logK: n;  Ke, dKe: nedges;  dlogK: n. */
+int dpf_geo_forward(dpf_topology *topo, const double *logK, double *Ke, const dpf_opts *opts);
+int dpf_geo_pullback(dpf_topology *topo, const double *Ke, const double *dKe, double *dlogK,
+                     const dpf_opts *opts);
+
 /* groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs; kwargs...) (src/groundwater.jl:75-83) and the forward
  * half of its rrule (src/groundwater.jl:85-92): assemble A = dr/dh and

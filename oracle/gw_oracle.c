@@ -281,6 +281,24 @@ void orc_hm_pullback(int64_t n, int64_t ne, const int64_t *nb, const double *log
   }
 }
 
+/* ---- geometric-mean edge conductivity and its pullback ----
+ * K_e = exp(0.5 (logK_a + logK_b)), the logK -> K_e map of
+ * examples/gw_steadystate_3d/inputdeck.jl:51.  Pullback:
+ * dlogK_c += dK_e * K_e / 2 for both ends, over edges in list order. */
+void orc_geo_forward(int64_t ne, const int64_t *nb, const double *logK, double *Ke) {
+  for (int64_t e = 0; e < ne; e++) Ke[e] = exp(0.5 * (logK[nb[2 * e]] + logK[nb[2 * e + 1]]));
+}
+
+void orc_geo_pullback(int64_t n, int64_t ne, const int64_t *nb, const double *logK,
+                      const double *dKe, double *dlogK) {
+  for (int64_t i = 0; i < n; i++) dlogK[i] = 0.0;
+  for (int64_t e = 0; e < ne; e++) {
+    double g = dKe[e] * 0.5 * exp(0.5 * (logK[nb[2 * e]] + logK[nb[2 * e + 1]]));
+    dlogK[nb[2 * e]] += g;
+    dlogK[nb[2 * e + 1]] += g;
+  }
+}
+
 /* ---- steady state: src/groundwater.jl:75-83 ----
  * Assemble at hfree = 0 with Ss = V = 1: b = -r(0), A = dr/dh; solve;
  * h = hfree on free cells, dirichleths on Dirichlet cells.

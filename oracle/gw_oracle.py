@@ -37,6 +37,9 @@ Function (restates) -- pinned by (tests/test_oracle.py unless noted)
 * ``hm_forward``/``hm_pullback`` src/richards.jl:50-52 (hm) with the
   logK -> K_e map of examples/gw_steadystate_3d/inputdeck.jl:51
   -- central finite differences; closed form at equal arguments.
+* ``geo_forward``/``geo_pullback`` the geometric-mean map K_e =
+  exp(0.5 (logK_a + logK_b)) of examples/gw_steadystate_3d/inputdeck.jl:51
+  -- central finite differences; closed form sqrt(k_a k_b).
 * ``transient``/``transient_adjoint`` examples/gw_transient_3d/inputdeck.jl:
   60-92 (f = -groundwater_residuals) with fixed-step backward Euler in place
   of DifferentiableBackwardEuler.steps
@@ -251,6 +254,22 @@ def hm_pullback(logK, neighbors, dKe):
                            _p(lk, "f"), _p(dk, "f"), _p(out, "f"))
     return out
 
+
+def geo_forward(logK, neighbors):
+    nb = _edges0(neighbors)
+    lk = _f64(logK)
+    Ke = np.empty(nb.shape[0])
+    _lib().orc_geo_forward(ctypes.c_int64(nb.shape[0]), _p(nb, "i"), _p(lk, "f"), _p(Ke, "f"))
+    return Ke
+
+
+def geo_pullback(logK, neighbors, dKe):
+    nb = _edges0(neighbors)
+    lk, dk = _f64(logK), _f64(dKe)
+    out = np.empty(lk.size)
+    _lib().orc_geo_pullback(ctypes.c_int64(lk.size), ctypes.c_int64(nb.shape[0]), _p(nb, "i"),
+                            _p(lk, "f"), _p(dk, "f"), _p(out, "f"))
+    return out
 
 def transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
               specificstorage, volumes, u0, nsteps, dt, qscale, reltol=1e-13, maxiter=20000):

@@ -249,3 +249,62 @@ def test_2m_properties(pkg, api):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.5e-10
     assert rel(res["mixed"], res["fp64"]) < 1e-8
     assert rel(res["mixed_pc"], res["fp64"]) < 1e-8
+
+
+def test_live_states_and_buffer_reuse(pkg, api):
+    """Forward states keep their own K_e / heads while later solves run on the
+    same topology, and buffers released by destroyed states are reused safely:
+    three states pulled back in reverse order and destroyed out of order, then
+    a fourth solve (on reused buffers) must reproduce each gradient bitwise."""
+    _say("live states")
+    w = pkg.workloads.steady_2m(ns=(24, 20, 10), maxs=(12.0, 10.0, 5.0))
+    rng = np.random.default_rng(7)
+    Ks = [w.Ks() * np.exp(0.3 * rng.normal(size=w.nedges)) for _ in range(3)]
+    delta = np.where(np.arange(w.n) % 13 == 0, 1.0, 0.0)
+    args = (w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs)
+    ref = []
+    for K in Ks:
+        h, pb = api.rrule_groundwater_steadystate(K, *args)
+        ref.append((h, pb(delta)[1]))
+        pb.state.close()
+    live = [api.rrule_groundwater_steadystate(K, *args) for K in Ks]
+    grads = [None] * 3
+    for k in (2, 0, 1):
+        grads[k] = live[k][1](delta)[1]
+    for k in (1, 2, 0):
+        live[k][1].state.close()
+    for k in range(3):
+        assert np.array_equal(live[k][0], ref[k][0])
+        assert np.array_equal(grads[k], ref[k][1])
+    h, pb = api.rrule_groundwater_steadystate(Ks[1], *args)
+    assert np.array_equal(h, ref[1][0]) and np.array_equal(pb(delta)[1], ref[1][1])
+
+
+def test_geometric_map_and_gradient(pkg, api, oracle):
+    """The geometric-mean K map of examples/gw_steadystate_3d/inputdeck.jl:51 and
+    its pullback against the oracle, then a logK gradient through it (solve +
+    pullback + map pullback) against the oracle's chain."""
+    _say("geometric map")
+    w = pkg.workloads.steady_2m(ns=(20, 18, 10), maxs=(10.0, 9.0, 5.0))
+    Ke = api.logKs2Ks_geometric(w.logK, w.neighbors, w.dirichletnodes, w.areasoverlengths)
+    assert rel(Ke, oracle.geo_forward(w.logK, w.neighbors)) < 1e-14
+    rng = np.random.default_rng(9)
+    dKe = rng.normal(size=w.nedges)
+    g = api.logKs2Ks_geometric_pullback(w.logK, Ke, dKe, w.neighbors, w.dirichletnodes,
+                                        w.areasoverlengths)
+    assert rel(g, oracle.geo_pullback(w.logK, w.neighbors, dKe)) < 1e-13
+    h, pb = api.rrule_groundwater_steadystate(Ke, w.neighbors, w.areasoverlengths,
+                                              w.dirichletnodes, w.dirichleths, w.Qs,
+                                              reltol=1e-12)
+    delta = np.zeros(w.n)
+    delta[w.obs - 1] = 2.0 * h[w.obs - 1]
+    dK = pb(delta)[1]
+    dlogK = api.logKs2Ks_geometric_pullback(w.logK, Ke, dK, w.neighbors, w.dirichletnodes,
+                                            w.areasoverlengths)
+    Ko = oracle.geo_forward(w.logK, w.neighbors)
+    ho, pbo = oracle.steadystate_rrule(Ko, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                       w.dirichleths, w.Qs, reltol=1e-14)
+    delta_o = np.zeros(w.n)
+    delta_o[w.obs - 1] = 2.0 * ho[w.obs - 1]
+    dKo = pbo(delta_o)[1]
+    assert rel(dlogK, oracle.geo_pullback(w.logK, w.neighbors, dKo)) < 1e-7

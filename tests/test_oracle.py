@@ -278,6 +278,26 @@ def test_hm_chain_rule(pkg, oracle):
         assert np.isclose(g[c], fd, rtol=1e-6, atol=1e-12)
 
 
+
+def test_geo_chain_rule(pkg, oracle):
+    """Geometric-mean map of examples/gw_steadystate_3d/inputdeck.jl:51: the
+    closed form sqrt(k_a k_b), and its pullback against central differences."""
+    nb, aol, vol, dn, dh, Q, K, n = _tiny(pkg, seed=14)
+    rng = np.random.default_rng(15)
+    logK = rng.normal(size=n) - 9.0
+    Ke = oracle.geo_forward(logK, nb)
+    k = np.exp(logK)
+    assert np.allclose(Ke, np.sqrt(k[nb[:, 0] - 1] * k[nb[:, 1] - 1]), rtol=1e-14)
+    w = rng.normal(size=nb.shape[0]) * 1e4
+    g = oracle.geo_pullback(logK, nb, w)
+    eps = 1e-6
+    for c in range(0, n, 7):
+        lp, lm = logK.copy(), logK.copy()
+        lp[c] += eps
+        lm[c] -= eps
+        fd = (w @ oracle.geo_forward(lp, nb) - w @ oracle.geo_forward(lm, nb)) / (2 * eps)
+        assert np.isclose(g[c], fd, rtol=1e-6, atol=1e-12)
+
 # ---------------------------------------------------------------- transient
 
 def _dbe_dense(oracle, K, nb, aol, dn, dh, Q, Ss, V, u0, nsteps, dt, qscale):
