@@ -143,11 +143,13 @@ __device__ __forceinline__ int member_of(const int2* __restrict__ bm, int64_t ro
 // stream; the kernel boundary makes the partials visible without any
 // device-scope fence or atomic (on a multi-XCD part a release fence per block
 // writes back the XCD's L2 and costs far more than the kernel's own traffic).
+// two = false (a single system, no member table): slot 1 is 0 and only one
+// block reduction runs.
 __device__ __forceinline__ void block_partial2(double v, bool second,
-                                               double* __restrict__ partial) {
+                                               double* __restrict__ partial, bool two) {
   __shared__ double sh[4];
   double a = block_sum(second ? 0.0 : v, sh);
-  double b = block_sum(second ? v : 0.0, sh);
+  double b = two ? block_sum(second ? v : 0.0, sh) : 0.0;
   if (threadIdx.x == 0) {
     int64_t lb = logical_row() / BLK;
     partial[2 * lb] = a;
@@ -870,7 +872,7 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   if (fuse_dot) {  // finest level only (lg == 0): row == sub
     bool second;
     member_of(bm, sub, second);
-    block_partial2(local, second, partial);
+    block_partial2(local, second, partial, bm != nullptr);
   }
 }
 
@@ -956,7 +958,7 @@ __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict
   bool second;
   member_of(bm, i, second);
   double local = i < n ? (double)b[i] * (double)z[i] : 0.0;
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // ------------------------------------------------------------------ CG
@@ -990,7 +992,7 @@ __global__ void k_cg_start(int64_t n, const TK* __restrict__ b, TK* __restrict__
     z0[i] = dd;
     local = (double)bv * (double)bv;
   }
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // p_new = z + beta p_old,  q = A p_new (formed from z and p_old at the
@@ -1026,7 +1028,7 @@ __global__ void k_cg_pq(SellD<T> A, const TZ* __restrict__ z, const T* __restric
     q[i] = s;
     local = (double)pn * (double)s;
   }
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // x += alpha p, r -= alpha q (members still running), partials of r.r; first
@@ -1058,7 +1060,7 @@ __global__ void k_cg_xr(int64_t n, TK* __restrict__ x, TK* __restrict__ r,
     z0[i] = dd;
     local = (double)rv * (double)rv;
   }
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // ------------------------------------------------------------------ FP64 outer residual / refinement
@@ -1099,8 +1101,8 @@ __global__ void k_warm_dots(int64_t n, const double* __restrict__ x, const doubl
     xb = xi * (b[i] * sc[mi].bscale);
     xr = xi * r[i];
   }
-  block_partial2(xb, second, pxb);
-  block_partial2(xr, second, pxr);
+  block_partial2(xb, second, pxb, bm != nullptr);
+  block_partial2(xr, second, pxr, bm != nullptr);
 }
 
 // x = c x0, r = b bscale - c A x0 = (1 - c) b bscale + c r0, partials of ||r||^2
@@ -1119,7 +1121,7 @@ __global__ void k_warm_apply(int64_t n, double* __restrict__ x, const double* __
     r[i] = rv;
     local = rv * rv;
   }
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // r = b bscale - A x in FP64 (bscale = 1 when sc is null; x = null stands
@@ -1144,7 +1146,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
     r[i] = rv;
     local = rv * rv;
   }
-  block_partial2(local, second, partial);
+  block_partial2(local, second, partial, bm != nullptr);
 }
 
 // inner right-hand side in FP32: b32 = r64 / s for members above their
