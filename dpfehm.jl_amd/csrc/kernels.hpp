@@ -100,11 +100,41 @@ __device__ __forceinline__ int64_t logical_row() {
   return (int64_t)lb * BLK + threadIdx.x;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// Cross-lane moves of the data-parallel primitives (DPP): a register of
+// another lane of the same row of 16 (or, for row_bcast, of the previous
+// rows), with 0 in the rows outside row_mask.  32-bit moves; a double moves
+// as two halves.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each group of 2^lg consecutive lanes (lg <= 6), in a fixed
+// pattern: lane pairs, quads, half-rows and rows (every lane of a group of
+// up to 16 then holds the sum), then rows combined by row broadcasts, which
+// leave the sum of a group of 32 or 64 in the group's last lane.  The result
+// is read from the group's LAST lane.  DPP moves instead of LDS permutes: a
+// six-step wave sum is six dependent ALU steps instead of six LDS round trips.
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int lg) {
+  if (lg >= 1) v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  if (lg >= 2) v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  if (lg >= 3) v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  if (lg >= 4) v += dpp_move<0x140, 0xF>(v);  // row_mirror
+  if (lg >= 5) v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  if (lg >= 6) v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
   return v;
 }
+
+// Sum over the wavefront, valid in lane 63.
+__device__ __forceinline__ double wave_sum(double v) { return group_sum(v, 6); }
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -115,7 +145,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // Sum over the block; the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
@@ -328,7 +358,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
       for (int k = 0; k < 8; k++) s = mx ? fmax(s, v[k]) : s + v[k];
     }
     s = mx ? wave_max(s) : wave_sum(s);
-    if (lane == 0) sh[wave] = s;
+    if (lane == 63) sh[wave] = s;
     __syncthreads();
     if (threadIdx.x != 0) return;
     double tot = 0.0;
@@ -360,7 +390,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
         s = mx ? fmax(s, v) : s + v;
       }
       s = mx ? wave_max(s) : wave_sum(s);
-      if (lane == 0) {
+      if (lane == 63) {
         SolveScalars q = sc[m];
         finish_member(mode, s, q, reltol, abstol, maxiter, abs_target, 1);
         sc[m] = q;
@@ -369,7 +399,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
       }
     }
     if (mode > FIN_OUTER) return;
-    if (lane == 0) {
+    if (lane == 63) {
       sh_flags[wave] = flags;
       sh_maxit[wave] = maxit;
       sh_active[wave] = active;
@@ -670,8 +700,8 @@ __global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict_
         if (q0 + (int64_t)u * G < q1) acc += xa[u] * yb[u];
     }
   }
-  for (int o = G >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  if ((sub & (G - 1)) == 0 && k < nlists) out[dest ? dest[k] : k] = acc;
+  acc = group_sum(acc, lg);
+  if ((sub & (G - 1)) == G - 1 && k < nlists) out[dest ? dest[k] : k] = acc;
 }
 
 // invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator
@@ -764,9 +794,6 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 
 // ------------------------------------------------------------------ V-cycle
 
-// (A x)_r with the rows of A split over G = 2^lg consecutive lanes: every
-// lane of the group returns the full row sum (fixed shuffle pattern).  The
-// caller's lanes must all reach this call (no early exit before it).
 // sum_k A.val[e_k] x[A.col[e_k]] over one SELL row (e_k = base + 64 k, k < w),
 // added in ascending k.  The loads go out in groups of 8 entries (values and
 // columns first, then the gathers), so a lane has up to 8 independent gather
@@ -829,6 +856,9 @@ __device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t r, int64_t b
   return s;
 }
 
+// (A x)_r with the rows of A split over G = 2^lg consecutive lanes: the
+// group's last lane returns the full row sum (fixed reduction pattern).  The
+// caller's lanes must all reach this call (no early exit before it).
 template <typename T>
 __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int lg,
                                            const T* __restrict__ x) {
@@ -839,7 +869,7 @@ __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int l
     sell_row(A, sub, base, w);
     ax = sell_dot(A, sub, base, w, x);
   }
-  for (int o = (1 << lg) >> 1; o > 0; o >>= 1) ax += __shfl_xor(ax, o, 64);
+  ax = group_sum(ax, lg);
   return ax;
 }
 
@@ -859,7 +889,7 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   T ax = split_row_dot(A, sub, lg, xin);
   int64_t r = sub >> lg;
   double local = 0.0;
-  if ((sub & ((1 << lg) - 1)) == 0 && r < n) {
+  if ((sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n) {
     T br = b[r];
     // c1 = 0 on the first sweep after a coarse correction: d is not read
     T dn = c2 * invd[r] * (br - ax);
@@ -885,7 +915,7 @@ __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ 
   int64_t sub = logical_row();
   T ax = split_row_dot(A, sub, lg, x);
   int64_t i = sub >> lg;
-  if ((sub & ((1 << lg) - 1)) == 0 && i < n) r[i] = b[i] - ax;
+  if ((sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n) r[i] = b[i] - ax;
 }
 
 // bc = R rf with every row of R split over G = 2^lg consecutive lanes
@@ -906,9 +936,9 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
     sell_row(Rs, sr, base, w);
     s = sell_dot(Rs, sr, base, w, rf);
   }
-  for (int o = (1 << lg) >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = group_sum(s, lg);
   int64_t I = sr >> lg;
-  if ((sr & ((1 << lg) - 1)) == 0 && I < nc) {
+  if ((sr & ((1 << lg) - 1)) == (1 << lg) - 1 && I < nc) {
     bc[I] = s;
     if (init) {
       T dd = c0 * invdc[I] * s;
