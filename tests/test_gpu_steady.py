@@ -308,3 +308,41 @@ def test_geometric_map_and_gradient(pkg, api, oracle):
     delta_o[w.obs - 1] = 2.0 * ho[w.obs - 1]
     dKo = pbo(delta_o)[1]
     assert rel(dlogK, oracle.geo_pullback(w.logK, w.neighbors, dKo)) < 1e-7
+
+
+def _knn_mesh(n=3000, k=12, seed=11):
+    """An unstructured mesh: random points in the unit cube, each joined to
+    its k nearest neighbours (pairs deduplicated), so rows are wider than a
+    7-point stencil and no slice of the operator has a regular offset
+    pattern."""
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, 3))
+    d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    nbr = np.argsort(d2, axis=1)[:, :k]
+    pairs = {(min(i, j), max(i, j)) for i in range(n) for j in nbr[i]}
+    nb = np.array(sorted(pairs), dtype=np.int64) + 1
+    aol = 0.5 + rng.random(nb.shape[0])
+    logK = -9.0 + rng.normal(size=n)
+    dn = np.array([1, 2, 3, n], dtype=np.int64)
+    dh = np.zeros(n)
+    dh[dn - 1] = [1.0, 0.5, 0.0, 2.0]
+    Q = np.zeros(n)
+    Q[rng.choice(n, 5, replace=False)] = 1e-4
+    return nb, aol, logK, dn, dh, Q
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_pc"])
+def test_unstructured_mesh_vs_oracle(pkg, api, oracle, precision):
+    _say("unstructured", precision)
+    nb, aol, logK, dn, dh, Q = _knn_mesh()
+    Ke = api.logKs2Ks_harmonic(logK, nb, dn, aol)
+    h, pb = api.rrule_groundwater_steadystate(Ke, nb, aol, dn, dh, Q, reltol=1e-12,
+                                              precision=precision)
+    ho, pbo = oracle.steadystate_rrule(Ke, nb, aol, dn, dh, Q, reltol=1e-14)
+    assert rel(h, ho) < (1e-10 if precision == "fp64" else 1e-8)
+    delta = np.zeros(h.size)
+    delta[::37] = 1.0
+    g = pb(delta)
+    go = pbo(delta)
+    assert rel(g[1], go[1]) < 1e-8 and rel(g[5], go[5]) < 1e-8 and rel(g[6], go[6]) < 1e-8
