@@ -609,12 +609,19 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
                      (const T*)level_invd(t, h, l + 1),
                      nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0,
                      coarse_next ? 0 : 1, (const int*)done);
-  T* xc = nullptr;
-  vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
-            nn.xs.template as<T>(), &xc, chs, done, false);
   SellD<T> P = d.P.view<T>(n.pval.template as<T>());
-  hipLaunchKernelGGL(k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur,
-                     (const int*)done);
+  if (coarse_next && dn.A.nrows <= COARSE_FOLD_MAX) {
+    // the dense coarsest solve is folded into the prolongation
+    hipLaunchKernelGGL(k_prolong_coarse<T>, dim3(g), dim3(BLK), 0, s, P, (int)dn.A.nrows,
+                       (const double*)h.W.template as<double>(), (const T*)nn.b.template as<T>(),
+                       cur, (const int*)done);
+  } else {
+    T* xc = nullptr;
+    vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
+              nn.xs.template as<T>(), &xc, chs, done, false);
+    hipLaunchKernelGGL(k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur,
+                       (const int*)done);
+  }
   for (int k = 0; k < m; k++) {
     double c1 = k == 0 ? 0.0 : ch.c1[k];
     double c2 = k == 0 ? ch.c0 : ch.c2[k];

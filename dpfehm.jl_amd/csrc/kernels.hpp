@@ -948,6 +948,38 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
   }
 }
 
+// Dense coarsest solve folded into the prolongation that follows it: every
+// workgroup loads W (nc <= COARSE_FOLD_MAX rows, at most 32 KiB) and bc into
+// LDS with independent loads, forms xc = W bc there (each row in the order
+// k_coarse_solve uses), then xf += P xc for its rows.  One launch instead of
+// two; results identical to k_coarse_solve + k_prolong.
+constexpr int COARSE_FOLD_MAX = 64;
+template <typename T>
+__global__ void __launch_bounds__(BLK)
+    k_prolong_coarse(SellD<T> P, int nc, const double* __restrict__ W,
+                     const T* __restrict__ bc, T* __restrict__ xf,
+                     const int* __restrict__ done) {
+  if (*done) return;
+  __shared__ double ws[COARSE_FOLD_MAX * COARSE_FOLD_MAX];
+  __shared__ double bs[COARSE_FOLD_MAX], xs[COARSE_FOLD_MAX];
+  for (int t = threadIdx.x; t < nc * nc; t += BLK) ws[t] = W[t];
+  for (int t = threadIdx.x; t < nc; t += BLK) bs[t] = (double)bc[t];
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    int i = threadIdx.x;
+    double a = 0.0;
+    for (int j = 0; j < nc; j++) a += ws[j * nc + i] * bs[j];
+    xs[i] = (double)(T)a;
+  }
+  __syncthreads();
+  int64_t i = logical_row();
+  if (i >= P.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(P, i, base, w);
+  xf[i] += sell_dot(P, i, base, w, (const double*)xs);
+}
+
 // xf += P xc
 template <typename T>
 __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ xf,
