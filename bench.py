@@ -368,7 +368,9 @@ def bench_ensemble(args, P, api, _lib, torch):
         res[precision] = (dt, stats, float(rr.max()))
         log(f"[bench] ensemble {precision}: {args.steps / dt:.2f} batch evals/s = "
             f"{len(ms) * args.steps / dt:.1f} member forward+adjoint evals/s; iterations "
-            f"{stats[-1][0]['iterations']}/{stats[-1][1]['iterations']}, worst member relres {rr.max():.2e}")
+            f"{stats[-1][0]['iterations']}/{stats[-1][1]['iterations']} in "
+            f"{stats[-1][0]['outer']}/{stats[-1][1]['outer']} refinement steps, worst member "
+            f"relres {rr.max():.2e}, tuned {topo.tuned_degrees().get(precision)}")
     value = len(ms) * args.steps / res["mixed"][0]
     out = {
         "metric": "member forward+adjoint evals/s, 96-realization packed ensemble, mixed",

@@ -244,7 +244,16 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
       if (abs_target) {
         stop = m.target * m.target;
       } else {
-        stop = reltol * reltol * tot;
+        // the inner solve of a refinement step, whose right-hand side is the
+        // FP64 residual over its norm outer_s: relative tolerance reltol,
+        // unless the step can meet the refinement target by itself (target /
+        // outer_s at most ten times below reltol), in which case it aims at
+        // half the target so that no further step is needed for want of a
+        // few digits
+        double rel = reltol;
+        double rho = m.outer_s > 0.0 ? m.target / m.outer_s : 0.0;
+        if (rho >= 0.1 * reltol) rel = 0.5 * rho;
+        stop = rel * rel * tot;
         if (stop < abstol * abstol) stop = abstol * abstol;
       }
       m.rr_stop = stop;

@@ -50,7 +50,9 @@ typedef struct {
   double abstol;
   int32_t maxiter;     /* PCG iteration cap, summed over refinement steps */
   int32_t precision;   /* DPF_FP64, DPF_MIXED or DPF_MIXED_PC (above) */
-  double inner_reltol; /* DPF_MIXED: relative tolerance of each FP32 inner solve */
+  double inner_reltol; /* DPF_MIXED: relative tolerance of each FP32 inner solve; a step
+                          whose residual is within 10/inner_reltol of the target solves
+                          to half the target instead */
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
   int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = autotuned:
                           the first solve on a topology (per precision) times
