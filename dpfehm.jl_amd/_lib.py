@@ -74,6 +74,7 @@ SIGNATURES = {
     "dpf_batch_topology_create": (ctypes.c_int, [_i32, _p, _p, _p, _p, _p, _p, ctypes.POINTER(_p)]),
     "dpf_member_stats": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_info": (ctypes.c_int, [_p, _p, _i32]),
+    "dpf_topology_tuning": (ctypes.c_int, [_p, _i32, _p, _p, _i32]),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
@@ -124,11 +125,19 @@ def check(code):
         raise DpfError(code, msg)
 
 
+def precision_code(v):
+    """DPF_FP64 / DPF_MIXED / DPF_MIXED_PC from "fp64" / "mixed" / "mixed_pc"
+    (an int passes through)."""
+    if isinstance(v, str):
+        return {"fp64": DPF_FP64, "mixed": DPF_MIXED, "mixed_pc": DPF_MIXED_PC}[v]
+    return int(v)
+
+
 def default_opts(**kw):
     o = Opts()
     lib().dpf_opts_init(ctypes.byref(o))
     for k, v in kw.items():
-        if k == "precision" and isinstance(v, str):
-            v = {"fp64": DPF_FP64, "mixed": DPF_MIXED, "mixed_pc": DPF_MIXED_PC}[v]
+        if k == "precision":
+            v = precision_code(v)
         setattr(o, k, v)
     return o

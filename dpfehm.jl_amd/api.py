@@ -19,7 +19,7 @@ Every call goes through libdpfgpu.so (HIP kernels on the GPU).  The topology
 recomputes it in every call.
 
 Keyword arguments: ``reltol`` (default 1e-10), ``abstol``, ``maxiter``,
-``precision`` ("fp64" or "mixed"), ``inner_reltol``, ``max_outer``,
+``precision`` ("fp64", "mixed" or "mixed_pc"), ``inner_reltol``, ``max_outer``,
 ``cheb_degree``.  ``linear_solver`` is accepted and ignored, as the
 reference's rrule ignores it (src/groundwater.jl:90).  Unknown keywords raise
 TypeError.  Non-convergence raises ``NotConverged`` instead of returning
@@ -131,6 +131,16 @@ class Topology:
         if i[13]:
             out["mixed_pc"] = (int(i[13]), int(i[14]), int(i[17]))
         return out
+
+    def tuning(self, precision="fp64"):
+        """The autotuner's trials for ``precision``: [(ms, iterations)] for
+        the degree pairs (1,1), (1,2), (2,1), (2,2), (3,2) at interval ratio
+        15, then the best pair at ratios 30 and 8 (empty before tuning)."""
+        ms = np.zeros(7)
+        its = np.zeros(7, dtype=np.int32)
+        _lib.check(_lib.lib().dpf_topology_tuning(self._h, _lib.precision_code(precision),
+                                                  _ptr(ms), _ptr(its), 7))
+        return [(float(a), int(b)) for a, b in zip(ms, its)] if its.any() else []
 
     @property
     def handle(self):

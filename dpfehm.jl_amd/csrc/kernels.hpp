@@ -354,6 +354,9 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     active += q.outer_s > q.target ? 1 : 0;
   };
   if (nm == 1) {
+    // the member's record is fetched while the partials load
+    SolveScalars q;
+    if (threadIdx.x == 0) q = sc[0];
     // eight independent loads per thread in flight at a time
     double s = 0.0;
     for (int b0 = threadIdx.x; b0 < nblk; b0 += 8 * 1024) {
@@ -372,7 +375,6 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     if (threadIdx.x != 0) return;
     double tot = 0.0;
     for (int k = 0; k < NW; k++) tot = mx ? fmax(tot, sh[k]) : tot + sh[k];
-    SolveScalars q = sc[0];
     finish_member(mode, tot, q, reltol, abstol, maxiter, abs_target, 0);
     sc[0] = q;
     if (mode > FIN_OUTER) return;
@@ -437,9 +439,13 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
   }
   int dn = running ? 0 : (broke ? 3 : (capped ? 2 : 1));
   *gdone = dn;
-  mput(&hm->rr, rr);
-  mput(&hm->iters, maxit);
-  mput(&hm->done, dn);
+  // the host follows the iteration count (FIN_XR) and the end of the solve;
+  // the other steps of a running iteration leave the mirror as it is
+  if (mode == FIN_START || mode == FIN_XR || dn != 0) {
+    mput(&hm->rr, rr);
+    mput(&hm->iters, maxit);
+    mput(&hm->done, dn);
+  }
   if (mode == FIN_START) mpublish(&hm->seq, seq);
 }
 
@@ -662,7 +668,7 @@ __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict
 template <typename T>
 __global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
                           const T* __restrict__ pval, T* __restrict__ rval) {
-  int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int64_t e = logical_row();
   if (e >= nent) return;
   int32_t s = r_src[e];
   rval[e] = s >= 0 ? pval[s] : (T)0;
@@ -681,7 +687,9 @@ __global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict_
                               const int32_t* __restrict__ dest, const int32_t* __restrict__ s1,
                               const int32_t* __restrict__ s2, const T* __restrict__ X,
                               const T* __restrict__ Y, T* __restrict__ out) {
-  int64_t sub = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  // each XCD takes a contiguous range of lists, so that the rows of X and Y
+  // the lists gather from stay in that XCD's L2
+  int64_t sub = logical_row();
   int64_t k = sub >> lg;
   int G = 1 << lg;
   T acc = (T)0;
@@ -963,6 +971,8 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
 // k_coarse_solve uses), then xf += P xc for its rows.  One launch instead of
 // two; results identical to k_coarse_solve + k_prolong.
 constexpr int COARSE_FOLD_MAX = 64;
+static_assert(COARSE_FOLD_MAX * COARSE_FOLD_MAX % BLK == 0 && COARSE_FOLD_MAX <= BLK,
+              "the fold kernel's W staging assumes these");
 template <typename T>
 __global__ void __launch_bounds__(BLK)
     k_prolong_coarse(SellD<T> P, int nc, const double* __restrict__ W,
@@ -971,12 +981,26 @@ __global__ void __launch_bounds__(BLK)
   if (*done) return;
   __shared__ double ws[COARSE_FOLD_MAX * COARSE_FOLD_MAX];
   __shared__ double bs[COARSE_FOLD_MAX], xs[COARSE_FOLD_MAX];
-  for (int t = threadIdx.x; t < nc * nc; t += BLK) ws[t] = W[t];
-  for (int t = threadIdx.x; t < nc; t += BLK) bs[t] = (double)bc[t];
+  // all of a thread's loads of W in flight at once, then the LDS stores
+  constexpr int PER = COARSE_FOLD_MAX * COARSE_FOLD_MAX / BLK;
+  const int nn = nc * nc;
+  double wv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    int t = threadIdx.x + k * BLK;
+    wv[k] = t < nn ? W[t] : 0.0;
+  }
+  if (threadIdx.x < nc) bs[threadIdx.x] = (double)bc[threadIdx.x];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    int t = threadIdx.x + k * BLK;
+    if (t < nn) ws[t] = wv[k];
+  }
   __syncthreads();
   if (threadIdx.x < nc) {
     int i = threadIdx.x;
     double a = 0.0;
+#pragma unroll 8
     for (int j = 0; j < nc; j++) a += ws[j * nc + i] * bs[j];
     xs[i] = (double)(T)a;
   }
@@ -1012,8 +1036,17 @@ __global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __
   for (int j = threadIdx.x; j < nc; j += blockDim.x) bs[j] = (double)b[j];
   __syncthreads();
   for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+    // sixteen loads of W in flight at a time; the sum stays in j order
     double s = 0.0;
-    for (int j = 0; j < nc; j++) s += W[(int64_t)j * nc + i] * bs[j];
+    int j = 0;
+    for (; j + 16 <= nc; j += 16) {
+      double v[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) v[k] = W[(int64_t)(j + k) * nc + i];
+#pragma unroll
+      for (int k = 0; k < 16; k++) s += v[k] * bs[j + k];
+    }
+    for (; j < nc; j++) s += W[(int64_t)j * nc + i] * bs[j];
     x[i] = (T)s;
   }
 }

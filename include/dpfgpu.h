@@ -125,6 +125,16 @@ int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers)
  * DPF_MIXED and DPF_MIXED_PC. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
+/* The Chebyshev autotuner's trials for one precision (DPF_FP64, DPF_MIXED or
+ * DPF_MIXED_PC), zeros before its first solve: ms[k] / iters[k] for k < min(n,
+ * 7) = wall time and PCG iterations of trial k, which solves the first
+ * refinement step's system with (finest, coarse) degrees (1,1), (1,2), (2,1),
+ * (2,2), (3,2) at interval ratio 15 (k = 0..4), then with the best of these
+ * at ratios 30 and 8 (k = 5, 6); -1 ms marks a trial that did not converge.
+ * No counterpart in the reference (its AMG smoother is fixed). */
+int dpf_topology_tuning(const dpf_topology *topo, int32_t precision, double *ms,
+                        int32_t *iters, int32_t n);
+
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs, specificstorage, volumes), src/groundwater.jl:1-38.
  *   hfree: nfree;  Ks: nedges;  dirichleths, Qs, specificstorage, volumes: n;

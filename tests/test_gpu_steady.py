@@ -346,3 +346,35 @@ def test_unstructured_mesh_vs_oracle(pkg, api, oracle, precision):
     g = pb(delta)
     go = pbo(delta)
     assert rel(g[1], go[1]) < 1e-8 and rel(g[5], go[5]) < 1e-8 and rel(g[6], go[6]) < 1e-8
+
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_long_range_edges_vs_oracle(pkg, api, oracle, precision):
+    """A 50x40x36 box whose first 64 cells also connect to cells near the far
+    end (72,000 cells): those rows of the fine operator and of R span more
+    than 65536 columns and break the offset-slice pattern; heads and gradient
+    must match the oracle's."""
+    _say("wide slices", precision)
+    ns = (50, 40, 36)
+    coords, nb, aol, _ = pkg.grid.regulargrid3d((0.0, 0.0, 0.0), (49.0, 39.0, 35.0), ns)
+    n = coords.shape[1]
+    far = np.stack([np.arange(1, 65), n - 7 * np.arange(1, 65)], axis=1)
+    nb = np.concatenate([nb, far]).astype(np.int64)
+    rng = np.random.default_rng(5)
+    aol = np.concatenate([aol, 0.5 + rng.random(64)])
+    logK = -9.0 + 0.5 * rng.normal(size=n)
+    dn = np.arange(ns[2], n + 1, ns[2], dtype=np.int64)
+    dh = np.zeros(n)
+    Q = np.zeros(n)
+    Q[[0, 999, 40000]] = [1e-4, -5e-5, 2e-5]
+    lay = api.Topology(n, nb, aol, dn).layout()
+    assert 0 < lay["fine_offset_slices"] < (n - dn.size + 63) // 64
+    Ke = api.logKs2Ks_harmonic(logK, nb, dn, aol)
+    h, pb = api.rrule_groundwater_steadystate(Ke, nb, aol, dn, dh, Q, reltol=1e-12,
+                                              precision=precision)
+    ho, pbo = oracle.steadystate_rrule(Ke, nb, aol, dn, dh, Q, reltol=1e-14)
+    assert rel(h, ho) < (1e-10 if precision == "fp64" else 1e-8)
+    delta = np.zeros(n)
+    delta[::101] = 1.0
+    assert rel(pb(delta)[1], pbo(delta)[1]) < 1e-8
