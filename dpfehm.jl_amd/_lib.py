@@ -75,6 +75,11 @@ SIGNATURES = {
     "dpf_member_stats": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_info": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_tuning": (ctypes.c_int, [_p, _i32, _p, _p, _i32]),
+    "dpf_twophase_tpfa": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p, _p]),
+    "dpf_twophase_upstream": (ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p, _p,
+                                             _p]),
+    "dpf_solvetwophase": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i32,
+                                         _i32, _p, _p, _p, _p, _p, _p, _p]),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),

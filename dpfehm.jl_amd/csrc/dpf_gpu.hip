@@ -26,6 +26,7 @@
 #include "../../include/dpfgpu.h"
 #include "dpf_host.h"
 #include "kernels.hpp"
+#include "twophase.hpp"
 
 namespace dpf {
 
@@ -1982,3 +1983,280 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     return status;
   });
 }
+
+// ====================================================================== two-phase (IMPES)
+
+namespace {
+
+Fluid make_fluid(const double* fl) {
+  Fluid f;
+  f.vw = fl[0];
+  f.vo = fl[1];
+  f.swc = fl[2];
+  f.sor = fl[3];
+  f.span = (1.0 - f.swc) - f.sor;
+  return f;
+}
+
+int twophase_checks(const dpf_topology* t) {
+  if (t->nm > 1) {
+    set_err("the two-phase calls take a single-system topology");
+    return DPF_EINVAL;
+  }
+  return DPF_OK;
+}
+
+// With t->Ke = K_a + K_b, t->dh and t->Q in place: pressure P (device, n) of
+// the transmissivity system, shifted to mean zero if asked, and fluxes Vn
+// (device, ne).
+int tpfa_core(dpf_topology* t, hipStream_t s, const dpf_opts* o, dpf_stats* st, double* P,
+              double* Vn, int zero_mean) {
+  int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+  prepare(t, s, o->precision, t->Q.as<double>(), true);
+  DBuf hf(t->pool.get());
+  hf.alloc(nf * sizeof(double));
+  int status = solve_system(t, s, t->b64.as<double>(), hf.as<double>(), o, st);
+  hipLaunchKernelGGL(k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                     (const double*)hf.as<double>(), t->dh.as<double>(), P);
+  if (zero_mean) {
+    unsigned nb = nblocks(n);
+    DBuf part(t->pool.get());
+    part.alloc((nb + 1) * sizeof(double));
+    double* pp = part.as<double>();
+    hipLaunchKernelGGL(k_tp_blocksum, dim3(nb), dim3(BLK), 0, s, n, (const double*)P, pp);
+    hipLaunchKernelGGL(k_tp_mean, dim3(1), dim3(64), 0, s, n, (int64_t)nb, (const double*)pp,
+                       pp + nb);
+    hipLaunchKernelGGL(k_tp_sub, dim3(nb), dim3(BLK), 0, s, n, P, (const double*)(pp + nb));
+  }
+  hipLaunchKernelGGL(k_tp_flux, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
+                     t->aol.as<double>(), (const double*)t->Ke.as<double>(), (const double*)P,
+                     Vn);
+  HIPTRY(hipGetLastError());
+  if (status == DPF_ENOCONV) set_err("AMG-PCG did not reach the tolerance within maxiter");
+  if (status == DPF_EBREAKDOWN) set_err("AMG-PCG breakdown (non-SPD operator or NaN input)");
+  return status;
+}
+
+// Upstream transport over time T of S (device, n, in place), as
+// src/twophase.jl:208-235.  Returns the number of CFL sub-steps.
+int64_t upstream_core(dpf_topology* t, hipStream_t s, double* S, const Fluid& f,
+                      const double* Q, double T, const double* P, const double* Vn,
+                      const double* pv) {
+  int64_t n = t->H.n;
+  unsigned nb = nblocks(n);
+  DBuf part(t->pool.get());
+  part.alloc(nb * sizeof(double));
+  hipLaunchKernelGGL(k_tp_cfl, dim3(nb), dim3(BLK), 0, s, n, t->edges.as<int2>(),
+                     t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(), Vn, Q, pv,
+                     part.as<double>());
+  HIPTRY(hipGetLastError());
+  std::vector<double> hp(nb);
+  HIPTRY(hipMemcpyAsync(hp.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  // minimum over the blocks, NaN-propagating like Julia's minimum
+  double pm = INFINITY;
+  for (double v : hp) pm = (v != v || pm != pm) ? v + pm : std::min(pm, v);
+  double cfl = (f.span / 3.0) * pm;
+  double steps = std::ceil(T / cfl);
+  if (!(steps == steps) || std::fabs(steps) > 1e9)
+    throw Err{DPF_EINVAL, "CFL sub-step count T/cfl = " + std::to_string(T / cfl) +
+                              " is not finite or exceeds 1e9 (no inflow, zero pore volume or "
+                              "non-finite fluxes?)"};
+  int64_t nts = (int64_t)steps;
+  if (nts <= 0) return 0;
+  double tau = T / (double)nts;
+  DBuf other(t->pool.get());
+  other.alloc(n * sizeof(double));
+  double* cur = S;
+  double* oth = other.as<double>();
+  double lo = f.swc, hi = 1.0 - f.sor;
+  for (int64_t k = 0; k < nts; k++) {
+    hipLaunchKernelGGL(k_tp_upstream, dim3(nb), dim3(BLK), 0, s, n, t->inc_ptr.as<int64_t>(),
+                       t->inc_edge.as<int32_t>(), t->inc_other.as<int32_t>(),
+                       (const double*)cur, oth, Q, P, Vn, pv, tau, f, lo, hi);
+    std::swap(cur, oth);
+  }
+  HIPTRY(hipGetLastError());
+  if (cur != S) copy_dev(S, cur, n * sizeof(double), s);
+  return nts;
+}
+
+// Host or device input -> device pointer (staged through buf for DPF_HOST).
+const double* dev_in(dpf_topology* t, const dpf_opts* o, DBuf& buf, const double* p,
+                     int64_t count, hipStream_t s) {
+  return stage_in(t, o, buf, p, count, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpf_twophase_tpfa(dpf_topology* t, const double* K, const double* dirichleths,
+                      const double* Qs, double* P, double* Vn, int32_t zero_mean,
+                      const dpf_opts* opts, dpf_stats* stats) {
+  if (!t || !K || !dirichleths || !Qs || !P || !Vn) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne;
+    DBuf kb(t->pool.get()), pb(t->pool.get()), vb(t->pool.get());
+    const double* Kd = dev_in(t, o, kb, K, n, s);
+    hipLaunchKernelGGL(k_tp_ksum, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
+                       Kd, t->Ke.as<double>());
+    if (o->memory == DPF_DEVICE) {
+      copy_dev(t->dh.p, dirichleths, n * sizeof(double), s);
+      copy_dev(t->Q.p, Qs, n * sizeof(double), s);
+    } else {
+      HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPTRY(hipMemcpyAsync(t->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    double* Pd = P;
+    double* Vd = Vn;
+    if (o->memory == DPF_HOST) {
+      pb.alloc(n * sizeof(double));
+      vb.alloc(ne * sizeof(double));
+      Pd = pb.as<double>();
+      Vd = vb.as<double>();
+    }
+    int status = tpfa_core(t, s, o, stats, Pd, Vd, zero_mean);
+    if (o->memory == DPF_HOST) {
+      HIPTRY(hipMemcpyAsync(P, Pd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPTRY(hipMemcpyAsync(Vn, Vd, ne * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) stats->total_ms = now_ms() - t0;
+    return status;
+  });
+}
+
+int dpf_twophase_upstream(dpf_topology* t, double* S, const double* fluid, const double* Qs,
+                          double T, const double* P, const double* Vn, const double* volumes,
+                          int64_t* nsub, const dpf_opts* opts) {
+  if (nsub) *nsub = 0;
+  if (!t || !S || !fluid || !Qs || !P || !Vn || !volumes) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne;
+    DBuf sb(t->pool.get()), qb(t->pool.get()), pb(t->pool.get()), vb(t->pool.get()),
+        wb(t->pool.get());
+    double* Sd = S;
+    if (o->memory == DPF_HOST) {
+      sb.alloc(n * sizeof(double));
+      Sd = sb.as<double>();
+      HIPTRY(hipMemcpyAsync(Sd, S, n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    const double* Qd = dev_in(t, o, qb, Qs, n, s);
+    const double* Pd = dev_in(t, o, pb, P, n, s);
+    const double* Vd = dev_in(t, o, vb, Vn, ne, s);
+    const double* Wd = dev_in(t, o, wb, volumes, n, s);
+    int64_t k = upstream_core(t, s, Sd, make_fluid(fluid), Qd, T, Pd, Vd, Wd);
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(S, Sd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (nsub) *nsub = k;
+    return DPF_OK;
+  });
+}
+
+int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const double* K,
+                      const double* dirichleths, const double* Qs, const double* volumes,
+                      const double* fluid, double dt, int32_t nt, int32_t zero_mean,
+                      double* P_out, double* S_out, double* P_all, double* S_all,
+                      int64_t* nsub_total, const dpf_opts* opts, dpf_stats* stats) {
+  if (nsub_total) *nsub_total = 0;
+  if (!t || !h0 || !S0 || !K || !dirichleths || !Qs || !volumes || !fluid || !P_out ||
+      !S_out || nt < 0) {
+    set_err(nt < 0 ? "nt must be >= 0" : "NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne;
+    bool host = o->memory == DPF_HOST;
+    auto out = [&](double* dst, const double* src, int64_t count) {
+      if (host)
+        HIPTRY(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+      else
+        copy_dev(dst, src, count * sizeof(double), s);
+    };
+    DBuf kb(t->pool.get()), qb(t->pool.get()), wb(t->pool.get()), sb(t->pool.get()),
+        pb(t->pool.get()), vb(t->pool.get()), mb(t->pool.get());
+    const double* Kd = dev_in(t, o, kb, K, n, s);
+    const double* Qd = dev_in(t, o, qb, Qs, n, s);
+    const double* Wd = dev_in(t, o, wb, volumes, n, s);
+    sb.alloc(n * sizeof(double));
+    pb.alloc(n * sizeof(double));
+    vb.alloc(ne * sizeof(double));
+    mb.alloc(n * sizeof(double));
+    double* Sd = sb.as<double>();
+    double* Pd = pb.as<double>();
+    double* Vd = vb.as<double>();
+    HIPTRY(hipMemcpyAsync(Sd, S0, n * sizeof(double),
+                          host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipMemcpyAsync(Pd, h0, n * sizeof(double),
+                          host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double),
+                          host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    copy_dev(t->Q.p, Qd, n * sizeof(double), s);
+    Fluid f = make_fluid(fluid);
+    int64_t subs = 0;
+    int status = DPF_OK;
+    int iters = 0;
+    for (int32_t step = 0; step < nt && status == DPF_OK; step++) {
+      // total mobility times K, arithmetic mean per edge, pressure and fluxes,
+      // then the saturation transport over dt
+      hipLaunchKernelGGL(k_tp_kmob, dim3(nblocks(n)), dim3(BLK), 0, s, n, (const double*)Sd,
+                         Kd, f, mb.as<double>());
+      hipLaunchKernelGGL(k_tp_ksum, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                         t->edges.as<int2>(), (const double*)mb.as<double>(),
+                         t->Ke.as<double>());
+      status = tpfa_core(t, s, o, stats, Pd, Vd, zero_mean);
+      if (stats) iters += stats->iterations;
+      if (status != DPF_OK) break;
+      subs += upstream_core(t, s, Sd, f, Qd, dt, Pd, Vd, Wd);
+      if (P_all) out(P_all + (int64_t)step * n, Pd, n);
+      if (S_all) out(S_all + (int64_t)step * n, Sd, n);
+    }
+    out(P_out, Pd, n);
+    out(S_out, Sd, n);
+    HIPTRY(hipStreamSynchronize(s));
+    if (nsub_total) *nsub_total = subs;
+    if (stats) {
+      stats->iterations = iters;
+      stats->total_ms = now_ms() - t0;
+    }
+    return status;
+  });
+}
+
+}  // extern "C"

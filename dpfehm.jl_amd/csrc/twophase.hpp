@@ -1,0 +1,181 @@
+// Two-phase flow, IMPES (src/twophase.jl): total-mobility pressure solve on the
+// groundwater operator, Darcy fluxes, and the explicit upstream saturation
+// transport with its CFL sub-steps.
+//
+// Every kernel here turns floating-point contraction off, so that each
+// product and sum is rounded on its own, in the order of the reference's
+// element-wise expressions; the results then agree bit for bit with a plain
+// IEEE restatement of those expressions.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace dpf {
+
+// Fluid of src/twophase.jl:201-206 and test/twophase.jl:11-16; span =
+// (1 - swc) - sor, the denominator of the effective saturation.
+struct Fluid {
+  double vw, vo, swc, sor, span;
+};
+
+// Water and oil mobilities of saturation s (relativeperm, src/twophase.jl:201-206).
+__device__ __forceinline__ void tp_mobility(double s, const Fluid& f, double& mw, double& mo) {
+#pragma clang fp contract(off)
+  double S = (s - f.swc) / f.span;
+  mw = S * S / f.vw;
+  double o = 1.0 - S;
+  mo = o * o / f.vo;
+}
+
+// Water fractional flow mw / (mw + mo) (src/twophase.jl:229-230).
+__device__ __forceinline__ double tp_fracflow(double s, const Fluid& f) {
+#pragma clang fp contract(off)
+  double mw, mo;
+  tp_mobility(s, f, mw, mo);
+  return mw / (mw + mo);
+}
+
+// Km_c = (Mw + Mo)(S_c) K_c (src/twophase.jl:247-249).
+__global__ void k_tp_kmob(int64_t n, const double* __restrict__ S, const double* __restrict__ K,
+                          Fluid f, double* __restrict__ Km) {
+#pragma clang fp contract(off)
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double mw, mo;
+  tp_mobility(S[c], f, mw, mo);
+  Km[c] = (mw + mo) * K[c];
+}
+
+// Edge factor of the pressure operator: K_a + K_b.  The reference's
+// transmissibility (2 aol_e)(0.5 (K_a + K_b)) (src/twophase.jl:26,190) equals
+// aol_e (K_a + K_b) exactly (scalings by powers of two), so the groundwater
+// operator with K_e = K_a + K_b is the reference's transmissivity operator.
+__global__ void k_tp_ksum(int64_t ne, const int2* __restrict__ edges,
+                          const double* __restrict__ K, double* __restrict__ Ksum) {
+  int64_t e = logical_row();
+  if (e >= ne) return;
+  int2 ab = edges[e];
+  Ksum[e] = K[ab.x] + K[ab.y];
+}
+
+// Darcy flux Vn_e = (P_a - P_b)(aol_e Ksum_e), which is the reference's
+// P_n (tx K_e) (src/twophase.jl:194-196) exactly.
+__global__ void k_tp_flux(int64_t ne, const int2* __restrict__ edges,
+                          const double* __restrict__ aol, const double* __restrict__ Ksum,
+                          const double* __restrict__ P, double* __restrict__ Vn) {
+#pragma clang fp contract(off)
+  int64_t e = logical_row();
+  if (e >= ne) return;
+  int2 ab = edges[e];
+  Vn[e] = (P[ab.x] - P[ab.y]) * (aol[e] * Ksum[e]);
+}
+
+// NaN-propagating minimum, as Julia's min / minimum.
+__device__ __forceinline__ double tp_min(double a, double b) {
+  return (a != a || b != b) ? a + b : (a < b ? a : b);
+}
+
+// Per cell c the pore-volume-to-inflow ratio pv_c / (Vi_c + max(Q_c, 0)),
+// Vi_c summing the fluxes that enter c, in edge order (src/twophase.jl:213-220);
+// one minimum per block into part[block].
+__global__ void __launch_bounds__(BLK) k_tp_cfl(int64_t n, const int2* __restrict__ edges,
+                                                const int64_t* __restrict__ inc_ptr,
+                                                const int32_t* __restrict__ inc_edge,
+                                                const double* __restrict__ Vn,
+                                                const double* __restrict__ Q,
+                                                const double* __restrict__ pv,
+                                                double* __restrict__ part) {
+#pragma clang fp contract(off)
+  int64_t c = logical_row();
+  double v = INFINITY;
+  if (c < n) {
+    double vi = 0.0;
+    for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+      int32_t e = inc_edge[p];
+      double f = Vn[e];
+      if (f < 0.0) {
+        if (edges[e].x == c) vi = vi - f;
+      } else if (edges[e].y == c) {
+        vi = vi + f;
+      }
+    }
+    double q = Q[c];
+    double fi = (q != q) ? q : (q > 0.0 ? q : 0.0);
+    v = pv[c] / (vi + fi);
+  }
+  // block minimum (the order does not matter for a minimum)
+  for (int o = 32; o >= 1; o >>= 1) v = tp_min(v, __shfl_xor(v, o));
+  __shared__ double sh[BLK / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int k = 1; k < BLK / 64; k++) m = tp_min(m, sh[k]);
+    part[logical_row() / BLK] = m;
+  }
+}
+
+// One CFL sub-step of the upstream transport (src/twophase.jl:227-233): the
+// residual of saturationcalc (src/twophase.jl:85-112, every cell an unknown)
+// R_c = min(Q_c, 0) f_c + sum over the edges at c, in edge order, of the
+// upwind term, then S_c <- clamp(S_c + (R_c dtx_c + max(Q_c, 0) dtx_c), swc,
+// 1 - sor) with dtx_c = tau / pv_c, tau = T / Nts.
+__global__ void k_tp_upstream(int64_t n, const int64_t* __restrict__ inc_ptr,
+                              const int32_t* __restrict__ inc_edge,
+                              const int32_t* __restrict__ inc_other,
+                              const double* __restrict__ S, double* __restrict__ Snew,
+                              const double* __restrict__ Q, const double* __restrict__ P,
+                              const double* __restrict__ Vn, const double* __restrict__ pv,
+                              double tau, Fluid f, double lo, double hi) {
+#pragma clang fp contract(off)
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double sc = S[c];
+  double fc = tp_fracflow(sc, f);
+  double q = Q[c];
+  double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
+  double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
+  double R = qneg * fc;
+  double pc = P[c];
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+    int32_t e = inc_edge[p], k = inc_other[p];
+    double v = Vn[e];
+    double term;
+    if (P[k] - pc >= 0.0) {
+      double fk = tp_fracflow(S[k], f);
+      term = v > 0.0 ? fk * v : -fk * v;
+    } else {
+      term = v > 0.0 ? -fc * v : fc * v;
+    }
+    R = R + term;
+  }
+  double dtx = tau / pv[c];
+  double s = sc + (R * dtx + qpos * dtx);
+  Snew[c] = s > hi ? hi : (s < lo ? lo : s);
+}
+
+// Per-block sums of x (block_sum order) into part[block].
+__global__ void __launch_bounds__(BLK) k_tp_blocksum(int64_t n, const double* __restrict__ x,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[4];
+  int64_t i = logical_row();
+  double s = block_sum(i < n ? x[i] : 0.0, sh);
+  if (threadIdx.x == 0) part[i / BLK] = s;
+}
+
+// *mean = (sum of the nb block sums, in block order) / n (one thread).
+__global__ void k_tp_mean(int64_t n, int64_t nb, const double* __restrict__ part,
+                          double* __restrict__ mean) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int64_t b = 0; b < nb; b++) s += part[b];
+  *mean = s / (double)n;
+}
+
+// x -= *mean
+__global__ void k_tp_sub(int64_t n, double* __restrict__ x, const double* __restrict__ mean) {
+  int64_t i = logical_row();
+  if (i < n) x[i] -= *mean;
+}
+
+}  // namespace dpf

@@ -7,6 +7,7 @@ Host-side input generation (not timed, not on the GPU).  Mirrors
 * ``regulargrid1d``  -- src/grid.jl:79-96
 * ``regulargrid2d``  -- src/grid.jl:109-138
 * ``regulargrid3d``  -- src/grid.jl:150-188
+* ``load_uge``       -- src/grid.jl:38-65 (PFLOTRAN explicit unstructured grid)
 
 Conventions kept from the reference:
 
@@ -96,3 +97,40 @@ def regulargrid3d(mins, maxs, ns):
 def grid_spacing(mins, maxs, ns):
     """Node spacing per axis, ``(max - min) / (n - 1)`` (src/grid.jl:156-158)."""
     return tuple((float(b) - float(a)) / (int(k) - 1) for a, b, k in zip(mins, maxs, ns))
+
+
+def load_uge(filename):
+    """src/grid.jl:38-65: read an ASCII PFLOTRAN explicit unstructured grid
+    (``.uge``) and return ``coords`` (3, n), ``volumes`` (n), ``neighbors``
+    (E, 2) 1-based, ``areas`` (E) and ``lengths`` (E), the distance between
+    the two cell centres of each connection.
+
+    The file holds ``CELLS n`` followed by n lines ``id x y z volume``, then
+    ``CONNECTIONS m`` followed by m lines ``id1 id2 x y z area``.  As in the
+    reference, the cell ids of the CELLS block are not read (cells are taken
+    in file order), the face centre (x, y, z) of a connection is not used,
+    and a malformed line raises.  ``areas / lengths`` is the
+    ``areasoverlengths`` argument of the solvers."""
+    with open(filename) as f:
+        def line():
+            s = f.readline()
+            if not s:
+                raise ValueError(f"{filename}: unexpected end of file")
+            return s.split()
+        numcells = int(line()[1])
+        coords = np.empty((3, numcells))
+        volumes = np.empty(numcells)
+        for i in range(numcells):
+            parts = line()
+            coords[:, i] = [float(parts[1]), float(parts[2]), float(parts[3])]
+            volumes[i] = float(parts[4])
+        numconnections = int(line()[1])
+        neighbors = np.empty((numconnections, 2), dtype=np.int64)
+        areas = np.empty(numconnections)
+        for i in range(numconnections):
+            parts = line()
+            neighbors[i] = int(parts[0]), int(parts[1])
+            areas[i] = float(parts[5])
+    d = coords[:, neighbors[:, 0] - 1] - coords[:, neighbors[:, 1] - 1]
+    lengths = np.sqrt((d * d).sum(axis=0))
+    return coords, volumes, neighbors, areas, lengths

@@ -163,6 +163,46 @@ int dpf_geo_forward(dpf_topology *topo, const double *logK, double *Ke, const dp
 int dpf_geo_pullback(dpf_topology *topo, const double *Ke, const double *dKe, double *dlogK,
                      const dpf_opts *opts);
 
+/* Two-phase flow, IMPES (src/twophase.jl).  Single-system topologies only.
+ * The pressure system is singular when the topology has no Dirichlet cell;
+ * a caller then creates the topology with one pinned cell (head 0) and
+ * passes zero_mean = 1, which shifts P to mean zero afterwards (the
+ * reference's AMG solve of the singular system fixes the constant through
+ * its own iterates; fluxes and saturations do not depend on it).
+ *
+ * tpfa (src/twophase.jl:189-198, with solvepressure :48-56 and its
+ * transmissivity equations :23-44): cell values K (n) -> K_e = 0.5 (K_a +
+ * K_b), pressure P (n; Dirichlet cells hold dirichleths) of the TPFA system
+ * with transmissibility 2 aol_e K_e, solved by AMG-PCG to opts->reltol, and
+ * Darcy fluxes Vn (nedges) = (P_a - P_b) 2 aol_e K_e. */
+int dpf_twophase_tpfa(dpf_topology *topo, const double *K, const double *dirichleths,
+                      const double *Qs, double *P, double *Vn, int32_t zero_mean,
+                      const dpf_opts *opts, dpf_stats *stats);
+
+/* upstream (src/twophase.jl:208-235): explicit upwind transport of the water
+ * saturation S (n, updated in place) over time T, in ceil(T / cfl) equal
+ * sub-steps, cfl = ((1 - swc - sor) / 3) min(pv / (inflow + max(Q, 0))),
+ * pv = volumes; each sub-step adds the saturationcalc residual
+ * (src/twophase.jl:85-112) times dtx = (T / Nts) / pv and max(Q, 0) dtx, then
+ * clamps to [swc, 1 - sor].  fluid = {vw, vo, swc, sor}
+ * (relativeperm, src/twophase.jl:201-206).  *nsub (may be NULL) = Nts.
+ * opts: memory and stream only. */
+int dpf_twophase_upstream(dpf_topology *topo, double *S, const double *fluid, const double *Qs,
+                          double T, const double *P, const double *Vn, const double *volumes,
+                          int64_t *nsub, const dpf_opts *opts);
+
+/* solvetwophase (src/twophase.jl:238-264): nt IMPES steps from (h0, S0); each
+ * step forms the total mobility Mt(S) K, runs tpfa, then upstream over dt.
+ * P_out, S_out (n): the last step's (h0 and S0 when nt = 0); P_all, S_all
+ * (nt x n, step-major, may be NULL): every step's, the reference's everyStep
+ * output.  *nsub_total (may be NULL) = CFL sub-steps over all steps; stats:
+ * PCG iterations summed over the pressure solves. */
+int dpf_solvetwophase(dpf_topology *topo, const double *h0, const double *S0, const double *K,
+                      const double *dirichleths, const double *Qs, const double *volumes,
+                      const double *fluid, double dt, int32_t nt, int32_t zero_mean,
+                      double *P_out, double *S_out, double *P_all, double *S_all,
+                      int64_t *nsub_total, const dpf_opts *opts, dpf_stats *stats);
+
 /* groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs; kwargs...) (src/groundwater.jl:75-83) and the forward
  * half of its rrule (src/groundwater.jl:85-92): assemble A = dr/dh and

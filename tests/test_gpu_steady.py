@@ -378,3 +378,51 @@ def test_long_range_edges_vs_oracle(pkg, api, oracle, precision):
     delta = np.zeros(n)
     delta[::101] = 1.0
     assert rel(pb(delta)[1], pbo(delta)[1]) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_uge_mesh_geometric_dirichlet_faces(pkg, api, oracle, precision, tmp_path):
+    """The fracture-network set-up of
+    examples/fracture_network_solver_scaling/utilities.jl:13-32 on a mesh read
+    with load_uge: areasoverlengths = areas ./ lengths, Dirichlet heads 2e6 on
+    the left (x < 0.1) and 1e6 on the right (x > 0.9), no sources, geometric-
+    mean K_e = sqrt(K_a K_b) from cell permeabilities.  The mesh is a
+    k-nearest-neighbour graph on random points, written as a .uge first."""
+    from test_grid import _write_uge
+    _say("uge mesh", precision)
+    rng = np.random.default_rng(21)
+    n, k = 4000, 8
+    pts = rng.random((3, n))
+    d2 = ((pts[:, :, None] - pts[:, None, :]) ** 2).sum(0)
+    np.fill_diagonal(d2, np.inf)
+    nbr = np.argsort(d2, axis=1)[:, :k]
+    pairs = sorted({(min(i, j), max(i, j)) for i in range(n) for j in nbr[i]})
+    nb = np.array(pairs, dtype=np.int64) + 1
+    areas = 1e-2 * (0.5 + rng.random(nb.shape[0]))
+    vols = 1e-3 * (0.5 + rng.random(n))
+    path = tmp_path / "dfn.uge"
+    _write_uge(path, pts, vols, nb, areas)
+    coords, volumes, neighbors, areas2, lengths = pkg.grid.load_uge(str(path))
+    aol = areas2 / lengths
+    left = np.nonzero(coords[0] < 0.1)[0] + 1
+    right = np.nonzero(coords[0] > 0.9)[0] + 1
+    dn = np.concatenate([left, right])
+    dh = np.zeros(n)
+    dh[left - 1] = 2e6
+    dh[right - 1] = 1e6
+    Q = np.zeros(n)
+    logK = np.log(1e-12) + rng.normal(size=n)
+    Ke = api.logKs2Ks_geometric(logK, neighbors, dn, aol)
+    assert rel(Ke, np.sqrt(np.exp(logK)[neighbors[:, 0] - 1] * np.exp(logK)[neighbors[:, 1] - 1])) < 1e-14
+    h, pb = api.rrule_groundwater_steadystate(Ke, neighbors, aol, dn, dh, Q, reltol=1e-12,
+                                              precision=precision)
+    ho, pbo = oracle.steadystate_rrule(Ke, neighbors, aol, dn, dh, Q, reltol=1e-14)
+    assert rel(h, ho) < (1e-10 if precision == "fp64" else 1e-8)
+    assert h[left - 1].min() == 2e6 and h[right - 1].max() == 1e6
+    free = np.setdiff1d(np.arange(n), dn - 1)
+    # maximum principle (to the solve's tolerance)
+    assert h[free].min() >= 1e6 * (1 - 1e-7) and h[free].max() <= 2e6 * (1 + 1e-7)
+    delta = np.zeros(n)
+    delta[free[::50]] = 1.0
+    g, go = pb(delta), pbo(delta)
+    assert rel(g[1], go[1]) < 1e-8 and rel(g[5], go[5]) < 1e-8

@@ -1,0 +1,135 @@
+"""Two-phase IMPES on the GPU (dpf_twophase_tpfa / _upstream / dpf_solvetwophase)
+against the oracle, the textbook computation and the Buckley-Leverett gate of
+the reference's test/twophase.jl."""
+import numpy as np
+import pytest
+
+from test_oracle_twophase import (FLUID, _random_box, _to_grid_order, buckley_leverett_column,
+                                  quarter_five_spot)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tpo():
+    from oracle import twophase_oracle
+    return twophase_oracle
+
+
+@pytest.fixture(scope="module")
+def tp(pkg):
+    import importlib
+    return importlib.import_module(pkg.__name__ + ".twophase")
+
+
+def _say(*a):
+    print("[twophase]", *a, flush=True)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_upstream_matches_oracle_bitwise(pkg, tp, tpo):
+    """Same P and Vn in: the GPU sub-steps reproduce the oracle's arithmetic
+    (no contraction, same order) bit for bit."""
+    _say("upstream")
+    nb, aol, vol, K, Q, S, n, _ = _random_box(pkg, ns=(12, 10, 8))
+    dn = np.array([n // 3], dtype=np.int64)
+    P, Vn = tpo.tpfa(K, np.zeros(n), dn, Q, aol, nb)
+    for T in (0.3, 2.0):
+        So, nts = tpo.upstream(S, FLUID, Q, T, P, Vn, nb, vol)
+        Sg = tp.upstream(S, FLUID, Q, T, P, Vn, nb, vol, dn, aol)
+        assert tp.upstream.last_substeps == nts > 0
+        assert np.array_equal(Sg, So)
+
+
+def test_tpfa_matches_oracle(pkg, tp, tpo):
+    _say("tpfa")
+    nb, aol, vol, K, Q, S, n, _ = _random_box(pkg, ns=(20, 16, 10))
+    dn = np.array([1, n // 3, n - 5], dtype=np.int64)
+    dh = np.zeros(n)
+    dh[dn - 1] = [1.0, 0.5, -0.25]
+    P, Vn = tp.tpfa(K, dh, dn, Q, aol, nb)
+    Po, Vo = tpo.tpfa(K, dh, dn, Q, aol, nb)
+    assert rel(P, Po) < 1e-10 and rel(Vn, Vo) < 1e-9
+    assert np.array_equal(P[dn - 1], dh[dn - 1])
+
+
+def test_quarter_five_spot(pkg, tp, tpo):
+    """The reference test's 16x16 case (no Dirichlet cell): against the textbook
+    computation per step and the reference's total_norm gate."""
+    _say("quarter five-spot")
+    N = 16
+    nb, aol, vol, q, n = quarter_five_spot(pkg, N)
+    Ps, Ss = tp.solvetwophase(np.zeros(n), np.zeros(n), np.ones(n), np.zeros(n), [], q, vol,
+                              aol, FLUID, 0.7 / 25, nb, 5, everyStep=True)
+    Pt, St = tpo.textbook_quarter_five_spot(N, 5)
+    for k in range(5):
+        pt, st = _to_grid_order(Pt[k], N), _to_grid_order(St[k], N)
+        assert rel(Ps[k], pt) < 1e-9
+        assert np.abs(Ss[k] - st).max() < 1e-9
+    P, S = tp.solvetwophase(np.zeros(n), np.zeros(n), np.ones(n), np.zeros(n), [], q, vol, aol,
+                            FLUID, 0.7 / 25, nb, 5)
+    assert np.array_equal(P, Ps[-1]) and np.array_equal(S, Ss[-1])
+    total = np.linalg.norm(P - _to_grid_order(Pt[-1], N)) + \
+        np.linalg.norm(S - _to_grid_order(St[-1], N))
+    assert total < 1e-5
+
+
+def test_buckley_leverett(pkg, tp, tpo):
+    _say("Buckley-Leverett")
+    coords, nb, aol, vol, dn, Q, n = buckley_leverett_column(pkg)
+    dh = np.zeros(n)
+    P, S = tp.solvetwophase(np.zeros(n), np.zeros(n), np.ones(n), dh, dn, Q, vol, aol, FLUID,
+                            0.7 / 25, nb, 25)
+    Po, So = tpo.solvetwophase(np.zeros(n), np.zeros(n), np.ones(n), dh, dn, Q, vol, aol, FLUID,
+                               0.7 / 25, nb, 25)
+    assert rel(P, Po) < 1e-10 and np.abs(S - So).max() < 1e-10
+    S_BL = tpo.buckley_leverett(coords[0], Q[0] * 0.7)
+    assert abs(np.mean(S_BL - S)) < 1e-2
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_five_spot_3d_vs_oracle(pkg, tp, tpo, precision):
+    """A heterogeneous 3-D five-spot with a clamped fluid (swc, sor > 0) over
+    several steps; the mixed pressure solve within its tolerance."""
+    _say("3-D five-spot", precision)
+    rng = np.random.default_rng(8)
+    ns = (24, 20, 6)
+    coords, nb, aol, vol = pkg.grid.regulargrid3d((0, 0, 0), (23.0, 19.0, 5.0), ns)
+    n = coords.shape[1]
+    K = np.exp(0.7 * rng.normal(size=n))
+    Q = np.zeros(n)
+    Q[[0, 5, 2000]] = [2.0, 1.0, 1.5]
+    dn = np.array([n - i for i in range(6)], dtype=np.int64)
+    dh = np.zeros(n)
+    fluid = (1.0, 5.0, 0.1, 0.15)
+    S0 = np.full(n, 0.1)
+    kw = dict(precision=precision, reltol=1e-12)
+    st = {}
+    P, S = tp.solvetwophase(np.zeros(n), S0, K, dh, dn, Q, vol, aol, fluid, 3.0, nb, 4,
+                            stats=st, **kw)
+    Po, So = tpo.solvetwophase(np.zeros(n), S0, K, dh, dn, Q, vol, aol, fluid, 3.0, nb, 4)
+    assert st["substeps"] > 4
+    assert rel(P, Po) < (1e-10 if precision == "fp64" else 1e-8)
+    assert np.abs(S - So).max() < (1e-9 if precision == "fp64" else 1e-6)
+    assert S.min() >= 0.1 and S.max() <= 0.85 and S.max() > 0.5
+
+
+def test_nt_zero_and_errors(pkg, tp):
+    _say("edge cases")
+    nb, aol, vol, q, n = quarter_five_spot(pkg, 8)
+    h0 = np.linspace(0, 1, n)
+    S0 = np.full(n, 0.2)
+    P, S = tp.solvetwophase(h0, S0, np.ones(n), np.zeros(n), [], q, vol, aol, FLUID, 0.1, nb, 0)
+    assert np.array_equal(P, h0) and np.array_equal(S, S0)
+    # no flow at all: no sub-step, S unchanged
+    S1 = tp.upstream(S0, FLUID, np.zeros(n), 0.5, np.zeros(n), np.zeros(nb.shape[0]), nb, vol)
+    assert tp.upstream.last_substeps == 0 and np.array_equal(S1, S0)
+    # a cell of zero pore volume with inflow: the CFL step is zero
+    P, Vn = tp.tpfa(np.ones(n), np.zeros(n), [], q, aol, nb)
+    v0 = vol.copy()
+    v0[3] = 0.0
+    with pytest.raises(pkg._lib.DpfError):
+        tp.upstream(S0, FLUID, q, 0.5, P, Vn, nb, v0)

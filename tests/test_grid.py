@@ -76,3 +76,44 @@ def test_workload_sizes(pkg):
     assert np.count_nonzero(w.Qs) == 4 and np.all(w.Qs[w.Qs != 0] == 1e-4)
     assert np.intersect1d(np.nonzero(w.Qs)[0] + 1, w.dirichletnodes).size == 0
     assert w.obs.size == 200 and np.intersect1d(w.obs, w.dirichletnodes).size == 0
+
+
+def _write_uge(path, coords, volumes, nb, areas, shuffle_ids=False):
+    with open(path, "w") as f:
+        f.write(f"CELLS {coords.shape[1]}\n")
+        for i in range(coords.shape[1]):
+            cid = coords.shape[1] - i if shuffle_ids else i + 1
+            x, y, z, v = (float(t) for t in (*coords[:, i], volumes[i]))
+            f.write(f"{cid} {x!r} {y!r} {z!r} {v!r}\n")
+        f.write(f"CONNECTIONS {nb.shape[0]}\n")
+        for e in range(nb.shape[0]):
+            c = [float(t) for t in 0.5 * (coords[:, nb[e, 0] - 1] + coords[:, nb[e, 1] - 1])]
+            f.write(f"{nb[e, 0]} {nb[e, 1]} {c[0]!r} {c[1]!r} {c[2]!r} {float(areas[e])!r}\n")
+
+
+def test_load_uge_reproduces_regular_grid(pkg, tmp_path):
+    """A regular grid written as a .uge (face areas dy*dz, dx*dz, dx*dy) reads
+    back with the same neighbours and volumes, lengths equal to the grid
+    spacing along each edge's axis, and areas / lengths equal to
+    regulargrid3d's areasoverlengths (src/grid.jl:38-65 against :150-188)."""
+    ns, mins, maxs = (4, 3, 5), (0.0, 0.0, 0.0), (3.0, 4.0, 2.0)
+    coords, nb, aol, vol = pkg.grid.regulargrid3d(mins, maxs, ns)
+    dx, dy, dz = pkg.grid.grid_spacing(mins, maxs, ns)
+    step = nb[:, 1] - nb[:, 0]
+    spacing = np.where(step == ns[1] * ns[2], dx, np.where(step == ns[2], dy, dz))
+    areas = aol * spacing
+    path = tmp_path / "box.uge"
+    _write_uge(path, coords, vol, nb, areas, shuffle_ids=True)
+    c2, v2, nb2, a2, l2 = pkg.grid.load_uge(str(path))
+    assert np.array_equal(nb2, nb) and nb2.dtype == np.int64
+    assert np.array_equal(c2, coords) and np.array_equal(v2, vol) and np.array_equal(a2, areas)
+    assert np.allclose(l2, spacing, rtol=1e-15, atol=0)
+    assert np.allclose(a2 / l2, aol, rtol=1e-14, atol=0)
+
+
+def test_load_uge_truncated_file_raises(pkg, tmp_path):
+    path = tmp_path / "bad.uge"
+    path.write_text("CELLS 2\n1 0 0 0 1\n")
+    import pytest
+    with pytest.raises(ValueError):
+        pkg.grid.load_uge(str(path))
