@@ -42,12 +42,12 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient"],
+    ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase"],
                     default="steady2m",
                     help="steady2m (default, the headline line); ensemble: 96 packed "
                          "realizations on 32x64^3 + 32x96x96x24 + 32x128x128x16 grids; "
                          "transient: 200 backward-Euler steps on the 2M grid with the "
-                         "checkpointed adjoint")
+                         "checkpointed adjoint; twophase: IMPES steps on the 2M grid")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -66,6 +66,8 @@ def main():
         return bench_ensemble(args, P, api, _lib, torch)
     if args.workload == "transient":
         return bench_transient(args, P, api, _lib, torch)
+    if args.workload == "twophase":
+        return bench_twophase(args, P, api, _lib, torch)
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -448,6 +450,126 @@ def bench_transient(args, P, api, _lib, torch):
            "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
            "dtype": "f32 inner / f64 refinement", "config": {"workload": "transient 2M, 200 steps, "
            "checkpoint every 20"}, "detail": out}
+    print(json.dumps(res), flush=True)
+
+
+def bench_twophase(args, P, api, _lib, torch):
+    """IMPES (src/twophase.jl:238-264) on the 2M-cell grid: K = exp(logK - mean)
+    of the 2M field per cell, the 4 injectors at 1 m^3/s each, head 0 on the
+    top face, fluid (vw, vo, swc, sor) = (1, 5, 0.1, 0.15), S0 = swc, and dt =
+    100 CFL limits of the first pressure field, so a step is one pressure
+    solve plus ~100 upstream sub-steps.  Steps are timed with inputs resident
+    on the GPU; the upstream kernel is timed alone on the first step's fluxes.
+    cpu_baseline: the numpy oracle's sub-step on the same grid."""
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    w = P.workloads.steady_2m()
+    n, ne = w.n, w.nedges
+    topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    Kc = np.exp(w.logK - w.logK.mean())
+    Qh = np.where(w.Qs > 0, 1.0, 0.0)
+    fluid_h = np.array([1.0, 5.0, 0.1, 0.15])
+    K = torch.from_numpy(Kc).to(dev)
+    Q = torch.from_numpy(Qh).to(dev)
+    dh = torch.from_numpy(w.dirichleths).to(dev)
+    vol = torch.from_numpy(w.volumes).to(dev)
+    S0 = torch.full((n,), 0.1, dtype=torch.float64, device=dev)
+    Pd = torch.zeros(n, dtype=torch.float64, device=dev)
+    Sd = torch.empty(n, dtype=torch.float64, device=dev)
+    Vn = torch.empty(ne, dtype=torch.float64, device=dev)
+    fl = ctypes.c_void_p(fluid_h.ctypes.data)  # the fluid parameters stay in host memory
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    # dt from the first pressure field (set-up, untimed)
+    o = api._opts({"precision": "fp64", "reltol": 1e-10}, memory=_lib.DPF_DEVICE, stream=sptr)
+    mw = ((0.1 - 0.1) / 0.75) ** 2 / 1.0
+    mo = (1 - (0.1 - 0.1) / 0.75) ** 2 / 5.0
+    Km = torch.from_numpy((mw + mo) * Kc).to(dev)
+    _lib.check(L.dpf_twophase_tpfa(topo.handle, p(Km), p(dh), p(Q), p(Pd), p(Vn), 0,
+                                   ctypes.byref(o), None))
+    torch.cuda.synchronize()
+    vn = Vn.cpu().numpy()
+    nb0 = w.neighbors - 1
+    Vi = np.zeros(n)
+    np.add.at(Vi, np.where(vn < 0, nb0[:, 0], nb0[:, 1]), np.abs(vn))
+    cfl = (0.75 / 3.0) * np.min(w.volumes / (Vi + Qh))
+    dt = 100.0 * cfl
+    log(f"[bench] twophase: n={n}, cfl {cfl:.4g}, dt {dt:.4g} (100 CFL limits)")
+    out = {}
+    for precision in ("fp64", "mixed"):
+        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
+                      stream=sptr)
+        nsub = ctypes.c_int64()
+        st = _lib.Stats()
+
+        def run(nt, S_in):
+            _lib.check(L.dpf_solvetwophase(topo.handle, p(Pd), p(S_in), p(K), p(dh), p(Q),
+                                           p(vol), fl, dt, nt, 0, p(Pd), p(Sd), None, None,
+                                           ctypes.byref(nsub), ctypes.byref(o),
+                                           ctypes.byref(st)))
+        run(max(args.warmup, 1), S0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(args.steps, S0)
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        out[precision] = dict(steps_per_s=round(args.steps / dt_s, 3),
+                              ms_per_step=round(1e3 * dt_s / args.steps, 3),
+                              substeps=int(nsub.value), pcg_iterations=st.iterations)
+        log(f"[bench] twophase {precision}: {args.steps / dt_s:.2f} IMPES steps/s "
+            f"({1e3 * dt_s / args.steps:.1f} ms/step, {nsub.value} sub-steps, "
+            f"{st.iterations} PCG iterations over {args.steps} steps)")
+    # the upstream kernel alone, on the first pressure field's fluxes
+    _lib.check(L.dpf_twophase_tpfa(topo.handle, p(Km), p(dh), p(Q), p(Pd), p(Vn), 0,
+                                   ctypes.byref(o), None))
+    Sd.copy_(S0)
+    nsub = ctypes.c_int64()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.dpf_twophase_upstream(topo.handle, p(Sd), fl, p(Q), dt, p(Pd), p(Vn), p(vol),
+                                       ctypes.byref(nsub), ctypes.byref(o)))
+    torch.cuda.synchronize()
+    up_s = time.perf_counter() - t0
+    us = 1e6 * up_s / max(nsub.value, 1)
+    # algorithmic bytes per sub-step of k_tp_upstream: per cell S and f read and
+    # written, Q, pv and inc_ptr read (7 x 8 B); per incidence entry (2 per edge)
+    # the neighbour index (4 B) and the upwind-signed flux weight (8 B)
+    ninc = 2 * ne
+    alg = 56 * n + 12 * ninc
+    gbs = alg / (us * 1e-6) / 1e9
+    log(f"[bench] upstream alone: {nsub.value} sub-steps, {us:.1f} us/sub-step (incl. the CFL "
+        f"reduction and launch gaps), {alg / 1e6:.0f} MB algorithmic -> {gbs:.0f} GB/s")
+    # cpu baseline: one numpy oracle sub-step on the same arrays
+    cpu = None
+    if args.cpu_baseline == "auto":
+        from oracle import twophase_oracle as TPO
+        Sh = np.full(n, 0.1)
+        Ph = Pd.cpu().numpy()
+        vnh = Vn.cpu().numpy()
+        t0 = time.perf_counter()
+        f = TPO.fracflow(Sh, tuple(fluid_h))
+        R = TPO.saturation_residual(f, Qh, w.neighbors, Ph, vnh)
+        dtx = (dt / max(nsub.value, 1)) / w.volumes
+        Sh = np.clip(Sh + (R * dtx + Qh * dtx), 0.1, 0.85)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / cs, 3), "unit": "sub-steps/s", "cores": 1, "kind": "port",
+               "sample": "one upstream sub-step of the numpy oracle on the same 2M-cell arrays"}
+        log(f"[bench] cpu baseline: numpy oracle sub-step {cs:.2f} s")
+    res = {"metric": "IMPES steps/s, 2M-cell two-phase five-spot (pressure solve + ~100 CFL "
+                     "sub-steps per step)",
+           "value": out["mixed"]["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "data": "synthetic", "dtype": "f64 transport; f32 inner / f64 refinement pressure",
+           "config": {"workload": "twophase 2M (200x200x50), fluid (1, 5, 0.1, 0.15), 4 "
+                                  "injectors, top-face Dirichlet, dt = 100 CFL limits"},
+           "detail": out,
+           "upstream": {"us_per_substep": round(us, 2), "algorithmic_bytes": alg,
+                        "GBps": round(gbs, 1), "peak_GBps": 8000.0, "frac": round(gbs / 8000, 4)},
+           "cpu_baseline": cpu}
     print(json.dumps(res), flush=True)
 
 

@@ -2065,16 +2065,29 @@ int64_t upstream_core(dpf_topology* t, hipStream_t s, double* S, const Fluid& f,
   int64_t nts = (int64_t)steps;
   if (nts <= 0) return 0;
   double tau = T / (double)nts;
-  DBuf other(t->pool.get());
+  // upwind-signed flux weights per incidence entry, and f at S
+  int64_t ninc = t->H.inc_ptr.back();
+  DBuf wb(t->pool.get()), other(t->pool.get()), f0(t->pool.get()), f1(t->pool.get());
+  wb.alloc(std::max<int64_t>(ninc, 1) * sizeof(double));
   other.alloc(n * sizeof(double));
+  f0.alloc(n * sizeof(double));
+  f1.alloc(n * sizeof(double));
+  hipLaunchKernelGGL(k_tp_weights, dim3(nb), dim3(BLK), 0, s, n, t->inc_ptr.as<int64_t>(),
+                     t->inc_edge.as<int32_t>(), t->inc_other.as<int32_t>(), P, Vn,
+                     wb.as<double>());
+  hipLaunchKernelGGL(k_tp_frac, dim3(nb), dim3(BLK), 0, s, n, (const double*)S, f,
+                     f0.as<double>());
   double* cur = S;
   double* oth = other.as<double>();
+  double* fc = f0.as<double>();
+  double* fo = f1.as<double>();
   double lo = f.swc, hi = 1.0 - f.sor;
   for (int64_t k = 0; k < nts; k++) {
     hipLaunchKernelGGL(k_tp_upstream, dim3(nb), dim3(BLK), 0, s, n, t->inc_ptr.as<int64_t>(),
-                       t->inc_edge.as<int32_t>(), t->inc_other.as<int32_t>(),
-                       (const double*)cur, oth, Q, P, Vn, pv, tau, f, lo, hi);
+                       t->inc_other.as<int32_t>(), (const double*)wb.as<double>(),
+                       (const double*)cur, (const double*)fc, oth, fo, Q, pv, tau, f, lo, hi);
     std::swap(cur, oth);
+    std::swap(fc, fo);
   }
   HIPTRY(hipGetLastError());
   if (cur != S) copy_dev(S, cur, n * sizeof(double), s);

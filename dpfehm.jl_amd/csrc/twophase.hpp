@@ -115,43 +115,82 @@ __global__ void __launch_bounds__(BLK) k_tp_cfl(int64_t n, const int2* __restric
   }
 }
 
+// Fluxes and pressures are fixed during one upstream call, so the upwind
+// choice of every (cell, edge) pair is too: w[p] = |Vn_e| where the
+// neighbour k is upwind of c (P_k - P_c >= 0), -|Vn_e| where c is.  The
+// reference's term (src/twophase.jl:96-107) is then f_k w[p] or f_c w[p]
+// with the sign of w[p] choosing: f (v > 0 ? v : -v) = f |v| and
+// -f v = f (-v) hold exactly in IEEE arithmetic, so the product is the
+// reference's bit for bit (a zero flux gives a signed zero, which leaves
+// the sum unchanged).  Entry p of the incidence of cell c.
+__global__ void k_tp_weights(int64_t n, const int64_t* __restrict__ inc_ptr,
+                             const int32_t* __restrict__ inc_edge,
+                             const int32_t* __restrict__ inc_other,
+                             const double* __restrict__ P, const double* __restrict__ Vn,
+                             double* __restrict__ w) {
+#pragma clang fp contract(off)
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double pc = P[c];
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+    double a = fabs(Vn[inc_edge[p]]);
+    w[p] = P[inc_other[p]] - pc >= 0.0 ? a : -a;
+  }
+}
+
+// f_c = mw / (mw + mo) at S_c.
+__global__ void k_tp_frac(int64_t n, const double* __restrict__ S, Fluid f,
+                          double* __restrict__ fw) {
+  int64_t c = logical_row();
+  if (c < n) fw[c] = tp_fracflow(S[c], f);
+}
+
 // One CFL sub-step of the upstream transport (src/twophase.jl:227-233): the
 // residual of saturationcalc (src/twophase.jl:85-112, every cell an unknown)
 // R_c = min(Q_c, 0) f_c + sum over the edges at c, in edge order, of the
 // upwind term, then S_c <- clamp(S_c + (R_c dtx_c + max(Q_c, 0) dtx_c), swc,
-// 1 - sor) with dtx_c = tau / pv_c, tau = T / Nts.
+// 1 - sor) with dtx_c = tau / pv_c, tau = T / Nts.  fw holds f at S (every
+// cell) and receives f at the new S, so no sub-step evaluates the fractional
+// flow of a cell more than once; the loads of a cell's incidence go out eight
+// at a time.
 __global__ void k_tp_upstream(int64_t n, const int64_t* __restrict__ inc_ptr,
-                              const int32_t* __restrict__ inc_edge,
                               const int32_t* __restrict__ inc_other,
-                              const double* __restrict__ S, double* __restrict__ Snew,
-                              const double* __restrict__ Q, const double* __restrict__ P,
-                              const double* __restrict__ Vn, const double* __restrict__ pv,
-                              double tau, Fluid f, double lo, double hi) {
+                              const double* __restrict__ w, const double* __restrict__ S,
+                              const double* __restrict__ fw, double* __restrict__ Snew,
+                              double* __restrict__ fwnew, const double* __restrict__ Q,
+                              const double* __restrict__ pv, double tau, Fluid f, double lo,
+                              double hi) {
 #pragma clang fp contract(off)
   int64_t c = logical_row();
   if (c >= n) return;
   double sc = S[c];
-  double fc = tp_fracflow(sc, f);
+  double fc = fw[c];
   double q = Q[c];
   double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
   double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
   double R = qneg * fc;
-  double pc = P[c];
-  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
-    int32_t e = inc_edge[p], k = inc_other[p];
-    double v = Vn[e];
-    double term;
-    if (P[k] - pc >= 0.0) {
-      double fk = tp_fracflow(S[k], f);
-      term = v > 0.0 ? fk * v : -fk * v;
-    } else {
-      term = v > 0.0 ? -fc * v : fc * v;
+  int64_t p0 = inc_ptr[c], p1 = inc_ptr[c + 1];
+  constexpr int U = 8;
+  for (int64_t b = p0; b < p1; b += U) {
+    double wv[U], fo[U];
+    int32_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      wv[u] = ok ? w[b + u] : 0.0;
+      k[u] = ok ? inc_other[b + u] : 0;
     }
-    R = R + term;
+#pragma unroll
+    for (int u = 0; u < U; u++) fo[u] = b + u < p1 ? fw[k[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (b + u < p1) R = R + (signbit(wv[u]) ? fc : fo[u]) * wv[u];
   }
   double dtx = tau / pv[c];
   double s = sc + (R * dtx + qpos * dtx);
-  Snew[c] = s > hi ? hi : (s < lo ? lo : s);
+  s = s > hi ? hi : (s < lo ? lo : s);
+  Snew[c] = s;
+  fwnew[c] = tp_fracflow(s, f);
 }
 
 // Per-block sums of x (block_sum order) into part[block].

@@ -185,14 +185,15 @@ int dpf_twophase_tpfa(dpf_topology *topo, const double *K, const double *dirichl
  * pv = volumes; each sub-step adds the saturationcalc residual
  * (src/twophase.jl:85-112) times dtx = (T / Nts) / pv and max(Q, 0) dtx, then
  * clamps to [swc, 1 - sor].  fluid = {vw, vo, swc, sor}
- * (relativeperm, src/twophase.jl:201-206).  *nsub (may be NULL) = Nts.
- * opts: memory and stream only. */
+ * (relativeperm, src/twophase.jl:201-206), 4 doubles in host memory in every
+ * memory mode.  *nsub (may be NULL) = Nts.  opts: memory and stream only. */
 int dpf_twophase_upstream(dpf_topology *topo, double *S, const double *fluid, const double *Qs,
                           double T, const double *P, const double *Vn, const double *volumes,
                           int64_t *nsub, const dpf_opts *opts);
 
 /* solvetwophase (src/twophase.jl:238-264): nt IMPES steps from (h0, S0); each
- * step forms the total mobility Mt(S) K, runs tpfa, then upstream over dt.
+ * step forms the total mobility Mt(S) K, runs tpfa, then upstream over dt
+ * (fluid: 4 host doubles, as for dpf_twophase_upstream).
  * P_out, S_out (n): the last step's (h0 and S0 when nt = 0); P_all, S_all
  * (nt x n, step-major, may be NULL): every step's, the reference's everyStep
  * output.  *nsub_total (may be NULL) = CFL sub-steps over all steps; stats:
