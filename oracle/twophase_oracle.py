@@ -26,6 +26,12 @@ Function (restates) -- pinned by (tests/test_oracle_twophase.py)
   comparison with the textbook solution (test/twophase.jl:150-163,198-206,
   total_norm < 1e-5, here with the pressure constant removed) and its
   Buckley-Leverett check (test/twophase.jl:208-249, RMS error < 1e-2).
+* ``solvetwophase_grad``      reverse mode of the IMPES loop (the gradient
+  test/twophase.jl:192-205 takes with Zygote through the rrules of
+  solvepressure and saturationcalc, src/twophase.jl:59-80,117-183) --
+  central differences in K of c_P . P + c_S . S on a box, and of the
+  reference test's FindGrad objective (P at the centre cell after two steps)
+  on its singular quarter five-spot.
 
 Singular pressure systems (no Dirichlet cell): the reference's AMG-CG picks
 some constant; here one cell is pinned and P is shifted to mean zero, and the
@@ -253,3 +259,105 @@ def buckley_leverett(x, t):
                 hi = mid
         out[i] = 0.5 * (lo + hi)
     return out
+
+
+# ---------------------------------------------------------------- reverse mode
+
+def _mobility_and_slope(s, fluid):
+    """Mt = Mw + Mo and dMt/ds, f = Mw / Mt and df/ds."""
+    vw, vo, swc, sor = fluid
+    span = (1.0 - swc) - sor
+    S = (np.asarray(s, dtype=np.float64) - swc) / span
+    mw, mo = S * S / vw, (1.0 - S) ** 2 / vo
+    dmw, dmo = 2.0 * S / vw / span, -2.0 * (1.0 - S) / vo / span
+    mt = mw + mo
+    return mt, dmw + dmo, mw / mt, (dmw * mo - mw * dmo) / (mt * mt)
+
+
+def solvetwophase_grad(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasoverlengths,
+                       fluid, dt, neighbors, nt, seedP, seedS, reltol=1e-14):
+    """Reverse mode of ``solvetwophase`` for J with dJ/dP_nt = seedP and
+    dJ/dS_nt = seedS.  Returns (P, S, dJ/dK, dJ/dS0).  Sub-step counts, upwind
+    choices and clamp activity are held fixed (they are piecewise constant in
+    K); the clamp passes the tangent where lo <= u <= hi."""
+    vw, vo, swc, sor = fluid
+    lo, hi = swc, 1.0 - sor
+    nb = _nb0(neighbors)
+    a, b = nb[:, 0], nb[:, 1]
+    aol = np.asarray(areasoverlengths, dtype=np.float64)[:nb.shape[0]]
+    K = np.asarray(K, dtype=np.float64)
+    Q = np.asarray(Qs, dtype=np.float64)
+    pv = np.asarray(volumes, dtype=np.float64)
+    n = K.size
+    dn = np.asarray(dirichletnodes, dtype=np.int64).ravel()
+    pinned = dn.size == 0
+    dn_eff = np.array([n], dtype=np.int64) if pinned else dn
+    dh_eff = np.asarray(dirichleths, dtype=np.float64).copy()
+    if pinned:
+        dh_eff[n - 1] = 0.0
+    qneg = np.minimum(Q, 0.0)
+    tape = []
+    S = np.asarray(S0, dtype=np.float64).copy()
+    P = np.asarray(h0, dtype=np.float64).copy()
+    for _ in range(nt):
+        mt = relativeperm(S, fluid)
+        Km = (mt[0] + mt[1]) * K
+        Ksum = Km[a] + Km[b]
+        P, pb = G.steadystate_rrule(Ksum, neighbors, aol, dn_eff, dh_eff, Q, reltol=reltol)
+        if pinned:
+            P = P - P.sum() / n
+        Vn = (P[a] - P[b]) * (aol * Ksum)
+        Snew, nts = upstream(S, fluid, Q, dt, P, Vn, neighbors, volumes)
+        tape.append((S, Ksum, P, Vn, pb, nts))
+        S = Snew
+    Pfin, Sfin = P, S
+    dS = np.asarray(seedS, dtype=np.float64).copy()
+    dK = np.zeros(n)
+    for t in range(nt - 1, -1, -1):
+        S_prev, Ksum, P, Vn, pb, nts = tape[t]
+        av = np.abs(Vn)
+        up_a = (P[b] - P[a]) >= 0  # the term at a takes f_b
+        up_b = (P[a] - P[b]) >= 0
+        dabs = np.zeros(Vn.size)
+        dVn = np.zeros(Vn.size)
+        if nts > 0:
+            dtx = (dt / nts) / pv
+            fi = np.maximum(Q, 0.0) * dtx
+            states = [S_prev]
+            for _ in range(nts - 1):
+                s_ = states[-1]
+                f = fracflow(s_, fluid)
+                u = s_ + (saturation_residual(f, Q, neighbors, P, Vn) * dtx + fi)
+                states.append(np.where(u > hi, hi, np.where(u < lo, lo, u)))
+            for k in range(nts - 1, -1, -1):
+                s_ = states[k]
+                _, _, f, dfds = _mobility_and_slope(s_, fluid)
+                u = s_ + (saturation_residual(f, Q, neighbors, P, Vn) * dtx + fi)
+                du = dS * ((u >= lo) & (u <= hi))
+                dR = du * dtx
+                df = qneg * dR
+                # term at a: up_a ? f_b |v| : -f_a |v|; at b: up_b ? f_a |v| : -f_b |v|
+                np.add.at(df, b, np.where(up_a, av * dR[a], 0.0))
+                np.add.at(df, a, np.where(up_a, 0.0, -av * dR[a]))
+                np.add.at(df, a, np.where(up_b, av * dR[b], 0.0))
+                np.add.at(df, b, np.where(up_b, 0.0, -av * dR[b]))
+                dabs += np.where(up_a, f[b], -f[a]) * dR[a] + np.where(up_b, f[a], -f[b]) * dR[b]
+                dS = du + dfds * df
+            dVn = np.sign(Vn) * dabs
+        # Vn = (P_a - P_b) aol Ksum
+        dP = np.zeros(n)
+        np.add.at(dP, a, dVn * aol * Ksum)
+        np.add.at(dP, b, -dVn * aol * Ksum)
+        if t == nt - 1:
+            dP += np.asarray(seedP, dtype=np.float64)
+        dKsum = dVn * (P[a] - P[b]) * aol
+        if pinned:
+            dP = dP - dP.sum() / n
+        dKsum = dKsum + pb(dP)[1]
+        dKm = np.zeros(n)
+        np.add.at(dKm, a, dKsum)
+        np.add.at(dKm, b, dKsum)
+        mt, dmt, _, _ = _mobility_and_slope(S_prev, fluid)
+        dK += mt * dKm
+        dS = dS + K * dmt * dKm
+    return Pfin, Sfin, dK, dS

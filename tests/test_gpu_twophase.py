@@ -133,3 +133,36 @@ def test_nt_zero_and_errors(pkg, tp):
     v0[3] = 0.0
     with pytest.raises(pkg._lib.DpfError):
         tp.upstream(S0, FLUID, q, 0.5, P, Vn, nb, v0)
+
+
+def test_rrule_solvepressure(pkg, tp, oracle):
+    """src/twophase.jl:59-80: the transmissivity system with edge
+    permeabilities (tx = 2 aol); against the oracle's groundwater rrule at
+    2 K_e, whose K-tangent is halved by the chain rule, and, without a
+    Dirichlet cell, against central differences of the mean-zero pressure."""
+    _say("rrule_solvepressure")
+    nb, aol, vol, K, Q, S, n, rng = _random_box(pkg, ns=(10, 8, 6))
+    Ke = 0.5 * (K[nb[:, 0] - 1] + K[nb[:, 1] - 1])
+    dn = np.array([2, n - 3], dtype=np.int64)
+    dh = np.zeros(n)
+    dh[dn - 1] = [1.0, -1.0]
+    h, pb = tp.rrule_solvepressure(Ke, nb, aol, dn, dh, Q)
+    ho, pbo = oracle.steadystate_rrule(2.0 * Ke, nb, aol, dn, dh, Q, reltol=1e-14)
+    assert rel(h, ho) < 1e-10
+    delta = rng.normal(size=n)
+    g, go = pb(delta), pbo(delta)
+    assert rel(g[1], 2.0 * go[1]) < 1e-8 and rel(g[5], go[5]) < 1e-8 and rel(g[6], go[6]) < 1e-8
+    # singular system (no Dirichlet cell, balanced sources): mean-zero pressure
+    Qb = Q - Q.mean()
+    h0, pb0 = tp.rrule_solvepressure(Ke, nb, aol, [], np.zeros(n), Qb)
+    assert abs(h0.mean()) < 1e-12 * np.abs(h0).max()
+    d = rng.normal(size=n)
+    gK = pb0(d)[1]
+    for e in np.argsort(-np.abs(gK))[:3]:
+        eps = 1e-6 * Ke[e]
+        Kp, Km = Ke.copy(), Ke.copy()
+        Kp[e] += eps
+        Km[e] -= eps
+        fd = (d @ tp.solvepressure(Kp, nb, aol, [], np.zeros(n), Qb) -
+              d @ tp.solvepressure(Km, nb, aol, [], np.zeros(n), Qb)) / (2 * eps)
+        assert abs(fd - gK[e]) < 1e-5 * abs(gK[e])

@@ -154,3 +154,51 @@ def test_upstream_water_balance_one_substep(pkg, tpo):
     lhs = (vol * (S1 - S)).sum()
     rhs = 0.5 * cfl * (np.maximum(Q, 0).sum() + (np.minimum(Q, 0) * tpo.fracflow(S, FLUID)).sum())
     assert abs(lhs - rhs) < 1e-13 * abs(rhs)
+
+
+def test_solvetwophase_gradient_fd(pkg, tpo):
+    """Reverse mode of the oracle's IMPES loop against central differences in
+    K, for J = c_P . P_nt + c_S . S_nt, on a heterogeneous box with a Dirichlet
+    face (and on the reference test's singular quarter five-spot with its
+    FindGrad objective, P at the centre cell, test/twophase.jl:192-204)."""
+    nb, aol, vol, K, Q, S, n, rng = _random_box(pkg, ns=(6, 5, 4))
+    dn = np.arange(4, n + 1, 4, dtype=np.int64)
+    dh = np.zeros(n)
+    fluid = (1.0, 3.0, 0.05, 0.1)
+    S0 = np.full(n, 0.05)
+    cP, cS = rng.normal(size=n), rng.normal(size=n)
+    args = (np.zeros(n), S0, None, dh, dn, Q, vol, aol, fluid, 0.8, nb, 3)
+
+    def J(Kv):
+        a = list(args)
+        a[2] = Kv
+        P, S = tpo.solvetwophase(*a)
+        return cP @ P + cS @ S
+    a = list(args)
+    a[2] = K
+    P, S, dK, _ = tpo.solvetwophase_grad(*a, cP, cS)
+    for i in np.argsort(-np.abs(dK))[:5]:
+        eps = 1e-6 * K[i]
+        Kp, Km = K.copy(), K.copy()
+        Kp[i] += eps
+        Km[i] -= eps
+        fd = (J(Kp) - J(Km)) / (2 * eps)
+        assert abs(fd - dK[i]) < 1e-5 * abs(dK[i]) + 1e-9 * np.abs(dK).max()
+    # the reference test's objective on its singular 16x16 case, two steps
+    N = 16
+    nb, aol, vol, q, n = quarter_five_spot(pkg, N)
+    crit = int(np.ceil(N * N / 2)) - 1
+    seed = np.zeros(n)
+    seed[crit] = 1.0
+    Kq = np.ones(n)
+    P, S, dK, _ = tpo.solvetwophase_grad(np.zeros(n), np.zeros(n), Kq, np.zeros(n), [], q, vol,
+                                         aol, FLUID, 0.7 / 25, nb, 2, seed, np.zeros(n))
+    for i in np.argsort(-np.abs(dK))[:3]:
+        Kp, Km = Kq.copy(), Kq.copy()
+        Kp[i] += 1e-6
+        Km[i] -= 1e-6
+        fp = tpo.solvetwophase(np.zeros(n), np.zeros(n), Kp, np.zeros(n), [], q, vol, aol, FLUID,
+                               0.7 / 25, nb, 2)[0][crit]
+        fm = tpo.solvetwophase(np.zeros(n), np.zeros(n), Km, np.zeros(n), [], q, vol, aol, FLUID,
+                               0.7 / 25, nb, 2)[0][crit]
+        assert abs((fp - fm) / 2e-6 - dK[i]) < 1e-4 * abs(dK[i])
