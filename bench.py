@@ -509,7 +509,7 @@ def bench_twophase(args, P, api, _lib, torch):
         def run(nt, S_in):
             _lib.check(L.dpf_solvetwophase(topo.handle, p(Pd), p(S_in), p(K), p(dh), p(Q),
                                            p(vol), fl, dt, nt, 0, p(Pd), p(Sd), None, None,
-                                           ctypes.byref(nsub), ctypes.byref(o),
+                                           ctypes.byref(nsub), None, ctypes.byref(o),
                                            ctypes.byref(st)))
         run(max(args.warmup, 1), S0)
         torch.cuda.synchronize()

@@ -79,7 +79,8 @@ SIGNATURES = {
     "dpf_twophase_upstream": (ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p, _p,
                                              _p]),
     "dpf_solvetwophase": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i32,
-                                         _i32, _p, _p, _p, _p, _p, _p, _p]),
+                                         _i32, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "dpf_solvetwophase_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),

@@ -275,6 +275,10 @@ struct dpf_topology {
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
   // fine operator layout: offset slices, and nonzeros whose column is stored
   int64_t fine_offset_slices = 0, fine_explicit_nnz = 0;
+  // two-phase reverse mode: per incidence entry the same edge's entry in the
+  // other cell's list, per edge its two entries (built on first use)
+  DBuf tp_mirror, tp_epos;
+  bool tp_maps = false;
   // Chebyshev interval [1/ratio, 1] of D_l1^-1 A: default, and tuned per precision
   double cheb_ratio = 15.0;
   double ratio_tuned[3] = {15.0, 15.0, 15.0};
@@ -310,11 +314,20 @@ struct dpf_state {
   std::vector<double> qs;  // Q scale per time level (nsteps + 1)
   DBuf svec;               // S = Ss V on free cells
   DBuf ckpt;               // checkpointed states u_0, u_ck, u_2ck, ...
+  // two-phase runs (kind 2): per IMPES step the saturation it started from,
+  // the edge factor Ksum, pressure and fluxes (step-major), and its sub-step
+  // count; the run's K, Q, pore volumes, fluid, dt and pressure constant
+  DBuf tp_S, tp_Ksum, tp_P, tp_Vn, tp_K, tp_pv;
+  std::vector<int64_t> tp_nts;
+  double tp_fluid[4] = {0, 0, 0, 0};
+  int tp_zero_mean = 0;
   explicit dpf_state(dpf_topology* t);
 };
 
 dpf_state::dpf_state(dpf_topology* t) : topo(t), pool(t->pool), version(0), precision(0) {
-  for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt}) b->pool = pool.get();
+  for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt, &tp_S, &tp_Ksum, &tp_P,
+                  &tp_Vn, &tp_K, &tp_pv})
+    b->pool = pool.get();
 }
 
 namespace {
@@ -2195,8 +2208,10 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
                       const double* dirichleths, const double* Qs, const double* volumes,
                       const double* fluid, double dt, int32_t nt, int32_t zero_mean,
                       double* P_out, double* S_out, double* P_all, double* S_all,
-                      int64_t* nsub_total, const dpf_opts* opts, dpf_stats* stats) {
+                      int64_t* nsub_total, dpf_state** state_out, const dpf_opts* opts,
+                      dpf_stats* stats) {
   if (nsub_total) *nsub_total = 0;
+  if (state_out) *state_out = nullptr;
   if (!t || !h0 || !S0 || !K || !dirichleths || !Qs || !volumes || !fluid || !P_out ||
       !S_out || nt < 0) {
     set_err(nt < 0 ? "nt must be >= 0" : "NULL argument");
@@ -2245,6 +2260,30 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
     int64_t subs = 0;
     int status = DPF_OK;
     int iters = 0;
+    std::unique_ptr<dpf_state> st;
+    if (state_out) {
+      st = std::make_unique<dpf_state>(t);
+      st->kind = 2;
+      st->precision = o->precision;
+      st->nsteps = nt;
+      st->dt = dt;
+      st->tp_zero_mean = zero_mean;
+      std::memcpy(st->tp_fluid, fluid, sizeof(st->tp_fluid));
+      size_t nn = (size_t)std::max<int64_t>((int64_t)nt * n, 1) * sizeof(double);
+      size_t ee = (size_t)std::max<int64_t>((int64_t)nt * ne, 1) * sizeof(double);
+      st->tp_S.alloc(nn);
+      st->tp_P.alloc(nn);
+      st->tp_Ksum.alloc(ee);
+      st->tp_Vn.alloc(ee);
+      st->tp_K.alloc(n * sizeof(double));
+      st->tp_pv.alloc(n * sizeof(double));
+      st->Q.alloc(n * sizeof(double));
+      st->dh.alloc(n * sizeof(double));
+      copy_dev(st->tp_K.p, Kd, n * sizeof(double), s);
+      copy_dev(st->tp_pv.p, Wd, n * sizeof(double), s);
+      copy_dev(st->Q.p, Qd, n * sizeof(double), s);
+      copy_dev(st->dh.p, t->dh.p, n * sizeof(double), s);
+    }
     for (int32_t step = 0; step < nt && status == DPF_OK; step++) {
       // total mobility times K, arithmetic mean per edge, pressure and fluxes,
       // then the saturation transport over dt
@@ -2256,7 +2295,15 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
       status = tpfa_core(t, s, o, stats, Pd, Vd, zero_mean);
       if (stats) iters += stats->iterations;
       if (status != DPF_OK) break;
-      subs += upstream_core(t, s, Sd, f, Qd, dt, Pd, Vd, Wd);
+      if (st) {
+        copy_dev(st->tp_S.as<double>() + (int64_t)step * n, Sd, n * sizeof(double), s);
+        copy_dev(st->tp_P.as<double>() + (int64_t)step * n, Pd, n * sizeof(double), s);
+        copy_dev(st->tp_Ksum.as<double>() + (int64_t)step * ne, t->Ke.p, ne * sizeof(double), s);
+        copy_dev(st->tp_Vn.as<double>() + (int64_t)step * ne, Vd, ne * sizeof(double), s);
+      }
+      int64_t k = upstream_core(t, s, Sd, f, Qd, dt, Pd, Vd, Wd);
+      if (st) st->tp_nts.push_back(k);
+      subs += k;
       if (P_all) out(P_all + (int64_t)step * n, Pd, n);
       if (S_all) out(S_all + (int64_t)step * n, Sd, n);
     }
@@ -2268,8 +2315,190 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
       stats->iterations = iters;
       stats->total_ms = now_ms() - t0;
     }
+    if (st && status == DPF_OK) {
+      st->version = t->version;
+      *state_out = st.release();
+    }
     return status;
   });
 }
 
 }  // extern "C"
+
+namespace {
+
+// Per incidence entry the same edge's entry in the other cell's list, and per
+// edge its entries in its first and second cell's lists.
+void tp_build_maps(dpf_topology* t) {
+  if (t->tp_maps) return;
+  const TopologyHost& H = t->H;
+  int64_t n = H.n, ne = H.ne;
+  int64_t ninc = H.inc_ptr[n];
+  std::vector<int32_t> mirror(std::max<int64_t>(ninc, 1), 0);
+  std::vector<int32_t> epos(2 * std::max<int64_t>(ne, 1), 0);
+  for (int64_t c = 0; c < n; c++)
+    for (int64_t p = H.inc_ptr[c]; p < H.inc_ptr[c + 1]; p++) {
+      int32_t e = H.inc_edge[p];
+      epos[2 * e + (H.edges[2 * e] == c ? 0 : 1)] = (int32_t)p;
+    }
+  for (int64_t c = 0; c < n; c++)
+    for (int64_t p = H.inc_ptr[c]; p < H.inc_ptr[c + 1]; p++) {
+      int32_t e = H.inc_edge[p];
+      mirror[p] = H.edges[2 * e] == c ? epos[2 * e + 1] : epos[2 * e];
+    }
+  upload(t->tp_mirror, mirror);
+  upload(t->tp_epos, epos);
+  t->tp_maps = true;
+}
+
+}  // namespace
+
+extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
+                                          const double* dS_final, double* dK_out,
+                                          double* dS0_out, const dpf_opts* opts,
+                                          dpf_stats* stats) {
+  if (!st || !st->topo || !dK_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (st->kind != 2) {
+    set_err("the state does not come from dpf_solvetwophase");
+    return DPF_EINVAL;
+  }
+  dpf_topology* t = st->topo;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    int64_t ninc = t->H.inc_ptr.back();
+    int nt = st->nsteps;
+    bool host = o->memory == DPF_HOST;
+    tp_build_maps(t);
+    Fluid f = make_fluid(st->tp_fluid);
+    double lo = f.swc, hi = 1.0 - f.sor;
+    int64_t maxnts = 0;
+    for (int64_t k : st->tp_nts) maxnts = std::max(maxnts, k);
+    BufPool* pl = t->pool.get();
+    DBuf dS(pl), dK(pl), seedb(pl), w(pl), dabs(pl), du(pl), dR(pl), dVn(pl), dKsum(pl), dP(pl),
+        rhs(pl), lam(pl), hf(pl), part(pl), tS(pl), tf(pl);
+    unsigned nb = nblocks(n);
+    dS.alloc(n * sizeof(double));
+    dK.alloc(n * sizeof(double));
+    w.alloc(std::max<int64_t>(ninc, 1) * sizeof(double));
+    dabs.alloc(std::max<int64_t>(ninc, 1) * sizeof(double));
+    du.alloc(n * sizeof(double));
+    dR.alloc(n * sizeof(double));
+    dVn.alloc(ne * sizeof(double));
+    dKsum.alloc(ne * sizeof(double));
+    dP.alloc(n * sizeof(double));
+    rhs.alloc(nf * sizeof(double));
+    lam.alloc(nf * sizeof(double));
+    hf.alloc(nf * sizeof(double));
+    part.alloc((nb + 1) * sizeof(double));
+    tS.alloc(std::max<int64_t>(maxnts, 1) * n * sizeof(double));
+    tf.alloc(std::max<int64_t>(maxnts, 1) * n * sizeof(double));
+    if (dS_final)
+      HIPTRY(hipMemcpyAsync(dS.p, dS_final, n * sizeof(double),
+                            host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    else
+      HIPTRY(hipMemsetAsync(dS.p, 0, n * sizeof(double), s));
+    HIPTRY(hipMemsetAsync(dK.p, 0, n * sizeof(double), s));
+    const double* seedP = dP_final ? stage_in(t, o, seedb, dP_final, n, s) : nullptr;
+    const int64_t* ip = t->inc_ptr.as<int64_t>();
+    const int32_t* ie = t->inc_edge.as<int32_t>();
+    const int32_t* io = t->inc_other.as<int32_t>();
+    const double* Qd = st->Q.as<double>();
+    const double* pv = st->tp_pv.as<double>();
+    int status = DPF_OK, iters = 0;
+    for (int step = nt - 1; step >= 0 && status == DPF_OK; step--) {
+      const double* Sp = st->tp_S.as<double>() + (int64_t)step * n;
+      const double* Pt = st->tp_P.as<double>() + (int64_t)step * n;
+      const double* Vt = st->tp_Vn.as<double>() + (int64_t)step * ne;
+      const double* Kt = st->tp_Ksum.as<double>() + (int64_t)step * ne;
+      int64_t nts = st->tp_nts[step];
+      HIPTRY(hipMemsetAsync(dabs.p, 0, std::max<int64_t>(ninc, 1) * sizeof(double), s));
+      if (nts > 0) {
+        double tau = st->dt / (double)nts;
+        hipLaunchKernelGGL(k_tp_weights, dim3(nb), dim3(BLK), 0, s, n, ip, ie, io, Pt, Vt,
+                           w.as<double>());
+        // the sub-steps' input states and fractional flows, recomputed
+        double* S0k = tS.as<double>();
+        double* f0k = tf.as<double>();
+        copy_dev(S0k, Sp, n * sizeof(double), s);
+        hipLaunchKernelGGL(k_tp_frac, dim3(nb), dim3(BLK), 0, s, n, (const double*)S0k, f,
+                           f0k);
+        for (int64_t k = 0; k + 1 < nts; k++)
+          hipLaunchKernelGGL(k_tp_upstream, dim3(nb), dim3(BLK), 0, s, n, ip, io,
+                             (const double*)w.as<double>(), (const double*)(S0k + k * n),
+                             (const double*)(f0k + k * n), S0k + (k + 1) * n, f0k + (k + 1) * n,
+                             Qd, pv, tau, f, lo, hi);
+        for (int64_t k = nts - 1; k >= 0; k--) {
+          hipLaunchKernelGGL(k_tp_adj_a, dim3(nb), dim3(BLK), 0, s, n, ip, io,
+                             (const double*)w.as<double>(), (const double*)(S0k + k * n),
+                             (const double*)(f0k + k * n), Qd, pv, tau, lo, hi,
+                             (const double*)dS.as<double>(), du.as<double>(), dR.as<double>(),
+                             dabs.as<double>());
+          hipLaunchKernelGGL(k_tp_adj_b, dim3(nb), dim3(BLK), 0, s, n, ip, io,
+                             t->tp_mirror.as<int32_t>(), (const double*)w.as<double>(),
+                             (const double*)(S0k + k * n), Qd, f, (const double*)du.as<double>(),
+                             (const double*)dR.as<double>(), dS.as<double>());
+        }
+      }
+      hipLaunchKernelGGL(k_tp_flux_adj, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                         t->edges.as<int2>(), t->tp_epos.as<int2>(), t->aol.as<double>(), Pt, Vt,
+                         (const double*)dabs.as<double>(), dVn.as<double>(), dKsum.as<double>());
+      hipLaunchKernelGGL(k_tp_dP, dim3(nb), dim3(BLK), 0, s, n, t->edges.as<int2>(), ip, ie,
+                         t->aol.as<double>(), Kt, (const double*)dVn.as<double>(),
+                         step == nt - 1 ? seedP : (const double*)nullptr, dP.as<double>());
+      if (st->tp_zero_mean) {
+        double* pp = part.as<double>();
+        hipLaunchKernelGGL(k_tp_blocksum, dim3(nb), dim3(BLK), 0, s, n,
+                           (const double*)dP.as<double>(), pp);
+        hipLaunchKernelGGL(k_tp_mean, dim3(1), dim3(64), 0, s, n, (int64_t)nb,
+                           (const double*)pp, pp + nb);
+        hipLaunchKernelGGL(k_tp_sub, dim3(nb), dim3(BLK), 0, s, n, dP.as<double>(),
+                           (const double*)(pp + nb));
+      }
+      // the pressure solve's adjoint at this step's operator
+      copy_dev(t->Ke.p, Kt, ne * sizeof(double), s);
+      copy_dev(t->dh.p, st->dh.p, n * sizeof(double), s);
+      prepare(t, s, o->precision, nullptr, false);
+      hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                         t->f2n.as<int32_t>(), (const double*)dP.as<double>(), rhs.as<double>());
+      status = solve_system(t, s, rhs.as<double>(), lam.as<double>(), o, stats);
+      if (stats) iters += stats->iterations;
+      hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                         t->f2n.as<int32_t>(), Pt, hf.as<double>());
+      hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
+                         t->n2f.as<int32_t>(), t->aol.as<double>(), (const double*)hf.as<double>(),
+                         Pt, (const double*)lam.as<double>(), 1.0, 1, dKsum.as<double>());
+      hipLaunchKernelGGL(k_tp_kmob_adj, dim3(nb), dim3(BLK), 0, s, n, ip, ie,
+                         (const double*)dKsum.as<double>(), Sp,
+                         (const double*)st->tp_K.as<double>(), f, dK.as<double>(),
+                         dS.as<double>());
+      HIPTRY(hipGetLastError());
+    }
+    auto out = [&](double* dst, const DBuf& src) {
+      HIPTRY(hipMemcpyAsync(dst, src.p, n * sizeof(double),
+                            host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    };
+    out(dK_out, dK);
+    if (dS0_out) out(dS0_out, dS);
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = iters;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status == DPF_ENOCONV) set_err("adjoint AMG-PCG did not reach the tolerance within maxiter");
+    if (status == DPF_EBREAKDOWN) set_err("adjoint AMG-PCG breakdown");
+    return status;
+  });
+}

@@ -193,6 +193,145 @@ __global__ void k_tp_upstream(int64_t n, const int64_t* __restrict__ inc_ptr,
   fwnew[c] = tp_fracflow(s, f);
 }
 
+// ------------------------------------------------------------------ reverse mode
+//
+// Adjoint of the IMPES loop with sub-step counts, upwind choices and clamp
+// activity held fixed (piecewise constant in K).  Gradients are sums of
+// products whose order is fixed by the incidence lists, so they are
+// reproducible; contraction is allowed here.
+
+// dMt/ds and df/ds at saturation s (Mt = Mw + Mo, f = Mw / Mt).
+__device__ __forceinline__ void tp_slopes(double s, const Fluid& f, double& mt, double& dmt,
+                                          double& dfds) {
+  double S = (s - f.swc) / f.span;
+  double mw = S * S / f.vw, mo = (1.0 - S) * (1.0 - S) / f.vo;
+  double dmw = 2.0 * S / f.vw / f.span, dmo = -2.0 * (1.0 - S) / f.vo / f.span;
+  mt = mw + mo;
+  dmt = dmw + dmo;
+  dfds = (dmw * mo - mw * dmo) / (mt * mt);
+}
+
+// Reverse of one sub-step, part 1 (per cell c): the forward update is
+// recomputed from the taped S and f to find the clamp's activity; du = dS'
+// where lo <= u <= hi, dR = du dtx; dabs[p] += d(term)/d|Vn| dR_c over the
+// cell's incidence (f of the upwind cell, or -f_c).
+__global__ void k_tp_adj_a(int64_t n, const int64_t* __restrict__ inc_ptr,
+                           const int32_t* __restrict__ inc_other, const double* __restrict__ w,
+                           const double* __restrict__ S, const double* __restrict__ fw,
+                           const double* __restrict__ Q, const double* __restrict__ pv,
+                           double tau, double lo, double hi, const double* __restrict__ dSn,
+                           double* __restrict__ du, double* __restrict__ dR,
+                           double* __restrict__ dabs) {
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double fc = fw[c];
+  double q = Q[c];
+  double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
+  double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
+  double R = qneg * fc;
+  int64_t p0 = inc_ptr[c], p1 = inc_ptr[c + 1];
+  {
+#pragma clang fp contract(off)
+    for (int64_t p = p0; p < p1; p++) {
+      double wv = w[p];
+      R = R + (signbit(wv) ? fc : fw[inc_other[p]]) * wv;
+    }
+  }
+  double dtx = tau / pv[c];
+  double u;
+  {
+#pragma clang fp contract(off)
+    u = S[c] + (R * dtx + qpos * dtx);
+  }
+  double d = (u >= lo && u <= hi) ? dSn[c] : 0.0;
+  double r = d * dtx;
+  du[c] = d;
+  dR[c] = r;
+  for (int64_t p = p0; p < p1; p++) {
+    double wv = w[p];
+    dabs[p] += (signbit(wv) ? -fc : fw[inc_other[p]]) * r;
+  }
+}
+
+// Reverse of one sub-step, part 2 (per cell k): df_k from its own terms
+// (mass term, and the edges where k is downwind) and from the neighbours'
+// terms that take f_k (k upwind of them: the mirror entry's weight is
+// non-negative), then dS_k = du_k + f'(S_k) df_k into dS.
+__global__ void k_tp_adj_b(int64_t n, const int64_t* __restrict__ inc_ptr,
+                           const int32_t* __restrict__ inc_other,
+                           const int32_t* __restrict__ mirror, const double* __restrict__ w,
+                           const double* __restrict__ S, const double* __restrict__ Q, Fluid f,
+                           const double* __restrict__ du, const double* __restrict__ dR,
+                           double* __restrict__ dS) {
+  int64_t k = logical_row();
+  if (k >= n) return;
+  double q = Q[k];
+  double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
+  double rk = dR[k];
+  double df = qneg * rk;
+  for (int64_t p = inc_ptr[k]; p < inc_ptr[k + 1]; p++) {
+    double wv = w[p];
+    if (signbit(wv)) df += wv * rk;
+    double wm = w[mirror[p]];
+    if (!signbit(wm)) df += wm * dR[inc_other[p]];
+  }
+  double mt, dmt, dfds;
+  tp_slopes(S[k], f, mt, dmt, dfds);
+  dS[k] = du[k] + dfds * df;
+}
+
+// Per edge: dVn_e = sign(Vn_e) (dabs at a + dabs at b); dKsum_e += dVn_e
+// (P_a - P_b) aol_e.  epos[e] = the edge's entries in a's and b's incidence.
+__global__ void k_tp_flux_adj(int64_t ne, const int2* __restrict__ edges,
+                              const int2* __restrict__ epos, const double* __restrict__ aol,
+                              const double* __restrict__ P, const double* __restrict__ Vn,
+                              const double* __restrict__ dabs, double* __restrict__ dVn,
+                              double* __restrict__ dKsum) {
+  int64_t e = logical_row();
+  if (e >= ne) return;
+  int2 ab = edges[e], pp = epos[e];
+  double v = Vn[e];
+  double sg = v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0);
+  double g = ab.x == ab.y ? 0.0 : sg * (dabs[pp.x] + dabs[pp.y]);
+  dVn[e] = g;
+  dKsum[e] = g * (P[ab.x] - P[ab.y]) * aol[e];
+}
+
+// Per cell: dP_c = seed_c + sum over its edges of +-dVn_e aol_e Ksum_e (+ at
+// the edge's first cell, - at its second).
+__global__ void k_tp_dP(int64_t n, const int2* __restrict__ edges,
+                        const int64_t* __restrict__ inc_ptr,
+                        const int32_t* __restrict__ inc_edge, const double* __restrict__ aol,
+                        const double* __restrict__ Ksum, const double* __restrict__ dVn,
+                        const double* __restrict__ seed, double* __restrict__ dP) {
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double s = seed ? seed[c] : 0.0;
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
+    int32_t e = inc_edge[p];
+    double g = dVn[e] * aol[e] * Ksum[e];
+    s += edges[e].x == c ? g : -g;
+  }
+  dP[c] = s;
+}
+
+// Per cell: dKm_c = sum of dKsum over the cell's edges (Ksum_e = Km_a + Km_b);
+// through Km = Mt(S) K: dK_c += Mt dKm_c, dS_c += K_c Mt'(S_c) dKm_c.
+__global__ void k_tp_kmob_adj(int64_t n, const int64_t* __restrict__ inc_ptr,
+                              const int32_t* __restrict__ inc_edge,
+                              const double* __restrict__ dKsum, const double* __restrict__ S,
+                              const double* __restrict__ K, Fluid f, double* __restrict__ dK,
+                              double* __restrict__ dS) {
+  int64_t c = logical_row();
+  if (c >= n) return;
+  double g = 0.0;
+  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) g += dKsum[inc_edge[p]];
+  double mt, dmt, dfds;
+  tp_slopes(S[c], f, mt, dmt, dfds);
+  dK[c] += mt * g;
+  dS[c] += K[c] * dmt * g;
+}
+
 // Per-block sums of x (block_sum order) into part[block].
 __global__ void __launch_bounds__(BLK) k_tp_blocksum(int64_t n, const double* __restrict__ x,
                                                      double* __restrict__ part) {

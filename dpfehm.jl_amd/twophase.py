@@ -10,6 +10,8 @@ Same names, argument meaning and order as src/twophase.jl:
 * ``upstream``      -- src/twophase.jl:208-235: CFL-sub-stepped upwind
   saturation transport
 * ``solvetwophase`` -- src/twophase.jl:238-264: the IMPES time loop
+* ``rrule_solvetwophase`` -- its reverse mode (dK, dS0), the gradient the
+  reference's test/twophase.jl:192-205 takes with Zygote
 
 ``fluid`` is any object with ``vw``, ``vo``, ``swc``, ``sor`` attributes (the
 reference's ``Fluid``), or the 4-sequence ``(vw, vo, swc, sor)``.
@@ -27,7 +29,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .api import _f64, _opts, _ptr, get_topology, NoTangent, rrule_groundwater_steadystate
+from .api import (NoTangent, State, _f64, _opts, _ptr, get_topology,
+                  rrule_groundwater_steadystate)
 
 _REL = 1e-12
 
@@ -146,11 +149,8 @@ def upstream(S, fluid, Qs, T, P, Vn, neighbors, volumes, dirichletnodes=(),
     return out
 
 
-def solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasoverlengths, fluid,
-                  dt, neighbors, nt, everyStep=False, stats=None, **kwargs):
-    """src/twophase.jl:238-264 (the reference's ``args`` tuple, in order).
-    Returns (P, S) of the last step, or the lists of every step's P and S
-    when ``everyStep``."""
+def _solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasoverlengths,
+                   fluid, dt, neighbors, nt, everyStep, stats, keep, kwargs):
     Qs = _f64(Qs)
     n = Qs.shape[0]
     nt = int(nt)
@@ -162,15 +162,58 @@ def solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasover
     nsub = ctypes.c_int64()
     o = _opts(_kw(kwargs))
     st = _lib.Stats()
+    sh = ctypes.c_void_p()
     code = _lib.lib().dpf_solvetwophase(
         topo.handle, _ptr(_f64(h0)), _ptr(_f64(S0)), _ptr(_f64(K)), _ptr(dh), _ptr(Qs),
         _ptr(_f64(volumes)), _ptr(_fluid(fluid)), float(dt), nt, zm, _ptr(P), _ptr(S),
         _ptr(Pall) if everyStep else None, _ptr(Sall) if everyStep else None,
-        ctypes.byref(nsub), ctypes.byref(o), ctypes.byref(st))
+        ctypes.byref(nsub), ctypes.byref(sh) if keep else None, ctypes.byref(o),
+        ctypes.byref(st))
     if stats is not None:
         stats.update(st.as_dict())
         stats["substeps"] = int(nsub.value)
     _lib.check(code)
-    if everyStep:
-        return list(Pall), list(Sall)
-    return P, S
+    out = (list(Pall), list(Sall)) if everyStep else (P, S)
+    return out, (State(topo, sh) if keep else None), o
+
+
+def solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasoverlengths, fluid,
+                  dt, neighbors, nt, everyStep=False, stats=None, **kwargs):
+    """src/twophase.jl:238-264 (the reference's ``args`` tuple, in order).
+    Returns (P, S) of the last step, or the lists of every step's P and S
+    when ``everyStep``."""
+    out, _, _ = _solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes,
+                               areasoverlengths, fluid, dt, neighbors, nt, everyStep, stats,
+                               False, kwargs)
+    return out
+
+
+def rrule_solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes, areasoverlengths,
+                        fluid, dt, neighbors, nt, stats=None, **kwargs):
+    """Forward ``solvetwophase`` (last step) and its pullback: the gradient
+    test/twophase.jl:192-205 takes with Zygote.  ``pullback(dP, dS)`` (either
+    may be None) returns (dh0, dS0, dK): dh0 is zero for nt >= 1 (every step
+    recomputes the pressure).  Tangents in dirichleths, Qs, volumes,
+    areasoverlengths, fluid and dt are not formed."""
+    (P, S), state, o = _solvetwophase(h0, S0, K, dirichleths, dirichletnodes, Qs, volumes,
+                                      areasoverlengths, fluid, dt, neighbors, nt, False, stats,
+                                      True, kwargs)
+    n = P.shape[0]
+
+    def pullback(dP=None, dS=None, stats=None):
+        if int(nt) == 0:
+            return (_f64(dP) if dP is not None else np.zeros(n),
+                    _f64(dS) if dS is not None else np.zeros(n), np.zeros(n))
+        dK, dS0 = np.empty(n), np.empty(n)
+        st = _lib.Stats()
+        code = _lib.lib().dpf_solvetwophase_pullback(
+            state._h, _ptr(_f64(dP)) if dP is not None else None,
+            _ptr(_f64(dS)) if dS is not None else None, _ptr(dK), _ptr(dS0), ctypes.byref(o),
+            ctypes.byref(st))
+        if stats is not None:
+            stats.update(st.as_dict())
+        _lib.check(code)
+        return np.zeros(n), dS0, dK
+
+    pullback.state = state
+    return (P, S), pullback

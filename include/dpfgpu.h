@@ -197,12 +197,27 @@ int dpf_twophase_upstream(dpf_topology *topo, double *S, const double *fluid, co
  * P_out, S_out (n): the last step's (h0 and S0 when nt = 0); P_all, S_all
  * (nt x n, step-major, may be NULL): every step's, the reference's everyStep
  * output.  *nsub_total (may be NULL) = CFL sub-steps over all steps; stats:
- * PCG iterations summed over the pressure solves. */
+ * PCG iterations summed over the pressure solves.  state_out (may be NULL)
+ * receives the run's tape for dpf_solvetwophase_pullback (per step: the
+ * starting saturation, the edge factors, pressure and fluxes; release with
+ * dpf_state_destroy). */
 int dpf_solvetwophase(dpf_topology *topo, const double *h0, const double *S0, const double *K,
                       const double *dirichleths, const double *Qs, const double *volumes,
                       const double *fluid, double dt, int32_t nt, int32_t zero_mean,
                       double *P_out, double *S_out, double *P_all, double *S_all,
-                      int64_t *nsub_total, const dpf_opts *opts, dpf_stats *stats);
+                      int64_t *nsub_total, dpf_state **state_out, const dpf_opts *opts,
+                      dpf_stats *stats);
+
+/* Reverse mode of dpf_solvetwophase: the gradient that test/twophase.jl:192-205
+ * takes with Zygote through the rrules of solvepressure and saturationcalc
+ * (src/twophase.jl:59-80,117-183).  Given dJ/dP and dJ/dS of the last step
+ * (either may be NULL = 0), dK (n) = dJ/dK and dS0 (n, may be NULL) = dJ/dS0.
+ * Sub-step counts, upwind choices and the clamp's activity are held fixed (they
+ * are piecewise constant in K; the clamp passes the tangent where
+ * swc <= u <= 1 - sor).  One adjoint pressure solve per step, at that step's
+ * operator; the sub-steps are recomputed from the tape. */
+int dpf_solvetwophase_pullback(dpf_state *state, const double *dP_final, const double *dS_final,
+                               double *dK, double *dS0, const dpf_opts *opts, dpf_stats *stats);
 
 /* groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs; kwargs...) (src/groundwater.jl:75-83) and the forward

@@ -166,3 +166,51 @@ def test_rrule_solvepressure(pkg, tp, oracle):
         fd = (d @ tp.solvepressure(Kp, nb, aol, [], np.zeros(n), Qb) -
               d @ tp.solvepressure(Km, nb, aol, [], np.zeros(n), Qb)) / (2 * eps)
         assert abs(fd - gK[e]) < 1e-5 * abs(gK[e])
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+def test_solvetwophase_gradient_vs_oracle(pkg, tp, tpo, pinned):
+    """Reverse mode of the IMPES loop against the oracle's (itself pinned by
+    central differences), J = c_P . P + c_S . S after three steps."""
+    _say("twophase gradient", pinned)
+    nb, aol, vol, K, Q, S, n, rng = _random_box(pkg, ns=(12, 10, 6))
+    if pinned:
+        dn = np.array([], dtype=np.int64)
+        Q = Q - Q.mean()
+    else:
+        dn = np.arange(6, n + 1, 6, dtype=np.int64)
+    dh = np.zeros(n)
+    fluid = (1.0, 3.0, 0.05, 0.1)
+    S0 = np.full(n, 0.05)
+    cP, cS = rng.normal(size=n), rng.normal(size=n)
+    args = (np.zeros(n), S0, K, dh, dn, Q, vol, aol, fluid, 0.8, nb, 3)
+    (P, Sg), pb = tp.rrule_solvetwophase(*args)
+    dh0, dS0, dK = pb(cP, cS)
+    Po, So, dKo, dS0o = tpo.solvetwophase_grad(*args, cP, cS)
+    assert rel(P, Po) < 1e-10 and np.abs(Sg - So).max() < 1e-10
+    assert rel(dK, dKo) < 1e-8 and rel(dS0, dS0o) < 1e-8
+    assert not dh0.any()
+
+
+def test_findgrad_quarter_five_spot(pkg, tp):
+    """test/twophase.jl:192-205: the gradient of P at the centre cell after
+    two steps, against one-sided differences with delta 1e-8 and rtol 0.1 at
+    the three largest entries, as the reference checks it."""
+    _say("FindGrad")
+    N = 16
+    nb, aol, vol, q, n = quarter_five_spot(pkg, N)
+    crit = int(np.ceil(N * N / 2)) - 1
+    K = np.ones(n)
+    args = [np.zeros(n), np.zeros(n), K, np.zeros(n), [], q, vol, aol, FLUID, 0.7 / 25, nb, 2]
+    (P, S), pb = tp.rrule_solvetwophase(*args)
+    seed = np.zeros(n)
+    seed[crit] = 1.0
+    dK = pb(seed, None)[2]
+    f0 = P[crit]
+    for i in np.argsort(-np.abs(dK))[:3]:
+        Kp = K.copy()
+        Kp[i] += 1e-8
+        a = list(args)
+        a[2] = Kp
+        fd = (tp.solvetwophase(*a)[0][crit] - f0) / 1e-8
+        assert abs(fd - dK[i]) <= 0.1 * abs(dK[i])
