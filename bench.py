@@ -523,6 +523,31 @@ def bench_twophase(args, P, api, _lib, torch):
         log(f"[bench] twophase {precision}: {args.steps / dt_s:.2f} IMPES steps/s "
             f"({1e3 * dt_s / args.steps:.1f} ms/step, {nsub.value} sub-steps, "
             f"{st.iterations} PCG iterations over {args.steps} steps)")
+        # gradient of the observed pressures after the same steps with respect
+        # to the cell permeabilities: forward with the tape, then the pullback
+        sh = ctypes.c_void_p()
+        seed = torch.zeros(n, dtype=torch.float64, device=dev)
+        seed[torch.from_numpy(w.obs - 1).to(dev)] = 1.0
+        dKd = torch.empty(n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.dpf_solvetwophase(topo.handle, p(Pd), p(S0), p(K), p(dh), p(Q), p(vol), fl,
+                                       dt, args.steps, 0, p(Pd), p(Sd), None, None,
+                                       ctypes.byref(nsub), ctypes.byref(sh), ctypes.byref(o),
+                                       ctypes.byref(st)))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st2 = _lib.Stats()
+        code = L.dpf_solvetwophase_pullback(sh, p(seed), None, p(dKd), None, ctypes.byref(o),
+                                            ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[precision].update(taped_forward_s=round(t1 - t0, 3), adjoint_s=round(t2 - t1, 3),
+                              adjoint_pcg_iterations=st2.iterations)
+        log(f"[bench] twophase {precision} gradient: taped forward {t1 - t0:.3f} s, adjoint "
+            f"{t2 - t1:.3f} s ({st2.iterations} adjoint PCG iterations) for {args.steps} steps")
     # the upstream kernel alone, on the first pressure field's fluxes
     _lib.check(L.dpf_twophase_tpfa(topo.handle, p(Km), p(dh), p(Q), p(Pd), p(Vn), 0,
                                    ctypes.byref(o), None))

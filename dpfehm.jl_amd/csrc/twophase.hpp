@@ -132,9 +132,29 @@ __global__ void k_tp_weights(int64_t n, const int64_t* __restrict__ inc_ptr,
   int64_t c = logical_row();
   if (c >= n) return;
   double pc = P[c];
-  for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
-    double a = fabs(Vn[inc_edge[p]]);
-    w[p] = P[inc_other[p]] - pc >= 0.0 ? a : -a;
+  int64_t p0 = inc_ptr[c], p1 = inc_ptr[c + 1];
+  constexpr int U = 8;
+  for (int64_t b = p0; b < p1; b += U) {
+    int32_t e[U], k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      e[u] = ok ? inc_edge[b + u] : 0;
+      k[u] = ok ? inc_other[b + u] : 0;
+    }
+    double v[U], pk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      v[u] = ok ? Vn[e[u]] : 0.0;
+      pk[u] = ok ? P[k[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (b + u < p1) {
+        double a = fabs(v[u]);
+        w[b + u] = pk[u] - pc >= 0.0 ? a : -a;
+      }
   }
 }
 
@@ -228,28 +248,51 @@ __global__ void k_tp_adj_a(int64_t n, const int64_t* __restrict__ inc_ptr,
   double q = Q[c];
   double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
   double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
-  double R = qneg * fc;
+  double sc = S[c], pvc = pv[c], dsn = dSn[c];
   int64_t p0 = inc_ptr[c], p1 = inc_ptr[c + 1];
-  {
+  constexpr int U = 8;
+  double wv[U], fo[U];
+  // the cell's incidence in batches of eight (one batch on a 3-D grid): the
+  // forward update recomputed in the forward's order, then the tangents
+  auto load = [&](int64_t b) {
+    int32_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      wv[u] = ok ? w[b + u] : 0.0;
+      k[u] = ok ? inc_other[b + u] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) fo[u] = b + u < p1 ? fw[k[u]] : 0.0;
+  };
+  double R = qneg * fc;
+  for (int64_t b = p0; b < p1; b += U) {
+    load(b);
+    {
 #pragma clang fp contract(off)
-    for (int64_t p = p0; p < p1; p++) {
-      double wv = w[p];
-      R = R + (signbit(wv) ? fc : fw[inc_other[p]]) * wv;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (b + u < p1) R = R + (signbit(wv[u]) ? fc : fo[u]) * wv[u];
     }
   }
-  double dtx = tau / pv[c];
+  double dtx = tau / pvc;
   double u;
   {
 #pragma clang fp contract(off)
-    u = S[c] + (R * dtx + qpos * dtx);
+    u = sc + (R * dtx + qpos * dtx);
   }
-  double d = (u >= lo && u <= hi) ? dSn[c] : 0.0;
+  double d = (u >= lo && u <= hi) ? dsn : 0.0;
   double r = d * dtx;
   du[c] = d;
   dR[c] = r;
-  for (int64_t p = p0; p < p1; p++) {
-    double wv = w[p];
-    dabs[p] += (signbit(wv) ? -fc : fw[inc_other[p]]) * r;
+  for (int64_t b = p0; b < p1; b += U) {
+    if (p1 - p0 > U) load(b);
+    double old[U];
+#pragma unroll
+    for (int u2 = 0; u2 < U; u2++) old[u2] = b + u2 < p1 ? dabs[b + u2] : 0.0;
+#pragma unroll
+    for (int u2 = 0; u2 < U; u2++)
+      if (b + u2 < p1) dabs[b + u2] = old[u2] + (signbit(wv[u2]) ? -fc : fo[u2]) * r;
   }
 }
 
@@ -269,11 +312,29 @@ __global__ void k_tp_adj_b(int64_t n, const int64_t* __restrict__ inc_ptr,
   double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
   double rk = dR[k];
   double df = qneg * rk;
-  for (int64_t p = inc_ptr[k]; p < inc_ptr[k + 1]; p++) {
-    double wv = w[p];
-    if (signbit(wv)) df += wv * rk;
-    double wm = w[mirror[p]];
-    if (!signbit(wm)) df += wm * dR[inc_other[p]];
+  int64_t p0 = inc_ptr[k], p1 = inc_ptr[k + 1];
+  constexpr int U = 8;
+  for (int64_t b = p0; b < p1; b += U) {
+    double wv[U], wm[U], ro[U];
+    int32_t m[U], o[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      wv[u] = ok ? w[b + u] : 0.0;
+      m[u] = ok ? mirror[b + u] : 0;
+      o[u] = ok ? inc_other[b + u] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      bool ok = b + u < p1;
+      wm[u] = ok ? w[m[u]] : -1.0;
+      ro[u] = ok ? dR[o[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (signbit(wv[u])) df += wv[u] * rk;
+      if (!signbit(wm[u])) df += wm[u] * ro[u];
+    }
   }
   double mt, dmt, dfds;
   tp_slopes(S[k], f, mt, dmt, dfds);
