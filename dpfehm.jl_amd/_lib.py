@@ -81,6 +81,9 @@ SIGNATURES = {
     "dpf_solvetwophase": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _i32,
                                          _i32, _p, _p, _p, _p, _p, _p, _p, _p]),
     "dpf_solvetwophase_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "dpf_richards_residuals": (ctypes.c_int, [_p] * 12),
+    "dpf_richards_steady": (ctypes.c_int, [_p] * 12),
+    "dpf_richards_pullback": (ctypes.c_int, [_p] * 9),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),

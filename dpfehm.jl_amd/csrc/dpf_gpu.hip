@@ -27,6 +27,7 @@
 #include "dpf_host.h"
 #include "kernels.hpp"
 #include "twophase.hpp"
+#include "richards.hpp"
 
 namespace dpf {
 
@@ -318,6 +319,9 @@ struct dpf_state {
   // the edge factor Ksum, pressure and fluxes (step-major), and its sub-step
   // count; the run's K, Q, pore volumes, fluid, dt and pressure constant
   DBuf tp_S, tp_Ksum, tp_P, tp_Vn, tp_K, tp_pv;
+  // steady Richards (kind 3): elevation and per-edge van Genuchten parameters
+  // (hfree holds the free heads, Ke the edge K, dh the Dirichlet heads)
+  DBuf r_z, r_al, r_N;
   std::vector<int64_t> tp_nts;
   double tp_fluid[4] = {0, 0, 0, 0};
   int tp_zero_mean = 0;
@@ -326,7 +330,7 @@ struct dpf_state {
 
 dpf_state::dpf_state(dpf_topology* t) : topo(t), pool(t->pool), version(0), precision(0) {
   for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt, &tp_S, &tp_Ksum, &tp_P,
-                  &tp_Vn, &tp_K, &tp_pv})
+                  &tp_Vn, &tp_K, &tp_pv, &r_z, &r_al, &r_N})
     b->pool = pool.get();
 }
 
@@ -2500,5 +2504,443 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
     if (status == DPF_ENOCONV) set_err("adjoint AMG-PCG did not reach the tolerance within maxiter");
     if (status == DPF_EBREAKDOWN) set_err("adjoint AMG-PCG breakdown");
     return status;
+  });
+}
+
+// ====================================================================== steady Richards
+
+namespace {
+
+// Deterministic x.y: per-block sums on the device, their sum in block order on
+// the host (one synchronisation).
+double host_dot(dpf_topology* t, hipStream_t s, DBuf& part, std::vector<double>& hp, int64_t n,
+                const double* x, const double* y) {
+  unsigned nb = nblocks(n);
+  hipLaunchKernelGGL(k_rich_dot, dim3(nb), dim3(BLK), 0, s, n, x, y, part.as<double>());
+  HIPTRY(hipMemcpyAsync(hp.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  double a = 0.0;
+  for (unsigned b = 0; b < nb; b++) a += hp[b];
+  return a;
+}
+
+// z = V-cycle(b) on the FP64 hierarchy (the Picard operator's), Chebyshev
+// degrees 1 on the finest level and 2 below, interval ratio 15.
+void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, const double* b,
+                  double* z) {
+  int64_t nf = t->H.nf;
+  LevelNum<double>& n0 = t->h64.lev[0];
+  double* x0 = n0.x.as<double>();
+  hipLaunchKernelGGL(k_rich_pre, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, b,
+                     (const double*)level_invd(t, t->h64, 0), chs[0].c0, n0.d.as<double>(), x0);
+  double* out = nullptr;
+  vcycle<double>(t, t->h64, s, 0, b, x0, n0.xs.as<double>(), &out, chs, t->zero_flag.as<int>(),
+                 false);
+  copy_dev(z, out, nf * sizeof(double), s);
+}
+
+// Right-preconditioned BiCGStab for M x = b (M on the fine layout), x from 0,
+// to ||b - M x|| <= tol ||b||.  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN;
+// *iters counts the iterations.
+int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
+                  SellD<double> M, const double* b, double* x, double tol, int maxit,
+                  int* iters) {
+  int64_t nf = t->H.nf;
+  unsigned nb = nblocks(nf);
+  BufPool* pl = t->pool.get();
+  DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl), part(pl);
+  for (DBuf* d : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) d->alloc(nf * sizeof(double));
+  part.alloc(nb * sizeof(double));
+  std::vector<double> hp(nb);
+  size_t bytes = nf * sizeof(double);
+  HIPTRY(hipMemsetAsync(x, 0, bytes, s));
+  copy_dev(r.p, b, bytes, s);
+  copy_dev(rh.p, b, bytes, s);
+  HIPTRY(hipMemsetAsync(p.p, 0, bytes, s));
+  HIPTRY(hipMemsetAsync(v.p, 0, bytes, s));
+  double bn = std::sqrt(host_dot(t, s, part, hp, nf, b, b));
+  *iters = 0;
+  if (!(bn > 0.0)) return bn == 0.0 ? DPF_OK : DPF_EBREAKDOWN;
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  dim3 g(nblocks(nf)), bl(BLK);
+  for (int it = 1; it <= maxit; it++) {
+    double rho1 = host_dot(t, s, part, hp, nf, rh.as<double>(), r.as<double>());
+    if (rho1 == 0.0 || !std::isfinite(rho1)) return DPF_EBREAKDOWN;
+    double beta = (rho1 / rho) * (alpha / omega);
+    // p = r + beta (p - omega v)
+    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, 1.0, (const double*)r.as<double>(), beta,
+                       p.as<double>(), -beta * omega, (const double*)v.as<double>());
+    rich_precond(t, s, chs, p.as<double>(), ph.as<double>());
+    hipLaunchKernelGGL(k_rich_spmv, g, bl, 0, s, M, (const double*)ph.as<double>(),
+                       v.as<double>());
+    double rv = host_dot(t, s, part, hp, nf, rh.as<double>(), v.as<double>());
+    if (rv == 0.0 || !std::isfinite(rv)) return DPF_EBREAKDOWN;
+    alpha = rho1 / rv;
+    // s = r - alpha v
+    copy_dev(sv.p, r.p, bytes, s);
+    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(),
+                       1.0, sv.as<double>(), 0.0, (const double*)nullptr);
+    double ss = host_dot(t, s, part, hp, nf, sv.as<double>(), sv.as<double>());
+    *iters = it;
+    if (std::sqrt(ss) <= tol * bn) {
+      hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, alpha, (const double*)ph.as<double>(),
+                         1.0, x, 0.0, (const double*)nullptr);
+      return DPF_OK;
+    }
+    rich_precond(t, s, chs, sv.as<double>(), sh.as<double>());
+    hipLaunchKernelGGL(k_rich_spmv, g, bl, 0, s, M, (const double*)sh.as<double>(),
+                       tv.as<double>());
+    double ts = host_dot(t, s, part, hp, nf, tv.as<double>(), sv.as<double>());
+    double tt = host_dot(t, s, part, hp, nf, tv.as<double>(), tv.as<double>());
+    if (tt == 0.0 || !std::isfinite(tt)) return DPF_EBREAKDOWN;
+    omega = ts / tt;
+    // x += alpha ph + omega sh ; r = s - omega t
+    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, alpha, (const double*)ph.as<double>(), 1.0,
+                       x, omega, (const double*)sh.as<double>());
+    copy_dev(r.p, sv.p, bytes, s);
+    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -omega, (const double*)tv.as<double>(),
+                       1.0, r.as<double>(), 0.0, (const double*)nullptr);
+    double rr = host_dot(t, s, part, hp, nf, r.as<double>(), r.as<double>());
+    if (std::sqrt(rr) <= tol * bn) return DPF_OK;
+    if (omega == 0.0) return DPF_EBREAKDOWN;
+    rho = rho1;
+  }
+  return DPF_ENOCONV;
+}
+
+// The inputs of one Richards problem on the device.
+struct RichIn {
+  const double *K, *dpsi, *z, *al, *N, *Q;
+};
+
+// Edge fluxes F (and dFa, dFb) and the residual r at psi (free); returns
+// (max |r|, ||r||_2).
+std::pair<double, double> rich_residual(dpf_topology* t, hipStream_t s, const RichIn& in,
+                                        const double* psi, double* F, double* dFa, double* dFb,
+                                        double* r, DBuf& part) {
+  int64_t ne = t->H.ne, nf = t->H.nf;
+  hipLaunchKernelGGL(k_rich_edge_flux, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                     t->edges.as<int2>(), t->n2f.as<int32_t>(), psi, in.dpsi, in.z, in.al, in.N,
+                     in.K, t->aol.as<double>(), F, dFa, dFb);
+  unsigned nb = nblocks(nf);
+  hipLaunchKernelGGL(k_rich_residual, dim3(nb), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                     t->edges.as<int2>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                     (const double*)F, in.Q, (const double*)nullptr, r, part.as<double>());
+  std::vector<double> hp(2 * nb);
+  HIPTRY(hipMemcpyAsync(hp.data(), part.p, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPTRY(hipStreamSynchronize(s));
+  double mx = 0.0, sq = 0.0;
+  for (unsigned b = 0; b < nb; b++) {
+    mx = (hp[2 * b] != hp[2 * b] || hp[2 * b] > mx) ? hp[2 * b] : mx;
+    sq += hp[2 * b + 1];
+  }
+  return {mx, std::sqrt(sq)};
+}
+
+// Newton matrix M (and MT) at psi: Picard operator (and its AMG hierarchy) for
+// the preconditioner, then the kr-slope terms.
+void rich_matrices(dpf_topology* t, hipStream_t s, const RichIn& in, const double* psi,
+                   const double* dFa, const double* dFb, double* M, double* MT) {
+  int64_t ne = t->H.ne, nf = t->H.nf;
+  hipLaunchKernelGGL(k_rich_picard, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
+                     t->n2f.as<int32_t>(), psi, in.dpsi, in.z, in.al, in.N, in.K,
+                     t->Ke.as<double>());
+  prepare(t, s, DPF_FP64, nullptr, false);
+  LevelDev& d0 = *t->lev[0];
+  size_t bytes = d0.A.entries * sizeof(double);
+  HIPTRY(hipMemsetAsync(M, 0, bytes, s));
+  if (MT) HIPTRY(hipMemsetAsync(MT, 0, bytes, s));
+  hipLaunchKernelGGL(k_rich_newton_matrix, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                     t->f2n.as<int32_t>(), t->n2f.as<int32_t>(), t->edges.as<int2>(),
+                     t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                     t->inc_other.as<int32_t>(), t->tp_mirror.as<int32_t>(),
+                     t->inc_slot.as<uint16_t>(), d0.diag_slot.as<uint16_t>(),
+                     d0.A.view<double>(M), dFa, dFb, (const double*)nullptr, M, MT);
+  HIPTRY(hipGetLastError());
+}
+
+std::vector<Cheb> rich_chebs(dpf_topology* t) {
+  std::vector<Cheb> v(t->levels());
+  for (int l = 0; l < t->levels(); l++) v[l].make(l == 0 ? 1 : 2, 1.0, 15.0);
+  return v;
+}
+
+}  // namespace
+
+extern "C" int dpf_richards_steady(dpf_topology* t, const double* psi0, const double* Ks,
+                                   const double* dirichletpsis, const double* z,
+                                   const double* alphas, const double* Ns, const double* Qs,
+                                   double* psi_out, const dpf_opts* opts, dpf_stats* stats,
+                                   dpf_state** state_out) {
+  if (state_out) *state_out = nullptr;
+  if (!t || !psi0 || !Ks || !dirichletpsis || !z || !alphas || !Ns || !Qs || !psi_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  if (t->levels() < 2) {
+    set_err("the Richards solver needs a multilevel hierarchy (a larger mesh)");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    t->timing = false;
+    tp_build_maps(t);
+    BufPool* pl = t->pool.get();
+    std::unique_ptr<dpf_state> st = std::make_unique<dpf_state>(t);
+    st->kind = 3;
+    st->precision = DPF_FP64;
+    // inputs, kept in the state for the pullback
+    auto keep = [&](DBuf& d, const double* src, int64_t count) -> const double* {
+      d.alloc(count * sizeof(double));
+      HIPTRY(hipMemcpyAsync(d.p, src, count * sizeof(double),
+                            o->memory == DPF_HOST ? hipMemcpyHostToDevice
+                                                  : hipMemcpyDeviceToDevice, s));
+      return d.as<double>();
+    };
+    RichIn in;
+    in.K = keep(st->Ke, Ks, ne);
+    in.dpsi = keep(st->dh, dirichletpsis, n);
+    in.z = keep(st->r_z, z, n);
+    in.al = keep(st->r_al, alphas, ne);
+    in.N = keep(st->r_N, Ns, ne);
+    in.Q = keep(st->Q, Qs, n);
+    DBuf p0(pl), F(pl), dFa(pl), dFb(pl), r(pl), rn(pl), part(pl), dx(pl), xn(pl), Mv(pl),
+        out(pl);
+    const double* P0 = stage_in(t, o, p0, psi0, n, s);
+    st->hfree.alloc(nf * sizeof(double));
+    double* x = st->hfree.as<double>();
+    hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                       t->f2n.as<int32_t>(), P0, x);
+    for (DBuf* d : {&F, &dFa, &dFb}) d->alloc(ne * sizeof(double));
+    for (DBuf* d : {&r, &rn, &dx, &xn}) d->alloc(nf * sizeof(double));
+    part.alloc(2 * nblocks(nf) * sizeof(double));
+    LevelDev& d0 = *t->lev[0];
+    Mv.alloc(d0.A.entries * sizeof(double));
+    std::vector<Cheb> chs = rich_chebs(t);
+    auto res = rich_residual(t, s, in, x, F.as<double>(), dFa.as<double>(), dFb.as<double>(),
+                             r.as<double>(), part);
+    double r0 = res.first;
+    double tol = std::max(o->reltol * r0, o->abstol);
+    int newton = 0, lin = 0, status = DPF_OK;
+    while (res.first > tol) {
+      if (newton >= 100 || lin >= o->maxiter * 50) {
+        status = DPF_ENOCONV;
+        break;
+      }
+      if (!std::isfinite(res.first)) {
+        status = DPF_EBREAKDOWN;
+        break;
+      }
+      rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), Mv.as<double>(), nullptr);
+      int its = 0;
+      int ls = rich_bicgstab(t, s, chs, d0.A.view<double>(Mv.as<double>()), r.as<double>(),
+                             dx.as<double>(), 1e-10, o->maxiter, &its);
+      lin += its;
+      if (ls == DPF_EBREAKDOWN) {
+        status = DPF_EBREAKDOWN;
+        break;
+      }
+      // M = -dr/dpsi, so the Newton step is +dx; backtrack on ||r||_2
+      double lam = 1.0;
+      std::pair<double, double> rnew;
+      while (true) {
+        copy_dev(xn.p, x, nf * sizeof(double), s);
+        hipLaunchKernelGGL(k_rich_axpby, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, lam,
+                           (const double*)dx.as<double>(), 1.0, xn.as<double>(), 0.0,
+                           (const double*)nullptr);
+        rnew = rich_residual(t, s, in, xn.as<double>(), F.as<double>(), dFa.as<double>(),
+                             dFb.as<double>(), rn.as<double>(), part);
+        if (rnew.second < (1.0 - 1e-4 * lam) * res.second || lam < 1e-6) break;
+        lam *= 0.5;
+      }
+      copy_dev(x, xn.p, nf * sizeof(double), s);
+      copy_dev(r.p, rn.p, nf * sizeof(double), s);
+      res = rnew;
+      newton++;
+    }
+    double* pdev = psi_out;
+    if (o->memory == DPF_HOST) {
+      out.alloc(n * sizeof(double));
+      pdev = out.as<double>();
+    }
+    hipLaunchKernelGGL(k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                       (const double*)x, in.dpsi, pdev);
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(psi_out, pdev, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iterations = lin;
+      stats->outer = newton;
+      stats->converged = status == DPF_OK;
+      stats->levels = t->levels();
+      stats->relres = r0 > 0 ? res.first / r0 : res.first;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status == DPF_ENOCONV) set_err("Richards Newton iteration did not converge");
+    if (status == DPF_EBREAKDOWN) set_err("Richards Newton iteration broke down (non-finite "
+                                          "residual or a failed linear solve)");
+    if (state_out && status == DPF_OK) *state_out = st.release();
+    return status;
+  });
+}
+
+extern "C" int dpf_richards_pullback(dpf_state* st, const double* delta, double* dKs,
+                                     double* ddirichletpsis, double* dalphas, double* dNs,
+                                     double* dQs, const dpf_opts* opts, dpf_stats* stats) {
+  if (!st || !st->topo || !delta) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (st->kind != 3) {
+    set_err("the state does not come from dpf_richards_steady");
+    return DPF_EINVAL;
+  }
+  dpf_topology* t = st->topo;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double t0 = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    bool host = o->memory == DPF_HOST;
+    t->timing = false;
+    tp_build_maps(t);
+    BufPool* pl = t->pool.get();
+    RichIn in{st->Ke.as<double>(), st->dh.as<double>(), st->r_z.as<double>(),
+              st->r_al.as<double>(), st->r_N.as<double>(), st->Q.as<double>()};
+    const double* x = st->hfree.as<double>();
+    DBuf F(pl), dFa(pl), dFb(pl), Mv(pl), MTv(pl), db(pl), rhs(pl), mu(pl), ga(pl), gb(pl),
+        oK(pl), oA(pl), oN(pl), oD(pl), oQ(pl);
+    for (DBuf* d : {&F, &dFa, &dFb, &ga, &gb}) d->alloc(ne * sizeof(double));
+    LevelDev& d0 = *t->lev[0];
+    Mv.alloc(d0.A.entries * sizeof(double));
+    MTv.alloc(d0.A.entries * sizeof(double));
+    hipLaunchKernelGGL(k_rich_edge_flux, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                       t->edges.as<int2>(), t->n2f.as<int32_t>(), x, in.dpsi, in.z, in.al, in.N,
+                       in.K, t->aol.as<double>(), F.as<double>(), dFa.as<double>(),
+                       dFb.as<double>());
+    rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), Mv.as<double>(),
+                  MTv.as<double>());
+    const double* dl = stage_in(t, o, db, delta, n, s);
+    rhs.alloc(nf * sizeof(double));
+    mu.alloc(nf * sizeof(double));
+    hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                       t->f2n.as<int32_t>(), dl, rhs.as<double>());
+    // lambda = J^-T delta = -(M^T)^-1 delta
+    int its = 0;
+    std::vector<Cheb> chs = rich_chebs(t);
+    int status = rich_bicgstab(t, s, chs, d0.A.view<double>(MTv.as<double>()), rhs.as<double>(),
+                               mu.as<double>(), 1e-12, o->maxiter, &its);
+    hipLaunchKernelGGL(k_rich_axpby, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, 0.0,
+                       (const double*)mu.as<double>(), -1.0, mu.as<double>(), 0.0,
+                       (const double*)nullptr);
+    auto dev_out = [&](DBuf& b, double* user, int64_t count) -> double* {
+      if (!user) return nullptr;
+      if (!host) return user;
+      b.alloc(count * sizeof(double));
+      return b.as<double>();
+    };
+    double* k_ = dev_out(oK, dKs, ne);
+    double* a_ = dev_out(oA, dalphas, ne);
+    double* n_ = dev_out(oN, dNs, ne);
+    double* d_ = dev_out(oD, ddirichletpsis, n);
+    double* q_ = dev_out(oQ, dQs, n);
+    hipLaunchKernelGGL(k_rich_tangents, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                       t->edges.as<int2>(), t->n2f.as<int32_t>(), x, in.dpsi, in.z, in.al, in.N,
+                       in.K, t->aol.as<double>(), (const double*)mu.as<double>(), k_, a_, n_,
+                       ga.as<double>(), gb.as<double>());
+    hipLaunchKernelGGL(k_rich_cell_tangents, dim3(nblocks(n)), dim3(BLK), 0, s, n,
+                       t->n2f.as<int32_t>(), t->edges.as<int2>(), t->inc_ptr.as<int64_t>(),
+                       t->inc_edge.as<int32_t>(), (const double*)ga.as<double>(),
+                       (const double*)gb.as<double>(), (const double*)mu.as<double>(), dl, d_,
+                       q_);
+    HIPTRY(hipGetLastError());
+    if (host) {
+      auto back = [&](double* user, double* dev, int64_t count) {
+        if (user)
+          HIPTRY(hipMemcpyAsync(user, dev, count * sizeof(double), hipMemcpyDeviceToHost, s));
+      };
+      back(dKs, k_, ne);
+      back(dalphas, a_, ne);
+      back(dNs, n_, ne);
+      back(ddirichletpsis, d_, n);
+      back(dQs, q_, n);
+    }
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iterations = its;
+      stats->converged = status == DPF_OK;
+      stats->total_ms = now_ms() - t0;
+    }
+    if (status == DPF_ENOCONV) set_err("adjoint BiCGStab did not reach the tolerance");
+    if (status == DPF_EBREAKDOWN) set_err("adjoint BiCGStab broke down");
+    return status;
+  });
+}
+
+extern "C" int dpf_richards_residuals(dpf_topology* t, const double* psi_free, const double* Ks,
+                                      const double* dirichletpsis, const double* z,
+                                      const double* alphas, const double* Ns, const double* Qs,
+                                      const double* Ss, const double* V, double* r_out,
+                                      const dpf_opts* opts) {
+  if (!t || !psi_free || !Ks || !dirichletpsis || !z || !alphas || !Ns || !Qs || !Ss || !V ||
+      !r_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    BufPool* pl = t->pool.get();
+    DBuf b1(pl), b2(pl), b3(pl), b4(pl), b5(pl), b6(pl), b7(pl), b8(pl), b9(pl), F(pl), sv(pl),
+        r(pl), part(pl);
+    const double* x = stage_in(t, o, b1, psi_free, nf, s);
+    RichIn in{stage_in(t, o, b2, Ks, ne, s), stage_in(t, o, b3, dirichletpsis, n, s),
+              stage_in(t, o, b4, z, n, s),   stage_in(t, o, b5, alphas, ne, s),
+              stage_in(t, o, b6, Ns, ne, s), stage_in(t, o, b7, Qs, n, s)};
+    const double* ss = stage_in(t, o, b8, Ss, n, s);
+    const double* vv = stage_in(t, o, b9, V, n, s);
+    F.alloc(ne * sizeof(double));
+    sv.alloc(n * sizeof(double));
+    part.alloc(2 * nblocks(nf) * sizeof(double));
+    double* rd = r_out;
+    if (o->memory == DPF_HOST) {
+      r.alloc(nf * sizeof(double));
+      rd = r.as<double>();
+    }
+    hipLaunchKernelGGL(k_rich_mul, dim3(nblocks(n)), dim3(BLK), 0, s, n, ss, vv, sv.as<double>());
+    hipLaunchKernelGGL(k_rich_edge_flux, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                       t->edges.as<int2>(), t->n2f.as<int32_t>(), x, in.dpsi, in.z, in.al, in.N,
+                       in.K, t->aol.as<double>(), F.as<double>(), (double*)nullptr,
+                       (double*)nullptr);
+    hipLaunchKernelGGL(k_rich_residual, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                       t->f2n.as<int32_t>(), t->edges.as<int2>(), t->inc_ptr.as<int64_t>(),
+                       t->inc_edge.as<int32_t>(), (const double*)F.as<double>(), in.Q,
+                       (const double*)sv.as<double>(), rd, part.as<double>());
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(r_out, rd, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return DPF_OK;
   });
 }

@@ -208,6 +208,41 @@ int dpf_solvetwophase(dpf_topology *topo, const double *h0, const double *S0, co
                       int64_t *nsub_total, dpf_state **state_out, const dpf_opts *opts,
                       dpf_stats *stats);
 
+/* Steady Richards equation (src/richards.jl:97-205), FP64.  Per-edge van
+ * Genuchten parameters alphas, Ns (nedges), the cell elevation z (n, the
+ * third row of the reference's coords), Dirichlet pressure heads
+ * dirichletpsis (n).  Single-system topologies with at least two AMG levels.
+ *
+ * richards_residuals (src/richards.jl:97-117): r (nfree) at the free heads
+ * psi_free (nfree), with storage Ss and volumes V (n). */
+int dpf_richards_residuals(dpf_topology *topo, const double *psi_free, const double *Ks,
+                           const double *dirichletpsis, const double *z, const double *alphas,
+                           const double *Ns, const double *Qs, const double *Ss, const double *V,
+                           double *r_out, const dpf_opts *opts);
+
+/* richards_steadystate (src/richards.jl:156-179; Ss = V = 1): Newton's
+ * method from psi0 (n) with backtracking on ||r||_2, until max|r| <=
+ * max(reltol max|r(psi0)|, abstol).  Each Newton system M dpsi = r, M =
+ * -dr/dpsi (the Picard operator K_e hm(kr_a, kr_b) plus the kr-slope terms,
+ * nonsymmetric), is solved by BiCGStab to 1e-10 preconditioned by the AMG
+ * V-cycle of the Picard operator.  psi_out (n): free heads, Dirichlet cells
+ * hold dirichletpsis.  stats: iterations = BiCGStab iterations, outer = Newton
+ * steps, relres = final max|r| / max|r(psi0)|.  state_out (may be NULL) keeps
+ * what the pullback needs. */
+int dpf_richards_steady(dpf_topology *topo, const double *psi0, const double *Ks,
+                        const double *dirichletpsis, const double *z, const double *alphas,
+                        const double *Ns, const double *Qs, double *psi_out,
+                        const dpf_opts *opts, dpf_stats *stats, dpf_state **state_out);
+
+/* The rrule's pullback (src/richards.jl:185-203): lambda = J^-T delta[free]
+ * (BiCGStab on the transposed Newton matrix, same preconditioner), then
+ * dKs, dalphas, dNs (nedges), ddirichletpsis, dQs (n) = -(dr/d.)^T lambda
+ * (+ delta on the Dirichlet cells); any output may be NULL.  (The tangent in
+ * psi0 is zero.) */
+int dpf_richards_pullback(dpf_state *state, const double *delta, double *dKs,
+                          double *ddirichletpsis, double *dalphas, double *dNs, double *dQs,
+                          const dpf_opts *opts, dpf_stats *stats);
+
 /* Reverse mode of dpf_solvetwophase: the gradient that test/twophase.jl:192-205
  * takes with Zygote through the rrules of solvepressure and saturationcalc
  * (src/twophase.jl:59-80,117-183).  Given dJ/dP and dJ/dS of the last step
