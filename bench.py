@@ -42,12 +42,14 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase"],
+    ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase",
+                                           "richards"],
                     default="steady2m",
                     help="steady2m (default, the headline line); ensemble: 96 packed "
                          "realizations on 32x64^3 + 32x96x96x24 + 32x128x128x16 grids; "
                          "transient: 200 backward-Euler steps on the 2M grid with the "
-                         "checkpointed adjoint; twophase: IMPES steps on the 2M grid")
+                         "checkpointed adjoint; twophase: IMPES steps on the 2M grid; "
+                         "richards: steady Richards solve + gradient on the 2M grid")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -68,6 +70,8 @@ def main():
         return bench_transient(args, P, api, _lib, torch)
     if args.workload == "twophase":
         return bench_twophase(args, P, api, _lib, torch)
+    if args.workload == "richards":
+        return bench_richards(args, P, api, _lib, torch)
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -595,6 +599,74 @@ def bench_twophase(args, P, api, _lib, torch):
            "upstream": {"us_per_substep": round(us, 2), "algorithmic_bytes": alg,
                         "GBps": round(gbs, 1), "peak_GBps": 8000.0, "frac": round(gbs / 8000, 4)},
            "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
+
+
+def bench_richards(args, P, api, _lib, torch):
+    """Steady Richards (src/richards.jl:156-205) on the 2M-cell grid: the
+    example's van Genuchten parameters (alpha 0.375, N 1.25 on every edge,
+    examples/richards_steadystate_3d/inputdeck.jl:25-26), geometric-mean K_e
+    from the 2M logK field, pressure head -1 m on the top face (unsaturated
+    near the top, saturated below) and 4 extraction wells of 1e-5 m^3/s at the
+    2M injector cells.  One step = forward Newton solve from psi0 = 0 plus the
+    rrule's pullback for the heads at the 200 observation cells; inputs
+    resident on the GPU."""
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    w = P.workloads.steady_2m()
+    n, ne = w.n, w.nedges
+    topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    coords, _, _, _ = P.grid.regulargrid3d((0.0, 0.0, 0.0), w.maxs, w.ns)
+    nb0 = w.neighbors - 1
+    Ks = torch.from_numpy(np.exp(0.5 * (w.logK[nb0[:, 0]] + w.logK[nb0[:, 1]]))).to(dev)
+    dpsi = torch.from_numpy(np.where(np.isin(np.arange(n), w.dirichletnodes - 1), -1.0, 0.0)).to(dev)
+    z = torch.from_numpy(np.ascontiguousarray(coords[2])).to(dev)
+    al = torch.full((ne,), 0.375, dtype=torch.float64, device=dev)
+    Nv = torch.full((ne,), 1.25, dtype=torch.float64, device=dev)
+    Q = torch.from_numpy(np.where(w.Qs > 0, -1e-5, 0.0)).to(dev)
+    psi0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    psi = torch.empty(n, dtype=torch.float64, device=dev)
+    delta = torch.zeros(n, dtype=torch.float64, device=dev)
+    delta[torch.from_numpy(w.obs - 1).to(dev)] = 1.0
+    dK = torch.empty(ne, dtype=torch.float64, device=dev)
+    o = api._opts({"reltol": 1e-10}, memory=_lib.DPF_DEVICE, stream=sptr)
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def step():
+        st, sh = _lib.Stats(), ctypes.c_void_p()
+        _lib.check(L.dpf_richards_steady(topo.handle, p(psi0), p(Ks), p(dpsi), p(z), p(al), p(Nv),
+                                         p(Q), p(psi), ctypes.byref(o), ctypes.byref(st),
+                                         ctypes.byref(sh)))
+        st2 = _lib.Stats()
+        code = L.dpf_richards_pullback(sh, p(delta), p(dK), None, None, None, None,
+                                       ctypes.byref(o), ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        return st, st2
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    a, b = stats[-1]
+    frac = float((psi < 0).double().mean())
+    log(f"[bench] richards: {args.steps / dt:.2f} forward+gradient evals/s ({1e3 * dt / args.steps:.0f} ms), "
+        f"Newton steps {a.outer}, BiCGStab iterations {a.iterations} forward / {b.iterations} adjoint, "
+        f"relres {a.relres:.1e}, {100 * frac:.1f} % of cells unsaturated")
+    res = {"metric": "steady Richards forward+gradient evals/s, 2M cells",
+           "value": round(args.steps / dt, 3), "unit": "evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "data": "synthetic", "dtype": "f64",
+           "config": {"workload": "richards 2M (200x200x50), alpha 0.375, N 1.25, psi = -1 on "
+                                  "the top face, 4 extraction wells of 1e-5 m^3/s"},
+           "newton_steps": a.outer, "bicgstab_forward": a.iterations,
+           "bicgstab_adjoint": b.iterations, "unsaturated_fraction": round(frac, 4)}
     print(json.dumps(res), flush=True)
 
 
