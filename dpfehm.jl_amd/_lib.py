@@ -84,6 +84,10 @@ SIGNATURES = {
     "dpf_richards_residuals": (ctypes.c_int, [_p] * 12),
     "dpf_richards_steady": (ctypes.c_int, [_p] * 12),
     "dpf_richards_pullback": (ctypes.c_int, [_p] * 9),
+    "dpf_transport_residuals": (ctypes.c_int, [_p] * 12 + [_i32, _p, _p]),
+    "dpf_transport_steps": (ctypes.c_int, [_p] * 12 + [_i32, ctypes.c_double, ctypes.c_double,
+                                                       _i32, _p, _p, _p, _p, _p]),
+    "dpf_transport_pullback": (ctypes.c_int, [_p] * 9),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),

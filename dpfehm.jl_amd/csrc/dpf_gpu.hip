@@ -28,6 +28,7 @@
 #include "kernels.hpp"
 #include "twophase.hpp"
 #include "richards.hpp"
+#include "transport.hpp"
 
 namespace dpf {
 
@@ -322,6 +323,9 @@ struct dpf_state {
   // steady Richards (kind 3): elevation and per-edge van Genuchten parameters
   // (hfree holds the free heads, Ke the edge K, dh the Dirichlet heads)
   DBuf r_z, r_al, r_N;
+  // transport (kind 4): every state u_0 .. u_N and the run's inputs
+  DBuf tr_u, tr_v, tr_D, tr_ud, tr_Q, tr_V, tr_c;
+  int tr_upwind = 1;
   std::vector<int64_t> tp_nts;
   double tp_fluid[4] = {0, 0, 0, 0};
   int tp_zero_mean = 0;
@@ -330,7 +334,8 @@ struct dpf_state {
 
 dpf_state::dpf_state(dpf_topology* t) : topo(t), pool(t->pool), version(0), precision(0) {
   for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt, &tp_S, &tp_Ksum, &tp_P,
-                  &tp_Vn, &tp_K, &tp_pv, &r_z, &r_al, &r_N})
+                  &tp_Vn, &tp_K, &tp_pv, &r_z, &r_al, &r_N, &tr_u, &tr_v, &tr_D, &tr_ud,
+                  &tr_Q, &tr_V, &tr_c})
     b->pool = pool.get();
 }
 
@@ -2539,13 +2544,13 @@ void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, 
   copy_dev(z, out, nf * sizeof(double), s);
 }
 
-// Right-preconditioned BiCGStab for M x = b (M on the fine layout), x from 0,
-// to ||b - M x|| <= tol ||b||.  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN;
-// *iters counts the iterations.
-int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
-                  SellD<double> M, const double* b, double* x, double tol, int maxit,
-                  int* iters) {
-  int64_t nf = t->H.nf;
+// Right-preconditioned BiCGStab for A x = b (apply(x, y): y = A x;
+// prec(b, z): z ~ A^-1 b), vectors of length nf, x from 0, to
+// ||b - A x|| <= tol ||b||.  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN; *iters
+// counts the iterations.
+template <class Apply, class Prec>
+int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec, const double* b,
+             double* x, double tol, int maxit, int* iters) {
   unsigned nb = nblocks(nf);
   BufPool* pl = t->pool.get();
   DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl), part(pl);
@@ -2570,9 +2575,8 @@ int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
     // p = r + beta (p - omega v)
     hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, 1.0, (const double*)r.as<double>(), beta,
                        p.as<double>(), -beta * omega, (const double*)v.as<double>());
-    rich_precond(t, s, chs, p.as<double>(), ph.as<double>());
-    hipLaunchKernelGGL(k_rich_spmv, g, bl, 0, s, M, (const double*)ph.as<double>(),
-                       v.as<double>());
+    prec((const double*)p.as<double>(), ph.as<double>());
+    apply((const double*)ph.as<double>(), v.as<double>());
     double rv = host_dot(t, s, part, hp, nf, rh.as<double>(), v.as<double>());
     if (rv == 0.0 || !std::isfinite(rv)) return DPF_EBREAKDOWN;
     alpha = rho1 / rv;
@@ -2587,9 +2591,8 @@ int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
                          1.0, x, 0.0, (const double*)nullptr);
       return DPF_OK;
     }
-    rich_precond(t, s, chs, sv.as<double>(), sh.as<double>());
-    hipLaunchKernelGGL(k_rich_spmv, g, bl, 0, s, M, (const double*)sh.as<double>(),
-                       tv.as<double>());
+    prec((const double*)sv.as<double>(), sh.as<double>());
+    apply((const double*)sh.as<double>(), tv.as<double>());
     double ts = host_dot(t, s, part, hp, nf, tv.as<double>(), sv.as<double>());
     double tt = host_dot(t, s, part, hp, nf, tv.as<double>(), tv.as<double>());
     if (tt == 0.0 || !std::isfinite(tt)) return DPF_EBREAKDOWN;
@@ -2606,6 +2609,19 @@ int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
     rho = rho1;
   }
   return DPF_ENOCONV;
+}
+
+// BiCGStab on a Newton matrix of the fine layout, preconditioned by the
+// V-cycle of the current (Picard) hierarchy.
+int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
+                  SellD<double> M, const double* b, double* x, double tol, int maxit,
+                  int* iters) {
+  int64_t nf = t->H.nf;
+  auto apply = [&](const double* xx, double* yy) {
+    hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, M, xx, yy);
+  };
+  auto prec = [&](const double* bb, double* zz) { rich_precond(t, s, chs, bb, zz); };
+  return bicgstab(t, s, nf, apply, prec, b, x, tol, maxit, iters);
 }
 
 // The inputs of one Richards problem on the device.
@@ -2942,5 +2958,276 @@ extern "C" int dpf_richards_residuals(dpf_topology* t, const double* psi_free, c
       HIPTRY(hipMemcpyAsync(r_out, rd, nf * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     return DPF_OK;
+  });
+}
+
+// ====================================================================== advection-dispersion
+
+namespace {
+
+// Device copies of one transport problem's inputs (kept in st when given).
+struct TrBufs {
+  DBuf v, D, ud, Q, V, c;
+  explicit TrBufs(BufPool* p) : v(p), D(p), ud(p), Q(p), V(p), c(p) {}
+};
+
+TrIn tr_inputs(dpf_topology* t, const dpf_opts* o, hipStream_t s, DBuf& v, DBuf& D, DBuf& ud,
+               DBuf& Q, DBuf& V, DBuf& c, const double* vx, const double* vy, const double* vz,
+               const double* Ds, const double* dirichletus, const double* Qs,
+               const double* volumes, const double* cx, const double* cy, const double* cz,
+               int upwind) {
+  int64_t n = t->H.n, ne = t->H.ne;
+  auto kind = o->memory == DPF_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  auto put = [&](DBuf& d, std::initializer_list<const double*> src, int64_t count) {
+    d.alloc(std::max<int64_t>(count * (int64_t)src.size(), 1) * sizeof(double));
+    int64_t k = 0;
+    for (const double* p : src)
+      HIPTRY(hipMemcpyAsync(d.as<double>() + (k++) * count, p, count * sizeof(double), kind, s));
+  };
+  put(v, {vx, vy, vz}, ne);
+  put(D, {Ds}, ne);
+  put(ud, {dirichletus}, n);
+  put(Q, {Qs}, n);
+  put(V, {volumes}, n);
+  put(c, {cx, cy, cz}, n);
+  TrIn in;
+  in.vx = v.as<double>();
+  in.vy = in.vx + ne;
+  in.vz = in.vy + ne;
+  in.D = D.as<double>();
+  in.ud = ud.as<double>();
+  in.Q = Q.as<double>();
+  in.V = V.as<double>();
+  in.cx = c.as<double>();
+  in.cy = in.cx + n;
+  in.cz = in.cy + n;
+  in.upwind = upwind;
+  return in;
+}
+
+void tr_rows(dpf_topology* t, hipStream_t s, const TrIn& in, const double* x, double a, double b,
+             double g, double* y) {
+  int64_t n = t->H.n;
+  hipLaunchKernelGGL(k_tr_rows, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                     t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                     t->inc_other.as<int32_t>(), t->aol.as<double>(), in, x, a, b, g, y);
+}
+
+}  // namespace
+
+extern "C" int dpf_transport_residuals(dpf_topology* t, const double* u, const double* vx,
+                                       const double* vy, const double* vz, const double* Ds,
+                                       const double* dirichletus, const double* Qs,
+                                       const double* volumes, const double* cx,
+                                       const double* cy, const double* cz, int32_t use_upwind,
+                                       double* r_out, const dpf_opts* opts) {
+  if (!t || !u || !vx || !vy || !vz || !Ds || !dirichletus || !Qs || !volumes || !cx || !cy ||
+      !cz || !r_out) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n;
+    BufPool* pl = t->pool.get();
+    TrBufs b(pl);
+    DBuf ub(pl), rb(pl);
+    TrIn in = tr_inputs(t, o, s, b.v, b.D, b.ud, b.Q, b.V, b.c, vx, vy, vz, Ds, dirichletus, Qs,
+                        volumes, cx, cy, cz, use_upwind);
+    const double* ud = stage_in(t, o, ub, u, n, s);
+    double* rd = r_out;
+    if (o->memory == DPF_HOST) {
+      rb.alloc(n * sizeof(double));
+      rd = rb.as<double>();
+    }
+    tr_rows(t, s, in, ud, 0.0, 1.0, 1.0, rd);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST)
+      HIPTRY(hipMemcpyAsync(r_out, rd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return DPF_OK;
+  });
+}
+
+extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const double* vx,
+                                   const double* vy, const double* vz, const double* Ds,
+                                   const double* dirichletus, const double* Qs,
+                                   const double* volumes, const double* cx, const double* cy,
+                                   const double* cz, int32_t use_upwind, double t0, double t1,
+                                   int32_t nsteps, double* u_final, double* u_all,
+                                   const dpf_opts* opts, dpf_stats* stats,
+                                   dpf_state** state_out) {
+  if (state_out) *state_out = nullptr;
+  if (!t || !u0 || !vx || !vy || !vz || !Ds || !dirichletus || !Qs || !volumes || !cx || !cy ||
+      !cz || !u_final || nsteps < 1 || !(t1 > t0)) {
+    set_err(nsteps < 1 || !(t1 > t0) ? "need nsteps >= 1 and t1 > t0" : "NULL argument");
+    return DPF_EINVAL;
+  }
+  if (int rc = twophase_checks(t)) return rc;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double tstart = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n;
+    bool host = o->memory == DPF_HOST;
+    BufPool* pl = t->pool.get();
+    auto st = std::make_unique<dpf_state>(t);
+    st->kind = 4;
+    st->nsteps = nsteps;
+    st->dt = (t1 - t0) / nsteps;
+    st->tr_upwind = use_upwind;
+    TrIn in = tr_inputs(t, o, s, st->tr_v, st->tr_D, st->tr_ud, st->tr_Q, st->tr_V, st->tr_c, vx,
+                        vy, vz, Ds, dirichletus, Qs, volumes, cx, cy, cz, use_upwind);
+    double dt = st->dt;
+    st->tr_u.alloc((size_t)(nsteps + 1) * n * sizeof(double));
+    double* U = st->tr_u.as<double>();
+    HIPTRY(hipMemcpyAsync(U, u0, n * sizeof(double),
+                          host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    DBuf dg(pl), rhs(pl);
+    dg.alloc(n * sizeof(double));
+    rhs.alloc(n * sizeof(double));
+    hipLaunchKernelGGL(k_tr_diag, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                       t->inc_other.as<int32_t>(), t->aol.as<double>(), in, dt, dg.as<double>());
+    auto apply = [&](const double* xx, double* yy) { tr_rows(t, s, in, xx, 1.0, -dt, 0.0, yy); };
+    auto prec = [&](const double* bb, double* zz) {
+      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(n)), dim3(BLK), 0, s, n, bb,
+                         (const double*)dg.as<double>(), zz);
+    };
+    int status = DPF_OK, iters = 0;
+    for (int k = 0; k < nsteps && status == DPF_OK; k++) {
+      const double* uk = U + (int64_t)k * n;
+      tr_rows(t, s, in, uk, 1.0, 0.0, dt, rhs.as<double>());
+      int it = 0;
+      status = bicgstab(t, s, n, apply, prec, (const double*)rhs.as<double>(),
+                        U + (int64_t)(k + 1) * n, std::max(o->reltol, 1e-14), o->maxiter, &it);
+      iters += it;
+    }
+    auto out = [&](double* dst, const double* src, int64_t count) {
+      HIPTRY(hipMemcpyAsync(dst, src, count * sizeof(double),
+                            host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    };
+    out(u_final, U + (int64_t)nsteps * n, n);
+    if (u_all) out(u_all, U, (int64_t)(nsteps + 1) * n);
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iterations = iters;
+      stats->outer = nsteps;
+      stats->converged = status == DPF_OK;
+      stats->total_ms = now_ms() - tstart;
+    }
+    if (status == DPF_ENOCONV) set_err("BiCGStab of a backward-Euler step did not converge");
+    if (status == DPF_EBREAKDOWN) set_err("BiCGStab of a backward-Euler step broke down");
+    if (state_out && status == DPF_OK) *state_out = st.release();
+    return status;
+  });
+}
+
+extern "C" int dpf_transport_pullback(dpf_state* st, const double* dJ_du_final, double* dvx,
+                                      double* dvy, double* dvz, double* dDs, double* dQs,
+                                      const dpf_opts* opts, dpf_stats* stats) {
+  if (!st || !st->topo || !dJ_du_final) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (st->kind != 4) {
+    set_err("the state does not come from dpf_transport_steps");
+    return DPF_EINVAL;
+  }
+  dpf_topology* t = st->topo;
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    double tstart = now_ms();
+    hipStream_t s = pick_stream(t, o);
+    int64_t n = t->H.n, ne = t->H.ne;
+    bool host = o->memory == DPF_HOST;
+    BufPool* pl = t->pool.get();
+    TrIn in;
+    in.vx = st->tr_v.as<double>();
+    in.vy = in.vx + ne;
+    in.vz = in.vy + ne;
+    in.D = st->tr_D.as<double>();
+    in.ud = st->tr_ud.as<double>();
+    in.Q = st->tr_Q.as<double>();
+    in.V = st->tr_V.as<double>();
+    in.cx = st->tr_c.as<double>();
+    in.cy = in.cx + n;
+    in.cz = in.cy + n;
+    in.upwind = st->tr_upwind;
+    double dt = st->dt;
+    DBuf lam(pl), mu(pl), dg(pl), gv(pl), gD(pl), gQ(pl);
+    lam.alloc(n * sizeof(double));
+    mu.alloc(n * sizeof(double));
+    dg.alloc(n * sizeof(double));
+    gv.alloc(3 * ne * sizeof(double));
+    gD.alloc(ne * sizeof(double));
+    gQ.alloc(n * sizeof(double));
+    HIPTRY(hipMemsetAsync(gv.p, 0, 3 * ne * sizeof(double), s));
+    HIPTRY(hipMemsetAsync(gD.p, 0, ne * sizeof(double), s));
+    HIPTRY(hipMemsetAsync(gQ.p, 0, n * sizeof(double), s));
+    HIPTRY(hipMemcpyAsync(lam.p, dJ_du_final, n * sizeof(double),
+                          host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+    // the transpose has the same diagonal
+    hipLaunchKernelGGL(k_tr_diag, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                       t->inc_other.as<int32_t>(), t->aol.as<double>(), in, dt, dg.as<double>());
+    auto apply = [&](const double* xx, double* yy) {
+      hipLaunchKernelGGL(k_tr_rows_t, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                         t->inc_other.as<int32_t>(), t->aol.as<double>(), in, xx, dt, yy);
+    };
+    auto prec = [&](const double* bb, double* zz) {
+      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(n)), dim3(BLK), 0, s, n, bb,
+                         (const double*)dg.as<double>(), zz);
+    };
+    int status = DPF_OK, iters = 0;
+    double* gvx = gv.as<double>();
+    for (int k = st->nsteps; k >= 1 && status == DPF_OK; k--) {
+      int it = 0;
+      status = bicgstab(t, s, n, apply, prec, (const double*)lam.as<double>(), mu.as<double>(),
+                        std::max(o->reltol, 1e-14), o->maxiter, &it);
+      iters += it;
+      const double* uk = st->tr_u.as<double>() + (int64_t)k * n;
+      hipLaunchKernelGGL(k_tr_grad_edges, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                         t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), in, uk,
+                         (const double*)mu.as<double>(), dt, gvx, gvx + ne, gvx + 2 * ne,
+                         gD.as<double>());
+      hipLaunchKernelGGL(k_tr_grad_q, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                         in, (const double*)mu.as<double>(), dt, gQ.as<double>());
+      copy_dev(lam.p, mu.p, n * sizeof(double), s);
+    }
+    HIPTRY(hipGetLastError());
+    auto out = [&](double* dst, const double* src, int64_t count) {
+      if (!dst) return;
+      HIPTRY(hipMemcpyAsync(dst, src, count * sizeof(double),
+                            host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    };
+    out(dvx, gvx, ne);
+    out(dvy, gvx + ne, ne);
+    out(dvz, gvx + 2 * ne, ne);
+    out(dDs, gD.as<double>(), ne);
+    out(dQs, gQ.as<double>(), n);
+    HIPTRY(hipStreamSynchronize(s));
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iterations = iters;
+      stats->converged = status == DPF_OK;
+      stats->total_ms = now_ms() - tstart;
+    }
+    if (status == DPF_ENOCONV) set_err("adjoint BiCGStab did not converge");
+    if (status == DPF_EBREAKDOWN) set_err("adjoint BiCGStab broke down");
+    return status;
   });
 }

@@ -243,6 +243,41 @@ int dpf_richards_pullback(dpf_state *state, const double *delta, double *dKs,
                           double *ddirichletpsis, double *dalphas, double *dNs, double *dQs,
                           const dpf_opts *opts, dpf_stats *stats);
 
+/* Advection-dispersion (src/transport.jl:2-33), FP64, single-system
+ * topologies (their Dirichlet cells are the transport's).  Per-edge velocity
+ * components vx, vy, vz and dispersion Ds (nedges); cell coordinates cx, cy,
+ * cz, volumes, Qs, dirichletus (n); use_upwind 1 (the reference's default) or
+ * 0 (downwind).
+ *
+ * transport_residuals: r (n) at u (n), Dirichlet rows u - dirichletus. */
+int dpf_transport_residuals(dpf_topology *topo, const double *u, const double *vx,
+                            const double *vy, const double *vz, const double *Ds,
+                            const double *dirichletus, const double *Qs, const double *volumes,
+                            const double *cx, const double *cy, const double *cz,
+                            int32_t use_upwind, double *r_out, const dpf_opts *opts);
+
+/* du/dt = transport_residuals(u) from t0 to t1 by nsteps fixed backward-Euler
+ * steps (the reference integrates with DifferentiableBackwardEuler.steps,
+ * adaptive, test/transport.jl:30-33); each step solves (I - dt J) u' = u +
+ * dt r(0) by Jacobi-preconditioned BiCGStab to opts->reltol.  u_final (n);
+ * u_all ((nsteps + 1) x n, may be NULL) every state.  state_out (may be NULL)
+ * keeps the trajectory for dpf_transport_pullback.  dt = 1 makes the Dirichlet
+ * rows singular, as in the reference's equations. */
+int dpf_transport_steps(dpf_topology *topo, const double *u0, const double *vx, const double *vy,
+                        const double *vz, const double *Ds, const double *dirichletus,
+                        const double *Qs, const double *volumes, const double *cx,
+                        const double *cy, const double *cz, int32_t use_upwind, double t0,
+                        double t1, int32_t nsteps, double *u_final, double *u_all,
+                        const dpf_opts *opts, dpf_stats *stats, dpf_state **state_out);
+
+/* Gradient of an objective of u_final (dJ_du_final, n) through those steps
+ * (the discrete adjoint; test/transport.jl:50-70 takes the same gradient with
+ * Zygote): dvx, dvy, dvz, dDs (nedges), dQs (n), any of them NULL.  Upwind
+ * choices are held fixed. */
+int dpf_transport_pullback(dpf_state *state, const double *dJ_du_final, double *dvx, double *dvy,
+                           double *dvz, double *dDs, double *dQs, const dpf_opts *opts,
+                           dpf_stats *stats);
+
 /* Reverse mode of dpf_solvetwophase: the gradient that test/twophase.jl:192-205
  * takes with Zygote through the rrules of solvepressure and saturationcalc
  * (src/twophase.jl:59-80,117-183).  Given dJ/dP and dJ/dS of the last step
