@@ -43,13 +43,14 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase",
-                                           "richards"],
+                                           "richards", "transport"],
                     default="steady2m",
                     help="steady2m (default, the headline line); ensemble: 96 packed "
                          "realizations on 32x64^3 + 32x96x96x24 + 32x128x128x16 grids; "
                          "transient: 200 backward-Euler steps on the 2M grid with the "
                          "checkpointed adjoint; twophase: IMPES steps on the 2M grid; "
-                         "richards: steady Richards solve + gradient on the 2M grid")
+                         "richards: steady Richards solve + gradient on the 2M grid; "
+                         "transport: advection-dispersion steps + gradient on the 2M grid")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -72,6 +73,8 @@ def main():
         return bench_twophase(args, P, api, _lib, torch)
     if args.workload == "richards":
         return bench_richards(args, P, api, _lib, torch)
+    if args.workload == "transport":
+        return bench_transport(args, P, api, _lib, torch)
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -667,6 +670,76 @@ def bench_richards(args, P, api, _lib, torch):
                                   "the top face, 4 extraction wells of 1e-5 m^3/s"},
            "newton_steps": a.outer, "bicgstab_forward": a.iterations,
            "bicgstab_adjoint": b.iterations, "unsaturated_fraction": round(frac, 4)}
+    print(json.dumps(res), flush=True)
+
+
+def bench_transport(args, P, api, _lib, torch):
+    """Advection-dispersion (src/transport.jl) on the 2M-cell grid: velocity
+    (1, 0.5, 0.1) m/s and dispersion 1e-3 m^2/s on every edge, a Gaussian
+    plume of 10 m radius, u = 0 on the top face; 50 backward-Euler steps over
+    10 s, then the gradient of the concentrations at the 200 observation cells
+    in the velocities, dispersion and sources.  Inputs resident on the GPU."""
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    w = P.workloads.steady_2m()
+    n, ne = w.n, w.nedges
+    topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    L = _lib.lib()
+    sptr = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    coords, _, _, vol = P.grid.regulargrid3d((0.0, 0.0, 0.0), w.maxs, w.ns)
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    c = [d(coords[k]) for k in range(3)]
+    vx, vy, vz = d(np.full(ne, 1.0)), d(np.full(ne, 0.5)), d(np.full(ne, 0.1))
+    Ds = d(np.full(ne, 1e-3))
+    ud, Q, V = d(np.zeros(n)), d(np.zeros(n)), d(vol)
+    r2 = (coords[0] - 30.0) ** 2 + (coords[1] - 30.0) ** 2 + (coords[2] - 12.0) ** 2
+    u0 = d(np.where(np.isin(np.arange(n), w.dirichletnodes - 1), 0.0, np.exp(-r2 / 100.0)))
+    u1 = torch.empty(n, dtype=torch.float64, device=dev)
+    seed = torch.zeros(n, dtype=torch.float64, device=dev)
+    seed[torch.from_numpy(w.obs - 1).to(dev)] = 1.0
+    g = [torch.empty(ne, dtype=torch.float64, device=dev) for _ in range(4)]
+    gq = torch.empty(n, dtype=torch.float64, device=dev)
+    o = api._opts({"reltol": 1e-10}, memory=_lib.DPF_DEVICE, stream=sptr)
+
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+    nsteps = 50
+
+    def step():
+        st, sh = _lib.Stats(), ctypes.c_void_p()
+        _lib.check(L.dpf_transport_steps(topo.handle, p(u0), p(vx), p(vy), p(vz), p(Ds), p(ud),
+                                         p(Q), p(V), p(c[0]), p(c[1]), p(c[2]), 1, 0.0, 10.0,
+                                         nsteps, p(u1), None, ctypes.byref(o), ctypes.byref(st),
+                                         ctypes.byref(sh)))
+        torch.cuda.synchronize()
+        tf = time.perf_counter()
+        st2 = _lib.Stats()
+        code = L.dpf_transport_pullback(sh, p(seed), p(g[0]), p(g[1]), p(g[2]), p(g[3]), p(gq),
+                                        ctypes.byref(o), ctypes.byref(st2))
+        L.dpf_state_destroy(sh)
+        _lib.check(code)
+        return st, st2, tf
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, st2, tf = step()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    mass0 = float((u0 * V).sum())
+    mass1 = float((u1 * V).sum())
+    log(f"[bench] transport: {nsteps} backward-Euler steps {tf - t0:.3f} s ({st.iterations} "
+        f"BiCGStab iterations), gradient {t2 - tf:.3f} s ({st2.iterations}); mass "
+        f"{mass0:.4g} -> {mass1:.4g}")
+    res = {"metric": "advection-dispersion forward+gradient wall time, 2M cells x 50 backward-"
+                     "Euler steps", "value": round(t2 - t0, 3), "unit": "s", "n_gpus": 1,
+           "higher_is_better": False, "data": "synthetic", "dtype": "f64",
+           "config": {"workload": "transport 2M (200x200x50), v = (1, 0.5, 0.1), D = 1e-3, "
+                                  "50 steps over 10 s"},
+           "forward_s": round(tf - t0, 3), "gradient_s": round(t2 - tf, 3),
+           "bicgstab_forward": st.iterations, "bicgstab_adjoint": st2.iterations}
     print(json.dumps(res), flush=True)
 
 
