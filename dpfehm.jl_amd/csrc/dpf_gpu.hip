@@ -2516,19 +2516,6 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
 
 namespace {
 
-// Deterministic x.y: per-block sums on the device, their sum in block order on
-// the host (one synchronisation).
-double host_dot(dpf_topology* t, hipStream_t s, DBuf& part, std::vector<double>& hp, int64_t n,
-                const double* x, const double* y) {
-  unsigned nb = nblocks(n);
-  hipLaunchKernelGGL(k_rich_dot, dim3(nb), dim3(BLK), 0, s, n, x, y, part.as<double>());
-  HIPTRY(hipMemcpyAsync(hp.data(), part.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPTRY(hipStreamSynchronize(s));
-  double a = 0.0;
-  for (unsigned b = 0; b < nb; b++) a += hp[b];
-  return a;
-}
-
 // z = V-cycle(b) on the FP64 hierarchy (the Picard operator's), Chebyshev
 // degrees 1 on the finest level and 2 below, interval ratio 15.
 void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, const double* b,
@@ -2544,32 +2531,64 @@ void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, 
   copy_dev(z, out, nf * sizeof(double), s);
 }
 
+// Up to three dot products x_k . y_k of length n in one pass, finished on the
+// device in a fixed order; the three sums land in host memory (one
+// synchronisation).
+struct Dots3 {
+  DBuf part, out;
+  double h[3];
+  Dots3(BufPool* p, int64_t n) : part(p), out(p) {
+    part.alloc(3 * std::max<int64_t>(nblocks(n), 1) * sizeof(double));
+    out.alloc(3 * sizeof(double));
+  }
+  void run(hipStream_t s, int64_t n, const double* x0, const double* y0, const double* x1 = nullptr,
+           const double* y1 = nullptr, const double* x2 = nullptr, const double* y2 = nullptr) {
+    unsigned nb = nblocks(n);
+    hipLaunchKernelGGL(k_dots3, dim3(nb), dim3(BLK), 0, s, n, x0, y0, x1, y1, x2, y2,
+                       part.as<double>());
+    hipLaunchKernelGGL(k_dots3_fin, dim3(1), dim3(BLK), 0, s, (int64_t)nb,
+                       (const double*)part.as<double>(), out.as<double>());
+    HIPTRY(hipMemcpyAsync(h, out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+  }
+};
+
 // Right-preconditioned BiCGStab for A x = b (apply(x, y): y = A x;
-// prec(b, z): z ~ A^-1 b), vectors of length nf, x from 0, to
-// ||b - A x|| <= tol ||b||.  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN; *iters
-// counts the iterations.
+// prec(b, z): z ~ A^-1 b), vectors of length nf, x from 0 (guess: from the x
+// passed in), to ||b - A x|| <= tol ||b||.  Three synchronisations per
+// iteration: r^.v; s.s, t.s and t.t together; r.r and r^.r together.
+// Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN; *iters counts the iterations.
 template <class Apply, class Prec>
 int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec, const double* b,
-             double* x, double tol, int maxit, int* iters) {
-  unsigned nb = nblocks(nf);
+             double* x, double tol, int maxit, int* iters, bool guess = false) {
   BufPool* pl = t->pool.get();
-  DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl), part(pl);
+  DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl);
   for (DBuf* d : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) d->alloc(nf * sizeof(double));
-  part.alloc(nb * sizeof(double));
-  std::vector<double> hp(nb);
+  Dots3 dots(pl, nf);
   size_t bytes = nf * sizeof(double);
-  HIPTRY(hipMemsetAsync(x, 0, bytes, s));
-  copy_dev(r.p, b, bytes, s);
-  copy_dev(rh.p, b, bytes, s);
+  dim3 g(nblocks(nf)), bl(BLK);
+  if (guess) {
+    // r = b - A x
+    apply((const double*)x, r.as<double>());
+    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, 1.0, b, -1.0, r.as<double>(), 0.0,
+                       (const double*)nullptr);
+  } else {
+    HIPTRY(hipMemsetAsync(x, 0, bytes, s));
+    copy_dev(r.p, b, bytes, s);
+  }
+  copy_dev(rh.p, r.p, bytes, s);
   HIPTRY(hipMemsetAsync(p.p, 0, bytes, s));
   HIPTRY(hipMemsetAsync(v.p, 0, bytes, s));
-  double bn = std::sqrt(host_dot(t, s, part, hp, nf, b, b));
+  dots.run(s, nf, b, b, r.as<double>(), r.as<double>());
+  double bn = std::sqrt(dots.h[0]);
   *iters = 0;
-  if (!(bn > 0.0)) return bn == 0.0 ? DPF_OK : DPF_EBREAKDOWN;
-  double rho = 1.0, alpha = 1.0, omega = 1.0;
-  dim3 g(nblocks(nf)), bl(BLK);
+  if (!(bn > 0.0)) {
+    if (bn == 0.0) HIPTRY(hipMemsetAsync(x, 0, bytes, s));
+    return bn == 0.0 ? DPF_OK : DPF_EBREAKDOWN;
+  }
+  if (std::sqrt(dots.h[1]) <= tol * bn) return DPF_OK;
+  double rho = 1.0, alpha = 1.0, omega = 1.0, rho1 = dots.h[1];
   for (int it = 1; it <= maxit; it++) {
-    double rho1 = host_dot(t, s, part, hp, nf, rh.as<double>(), r.as<double>());
     if (rho1 == 0.0 || !std::isfinite(rho1)) return DPF_EBREAKDOWN;
     double beta = (rho1 / rho) * (alpha / omega);
     // p = r + beta (p - omega v)
@@ -2577,24 +2596,26 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
                        p.as<double>(), -beta * omega, (const double*)v.as<double>());
     prec((const double*)p.as<double>(), ph.as<double>());
     apply((const double*)ph.as<double>(), v.as<double>());
-    double rv = host_dot(t, s, part, hp, nf, rh.as<double>(), v.as<double>());
+    dots.run(s, nf, rh.as<double>(), v.as<double>());
+    double rv = dots.h[0];
     if (rv == 0.0 || !std::isfinite(rv)) return DPF_EBREAKDOWN;
     alpha = rho1 / rv;
-    // s = r - alpha v
+    // s = r - alpha v, then t = A M^-1 s (also at the last iteration: one
+    // synchronisation fewer per iteration)
     copy_dev(sv.p, r.p, bytes, s);
     hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(),
                        1.0, sv.as<double>(), 0.0, (const double*)nullptr);
-    double ss = host_dot(t, s, part, hp, nf, sv.as<double>(), sv.as<double>());
+    prec((const double*)sv.as<double>(), sh.as<double>());
+    apply((const double*)sh.as<double>(), tv.as<double>());
+    dots.run(s, nf, sv.as<double>(), sv.as<double>(), tv.as<double>(), sv.as<double>(),
+             tv.as<double>(), tv.as<double>());
+    double ss = dots.h[0], ts = dots.h[1], tt = dots.h[2];
     *iters = it;
     if (std::sqrt(ss) <= tol * bn) {
       hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, alpha, (const double*)ph.as<double>(),
                          1.0, x, 0.0, (const double*)nullptr);
       return DPF_OK;
     }
-    prec((const double*)sv.as<double>(), sh.as<double>());
-    apply((const double*)sh.as<double>(), tv.as<double>());
-    double ts = host_dot(t, s, part, hp, nf, tv.as<double>(), sv.as<double>());
-    double tt = host_dot(t, s, part, hp, nf, tv.as<double>(), tv.as<double>());
     if (tt == 0.0 || !std::isfinite(tt)) return DPF_EBREAKDOWN;
     omega = ts / tt;
     // x += alpha ph + omega sh ; r = s - omega t
@@ -2603,10 +2624,11 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
     copy_dev(r.p, sv.p, bytes, s);
     hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -omega, (const double*)tv.as<double>(),
                        1.0, r.as<double>(), 0.0, (const double*)nullptr);
-    double rr = host_dot(t, s, part, hp, nf, r.as<double>(), r.as<double>());
-    if (std::sqrt(rr) <= tol * bn) return DPF_OK;
+    dots.run(s, nf, r.as<double>(), r.as<double>(), rh.as<double>(), r.as<double>());
+    if (std::sqrt(dots.h[0]) <= tol * bn) return DPF_OK;
     if (omega == 0.0) return DPF_EBREAKDOWN;
     rho = rho1;
+    rho1 = dots.h[1];
   }
   return DPF_ENOCONV;
 }
@@ -3015,6 +3037,24 @@ void tr_rows(dpf_topology* t, hipStream_t s, const TrIn& in, const double* x, do
 
 }  // namespace
 
+// M = I - dt J on the free cells' fine pattern (and/or its transpose MT), its
+// diagonal dg and the free rows' constant part c0 (nullable).
+void tr_matrix(dpf_topology* t, hipStream_t s, const TrIn& in, double dt, double* M, double* MT,
+               double* dg, double* c0) {
+  tp_build_maps(t);
+  LevelDev& d0 = *t->lev[0];
+  int64_t nf = t->H.nf;
+  size_t bytes = d0.A.entries * sizeof(double);
+  if (M) HIPTRY(hipMemsetAsync(M, 0, bytes, s));
+  if (MT) HIPTRY(hipMemsetAsync(MT, 0, bytes, s));
+  hipLaunchKernelGGL(k_tr_matrix, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                     t->n2f.as<int32_t>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                     t->inc_other.as<int32_t>(), t->tp_mirror.as<int32_t>(),
+                     t->inc_slot.as<uint16_t>(), d0.diag_slot.as<uint16_t>(),
+                     d0.A.view<double>(M ? M : MT), t->aol.as<double>(), in, dt, M, MT, dg, c0);
+  HIPTRY(hipGetLastError());
+}
+
 extern "C" int dpf_transport_residuals(dpf_topology* t, const double* u, const double* vx,
                                        const double* vy, const double* vz, const double* Ds,
                                        const double* dirichletus, const double* Qs,
@@ -3069,6 +3109,11 @@ extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const doub
     return DPF_EINVAL;
   }
   if (int rc = twophase_checks(t)) return rc;
+  if (t->H.nf < t->H.n && (t1 - t0) / nsteps == 1.0) {
+    // a Dirichlet row of backward Euler is (1 - dt) u' = u - dt ud
+    set_err("a step of exactly 1 makes the Dirichlet rows of backward Euler singular");
+    return DPF_EINVAL;
+  }
   dpf_opts dflt;
   dpf_opts_init(&dflt);
   const dpf_opts* o = opts ? opts : &dflt;
@@ -3091,24 +3136,33 @@ extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const doub
     double* U = st->tr_u.as<double>();
     HIPTRY(hipMemcpyAsync(U, u0, n * sizeof(double),
                           host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
-    DBuf dg(pl), rhs(pl);
-    dg.alloc(n * sizeof(double));
-    rhs.alloc(n * sizeof(double));
-    hipLaunchKernelGGL(k_tr_diag, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
-                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                       t->inc_other.as<int32_t>(), t->aol.as<double>(), in, dt, dg.as<double>());
-    auto apply = [&](const double* xx, double* yy) { tr_rows(t, s, in, xx, 1.0, -dt, 0.0, yy); };
+    int64_t nf = t->H.nf;
+    LevelDev& d0 = *t->lev[0];
+    DBuf M(pl), dg(pl), c0(pl), rhs(pl), x(pl);
+    M.alloc(d0.A.entries * sizeof(double));
+    for (DBuf* d : {&dg, &c0, &rhs, &x}) d->alloc(std::max<int64_t>(nf, 1) * sizeof(double));
+    tr_matrix(t, s, in, dt, M.as<double>(), nullptr, dg.as<double>(), c0.as<double>());
+    SellD<double> Mv = d0.A.view<double>(M.as<double>());
+    auto apply = [&](const double* xx, double* yy) {
+      hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, Mv, xx, yy);
+    };
     auto prec = [&](const double* bb, double* zz) {
-      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(n)), dim3(BLK), 0, s, n, bb,
+      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, bb,
                          (const double*)dg.as<double>(), zz);
     };
     int status = DPF_OK, iters = 0;
     for (int k = 0; k < nsteps && status == DPF_OK; k++) {
       const double* uk = U + (int64_t)k * n;
-      tr_rows(t, s, in, uk, 1.0, 0.0, dt, rhs.as<double>());
+      double* un = U + (int64_t)(k + 1) * n;
+      // Dirichlet cells in place; the free cells' system starts from u_k
+      hipLaunchKernelGGL(k_tr_step_in, dim3(nblocks(n)), dim3(BLK), 0, s, n,
+                         t->n2f.as<int32_t>(), in.ud, (const double*)c0.as<double>(), dt, uk, un,
+                         rhs.as<double>(), x.as<double>());
       int it = 0;
-      status = bicgstab(t, s, n, apply, prec, (const double*)rhs.as<double>(),
-                        U + (int64_t)(k + 1) * n, std::max(o->reltol, 1e-14), o->maxiter, &it);
+      status = bicgstab(t, s, nf, apply, prec, (const double*)rhs.as<double>(), x.as<double>(),
+                        std::max(o->reltol, 1e-14), o->maxiter, &it, true);
+      hipLaunchKernelGGL(k_tr_step_out, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                         t->f2n.as<int32_t>(), (const double*)x.as<double>(), un);
       iters += it;
     }
     auto out = [&](double* dst, const double* src, int64_t count) {
@@ -3167,46 +3221,52 @@ extern "C" int dpf_transport_pullback(dpf_state* st, const double* dJ_du_final, 
     in.cz = in.cy + n;
     in.upwind = st->tr_upwind;
     double dt = st->dt;
-    DBuf lam(pl), mu(pl), dg(pl), gv(pl), gD(pl), gQ(pl);
-    lam.alloc(n * sizeof(double));
-    mu.alloc(n * sizeof(double));
-    dg.alloc(n * sizeof(double));
+    // (I - dt J)^T is block diagonal over free and Dirichlet cells, and the
+    // gradients read the adjoint at free cells only: the adjoint steps solve
+    // the free block's transpose
+    int64_t nf = t->H.nf;
+    LevelDev& d0 = *t->lev[0];
+    DBuf MT(pl), lamn(pl), lam(pl), mu(pl), dg(pl), gv(pl), gD(pl), gQ(pl);
+    MT.alloc(d0.A.entries * sizeof(double));
+    lamn.alloc(n * sizeof(double));
+    for (DBuf* d : {&lam, &mu, &dg}) d->alloc(std::max<int64_t>(nf, 1) * sizeof(double));
     gv.alloc(3 * ne * sizeof(double));
     gD.alloc(ne * sizeof(double));
     gQ.alloc(n * sizeof(double));
     HIPTRY(hipMemsetAsync(gv.p, 0, 3 * ne * sizeof(double), s));
     HIPTRY(hipMemsetAsync(gD.p, 0, ne * sizeof(double), s));
     HIPTRY(hipMemsetAsync(gQ.p, 0, n * sizeof(double), s));
-    HIPTRY(hipMemcpyAsync(lam.p, dJ_du_final, n * sizeof(double),
+    HIPTRY(hipMemcpyAsync(lamn.p, dJ_du_final, n * sizeof(double),
                           host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
-    // the transpose has the same diagonal
-    hipLaunchKernelGGL(k_tr_diag, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
-                       t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                       t->inc_other.as<int32_t>(), t->aol.as<double>(), in, dt, dg.as<double>());
+    tr_matrix(t, s, in, dt, nullptr, MT.as<double>(), dg.as<double>(), nullptr);
+    SellD<double> MTv = d0.A.view<double>(MT.as<double>());
+    hipLaunchKernelGGL(k_tr_gather, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
+                       (const double*)lamn.as<double>(), lam.as<double>());
     auto apply = [&](const double* xx, double* yy) {
-      hipLaunchKernelGGL(k_tr_rows_t, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
-                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                         t->inc_other.as<int32_t>(), t->aol.as<double>(), in, xx, dt, yy);
+      hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, MTv, xx, yy);
     };
     auto prec = [&](const double* bb, double* zz) {
-      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(n)), dim3(BLK), 0, s, n, bb,
+      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, bb,
                          (const double*)dg.as<double>(), zz);
     };
     int status = DPF_OK, iters = 0;
     double* gvx = gv.as<double>();
+    // each adjoint step starts from the previous one (mu holds lam)
+    copy_dev(mu.p, lam.p, nf * sizeof(double), s);
     for (int k = st->nsteps; k >= 1 && status == DPF_OK; k--) {
       int it = 0;
-      status = bicgstab(t, s, n, apply, prec, (const double*)lam.as<double>(), mu.as<double>(),
-                        std::max(o->reltol, 1e-14), o->maxiter, &it);
+      status = bicgstab(t, s, nf, apply, prec, (const double*)lam.as<double>(), mu.as<double>(),
+                        std::max(o->reltol, 1e-14), o->maxiter, &it, true);
       iters += it;
       const double* uk = st->tr_u.as<double>() + (int64_t)k * n;
       hipLaunchKernelGGL(k_tr_grad_edges, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
                          t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), in, uk,
                          (const double*)mu.as<double>(), dt, gvx, gvx + ne, gvx + 2 * ne,
                          gD.as<double>());
-      hipLaunchKernelGGL(k_tr_grad_q, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
-                         in, (const double*)mu.as<double>(), dt, gQ.as<double>());
-      copy_dev(lam.p, mu.p, n * sizeof(double), s);
+      hipLaunchKernelGGL(k_tr_grad_q, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                         t->f2n.as<int32_t>(), in, (const double*)mu.as<double>(), dt,
+                         gQ.as<double>());
+      copy_dev(lam.p, mu.p, nf * sizeof(double), s);
     }
     HIPTRY(hipGetLastError());
     auto out = [&](double* dst, const double* src, int64_t count) {

@@ -154,7 +154,8 @@ __global__ void __launch_bounds__(BLK) k_rich_residual(
 
 // The Newton matrix M = -dr/dpsi on the fine pattern (row j = free cell i),
 // and, if MT is given, its transpose there (entry (k, j) at row k's slot for
-// the same edge).  The arrays are zeroed by the caller.  M is the Picard (groundwater)
+// the same edge).  The arrays are zeroed by the caller; parallel edges add
+// into one slot (only thread j writes row j of M and column j of MT).  M is the Picard (groundwater)
 // operator plus the kr-slope terms: M_jj = sum (H - Hi kr_i' dlt_io) K aol / sv_i,
 // M_jk = -(Ho kr_o' dlt_io + H) K aol / sv_i, dlt_io = (psi + z)_o - (psi + z)_i.
 __global__ void k_rich_newton_matrix(
@@ -183,11 +184,11 @@ __global__ void k_rich_newton_matrix(
     int32_t fo = n2f[inc_other[p]];
     if (fo >= 0) {
       double v = oth / s;
-      M[base + (int64_t)inc_slot[p] * 64] = v;
+      M[base + (int64_t)inc_slot[p] * 64] += v;
       if (MT) {
         // row fo differs from lane to lane: its slice pointer is a vector load
         int64_t bo = A.sptr[fo >> 6] + (fo & 63);
-        MT[bo + (int64_t)inc_slot[mirror[p]] * 64] = v;
+        MT[bo + (int64_t)inc_slot[mirror[p]] * 64] += v;
       }
     }
   }
@@ -275,14 +276,41 @@ __global__ void k_rich_pre(int64_t n, const double* __restrict__ b,
   x[i] = v;
 }
 
-// Per-block partial sums of x.y (fixed order) into part[block].
-__global__ void __launch_bounds__(BLK) k_rich_dot(int64_t n, const double* __restrict__ x,
-                                                  const double* __restrict__ y,
-                                                  double* __restrict__ part) {
+// Up to three dot products in one pass: part[3 * block + k] = block sum of
+// x_k . y_k (null x_k: 0).
+__global__ void __launch_bounds__(BLK) k_dots3(int64_t n, const double* __restrict__ x0,
+                                               const double* __restrict__ y0,
+                                               const double* __restrict__ x1,
+                                               const double* __restrict__ y1,
+                                               const double* __restrict__ x2,
+                                               const double* __restrict__ y2,
+                                               double* __restrict__ part) {
   __shared__ double sh[4];
   int64_t i = logical_row();
-  double s = block_sum(i < n ? x[i] * y[i] : 0.0, sh);
-  if (threadIdx.x == 0) part[i / BLK] = s;
+  bool ok = i < n;
+  double a = ok && x0 ? x0[i] * y0[i] : 0.0;
+  double b = ok && x1 ? x1[i] * y1[i] : 0.0;
+  double c = ok && x2 ? x2[i] * y2[i] : 0.0;
+  int64_t lb = i / BLK;
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) part[3 * lb] = a;
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0) part[3 * lb + 1] = b;
+  c = block_sum(c, sh);
+  if (threadIdx.x == 0) part[3 * lb + 2] = c;
+}
+
+// out[k] = sum over the nb blocks of part[3 b + k], k < 3, in a fixed order
+// (one block: strided per-thread sums, then the block tree).
+__global__ void __launch_bounds__(BLK) k_dots3_fin(int64_t nb, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[4];
+  for (int k = 0; k < 3; k++) {
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += BLK) s += part[3 * b + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[k] = s;
+  }
 }
 
 // out = a * b elementwise

@@ -64,56 +64,91 @@ __global__ void k_tr_rows(int64_t n, const int32_t* __restrict__ n2f,
   y[c] = alpha * xc + beta * jx + gamma * r0;
 }
 
-// y = (I - dt J)^T x: y_j = x_j - dt [J_jj x_j + sum over free neighbours i
-// of J_ij x_i] (J_ij from i's edge term; a Dirichlet column has only J_jj = 1).
-__global__ void k_tr_rows_t(int64_t n, const int32_t* __restrict__ n2f,
-                            const int64_t* __restrict__ inc_ptr,
+// Backward Euler's matrix M = I - dt J restricted to the free cells, on the
+// fine pattern (row j = free cell i = f2n[j]), built once per solve since J is
+// constant: M_jj = 1 - dt sum (adv - dif), M_jk = -dt (dif - adv) for a free
+// neighbour k (parallel edges add into one slot).  MT (nullable) receives the
+// transpose (entry (k, j) at row k's slot for the same edge; only thread j
+// writes column j, so the adds do not race), dg the diagonal, c0 (nullable)
+// the constant part of the free rows of r: Q_i / V_i + sum over Dirichlet
+// neighbours o of (dif - adv) ud_o.  A Dirichlet row of J is the identity and
+// no free row depends on a Dirichlet cell's state, so the free block is the
+// whole coupled system.  M and MT are zeroed by the caller.
+__global__ void k_tr_matrix(int64_t nf, const int32_t* __restrict__ f2n,
+                            const int32_t* __restrict__ n2f, const int64_t* __restrict__ inc_ptr,
                             const int32_t* __restrict__ inc_edge,
                             const int32_t* __restrict__ inc_other,
-                            const double* __restrict__ aol, TrIn in,
-                            const double* __restrict__ x, double dt, double* __restrict__ y) {
+                            const int32_t* __restrict__ mirror,
+                            const uint16_t* __restrict__ inc_slot,
+                            const uint16_t* __restrict__ diag_slot, SellD<double> A,
+                            const double* __restrict__ aol, TrIn in, double dt,
+                            double* __restrict__ M, double* __restrict__ MT,
+                            double* __restrict__ dg, double* __restrict__ c0) {
   int64_t j = logical_row();
-  if (j >= n) return;
-  double xj = x[j];
-  bool jf = n2f[j] >= 0;
-  double s = 0.0;
-  if (!jf) {
-    s = xj;  // Dirichlet row of J: identity, and no free row couples to j
-  } else {
-    for (int64_t p = inc_ptr[j]; p < inc_ptr[j + 1]; p++) {
-      int32_t e = inc_edge[p], i = inc_other[p];
-      double adv, dif, w;
-      tr_edge(in, aol, e, (int32_t)j, i, adv, dif, w);
-      s += (adv - dif) * xj;  // J_jj contributions of j's own row
-      if (n2f[i] >= 0) {
-        double adv2, dif2, w2;
-        tr_edge(in, aol, e, i, (int32_t)j, adv2, dif2, w2);
-        s += (-adv2 + dif2) * x[i];  // J_ij, i's row
+  if (j >= nf) return;
+  int32_t i = f2n[j];
+  int64_t base;
+  int w;
+  sell_row(A, j, base, w);
+  double jd = 0.0, r0 = in.Q[i] / in.V[i];
+  for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
+    int32_t o = inc_other[p];
+    double adv, dif, wv;
+    tr_edge(in, aol, inc_edge[p], i, o, adv, dif, wv);
+    jd += adv - dif;
+    int32_t fo = n2f[o];
+    if (fo >= 0) {
+      double v = -dt * (dif - adv);
+      if (M) M[base + (int64_t)inc_slot[p] * 64] += v;
+      if (MT) {
+        // row fo differs from lane to lane: its slice pointer is a vector load
+        int64_t bo = A.sptr[fo >> 6] + (fo & 63);
+        MT[bo + (int64_t)inc_slot[mirror[p]] * 64] += v;
       }
+    } else {
+      r0 += (dif - adv) * in.ud[o];
     }
   }
-  y[j] = xj - dt * s;
+  double d = 1.0 - dt * jd;
+  int64_t dpos = base + (int64_t)diag_slot[j] * 64;
+  if (M) M[dpos] = d;
+  if (MT) MT[dpos] = d;
+  dg[j] = d;
+  if (c0) c0[j] = r0;
 }
 
-// diag of I - dt J (the Jacobi preconditioner of the backward-Euler solves).
-__global__ void k_tr_diag(int64_t n, const int32_t* __restrict__ n2f,
-                          const int64_t* __restrict__ inc_ptr,
-                          const int32_t* __restrict__ inc_edge,
-                          const int32_t* __restrict__ inc_other, const double* __restrict__ aol,
-                          TrIn in, double dt, double* __restrict__ diag) {
+// One backward-Euler step's set-up over all cells: a free cell c = f2n[j]
+// gets rhs_j = u_c + dt c0_j and the initial guess x_j = u_c; a Dirichlet
+// cell, whose row (1 - dt) u' = u - dt ud is decoupled, is solved in place
+// into unext.
+__global__ void k_tr_step_in(int64_t n, const int32_t* __restrict__ n2f,
+                             const double* __restrict__ ud, const double* __restrict__ c0,
+                             double dt, const double* __restrict__ u, double* __restrict__ unext,
+                             double* __restrict__ rhs, double* __restrict__ x) {
   int64_t c = logical_row();
   if (c >= n) return;
-  double jcc = 0.0;
-  if (n2f[c] < 0) {
-    jcc = 1.0;
-  } else {
-    for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
-      double adv, dif, w;
-      tr_edge(in, aol, inc_edge[p], (int32_t)c, inc_other[p], adv, dif, w);
-      jcc += adv - dif;
-    }
+  int32_t f = n2f[c];
+  double uc = u[c];
+  if (f < 0) {
+    unext[c] = (uc - dt * ud[c]) / (1.0 - dt);
+    return;
   }
-  diag[c] = 1.0 - dt * jcc;
+  rhs[f] = uc + dt * c0[f];
+  x[f] = uc;
+}
+
+// unext[f2n[j]] = x[j] (free cells).
+__global__ void k_tr_step_out(int64_t nf, const int32_t* __restrict__ f2n,
+                              const double* __restrict__ x, double* __restrict__ unext) {
+  int64_t j = logical_row();
+  if (j < nf) unext[f2n[j]] = x[j];
+}
+
+// x[j] = src[f2n[j]] (free cells).
+__global__ void k_tr_gather(int64_t nf, const int32_t* __restrict__ f2n,
+                            const double* __restrict__ src, double* __restrict__ x) {
+  int64_t j = logical_row();
+  if (j < nf) x[j] = src[f2n[j]];
 }
 
 // z = x / d
@@ -123,9 +158,9 @@ __global__ void k_tr_jacobi(int64_t n, const double* __restrict__ x,
   if (i < n) z[i] = x[i] / d[i];
 }
 
-// Adjoint accumulation of one backward-Euler step with adjoint mu and state u
-// (dt dr/dp^T mu): per edge the velocity components and D, over both
-// directions of the edge whose first cell is free.
+// Adjoint accumulation of one backward-Euler step with adjoint mu (per free
+// cell) and state u (dt dr/dp^T mu): per edge the velocity components and D,
+// over both directions of the edge whose first cell is free.
 __global__ void k_tr_grad_edges(int64_t ne, const int2* __restrict__ edges,
                                 const int32_t* __restrict__ n2f, const double* __restrict__ aol,
                                 TrIn in, const double* __restrict__ u,
@@ -139,11 +174,12 @@ __global__ void k_tr_grad_edges(int64_t ne, const int2* __restrict__ edges,
   double gx = 0.0, gy = 0.0, gz = 0.0, gd = 0.0;
   for (int dir = 0; dir < 2; dir++) {
     int32_t c = dir ? ab.y : ab.x, o = dir ? ab.x : ab.y;
-    if (n2f[c] < 0) continue;
+    int32_t fc = n2f[c];
+    if (fc < 0) continue;
     double adv, dif, w;
     tr_edge(in, aol, (int32_t)e, c, o, adv, dif, w);
     double uo = n2f[o] >= 0 ? u[o] : in.ud[o];
-    double s = aol[e] / in.V[c] * mu[c] * dt;
+    double s = aol[e] / in.V[c] * mu[fc] * dt;
     if ((w > 0.0) == (in.upwind != 0)) {
       double g = -(u[c] - uo) * s;
       gx += g * (in.cx[c] - in.cx[o]);
@@ -158,11 +194,13 @@ __global__ void k_tr_grad_edges(int64_t ne, const int2* __restrict__ edges,
   if (dD) dD[e] += gd;
 }
 
-// dQ_c += dt mu_c / V_c at free cells
-__global__ void k_tr_grad_q(int64_t n, const int32_t* __restrict__ n2f, TrIn in,
+// dQ_c += dt mu_j / V_c at free cells c = f2n[j]
+__global__ void k_tr_grad_q(int64_t nf, const int32_t* __restrict__ f2n, TrIn in,
                             const double* __restrict__ mu, double dt, double* __restrict__ dQ) {
-  int64_t c = logical_row();
-  if (c < n && n2f[c] >= 0) dQ[c] += dt * mu[c] / in.V[c];
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  int32_t c = f2n[j];
+  dQ[c] += dt * mu[j] / in.V[c];
 }
 
 }  // namespace dpf

@@ -107,3 +107,29 @@ def test_errors(pkg, rich):
         rich.richards_steadystate(np.zeros(n), Ks, nb, aol, dn, dpsi, coords, alphas, Ns, Qs)
     with pytest.raises(ValueError):
         rich.richards_steadystate(np.zeros(n), Ks, nb, aol, dn, dpsi, coords[:2], alphas, Ns, Qs)
+
+
+def test_parallel_edges(pkg, rich, ro):
+    """Edges listed twice (reversed) with their aol split in half give the
+    same heads; both copies add into one slot of the Newton matrix and of its
+    transpose."""
+    _say("parallel edges")
+    coords, nb, aol, n, Ks, Qs, dn, dpsi, alphas, Ns = richards_box(pkg, ns=(10, 8, 5), seed=9)
+    dpsi[n - 1] = -0.2
+    rng = np.random.default_rng(10)
+    ne = nb.shape[0]
+    dup = rng.choice(ne, ne // 3, replace=False)
+    nb2 = np.vstack([nb, nb[dup][:, ::-1]])
+    aol2 = np.concatenate([aol[:ne], aol[dup] * 0.5])
+    aol2[dup] *= 0.5
+    ext = lambda a: np.concatenate([a, a[dup]])  # noqa: E731
+    args2 = (ext(Ks), nb2, aol2, dn, dpsi, coords, ext(alphas), ext(Ns), Qs)
+    psi1 = rich.richards_steadystate(np.zeros(n), Ks, nb, aol, dn, dpsi, coords, alphas, Ns, Qs,
+                                     reltol=1e-12)
+    psi2, pb = rich.rrule_richards_steadystate(np.zeros(n), *args2, reltol=1e-12)
+    pso, pbo = ro.richards_rrule(np.zeros(n), *args2)
+    assert rel(psi2, pso) < 1e-9 and rel(psi2, psi1) < 1e-9 and psi2.min() < -0.1
+    w = rng.normal(size=n)
+    g, go = pb(w), pbo(w)
+    assert rel(g[2], go["Ks"]) < 1e-7
+    assert rel(g[10], go["Qs"]) < 1e-7

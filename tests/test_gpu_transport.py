@@ -94,3 +94,31 @@ def test_reference_column(pkg, tr):
         a[1] = vp
         fd = (tr.transport_steps(*a, reltol=1e-12)[node] - u1[node]) / 1e-8
         assert np.isclose(grad[i], fd, atol=1e-4, rtol=1e-3)
+
+
+def test_parallel_edges_and_unit_step(pkg, tr, to):
+    """Edges listed twice with their aol split in half give the same system
+    (both copies add into one matrix slot); a step of exactly 1 is refused
+    when there are Dirichlet cells (their backward-Euler row is singular)."""
+    _say("parallel edges")
+    coords, nb, aol, vol, n, u0, vx, vy, vz, Ds, dn, ud, Q, rng = _box2d(pkg, seed=5)
+    ne = nb.shape[0]
+    dup = rng.choice(ne, ne // 3, replace=False)
+    nb2 = np.vstack([nb, nb[dup][:, ::-1]])
+    aol2 = np.concatenate([aol[:ne], aol[dup] * 0.5])
+    aol2[dup] *= 0.5
+    ext = lambda a: np.concatenate([a, a[dup]])  # noqa: E731
+    args2 = (ext(vx), ext(vy), ext(vz), ext(Ds), nb2, aol2, dn, ud, Q, vol, coords, 0.0, 0.7, 20)
+    u2 = tr.transport_steps(u0, *args2, reltol=1e-13)
+    uo2 = to.transport_steps(u0, *args2)
+    u1 = tr.transport_steps(u0, vx, vy, vz, Ds, nb, aol, dn, ud, Q, vol, coords, 0.0, 0.7, 20,
+                            reltol=1e-13)
+    assert rel(u2, uo2) < 1e-11 and rel(u2, u1) < 1e-11
+    g = rng.normal(size=n)
+    _, pb = tr.rrule_transport_steps(u0, *args2, reltol=1e-13)
+    _, go = to.transport_adjoint(u0, *args2, g)
+    gr = pb(g)
+    for k in ("vxs", "vys", "Ds", "Qs"):
+        assert rel(gr[k], go[k]) < 1e-9, k
+    with pytest.raises(Exception, match="singular"):
+        tr.transport_steps(u0, vx, vy, vz, Ds, nb, aol, dn, ud, Q, vol, coords, 0.0, 3.0, 3)
