@@ -2537,16 +2537,21 @@ void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, 
 struct Dots3 {
   DBuf part, out;
   double h[3];
+  unsigned grid;
   Dots3(BufPool* p, int64_t n) : part(p), out(p) {
-    part.alloc(3 * std::max<int64_t>(nblocks(n), 1) * sizeof(double));
+    grid = std::max<unsigned>(1, std::min<unsigned>(nblocks(n), DOT_GRID));
+    part.alloc(3 * (size_t)grid * sizeof(double));
     out.alloc(3 * sizeof(double));
   }
   void run(hipStream_t s, int64_t n, const double* x0, const double* y0, const double* x1 = nullptr,
            const double* y1 = nullptr, const double* x2 = nullptr, const double* y2 = nullptr) {
-    unsigned nb = nblocks(n);
-    hipLaunchKernelGGL(k_dots3, dim3(nb), dim3(BLK), 0, s, n, x0, y0, x1, y1, x2, y2,
+    hipLaunchKernelGGL(k_dots3, dim3(grid), dim3(BLK), 0, s, n, x0, y0, x1, y1, x2, y2,
                        part.as<double>());
-    hipLaunchKernelGGL(k_dots3_fin, dim3(1), dim3(BLK), 0, s, (int64_t)nb,
+    finish(s);
+  }
+  // sums the partials a kernel of `grid` blocks left in part
+  void finish(hipStream_t s) {
+    hipLaunchKernelGGL(k_dots3_fin, dim3(1), dim3(BLK), 0, s, (int64_t)grid,
                        (const double*)part.as<double>(), out.as<double>());
     HIPTRY(hipMemcpyAsync(h, out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
@@ -2556,8 +2561,9 @@ struct Dots3 {
 // Right-preconditioned BiCGStab for A x = b (apply(x, y): y = A x;
 // prec(b, z): z ~ A^-1 b), vectors of length nf, x from 0 (guess: from the x
 // passed in), to ||b - A x|| <= tol ||b||.  Three synchronisations per
-// iteration: r^.v; s.s, t.s and t.t together; r.r and r^.r together.
-// Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN; *iters counts the iterations.
+// iteration: r^.v; s.s, t.s and t.t together; r.r and r^.r together (fused
+// with the x and r updates).  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN;
+// *iters counts the iterations.
 template <class Apply, class Prec>
 int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec, const double* b,
              double* x, double tol, int maxit, int* iters, bool guess = false) {
@@ -2602,9 +2608,8 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
     alpha = rho1 / rv;
     // s = r - alpha v, then t = A M^-1 s (also at the last iteration: one
     // synchronisation fewer per iteration)
-    copy_dev(sv.p, r.p, bytes, s);
-    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(),
-                       1.0, sv.as<double>(), 0.0, (const double*)nullptr);
+    hipLaunchKernelGGL(k_lin2, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(), 1.0,
+                       (const double*)r.as<double>(), sv.as<double>());
     prec((const double*)sv.as<double>(), sh.as<double>());
     apply((const double*)sh.as<double>(), tv.as<double>());
     dots.run(s, nf, sv.as<double>(), sv.as<double>(), tv.as<double>(), sv.as<double>(),
@@ -2618,13 +2623,13 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
     }
     if (tt == 0.0 || !std::isfinite(tt)) return DPF_EBREAKDOWN;
     omega = ts / tt;
-    // x += alpha ph + omega sh ; r = s - omega t
-    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, alpha, (const double*)ph.as<double>(), 1.0,
-                       x, omega, (const double*)sh.as<double>());
-    copy_dev(r.p, sv.p, bytes, s);
-    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, -omega, (const double*)tv.as<double>(),
-                       1.0, r.as<double>(), 0.0, (const double*)nullptr);
-    dots.run(s, nf, r.as<double>(), r.as<double>(), rh.as<double>(), r.as<double>());
+    // x += alpha ph + omega sh ; r = s - omega t ; r.r and r^.r
+    hipLaunchKernelGGL(k_bicg_xr, dim3(dots.grid), bl, 0, s, nf, alpha,
+                       (const double*)ph.as<double>(), omega, (const double*)sh.as<double>(),
+                       (const double*)sv.as<double>(), (const double*)tv.as<double>(),
+                       (const double*)rh.as<double>(), x, r.as<double>(),
+                       dots.part.as<double>());
+    dots.finish(s);
     if (std::sqrt(dots.h[0]) <= tol * bn) return DPF_OK;
     if (omega == 0.0) return DPF_EBREAKDOWN;
     rho = rho1;
@@ -2633,8 +2638,6 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
   return DPF_ENOCONV;
 }
 
-// BiCGStab on a Newton matrix of the fine layout, preconditioned by the
-// V-cycle of the current (Picard) hierarchy.
 int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
                   SellD<double> M, const double* b, double* x, double tol, int maxit,
                   int* iters) {

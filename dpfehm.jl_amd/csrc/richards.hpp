@@ -276,8 +276,12 @@ __global__ void k_rich_pre(int64_t n, const double* __restrict__ b,
   x[i] = v;
 }
 
+// Fixed grid of the reduction kernels below: a thread's grid-stride
+// sequence, and so every sum, is the same on every run.
+constexpr int DOT_GRID = 1024;
+
 // Up to three dot products in one pass: part[3 * block + k] = block sum of
-// x_k . y_k (null x_k: 0).
+// x_k . y_k (null x_k: 0), grid-stride over a grid of at most DOT_GRID blocks.
 __global__ void __launch_bounds__(BLK) k_dots3(int64_t n, const double* __restrict__ x0,
                                                const double* __restrict__ y0,
                                                const double* __restrict__ x1,
@@ -286,18 +290,49 @@ __global__ void __launch_bounds__(BLK) k_dots3(int64_t n, const double* __restri
                                                const double* __restrict__ y2,
                                                double* __restrict__ part) {
   __shared__ double sh[4];
-  int64_t i = logical_row();
-  bool ok = i < n;
-  double a = ok && x0 ? x0[i] * y0[i] : 0.0;
-  double b = ok && x1 ? x1[i] * y1[i] : 0.0;
-  double c = ok && x2 ? x2[i] * y2[i] : 0.0;
-  int64_t lb = i / BLK;
+  double a = 0.0, b = 0.0, c = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLK;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += stride) {
+    a += x0[i] * y0[i];
+    if (x1) b += x1[i] * y1[i];
+    if (x2) c += x2[i] * y2[i];
+  }
   a = block_sum(a, sh);
-  if (threadIdx.x == 0) part[3 * lb] = a;
+  if (threadIdx.x == 0) part[3 * blockIdx.x] = a;
   b = block_sum(b, sh);
-  if (threadIdx.x == 0) part[3 * lb + 1] = b;
+  if (threadIdx.x == 0) part[3 * blockIdx.x + 1] = b;
   c = block_sum(c, sh);
-  if (threadIdx.x == 0) part[3 * lb + 2] = c;
+  if (threadIdx.x == 0) part[3 * blockIdx.x + 2] = c;
+}
+
+// BiCGStab's closing update of one iteration in one pass:
+// x += a ph + w sh; r = sv - w tv; and the partial sums of r.r and rh.r
+// (part as k_dots3's, third sum 0).
+__global__ void __launch_bounds__(BLK) k_bicg_xr(int64_t n, double a,
+                                                 const double* __restrict__ ph, double w,
+                                                 const double* __restrict__ shv,
+                                                 const double* __restrict__ sv,
+                                                 const double* __restrict__ tv,
+                                                 const double* __restrict__ rh,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[4];
+  double rr = 0.0, hr = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLK;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] + a * ph[i] + w * shv[i];
+    double ri = sv[i] - w * tv[i];
+    r[i] = ri;
+    rr += ri * ri;
+    hr += rh[i] * ri;
+  }
+  rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) part[3 * blockIdx.x] = rr;
+  hr = block_sum(hr, sh);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x + 1] = hr;
+    part[3 * blockIdx.x + 2] = 0.0;
+  }
 }
 
 // out[k] = sum over the nb blocks of part[3 b + k], k < 3, in a fixed order
@@ -311,6 +346,13 @@ __global__ void __launch_bounds__(BLK) k_dots3_fin(int64_t nb, const double* __r
     s = block_sum(s, sh);
     if (threadIdx.x == 0) out[k] = s;
   }
+}
+
+// out = a x + b y
+__global__ void k_lin2(int64_t n, double a, const double* __restrict__ x, double b,
+                       const double* __restrict__ y, double* __restrict__ out) {
+  int64_t i = logical_row();
+  if (i < n) out[i] = a * x[i] + b * y[i];
 }
 
 // out = a * b elementwise
