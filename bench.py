@@ -51,6 +51,11 @@ def main():
                          "checkpointed adjoint; twophase: IMPES steps on the 2M grid; "
                          "richards: steady Richards solve + gradient on the 2M grid; "
                          "transport: advection-dispersion steps + gradient on the 2M grid")
+    ap.add_argument("--tuning", default=None, metavar="FINE,COARSE,RATIO",
+                    help="steady2m: fix the Chebyshev degrees and interval ratio of every "
+                         "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
+                         "run's configuration under a profiler, whose overheads change the "
+                         "autotuner's timings)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -85,6 +90,10 @@ def main():
     log(f"[bench] problem n={w.n} E={w.nedges} built in {time.time() - t0:.1f}s")
     t0 = time.time()
     topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    if args.tuning:
+        f, c, r = args.tuning.split(",")
+        for prec in ("fp64", "mixed", "mixed_pc"):
+            topo.set_tuning(prec, int(f), int(c), float(r))
     topo_s = time.time() - t0
     log(f"[bench] topology: nfree={topo.nfree} nnz={topo.nnz} levels={topo.levels} "
         f"coarse={topo.coarse_rows} opcx={topo.operator_complexity:.2f} "
@@ -265,6 +274,7 @@ def main():
                                     results["fp64"]["stats"][0][1]["iterations"]],
         "topology_setup_s": round(topo_s, 3),
         "cheb_tuned_finest_coarse_ratio": tuned,
+        "cheb_pinned": bool(args.tuning),
         "roofline": {
             "bound": "hbm",
             "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",

@@ -75,6 +75,7 @@ SIGNATURES = {
     "dpf_member_stats": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_info": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_tuning": (ctypes.c_int, [_p, _i32, _p, _p, _i32]),
+    "dpf_topology_set_tuning": (ctypes.c_int, [_p, _i32, _i32, _i32, ctypes.c_double]),
     "dpf_twophase_tpfa": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "dpf_twophase_upstream": (ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p, _p,
                                              _p]),

@@ -142,6 +142,12 @@ class Topology:
                                                   _ptr(ms), _ptr(its), 7))
         return [(float(a), int(b)) for a, b in zip(ms, its)] if its.any() else []
 
+    def set_tuning(self, precision, fine, coarse, ratio):
+        """Fix the Chebyshev degrees (finest level, coarser levels) and the
+        interval ratio of ``precision`` instead of autotuning them."""
+        _lib.check(_lib.lib().dpf_topology_set_tuning(self._h, _lib.precision_code(precision),
+                                                      int(fine), int(coarse), float(ratio)))
+
     @property
     def handle(self):
         return self._h

@@ -1048,6 +1048,25 @@ int dpf_topology_tuning(const dpf_topology* t, int32_t precision, double* ms, in
   return DPF_OK;
 }
 
+int dpf_topology_set_tuning(dpf_topology* t, int32_t precision, int32_t deg_fine,
+                            int32_t deg_coarse, double ratio) {
+  if (!t || precision < DPF_FP64 || precision > DPF_MIXED_PC || deg_fine < 1 || deg_fine > 16 ||
+      deg_coarse < 1 || deg_coarse > 16 || !(ratio > 1.0) || !std::isfinite(ratio)) {
+    set_err("need a precision code, degrees in 1..16 and an interval ratio > 1");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  t->deg_fine[precision] = deg_fine;
+  t->deg_coarse[precision] = deg_coarse;
+  t->ratio_tuned[precision] = ratio;
+  for (int k = 0; k < 7; k++) {
+    t->tune_ms[precision][k] = 0.0;
+    t->tune_iters[precision][k] = 0;
+  }
+  t->tuned[precision] = true;
+  return DPF_OK;
+}
+
 int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   if (!t || !info) {
     set_err("NULL argument");

@@ -135,6 +135,14 @@ int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 int dpf_topology_tuning(const dpf_topology *topo, int32_t precision, double *ms,
                         int32_t *iters, int32_t n);
 
+/* Fixes the smoother of one precision on this topology, in place of the
+ * autotuner (which then does not run for it): Chebyshev degree deg_fine on
+ * the finest level and deg_coarse below (1..16), interval [lmax/ratio, lmax],
+ * ratio > 1.  For runs that must not depend on timing, e.g. to repeat a tuned
+ * configuration under a profiler.  No counterpart in the reference. */
+int dpf_topology_set_tuning(dpf_topology *topo, int32_t precision, int32_t deg_fine,
+                            int32_t deg_coarse, double ratio);
+
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs, specificstorage, volumes), src/groundwater.jl:1-38.
  *   hfree: nfree;  Ks: nedges;  dirichleths, Qs, specificstorage, volumes: n;

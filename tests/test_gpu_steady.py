@@ -426,3 +426,35 @@ def test_uge_mesh_geometric_dirichlet_faces(pkg, api, oracle, precision, tmp_pat
     delta[free[::50]] = 1.0
     g, go = pb(delta), pbo(delta)
     assert rel(g[1], go[1]) < 1e-8 and rel(g[5], go[5]) < 1e-8
+
+
+def test_pinned_tuning(pkg, api):
+    """dpf_topology_set_tuning fixes the smoother instead of the autotuner:
+    the topology reports the pinned degrees and no trials, two topologies
+    pinned alike give bitwise-equal heads, and the heads agree with an
+    autotuned solve to the solve tolerance."""
+    import ctypes
+    _say("pinned tuning")
+    from dpfehm_jl_amd import _lib
+    w = pkg.workloads.steady_2m(ns=(24, 20, 10), maxs=(12.0, 10.0, 5.0))
+    K = w.Ks()
+    hs = []
+    for _ in range(2):
+        topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+        topo.set_tuning("fp64", 2, 1, 8.0)
+        h = np.empty(w.n)
+        o = api._opts(dict(reltol=1e-12))
+        _lib.check(_lib.lib().dpf_steady_forward(
+            topo.handle, api._ptr(api._f64(K)), api._ptr(api._f64(w.dirichleths)),
+            api._ptr(api._f64(w.Qs)), api._ptr(h), ctypes.byref(o), None, None))
+        assert topo.tuned_degrees()["fp64"] == (2, 1, 8) and topo.tuning("fp64") == []
+        hs.append(h)
+        with pytest.raises(_lib.DpfError):
+            topo.set_tuning("fp64", 0, 1, 8.0)
+        with pytest.raises(_lib.DpfError):
+            topo.set_tuning("mixed", 1, 2, 1.0)
+        topo.close()
+    assert np.array_equal(hs[0], hs[1])
+    h_auto = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                         w.dirichleths, w.Qs, reltol=1e-12)
+    assert rel(hs[0], h_auto) < 1e-10
