@@ -51,6 +51,9 @@ def main():
                          "checkpointed adjoint; twophase: IMPES steps on the 2M grid; "
                          "richards: steady Richards solve + gradient on the 2M grid; "
                          "transport: advection-dispersion steps + gradient on the 2M grid")
+    ap.add_argument("--sigma", type=float, default=1.0,
+                    help="steady2m: standard deviation of the logK fields (the conditioning "
+                         "sweep runs 1, 2 and 3; the headline line is sigma 1)")
     ap.add_argument("--tuning", default=None, metavar="FINE,COARSE,RATIO",
                     help="steady2m: fix the Chebyshev degrees and interval ratio of every "
                          "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
@@ -86,7 +89,7 @@ def main():
     ns = tuple(args.nx)
     maxs = (100.0 * (ns[0] - 1) / 199.0, 100.0 * (ns[1] - 1) / 199.0, 25.0 * (ns[2] - 1) / 49.0)
     t0 = time.time()
-    w = P.workloads.steady_2m(ns=ns, maxs=maxs)
+    w = P.workloads.steady_2m(sigma=args.sigma, ns=ns, maxs=maxs)
     log(f"[bench] problem n={w.n} E={w.nedges} built in {time.time() - t0:.1f}s")
     t0 = time.time()
     topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
@@ -103,7 +106,8 @@ def main():
     sptr = ctypes.c_void_p(stream.cuda_stream)
     L = _lib.lib()
     sp = grid_sp = P.grid.grid_spacing(w.mins, w.maxs, w.ns)
-    fields = [w.logK] + [P.fields.layered_logK(w.ns, sp, -9.0, 50.0, 1.0, s) for s in (10, 11, 12)]
+    fields = [w.logK] + [P.fields.layered_logK(w.ns, sp, -9.0, 50.0, args.sigma, s)
+                         for s in (10, 11, 12)]
     logK_d = [torch.from_numpy(f).to(dev) for f in fields]
     dh_d = torch.from_numpy(w.dirichleths).to(dev)
     Q_d = torch.from_numpy(w.Qs).to(dev)
@@ -257,7 +261,8 @@ def main():
         "config": {
             "workload": f"steady groundwater forward+adjoint, {ns[0]}x{ns[1]}x{ns[2]} cells "
                         f"({w.n} cells, {w.nedges} edges), harmonic-mean K from layered "
-                        "Matern(nu=1, l=50 m, sigma=1) logK, top-face Dirichlet, 4 injectors, "
+                        f"Matern(nu=1, l=50 m, sigma={args.sigma:g}) logK, top-face Dirichlet, "
+                        "4 injectors, "
                         "200 observations, AMG-PCG to 1e-10, mixed precision",
             "cells": w.n, "edges": w.nedges, "nfree": topo.nfree, "nnz": topo.nnz,
             "amg_levels": topo.levels, "reltol": 1e-10, "inner_reltol": 1e-5,

@@ -251,6 +251,51 @@ def test_2m_properties(pkg, api):
     assert rel(res["mixed_pc"], res["fp64"]) < 1e-8
 
 
+def test_2m_gradient_central_fd(pkg, api):
+    """The 2M-cell adjoint check: J = sum over the 200 observations of
+    (h - h_obs)^2, h_obs from the second seeded field plus N(0, 1e-3) noise;
+    the FP64 gradient in logK (harmonic map, solve, pullback) against central
+    finite differences (step 1e-6) on 32 seeded cells and the 4 cells of
+    largest sensitivity, and the mixed-precision gradient against the FP64
+    one (relative error <= 1e-6)."""
+    if os.environ.get("DPF_SKIP_2M"):
+        pytest.skip("DPF_SKIP_2M set")
+    _say("2m fd")
+    w = pkg.workloads.steady_2m()
+    args = (w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs)
+    K_obs = api.logKs2Ks_harmonic(pkg.workloads.observation_logK(w), w.neighbors,
+                                  w.dirichletnodes, w.areasoverlengths)
+    h_obs = api.groundwater_steadystate(K_obs, *args, reltol=1e-13)[w.obs - 1]
+    h_obs = h_obs + pkg.workloads.observation_noise(w.obs.size)
+
+    def J(logK):
+        K = api.logKs2Ks_harmonic(logK, w.neighbors, w.dirichletnodes, w.areasoverlengths)
+        h = api.groundwater_steadystate(K, *args, reltol=1e-13)
+        return np.sum((h[w.obs - 1] - h_obs) ** 2)
+
+    def grad(precision):
+        K = api.logKs2Ks_harmonic(w.logK, w.neighbors, w.dirichletnodes, w.areasoverlengths)
+        h, pb = api.rrule_groundwater_steadystate(K, *args, reltol=1e-13 if precision == "fp64"
+                                                  else 1e-10, precision=precision)
+        delta = np.zeros(w.n)
+        delta[w.obs - 1] = 2.0 * (h[w.obs - 1] - h_obs)
+        return api.logKs2Ks_harmonic_pullback(w.logK, K, pb(delta)[1], w.neighbors,
+                                              w.dirichletnodes, w.areasoverlengths)
+
+    g = grad("fp64")
+    assert rel(grad("mixed"), g) < 1e-6
+    cells = pkg.workloads.fd_cells(w, k=32, seed=6)
+    cells = np.unique(np.concatenate([cells, np.argsort(-np.abs(g))[:4] + 1]))
+    eps, gmax = 1e-6, np.abs(g).max()
+    for c in cells:
+        lp, lm = w.logK.copy(), w.logK.copy()
+        lp[c - 1] += eps
+        lm[c - 1] -= eps
+        fd = (J(lp) - J(lm)) / (2 * eps)
+        assert abs(g[c - 1] - fd) <= 1e-6 * gmax + 1e-6 * abs(fd), (c, g[c - 1], fd)
+    _say("2m fd done", len(cells), "cells, max|g|", gmax)
+
+
 def test_live_states_and_buffer_reuse(pkg, api):
     """Forward states keep their own K_e / heads while later solves run on the
     same topology, and buffers released by destroyed states are reused safely:
