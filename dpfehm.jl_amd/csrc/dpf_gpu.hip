@@ -272,6 +272,8 @@ struct dpf_topology {
   SolveScalars* scal_host = nullptr;
   HostMirror* hm_host = nullptr;  // pinned coherent host memory the finish kernel writes
   HostMirror* hm_dev = nullptr;   // the same memory as seen by the device
+  double* dots_host = nullptr;    // pinned coherent: the three sums of a BiCGStab reduction
+  double* dots_dev = nullptr;
   int seq = 0;                    // solve / residual sequence numbers
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
@@ -909,6 +911,9 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
                          hipHostMallocCoherent));
     std::memset(t->hm_outer_host, 0, t->nm * sizeof(double));
     HIPTRY(hipHostGetDevicePointer((void**)&t->hm_outer_dev, t->hm_outer_host, 0));
+    HIPTRY(hipHostMalloc((void**)&t->dots_host, 3 * sizeof(double), hipHostMallocCoherent));
+    std::memset(t->dots_host, 0, 3 * sizeof(double));
+    HIPTRY(hipHostGetDevicePointer((void**)&t->dots_dev, t->dots_host, 0));
     t->A64.alloc(t->lev[0]->A.entries * sizeof(double));
     t->invd64.alloc(H.nf * sizeof(double));
     t->b64.alloc(H.nf * sizeof(double));
@@ -1028,6 +1033,7 @@ int dpf_topology_destroy(dpf_topology* t) {
     if (t->scal_host) (void)hipHostFree(t->scal_host);
     if (t->hm_host) (void)hipHostFree(t->hm_host);
     if (t->hm_outer_host) (void)hipHostFree(t->hm_outer_host);
+    if (t->dots_host) (void)hipHostFree(t->dots_host);
     if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
   }
   delete t;
@@ -2551,16 +2557,17 @@ void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, 
 }
 
 // Up to three dot products x_k . y_k of length n in one pass, finished on the
-// device in a fixed order; the three sums land in host memory (one
-// synchronisation).
+// device in a fixed order; the finishing kernel stores the three sums in the
+// topology's pinned host buffer (one synchronisation, no copy).  Calls on one
+// topology are serialised by its mutex, so the buffer is not shared.
 struct Dots3 {
-  DBuf part, out;
+  DBuf part;
   double h[3];
   unsigned grid;
-  Dots3(BufPool* p, int64_t n) : part(p), out(p) {
+  double *host, *dev;
+  Dots3(dpf_topology* t, int64_t n) : part(t->pool.get()), host(t->dots_host), dev(t->dots_dev) {
     grid = std::max<unsigned>(1, std::min<unsigned>(nblocks(n), DOT_GRID));
     part.alloc(3 * (size_t)grid * sizeof(double));
-    out.alloc(3 * sizeof(double));
   }
   void run(hipStream_t s, int64_t n, const double* x0, const double* y0, const double* x1 = nullptr,
            const double* y1 = nullptr, const double* x2 = nullptr, const double* y2 = nullptr) {
@@ -2571,9 +2578,9 @@ struct Dots3 {
   // sums the partials a kernel of `grid` blocks left in part
   void finish(hipStream_t s) {
     hipLaunchKernelGGL(k_dots3_fin, dim3(1), dim3(BLK), 0, s, (int64_t)grid,
-                       (const double*)part.as<double>(), out.as<double>());
-    HIPTRY(hipMemcpyAsync(h, out.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+                       (const double*)part.as<double>(), dev);
     HIPTRY(hipStreamSynchronize(s));
+    for (int k = 0; k < 3; k++) h[k] = mirror_load(&host[k]);
   }
 };
 
@@ -2589,7 +2596,7 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
   BufPool* pl = t->pool.get();
   DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl);
   for (DBuf* d : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) d->alloc(nf * sizeof(double));
-  Dots3 dots(pl, nf);
+  Dots3 dots(t, nf);
   size_t bytes = nf * sizeof(double);
   dim3 g(nblocks(nf)), bl(BLK);
   if (guess) {

@@ -336,7 +336,8 @@ __global__ void __launch_bounds__(BLK) k_bicg_xr(int64_t n, double a,
 }
 
 // out[k] = sum over the nb blocks of part[3 b + k], k < 3, in a fixed order
-// (one block: strided per-thread sums, then the block tree).
+// (one block: strided per-thread sums, then the block tree); out is pinned
+// host memory, stored to at system scope.
 __global__ void __launch_bounds__(BLK) k_dots3_fin(int64_t nb, const double* __restrict__ part,
                                                    double* __restrict__ out) {
   __shared__ double sh[4];
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__(BLK) k_dots3_fin(int64_t nb, const double* __r
     double s = 0.0;
     for (int64_t b = threadIdx.x; b < nb; b += BLK) s += part[3 * b + k];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[k] = s;
+    if (threadIdx.x == 0) mput(&out[k], s);
   }
 }
 
