@@ -2585,14 +2585,16 @@ struct Dots3 {
 };
 
 // Right-preconditioned BiCGStab for A x = b (apply(x, y): y = A x;
-// prec(b, z): z ~ A^-1 b), vectors of length nf, x from 0 (guess: from the x
-// passed in), to ||b - A x|| <= tol ||b||.  Three synchronisations per
-// iteration: r^.v; s.s, t.s and t.t together; r.r and r^.r together (fused
-// with the x and r updates).  Returns 0, DPF_ENOCONV or DPF_EBREAKDOWN;
-// *iters counts the iterations.
+// prec(b, z): z ~ A^-1 b; or, with jac given, the Jacobi preconditioner
+// z = b / jac fused into the vector updates and prec unused), vectors of
+// length nf, x from 0 (guess: from the x passed in), to ||b - A x|| <= tol
+// ||b||.  Three synchronisations per iteration: r^.v; s.s, t.s and t.t
+// together; r.r and r^.r together (fused with the x and r updates).  Returns
+// 0, DPF_ENOCONV or DPF_EBREAKDOWN; *iters counts the iterations.
 template <class Apply, class Prec>
 int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec, const double* b,
-             double* x, double tol, int maxit, int* iters, bool guess = false) {
+             double* x, double tol, int maxit, int* iters, bool guess = false,
+             const double* jac = nullptr) {
   BufPool* pl = t->pool.get();
   DBuf r(pl), rh(pl), p(pl), v(pl), ph(pl), sv(pl), sh(pl), tv(pl);
   for (DBuf* d : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) d->alloc(nf * sizeof(double));
@@ -2623,10 +2625,16 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
   for (int it = 1; it <= maxit; it++) {
     if (rho1 == 0.0 || !std::isfinite(rho1)) return DPF_EBREAKDOWN;
     double beta = (rho1 / rho) * (alpha / omega);
-    // p = r + beta (p - omega v)
-    hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, 1.0, (const double*)r.as<double>(), beta,
-                       p.as<double>(), -beta * omega, (const double*)v.as<double>());
-    prec((const double*)p.as<double>(), ph.as<double>());
+    // p = r + beta (p - omega v), ph = M^-1 p
+    if (jac) {
+      hipLaunchKernelGGL(k_bicg_jacobi, g, bl, 0, s, nf, (const double*)r.as<double>(), beta,
+                         -beta * omega, (const double*)v.as<double>(), jac, p.as<double>(),
+                         ph.as<double>());
+    } else {
+      hipLaunchKernelGGL(k_rich_axpby, g, bl, 0, s, nf, 1.0, (const double*)r.as<double>(),
+                         beta, p.as<double>(), -beta * omega, (const double*)v.as<double>());
+      prec((const double*)p.as<double>(), ph.as<double>());
+    }
     apply((const double*)ph.as<double>(), v.as<double>());
     dots.run(s, nf, rh.as<double>(), v.as<double>());
     double rv = dots.h[0];
@@ -2634,9 +2642,15 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
     alpha = rho1 / rv;
     // s = r - alpha v, then t = A M^-1 s (also at the last iteration: one
     // synchronisation fewer per iteration)
-    hipLaunchKernelGGL(k_lin2, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(), 1.0,
-                       (const double*)r.as<double>(), sv.as<double>());
-    prec((const double*)sv.as<double>(), sh.as<double>());
+    if (jac) {
+      hipLaunchKernelGGL(k_bicg_jacobi, g, bl, 0, s, nf, (const double*)r.as<double>(), 0.0,
+                         -alpha, (const double*)v.as<double>(), jac, sv.as<double>(),
+                         sh.as<double>());
+    } else {
+      hipLaunchKernelGGL(k_lin2, g, bl, 0, s, nf, -alpha, (const double*)v.as<double>(), 1.0,
+                         (const double*)r.as<double>(), sv.as<double>());
+      prec((const double*)sv.as<double>(), sh.as<double>());
+    }
     apply((const double*)sh.as<double>(), tv.as<double>());
     dots.run(s, nf, sv.as<double>(), sv.as<double>(), tv.as<double>(), sv.as<double>(),
              tv.as<double>(), tv.as<double>());
@@ -3175,10 +3189,8 @@ extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const doub
     auto apply = [&](const double* xx, double* yy) {
       hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, Mv, xx, yy);
     };
-    auto prec = [&](const double* bb, double* zz) {
-      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, bb,
-                         (const double*)dg.as<double>(), zz);
-    };
+    // the Jacobi preconditioner (diagonal dg) is fused into BiCGStab's updates
+    auto prec = [](const double*, double*) {};
     int status = DPF_OK, iters = 0;
     for (int k = 0; k < nsteps && status == DPF_OK; k++) {
       const double* uk = U + (int64_t)k * n;
@@ -3189,7 +3201,8 @@ extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const doub
                          rhs.as<double>(), x.as<double>());
       int it = 0;
       status = bicgstab(t, s, nf, apply, prec, (const double*)rhs.as<double>(), x.as<double>(),
-                        std::max(o->reltol, 1e-14), o->maxiter, &it, true);
+                        std::max(o->reltol, 1e-14), o->maxiter, &it, true,
+                        (const double*)dg.as<double>());
       hipLaunchKernelGGL(k_tr_step_out, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                          t->f2n.as<int32_t>(), (const double*)x.as<double>(), un);
       iters += it;
@@ -3274,10 +3287,8 @@ extern "C" int dpf_transport_pullback(dpf_state* st, const double* dJ_du_final, 
     auto apply = [&](const double* xx, double* yy) {
       hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, MTv, xx, yy);
     };
-    auto prec = [&](const double* bb, double* zz) {
-      hipLaunchKernelGGL(k_tr_jacobi, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, bb,
-                         (const double*)dg.as<double>(), zz);
-    };
+    // the Jacobi preconditioner (diagonal dg) is fused into BiCGStab's updates
+    auto prec = [](const double*, double*) {};
     int status = DPF_OK, iters = 0;
     double* gvx = gv.as<double>();
     // each adjoint step starts from the previous one (mu holds lam)
@@ -3285,7 +3296,8 @@ extern "C" int dpf_transport_pullback(dpf_state* st, const double* dJ_du_final, 
     for (int k = st->nsteps; k >= 1 && status == DPF_OK; k--) {
       int it = 0;
       status = bicgstab(t, s, nf, apply, prec, (const double*)lam.as<double>(), mu.as<double>(),
-                        std::max(o->reltol, 1e-14), o->maxiter, &it, true);
+                        std::max(o->reltol, 1e-14), o->maxiter, &it, true,
+                        (const double*)dg.as<double>());
       iters += it;
       const double* uk = st->tr_u.as<double>() + (int64_t)k * n;
       hipLaunchKernelGGL(k_tr_grad_edges, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,

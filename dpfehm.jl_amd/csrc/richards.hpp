@@ -349,6 +349,20 @@ __global__ void __launch_bounds__(BLK) k_dots3_fin(int64_t nb, const double* __r
   }
 }
 
+// BiCGStab's vector updates with the Jacobi preconditioner z = x / d fused
+// in: p = r + beta (p - omega v), ph = p / d (a = beta, b = -beta omega,
+// x = r, y = v); or s = r - alpha v, sh = s / d (pass p = s, beta 0: a = 0,
+// b = -alpha).  out = x + a out + b y; z = out / d.
+__global__ void k_bicg_jacobi(int64_t n, const double* __restrict__ x, double a, double b,
+                             const double* __restrict__ y, const double* __restrict__ d,
+                             double* __restrict__ out, double* __restrict__ z) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  double v = a != 0.0 ? x[i] + a * out[i] + b * y[i] : x[i] + b * y[i];
+  out[i] = v;
+  z[i] = v / d[i];
+}
+
 // out = a x + b y
 __global__ void k_lin2(int64_t n, double a, const double* __restrict__ x, double b,
                        const double* __restrict__ y, double* __restrict__ out) {

@@ -151,13 +151,6 @@ __global__ void k_tr_gather(int64_t nf, const int32_t* __restrict__ f2n,
   if (j < nf) x[j] = src[f2n[j]];
 }
 
-// z = x / d
-__global__ void k_tr_jacobi(int64_t n, const double* __restrict__ x,
-                            const double* __restrict__ d, double* __restrict__ z) {
-  int64_t i = logical_row();
-  if (i < n) z[i] = x[i] / d[i];
-}
-
 // Adjoint accumulation of one backward-Euler step with adjoint mu (per free
 // cell) and state u (dt dr/dp^T mu): per edge the velocity components and D,
 // over both directions of the edge whose first cell is free.
