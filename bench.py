@@ -649,42 +649,54 @@ def bench_richards(args, P, api, _lib, torch):
     delta = torch.zeros(n, dtype=torch.float64, device=dev)
     delta[torch.from_numpy(w.obs - 1).to(dev)] = 1.0
     dK = torch.empty(ne, dtype=torch.float64, device=dev)
-    o = api._opts({"reltol": 1e-10}, memory=_lib.DPF_DEVICE, stream=sptr)
-
     def p(t):
         return ctypes.c_void_p(t.data_ptr())
 
-    def step():
-        st, sh = _lib.Stats(), ctypes.c_void_p()
-        _lib.check(L.dpf_richards_steady(topo.handle, p(psi0), p(Ks), p(dpsi), p(z), p(al), p(Nv),
-                                         p(Q), p(psi), ctypes.byref(o), ctypes.byref(st),
-                                         ctypes.byref(sh)))
-        st2 = _lib.Stats()
-        code = L.dpf_richards_pullback(sh, p(delta), p(dK), None, None, None, None,
-                                       ctypes.byref(o), ctypes.byref(st2))
-        L.dpf_state_destroy(sh)
-        _lib.check(code)
-        return st, st2
-    for _ in range(max(args.warmup, 1)):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = [step() for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    a, b = stats[-1]
-    frac = float((psi < 0).double().mean())
-    log(f"[bench] richards: {args.steps / dt:.2f} forward+gradient evals/s ({1e3 * dt / args.steps:.0f} ms), "
-        f"Newton steps {a.outer}, BiCGStab iterations {a.iterations} forward / {b.iterations} adjoint, "
-        f"relres {a.relres:.1e}, {100 * frac:.1f} % of cells unsaturated")
+    def run(precision):
+        o = api._opts({"reltol": 1e-10, "precision": precision}, memory=_lib.DPF_DEVICE,
+                      stream=sptr)
+
+        def step():
+            st, sh = _lib.Stats(), ctypes.c_void_p()
+            _lib.check(L.dpf_richards_steady(topo.handle, p(psi0), p(Ks), p(dpsi), p(z), p(al),
+                                             p(Nv), p(Q), p(psi), ctypes.byref(o),
+                                             ctypes.byref(st), ctypes.byref(sh)))
+            st2 = _lib.Stats()
+            code = L.dpf_richards_pullback(sh, p(delta), p(dK), None, None, None, None,
+                                           ctypes.byref(o), ctypes.byref(st2))
+            L.dpf_state_destroy(sh)
+            _lib.check(code)
+            return st, st2
+        for _ in range(max(args.warmup, 1)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [step() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        a, b = stats[-1]
+        log(f"[bench] richards {precision}: {args.steps / dt:.2f} forward+gradient evals/s "
+            f"({1e3 * dt / args.steps:.0f} ms), Newton steps {a.outer}, BiCGStab iterations "
+            f"{a.iterations} forward / {b.iterations} adjoint, relres {a.relres:.1e}")
+        return args.steps / dt, a, b, psi.clone(), dK.clone()
+
+    r64, a, b, psi64, dK64 = run("fp64")
+    r32, a32, b32, psi32, dK32 = run("mixed_pc")
+    frac = float((psi64 < 0).double().mean())
+    rel = lambda x, y: float(torch.linalg.norm(x - y) / torch.linalg.norm(y))  # noqa: E731
+    log(f"[bench] richards: {100 * frac:.1f} % of cells unsaturated; FP32 V-cycle vs FP64: "
+        f"heads {rel(psi32, psi64):.1e}, dK {rel(dK32, dK64):.1e}")
     res = {"metric": "steady Richards forward+gradient evals/s, 2M cells",
-           "value": round(args.steps / dt, 3), "unit": "evals/s", "n_gpus": 1,
+           "value": round(r64, 3), "unit": "evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
            "data": "synthetic", "dtype": "f64",
            "config": {"workload": "richards 2M (200x200x50), alpha 0.375, N 1.25, psi = -1 on "
                                   "the top face, 4 extraction wells of 1e-5 m^3/s"},
            "newton_steps": a.outer, "bicgstab_forward": a.iterations,
-           "bicgstab_adjoint": b.iterations, "unsaturated_fraction": round(frac, 4)}
+           "bicgstab_adjoint": b.iterations, "unsaturated_fraction": round(frac, 4),
+           "fp32_vcycle_evals_per_s": round(r32, 3),
+           "fp32_vcycle_bicgstab_fwd_adj": [a32.iterations, b32.iterations],
+           "fp32_vcycle_rel_err_head_grad": [rel(psi32, psi64), rel(dK32, dK64)]}
     print(json.dumps(res), flush=True)
 
 

@@ -2556,6 +2556,22 @@ void rich_precond(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs, 
   copy_dev(z, out, nf * sizeof(double), s);
 }
 
+// The same V-cycle on the FP32 hierarchy (b rounded to FP32, z widened back).
+void rich_precond32(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
+                    const double* b, double* z) {
+  int64_t nf = t->H.nf;
+  LevelNum<float>& n0 = t->h32.lev[0];
+  float* b32 = t->h32.cr.as<float>();
+  float* x0 = n0.x.as<float>();
+  hipLaunchKernelGGL(k_rich_pre32, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, b,
+                     (const float*)level_invd(t, t->h32, 0), (float)chs[0].c0, b32,
+                     n0.d.as<float>(), x0);
+  float* out = nullptr;
+  vcycle<float>(t, t->h32, s, 0, (const float*)b32, x0, n0.xs.as<float>(), &out, chs,
+                t->zero_flag.as<int>(), false);
+  hipLaunchKernelGGL(k_widen, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, (const float*)out, z);
+}
+
 // Up to three dot products x_k . y_k of length n in one pass, finished on the
 // device in a fixed order; the finishing kernel stores the three sums in the
 // topology's pinned host buffer (one synchronisation, no copy).  Calls on one
@@ -2678,14 +2694,21 @@ int bicgstab(dpf_topology* t, hipStream_t s, int64_t nf, Apply apply, Prec prec,
   return DPF_ENOCONV;
 }
 
+// BiCGStab on a Newton matrix of the fine layout; the AMG preconditioner runs
+// in FP32 when fp32 (the hierarchy prepared in that precision).
 int rich_bicgstab(dpf_topology* t, hipStream_t s, const std::vector<Cheb>& chs,
                   SellD<double> M, const double* b, double* x, double tol, int maxit,
-                  int* iters) {
+                  int* iters, bool fp32) {
   int64_t nf = t->H.nf;
   auto apply = [&](const double* xx, double* yy) {
     hipLaunchKernelGGL(k_rich_spmv, dim3(nblocks(nf)), dim3(BLK), 0, s, M, xx, yy);
   };
-  auto prec = [&](const double* bb, double* zz) { rich_precond(t, s, chs, bb, zz); };
+  auto prec = [&](const double* bb, double* zz) {
+    if (fp32)
+      rich_precond32(t, s, chs, bb, zz);
+    else
+      rich_precond(t, s, chs, bb, zz);
+  };
   return bicgstab(t, s, nf, apply, prec, b, x, tol, maxit, iters);
 }
 
@@ -2721,22 +2744,22 @@ std::pair<double, double> rich_residual(dpf_topology* t, hipStream_t s, const Ri
 // Newton matrix M (and MT) at psi: Picard operator (and its AMG hierarchy) for
 // the preconditioner, then the kr-slope terms.
 void rich_matrices(dpf_topology* t, hipStream_t s, const RichIn& in, const double* psi,
-                   const double* dFa, const double* dFb, double* M, double* MT) {
+                   const double* dFa, const double* dFb, double* M, double* MT, bool fp32) {
   int64_t ne = t->H.ne, nf = t->H.nf;
   hipLaunchKernelGGL(k_rich_picard, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
                      t->n2f.as<int32_t>(), psi, in.dpsi, in.z, in.al, in.N, in.K,
                      t->Ke.as<double>());
-  prepare(t, s, DPF_FP64, nullptr, false);
+  prepare(t, s, fp32 ? DPF_MIXED_PC : DPF_FP64, nullptr, false);
   LevelDev& d0 = *t->lev[0];
   size_t bytes = d0.A.entries * sizeof(double);
-  HIPTRY(hipMemsetAsync(M, 0, bytes, s));
+  if (M) HIPTRY(hipMemsetAsync(M, 0, bytes, s));
   if (MT) HIPTRY(hipMemsetAsync(MT, 0, bytes, s));
   hipLaunchKernelGGL(k_rich_newton_matrix, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                      t->f2n.as<int32_t>(), t->n2f.as<int32_t>(), t->edges.as<int2>(),
                      t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
                      t->inc_other.as<int32_t>(), t->tp_mirror.as<int32_t>(),
                      t->inc_slot.as<uint16_t>(), d0.diag_slot.as<uint16_t>(),
-                     d0.A.view<double>(M), dFa, dFb, (const double*)nullptr, M, MT);
+                     d0.A.view<double>(M ? M : MT), dFa, dFb, (const double*)nullptr, M, MT);
   HIPTRY(hipGetLastError());
 }
 
@@ -2775,6 +2798,7 @@ extern "C" int dpf_richards_steady(dpf_topology* t, const double* psi0, const do
     double t0 = now_ms();
     hipStream_t s = pick_stream(t, o);
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    bool fp32 = o->precision != DPF_FP64;  // FP32 V-cycle preconditioner
     t->timing = false;
     tp_build_maps(t);
     BufPool* pl = t->pool.get();
@@ -2823,10 +2847,11 @@ extern "C" int dpf_richards_steady(dpf_topology* t, const double* psi0, const do
         status = DPF_EBREAKDOWN;
         break;
       }
-      rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), Mv.as<double>(), nullptr);
+      rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), Mv.as<double>(), nullptr,
+                    fp32);
       int its = 0;
       int ls = rich_bicgstab(t, s, chs, d0.A.view<double>(Mv.as<double>()), r.as<double>(),
-                             dx.as<double>(), 1e-10, o->maxiter, &its);
+                             dx.as<double>(), 1e-10, o->maxiter, &its, fp32);
       lin += its;
       if (ls == DPF_EBREAKDOWN) {
         status = DPF_EBREAKDOWN;
@@ -2892,11 +2917,16 @@ extern "C" int dpf_richards_pullback(dpf_state* st, const double* delta, double*
   dpf_opts dflt;
   dpf_opts_init(&dflt);
   const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
     hipStream_t s = pick_stream(t, o);
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
+    bool fp32 = o->precision != DPF_FP64;  // FP32 V-cycle preconditioner
     bool host = o->memory == DPF_HOST;
     t->timing = false;
     tp_build_maps(t);
@@ -2904,18 +2934,17 @@ extern "C" int dpf_richards_pullback(dpf_state* st, const double* delta, double*
     RichIn in{st->Ke.as<double>(), st->dh.as<double>(), st->r_z.as<double>(),
               st->r_al.as<double>(), st->r_N.as<double>(), st->Q.as<double>()};
     const double* x = st->hfree.as<double>();
-    DBuf F(pl), dFa(pl), dFb(pl), Mv(pl), MTv(pl), db(pl), rhs(pl), mu(pl), ga(pl), gb(pl),
+    DBuf F(pl), dFa(pl), dFb(pl), MTv(pl), db(pl), rhs(pl), mu(pl), ga(pl), gb(pl),
         oK(pl), oA(pl), oN(pl), oD(pl), oQ(pl);
     for (DBuf* d : {&F, &dFa, &dFb, &ga, &gb}) d->alloc(ne * sizeof(double));
     LevelDev& d0 = *t->lev[0];
-    Mv.alloc(d0.A.entries * sizeof(double));
     MTv.alloc(d0.A.entries * sizeof(double));
     hipLaunchKernelGGL(k_rich_edge_flux, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
                        t->edges.as<int2>(), t->n2f.as<int32_t>(), x, in.dpsi, in.z, in.al, in.N,
                        in.K, t->aol.as<double>(), F.as<double>(), dFa.as<double>(),
                        dFb.as<double>());
-    rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), Mv.as<double>(),
-                  MTv.as<double>());
+    rich_matrices(t, s, in, x, dFa.as<double>(), dFb.as<double>(), nullptr, MTv.as<double>(),
+                  fp32);
     const double* dl = stage_in(t, o, db, delta, n, s);
     rhs.alloc(nf * sizeof(double));
     mu.alloc(nf * sizeof(double));
@@ -2925,7 +2954,7 @@ extern "C" int dpf_richards_pullback(dpf_state* st, const double* delta, double*
     int its = 0;
     std::vector<Cheb> chs = rich_chebs(t);
     int status = rich_bicgstab(t, s, chs, d0.A.view<double>(MTv.as<double>()), rhs.as<double>(),
-                               mu.as<double>(), 1e-12, o->maxiter, &its);
+                               mu.as<double>(), 1e-12, o->maxiter, &its, fp32);
     hipLaunchKernelGGL(k_rich_axpby, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, 0.0,
                        (const double*)mu.as<double>(), -1.0, mu.as<double>(), 0.0,
                        (const double*)nullptr);

@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(BLK) k_rich_residual(
   }
 }
 
-// The Newton matrix M = -dr/dpsi on the fine pattern (row j = free cell i),
-// and, if MT is given, its transpose there (entry (k, j) at row k's slot for
+// The Newton matrix M = -dr/dpsi on the fine pattern (row j = free cell i,
+// if M is given), and, if MT is given, its transpose there (entry (k, j) at row k's slot for
 // the same edge).  The arrays are zeroed by the caller; parallel edges add
 // into one slot (only thread j writes row j of M and column j of MT).  M is the Picard (groundwater)
 // operator plus the kr-slope terms: M_jj = sum (H - Hi kr_i' dlt_io) K aol / sv_i,
@@ -184,7 +184,7 @@ __global__ void k_rich_newton_matrix(
     int32_t fo = n2f[inc_other[p]];
     if (fo >= 0) {
       double v = oth / s;
-      M[base + (int64_t)inc_slot[p] * 64] += v;
+      if (M) M[base + (int64_t)inc_slot[p] * 64] += v;
       if (MT) {
         // row fo differs from lane to lane: its slice pointer is a vector load
         int64_t bo = A.sptr[fo >> 6] + (fo & 63);
@@ -193,7 +193,7 @@ __global__ void k_rich_newton_matrix(
     }
   }
   int64_t dpos = base + (int64_t)diag_slot[j] * 64;
-  M[dpos] = diag;
+  if (M) M[dpos] = diag;
   if (MT) MT[dpos] = diag;
 }
 
@@ -279,6 +279,26 @@ __global__ void k_rich_pre(int64_t n, const double* __restrict__ b,
 // Fixed grid of the reduction kernels below: a thread's grid-stride
 // sequence, and so every sum, is the same on every run.
 constexpr int DOT_GRID = 1024;
+
+// The FP32 V-cycle's input and zero-guess first step from an FP64 vector:
+// b32 = fl32(b), d = x = c0 invd b32.
+__global__ void k_rich_pre32(int64_t n, const double* __restrict__ b,
+                             const float* __restrict__ invd, float c0, float* __restrict__ b32,
+                             float* __restrict__ d, float* __restrict__ x) {
+  int64_t i = logical_row();
+  if (i >= n) return;
+  float bv = (float)b[i];
+  b32[i] = bv;
+  float v = c0 * invd[i] * bv;
+  d[i] = v;
+  x[i] = v;
+}
+
+// z = (double) x
+__global__ void k_widen(int64_t n, const float* __restrict__ x, double* __restrict__ z) {
+  int64_t i = logical_row();
+  if (i < n) z[i] = (double)x[i];
+}
 
 // Up to three dot products in one pass: part[3 * block + k] = block sum of
 // x_k . y_k (null x_k: 0), grid-stride over a grid of at most DOT_GRID blocks.

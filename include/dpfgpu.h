@@ -216,7 +216,9 @@ int dpf_solvetwophase(dpf_topology *topo, const double *h0, const double *S0, co
                       int64_t *nsub_total, dpf_state **state_out, const dpf_opts *opts,
                       dpf_stats *stats);
 
-/* Steady Richards equation (src/richards.jl:97-205), FP64.  Per-edge van
+/* Steady Richards equation (src/richards.jl:97-205), FP64 (with precision
+ * DPF_MIXED or DPF_MIXED_PC the preconditioner's V-cycle runs in FP32; the
+ * residuals, Newton matrices and BiCGStab stay FP64).  Per-edge van
  * Genuchten parameters alphas, Ns (nedges), the cell elevation z (n, the
  * third row of the reference's coords), Dirichlet pressure heads
  * dirichletpsis (n).  Single-system topologies with at least two AMG levels.

@@ -133,3 +133,21 @@ def test_parallel_edges(pkg, rich, ro):
     g, go = pb(w), pbo(w)
     assert rel(g[2], go["Ks"]) < 1e-7
     assert rel(g[10], go["Qs"]) < 1e-7
+
+
+def test_fp32_preconditioner(pkg, rich, ro):
+    """precision mixed_pc: the V-cycle preconditioner in FP32; heads and
+    tangents agree with the oracle as closely as the FP64 path's."""
+    _say("fp32 preconditioner")
+    coords, nb, aol, n, Ks, Qs, dn, dpsi, alphas, Ns = richards_box(pkg, ns=(12, 10, 6), seed=4)
+    dpsi[n - 1] = -0.2
+    args = (Ks, nb, aol, dn, dpsi, coords, alphas, Ns, Qs)
+    st = {}
+    psi, pb = rich.rrule_richards_steadystate(np.zeros(n), *args, stats=st, reltol=1e-12,
+                                              precision="mixed_pc")
+    pso, pbo = ro.richards_rrule(np.zeros(n), *args)
+    assert st["converged"] == 1 and rel(psi, pso) < 1e-9
+    w = np.random.default_rng(6).normal(size=n)
+    g, go = pb(w), pbo(w)
+    for k, key in ((2, "Ks"), (6, "dirichletpsis"), (8, "alphas"), (9, "Ns"), (10, "Qs")):
+        assert rel(g[k], go[key]) < 1e-7, key
