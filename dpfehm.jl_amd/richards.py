@@ -13,8 +13,9 @@ elevation) enters the equations.  ``alphas`` and ``Ns`` are per edge, as in
 the reference.  The reference solves with NLsolve to its default residual
 tolerance; here Newton's method runs on the GPU until max|r| falls below
 ``reltol`` (default 1e-10) times its initial value, each Newton system solved
-by BiCGStab with the AMG V-cycle of the Picard operator as preconditioner.
-Non-convergence raises instead of returning.
+by BiCGStab with the AMG V-cycle of the Picard operator as preconditioner
+(``precision="mixed_pc"`` runs that V-cycle in FP32; everything else stays
+FP64).  Non-convergence raises instead of returning.
 """
 import ctypes
 
