@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0,
                     help="steady2m: standard deviation of the logK fields (the conditioning "
                          "sweep runs 1, 2 and 3; the headline line is sigma 1)")
+    ap.add_argument("--graph", action="store_true",
+                    help="steady2m and ensemble: replay each PCG iteration as a captured "
+                         "hipGraph (DPF_FLAG_GRAPH)")
     ap.add_argument("--tuning", default=None, metavar="FINE,COARSE,RATIO",
                     help="steady2m: fix the Chebyshev degrees and interval ratio of every "
                          "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
@@ -119,8 +122,8 @@ def main():
     dlogK_d = torch.empty(w.n, dtype=torch.float64, device=dev)
 
     def opts(precision, timing=False):
-        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
-                      stream=sptr)
+        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph},
+                      memory=_lib.DPF_DEVICE, stream=sptr)
         if timing:
             o.flags = _lib.DPF_FLAG_SWEEP_TIMING
         return o
@@ -280,6 +283,7 @@ def main():
         "topology_setup_s": round(topo_s, 3),
         "cheb_tuned_finest_coarse_ratio": tuned,
         "cheb_pinned": bool(args.tuning),
+        "hipgraph_iterations": bool(args.graph),
         "roofline": {
             "bound": "hbm",
             "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",
@@ -379,8 +383,8 @@ def bench_ensemble(args, P, api, _lib, torch):
 
     res = {}
     for precision in ("fp64", "mixed", "mixed_pc"):
-        o = api._opts({"precision": precision, "reltol": 1e-10}, memory=_lib.DPF_DEVICE,
-                      stream=sptr)
+        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph},
+                      memory=_lib.DPF_DEVICE, stream=sptr)
         for _ in range(args.warmup):
             evaluate(o)
         torch.cuda.synchronize()

@@ -13,6 +13,7 @@ DPF_OK, DPF_EINVAL, DPF_EHIP, DPF_ENOCONV, DPF_EBREAKDOWN, DPF_ESINGULAR, DPF_EN
 DPF_FP64, DPF_MIXED, DPF_MIXED_PC = 0, 1, 2
 DPF_HOST, DPF_DEVICE = 0, 1
 DPF_FLAG_SWEEP_TIMING = 1
+DPF_FLAG_GRAPH = 2
 
 _ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOCONV", 4: "EBREAKDOWN", 5: "ESINGULAR", 6: "ENOMEM"}
 

@@ -63,12 +63,17 @@ _OPT_KEYS = {"reltol", "abstol", "maxiter", "precision", "inner_reltol", "max_ou
 
 
 def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
+    """dpf_opts from keyword arguments; ``graph=True`` sets DPF_FLAG_GRAPH
+    (each PCG iteration replayed as one captured hipGraph)."""
     kw = dict(kwargs)
     kw.pop("linear_solver", None)
+    graph = bool(kw.pop("graph", False))
     bad = set(kw) - _OPT_KEYS
     if bad:
         raise TypeError(f"unexpected keyword argument(s): {sorted(bad)}")
     o = _lib.default_opts(**kw)
+    if graph:
+        o.flags |= _lib.DPF_FLAG_GRAPH
     o.memory = memory
     o.stream = stream
     return o

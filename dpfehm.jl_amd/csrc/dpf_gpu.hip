@@ -274,6 +274,10 @@ struct dpf_topology {
   HostMirror* hm_dev = nullptr;   // the same memory as seen by the device
   double* dots_host = nullptr;    // pinned coherent: the three sums of a BiCGStab reduction
   double* dots_dev = nullptr;
+  // DPF_FLAG_GRAPH: captured PCG iterations, keyed by precision, smoother,
+  // buffers and the p ping-pong parity
+  bool use_graphs = false;
+  std::map<std::string, hipGraphExec_t> iter_graphs;
   int seq = 0;                    // solve / residual sequence numbers
   uint64_t version = 0;        // bumped at every numeric refresh
   uint64_t assembled_for = 0;  // version whose K the fine operator holds
@@ -707,6 +711,64 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   TP* z = nullptr;
   vcycle<TP>(t, hp, s, 0, vb, z0, zs, &z, chs, done, true);
   int par = 0;
+  // one iteration's launches; the V-cycle's output buffer is the same on
+  // every call, so z does not change from one iteration to the next
+  auto body = [&](int pr) {
+    hipLaunchKernelGGL((k_cg_pq<TK, TP>), dim3(g), dim3(BLK), 0, s, A, (const TP*)z,
+                       (const TK*)pb[pr], pb[pr ^ 1], q, (const int*)done,
+                       (const SolveScalars*)sc, bm_ptr(t), part);
+    finish(t, s, FIN_PQ, g);
+    hipLaunchKernelGGL((k_cg_xr<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, x, r,
+                       (const TK*)pb[pr ^ 1], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0,
+                       (const int*)done, (const SolveScalars*)sc, bm_ptr(t), part);
+    finish(t, s, FIN_XR, g);
+    TP* zz = nullptr;
+    vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
+    if (zz != z) throw Err{DPF_EHIP, "V-cycle output buffer moved between iterations"};
+  };
+  bool graphs = t->use_graphs && !t->timing;
+  std::string gkey;
+  if (graphs) {
+    // everything the captured launches take as arguments and can differ
+    // between calls: types, the smoother's coefficients, the buffers
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "%zu/%zu/%p/%p/%p/%p/%p/", sizeof(TK), sizeof(TP), (void*)z,
+                  (const void*)vb, (void*)dd, (void*)A.val, (void*)x);
+    gkey = buf;
+    auto num = [&](double v) {  // exact (hexadecimal) text of a coefficient
+      std::snprintf(buf, sizeof(buf), "%a,", v);
+      gkey += buf;
+    };
+    for (const Cheb& c : chs) {
+      num(c.c0);
+      for (size_t k = 0; k < c.c1.size(); k++) {
+        num(c.c1[k]);
+        num(c.c2[k]);
+      }
+      gkey += ";";
+    }
+  }
+  auto graph_for = [&](int pr) -> hipGraphExec_t {
+    std::string key = gkey + std::to_string(pr);
+    auto it = t->iter_graphs.find(key);
+    if (it != t->iter_graphs.end()) return it->second;
+    HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t gr = nullptr;
+    try {
+      body(pr);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    HIPTRY(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPTRY(e);
+    t->iter_graphs[key] = ex;
+    return ex;
+  };
   SolveResult res{0, 0, 0, 0};
   // Keep one iteration queued beyond the last one whose convergence test the
   // device has published: the host enqueues iteration k+1 while the GPU runs
@@ -729,18 +791,11 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
       break;
     }
     if (launched < maxiter && launched <= its) {
-      hipLaunchKernelGGL((k_cg_pq<TK, TP>), dim3(g), dim3(BLK), 0, s, A, (const TP*)z,
-                         (const TK*)pb[par], pb[par ^ 1], q, (const int*)done,
-                         (const SolveScalars*)sc, bm_ptr(t), part);
-      finish(t, s, FIN_PQ, g);
+      if (graphs)
+        HIPTRY(hipGraphLaunch(graph_for(par), s));
+      else
+        body(par);
       par ^= 1;
-      hipLaunchKernelGGL((k_cg_xr<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, x, r,
-                         (const TK*)pb[par], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0,
-                         (const int*)done, (const SolveScalars*)sc, bm_ptr(t), part);
-      finish(t, s, FIN_XR, g);
-      TP* zz = nullptr;
-      vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
-      z = zz;
       launched++;
       HIPTRY(hipGetLastError());
       continue;
@@ -1034,6 +1089,7 @@ int dpf_topology_destroy(dpf_topology* t) {
     if (t->hm_host) (void)hipHostFree(t->hm_host);
     if (t->hm_outer_host) (void)hipHostFree(t->hm_outer_host);
     if (t->dots_host) (void)hipHostFree(t->dots_host);
+    for (auto& kv : t->iter_graphs) (void)hipGraphExecDestroy(kv.second);
     if (t->own_stream) (void)hipStreamDestroy(t->own_stream);
   }
   delete t;
@@ -1133,6 +1189,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
                  const dpf_opts* o, dpf_stats* st, const double* x0 = nullptr) {
   int64_t nf = t->H.nf;
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
+  t->use_graphs = (o->flags & DPF_FLAG_GRAPH) != 0;
   t->sweep_ms = 0;
   t->sweep_n = 0;
   int pidx = o->precision;
@@ -1258,6 +1315,8 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       // the whole refinement's worth of convergence, where a cheaper V-cycle
       // that converges more slowly loses what it seems to win over the first
       // few digits.
+      bool graphs_on = t->use_graphs;
+      t->use_graphs = false;  // trials use launches: a capture would distort their times
       static const int cand[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
       static const double ratios[] = {30.0, 8.0};
       double trial_tol = std::max(o->inner_reltol * o->inner_reltol, 1e-12);
@@ -1285,6 +1344,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       int m0 = t->deg_fine[pidx], mc = t->deg_coarse[pidx];
       for (int k = 0; k < 2; k++) trial(m0, mc, ratios[k], 5 + k);
       t->tuned[pidx] = true;
+      t->use_graphs = graphs_on;
       tune = false;
       chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
     }
@@ -1331,6 +1391,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     st->sweep_launches = t->sweep_n;
   }
   t->timing = false;
+  t->use_graphs = false;
   return status;
 }
 

@@ -40,10 +40,16 @@ enum {
  * iteration count, with the V-cycle's traffic halved. */
 enum { DPF_FP64 = 0, DPF_MIXED = 1, DPF_MIXED_PC = 2 };
 enum { DPF_HOST = 0, DPF_DEVICE = 1 };
-/* flags: time every finest-level smoothing sweep with a hipEvent pair and
- * report the sum in dpf_stats.sweep_ms / sweep_launches (adds two event
- * records per sweep) */
-enum { DPF_FLAG_SWEEP_TIMING = 1 };
+/* flags: DPF_FLAG_SWEEP_TIMING times every finest-level smoothing sweep with
+ * a hipEvent pair and reports the sum in dpf_stats.sweep_ms / sweep_launches
+ * (adds two event records per sweep).  DPF_FLAG_GRAPH replays each PCG
+ * iteration (SpMV and updates, the reductions' finishing kernels, the whole
+ * V-cycle: ~20 launches) as one hipGraph, captured on the topology's first
+ * iteration with the same precision and smoother and kept until it is
+ * destroyed; the kernels and their order are those of the launched form, so
+ * results are bitwise equal.  Ignored together with sweep timing and during
+ * the autotuner's trials.  The stream must not be the legacy NULL stream. */
+enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2 };
 
 typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
