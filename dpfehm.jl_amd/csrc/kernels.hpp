@@ -753,28 +753,75 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
   int tid = threadIdx.x, nt = blockDim.x;
   // up to DENSE_LDS_MAX rows the elimination runs in LDS and W is written once
   extern __shared__ double wl[];
-  double* Wk = nc <= DENSE_LDS_MAX ? wl : W;
+  double* Wk = nc <= DENSE_LDS_MAX && nc * nc <= 8 * nt ? wl : W;
   for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) Wk[t] = 0.0;
   __syncthreads();
-  for (int I = tid; I < nc; I += nt) {
-    int64_t cb;
-    int cw;
-    sell_row(Ac, I, cb, cw);
-    for (int k = 0; k < rowlen[I]; k++)
-      Wk[(int64_t)I * nc + Ac.col[cb + (int64_t)k * 64]] += (double)Ac.val[cb + (int64_t)k * 64];
+  // entry k of row I for every (I, k) with k < rowlen[I] <= nc, spread over
+  // the whole workgroup (a row's columns are distinct: plain stores)
+  for (int64_t e = tid; e < (int64_t)nc * nc; e += nt) {
+    int I = (int)(e % nc), k = (int)(e / nc);
+    if (k >= rowlen[I]) continue;
+    int64_t p = Ac.sptr[I >> 6] + (I & 63) + (int64_t)k * 64;
+    Wk[(int64_t)I * nc + Ac.col[p]] = (double)Ac.val[p];
   }
   __syncthreads();
   __shared__ int bad;
   if (tid == 0) bad = 0;
   __syncthreads();
-  // on the LDS path each thread owns at most 8 fixed positions of W
-  const bool small = Wk != W;
-  int pi[8], pj[8], np = 0;
-  if (small)
-    for (int t = tid; t < nc * nc && np < 8; t += nt, np++) {
-      pi[np] = t / nc;
-      pj[np] = t - pi[np] * nc;
+  if (Wk != W) {
+    // Small systems: each thread keeps at most 8 fixed positions of W in
+    // registers; the owners of step k's pivot row and column publish them in
+    // LDS (double-buffered by step parity, so one barrier per step suffices),
+    // and every thread updates its own positions.  Same arithmetic, in the
+    // same order, as the in-memory elimination below.
+    __shared__ double prow[2][DENSE_LDS_MAX], pcol[2][DENSE_LDS_MAX];
+    int pi[8], pj[8];
+    double v[8];
+    const int nn = nc * nc;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int t = tid + q * nt;
+      pi[q] = t < nn ? t / nc : -1;  // -1: no position
+      pj[q] = t < nn ? t - pi[q] * nc : -1;
+      v[q] = t < nn ? Wk[t] : 0.0;
     }
+    const int npos = (nn + nt - 1) / nt;  // positions per thread, uniform
+    for (int k = 0; k < nc; k++) {
+      const int b = k & 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (q >= npos) break;
+        if (pi[q] == k) prow[b][pj[q]] = v[q];
+        if (pj[q] == k) pcol[b][pi[q]] = v[q];
+      }
+      __syncthreads();
+      double piv = prow[b][k];
+      if (!(piv > 0.0)) {
+        if (tid == 0) bad = 1;
+        piv = 1.0;
+      }
+      double ip = 1.0 / piv;
+      // branch-free: the elimination update, the pivot row / column scaling
+      // and the pivot itself, selected per position
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (q >= npos) break;
+        int i = pi[q], j = pj[q];
+        bool ok = i >= 0, ri = i == k, cj = j == k;
+        double a = pcol[b][ok ? i : 0], c = prow[b][ok ? j : 0];
+        double upd = v[q] - a * c * ip;
+        double scl = v[q] * (ri ? ip : -ip);
+        double nv = (!ri && !cj) ? upd : ((ri && cj) ? ip : scl);
+        v[q] = ok ? nv : v[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (pi[q] >= 0) W[(int64_t)pi[q] * nc + pj[q]] = v[q];
+    __syncthreads();
+    if (tid == 0) status[0] = bad;
+    return;
+  }
   for (int k = 0; k < nc; k++) {
     double piv = Wk[(int64_t)k * nc + k];
     if (!(piv > 0.0)) {
@@ -782,16 +829,9 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
       piv = 1.0;
     }
     double ip = 1.0 / piv;
-    if (small) {
-      for (int q = 0; q < np; q++) {
-        int i = pi[q], j = pj[q];
-        if (i != k && j != k) Wk[i * nc + j] -= Wk[i * nc + k] * Wk[k * nc + j] * ip;
-      }
-    } else {
-      for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) {
-        int i = (int)(t / nc), j = (int)(t % nc);
-        if (i != k && j != k) Wk[t] -= Wk[(int64_t)i * nc + k] * Wk[(int64_t)k * nc + j] * ip;
-      }
+    for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) {
+      int i = (int)(t / nc), j = (int)(t % nc);
+      if (i != k && j != k) Wk[t] -= Wk[(int64_t)i * nc + k] * Wk[(int64_t)k * nc + j] * ip;
     }
     __syncthreads();
     for (int t = tid; t < nc; t += nt) {
@@ -804,8 +844,6 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
     if (tid == 0) Wk[(int64_t)k * nc + k] = ip;
     __syncthreads();
   }
-  if (Wk != W)
-    for (int64_t t = tid; t < (int64_t)nc * nc; t += nt) W[t] = Wk[t];
   if (tid == 0) status[0] = bad;
 }
 
