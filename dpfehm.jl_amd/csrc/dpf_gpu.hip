@@ -186,6 +186,8 @@ struct LevelDev {
   DBuf as_src;
   int64_t nc = 0;  // rows of the next level
   DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_dest, rap_p, rap_ap;
+  DBuf rap_mdst, rap_msrc;
+  int64_t n_mirror = 0;
   int ap_lg = 0, rap_lg = 0;
   int64_t ap_nnz = 0;
 };
@@ -448,6 +450,9 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
                        s, dn.A.entries, d.rap_lg, d.rap_off.as<int64_t>(),
                        d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(), d.rap_ap.as<int32_t>(),
                        (const T*)P.val, (const T*)apv, cval);
+    if (d.n_mirror > 0)
+      hipLaunchKernelGGL(k_mirror_copy<T>, dim3(nblocks(d.n_mirror)), dim3(BLK), 0, s,
+                         d.n_mirror, d.rap_mdst.as<int32_t>(), d.rap_msrc.as<int32_t>(), cval);
     hipLaunchKernelGGL(k_rowdiag<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s,
                        dn.A.view<T>(cval), dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
     if (dn.as_lg > 0)
@@ -936,6 +941,9 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
         upload(d->ap_p, lh.ap_p);
         upload(d->rap_off, lh.rap_off);
         upload(d->rap_dest, lh.rap_dest);
+        upload(d->rap_mdst, lh.rap_mdst);
+        upload(d->rap_msrc, lh.rap_msrc);
+        d->n_mirror = (int64_t)lh.rap_mdst.size();
         d->ap_lg = lh.ap_lg;
         d->rap_lg = lh.rap_lg;
         upload(d->rap_p, lh.rap_p);

@@ -69,9 +69,13 @@ struct LevelHost {
   // next level's A, one contribution list per SELL entry, lists in row-major
   // order of (row, slot) so that neighbouring threads share fine rows:
   //   Ac[rap_dest[k]] = sum_{q in [rap_off[k], rap_off[k+1])} P.val[rap_p[q]] * AP[rap_ap[q]]
-  // (padding entries have empty lists)
+  // (padding entries have empty lists).  The Galerkin operator of a
+  // symmetric A is symmetric, so only entries with column >= row have
+  // products; each entry below the diagonal is then copied from its mirror:
+  //   Ac[rap_mdst[m]] = Ac[rap_msrc[m]]
   std::vector<int64_t> rap_off;
   std::vector<int32_t> rap_dest, rap_p, rap_ap;
+  std::vector<int32_t> rap_mdst, rap_msrc;
   int ap_lg = 0, rap_lg = 0;  // log2 lanes per contribution list on the GPU
 };
 

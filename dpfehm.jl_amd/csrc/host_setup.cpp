@@ -469,7 +469,7 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
         for (int t = 0; t < R.rowlen[I]; t++) {
           int32_t i = R.at(I, t);
           for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
-            L.rap_off[rm_base[I] + slot_of[L.ap_col[a]] + 1]++;
+            if (L.ap_col[a] >= I) L.rap_off[rm_base[I] + slot_of[L.ap_col[a]] + 1]++;
         }
       }
     }
@@ -491,6 +491,7 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
           int32_t i = R.at(I, t);
           int32_t pe = L.r_src[R.idx(I, t)];
           for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) {
+            if (L.ap_col[a] < I) continue;  // below the diagonal: mirrored
             int64_t q = fill[slot_of[L.ap_col[a]]]++;
             L.rap_p[q] = pe;
             L.rap_ap[q] = (int32_t)a;
@@ -498,6 +499,35 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
         }
       }
     }
+    // entries below the diagonal and the positions of their mirrors (the
+    // pattern is symmetric: column I of row J exists whenever column J of
+    // row I does)
+    std::vector<int64_t> mo(nc + 1, 0);  // per row: its entries below the diagonal
+#pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < nc; I++) {
+      int64_t c = 0;
+      for (int64_t k = rp[I]; k < rp[I + 1] && ci[k] < I; k++) c++;
+      mo[I + 1] = c;
+    }
+    prefix(mo);
+    L.rap_mdst.assign(mo[nc], 0);
+    L.rap_msrc.assign(mo[nc], 0);
+    bool sym = true;
+#pragma omp parallel for schedule(dynamic, 512) reduction(&& : sym)
+    for (int64_t I = 0; I < nc; I++)
+      for (int64_t k = rp[I]; k < rp[I] + (mo[I + 1] - mo[I]); k++) {
+        int32_t J = ci[k];
+        auto b = ci.begin() + rp[J], e = ci.begin() + rp[J + 1];
+        auto it = std::lower_bound(b, e, (int32_t)I);
+        if (it == e || *it != (int32_t)I) {
+          sym = false;
+          continue;
+        }
+        int64_t m = mo[I] + (k - rp[I]);
+        L.rap_mdst[m] = (int32_t)C.idx(I, (int)(k - rp[I]));
+        L.rap_msrc[m] = (int32_t)C.idx(J, (int)(it - b));
+      }
+    if (!sym) return "coarse operator pattern is not symmetric";
     auto lanes = [](int64_t nprod_, int64_t nlists) {
       double avg = nlists > 0 ? (double)nprod_ / (double)nlists : 1.0;
       int lg = 0;

@@ -107,6 +107,8 @@ int with_array(const dpf_inspect* h, int32_t level, const std::string& s, Fn&& f
   if (s == "rap_p") return f(L.rap_p);
   if (s == "rap_dest") return f(L.rap_dest);
   if (s == "rap_ap") return f(L.rap_ap);
+  if (s == "rap_mdst") return f(L.rap_mdst);
+  if (s == "rap_msrc") return f(L.rap_msrc);
   return -1;
 }
 

@@ -721,6 +721,15 @@ __global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict_
   if ((sub & (G - 1)) == G - 1 && k < nlists) out[dest ? dest[k] : k] = acc;
 }
 
+// val[dst[m]] = val[src[m]]: the entries below the diagonal of a symmetric
+// Galerkin operator from their mirrors above it (disjoint sets).
+template <typename T>
+__global__ void k_mirror_copy(int64_t nm, const int32_t* __restrict__ dst,
+                              const int32_t* __restrict__ src, T* __restrict__ val) {
+  int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (m < nm) val[dst[m]] = val[src[m]];
+}
+
 // invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator
 template <typename T>
 __global__ void k_rowdiag(SellD<T> A, const uint16_t* __restrict__ diag_slot,

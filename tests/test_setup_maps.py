@@ -19,7 +19,8 @@ DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.in
       "Rs.sptr": np.int64, "Rs.col": np.int32, "Rs.rowlen": np.int32, "rs_src": np.int32,
       "ap_off": np.int64, "ap_a": np.int32, "ap_p": np.int32, "rap_off": np.int64,
       "rap_p": np.int32, "rap_ap": np.int32, "As.sptr": np.int64, "As.col": np.int32,
-      "As.rowlen": np.int32, "as_src": np.int32, "rap_dest": np.int32}
+      "As.rowlen": np.int32, "as_src": np.int32, "rap_dest": np.int32, "rap_mdst": np.int32,
+      "rap_msrc": np.int32}
 
 
 class Inspect:
@@ -114,6 +115,10 @@ def replay(I, Ke, aol, dh, Q):
         for k in range(Cs[-1]):
             cv[rap_dest[k]] = np.dot(pv[rap_p[rap_off[k]:rap_off[k + 1]]],
                                      apv[rap_ap[rap_off[k]:rap_off[k + 1]]])
+        # entries below the diagonal come from their mirrors (k_mirror_copy)
+        mdst, msrc = I.get("rap_mdst", l), I.get("rap_msrc", l)
+        assert np.intersect1d(mdst, msrc).size == 0
+        cv[mdst] = cv[msrc]
         cinv = np.zeros(nc)
         for Ir in range(nc):
             w = (Cs[Ir // 64 + 1] - Cs[Ir // 64]) // 64
