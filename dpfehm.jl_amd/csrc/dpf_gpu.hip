@@ -187,7 +187,7 @@ struct LevelDev {
   int64_t nc = 0;  // rows of the next level
   DBuf p_slot, p_self, rs_src, ap_off, ap_a, ap_p, rap_off, rap_dest, rap_p, rap_ap;
   DBuf rap_mdst, rap_msrc;
-  int64_t n_mirror = 0;
+  int64_t n_rap = 0, n_mirror = 0;  // Galerkin lists (upper entries), mirrored entries
   int ap_lg = 0, rap_lg = 0;
   int64_t ap_nnz = 0;
 };
@@ -367,6 +367,8 @@ void ensure_hier(dpf_topology* t, Hier<T>& h) {
     LevelNum<T>& n = h.lev[l];
     size_t nr = (size_t)d.A.nrows;
     if (!(l == 0 && sizeof(T) == sizeof(double))) n.val.alloc(d.A.entries * sizeof(T));
+    // coarse operators: the Galerkin refresh never writes SELL padding
+    if (l > 0) HIPTRY(hipMemsetAsync(n.val.p, 0, d.A.entries * sizeof(T), nullptr));
     if (!(l == 0 && sizeof(T) == sizeof(double))) n.invd.alloc(nr * sizeof(T));
     if (d.transfer) {
       n.pval.alloc(d.P.entries * sizeof(T));
@@ -380,6 +382,8 @@ void ensure_hier(dpf_topology* t, Hier<T>& h) {
     n.r.alloc(nr * sizeof(T));
     if (l > 0) n.b.alloc(nr * sizeof(T));
   }
+  // the zeroing above (null stream) completes before any stream uses the levels
+  HIPTRY(hipStreamSynchronize(nullptr));
   int64_t nc = t->lev[L - 1]->A.nrows;
   if (t->H.coarse_dense) h.W.alloc((size_t)nc * nc * sizeof(double));
   h.status.alloc(sizeof(int));
@@ -446,8 +450,8 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
                        d.ap_nnz, d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr,
                        d.ap_a.as<int32_t>(), d.ap_p.as<int32_t>(), (const T*)A.val,
                        (const T*)P.val, apv);
-    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(dn.A.entries << d.rap_lg)), dim3(BLK), 0,
-                       s, dn.A.entries, d.rap_lg, d.rap_off.as<int64_t>(),
+    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.n_rap << d.rap_lg)), dim3(BLK), 0,
+                       s, d.n_rap, d.rap_lg, d.rap_off.as<int64_t>(),
                        d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(), d.rap_ap.as<int32_t>(),
                        (const T*)P.val, (const T*)apv, cval);
     if (d.n_mirror > 0)
@@ -944,6 +948,7 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
         upload(d->rap_mdst, lh.rap_mdst);
         upload(d->rap_msrc, lh.rap_msrc);
         d->n_mirror = (int64_t)lh.rap_mdst.size();
+        d->n_rap = (int64_t)lh.rap_dest.size();
         d->ap_lg = lh.ap_lg;
         d->rap_lg = lh.rap_lg;
         upload(d->rap_p, lh.rap_p);

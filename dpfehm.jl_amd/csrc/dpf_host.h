@@ -66,13 +66,14 @@ struct LevelHost {
   std::vector<int32_t> ap_col;
   std::vector<int64_t> ap_off;
   std::vector<int32_t> ap_a, ap_p;
-  // next level's A, one contribution list per SELL entry, lists in row-major
-  // order of (row, slot) so that neighbouring threads share fine rows:
+  // next level's A.  The Galerkin operator of a symmetric A is symmetric, so
+  // only its entries with column >= row are summed, one contribution list
+  // each, lists in row-major order of (row, slot) so that neighbouring
+  // threads share fine rows:
   //   Ac[rap_dest[k]] = sum_{q in [rap_off[k], rap_off[k+1])} P.val[rap_p[q]] * AP[rap_ap[q]]
-  // (padding entries have empty lists).  The Galerkin operator of a
-  // symmetric A is symmetric, so only entries with column >= row have
-  // products; each entry below the diagonal is then copied from its mirror:
+  // and each entry below the diagonal is copied from its mirror:
   //   Ac[rap_mdst[m]] = Ac[rap_msrc[m]]
+  // (SELL padding is never written: zero from allocation on)
   std::vector<int64_t> rap_off;
   std::vector<int32_t> rap_dest, rap_p, rap_ap;
   std::vector<int32_t> rap_mdst, rap_msrc;

@@ -447,18 +447,16 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       next.diag_slot[I] = (uint16_t)(std::lower_bound(b, e, (int32_t)I) - b);
     }
     const SellHost& C = next.A;
-    int64_t nent = C.entries();
-    int64_t nrm = C.nslices() * SELL_C;
-    std::vector<int64_t> rm_base(nrm + 1, 0);
-    for (int64_t I = 0; I < nrm; I++) {
-      int64_t sl = I / SELL_C;
-      rm_base[I + 1] = rm_base[I] + (C.sptr[sl + 1] - C.sptr[sl]) / SELL_C;
-    }
+    // one contribution list per entry on or above the diagonal, in row-major
+    // order (row I's such entries are its slots diag_slot[I] ..)
+    std::vector<int64_t> up_base(nc + 1, 0);
+    for (int64_t I = 0; I < nc; I++) up_base[I + 1] = up_base[I] + (rp[I + 1] - rp[I]) - next.diag_slot[I];
+    int64_t nent = up_base[nc];
     L.rap_dest.resize(nent);
 #pragma omp parallel for schedule(static)
-    for (int64_t I = 0; I < nrm; I++)
-      for (int64_t k = 0; k < rm_base[I + 1] - rm_base[I]; k++)
-        L.rap_dest[rm_base[I] + k] = (int32_t)C.idx(I, (int)k);
+    for (int64_t I = 0; I < nc; I++)
+      for (int64_t u = up_base[I]; u < up_base[I + 1]; u++)
+        L.rap_dest[u] = (int32_t)C.idx(I, (int)(next.diag_slot[I] + (u - up_base[I])));
     L.rap_off.assign(nent + 1, 0);
 #pragma omp parallel
     {
@@ -466,10 +464,11 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
 #pragma omp for schedule(dynamic, 512)
       for (int64_t I = 0; I < nc; I++) {
         for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]);
+        int64_t ub = up_base[I] - next.diag_slot[I];
         for (int t = 0; t < R.rowlen[I]; t++) {
           int32_t i = R.at(I, t);
           for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
-            if (L.ap_col[a] >= I) L.rap_off[rm_base[I] + slot_of[L.ap_col[a]] + 1]++;
+            if (L.ap_col[a] >= I) L.rap_off[ub + slot_of[L.ap_col[a]] + 1]++;
         }
       }
     }
@@ -484,9 +483,10 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       std::vector<int64_t> fill;
 #pragma omp for schedule(dynamic, 512)
       for (int64_t I = 0; I < nc; I++) {
-        int64_t b0 = rm_base[I], b1 = rm_base[I + 1];
+        int64_t b0 = up_base[I], b1 = up_base[I + 1];
+        int ds = next.diag_slot[I];
         fill.assign(L.rap_off.begin() + b0, L.rap_off.begin() + b1);
-        for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]);
+        for (int64_t k = rp[I]; k < rp[I + 1]; k++) slot_of[ci[k]] = (int32_t)(k - rp[I]) - ds;
         for (int t = 0; t < R.rowlen[I]; t++) {
           int32_t i = R.at(I, t);
           int32_t pe = L.r_src[R.idx(I, t)];
@@ -534,7 +534,7 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
       while ((1 << lg) * 4 < avg && lg < 5) lg++;
       return lg;
     };
-    L.rap_lg = lanes(nprod, C.nnz);
+    L.rap_lg = lanes(nprod, nent);
     L.ap_lg = lanes(L.ap_off.back(), (int64_t)L.ap_col.size());
   }
   return "";

@@ -109,15 +109,17 @@ def replay(I, Ke, aol, dh, Q):
         nc = Cl.size
         rap_off, rap_p, rap_ap = I.get("rap_off", l), I.get("rap_p", l), I.get("rap_ap", l)
         rap_dest = I.get("rap_dest", l)
-        assert rap_off.size == Cs[-1] + 1
-        assert np.array_equal(np.sort(rap_dest), np.arange(Cs[-1]))  # every entry exactly once
+        mdst, msrc = I.get("rap_mdst", l), I.get("rap_msrc", l)
+        assert rap_off.size == rap_dest.size + 1
+        # every stored (non-padding) entry exactly once: summed or mirrored
+        real = np.array([sell_idx(Cs, r, k_) for r in range(Cl.size) for k_ in range(Cl[r])])
+        assert np.array_equal(np.sort(np.concatenate([rap_dest, mdst])), np.sort(real))
+        assert np.isin(msrc, rap_dest).all()
         cv = np.zeros(Cs[-1])
-        for k in range(Cs[-1]):
+        for k in range(rap_dest.size):
             cv[rap_dest[k]] = np.dot(pv[rap_p[rap_off[k]:rap_off[k + 1]]],
                                      apv[rap_ap[rap_off[k]:rap_off[k + 1]]])
         # entries below the diagonal come from their mirrors (k_mirror_copy)
-        mdst, msrc = I.get("rap_mdst", l), I.get("rap_msrc", l)
-        assert np.intersect1d(mdst, msrc).size == 0
         cv[mdst] = cv[msrc]
         cinv = np.zeros(nc)
         for Ir in range(nc):
