@@ -2053,11 +2053,17 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     int iters = 0, status = DPF_OK;
     for (int c = (N - 1) / C; c >= 0 && status == DPF_OK; c--) {
       int a = c * C, b = std::min(N, (c + 1) * C);
-      // recompute u_{a..b} from checkpoint a into the segment buffer
+      // u_{a..b} into the segment buffer: u_a and, when b closes a full
+      // segment, u_b from the checkpoints; the states in between recomputed
+      // (bitwise what the forward produced: same steps, same warm starts)
       copy_dev(segp, st->ckpt.as<double>() + (size_t)c * nf, nf * sizeof(double), s);
-      for (int k = a + 1; k <= b && status == DPF_OK; k++)
+      bool b_stored = b == (c + 1) * C;
+      for (int k = a + 1; k < (b_stored ? b : b + 1) && status == DPF_OK; k++)
         status = be_step(t, s, st, k, segp + (size_t)(k - a - 1) * nf,
                          segp + (size_t)(k - a) * nf, o, iters);
+      if (b_stored)
+        copy_dev(segp + (size_t)(b - a) * nf, st->ckpt.as<double>() + (size_t)(c + 1) * nf,
+                 nf * sizeof(double), s);
       for (int k = b; k > a && status == DPF_OK; k--) {
         hipLaunchKernelGGL(k_be_adj_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                            (const double*)st->svec.as<double>(),
