@@ -281,6 +281,8 @@ def main():
         "iterations_fp64_fwd_adj": [results["fp64"]["stats"][0][0]["iterations"],
                                     results["fp64"]["stats"][0][1]["iterations"]],
         "topology_setup_s": round(topo_s, 3),
+        "numeric_setup_ms_per_solve": {
+            k: round(results[k]["stats"][-1][0]["setup_ms"], 3) for k in ("fp64", "mixed")},
         "cheb_tuned_finest_coarse_ratio": tuned,
         "cheb_pinned": bool(args.tuning),
         "hipgraph_iterations": bool(args.graph),
