@@ -2136,13 +2136,17 @@ int twophase_checks(const dpf_topology* t) {
 // With t->Ke = K_a + K_b, t->dh and t->Q in place: pressure P (device, n) of
 // the transmissivity system, shifted to mean zero if asked, and fluxes Vn
 // (device, ne).
+// warm (nullable, nf): an initial guess for the free pressures when
+// have_warm (the previous IMPES step's), overwritten with this step's.
 int tpfa_core(dpf_topology* t, hipStream_t s, const dpf_opts* o, dpf_stats* st, double* P,
-              double* Vn, int zero_mean) {
+              double* Vn, int zero_mean, double* warm = nullptr, bool have_warm = false) {
   int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
   prepare(t, s, o->precision, t->Q.as<double>(), true);
   DBuf hf(t->pool.get());
   hf.alloc(nf * sizeof(double));
-  int status = solve_system(t, s, t->b64.as<double>(), hf.as<double>(), o, st);
+  int status = solve_system(t, s, t->b64.as<double>(), hf.as<double>(), o, st,
+                            have_warm ? (const double*)warm : nullptr);
+  if (warm) copy_dev(warm, hf.p, nf * sizeof(double), s);
   hipLaunchKernelGGL(k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
                      (const double*)hf.as<double>(), t->dh.as<double>(), P);
   if (zero_mean) {
@@ -2398,6 +2402,9 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
       copy_dev(st->Q.p, Qd, n * sizeof(double), s);
       copy_dev(st->dh.p, t->dh.p, n * sizeof(double), s);
     }
+    // each pressure solve starts from the previous step's pressures
+    DBuf hwarm(t->pool.get());
+    hwarm.alloc(t->H.nf * sizeof(double));
     for (int32_t step = 0; step < nt && status == DPF_OK; step++) {
       // total mobility times K, arithmetic mean per edge, pressure and fluxes,
       // then the saturation transport over dt
@@ -2406,7 +2413,7 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
       hipLaunchKernelGGL(k_tp_ksum, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
                          t->edges.as<int2>(), (const double*)mb.as<double>(),
                          t->Ke.as<double>());
-      status = tpfa_core(t, s, o, stats, Pd, Vd, zero_mean);
+      status = tpfa_core(t, s, o, stats, Pd, Vd, zero_mean, hwarm.as<double>(), step > 0);
       if (stats) iters += stats->iterations;
       if (status != DPF_OK) break;
       if (st) {
@@ -2587,7 +2594,9 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
       prepare(t, s, o->precision, nullptr, false);
       hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                          t->f2n.as<int32_t>(), (const double*)dP.as<double>(), rhs.as<double>());
-      status = solve_system(t, s, rhs.as<double>(), lam.as<double>(), o, stats);
+      // warm-started from the later step's multiplier (lam holds it)
+      status = solve_system(t, s, rhs.as<double>(), lam.as<double>(), o, stats,
+                            step < nt - 1 ? (const double*)lam.as<double>() : nullptr);
       if (stats) iters += stats->iterations;
       hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                          t->f2n.as<int32_t>(), Pt, hf.as<double>());
