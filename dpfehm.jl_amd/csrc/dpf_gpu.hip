@@ -2508,8 +2508,8 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
     int64_t maxnts = 0;
     for (int64_t k : st->tp_nts) maxnts = std::max(maxnts, k);
     BufPool* pl = t->pool.get();
-    DBuf dS(pl), dK(pl), seedb(pl), w(pl), dabs(pl), du(pl), dR(pl), dVn(pl), dKsum(pl), dP(pl),
-        rhs(pl), lam(pl), hf(pl), part(pl), tS(pl), tf(pl);
+    DBuf dS(pl), dK(pl), seedb(pl), w(pl), dabs(pl), du(pl), dR(pl), dR2(pl), dVn(pl), dKsum(pl),
+        dP(pl), rhs(pl), lam(pl), hf(pl), part(pl), tS(pl), tf(pl);
     unsigned nb = nblocks(n);
     dS.alloc(n * sizeof(double));
     dK.alloc(n * sizeof(double));
@@ -2517,6 +2517,7 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
     dabs.alloc(std::max<int64_t>(ninc, 1) * sizeof(double));
     du.alloc(n * sizeof(double));
     dR.alloc(n * sizeof(double));
+    dR2.alloc(n * sizeof(double));
     dVn.alloc(ne * sizeof(double));
     dKsum.alloc(ne * sizeof(double));
     dP.alloc(n * sizeof(double));
@@ -2561,16 +2562,20 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
                              (const double*)w.as<double>(), (const double*)(S0k + k * n),
                              (const double*)(f0k + k * n), S0k + (k + 1) * n, f0k + (k + 1) * n,
                              Qd, pv, tau, f, lo, hi);
-        for (int64_t k = nts - 1; k >= 0; k--) {
-          hipLaunchKernelGGL(k_tp_adj_a, dim3(nb), dim3(BLK), 0, s, n, ip, io,
-                             (const double*)w.as<double>(), (const double*)(S0k + k * n),
-                             (const double*)(f0k + k * n), Qd, pv, tau, lo, hi,
-                             (const double*)dS.as<double>(), du.as<double>(), dR.as<double>(),
-                             dabs.as<double>());
-          hipLaunchKernelGGL(k_tp_adj_b, dim3(nb), dim3(BLK), 0, s, n, ip, io,
-                             t->tp_mirror.as<int32_t>(), (const double*)w.as<double>(),
-                             (const double*)(S0k + k * n), Qd, f, (const double*)du.as<double>(),
-                             (const double*)dR.as<double>(), dS.as<double>());
+        // launch j: part 2 of sub-step j (j < nts) and part 1 of sub-step
+        // j - 1 (j > 0), for j = nts .. 0; dR ping-pongs between two buffers
+        double* dRs[2] = {dR.as<double>(), dR2.as<double>()};
+        int cur = 0;  // buffer holding dR of the sub-step part 2 reads
+        for (int64_t j = nts; j >= 0; j--) {
+          int do_b = j < nts, do_a = j > 0;
+          const double* Sb = S0k + (do_b ? j : 0) * n;
+          const double* Sa = S0k + (do_a ? j - 1 : 0) * n;
+          const double* fa = f0k + (do_a ? j - 1 : 0) * n;
+          hipLaunchKernelGGL(k_tp_adj, dim3(nb), dim3(BLK), 0, s, n, ip, io,
+                             t->tp_mirror.as<int32_t>(), (const double*)w.as<double>(), Qd, pv, f,
+                             tau, lo, hi, do_b, do_a, Sb, (const double*)dRs[cur], Sa, fa,
+                             du.as<double>(), dRs[cur ^ 1], dabs.as<double>(), dS.as<double>());
+          cur ^= 1;
         }
       }
       hipLaunchKernelGGL(k_tp_flux_adj, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,

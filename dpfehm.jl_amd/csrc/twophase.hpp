@@ -231,43 +231,95 @@ __device__ __forceinline__ void tp_slopes(double s, const Fluid& f, double& mt, 
   dfds = (dmw * mo - mw * dmo) / (mt * mt);
 }
 
-// Reverse of one sub-step, part 1 (per cell c): the forward update is
-// recomputed from the taped S and f to find the clamp's activity; du = dS'
+// Reverse sub-steps, per cell c.  Sub-step k maps (S_k, f_k) to S_{k+1}.
+//
+// Part 2 of sub-step k (do_b): df_c from the cell's own terms (mass term,
+// and the edges where c is downwind) and from the neighbours' terms that
+// take f_c (c upwind of them: the mirror entry's weight is non-negative),
+// then dS_k = du_k + f'(S_k) df_c.  Inputs S_k (Sb), du_k and dR_k (dRb, read
+// at the neighbours too).
+//
+// Part 1 of sub-step k - 1 (do_a): the forward update recomputed from the
+// taped S_{k-1}, f_{k-1} (Sa, fa) to find the clamp's activity; du = dS_k
 // where lo <= u <= hi, dR = du dtx; dabs[p] += d(term)/d|Vn| dR_c over the
-// cell's incidence (f of the upwind cell, or -f_c).
-__global__ void k_tp_adj_a(int64_t n, const int64_t* __restrict__ inc_ptr,
-                           const int32_t* __restrict__ inc_other, const double* __restrict__ w,
-                           const double* __restrict__ S, const double* __restrict__ fw,
-                           const double* __restrict__ Q, const double* __restrict__ pv,
-                           double tau, double lo, double hi, const double* __restrict__ dSn,
-                           double* __restrict__ du, double* __restrict__ dR,
-                           double* __restrict__ dabs) {
+// cell's incidence (f of the upwind cell, or -f_c).  It needs dS_k at c
+// only, so the two parts run in one pass: dS_k stays in a register.  With
+// do_b false, dS_k is read from dS; with do_a false, it is written there.
+//
+// du is per cell (read by part 2, then rewritten by part 1 at the same
+// cell); dR is double-buffered (dRb read at the neighbours, dRa written).
+// The arithmetic of each part, and its order, are those of the separate
+// passes.
+__global__ void k_tp_adj(int64_t n, const int64_t* __restrict__ inc_ptr,
+                         const int32_t* __restrict__ inc_other,
+                         const int32_t* __restrict__ mirror, const double* __restrict__ w,
+                         const double* __restrict__ Q, const double* __restrict__ pv, Fluid f,
+                         double tau, double lo, double hi, int do_b, int do_a,
+                         const double* __restrict__ Sb, const double* __restrict__ dRb,
+                         const double* __restrict__ Sa, const double* __restrict__ fa,
+                         double* __restrict__ du, double* __restrict__ dRa,
+                         double* __restrict__ dabs, double* __restrict__ dS) {
   int64_t c = logical_row();
   if (c >= n) return;
-  double fc = fw[c];
   double q = Q[c];
   double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
   double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
-  double sc = S[c], pvc = pv[c], dsn = dSn[c];
   int64_t p0 = inc_ptr[c], p1 = inc_ptr[c + 1];
+  const bool one = p1 - p0 <= 8;  // the whole incidence in one batch (3-D grids)
   constexpr int U = 8;
-  double wv[U], fo[U];
-  // the cell's incidence in batches of eight (one batch on a 3-D grid): the
-  // forward update recomputed in the forward's order, then the tangents
+  double wv[U];
+  int32_t o[U];
   auto load = [&](int64_t b) {
-    int32_t k[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       bool ok = b + u < p1;
       wv[u] = ok ? w[b + u] : 0.0;
-      k[u] = ok ? inc_other[b + u] : 0;
+      o[u] = ok ? inc_other[b + u] : 0;
     }
+  };
+  if (one) load(p0);
+  double dsn;
+  if (do_b) {
+    double rk = dRb[c];
+    double df = qneg * rk;
+    for (int64_t b = p0; b < p1; b += U) {
+      if (!one) load(b);
+      double wm[U], ro[U];
+      int32_t m[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) fo[u] = b + u < p1 ? fw[k[u]] : 0.0;
+      for (int u = 0; u < U; u++) m[u] = b + u < p1 ? mirror[b + u] : 0;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        bool ok = b + u < p1;
+        wm[u] = ok ? w[m[u]] : -1.0;
+        ro[u] = ok ? dRb[o[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (signbit(wv[u])) df += wv[u] * rk;
+        if (!signbit(wm[u])) df += wm[u] * ro[u];
+      }
+    }
+    double mt, dmt, dfds;
+    tp_slopes(Sb[c], f, mt, dmt, dfds);
+    dsn = du[c] + dfds * df;
+    if (!do_a) {
+      dS[c] = dsn;
+      return;
+    }
+  } else {
+    dsn = dS[c];
+  }
+  double fc = fa[c], sc = Sa[c], pvc = pv[c];
+  double fo[U];
+  auto gather_f = [&](int64_t b) {
+#pragma unroll
+    for (int u = 0; u < U; u++) fo[u] = b + u < p1 ? fa[o[u]] : 0.0;
   };
   double R = qneg * fc;
   for (int64_t b = p0; b < p1; b += U) {
-    load(b);
+    if (!one) load(b);
+    gather_f(b);
     {
 #pragma clang fp contract(off)
 #pragma unroll
@@ -276,17 +328,20 @@ __global__ void k_tp_adj_a(int64_t n, const int64_t* __restrict__ inc_ptr,
     }
   }
   double dtx = tau / pvc;
-  double u;
+  double uu;
   {
 #pragma clang fp contract(off)
-    u = sc + (R * dtx + qpos * dtx);
+    uu = sc + (R * dtx + qpos * dtx);
   }
-  double d = (u >= lo && u <= hi) ? dsn : 0.0;
+  double d = (uu >= lo && uu <= hi) ? dsn : 0.0;
   double r = d * dtx;
   du[c] = d;
-  dR[c] = r;
+  dRa[c] = r;
   for (int64_t b = p0; b < p1; b += U) {
-    if (p1 - p0 > U) load(b);
+    if (!one) {
+      load(b);
+      gather_f(b);
+    }
     double old[U];
 #pragma unroll
     for (int u2 = 0; u2 < U; u2++) old[u2] = b + u2 < p1 ? dabs[b + u2] : 0.0;
@@ -294,51 +349,6 @@ __global__ void k_tp_adj_a(int64_t n, const int64_t* __restrict__ inc_ptr,
     for (int u2 = 0; u2 < U; u2++)
       if (b + u2 < p1) dabs[b + u2] = old[u2] + (signbit(wv[u2]) ? -fc : fo[u2]) * r;
   }
-}
-
-// Reverse of one sub-step, part 2 (per cell k): df_k from its own terms
-// (mass term, and the edges where k is downwind) and from the neighbours'
-// terms that take f_k (k upwind of them: the mirror entry's weight is
-// non-negative), then dS_k = du_k + f'(S_k) df_k into dS.
-__global__ void k_tp_adj_b(int64_t n, const int64_t* __restrict__ inc_ptr,
-                           const int32_t* __restrict__ inc_other,
-                           const int32_t* __restrict__ mirror, const double* __restrict__ w,
-                           const double* __restrict__ S, const double* __restrict__ Q, Fluid f,
-                           const double* __restrict__ du, const double* __restrict__ dR,
-                           double* __restrict__ dS) {
-  int64_t k = logical_row();
-  if (k >= n) return;
-  double q = Q[k];
-  double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
-  double rk = dR[k];
-  double df = qneg * rk;
-  int64_t p0 = inc_ptr[k], p1 = inc_ptr[k + 1];
-  constexpr int U = 8;
-  for (int64_t b = p0; b < p1; b += U) {
-    double wv[U], wm[U], ro[U];
-    int32_t m[U], o[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      bool ok = b + u < p1;
-      wv[u] = ok ? w[b + u] : 0.0;
-      m[u] = ok ? mirror[b + u] : 0;
-      o[u] = ok ? inc_other[b + u] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      bool ok = b + u < p1;
-      wm[u] = ok ? w[m[u]] : -1.0;
-      ro[u] = ok ? dR[o[u]] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (signbit(wv[u])) df += wv[u] * rk;
-      if (!signbit(wm[u])) df += wm[u] * ro[u];
-    }
-  }
-  double mt, dmt, dfds;
-  tp_slopes(S[k], f, mt, dmt, dfds);
-  dS[k] = du[k] + dfds * df;
 }
 
 // Per edge: dVn_e = sign(Vn_e) (dabs at a + dabs at b); dKsum_e += dVn_e
