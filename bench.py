@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0,
                     help="steady2m: standard deviation of the logK fields (the conditioning "
                          "sweep runs 1, 2 and 3; the headline line is sigma 1)")
+    ap.add_argument("--inner-reltol", type=float, default=1e-5,
+                    help="steady2m: relative tolerance of each FP32 inner solve of the mixed "
+                         "precision's refinement")
     ap.add_argument("--graph", action="store_true",
                     help="steady2m and ensemble: replay each PCG iteration as a captured "
                          "hipGraph (DPF_FLAG_GRAPH)")
@@ -122,8 +125,8 @@ def main():
     dlogK_d = torch.empty(w.n, dtype=torch.float64, device=dev)
 
     def opts(precision, timing=False):
-        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph},
-                      memory=_lib.DPF_DEVICE, stream=sptr)
+        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph,
+                       "inner_reltol": args.inner_reltol}, memory=_lib.DPF_DEVICE, stream=sptr)
         if timing:
             o.flags = _lib.DPF_FLAG_SWEEP_TIMING
         return o
@@ -268,7 +271,7 @@ def main():
                         "4 injectors, "
                         "200 observations, AMG-PCG to 1e-10, mixed precision",
             "cells": w.n, "edges": w.nedges, "nfree": topo.nfree, "nnz": topo.nnz,
-            "amg_levels": topo.levels, "reltol": 1e-10, "inner_reltol": 1e-5,
+            "amg_levels": topo.levels, "reltol": 1e-10, "inner_reltol": args.inner_reltol,
         },
         "fp64_evals_per_s": round(fp64_rate, 3),
         "mixed_over_fp64": round(value / fp64_rate, 3),
