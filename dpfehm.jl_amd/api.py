@@ -20,9 +20,12 @@ recomputes it in every call.
 
 Keyword arguments: ``reltol`` (default 1e-10), ``abstol``, ``maxiter``,
 ``precision`` ("fp64", "mixed" or "mixed_pc"), ``inner_reltol``, ``max_outer``,
-``cheb_degree``.  ``linear_solver`` is accepted and ignored, as the
-reference's rrule ignores it (src/groundwater.jl:90).  Unknown keywords raise
-TypeError.  Non-convergence raises ``NotConverged`` instead of returning
+``cheb_degree``, ``graph``.  ``linear_solver`` selects amg_solver (the
+default: AMG-PCG to ``reltol``) or cholesky_solver (the same AMG-PCG to a
+relative residual of 1e-14, a direct solve's accuracy; tolerance keywords
+ignored), as a function or by name; the reference's rrule always uses AMG
+(src/groundwater.jl:90), and here the rrule honours the choice.  Unknown
+keywords raise TypeError.  Non-convergence raises ``NotConverged`` instead of returning
 silently.
 """
 import ctypes
@@ -62,11 +65,34 @@ _OPT_KEYS = {"reltol", "abstol", "maxiter", "precision", "inner_reltol", "max_ou
              "cheb_degree"}
 
 
+def amg_solver(A=None, b=None, **kwargs):
+    """Marker for ``linear_solver=`` (src/groundwater.jl:45-49), the default:
+    AMG-preconditioned CG to ``reltol`` on the GPU.  Not callable on its own."""
+    raise TypeError("amg_solver selects the GPU solve through linear_solver=; it is not "
+                    "called directly")
+
+
+def cholesky_solver(A=None, b=None, **kwargs):
+    """Marker for ``linear_solver=`` (src/groundwater.jl:56-60).  There is no
+    sparse Cholesky on the GPU: the system is solved by the same AMG-PCG to a
+    relative residual of 1e-14, the accuracy a direct solve delivers, and
+    tolerance keyword arguments are ignored as the reference's Cholesky
+    ignores them.  Not callable on its own."""
+    raise TypeError("cholesky_solver selects the GPU solve through linear_solver=; it is not "
+                    "called directly")
+
+
 def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
     """dpf_opts from keyword arguments; ``graph=True`` sets DPF_FLAG_GRAPH
-    (each PCG iteration replayed as one captured hipGraph)."""
+    (each PCG iteration replayed as one captured hipGraph); ``linear_solver``
+    is amg_solver (default) or cholesky_solver, as a function or by name."""
     kw = dict(kwargs)
-    kw.pop("linear_solver", None)
+    ls = kw.pop("linear_solver", None)
+    name = getattr(ls, "__name__", ls)
+    if name not in (None, "amg_solver", "cholesky_solver"):
+        raise TypeError(f"linear_solver must be amg_solver or cholesky_solver, not {ls!r}")
+    if name == "cholesky_solver":
+        kw["reltol"], kw["abstol"] = 1e-14, 0.0
     graph = bool(kw.pop("graph", False))
     bad = set(kw) - _OPT_KEYS
     if bad:
@@ -216,9 +242,10 @@ class State:
 
 def groundwater_steadystate(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
                             linear_solver=None, stats=None, **kwargs):
-    """src/groundwater.jl:75-83.  Returns h (n,)."""
+    """src/groundwater.jl:75-83.  Returns h (n,).  ``linear_solver``:
+    amg_solver (default) or cholesky_solver."""
     h, _ = _steady(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs, False,
-                   stats, kwargs)
+                   stats, dict(kwargs, linear_solver=linear_solver))
     return h
 
 

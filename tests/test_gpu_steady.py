@@ -503,3 +503,24 @@ def test_pinned_tuning(pkg, api):
     h_auto = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
                                          w.dirichleths, w.Qs, reltol=1e-12)
     assert rel(hs[0], h_auto) < 1e-10
+
+
+def test_linear_solver_choice(pkg, api, oracle):
+    """linear_solver=cholesky_solver (src/groundwater.jl:56-60) solves to direct-solve
+    accuracy (relative residual 1e-14, tolerance keywords ignored); amg_solver is the
+    default; anything else is refused."""
+    _say("linear_solver")
+    w = pkg.workloads.steady_2m(ns=(24, 20, 10), maxs=(12.0, 10.0, 5.0))
+    K = w.Ks()
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs)
+    h_ref = oracle.groundwater_steadystate(*args, reltol=1e-14)
+    st = {}
+    h = api.groundwater_steadystate(*args, linear_solver=api.cholesky_solver, reltol=1e-3,
+                                    stats=st)
+    st2 = {}
+    h2 = api.groundwater_steadystate(*args, linear_solver="amg_solver", stats=st2)
+    assert st["relres"] <= 1e-14 < st2["relres"] <= 1e-10
+    assert rel(h, h_ref) < 1e-11 and rel(h2, h_ref) < 1e-8
+    assert rel(h, h_ref) < rel(h2, h_ref)
+    with pytest.raises(TypeError):
+        api.groundwater_steadystate(*args, linear_solver=print)
