@@ -619,9 +619,9 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       c2.make(2 * m, 1.0, ch.ratio);
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
-        sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl, dd,
-                           invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse, bm_ptr(t),
-                           part);
+        sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
+                     (const T*)dd, k == 2 * m - 1 ? (T*)nullptr : dd, invd, (T)c2.c1[k],
+                     (T)c2.c2[k], (const int*)done, fuse, bm_ptr(t), part);
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
       }
@@ -631,8 +631,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   }
   // pre-smoothing (step 0 done by the producer of b)
   for (int k = 1; k < m; k++) {
-    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
-                       (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0, bm_ptr(t), part);
+    // the last pre-smoothing sweep's d is dead: the post-smoothing restarts at c1 = 0
+    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
+                 (const T*)dd, k == m - 1 ? (T*)nullptr : dd, invd, (T)ch.c1[k], (T)ch.c2[k],
+                 (const int*)done, 0, bm_ptr(t), part);
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
@@ -664,8 +666,9 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     double c1 = k == 0 ? 0.0 : ch.c1[k];
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
-    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl, dd, invd,
-                       (T)c1, (T)c2, (const int*)done, fuse, bm_ptr(t), part);
+    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
+                 (const T*)dd, k == m - 1 ? (T*)nullptr : dd, invd, (T)c1, (T)c2,
+                 (const int*)done, fuse, bm_ptr(t), part);
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
   }
@@ -1784,10 +1787,13 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
     T* xa = (k & 1) ? n0.xs.template as<T>() : n0.x.template as<T>();
     T* xb = (k & 1) ? n0.x.template as<T>() : n0.xs.template as<T>();
     if (kernel == 0) {
+      // as in the V-cycle with finest degree 1: the post-smoothing sweep after
+      // the coarse correction (c1 = 0: no d read, d dead: not written), with
+      // the fused r.z partials
       hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, 0, nf, (const T*)xa, xb,
-                         (const T*)h.cr.template as<T>(), n0.d.template as<T>(),
-                         (const T*)level_invd(t, h, 0), (T)ch.c1[1], (T)ch.c2[1], nodone, 0,
-                         bm_ptr(t), part);
+                         (const T*)h.cr.template as<T>(), (const T*)nullptr, (T*)nullptr,
+                         (const T*)level_invd(t, h, 0), (T)0, (T)ch.c0, nodone, 1, bm_ptr(t),
+                         part);
     } else if (kernel == 1) {
       hipLaunchKernelGGL((k_cg_pq<T, T>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
@@ -1809,8 +1815,8 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
   HIPTRY(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (kernel == 0)
-    *bytes = (double)nnz * w + cb + 6.0 * (double)nf * w;
+  if (kernel == 0)  // values (+ columns), reads of xin, b, invd, the write of xout
+    *bytes = (double)nnz * w + cb + 4.0 * (double)nf * w;
   else if (kernel == 1)
     *bytes = (double)nnz * w + cb + 4.0 * (double)nf * w;
   else

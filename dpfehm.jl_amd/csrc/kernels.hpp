@@ -940,12 +940,15 @@ __device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int l
 // One Chebyshev step of the l1-Jacobi-preconditioned level operator:
 //   d <- c1 d + c2 invd (b - A xin);  xout = xin + d,
 // A given with its rows split over 2^lg lanes (lg = 0 on the finest level).
+// d is read from din (only when c1 != 0: not on the first sweep after a
+// coarse correction) and written to dout (null on a level's last pre- and
+// post-smoothing sweeps, whose d no later step reads).
 // With fuse_dot, also writes the block partials of b . xout (the fine-level
 // sweep that completes z = M^-1 r, where b is the CG residual).
 // FINE marks the finest level's instantiation (a separate symbol in profiles).
 template <typename T, bool FINE>
 __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
-                       T* __restrict__ xout, const T* __restrict__ b, T* __restrict__ d,
+                       T* __restrict__ xout, const T* __restrict__ b, const T* din, T* dout,
                        const T* __restrict__ invd, T c1, T c2, const int* __restrict__ done,
                        int fuse_dot, const int2* __restrict__ bm, double* __restrict__ partial) {
   if (*done) return;
@@ -955,10 +958,9 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   double local = 0.0;
   if ((sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n) {
     T br = b[r];
-    // c1 = 0 on the first sweep after a coarse correction: d is not read
     T dn = c2 * invd[r] * (br - ax);
-    if (c1 != (T)0) dn = c1 * d[r] + dn;
-    d[r] = dn;
+    if (c1 != (T)0) dn = c1 * din[r] + dn;
+    if (dout) dout[r] = dn;
     T xo = xin[r] + dn;
     xout[r] = xo;
     local = (double)br * (double)xo;
