@@ -953,15 +953,19 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
                        int fuse_dot, const int2* __restrict__ bm, double* __restrict__ partial) {
   if (*done) return;
   int64_t sub = logical_row();
-  T ax = split_row_dot(A, sub, lg, xin);
   int64_t r = sub >> lg;
+  // the row's own vector entries are loaded before the row sum, so that they
+  // travel with the operator's loads instead of after them
+  const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n;
+  T br = own ? b[r] : (T)0, ir = own ? invd[r] : (T)0, xr = own ? xin[r] : (T)0;
+  T dr = own && c1 != (T)0 ? din[r] : (T)0;
+  T ax = split_row_dot(A, sub, lg, xin);
   double local = 0.0;
-  if ((sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n) {
-    T br = b[r];
-    T dn = c2 * invd[r] * (br - ax);
-    if (c1 != (T)0) dn = c1 * din[r] + dn;
+  if (own) {
+    T dn = c2 * ir * (br - ax);
+    if (c1 != (T)0) dn = c1 * dr + dn;
     if (dout) dout[r] = dn;
-    T xo = xin[r] + dn;
+    T xo = xr + dn;
     xout[r] = xo;
     local = (double)br * (double)xo;
   }
@@ -979,9 +983,11 @@ __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ 
                            const int* __restrict__ done) {
   if (*done) return;
   int64_t sub = logical_row();
-  T ax = split_row_dot(A, sub, lg, x);
   int64_t i = sub >> lg;
-  if ((sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n) r[i] = b[i] - ax;
+  const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n;
+  T bi = own ? b[i] : (T)0;  // loaded with the operator, not after it
+  T ax = split_row_dot(A, sub, lg, x);
+  if (own) r[i] = bi - ax;
 }
 
 // bc = R rf with every row of R split over G = 2^lg consecutive lanes
@@ -1030,6 +1036,8 @@ __global__ void __launch_bounds__(BLK)
   if (*done) return;
   __shared__ double ws[COARSE_FOLD_MAX * COARSE_FOLD_MAX];
   __shared__ double bs[COARSE_FOLD_MAX], xs[COARSE_FOLD_MAX];
+  const int64_t i = logical_row();
+  const T xi = i < P.nrows ? xf[i] : (T)0;  // in flight during the coarse solve
   // all of a thread's loads of W in flight at once, then the LDS stores
   constexpr int PER = COARSE_FOLD_MAX * COARSE_FOLD_MAX / BLK;
   const int nn = nc * nc;
@@ -1047,19 +1055,18 @@ __global__ void __launch_bounds__(BLK)
   }
   __syncthreads();
   if (threadIdx.x < nc) {
-    int i = threadIdx.x;
+    int ic = threadIdx.x;
     double a = 0.0;
 #pragma unroll 8
-    for (int j = 0; j < nc; j++) a += ws[j * nc + i] * bs[j];
-    xs[i] = (double)(T)a;
+    for (int j = 0; j < nc; j++) a += ws[j * nc + ic] * bs[j];
+    xs[ic] = (double)(T)a;
   }
   __syncthreads();
-  int64_t i = logical_row();
   if (i >= P.nrows) return;
   int64_t base;
   int w;
   sell_row(P, i, base, w);
-  xf[i] += sell_dot(P, i, base, w, (const double*)xs);
+  xf[i] = xi + sell_dot(P, i, base, w, (const double*)xs);
 }
 
 // xf += P xc
@@ -1069,10 +1076,11 @@ __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ 
   if (*done) return;
   int64_t i = logical_row();
   if (i >= P.nrows) return;
+  T xi = xf[i];  // loaded with P, not after the row sum
   int64_t base;
   int w;
   sell_row(P, i, base, w);
-  xf[i] += sell_dot(P, i, base, w, xc);
+  xf[i] = xi + sell_dot(P, i, base, w, xc);
 }
 
 // Coarsest level: x = W b with the FP64 inverse W (symmetric, so column j
@@ -1288,6 +1296,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
   int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < A.nrows) {
+    double bi = sc ? b[i] * sc[mi].bscale : b[i];  // loaded with A, not after it
     double ax = 0.0;
     if (x) {
       int64_t base;
@@ -1295,7 +1304,7 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
       sell_row(A, i, base, w);
       ax = sell_dot(A, i, base, w, x);
     }
-    double rv = (sc ? b[i] * sc[mi].bscale : b[i]) - ax;
+    double rv = bi - ax;
     r[i] = rv;
     local = rv * rv;
   }
