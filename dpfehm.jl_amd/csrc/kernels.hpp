@@ -1001,6 +1001,9 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
                            const int* __restrict__ done) {
   if (*done) return;
   int64_t sr = logical_row();
+  int64_t I = sr >> lg;
+  const bool own = (sr & ((1 << lg) - 1)) == (1 << lg) - 1 && I < nc;
+  T iv = own && init ? invdc[I] : (T)0;  // loaded with R, not after the row sum
   T s = (T)0;
   if (sr < Rs.nrows) {
     int64_t base;
@@ -1009,11 +1012,10 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
     s = sell_dot(Rs, sr, base, w, rf);
   }
   s = group_sum(s, lg);
-  int64_t I = sr >> lg;
-  if ((sr & ((1 << lg) - 1)) == (1 << lg) - 1 && I < nc) {
+  if (own) {
     bc[I] = s;
     if (init) {
-      T dd = c0 * invdc[I] * s;
+      T dd = c0 * iv * s;
       dc[I] = dd;
       xc[I] = dd;
     }
@@ -1174,16 +1176,17 @@ __global__ void k_cg_pq(SellD<T> A, const TZ* __restrict__ z, const T* __restric
     const SolveScalars& m = sc[mi];
     int first = m.first;
     T beta = first ? (T)0 : (T)(m.rz / m.rz_old);
+    T zi = (T)z[i], poi = first ? (T)0 : pold[i];  // loaded with A, not after it
     int64_t base;
     int w;
     sell_row(A, i, base, w);
     T s, pn;
     if (first) {
       s = sell_dot(A, i, base, w, z);
-      pn = (T)z[i];
+      pn = zi;
     } else {
       s = sell_dot_zp(A, i, base, w, z, pold, beta);
-      pn = (T)z[i] + beta * pold[i];
+      pn = zi + beta * poi;
     }
     pnew[i] = pn;
     q[i] = s;
