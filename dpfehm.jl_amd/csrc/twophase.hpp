@@ -186,6 +186,7 @@ __global__ void k_tp_upstream(int64_t n, const int64_t* __restrict__ inc_ptr,
   double sc = S[c];
   double fc = fw[c];
   double q = Q[c];
+  double pvc = pv[c];  // loaded with the cell's other entries, not after the sum
   double qneg = (q != q) ? q : (q < 0.0 ? q : 0.0);
   double qpos = (q != q) ? q : (q > 0.0 ? q : 0.0);
   double R = qneg * fc;
@@ -206,7 +207,7 @@ __global__ void k_tp_upstream(int64_t n, const int64_t* __restrict__ inc_ptr,
     for (int u = 0; u < U; u++)
       if (b + u < p1) R = R + (signbit(wv[u]) ? fc : fo[u]) * wv[u];
   }
-  double dtx = tau / pv[c];
+  double dtx = tau / pvc;
   double s = sc + (R * dtx + qpos * dtx);
   s = s > hi ? hi : (s < lo ? lo : s);
   Snew[c] = s;
@@ -278,6 +279,8 @@ __global__ void k_tp_adj(int64_t n, const int64_t* __restrict__ inc_ptr,
     }
   };
   if (one) load(p0);
+  // part 1's own-cell entries, in flight during part 2
+  double fc = do_a ? fa[c] : 0.0, sc = do_a ? Sa[c] : 0.0, pvc = do_a ? pv[c] : 1.0;
   double dsn;
   if (do_b) {
     double rk = dRb[c];
@@ -310,7 +313,6 @@ __global__ void k_tp_adj(int64_t n, const int64_t* __restrict__ inc_ptr,
   } else {
     dsn = dS[c];
   }
-  double fc = fa[c], sc = Sa[c], pvc = pv[c];
   double fo[U];
   auto gather_f = [&](int64_t b) {
 #pragma unroll
