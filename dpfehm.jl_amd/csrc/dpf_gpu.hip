@@ -1524,9 +1524,14 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
     if (geometric)
       hipLaunchKernelGGL(k_geo_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
                          t->edges.as<int2>(), lk, out);
-    else
+    else {
+      DBuf kb(t->pool.get());
+      kb.alloc(t->H.n * sizeof(double));
+      hipLaunchKernelGGL(k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, lk,
+                         kb.as<double>());
       hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
-                         t->edges.as<int2>(), lk, out);
+                         t->edges.as<int2>(), (const double*)kb.as<double>(), out);
+    }
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST) {
       HIPTRY(hipMemcpyAsync(Ke, out, t->H.ne * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1560,10 +1565,16 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
     if (geometric)
       hipLaunchKernelGGL(k_geo_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
                          t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(), ke, dk, out);
-    else
+    else {
+      DBuf kb(t->pool.get());
+      kb.alloc(t->H.n * sizeof(double));
+      hipLaunchKernelGGL(k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, lk,
+                         kb.as<double>());
       hipLaunchKernelGGL(k_hm_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
                          t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                         t->inc_other.as<int32_t>(), lk, ke, dk, out);
+                         t->inc_other.as<int32_t>(), (const double*)kb.as<double>(), ke, dk,
+                         out);
+    }
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(dlogK, out, t->H.n * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -462,13 +462,19 @@ __device__ __forceinline__ void sell_row(const SellD<T>& A, int64_t r, int64_t& 
 
 // ------------------------------------------------------------------ edge maps
 
-// K_e = hm(exp(logK_a), exp(logK_b)), hm(x, y) = 2 / (1/x + 1/y).
+// k_c = exp(logK_c), once per cell (the maps below gather it per edge end).
+__global__ void k_exp(int64_t n, const double* __restrict__ logK, double* __restrict__ k) {
+  int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (c < n) k[c] = exp(logK[c]);
+}
+
+// K_e = hm(k_a, k_b), hm(x, y) = 2 / (1/x + 1/y), k = exp(logK) per cell.
 __global__ void k_hm_forward(int64_t ne, const int2* __restrict__ edges,
-                             const double* __restrict__ logK, double* __restrict__ Ke) {
+                             const double* __restrict__ k, double* __restrict__ Ke) {
   int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (e >= ne) return;
   int2 ab = edges[e];
-  double ka = exp(logK[ab.x]), kb = exp(logK[ab.y]);
+  double ka = k[ab.x], kb = k[ab.y];
   Ke[e] = 2.0 / (1.0 / ka + 1.0 / kb);
 }
 
@@ -497,19 +503,19 @@ __global__ void k_geo_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
 }
 
 // dlogK_c = sum_{e incident to c} dK_e K_e k_other / (k_c + k_other), in
-// edge-list order per cell.
+// edge-list order per cell (k = exp(logK) per cell).
 __global__ void k_hm_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
                               const int32_t* __restrict__ inc_edge,
                               const int32_t* __restrict__ inc_other,
-                              const double* __restrict__ logK, const double* __restrict__ Ke,
+                              const double* __restrict__ k, const double* __restrict__ Ke,
                               const double* __restrict__ dKe, double* __restrict__ dlogK) {
   int64_t c = logical_row();
   if (c >= n) return;
-  double kc = exp(logK[c]);
+  double kc = k[c];
   double s = 0.0;
   for (int64_t p = inc_ptr[c]; p < inc_ptr[c + 1]; p++) {
     int32_t e = inc_edge[p];
-    double ko = exp(logK[inc_other[p]]);
+    double ko = k[inc_other[p]];
     s += dKe[e] * Ke[e] * ko / (kc + ko);
   }
   dlogK[c] = s;
