@@ -311,42 +311,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def ensemble_members(P):
-    """96 realizations: 32 each of 64^3, 96x96x24 and 128x128x16 cells at 0.5 m
-    node spacing, logK = -9 + layered Matern(nu=1) field with sigma 1.5 and a
-    correlation length drawn from U(20, 80) m (seeds 100..195), top-face
-    Dirichlet h = 0, 4 injectors of 1e-4 m^3/s and 50 observation cells per
-    member (the 2M-cell pattern scaled)."""
-    shapes = [(64, 64, 64)] * 32 + [(96, 96, 24)] * 32 + [(128, 128, 16)] * 32
-    grids = {}
-    out = []
-    for m, ns in enumerate(shapes):
-        seed = 100 + m
-        lam = float(np.random.Generator(np.random.Philox(seed)).uniform(20.0, 80.0))
-        maxs = tuple(0.5 * (k - 1) for k in ns)
-        if ns not in grids:
-            grids[ns] = P.grid.regulargrid3d((0.0, 0.0, 0.0), maxs, ns)
-        coords, nb, aol, vol = grids[ns]
-        n = coords.shape[1]
-        logK = P.fields.layered_logK(ns, (0.5, 0.5, 0.5), -9.0, lam, 1.5, seed)
-        dn = np.arange(ns[2], n + 1, ns[2], dtype=np.int64)
-        free = np.ones(n, dtype=bool)
-        free[dn - 1] = False
-        fc = np.nonzero(free)[0] + 1
-        rng = np.random.Generator(np.random.Philox(seed + 1000))
-        inj = rng.choice(fc, 4, replace=False)
-        obs = np.sort(rng.choice(fc, 50, replace=False))
-        Q = np.zeros(n)
-        Q[inj - 1] = 1e-4
-        out.append(dict(n=n, nb=nb, aol=aol, dn=dn, logK=logK, Q=Q, obs=obs))
-    return out
-
-
 def bench_ensemble(args, P, api, _lib, torch):
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
     t0 = time.time()
-    ms = ensemble_members(P)
+    ms = P.workloads.ensemble_members()
     log(f"[bench] ensemble: {len(ms)} members, {sum(m['n'] for m in ms)} cells, "
         f"generated in {time.time() - t0:.1f}s")
     t0 = time.time()

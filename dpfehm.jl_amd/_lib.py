@@ -105,6 +105,11 @@ SIGNATURES = {
                                              ctypes.POINTER(Stats), ctypes.POINTER(_p)]),
     "dpf_transient_pullback": (ctypes.c_int, [_p, _p, _p, _p, ctypes.POINTER(Opts),
                                               ctypes.POINTER(Stats)]),
+    "dpf_transient_forward_dts": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i32, _p, _p, _i32,
+                                                 _p, _p, ctypes.POINTER(Opts),
+                                                 ctypes.POINTER(Stats), ctypes.POINTER(_p)]),
+    "dpf_transient_pullback_seeds": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p,
+                                                    ctypes.POINTER(Opts), ctypes.POINTER(Stats)]),
     "dpf_profile_kernel": (ctypes.c_int, [_p, _i32, _i32, _i32, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_inspect_last_error": (ctypes.c_char_p, []),
     "dpf_inspect_create": (ctypes.c_int, [_i64, _i64, _p, _p, _i64, _p, _i32, _i32,

@@ -384,16 +384,24 @@ def _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
     qscale = _f64(np.ones(nsteps + 1) if qscale is None else qscale)
     if qscale.shape[0] != nsteps + 1:
         raise ValueError("qscale must have nsteps + 1 entries")
+    dts = None
+    if np.ndim(dt) > 0:
+        dts = _f64(dt)
+        if dts.shape != (int(nsteps),):
+            raise ValueError("dt must be a number or hold one length per step")
     uN = np.empty(topo.nfree)
     states = np.empty((nsteps + 1, topo.nfree)) if return_states else None
     st = _lib.Stats()
     sh = ctypes.c_void_p()
-    code = _lib.lib().dpf_transient_forward(
-        topo.handle, _ptr(_f64(Ks)), _ptr(_f64(dirichleths)), _ptr(Qs),
-        _ptr(_f64(specificstorage)), _ptr(_f64(volumes)), _ptr(u0), int(nsteps), float(dt),
-        _ptr(qscale), int(checkpoint_every), _ptr(uN),
-        _ptr(states) if return_states else None, ctypes.byref(o), ctypes.byref(st),
-        ctypes.byref(sh) if keep else None)
+    head = (topo.handle, _ptr(_f64(Ks)), _ptr(_f64(dirichleths)), _ptr(Qs),
+            _ptr(_f64(specificstorage)), _ptr(_f64(volumes)), _ptr(u0), int(nsteps))
+    tail = (_ptr(qscale), int(checkpoint_every), _ptr(uN),
+            _ptr(states) if return_states else None, ctypes.byref(o), ctypes.byref(st),
+            ctypes.byref(sh) if keep else None)
+    if dts is None:
+        code = _lib.lib().dpf_transient_forward(*head, float(dt), *tail)
+    else:
+        code = _lib.lib().dpf_transient_forward_dts(*head, _ptr(dts), *tail)
     if stats is not None:
         stats.update(st.as_dict())
     _lib.check(code)
@@ -403,11 +411,13 @@ def _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
 def groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
                           specificstorage, volumes, u0, nsteps, dt, qscale=None,
                           return_states=False, stats=None, **kwargs):
-    """Fixed-step backward Euler of du/dt = -groundwater_residuals(u, ...), the
-    system DifferentiableBackwardEuler.steps integrates in
+    """Backward Euler of du/dt = -groundwater_residuals(u, ...), the system
+    DifferentiableBackwardEuler.steps integrates in
     examples/gw_transient_3d/inputdeck.jl:60-92.  ``u0`` and the result are on
-    the free cells; ``qscale[n]`` scales Qs at t_n = n dt.  Returns u_N, or the
-    (nsteps + 1, nfree) matrix of states with ``return_states``."""
+    the free cells; ``dt`` is one step length or a sequence of ``nsteps``
+    lengths (e.g. the steps an adaptive integrator took); ``qscale[n]``
+    scales Qs at t_n.  Returns u_N, or the (nsteps + 1, nfree) matrix of
+    states with ``return_states`` (row k = u_k, the reference's columns)."""
     uN, states, _, _ = _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
                                   Qs, specificstorage, volumes, u0, nsteps, dt, qscale, 0, False,
                                   return_states, stats, kwargs)
@@ -417,24 +427,41 @@ def groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes, diric
 def rrule_groundwater_transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
                                 specificstorage, volumes, u0, nsteps, dt, qscale=None,
                                 checkpoint_every=20, stats=None, **kwargs):
-    """Returns (u_N, pullback); pullback(dJ/du_N) -> (dJ/dKs, dJ/dQs), the
-    states between checkpoints recomputed during the reverse sweep."""
+    """Returns (u_N, pullback).  ``pullback(dJdu)`` takes dJ/du_N (nfree,) or
+    the seeds of every state, an (nsteps + 1, nfree) array with row k =
+    dJ/du_k (rows of zeros are skipped), and returns
+    ``(dKs, ddirichleths, dQs, du0)``: the gradient in p = [Ks; dirichleths]
+    that examples/gw_transient_3d/inputdeck.jl:63,103-104 takes through
+    DifferentiableBackwardEuler.steps, plus the tangents in Qs and u0.  The
+    states between checkpoints are recomputed during the reverse sweep."""
     uN, _, state, o = _transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
                                  Qs, specificstorage, volumes, u0, nsteps, dt, qscale,
                                  checkpoint_every, True, False, stats, kwargs)
-    ne, n = state.topo.ne, state.topo.n
+    ne, n, nf = state.topo.ne, state.topo.n, state.topo.nfree
 
     def pullback(dJdu, stats=None):
         g = _f64(dJdu)
+        if g.shape == (nf,):
+            steps = np.array([nsteps], dtype=np.int32)
+            seeds = g
+        elif g.shape == (nsteps + 1, nf):
+            steps = np.nonzero(np.any(g != 0.0, axis=1))[0].astype(np.int32)
+            seeds = np.ascontiguousarray(g[steps])
+        else:
+            raise ValueError("dJdu must be (nfree,) or (nsteps + 1, nfree)")
+        steps = np.ascontiguousarray(steps)
         dK = np.empty(ne)
+        ddh = np.empty(n)
         dQ = np.empty(n)
+        du0 = np.empty(nf)
         st = _lib.Stats()
-        code = _lib.lib().dpf_transient_pullback(state._h, _ptr(g), _ptr(dK), _ptr(dQ),
-                                                 ctypes.byref(o), ctypes.byref(st))
+        code = _lib.lib().dpf_transient_pullback_seeds(
+            state._h, int(steps.size), _ptr(steps), _ptr(seeds), _ptr(dK), _ptr(ddh), _ptr(dQ),
+            _ptr(du0), ctypes.byref(o), ctypes.byref(st))
         if stats is not None:
             stats.update(st.as_dict())
         _lib.check(code)
-        return dK, dQ
+        return dK, ddh, dQ, du0
 
     pullback.state = state
     return uN, pullback

@@ -320,8 +320,10 @@ struct dpf_state {
   // transient runs
   int kind = 0;  // 0 steady, 1 backward Euler
   int nsteps = 0, ck = 0;
-  double dt = 0;
-  std::vector<double> qs;  // Q scale per time level (nsteps + 1)
+  double dt = 0;            // two-phase and transport runs: the (uniform) step length
+  std::vector<double> dts;  // backward Euler: step lengths (nsteps)
+  double cur_dt = 0;        // the step length the hierarchy was last prepared for
+  std::vector<double> qs;   // Q scale per time level (nsteps + 1)
   DBuf svec;               // S = Ss V on free cells
   DBuf ckpt;               // checkpointed states u_0, u_ck, u_2ck, ...
   // two-phase runs (kind 2): per IMPES step the saturation it started from,
@@ -1533,10 +1535,12 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
                          t->edges.as<int2>(), (const double*)kb.as<double>(), out);
     }
     HIPTRY(hipGetLastError());
-    if (o->memory == DPF_HOST) {
+    if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(Ke, out, t->H.ne * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIPTRY(hipStreamSynchronize(s));
-    }
+    // in every memory mode: the pooled exp(logK) buffer goes back to the pool
+    // when this call returns, and a later call on another stream may take it
+    // while k_hm_forward would still be reading it
+    HIPTRY(hipStreamSynchronize(s));
     return DPF_OK;
   });
 }
@@ -1866,17 +1870,24 @@ extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t ke
 
 namespace {
 
-// (S + dt A) and its hierarchy for the state's K, S and dt (bD -> t->b64)
-void prepare_be(dpf_topology* t, hipStream_t s, int precision, dpf_state* st) {
+// (S + dt A) and its hierarchy for the state's K, S and step length dt (the
+// Dirichlet part of b -> t->b64, independent of dt)
+void prepare_be(dpf_topology* t, hipStream_t s, int precision, dpf_state* st, double dt) {
   copy_dev(t->Ke.p, st->Ke.p, t->H.ne * sizeof(double), s);
   copy_dev(t->dh.p, st->dh.p, t->H.n * sizeof(double), s);
-  prepare(t, s, precision, nullptr, true, st->dt, st->svec.as<double>());
+  prepare(t, s, precision, nullptr, true, dt, st->svec.as<double>());
   st->version = t->version;
+  st->cur_dt = dt;
 }
 
-bool hier_fresh(dpf_topology* t, dpf_state* st, int precision) {
-  return t->version == st->version &&
+// The hierarchy holds this state's (S + dt A) in the requested precision.
+bool hier_fresh(dpf_topology* t, dpf_state* st, int precision, double dt) {
+  return t->version == st->version && st->cur_dt == dt &&
          (precision != DPF_FP64 ? t->h32.version == st->version : t->h64.version == st->version);
+}
+
+void ensure_be(dpf_topology* t, hipStream_t s, int precision, dpf_state* st, double dt) {
+  if (!hier_fresh(t, st, precision, dt)) prepare_be(t, s, precision, st, dt);
 }
 
 // Error text for a failed time step: which step, which phase, how far it got.
@@ -1889,15 +1900,18 @@ void step_failure(const char* what, int step, int code, const dpf_stats& ss) {
   set_err(buf);
 }
 
-// u_{n} from u_{n-1}: one backward-Euler step (warm-started at u_{n-1})
+// u_{n} from u_{n-1}: one backward-Euler step of length dts[n-1] (warm-started
+// at u_{n-1})
 int be_step(dpf_topology* t, hipStream_t s, dpf_state* st, int n, const double* uprev,
             double* unext, const dpf_opts* o, int& iters) {
   int64_t nf = t->H.nf;
+  double dt = st->dts[n - 1];
+  ensure_be(t, s, o->precision, st, dt);
   t->stage_f3.alloc(nf * sizeof(double));
   double* rhs = t->stage_f3.as<double>();
   hipLaunchKernelGGL(k_be_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
                      (const double*)st->svec.as<double>(), uprev,
-                     (const double*)st->Q.as<double>(), st->qs[n], st->dt,
+                     (const double*)st->Q.as<double>(), st->qs[n], dt,
                      (const double*)t->b64.as<double>(), rhs);
   dpf_stats ss;
   std::memset(&ss, 0, sizeof(ss));
@@ -1907,17 +1921,28 @@ int be_step(dpf_topology* t, hipStream_t s, dpf_state* st, int n, const double* 
   return code;
 }
 
-}  // namespace
+// Host or device array of count doubles -> host vector (the step lengths and
+// Q scales steer the host loop).
+void host_copy(std::vector<double>& dst, const double* src, int64_t count, const dpf_opts* o,
+               hipStream_t s) {
+  dst.resize(count);
+  if (o->memory == DPF_DEVICE) {
+    HIPTRY(hipMemcpyAsync(dst.data(), src, count * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+  } else {
+    std::memcpy(dst.data(), src, count * sizeof(double));
+  }
+}
 
-extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const double* dirichleths,
-                                     const double* Qs, const double* Ss, const double* V,
-                                     const double* u0, int32_t nsteps, double dt,
-                                     const double* qscale, int32_t checkpoint_every,
-                                     double* u_final, double* states_out, const dpf_opts* opts,
-                                     dpf_stats* stats, dpf_state** state_out) {
+int transient_forward_impl(dpf_topology* t, const double* Ks, const double* dirichleths,
+                           const double* Qs, const double* Ss, const double* V, const double* u0,
+                           int32_t nsteps, const double* dts, double dt_const,
+                           const double* qscale, int32_t checkpoint_every, double* u_final,
+                           double* states_out, const dpf_opts* opts, dpf_stats* stats,
+                           dpf_state** state_out) {
   if (state_out) *state_out = nullptr;
   if (!t || !Ks || !dirichleths || !Qs || !Ss || !V || !u0 || !qscale || !u_final ||
-      nsteps <= 0 || !(dt > 0)) {
+      nsteps <= 0 || (!dts && !(dt_const > 0 && std::isfinite(dt_const)))) {
     set_err("NULL argument or non-positive nsteps/dt");
     return DPF_EINVAL;
   }
@@ -1937,7 +1962,16 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
     st->kind = 1;
     st->precision = o->precision;
     st->nsteps = nsteps;
-    st->dt = dt;
+    if (dts) {
+      host_copy(st->dts, dts, nsteps, o, s);
+      for (int k = 0; k < nsteps; k++)
+        if (!(st->dts[k] > 0) || !std::isfinite(st->dts[k]))
+          throw Err{DPF_EINVAL, "step length " + std::to_string(k + 1) +
+                                    " is not positive and finite"};
+    } else {
+      st->dts.assign(nsteps, dt_const);
+    }
+    host_copy(st->qs, qscale, nsteps + 1, o, s);
     st->ck = checkpoint_every > 0 ? std::min<int>(checkpoint_every, nsteps) : nsteps;
     st->Ke.alloc(ne * sizeof(double));
     st->dh.alloc(n * sizeof(double));
@@ -1945,7 +1979,6 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
     st->svec.alloc(nf * sizeof(double));
     DBuf ssb(t->pool.get()), vb(t->pool.get()), u0b(t->pool.get());
     const double *ss_d, *v_d, *u0_d;
-    st->qs.resize(nsteps + 1);
     if (o->memory == DPF_DEVICE) {
       copy_dev(st->Ke.p, Ks, ne * sizeof(double), s);
       copy_dev(st->dh.p, dirichleths, n * sizeof(double), s);
@@ -1953,8 +1986,6 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
       ss_d = Ss;
       v_d = V;
       u0_d = u0;
-      HIPTRY(hipMemcpyAsync(st->qs.data(), qscale, (nsteps + 1) * sizeof(double),
-                            hipMemcpyDeviceToHost, s));
     } else {
       HIPTRY(hipMemcpyAsync(st->Ke.p, Ks, ne * sizeof(double), hipMemcpyHostToDevice, s));
       HIPTRY(hipMemcpyAsync(st->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1962,12 +1993,11 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
       ss_d = stage_in(t, o, ssb, Ss, n, s);
       v_d = stage_in(t, o, vb, V, n, s);
       u0_d = stage_in(t, o, u0b, u0, nf, s);
-      std::memcpy(st->qs.data(), qscale, (nsteps + 1) * sizeof(double));
     }
     hipLaunchKernelGGL(k_storage, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, t->f2n.as<int32_t>(),
                        ss_d, v_d, st->svec.as<double>());
     HIPTRY(hipStreamSynchronize(s));
-    prepare_be(t, s, o->precision, st.get());
+    prepare_be(t, s, o->precision, st.get(), st->dts[0]);
     double t1 = now_ms();
     int nck = nsteps / st->ck + 1;
     st->ckpt.alloc((size_t)nck * nf * sizeof(double));
@@ -2023,12 +2053,48 @@ extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const do
   });
 }
 
-extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, double* dKs,
-                                      double* dQs, const dpf_opts* opts, dpf_stats* stats) {
-  if (!st || !st->topo || st->kind != 1 || !dJ_du_final || !dKs) {
+}  // namespace
+
+extern "C" int dpf_transient_forward(dpf_topology* t, const double* Ks, const double* dirichleths,
+                                     const double* Qs, const double* Ss, const double* V,
+                                     const double* u0, int32_t nsteps, double dt,
+                                     const double* qscale, int32_t checkpoint_every,
+                                     double* u_final, double* states_out, const dpf_opts* opts,
+                                     dpf_stats* stats, dpf_state** state_out) {
+  return transient_forward_impl(t, Ks, dirichleths, Qs, Ss, V, u0, nsteps, nullptr, dt, qscale,
+                                checkpoint_every, u_final, states_out, opts, stats, state_out);
+}
+
+extern "C" int dpf_transient_forward_dts(dpf_topology* t, const double* Ks,
+                                         const double* dirichleths, const double* Qs,
+                                         const double* Ss, const double* V, const double* u0,
+                                         int32_t nsteps, const double* dts, const double* qscale,
+                                         int32_t checkpoint_every, double* u_final,
+                                         double* states_out, const dpf_opts* opts,
+                                         dpf_stats* stats, dpf_state** state_out) {
+  if (!dts) {
+    if (state_out) *state_out = nullptr;
+    set_err("dts is NULL");
+    return DPF_EINVAL;
+  }
+  return transient_forward_impl(t, Ks, dirichleths, Qs, Ss, V, u0, nsteps, dts, 0.0, qscale,
+                                checkpoint_every, u_final, states_out, opts, stats, state_out);
+}
+
+extern "C" int dpf_transient_pullback_seeds(dpf_state* st, int32_t nseed,
+                                            const int32_t* seed_steps, const double* seeds,
+                                            double* dKs, double* ddirichleths, double* dQs,
+                                            double* du0, const dpf_opts* opts,
+                                            dpf_stats* stats) {
+  if (!st || !st->topo || st->kind != 1 || nseed < 0 || (nseed > 0 && (!seed_steps || !seeds))) {
     set_err("invalid argument (state must come from dpf_transient_forward)");
     return DPF_EINVAL;
   }
+  for (int i = 0; i < nseed; i++)
+    if (seed_steps[i] < 0 || seed_steps[i] > st->nsteps || (i > 0 && seed_steps[i] <= seed_steps[i - 1])) {
+      set_err("seed_steps must be strictly ascending step indices in 0..nsteps");
+      return DPF_EINVAL;
+    }
   dpf_topology* t = st->topo;
   dpf_opts dflt;
   dpf_opts_init(&dflt);
@@ -2042,21 +2108,35 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     double t0 = now_ms();
     hipStream_t s = pick_stream(t, o);
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
-    if (!hier_fresh(t, st, o->precision)) prepare_be(t, s, o->precision, st);
-    const double* gN = stage_in(t, o, t->stage_f, dJ_du_final, nf, s);
-    double* dK = dKs;
-    double* dQ = dQs;
-    DBuf bk(t->pool.get()), bq(t->pool.get());
-    if (o->memory == DPF_HOST) {
-      bk.alloc(ne * sizeof(double));
-      dK = bk.as<double>();
-      if (dQs) {
-        bq.alloc(n * sizeof(double));
-        dQ = bq.as<double>();
-      }
+    BufPool* bp = t->pool.get();
+    // the seeds on the device, and per step the seed row (null: no seed)
+    DBuf seedb(bp);
+    const double* sd = seeds;
+    if (nseed > 0 && o->memory == DPF_HOST) {
+      seedb.alloc((size_t)nseed * nf * sizeof(double));
+      HIPTRY(hipMemcpyAsync(seedb.p, seeds, (size_t)nseed * nf * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+      sd = seedb.as<double>();
     }
     int N = st->nsteps, C = st->ck;
-    BufPool* bp = t->pool.get();
+    std::vector<const double*> gk(N + 1, nullptr);
+    for (int i = 0; i < nseed; i++) gk[seed_steps[i]] = sd + (size_t)i * nf;
+    int smax = nseed > 0 ? seed_steps[nseed - 1] : 0;  // no state after it carries a seed
+    // outputs (device), and the two accumulated multiplier sums:
+    // macc = sum_n dt_n mu_n (-> ddirichleths), qacc = sum_n dt_n q_n mu_n (-> dQs)
+    DBuf bk(bp), bd(bp), bq(bp), bu(bp), macc(bp), qacc(bp);
+    auto out = [&](DBuf& b, double* user, int64_t cnt) -> double* {
+      if (!user) return nullptr;
+      if (o->memory == DPF_DEVICE) return user;
+      b.alloc(cnt * sizeof(double));
+      return b.as<double>();
+    };
+    double* dK = out(bk, dKs, ne);
+    double* dd = out(bd, ddirichleths, n);
+    double* dQ = out(bq, dQs, n);
+    double* du = out(bu, du0, nf);
+    if (dd) macc.alloc(nf * sizeof(double));
+    if (dQ) qacc.alloc(nf * sizeof(double));
     DBuf seg(bp), mua(bp), mub(bp), rhsb(bp);
     seg.alloc((size_t)(C + 1) * nf * sizeof(double));
     mua.alloc(nf * sizeof(double));
@@ -2068,11 +2148,12 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     double* rhs = rhsb.as<double>();
     bool have_next = false, first_acc = true;
     int iters = 0, status = DPF_OK;
-    for (int c = (N - 1) / C; c >= 0 && status == DPF_OK; c--) {
-      int a = c * C, b = std::min(N, (c + 1) * C);
+    for (int c = smax >= 1 ? (smax - 1) / C : -1; c >= 0 && status == DPF_OK; c--) {
+      int a = c * C, b = std::min(smax, (c + 1) * C);
       // u_{a..b} into the segment buffer: u_a and, when b closes a full
       // segment, u_b from the checkpoints; the states in between recomputed
-      // (bitwise what the forward produced: same steps, same warm starts)
+      // (bitwise what the forward produced: same steps, operators and warm
+      // starts)
       copy_dev(segp, st->ckpt.as<double>() + (size_t)c * nf, nf * sizeof(double), s);
       bool b_stored = b == (c + 1) * C;
       for (int k = a + 1; k < (b_stored ? b : b + 1) && status == DPF_OK; k++)
@@ -2082,10 +2163,13 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
         copy_dev(segp + (size_t)(b - a) * nf, st->ckpt.as<double>() + (size_t)(c + 1) * nf,
                  nf * sizeof(double), s);
       for (int k = b; k > a && status == DPF_OK; k--) {
+        double dt = st->dts[k - 1];
+        ensure_be(t, s, o->precision, st, dt);
+        // (S + dt_k A) mu_k = dJ/du_k + S mu_{k+1}
         hipLaunchKernelGGL(k_be_adj_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
                            (const double*)st->svec.as<double>(),
-                           have_next ? (const double*)mu_next : (const double*)nullptr,
-                           k == N ? gN : (const double*)nullptr, rhs);
+                           have_next ? (const double*)mu_next : (const double*)nullptr, gk[k],
+                           rhs);
         dpf_stats ss;
         std::memset(&ss, 0, sizeof(ss));
         status = solve_system(t, s, rhs, mu, o, &ss, have_next ? mu_next : nullptr);
@@ -2095,28 +2179,61 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
           break;
         }
         const double* uk = segp + (size_t)(k - a) * nf;
-        hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
-                           t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), uk,
-                           (const double*)st->dh.as<double>(), (const double*)mu, st->dt,
-                           first_acc ? 0 : 1, dK);
-        if (dQ)
-          hipLaunchKernelGGL(k_dq_acc, dim3(nblocks(n)), dim3(BLK), 0, s, n,
-                             t->n2f.as<int32_t>(), (const double*)mu, st->dt * st->qs[k],
-                             first_acc ? 0 : 1, dQ);
+        if (dK)
+          hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                             t->edges.as<int2>(), t->n2f.as<int32_t>(), t->aol.as<double>(), uk,
+                             (const double*)st->dh.as<double>(), (const double*)mu, dt,
+                             first_acc ? 0 : 1, dK);
+        if (dd || dQ)
+          hipLaunchKernelGGL(k_adj_acc, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                             (const double*)mu, dt, dt * st->qs[k], first_acc ? 0 : 1,
+                             dd ? macc.as<double>() : (double*)nullptr,
+                             dQ ? qacc.as<double>() : (double*)nullptr);
         first_acc = false;
         std::swap(mu, mu_next);
         have_next = true;
       }
     }
     HIPTRY(hipGetLastError());
-    if (o->memory == DPF_HOST) {
-      HIPTRY(hipMemcpyAsync(dKs, dK, ne * sizeof(double), hipMemcpyDeviceToHost, s));
-      if (dQs) HIPTRY(hipMemcpyAsync(dQs, dQ, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (status == DPF_OK) {
+      if (first_acc) {  // no step carried a multiplier: every sum is zero
+        if (dK) hipLaunchKernelGGL(k_zero_f64, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, dK);
+        if (dd) hipLaunchKernelGGL(k_zero_f64, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, macc.as<double>());
+        if (dQ) hipLaunchKernelGGL(k_zero_f64, dim3(nblocks(nf)), dim3(BLK), 0, s, nf, qacc.as<double>());
+      }
+      // dirichleths: sum over the free neighbours of K_e aol_e macc (0 on free
+      // cells); Qs: qacc scattered to the free cells (0 on Dirichlet cells)
+      if (dd)
+        hipLaunchKernelGGL(k_ddh_dq, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                           t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                           t->inc_other.as<int32_t>(), (const double*)st->Ke.as<double>(),
+                           t->aol.as<double>(), (const double*)macc.as<double>(),
+                           (const double*)nullptr, dd, (double*)nullptr);
+      if (dQ)
+        hipLaunchKernelGGL(k_ddh_dq, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+                           t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                           t->inc_other.as<int32_t>(), (const double*)st->Ke.as<double>(),
+                           t->aol.as<double>(), (const double*)qacc.as<double>(),
+                           (const double*)nullptr, (double*)nullptr, dQ);
+      // u_0: dJ/du_0 + S mu_1
+      if (du)
+        hipLaunchKernelGGL(k_be_adj_rhs, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
+                           (const double*)st->svec.as<double>(),
+                           have_next ? (const double*)mu_next : (const double*)nullptr, gk[0],
+                           du);
+      HIPTRY(hipGetLastError());
+      if (o->memory == DPF_HOST) {
+        if (dKs) HIPTRY(hipMemcpyAsync(dKs, dK, ne * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (ddirichleths)
+          HIPTRY(hipMemcpyAsync(ddirichleths, dd, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (dQs) HIPTRY(hipMemcpyAsync(dQs, dQ, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (du0) HIPTRY(hipMemcpyAsync(du0, du, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+      }
     }
     HIPTRY(hipStreamSynchronize(s));
     if (stats) {
       stats->iterations = iters;
-      stats->outer = N;
+      stats->outer = smax;
       stats->converged = status == DPF_OK;
       stats->levels = t->levels();
       stats->relres = 0;
@@ -2126,6 +2243,17 @@ extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, 
     }
     return status;
   });
+}
+
+extern "C" int dpf_transient_pullback(dpf_state* st, const double* dJ_du_final, double* dKs,
+                                      double* dQs, const dpf_opts* opts, dpf_stats* stats) {
+  if (!st || st->kind != 1 || !dJ_du_final || !dKs) {
+    set_err("invalid argument (state must come from dpf_transient_forward)");
+    return DPF_EINVAL;
+  }
+  int32_t step = st->nsteps;
+  return dpf_transient_pullback_seeds(st, 1, &step, dJ_du_final, dKs, nullptr, dQs, nullptr,
+                                      opts, stats);
 }
 
 // ====================================================================== two-phase (IMPES)

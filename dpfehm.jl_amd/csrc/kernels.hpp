@@ -1426,18 +1426,20 @@ __global__ void k_be_adj_rhs(int64_t nf, const double* __restrict__ svec,
   rhs[j] = g ? v + g[j] : v;
 }
 
-// dQ_i (+)= c mu_{j(i)} on free cells, 0 on Dirichlet cells
-__global__ void k_dq_acc(int64_t n, const int32_t* __restrict__ n2f, const double* __restrict__ mu,
-                         double c, int accumulate, double* __restrict__ dQ) {
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
-  int32_t j = n2f[i];
-  double v = j >= 0 ? c * mu[j] : 0.0;
-  dQ[i] = accumulate ? dQ[i] + v : v;
+// The reverse sweep's multiplier sums over steps (free cells):
+//   macc (+)= cm mu,  qacc (+)= cq mu   (either may be null)
+__global__ void k_adj_acc(int64_t nf, const double* __restrict__ mu, double cm, double cq,
+                          int accumulate, double* __restrict__ macc, double* __restrict__ qacc) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  double m = mu[j];
+  if (macc) macc[j] = accumulate ? macc[j] + cm * m : cm * m;
+  if (qacc) qacc[j] = accumulate ? qacc[j] + cq * m : cq * m;
 }
 
 // dQ_i = lambda_{j(i)} (free), 0 (Dirichlet);
-// ddh_i = 0 (free), delta_i + sum_{e: other free} K_e aol_e lambda_other (Dirichlet)
+// ddh_i = 0 (free), delta_i + sum_{e: other free} K_e aol_e lambda_other (Dirichlet;
+// delta = null stands for 0)
 __global__ void k_ddh_dq(int64_t n, const int32_t* __restrict__ n2f,
                          const int64_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_edge,
                          const int32_t* __restrict__ inc_other, const double* __restrict__ Ke,
@@ -1454,7 +1456,7 @@ __global__ void k_ddh_dq(int64_t n, const int32_t* __restrict__ n2f,
   }
   if (dQ) dQ[i] = 0.0;
   if (ddh) {
-    double s = delta[i];
+    double s = delta ? delta[i] : 0.0;
     for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
       int32_t fo = n2f[inc_other[p]];
       if (fo >= 0) {

@@ -17,7 +17,8 @@ Workloads:
   h = 0 on the top face (i3 = 50).  200 observation cells (seed 3); the
   observed heads come from a second field (seed 4) plus N(0, (1e-3)^2)
   noise (seed 5).
-* ``ensemble_grid`` -- the cell-spacing-0.5 m grids of the batched ensemble.
+* ``ensemble_members`` -- the 96 realizations of the batched ensemble on
+  0.5 m grids.
 * ``transient_schedule`` -- the injection ramp of the transient run.
 """
 from dataclasses import dataclass, field
@@ -141,3 +142,38 @@ def transient_schedule(nsteps=200, dt=3600.0, ramp_steps=50):
     returned as the per-step scale factor (entry 0 is t = 0)."""
     t = np.arange(nsteps + 1) * dt
     return np.minimum(1.0, t / (ramp_steps * dt))
+
+
+def ensemble_members(shapes=None):
+    """The batched ensemble: 96 realizations, 32 each of 64^3, 96x96x24 and
+    128x128x16 cells at 0.5 m node spacing, logK = -9 + layered Matern(nu=1)
+    field with sigma 1.5 and a correlation length drawn from U(20, 80) m
+    (seeds 100..195), top-face Dirichlet h = 0, 4 injectors of 1e-4 m^3/s and
+    50 observation cells per member (the 2M-cell pattern scaled).  ``shapes``
+    (optional) replaces the list of member grid shapes; member m keeps seed
+    100 + m.  Returns dicts with n, nb, aol, dn, logK, Q, obs, ns, lam."""
+    if shapes is None:
+        shapes = [(64, 64, 64)] * 32 + [(96, 96, 24)] * 32 + [(128, 128, 16)] * 32
+    grids = {}
+    out = []
+    for m, ns in enumerate(shapes):
+        ns = tuple(ns)
+        seed = 100 + m
+        lam = float(np.random.Generator(np.random.Philox(seed)).uniform(20.0, 80.0))
+        maxs = tuple(0.5 * (k - 1) for k in ns)
+        if ns not in grids:
+            grids[ns] = grid.regulargrid3d((0.0, 0.0, 0.0), maxs, ns)
+        coords, nb, aol, vol = grids[ns]
+        n = coords.shape[1]
+        logK = fields.layered_logK(ns, (0.5, 0.5, 0.5), -9.0, lam, 1.5, seed)
+        dn = np.arange(ns[2], n + 1, ns[2], dtype=np.int64)
+        free = np.ones(n, dtype=bool)
+        free[dn - 1] = False
+        fc = np.nonzero(free)[0] + 1
+        rng = np.random.Generator(np.random.Philox(seed + 1000))
+        inj = rng.choice(fc, 4, replace=False)
+        obs = np.sort(rng.choice(fc, 50, replace=False))
+        Q = np.zeros(n)
+        Q[inj - 1] = 1e-4
+        out.append(dict(n=n, nb=nb, aol=aol, dn=dn, logK=logK, Q=Q, obs=obs, ns=ns, lam=lam))
+    return out

@@ -328,15 +328,20 @@ int dpf_steady_pullback(dpf_state *state, const double *delta, double *dKs,
 
 int dpf_state_destroy(dpf_state *state);
 
-/* Transient groundwater flow by fixed-step backward Euler: the
- * reference integrates du/dt = f(u) = -groundwater_residuals(u, Ks, ...,
- * specificstorage, volumes) with DifferentiableBackwardEuler.steps
- * (examples/gw_transient_3d/inputdeck.jl:60-92, test/theis.jl:61-106).  Each
- * step solves (S + dt A) u_n = S u_{n-1} + dt b(t_n), S = diag(Ss V) on free
- * cells, with Q(t_n) = qscale[n] Qs, by AMG-PCG warm-started at u_{n-1}.
+/* Transient groundwater flow by backward Euler: the reference integrates
+ * du/dt = f(u) = -groundwater_residuals(u, Ks, ..., specificstorage,
+ * volumes) with DifferentiableBackwardEuler.steps, parameters
+ * p = [Ks; dirichleths] (examples/gw_transient_3d/inputdeck.jl:60-92,
+ * test/theis.jl:61-106).  Step n (1..nsteps) of length dt_n solves
+ * (S + dt_n A) u_n = S u_{n-1} + dt_n b(t_n), S = diag(Ss V) on free cells,
+ * with Q(t_n) = qscale[n] Qs, by AMG-PCG warm-started at u_{n-1}.
  *   Ks: nedges;  dirichleths, Qs, specificstorage, volumes: n;  u0, u_final:
  *   nfree;  qscale: nsteps + 1;  states_out: NULL or (nsteps + 1) * nfree
  *   (row k = u_k, like the columns of the reference's steps matrix).
+ * dpf_transient_forward takes one step length dt; dpf_transient_forward_dts
+ * a caller-given sequence dts (nsteps lengths > 0, e.g. the steps an
+ * adaptive integrator chose; a hierarchy refresh runs whenever the length
+ * changes).  qscale and dts follow opts->memory.
  * The state keeps every checkpoint_every-th state (0 = only u_0) for the
  * pullback, which recomputes the states in between. */
 int dpf_transient_forward(dpf_topology *topo, const double *Ks, const double *dirichleths,
@@ -345,12 +350,34 @@ int dpf_transient_forward(dpf_topology *topo, const double *Ks, const double *di
                           const double *qscale, int32_t checkpoint_every, double *u_final,
                           double *states_out, const dpf_opts *opts, dpf_stats *stats,
                           dpf_state **state_out);
+int dpf_transient_forward_dts(dpf_topology *topo, const double *Ks, const double *dirichleths,
+                              const double *Qs, const double *specificstorage,
+                              const double *volumes, const double *u0, int32_t nsteps,
+                              const double *dts, const double *qscale, int32_t checkpoint_every,
+                              double *u_final, double *states_out, const dpf_opts *opts,
+                              dpf_stats *stats, dpf_state **state_out);
 
-/* Discrete adjoint of dpf_transient_forward for an objective of the final
- * state (the rrule DifferentiableBackwardEuler provides):
- *   (S + dt A) mu_N = dJ/du_N,  (S + dt A) mu_n = S mu_{n+1},
- *   dKs_e = -sum_n dt aol_e (u~n_a - u~n_b)(mu~n_a - mu~n_b),
- *   dQs_i = sum_n dt qscale[n] mu_n[i]  (0 on Dirichlet cells).
+/* Discrete adjoint of a transient run: the reverse mode the rrule of
+ * DifferentiableBackwardEuler.steps provides for the gradient in
+ * p = [Ks; dirichleths] (examples/gw_transient_3d/inputdeck.jl:63,86-91,
+ * 103-104), and in Qs and the initial state.  The objective may depend on
+ * any states: seeds holds dJ/du_k (nfree each, row i for step
+ * seed_steps[i]; seed_steps: nseed strictly ascending indices in 0..nsteps,
+ * host memory; seeds follow opts->memory).  With M_n = S + dt_n A:
+ *   M_n mu_n = dJ/du_n + S mu_{n+1}  (n = N..1, mu_{N+1} = 0),
+ *   dKs_e          = -sum_n dt_n aol_e (u~n_a - u~n_b)(mu~n_a - mu~n_b),
+ *   ddirichleths_o = sum_n dt_n sum_{e = (o, free i)} Ks_e aol_e mu_n[i]
+ *                    (Dirichlet cells; 0 on free cells),
+ *   dQs_i          = sum_n dt_n qscale[n] mu_n[i]  (0 on Dirichlet cells),
+ *   du0            = dJ/du_0 + S mu_1.
+ *   dKs: nedges;  ddirichleths, dQs: n;  du0: nfree; any output may be NULL.
+ * The reverse sweep starts at the last seeded step. */
+int dpf_transient_pullback_seeds(dpf_state *state, int32_t nseed, const int32_t *seed_steps,
+                                 const double *seeds, double *dKs, double *ddirichleths,
+                                 double *dQs, double *du0, const dpf_opts *opts,
+                                 dpf_stats *stats);
+
+/* dpf_transient_pullback_seeds for an objective of the final state alone:
  *   dJ_du_final: nfree;  dKs: nedges;  dQs: n or NULL. */
 int dpf_transient_pullback(dpf_state *state, const double *dJ_du_final, double *dKs,
                            double *dQs, const dpf_opts *opts, dpf_stats *stats);
@@ -359,12 +386,17 @@ int dpf_transient_pullback(dpf_state *state, const double *dJ_du_final, double *
  * Launches one kernel of the current hierarchy `reps` times back to back
  * between two hipEvents on opts->stream (or the topology stream) and returns
  * the mean duration per launch and the algorithmic bytes one launch moves:
- *   kernel 0: finest-level Chebyshev sweep  d = c1 d + c2 D^-1 (b - A x), x' = x + d
- *             bytes = nnz (w + 4) + 6 nfree w      (w = 8 FP64, 4 FP32)
+ *   kernel 0: finest-level Chebyshev sweep as the V-cycle runs it at finest
+ *             degree 1: the post-smoothing sweep after the coarse correction,
+ *             x' = x + c2 D^-1 (b - A x) (c1 = 0: no d read, none written),
+ *             with the fused b.x' block partials
+ *             bytes = nnz w + cols + 4 nfree w     (w = 8 FP64, 4 FP32)
  *   kernel 1: CG direction + SpMV  p' = z + beta p, q = A p'
- *             bytes = nnz (w + 4) + 4 nfree w
+ *             bytes = nnz w + cols + 4 nfree w
  *   kernel 2: FP64 refinement residual  r = b - A x
- *             bytes = nnz 12 + 3 nfree 8
+ *             bytes = nnz 8 + cols + 3 nfree 8
+ * where cols = 4 x (nonzeros of the fine operator's slices that store column
+ * indices; regular-grid slices store column offsets and no indices).
  * precision selects the FP64 or the FP32 (mixed) hierarchy; it must have been
  * used by a solve on this topology. */
 int dpf_profile_kernel(dpf_topology *topo, int32_t precision, int32_t kernel, int32_t reps,

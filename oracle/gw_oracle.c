@@ -390,12 +390,13 @@ int orc_steady_pullback(int64_t n, int64_t ne, const int64_t *nb, const double *
   return st;
 }
 
-/* ---- transient, fixed-step backward Euler ----
+/* ---- transient, backward Euler with a given step sequence ----
  * The reference integrates du/dt = f(u) = -groundwater_residuals(u, ..., Ss, V)
  * (examples/gw_transient_3d/inputdeck.jl:78-85) with DifferentiableBackwardEuler.
- * For this linear f, one backward-Euler step u_n = u_{n-1} + dt f(u_n) is
- *     (S + dt A) u_n = S u_{n-1} + dt b(t_n),   S = diag(Ss V) on free cells,
- * with A = dr/dh and b = -r(0) assembled at Ss = V = 1.  Q(t_n) = qscale[n] Q.
+ * For this linear f, one backward-Euler step u_n = u_{n-1} + dt_n f(u_n) is
+ *     (S + dt_n A) u_n = S u_{n-1} + dt_n b(t_n),   S = diag(Ss V) on free cells,
+ * with A = dr/dh and b = -r(0) assembled at Ss = V = 1.  Q(t_n) = qscale[n] Q;
+ * dts[n-1] is the length of step n.
  * states (nsteps+1) x nf row-major receives u_0 .. u_N. */
 static void assemble_sys(int64_t n, int64_t ne, const int64_t *nb, const double *K,
                          const double *aol, const uint8_t *isfree, const int64_t *n2f,
@@ -413,7 +414,7 @@ static void assemble_sys(int64_t n, int64_t ne, const int64_t *nb, const double 
 int orc_transient(int64_t n, int64_t ne, const int64_t *nb, const double *K,
                   const double *aol, int64_t nd, const int64_t *dnodes, const double *dh,
                   const double *Q, const double *Ss, const double *V, const double *u0,
-                  int64_t nsteps, double dt, const double *qscale, double reltol,
+                  int64_t nsteps, const double *dts, const double *qscale, double reltol,
                   int64_t maxiter, double *states, double *info) {
   uint8_t *isfree = (uint8_t *)malloc((size_t)n);
   int64_t *n2f = (int64_t *)malloc(sizeof(int64_t) * n);
@@ -424,10 +425,7 @@ int orc_transient(int64_t n, int64_t ne, const int64_t *nb, const double *K,
   assemble_sys(n, ne, nb, K, aol, isfree, n2f, nf, &rp, &ci, &v);
   double *s = (double *)malloc(sizeof(double) * (nf + 1));
   for (int64_t j = 0; j < nf; j++) s[j] = Ss[f2n[j]] * V[f2n[j]];
-  /* M = S + dt A */
   double *mv = (double *)malloc(sizeof(double) * (rp[nf] + 1));
-  for (int64_t j = 0; j < nf; j++)
-    for (int64_t k = rp[j]; k < rp[j + 1]; k++) mv[k] = dt * v[k] + (ci[k] == j ? s[j] : 0.0);
   /* b split as Q part and Dirichlet part: b(t) = qscale * Q + bD */
   double *ones = (double *)malloc(sizeof(double) * n);
   for (int64_t i = 0; i < n; i++) ones[i] = 1.0;
@@ -443,6 +441,10 @@ int orc_transient(int64_t n, int64_t ne, const int64_t *nb, const double *K,
   for (int64_t t = 1; t <= nsteps; t++) {
     const double *up = states + (t - 1) * nf;
     double *u = states + t * nf;
+    double dt = dts[t - 1];
+    /* M_t = S + dt_t A */
+    for (int64_t j = 0; j < nf; j++)
+      for (int64_t k = rp[j]; k < rp[j + 1]; k++) mv[k] = dt * v[k] + (ci[k] == j ? s[j] : 0.0);
     for (int64_t j = 0; j < nf; j++)
       rhs[j] = s[j] * up[j] + dt * (qscale[t] * Q[f2n[j]] + bD[j]);
     memcpy(u, up, sizeof(double) * nf);
@@ -458,18 +460,24 @@ int orc_transient(int64_t n, int64_t ne, const int64_t *nb, const double *K,
   return st;
 }
 
-/* Discrete adjoint of orc_transient for J = J(u_1..u_N) with dJdu
- * ((nsteps+1) x nf, row 0 ignored):
- *   M' mu_N = dJ/du_N,   M' mu_n = dJ/du_n + S mu_{n+1}
- *   dJ/dK_e = -sum_n dt aol_e (u~n_a - u~n_b)(mu~n_a - mu~n_b)
+/* Discrete adjoint of orc_transient for J = J(u_0..u_N) with dJdu
+ * ((nsteps+1) x nf, row n = dJ/du_n), M_n = S + dt_n A (symmetric):
+ *   M_N mu_N = dJ/du_N,   M_n mu_n = dJ/du_n + S mu_{n+1}   (n = N-1 .. 1)
+ *   dJ/du_0  = dJ/du_0 + S mu_1
+ *   dJ/dK_e  = -sum_n dt_n aol_e (u~n_a - u~n_b)(mu~n_a - mu~n_b)
+ *   dJ/dQ_i  = sum_n dt_n qscale[n] mu_n[j(i)] on free cells, 0 on Dirichlet cells
+ *   dJ/ddh_o = sum_n dt_n sum over edges e = (o, free i) of K_e aol_e mu_n[j(i)]
+ *              on Dirichlet cells (the tangent of dt_n b_D, b_D_j = sum K_e aol_e dh_o),
+ *              0 on free cells
  * where ~ puts dirichleths (states) or 0 (multipliers) on Dirichlet cells.
- * Also returns dJ/dQ_i = sum_n dt qscale[n] mu_n[j(i)] on free cells. */
+ * ddh and du0 may be NULL. */
 int orc_transient_adjoint(int64_t n, int64_t ne, const int64_t *nb, const double *K,
                           const double *aol, int64_t nd, const int64_t *dnodes,
                           const double *dh, const double *Ss, const double *V,
-                          int64_t nsteps, double dt, const double *qscale,
+                          int64_t nsteps, const double *dts, const double *qscale,
                           const double *states, const double *dJdu, double reltol,
-                          int64_t maxiter, double *dK, double *dQ, double *info) {
+                          int64_t maxiter, double *dK, double *dQ, double *ddh, double *du0,
+                          double *info) {
   uint8_t *isfree = (uint8_t *)malloc((size_t)n);
   int64_t *n2f = (int64_t *)malloc(sizeof(int64_t) * n);
   int64_t *f2n = (int64_t *)malloc(sizeof(int64_t) * n);
@@ -480,15 +488,18 @@ int orc_transient_adjoint(int64_t n, int64_t ne, const int64_t *nb, const double
   double *s = (double *)malloc(sizeof(double) * (nf + 1));
   for (int64_t j = 0; j < nf; j++) s[j] = Ss[f2n[j]] * V[f2n[j]];
   double *mv = (double *)malloc(sizeof(double) * (rp[nf] + 1));
-  for (int64_t j = 0; j < nf; j++)
-    for (int64_t k = rp[j]; k < rp[j + 1]; k++) mv[k] = dt * v[k] + (ci[k] == j ? s[j] : 0.0);
   double *mu = (double *)calloc((size_t)nf + 1, sizeof(double));
   double *rhs = (double *)malloc(sizeof(double) * (nf + 1));
   for (int64_t e = 0; e < ne; e++) dK[e] = 0.0;
   for (int64_t i = 0; i < n; i++) dQ[i] = 0.0;
+  if (ddh)
+    for (int64_t i = 0; i < n; i++) ddh[i] = 0.0;
   int st = 0;
   double itsum = 0.0, worst = 0.0, inf[2];
   for (int64_t t = nsteps; t >= 1; t--) {
+    double dt = dts[t - 1];
+    for (int64_t j = 0; j < nf; j++)
+      for (int64_t k = rp[j]; k < rp[j + 1]; k++) mv[k] = dt * v[k] + (ci[k] == j ? s[j] : 0.0);
     for (int64_t j = 0; j < nf; j++) rhs[j] = dJdu[t * nf + j] + s[j] * mu[j];
     int r = orc_pcg(nf, rp, ci, mv, rhs, mu, reltol, maxiter, inf); /* M symmetric */
     if (r) st = r;
@@ -500,9 +511,15 @@ int orc_transient_adjoint(int64_t n, int64_t ne, const int64_t *nb, const double
       double ua = isfree[a] ? u[n2f[a]] : dh[a], ub = isfree[b] ? u[n2f[b]] : dh[b];
       double ma = isfree[a] ? mu[n2f[a]] : 0.0, mb = isfree[b] ? mu[n2f[b]] : 0.0;
       dK[e] -= dt * aol[e] * (ua - ub) * (ma - mb);
+      if (ddh && a != b) {
+        if (!isfree[a] && isfree[b]) ddh[a] += dt * K[e] * aol[e] * mb;
+        if (!isfree[b] && isfree[a]) ddh[b] += dt * K[e] * aol[e] * ma;
+      }
     }
     for (int64_t j = 0; j < nf; j++) dQ[f2n[j]] += dt * qscale[t] * mu[j];
   }
+  if (du0)
+    for (int64_t j = 0; j < nf; j++) du0[j] = dJdu[j] + (nsteps > 0 ? s[j] * mu[j] : 0.0);
   info[0] = itsum;
   info[1] = worst;
   free(isfree); free(n2f); free(f2n); free(rp); free(ci); free(v); free(s); free(mv);

@@ -41,11 +41,15 @@ Function (restates) -- pinned by (tests/test_oracle.py unless noted)
   exp(0.5 (logK_a + logK_b)) of examples/gw_steadystate_3d/inputdeck.jl:51
   -- central finite differences; closed form sqrt(k_a k_b).
 * ``transient``/``transient_adjoint`` examples/gw_transient_3d/inputdeck.jl:
-  60-92 (f = -groundwater_residuals) with fixed-step backward Euler in place
-  of DifferentiableBackwardEuler.steps
-  -- a dense restatement in the reference's (I - dt f_u) form; the Theis
-  drawdown within 0.1 m (test/theis.jl:12-21,102); the steady limit;
-  central finite differences of the final-state objective.
+  60-92 (f = -groundwater_residuals, parameters p = [Ks; dirichleths] at
+  :63,86-91) with backward Euler over a given step sequence in place of
+  DifferentiableBackwardEuler.steps
+  -- a dense restatement in the reference's (I - dt f_u) form, uniform and
+  non-uniform steps; the Theis drawdown within 0.1 m (test/theis.jl:12-21,
+  102); the steady limit; central finite differences of objectives of one
+  and of several states for every tangent (Ks, dirichleths, Qs, u0); the
+  tangent in dirichleths against the dense product of the reference's
+  f_gw_p block -groundwater_dirichleths.
   The reference integrates with adaptive steps inside DiffEq; matching its
   exact step sequence would need that package: parity unpinned for the
   adaptive sequence, pinned for fixed steps by the checks above.
@@ -271,9 +275,19 @@ def geo_pullback(logK, neighbors, dKe):
                             _p(lk, "f"), _p(dk, "f"), _p(out, "f"))
     return out
 
+def _dts(dt, nsteps):
+    """Per-step lengths from a scalar dt or a sequence of nsteps lengths."""
+    d = np.asarray(dt, dtype=np.float64)
+    d = np.full(int(nsteps), float(d)) if d.ndim == 0 else np.ascontiguousarray(d)
+    if d.shape != (int(nsteps),):
+        raise ValueError("dt must be a scalar or have nsteps entries")
+    return d
+
+
 def transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
               specificstorage, volumes, u0, nsteps, dt, qscale, reltol=1e-13, maxiter=20000):
-    """Fixed-step backward Euler; returns states (nsteps+1, nf) on free cells."""
+    """Backward Euler with step lengths dt (a scalar, or one per step);
+    returns states (nsteps+1, nf) on free cells."""
     n = len(Qs)
     nb = _edges0(neighbors)
     ne = nb.shape[0]
@@ -283,12 +297,13 @@ def transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
     dh, Q, Ss, V = _f64(dirichleths), _f64(Qs), _f64(specificstorage), _f64(volumes)
     u0 = _f64(u0)
     qs = _f64(qscale)
+    dts = _dts(dt, nsteps)
     states = np.empty((nsteps + 1, nf))
     info = np.zeros(2)
     st = _lib().orc_transient(ctypes.c_int64(n), ctypes.c_int64(ne), _p(nb, "i"), _p(K, "f"),
                               _p(aol, "f"), ctypes.c_int64(d.size), _p(d, "i"), _p(dh, "f"),
                               _p(Q, "f"), _p(Ss, "f"), _p(V, "f"), _p(u0, "f"),
-                              ctypes.c_int64(nsteps), ctypes.c_double(dt), _p(qs, "f"),
+                              ctypes.c_int64(nsteps), _p(dts, "f"), _p(qs, "f"),
                               ctypes.c_double(reltol), ctypes.c_int64(maxiter),
                               _p(states, "f"), _p(info, "f"))
     if st:
@@ -298,8 +313,10 @@ def transient(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
 
 def transient_adjoint(Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
                       specificstorage, volumes, nsteps, dt, qscale, states, dJdu,
-                      reltol=1e-13, maxiter=20000):
-    """Returns (dJ/dKs (E,), dJ/dQs (n,)) for an objective of the states."""
+                      reltol=1e-13, maxiter=20000, full=False):
+    """Reverse mode of ``transient`` for an objective of the states; dJdu is
+    (nsteps+1, nf), row n = dJ/du_n.  Returns (dJ/dKs (E,), dJ/dQs (n,)), or
+    with ``full`` (dJ/dKs, dJ/ddirichleths (n,), dJ/dQs, dJ/du0 (nf,))."""
     n = len(volumes)
     nb = _edges0(neighbors)
     ne = nb.shape[0]
@@ -307,15 +324,18 @@ def transient_adjoint(Ks, neighbors, areasoverlengths, dirichletnodes, dirichlet
     K, aol = _f64(Ks), _f64(np.asarray(areasoverlengths)[:ne])
     dh, Ss, V = _f64(dirichleths), _f64(specificstorage), _f64(volumes)
     qs, S, G = _f64(qscale), _f64(states), _f64(dJdu)
+    dts = _dts(dt, nsteps)
     dK = np.empty(ne)
     dQ = np.empty(n)
+    ddh = np.empty(n)
+    du0 = np.empty(S.shape[1])
     info = np.zeros(2)
     st = _lib().orc_transient_adjoint(
         ctypes.c_int64(n), ctypes.c_int64(ne), _p(nb, "i"), _p(K, "f"), _p(aol, "f"),
         ctypes.c_int64(d.size), _p(d, "i"), _p(dh, "f"), _p(Ss, "f"), _p(V, "f"),
-        ctypes.c_int64(nsteps), ctypes.c_double(dt), _p(qs, "f"), _p(S, "f"), _p(G, "f"),
+        ctypes.c_int64(nsteps), _p(dts, "f"), _p(qs, "f"), _p(S, "f"), _p(G, "f"),
         ctypes.c_double(reltol), ctypes.c_int64(maxiter), _p(dK, "f"), _p(dQ, "f"),
-        _p(info, "f"))
+        _p(ddh, "f"), _p(du0, "f"), _p(info, "f"))
     if st:
         raise OracleError(f"oracle transient adjoint status {st}")
-    return dK, dQ
+    return (dK, ddh, dQ, du0) if full else (dK, dQ)

@@ -524,3 +524,48 @@ def test_linear_solver_choice(pkg, api, oracle):
     assert rel(h, h_ref) < rel(h2, h_ref)
     with pytest.raises(TypeError):
         api.groundwater_steadystate(*args, linear_solver=print)
+
+
+def test_user_stream_call_then_topology_stream_call(pkg, api, oracle):
+    """A device-memory dpf_hm_forward queued on a user stream, followed at once
+    by a host-memory dpf_hm_pullback on the topology's own stream, which takes
+    pooled buffers of the same size (the exp(logK) scratch of both calls):
+    neither result may be disturbed by the other.  Device memory and the
+    stream come from the HIP runtime the library itself links."""
+    import ctypes
+
+    from dpfehm_jl_amd import _lib
+    _say("streams")
+    L = _lib.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    vp = ctypes.c_void_p
+
+    def ok(code):
+        assert code == 0, code
+
+    w = pkg.workloads.steady_2m(ns=(96, 96, 48), maxs=(48.0, 48.0, 24.0))
+    topo = api.get_topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    stream = vp()
+    ok(hip.hipStreamCreateWithFlags(ctypes.byref(stream), 1))
+    d_lk, d_Ke = vp(), vp()
+    ok(hip.hipMalloc(ctypes.byref(d_lk), ctypes.c_size_t(8 * w.n)))
+    ok(hip.hipMalloc(ctypes.byref(d_Ke), ctypes.c_size_t(8 * topo.ne)))
+    lk = np.ascontiguousarray(w.logK)
+    ok(hip.hipMemcpy(d_lk, lk.ctypes.data_as(vp), ctypes.c_size_t(8 * w.n), 1))
+    o = api._opts({}, memory=_lib.DPF_DEVICE, stream=stream)
+    lk2 = np.ascontiguousarray(w.logK[::-1])
+    K2 = oracle.hm_forward(lk2, w.neighbors)
+    dK = np.random.default_rng(0).normal(size=topo.ne)
+    Ke = np.empty(topo.ne)
+    try:
+        for _ in range(3):
+            _lib.check(L.dpf_hm_forward(topo.handle, d_lk, d_Ke, ctypes.byref(o)))
+            g = api.hm_pullback(topo, lk2, K2, dK)
+            ok(hip.hipStreamSynchronize(stream))
+            ok(hip.hipMemcpy(Ke.ctypes.data_as(vp), d_Ke, ctypes.c_size_t(8 * topo.ne), 2))
+            assert rel(Ke, oracle.hm_forward(w.logK, w.neighbors)) < 1e-14
+            assert rel(g, oracle.hm_pullback(lk2, w.neighbors, dK)) < 1e-13
+    finally:
+        hip.hipFree(d_lk)
+        hip.hipFree(d_Ke)
+        hip.hipStreamDestroy(stream)

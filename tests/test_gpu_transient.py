@@ -44,7 +44,7 @@ def test_transient_forward_adjoint_vs_oracle(pkg, oracle, precision, ck):
     dK_ref, dQ_ref = oracle.transient_adjoint(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
                                               w.dirichleths, Ss, w.volumes, N, dt, qs, ref, G,
                                               reltol=1e-14)
-    dK, dQ = pb(g)
+    dK, _, dQ, _ = pb(g)
     assert rel(dK, dK_ref) < 1e-7
     assert rel(dQ, dQ_ref) < 1e-7
 
@@ -73,7 +73,8 @@ def test_stiff_adjoint_long_horizon(pkg, oracle, precision):
         uN, pb = api.rrule_groundwater_transient(*args, np.zeros(nf), N, dt, np.ones(N + 1),
                                                  checkpoint_every=25, reltol=1e-12,
                                                  precision=precision)
-        out[N] = (uN,) + tuple(pb(g))
+        dK_, _, dQ_, _ = pb(g)
+        out[N] = (uN, dK_, dQ_)
     ref = oracle.transient(*args, np.zeros(nf), 20, dt, np.ones(21), reltol=1e-14)
     G = np.zeros_like(ref)
     G[-1] = g
@@ -102,3 +103,54 @@ def test_transient_states_and_steady_limit(pkg):
     isfree = np.ones(w.n, dtype=bool)
     isfree[w.dirichletnodes - 1] = False
     assert rel(states[-1], h[isfree]) < 1e-8
+
+
+@pytest.mark.parametrize("precision,ck", [("fp64", 3), ("fp64", 0), ("mixed", 4),
+                                          ("mixed_pc", 1)])
+def test_transient_step_sequence_every_tangent_vs_oracle(pkg, oracle, precision, ck):
+    """A caller-given, non-uniform step sequence and an objective of several
+    states (u_0 included): all four tangents of the reverse mode (Ks,
+    dirichleths, Qs, u0) against the oracle's."""
+    from dpfehm_jl_amd import api
+    print("transient dts", precision, ck, flush=True)
+    w, Ss = _problem(pkg, ns=(18, 16, 8))
+    rng = np.random.default_rng(11)
+    w.dirichleths[w.dirichletnodes - 1] = rng.normal(size=w.dirichletnodes.size)
+    K = w.Ks()
+    nf = w.n - w.dirichletnodes.size
+    N = 13
+    dts = 3600.0 * rng.uniform(0.05, 2.0, N)
+    qs = np.minimum(1.0, np.concatenate([[0.0], np.cumsum(dts)]) / 20000.0)
+    u0 = 0.01 * rng.normal(size=nf)
+    args = (K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs, Ss,
+            w.volumes)
+    ref = oracle.transient(*args, u0, N, dts, qs, reltol=1e-14)
+    st = {}
+    uN, pb = api.rrule_groundwater_transient(*args, u0, N, dts, qs, checkpoint_every=ck,
+                                             reltol=1e-12, precision=precision, stats=st)
+    states = api.groundwater_transient(*args, u0, N, dts, qs, return_states=True,
+                                       reltol=1e-12, precision=precision)
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    assert rel(uN, ref[-1]) < tol
+    assert rel(states, ref) < tol and np.array_equal(states[-1], uN)
+    G = np.zeros_like(ref)
+    for k in (0, 4, 9, 11):
+        G[k, rng.choice(nf, 25, replace=False)] = rng.normal(size=25)
+    dK_r, ddh_r, dQ_r, du0_r = oracle.transient_adjoint(
+        K, w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, Ss, w.volumes, N,
+        dts, qs, ref, G, reltol=1e-14, full=True)
+    dK, ddh, dQ, du0 = pb(G)
+    print(rel(dK, dK_r), rel(ddh, ddh_r), rel(dQ, dQ_r), rel(du0, du0_r), flush=True)
+    assert rel(dK, dK_r) < 1e-7
+    assert rel(ddh, ddh_r) < 1e-7
+    assert rel(dQ, dQ_r) < 1e-7
+    assert rel(du0, du0_r) < 1e-7
+    isfree = np.ones(w.n, dtype=bool)
+    isfree[w.dirichletnodes - 1] = False
+    assert np.all(ddh[isfree] == 0.0) and np.all(dQ[~isfree] == 0.0)
+    # a seed at u_0 alone: du0 is the seed, every other tangent zero
+    G0 = np.zeros_like(ref)
+    G0[0] = rng.normal(size=nf)
+    dK0, ddh0, dQ0, du00 = pb(G0)
+    assert np.array_equal(du00, G0[0])
+    assert not dK0.any() and not ddh0.any() and not dQ0.any()

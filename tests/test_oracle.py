@@ -301,27 +301,29 @@ def test_geo_chain_rule(pkg, oracle):
 # ---------------------------------------------------------------- transient
 
 def _dbe_dense(oracle, K, nb, aol, dn, dh, Q, Ss, V, u0, nsteps, dt, qscale):
-    """Backward Euler in the reference's form: u_n = u_{n-1} + dt f(u_n) with
-    f = -groundwater_residuals, solved densely as (I - dt f_u) u_n = u_{n-1}
-    + dt (f(0) at t_n)."""
+    """Backward Euler in the reference's form: u_n = u_{n-1} + dt_n f(u_n) with
+    f = -groundwater_residuals, solved densely as (I - dt_n f_u) u_n = u_{n-1}
+    + dt_n (f(0) at t_n); dt is a scalar or one length per step."""
     nf = u0.size
+    dts = np.broadcast_to(np.asarray(dt, dtype=float), (nsteps,))
     us = [u0]
     for t in range(1, nsteps + 1):
         f0 = -oracle.groundwater_residuals(np.zeros(nf), K, nb, aol, dn, dh, Q * qscale[t], Ss, V)
         fu = -oracle.groundwater_h(np.zeros(nf), K, nb, aol, dn, dh, Q, Ss, V).toarray()
-        us.append(np.linalg.solve(np.eye(nf) - dt * fu, us[-1] + dt * f0))
+        us.append(np.linalg.solve(np.eye(nf) - dts[t - 1] * fu, us[-1] + dts[t - 1] * f0))
     return np.array(us)
 
 
-def test_transient_matches_dense_reference_form(pkg, oracle):
+@pytest.mark.parametrize("dt", [0.7, [0.2, 1.5, 0.05, 0.7, 3.0]])
+def test_transient_matches_dense_reference_form(pkg, oracle, dt):
     nb, aol, vol, dn, dh, Q, K, n = _tiny(pkg, seed=14)
     rng = np.random.default_rng(15)
     Ss = rng.uniform(0.1, 1.0, n)
     nf = n - dn.size
     u0 = rng.normal(size=nf)
     qs = np.linspace(0, 1, 6)
-    st = oracle.transient(K, nb, aol, dn, dh, Q, Ss, vol, u0, 5, 0.7, qs)
-    ref = _dbe_dense(oracle, K, nb, aol, dn, dh, Q, Ss, vol, u0, 5, 0.7, qs)
+    st = oracle.transient(K, nb, aol, dn, dh, Q, Ss, vol, u0, 5, dt, qs)
+    ref = _dbe_dense(oracle, K, nb, aol, dn, dh, Q, Ss, vol, u0, 5, dt, qs)
     assert np.allclose(st, ref, rtol=1e-11, atol=1e-11)
 
 
@@ -364,6 +366,89 @@ def test_transient_adjoint_fd(pkg, oracle):
         assert np.isclose(dQ[i], (J(K, qp) - J(K, qm)) / (2 * eps), rtol=1e-6, atol=1e-9)
 
 
+def _transient_case(pkg, seed):
+    nb, aol, vol, dn, dh, Q, K, n = _tiny(pkg, ns=(3, 3, 3), seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    Ss = rng.uniform(0.1, 1.0, n)
+    nf = n - dn.size
+    u0 = rng.normal(size=nf)
+    N = 7
+    dts = rng.uniform(0.05, 0.8, N)
+    qs = np.linspace(0, 1, N + 1)
+    # objective of several states, u_0 included: J = sum_n W[n] . u_n
+    W = np.zeros((N + 1, nf))
+    for k in (0, 2, 5, 7):
+        W[k] = rng.normal(size=nf)
+    return nb, aol, vol, dn, dh, Q, K, n, Ss, nf, u0, N, dts, qs, W
+
+
+def test_transient_adjoint_fd_every_tangent(pkg, oracle):
+    """Reverse mode of a non-uniform step sequence for an objective of several
+    states: every tangent (Ks, dirichleths, Qs, u0) against central finite
+    differences of the forward run."""
+    nb, aol, vol, dn, dh, Q, K, n, Ss, nf, u0, N, dts, qs, W = _transient_case(pkg, 21)
+
+    def J(K_=K, dh_=dh, Q_=Q, u0_=u0):
+        return np.sum(W * oracle.transient(K_, nb, aol, dn, dh_, Q_, Ss, vol, u0_, N, dts, qs))
+
+    st = oracle.transient(K, nb, aol, dn, dh, Q, Ss, vol, u0, N, dts, qs)
+    dK, ddh, dQ, du0 = oracle.transient_adjoint(K, nb, aol, dn, dh, Ss, vol, N, dts, qs, st, W,
+                                                full=True)
+    eps = 1e-6
+    for e in range(0, nb.shape[0], 2):
+        Kp, Km = K.copy(), K.copy()
+        Kp[e] *= 1 + eps
+        Km[e] *= 1 - eps
+        assert np.isclose(dK[e], (J(K_=Kp) - J(K_=Km)) / (2 * eps * K[e]), rtol=1e-6, atol=1e-9)
+    free = np.ones(n, dtype=bool)
+    free[dn - 1] = False
+    for i in range(n):
+        hp, hm = dh.copy(), dh.copy()
+        hp[i] += eps
+        hm[i] -= eps
+        fd = (J(dh_=hp) - J(dh_=hm)) / (2 * eps)
+        assert np.isclose(ddh[i], fd, rtol=1e-6, atol=1e-9), i
+        if free[i]:
+            assert ddh[i] == 0.0
+        qp, qm = Q.copy(), Q.copy()
+        qp[i] += eps
+        qm[i] -= eps
+        assert np.isclose(dQ[i], (J(Q_=qp) - J(Q_=qm)) / (2 * eps), rtol=1e-6, atol=1e-9)
+    for j in range(nf):
+        up, um = u0.copy(), u0.copy()
+        up[j] += eps
+        um[j] -= eps
+        assert np.isclose(du0[j], (J(u0_=up) - J(u0_=um)) / (2 * eps), rtol=1e-6, atol=1e-9)
+    assert np.any(ddh != 0.0)
+
+
+def test_transient_ddh_dense_f_gw_p(pkg, oracle):
+    """The dirichleths tangent as the reference's reverse sweep forms it: the
+    dense adjoint recursion with the parameter block of f_gw_p,
+    -groundwater_dirichleths (examples/gw_transient_3d/inputdeck.jl:86-91),
+    built column by column from groundwater_residuals (affine in dh)."""
+    nb, aol, vol, dn, dh, Q, K, n, Ss, nf, u0, N, dts, qs, W = _transient_case(pkg, 23)
+    st = oracle.transient(K, nb, aol, dn, dh, Q, Ss, vol, u0, N, dts, qs)
+    _, ddh, _, du0 = oracle.transient_adjoint(K, nb, aol, dn, dh, Ss, vol, N, dts, qs, st, W,
+                                              full=True)
+    r0 = oracle.groundwater_residuals(np.zeros(nf), K, nb, aol, dn, dh, Q, Ss, vol)
+    f_p = np.empty((nf, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        f_p[:, i] = -(oracle.groundwater_residuals(np.zeros(nf), K, nb, aol, dn, dh + e, Q, Ss,
+                                                   vol) - r0)
+    f_u = -oracle.groundwater_h(np.zeros(nf), K, nb, aol, dn, dh, Q, Ss, vol).toarray()
+    lam = np.zeros(nf)
+    g = np.zeros(n)
+    for t in range(N, 0, -1):
+        # u_t = u_{t-1} + dt f(u_t): (I - dt f_u)' lam_t = W_t + lam_{t+1}
+        lam = np.linalg.solve((np.eye(nf) - dts[t - 1] * f_u).T, W[t] + lam)
+        g += dts[t - 1] * f_p.T @ lam
+    assert np.allclose(ddh, g, rtol=1e-9, atol=1e-12)
+    assert np.allclose(du0, W[0] + lam, rtol=1e-9, atol=1e-12)
+
+
 def test_theis_drawdown(pkg, oracle):
     """test/theis.jl:27-102 (atol 0.1 m) with 200 fixed backward-Euler steps
     (geometric) in place of the adaptive DiffEq integrator."""
@@ -396,3 +481,30 @@ def test_theis_drawdown(pkg, oracle):
 
     theis = np.array([Q * W(r ** 2 * S / (4 * T * tfinal)) / (4 * np.pi * T) for r in rs])
     assert np.allclose(theis, 1e3 - h[good], atol=1e-1)
+
+
+# ---------------------------------------------------------------- multi-core CPU baseline
+
+def test_cpu_amg_baseline_matches_oracle(pkg, oracle):
+    """The multi-core SA-AMG-PCG baseline (oracle/amg_cpu.cpp) evaluates the
+    same objective, heads and logK gradient as the scalar oracle."""
+    from oracle import amg_cpu
+    w = pkg.workloads.steady_2m(sigma=2.0, ns=(40, 36, 20), maxs=(20.0, 18.0, 10.0))
+    rng = np.random.default_rng(5)
+    hobs = rng.normal(scale=1e-2, size=w.obs.size)
+    cpu = amg_cpu.CpuAmg(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes, threads=4)
+    assert cpu.levels >= 3
+    J, g, info, h = cpu.eval(w.logK, w.dirichleths, w.Qs, w.obs, hobs, reltol=1e-13,
+                             want_h=True)
+    K = oracle.hm_forward(w.logK, w.neighbors)
+    h_ref, pb = oracle.steadystate_rrule(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
+                                         w.dirichleths, w.Qs, reltol=1e-14)
+    delta = np.zeros(w.n)
+    delta[w.obs - 1] = 2.0 * (h_ref[w.obs - 1] - hobs)
+    g_ref = oracle.hm_pullback(w.logK, w.neighbors, pb(delta)[1])
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
+    assert rel(h, h_ref) < 1e-11
+    assert np.isclose(J, np.sum((h_ref[w.obs - 1] - hobs) ** 2), rtol=1e-10)
+    assert rel(g, g_ref) < 1e-9
+    assert 0 < info[0] < 60 and 0 < info[1] < 60
+    cpu.close()
