@@ -336,7 +336,8 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
                                                  double abstol, int maxiter, int abs_target,
                                                  int seq, HostMirror* hm,
                                                  double* __restrict__ hm_outer) {
-  if (mode != FIN_START && mode < FIN_OUTER && *gdone) return;
+  // the early-exit flag is tested once the partials' loads are in flight
+  const int gd = (mode != FIN_START && mode < FIN_OUTER) ? *gdone : 0;
   constexpr int NW = 16;  // waves of the 1024-thread workgroup
   __shared__ double sh[NW], sh_rr[NW];
   __shared__ int sh_flags[NW], sh_maxit[NW], sh_active[NW];
@@ -369,6 +370,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
 #pragma unroll
       for (int k = 0; k < 8; k++) s = mx ? fmax(s, v[k]) : s + v[k];
     }
+    if (gd) return;  // uniform: before the barrier
     s = mx ? wave_max(s) : wave_sum(s);
     if (lane == 63) sh[wave] = s;
     __syncthreads();
@@ -381,6 +383,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     fold(q);
     if (mode == FIN_OUTER) mput(&hm_outer[0], q.outer_s);
   } else {
+    if (gd) return;
     for (int m = wave; m < nm; m += NW) {
       // every block of the member's range but the first starts with the
       // member (slot 0); the first may start with the previous member
@@ -458,6 +461,39 @@ __device__ __forceinline__ void sell_row(const SellD<T>& A, int64_t r, int64_t& 
   int64_t p0 = A.sptr[s];
   base = p0 + (r & 63);
   w = (int)((A.sptr[s + 1] - p0) >> 6);
+}
+
+// The scalar part of a SELL row -- its slice's start and width and, for an
+// offset slice, the slice's column offsets -- from loads that issue together,
+// so that a kernel can put them in flight before it waits for anything else
+// (the early-exit flag in particular).  Every lane of a wave is in the same
+// slice; r must be a valid row.
+struct SellRow {
+  int64_t base;
+  int w;
+  int offs;  // nonzero: an offset slice, slot k of row r has column r + o[k]
+  int32_t o[8];
+};
+
+template <typename T>
+__device__ __forceinline__ SellRow sell_row_at(const SellD<T>& A, int64_t r) {
+  SellRow R;
+  int s = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+  int64_t p0 = A.sptr[s];
+  R.base = p0 + (r & 63);
+  R.w = (int)((A.sptr[s + 1] - p0) >> 6);
+  R.offs = 0;
+#pragma unroll
+  for (int u = 0; u < 8; u++) R.o[u] = 0;
+  if (A.soff) {
+    // the record always holds 8 offsets (zero past the width): no
+    // condition on its first word before the others load
+    const int32_t* q = A.soff + 9 * (int64_t)s;
+    R.offs = q[0];
+#pragma unroll
+    for (int u = 0; u < 8; u++) R.o[u] = q[1 + u];
+  }
+  return R;
 }
 
 // ------------------------------------------------------------------ edge maps
@@ -864,84 +900,179 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 
 // ------------------------------------------------------------------ V-cycle
 
+// Groups of N consecutive entries of one SELL row (N known at compile time):
+// all values and columns load, then all gathers, then the sums in ascending
+// entry order.  A whole group sits in one basic block, so its loads issue
+// back to back.  (Entry loads guarded by per-entry conditions make the
+// compiler wait for each load at the join of its branch, which serializes
+// the group.)
+template <int N, typename T, typename TX>
+__device__ __forceinline__ T dot_group(const SellD<T>& A, int64_t e0,
+                                       const TX* __restrict__ x, T s) {
+  T v[N];
+  int32_t c[N];
+  TX xv[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    v[u] = A.val[e0 + (int64_t)u * 64];
+    c[u] = A.col[e0 + (int64_t)u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) xv[u] = x[c[u]];
+#pragma unroll
+  for (int u = 0; u < N; u++) s += v[u] * (T)xv[u];
+  return s;
+}
+
+// The same over an offset slice's N slots: columns r + o[u], no column loads.
+template <int N, typename T, typename TX>
+__device__ __forceinline__ T dot_group_off(const SellD<T>& A, int64_t e0, int32_t r,
+                                           const int32_t* o, const TX* __restrict__ x, T s) {
+  T v[N];
+  TX xv[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    v[u] = A.val[e0 + (int64_t)u * 64];
+    xv[u] = x[r + o[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) s += v[u] * (T)xv[u];
+  return s;
+}
+
+// The p-forming row sum sum_u A[e_u] (z[c_u] + beta p[c_u]) over a group.
+template <int N, bool OFF, typename T, typename TZ>
+__device__ __forceinline__ T dot_group_zp(const SellD<T>& A, int64_t e0, int32_t r,
+                                          const int32_t* o, const TZ* __restrict__ z,
+                                          const T* __restrict__ p, T beta, T s) {
+  T v[N], zv[N], pv[N];
+  int32_t c[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    v[u] = A.val[e0 + (int64_t)u * 64];
+    c[u] = OFF ? r + o[u] : A.col[e0 + (int64_t)u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    zv[u] = (T)z[c[u]];
+    pv[u] = p[c[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) s += v[u] * (zv[u] + beta * pv[u]);
+  return s;
+}
+
 // sum_k A.val[e_k] x[A.col[e_k]] over one SELL row (e_k = base + 64 k, k < w),
-// added in ascending k.  The loads go out in groups of 8 entries (values and
-// columns first, then the gathers), so a lane has up to 8 independent gather
-// chains in flight instead of one: the sweeps are latency-bound otherwise.
+// added in ascending k, in groups of 8 entries and one final group of the
+// remaining 1..7 (the width is the same for every lane of a slice, so the
+// group sizes are uniform): a lane keeps up to 8 independent gather chains in
+// flight, where one chain at a time would leave the sweeps latency-bound.
+// In an offset slice (fine operator) the columns come from the slice record
+// already in registers, so the row's value and x loads all issue at once.
 template <typename T, typename TX>
-__device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t r, int64_t base, int w,
+__device__ __forceinline__ T sell_dot(const SellD<T>& A, const SellRow& R, int64_t r,
                                       const TX* __restrict__ x) {
-  constexpr int U = 8;
-  const int32_t* so = slice_offsets(A, r);
+  const int64_t base = R.base;
+  const int w = R.w;
   T s = (T)0;
-  for (int k0 = 0; k0 < w; k0 += U) {
-    T v[U];
-    int32_t c[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      bool ok = k0 + u < w;
-      int64_t e = base + (int64_t)(k0 + u) * 64;
-      v[u] = ok ? A.val[e] : (T)0;
-      c[u] = ok ? (so ? (int32_t)r + so[k0 + u] : A.col[e]) : 0;
+  if (R.offs) {
+    const int32_t rr = (int32_t)r;
+    switch (w) {
+      case 1: return dot_group_off<1>(A, base, rr, R.o, x, s);
+      case 2: return dot_group_off<2>(A, base, rr, R.o, x, s);
+      case 3: return dot_group_off<3>(A, base, rr, R.o, x, s);
+      case 4: return dot_group_off<4>(A, base, rr, R.o, x, s);
+      case 5: return dot_group_off<5>(A, base, rr, R.o, x, s);
+      case 6: return dot_group_off<6>(A, base, rr, R.o, x, s);
+      case 7: return dot_group_off<7>(A, base, rr, R.o, x, s);
+      default: return dot_group_off<8>(A, base, rr, R.o, x, s);
     }
-    T xv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) xv[u] = k0 + u < w ? (T)x[c[u]] : (T)0;
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (k0 + u < w) s += v[u] * xv[u];
+  }
+  int k0 = 0;
+  for (; k0 + 8 <= w; k0 += 8) s = dot_group<8>(A, base + (int64_t)k0 * 64, x, s);
+  const int64_t e0 = base + (int64_t)k0 * 64;
+  switch (w - k0) {
+    case 1: s = dot_group<1>(A, e0, x, s); break;
+    case 2: s = dot_group<2>(A, e0, x, s); break;
+    case 3: s = dot_group<3>(A, e0, x, s); break;
+    case 4: s = dot_group<4>(A, e0, x, s); break;
+    case 5: s = dot_group<5>(A, e0, x, s); break;
+    case 6: s = dot_group<6>(A, e0, x, s); break;
+    case 7: s = dot_group<7>(A, e0, x, s); break;
+    default: break;
   }
   return s;
 }
 
+// The same row sum for a caller that has only the row's start and width.
+template <typename T, typename TX>
+__device__ __forceinline__ T sell_dot(const SellD<T>& A, int64_t r, int64_t base, int w,
+                                      const TX* __restrict__ x) {
+  SellRow R = sell_row_at(A, r);
+  R.base = base;
+  R.w = w;
+  return sell_dot(A, R, r, x);
+}
+
 // sum_k A.val[e_k] (z[c_k] + beta p[c_k]), same grouping and order as sell_dot
 template <typename T, typename TZ>
-__device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, int64_t r, int64_t base, int w,
+__device__ __forceinline__ T sell_dot_zp(const SellD<T>& A, const SellRow& R, int64_t r,
                                          const TZ* __restrict__ z, const T* __restrict__ p,
                                          T beta) {
-  constexpr int U = 8;
-  const int32_t* so = slice_offsets(A, r);
+  const int64_t base = R.base;
+  const int w = R.w;
+  const int32_t rr = (int32_t)r;
   T s = (T)0;
-  for (int k0 = 0; k0 < w; k0 += U) {
-    T v[U];
-    int32_t c[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      bool ok = k0 + u < w;
-      int64_t e = base + (int64_t)(k0 + u) * 64;
-      v[u] = ok ? A.val[e] : (T)0;
-      c[u] = ok ? (so ? (int32_t)r + so[k0 + u] : A.col[e]) : 0;
+  if (R.offs) {
+    switch (w) {
+      case 1: return dot_group_zp<1, true>(A, base, rr, R.o, z, p, beta, s);
+      case 2: return dot_group_zp<2, true>(A, base, rr, R.o, z, p, beta, s);
+      case 3: return dot_group_zp<3, true>(A, base, rr, R.o, z, p, beta, s);
+      case 4: return dot_group_zp<4, true>(A, base, rr, R.o, z, p, beta, s);
+      case 5: return dot_group_zp<5, true>(A, base, rr, R.o, z, p, beta, s);
+      case 6: return dot_group_zp<6, true>(A, base, rr, R.o, z, p, beta, s);
+      case 7: return dot_group_zp<7, true>(A, base, rr, R.o, z, p, beta, s);
+      default: return dot_group_zp<8, true>(A, base, rr, R.o, z, p, beta, s);
     }
-    T zv[U], pv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      bool ok = k0 + u < w;
-      zv[u] = ok ? (T)z[c[u]] : (T)0;
-      pv[u] = ok ? p[c[u]] : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (k0 + u < w) s += v[u] * (zv[u] + beta * pv[u]);
+  }
+  int k0 = 0;
+  for (; k0 + 8 <= w; k0 += 8)
+    s = dot_group_zp<8, false>(A, base + (int64_t)k0 * 64, rr, R.o, z, p, beta, s);
+  const int64_t e0 = base + (int64_t)k0 * 64;
+  switch (w - k0) {
+    case 1: s = dot_group_zp<1, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 2: s = dot_group_zp<2, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 3: s = dot_group_zp<3, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 4: s = dot_group_zp<4, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 5: s = dot_group_zp<5, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 6: s = dot_group_zp<6, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    case 7: s = dot_group_zp<7, false>(A, e0, rr, R.o, z, p, beta, s); break;
+    default: break;
   }
   return s;
+}
+
+// The scalar part of row sub of a row-split operator (clamped into the
+// matrix for the lanes past its end, which add nothing).
+template <typename T>
+__device__ __forceinline__ SellRow split_row_at(const SellD<T>& A, int64_t sub) {
+  return sell_row_at(A, sub < A.nrows ? sub : A.nrows - 1);
 }
 
 // (A x)_r with the rows of A split over G = 2^lg consecutive lanes: the
 // group's last lane returns the full row sum (fixed reduction pattern).  The
 // caller's lanes must all reach this call (no early exit before it).
 template <typename T>
-__device__ __forceinline__ T split_row_dot(const SellD<T>& A, int64_t sub, int lg,
-                                           const T* __restrict__ x) {
-  T ax = (T)0;
-  if (sub < A.nrows) {
-    int64_t base;
-    int w;
-    sell_row(A, sub, base, w);
-    ax = sell_dot(A, sub, base, w, x);
-  }
-  ax = group_sum(ax, lg);
-  return ax;
+__device__ __forceinline__ T split_row_dot(const SellD<T>& A, const SellRow& R, int64_t sub,
+                                           int lg, const T* __restrict__ x) {
+  T ax = sub < A.nrows ? sell_dot(A, R, sub, x) : (T)0;
+  return group_sum(ax, lg);
 }
+
+// The V-cycle and CG kernels below test the early-exit flag (*done) only
+// once their first loads are in flight: the flag's load then overlaps with
+// the row's slice pointers and own entries instead of preceding them.  The
+// test comes before every store and every block reduction, and is uniform.
 
 // One Chebyshev step of the l1-Jacobi-preconditioned level operator:
 //   d <- c1 d + c2 invd (b - A xin);  xout = xin + d,
@@ -957,21 +1088,23 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
                        T* __restrict__ xout, const T* __restrict__ b, const T* din, T* dout,
                        const T* __restrict__ invd, T c1, T c2, const int* __restrict__ done,
                        int fuse_dot, const int2* __restrict__ bm, double* __restrict__ partial) {
-  if (*done) return;
+  const int dn = *done;
   int64_t sub = logical_row();
   int64_t r = sub >> lg;
+  const SellRow R = split_row_at(A, sub);
   // the row's own vector entries are loaded before the row sum, so that they
   // travel with the operator's loads instead of after them
   const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n;
   T br = own ? b[r] : (T)0, ir = own ? invd[r] : (T)0, xr = own ? xin[r] : (T)0;
   T dr = own && c1 != (T)0 ? din[r] : (T)0;
-  T ax = split_row_dot(A, sub, lg, xin);
+  if (dn || R.w < 0) return;
+  T ax = split_row_dot(A, R, sub, lg, xin);
   double local = 0.0;
   if (own) {
-    T dn = c2 * ir * (br - ax);
-    if (c1 != (T)0) dn = c1 * dr + dn;
-    if (dout) dout[r] = dn;
-    T xo = xr + dn;
+    T dnew = c2 * ir * (br - ax);
+    if (c1 != (T)0) dnew = c1 * dr + dnew;
+    if (dout) dout[r] = dnew;
+    T xo = xr + dnew;
     xout[r] = xo;
     local = (double)br * (double)xo;
   }
@@ -987,12 +1120,14 @@ template <typename T>
 __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ x,
                            const T* __restrict__ b, T* __restrict__ r,
                            const int* __restrict__ done) {
-  if (*done) return;
+  const int dn = *done;
   int64_t sub = logical_row();
   int64_t i = sub >> lg;
+  const SellRow R = split_row_at(A, sub);
   const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n;
   T bi = own ? b[i] : (T)0;  // loaded with the operator, not after it
-  T ax = split_row_dot(A, sub, lg, x);
+  if (dn || R.w < 0) return;
+  T ax = split_row_dot(A, R, sub, lg, x);
   if (own) r[i] = bi - ax;
 }
 
@@ -1005,19 +1140,14 @@ __global__ void k_restrict(SellD<T> Rs, int lg, int64_t nc, const T* __restrict_
                            T* __restrict__ bc, const T* __restrict__ invdc,
                            T* __restrict__ dc, T* __restrict__ xc, T c0, int init,
                            const int* __restrict__ done) {
-  if (*done) return;
+  const int dn = *done;
   int64_t sr = logical_row();
   int64_t I = sr >> lg;
+  const SellRow R = split_row_at(Rs, sr);
   const bool own = (sr & ((1 << lg) - 1)) == (1 << lg) - 1 && I < nc;
   T iv = own && init ? invdc[I] : (T)0;  // loaded with R, not after the row sum
-  T s = (T)0;
-  if (sr < Rs.nrows) {
-    int64_t base;
-    int w;
-    sell_row(Rs, sr, base, w);
-    s = sell_dot(Rs, sr, base, w, rf);
-  }
-  s = group_sum(s, lg);
+  if (dn || R.w < 0) return;
+  T s = split_row_dot(Rs, R, sr, lg, rf);
   if (own) {
     bc[I] = s;
     if (init) {
@@ -1041,10 +1171,11 @@ __global__ void __launch_bounds__(BLK)
     k_prolong_coarse(SellD<T> P, int nc, const double* __restrict__ W,
                      const T* __restrict__ bc, T* __restrict__ xf,
                      const int* __restrict__ done) {
-  if (*done) return;
+  const int dn = *done;
   __shared__ double ws[COARSE_FOLD_MAX * COARSE_FOLD_MAX];
   __shared__ double bs[COARSE_FOLD_MAX], xs[COARSE_FOLD_MAX];
   const int64_t i = logical_row();
+  const SellRow R = sell_row_at(P, i < P.nrows ? i : P.nrows - 1);
   const T xi = i < P.nrows ? xf[i] : (T)0;  // in flight during the coarse solve
   // all of a thread's loads of W in flight at once, then the LDS stores
   constexpr int PER = COARSE_FOLD_MAX * COARSE_FOLD_MAX / BLK;
@@ -1055,7 +1186,9 @@ __global__ void __launch_bounds__(BLK)
     int t = threadIdx.x + k * BLK;
     wv[k] = t < nn ? W[t] : 0.0;
   }
-  if (threadIdx.x < nc) bs[threadIdx.x] = (double)bc[threadIdx.x];
+  double bv = threadIdx.x < nc ? (double)bc[threadIdx.x] : 0.0;
+  if (dn || R.w < 0) return;  // uniform: before the block's barriers
+  if (threadIdx.x < nc) bs[threadIdx.x] = bv;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int t = threadIdx.x + k * BLK;
@@ -1071,24 +1204,20 @@ __global__ void __launch_bounds__(BLK)
   }
   __syncthreads();
   if (i >= P.nrows) return;
-  int64_t base;
-  int w;
-  sell_row(P, i, base, w);
-  xf[i] = xi + sell_dot(P, i, base, w, (const double*)xs);
+  xf[i] = xi + sell_dot(P, R, i, (const double*)xs);
 }
 
 // xf += P xc
 template <typename T>
 __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ xf,
                           const int* __restrict__ done) {
-  if (*done) return;
+  const int dn = *done;
   int64_t i = logical_row();
   if (i >= P.nrows) return;
+  const SellRow R = sell_row_at(P, i);
   T xi = xf[i];  // loaded with P, not after the row sum
-  int64_t base;
-  int w;
-  sell_row(P, i, base, w);
-  xf[i] = xi + sell_dot(P, i, base, w, xc);
+  if (dn || R.w < 0) return;
+  xf[i] = xi + sell_dot(P, R, i, xc);
 }
 
 // Coarsest level: x = W b with the FP64 inverse W (symmetric, so column j
@@ -1096,9 +1225,10 @@ __global__ void k_prolong(SellD<T> P, const T* __restrict__ xc, T* __restrict__ 
 template <typename T>
 __global__ void k_coarse_solve(int nc, const double* __restrict__ W, const T* __restrict__ b,
                                T* __restrict__ x, const int* __restrict__ done) {
-  if (*done) return;
+  const int dn = *done;
   extern __shared__ double bs[];
   for (int j = threadIdx.x; j < nc; j += blockDim.x) bs[j] = (double)b[j];
+  if (dn) return;  // uniform: before the barrier
   __syncthreads();
   for (int i = threadIdx.x; i < nc; i += blockDim.x) {
     // sixteen loads of W in flight at a time; the sum stays in j order
@@ -1122,11 +1252,12 @@ template <typename T>
 __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict__ z,
                          const int* __restrict__ done, const int2* __restrict__ bm,
                          double* __restrict__ partial) {
-  if (*done) return;
+  const int dn = *done;
   int64_t i = logical_row();
   bool second;
   member_of(bm, i, second);
   double local = i < n ? (double)b[i] * (double)z[i] : 0.0;
+  if (dn) return;
   block_partial2(local, second, partial, bm != nullptr);
 }
 
@@ -1173,25 +1304,27 @@ __global__ void k_cg_pq(SellD<T> A, const TZ* __restrict__ z, const T* __restric
                         T* __restrict__ pnew, T* __restrict__ q, const int* __restrict__ done,
                         const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
                         double* __restrict__ partial) {
-  if (*done) return;
+  const int dn = *done;
   int64_t i = logical_row();
   bool second;
   int mi = member_of(bm, i, second);
+  const bool in = i < A.nrows;
+  const SellRow R = sell_row_at(A, in ? i : A.nrows - 1);
+  const SolveScalars& m = sc[mi];
+  const int first = m.first;
+  const double rz = m.rz, rz_old = m.rz_old;
+  T zi = in ? (T)z[i] : (T)0;  // loaded with A, not after it
+  T poi = in && !first ? pold[i] : (T)0;
+  if (dn || R.w < 0) return;
   double local = 0.0;
-  if (i < A.nrows) {
-    const SolveScalars& m = sc[mi];
-    int first = m.first;
-    T beta = first ? (T)0 : (T)(m.rz / m.rz_old);
-    T zi = (T)z[i], poi = first ? (T)0 : pold[i];  // loaded with A, not after it
-    int64_t base;
-    int w;
-    sell_row(A, i, base, w);
+  if (in) {
+    T beta = first ? (T)0 : (T)(rz / rz_old);
     T s, pn;
     if (first) {
-      s = sell_dot(A, i, base, w, z);
+      s = sell_dot(A, R, i, z);
       pn = zi;
     } else {
-      s = sell_dot_zp(A, i, base, w, z, pold, beta);
+      s = sell_dot_zp(A, R, i, z, pold, beta);
       pn = zi + beta * poi;
     }
     pnew[i] = pn;
@@ -1210,22 +1343,28 @@ __global__ void k_cg_xr(int64_t n, TK* __restrict__ x, TK* __restrict__ r,
                         TP* __restrict__ z0, TP c0, const int* __restrict__ done,
                         const SolveScalars* __restrict__ sc, const int2* __restrict__ bm,
                         double* __restrict__ partial) {
-  if (*done) return;
+  const int dn = *done;
   int64_t i = logical_row();
   bool second;
   int mi = member_of(bm, i, second);
+  const bool in = i < n;
+  const int mdone = sc[mi].done;
+  const double alpha64 = sc[mi].alpha;
+  TK rv = in ? r[i] : (TK)0, xv = in ? x[i] : (TK)0, pv = in ? p[i] : (TK)0;
+  TK qv = in ? q[i] : (TK)0;
+  TP iv = in ? invd[i] : (TP)0;
+  if (dn) return;
   double local = 0.0;
-  if (i < n) {
-    TK rv = r[i];
-    if (!sc[mi].done) {
-      TK alpha = (TK)sc[mi].alpha;
-      x[i] += alpha * p[i];
-      rv = rv - alpha * q[i];
+  if (in) {
+    if (!mdone) {
+      TK alpha = (TK)alpha64;
+      x[i] = xv + alpha * pv;
+      rv = rv - alpha * qv;
       r[i] = rv;
     }
     TP rpv = (TP)rv;
     if (rp) rp[i] = rpv;
-    TP dd = c0 * invd[i] * rpv;
+    TP dd = c0 * iv * rpv;
     if (d0) d0[i] = dd;  // null when no sweep reads it (finest degree 1)
     z0[i] = dd;
     local = (double)rv * (double)rv;
