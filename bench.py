@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 200, 50])
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase",
                                            "richards", "transport"],
                     default="steady2m",
@@ -128,7 +128,7 @@ def main():
         o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph,
                        "inner_reltol": args.inner_reltol}, memory=_lib.DPF_DEVICE, stream=sptr)
         if timing:
-            o.flags = _lib.DPF_FLAG_SWEEP_TIMING
+            o.flags = _lib.DPF_FLAG_KERNEL_TIMING
         return o
 
     def p(t):
@@ -206,29 +206,22 @@ def main():
     log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}; "
         f"mixed_pc vs fp64: {pc_err[0]:.2e}, {pc_err[1]:.2e}")
 
-    # roofline of the dominant kernel, the finest-level FP32 Chebyshev sweep of
-    # the mixed path: the same timed loop again with a hipEvent pair around
-    # every launch of that kernel (on the stream it runs on), averaged
-    live = {}
+    # kernel table: the same evaluations again with every launch timed by a
+    # dispatch event pair on the stream it runs on, per kernel class, AMG
+    # level and value size, with each launch's algorithmic bytes
+    ktab = {}
     for precision in ("mixed", "fp64"):
         o = opts(precision, timing=True)
-        stats = []
+        kernel_table(L, topo, reset=True)
         for k in range(args.steps):
-            evaluate(k, o, stats)
+            evaluate(k, o, [])
         torch.cuda.synchronize()
-        ms_tot = sum(a["sweep_ms"] + b["sweep_ms"] for a, b in stats)
-        n_tot = sum(a["sweep_launches"] + b["sweep_launches"] for a, b in stats)
-        live[precision] = (ms_tot / max(n_tot, 1), n_tot)
-        log(f"[bench] live finest-level sweep {precision}: {n_tot} launches, "
-            f"{1e3 * ms_tot / max(n_tot, 1):.1f} us mean")
+        ktab[precision] = kernel_table(L, topo, reset=True)
 
     # back-to-back launches of single kernels, for reference
     prof = {}
     for precision, pc in (("mixed", _lib.DPF_MIXED), ("fp64", _lib.DPF_FP64)):
-        if precision == "fp64":
-            evaluate(1, opts("fp64"), [])
-        else:
-            evaluate(1, opts("mixed"), [])
+        evaluate(1, opts(precision), [])
         for kern in (0, 1):
             ms = ctypes.c_double()
             by = ctypes.c_double()
@@ -238,9 +231,6 @@ def main():
             prof[(precision, kern)] = (ms.value, by.value)
             log(f"[bench] kernel {kern} {precision}: {ms.value * 1e3:.1f} us/launch, "
                 f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e6:.0f} GB/s")
-    by = prof[("mixed", 0)][1]
-    ms = live["mixed"][0]
-    achieved = by / (ms * 1e-3) / 1e9
 
     tuned = topo.tuned_degrees()
     log(f"[bench] tuned Chebyshev degrees (finest, coarse): {tuned}; layout {topo.layout()}")
@@ -249,7 +239,8 @@ def main():
 
     cpu = None
     if args.cpu_baseline == "auto":
-        cpu = cpu_baseline(w, fields[0], h_obs_d.cpu().numpy(), args.cpu_seconds)
+        cpu = cpu_baseline(w, fields, h_obs_d.cpu().numpy(), args.cpu_seconds)
+    roof, evalroof, table = roofline_from_tables(ktab, results, args.steps)
 
     out = {
         "metric": "forward+adjoint evals/s @2M cells, mixed vs FP64 GPU, <=1e-8 rel head err",
@@ -289,20 +280,9 @@ def main():
         "cheb_tuned_finest_coarse_ratio": tuned,
         "cheb_pinned": bool(args.tuning),
         "hipgraph_iterations": bool(args.graph),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_cheb<float,true> (finest-level FP32 Chebyshev sweep)",
-            "achieved": round(achieved, 1),
-            "peak": 8000.0,
-            "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4),
-            "traffic": traffic_per_launch(ns),
-            "us_per_launch": round(ms * 1e3, 2),
-            "launches_timed": live["mixed"][1],
-            "bytes_per_launch": by,
-            "fp64_us_per_launch": round(live["fp64"][0] * 1e3, 2),
-            "fp64_GBps": round(prof[("fp64", 0)][1] / (live["fp64"][0] * 1e-3) / 1e9, 1),
-        },
+        "roofline": roof,
+        "roofline_eval": evalroof,
+        "kernel_table_mixed": table,
         "kernels_back_to_back": {f"{pr}/{k}": {"us": round(v[0] * 1e3, 2),
                                            "GBps": round(v[1] / v[0] / 1e6, 1)}
                              for (pr, k), v in prof.items()},
@@ -751,41 +731,127 @@ def bench_transport(args, P, api, _lib, torch):
     print(json.dumps(res), flush=True)
 
 
-def traffic_per_launch(ns):
-    """HBM bytes per launch of the finest-level FP32 Chebyshev sweep, from the
-    committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE measurement of this
-    workload (scaled by the same counters on a plain copy of known size), or
-    None if no measurement for this grid is committed."""
-    path = os.path.join(ROOT, "profiles", "k_cheb_fine_fp32_2m_traffic.json")
+def kernel_table(L, topo, reset=False):
+    """The topology's kernel table (dpf_kernel_table) as a list of dicts."""
+    from dpfehm_jl_amd import _lib
+    cap = 256
+    recs = (_lib.KernelRecord * cap)()
+    n = ctypes.c_int32()
+    _lib.check(L.dpf_kernel_table(topo.handle, recs, cap, ctypes.byref(n), 1 if reset else 0))
+    return [dict(kernel=r.kernel.decode(), level=r.level, w=r.value_bytes, launches=r.launches,
+                 ms=r.ms, bytes=r.bytes) for r in recs[:min(n.value, cap)]]
+
+
+KERNEL_SYMBOL = {"cheb_sweep": "k_cheb", "residual": "k_residual", "restrict": "k_restrict",
+                 "prolong": "k_prolong", "prolong_coarse_solve": "k_prolong_coarse",
+                 "cg_pq": "k_cg_pq", "cg_xr": "k_cg_xr", "finish": "k_finish",
+                 "outer_residual": "k_outer_residual", "galerkin_ap": "k_prod_gather",
+                 "galerkin_rap": "k_prod_gather", "assemble": "k_assemble"}
+
+
+def kernel_label(r):
+    typ = {4: "float", 8: "double"}.get(r["w"], "?")
+    sym = KERNEL_SYMBOL.get(r["kernel"], r["kernel"])
+    lev = f" level {r['level']}" if r["level"] >= 0 else ""
+    return f"{sym}<{typ}> ({r['kernel']}{lev})"
+
+
+def roofline_from_tables(ktab, results, steps):
+    """roofline: the kernel class with the most time per evaluation in the
+    mixed path, achieved = its algorithmic bytes / its summed launch time
+    (dispatch events, over every launch of `steps` evaluations).
+    roofline_eval: B_eval / t_eval, the algorithmic bytes of every launch of
+    one evaluation over the wall time of one evaluation in the timed region
+    (no event timing there), for both precisions."""
+    peak = 8000.0
+    mix = sorted(ktab["mixed"], key=lambda r: -r["ms"])
+    dom = mix[0]
+    ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": kernel_label(dom), "achieved": round(ach, 1),
+            "peak": peak, "unit": "GB/s", "frac": round(ach / peak, 4),
+            "traffic": traffic_per_launch(dom),
+            "us_per_launch": round(1e3 * dom["ms"] / dom["launches"], 2),
+            "bytes_per_launch": round(dom["bytes"] / dom["launches"]),
+            "launches_per_eval": round(dom["launches"] / steps, 1),
+            "share_of_kernel_time": round(dom["ms"] / sum(r["ms"] for r in mix), 4)}
+    evalroof = {}
+    for prec in ("mixed", "fp64"):
+        tab = ktab[prec]
+        b_eval = sum(r["bytes"] for r in tab) / steps
+        k_eval = sum(r["ms"] for r in tab) / steps * 1e-3
+        t_eval = results[prec]["seconds"] / steps
+        evalroof[prec] = {"B_eval_bytes": round(b_eval), "t_eval_s": round(t_eval, 6),
+                          "achieved": round(b_eval / t_eval / 1e9, 1),
+                          "frac": round(b_eval / t_eval / 1e9 / peak, 4),
+                          "kernel_busy_s": round(k_eval, 6),
+                          "achieved_over_kernel_busy": round(b_eval / k_eval / 1e9, 1)}
+    table = []
+    for r in mix[:16]:
+        table.append({"kernel": kernel_label(r), "launches_per_eval": round(r["launches"] / steps, 1),
+                      "us_per_eval": round(1e3 * r["ms"] / steps, 1),
+                      "us_per_launch": round(1e3 * r["ms"] / r["launches"], 2),
+                      "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1),
+                      "frac": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9 / peak, 3)})
+    for r in table[:12]:
+        log(f"[bench] {r['kernel']:55s} {r['launches_per_eval']:6.1f}/eval "
+            f"{r['us_per_eval']:8.1f} us/eval {r['us_per_launch']:7.2f} us {r['GBps']:7.1f} GB/s")
+    for prec, e in evalroof.items():
+        log(f"[bench] {prec}: B_eval {e['B_eval_bytes'] / 1e9:.2f} GB, t_eval {1e3 * e['t_eval_s']:.2f} ms "
+            f"-> {e['achieved']:.0f} GB/s ({e['frac']:.3f} of peak); kernel busy "
+            f"{1e3 * e['kernel_busy_s']:.2f} ms")
+    return roof, evalroof, table
+
+
+def traffic_per_launch(rec):
+    """Measured HBM bytes per launch of a kernel class (FETCH_SIZE + WRITE_SIZE
+    of rocprofv3 --pmc, scaled by the same counters on a plain copy of known
+    size) from the committed profiles/hbm_traffic_2m.json, or None if that
+    kernel has no measurement there."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_2m.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if tuple(t.get("grid", ())) != tuple(ns):
-        return None
-    return t.get("bytes_per_launch")
+    key = f"{rec['kernel']}/{rec['level']}/{rec['w']}"
+    ent = t.get("kernels", {}).get(key)
+    return ent.get("bytes_per_launch") if ent else None
 
 
-def cpu_baseline(w, logK, h_obs, budget_s):
-    """The FP64 oracle (oracle/gw_oracle.c: IC(0)-PCG, one core) on the same
-    workload.  Sample: the K_e map and one full forward solve are timed; the
-    adjoint solve has the same matrix, tolerance and cost, so one
-    forward+adjoint evaluation is taken as map + 2 x forward solve."""
-    from oracle import gw_oracle as O
+def cpu_baseline(w, fields, h_obs, budget_s):
+    """The multi-core FP64 CPU restatement (oracle/amg_cpu.cpp: the same
+    discretisation, SA-AMG-PCG to 1e-10, OpenMP on every host core), not the
+    reference (pure Julia, not installed).  Mesh set-up once, then whole
+    forward + adjoint evaluations (map, assembly, hierarchy refresh, both
+    solves, gradient) on the same fields as the GPU steps until the time
+    budget is spent."""
+    from oracle import amg_cpu
     t0 = time.perf_counter()
-    K = O.hm_forward(logK, w.neighbors)
-    t1 = time.perf_counter()
-    h, hf, info = O.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
-                                            w.dirichleths, w.Qs, reltol=1e-10, return_info=True)
-    t2 = time.perf_counter()
-    ev = (t1 - t0) + 2.0 * (t2 - t1)
-    log(f"[bench] cpu oracle: K map {t1 - t0:.2f}s, forward solve {t2 - t1:.1f}s "
-        f"({info['iterations']} IC(0)-PCG iterations) -> {ev:.1f}s per forward+adjoint eval")
-    return {"value": round(1.0 / ev, 5), "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": "K_e map plus one full forward solve of the same 2M-cell workload by the "
-                      "FP64 C oracle (IC(0)-PCG to 1e-10, one thread); the adjoint solve is "
-                      "counted as a second forward solve of equal cost"}
+    cpu = amg_cpu.CpuAmg(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    t_setup = time.perf_counter() - t0
+    n = 0
+    t0 = time.perf_counter()
+    its = None
+    while True:
+        J, g, info = cpu.eval(fields[n % len(fields)], w.dirichleths, w.Qs, w.obs, h_obs,
+                              reltol=1e-10)
+        its = (int(info[0]), int(info[1]))
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 50:
+            break
+    rate = n / el
+    model = amg_cpu.cpu_model()
+    log(f"[bench] cpu baseline: {cpu.threads} threads ({model}), mesh set-up {t_setup:.1f}s, "
+        f"{n} evals in {el:.1f}s -> {rate:.3f} evals/s, iterations {its}")
+    cpu.close()
+    return {"value": round(rate, 4), "unit": "evals/s", "cores": cpu.threads, "kind": "port",
+            "cpu_model": model,
+            "sample": f"{n} whole forward+adjoint evaluations of the same 2M-cell workload and "
+                      "fields (logK map, assembly, SA-AMG refresh, forward and adjoint PCG to "
+                      "1e-10, gradient) by the FP64 C++ restatement with OpenMP on every host "
+                      "core (oracle/amg_cpu.cpp); not the reference, which is Julia and not "
+                      "installed; mesh set-up excluded, as on the GPU"}
 
 
 if __name__ == "__main__":

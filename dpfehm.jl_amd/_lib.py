@@ -14,6 +14,7 @@ DPF_FP64, DPF_MIXED, DPF_MIXED_PC = 0, 1, 2
 DPF_HOST, DPF_DEVICE = 0, 1
 DPF_FLAG_SWEEP_TIMING = 1
 DPF_FLAG_GRAPH = 2
+DPF_FLAG_KERNEL_TIMING = 4
 
 _ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOCONV", 4: "EBREAKDOWN", 5: "ESINGULAR", 6: "ENOMEM"}
 
@@ -60,6 +61,17 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class KernelRecord(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_char * 32),
+        ("level", ctypes.c_int32),
+        ("value_bytes", ctypes.c_int32),
+        ("launches", ctypes.c_int64),
+        ("ms", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+    ]
 
 
 _p = ctypes.c_void_p
@@ -110,6 +122,7 @@ SIGNATURES = {
                                                  ctypes.POINTER(Stats), ctypes.POINTER(_p)]),
     "dpf_transient_pullback_seeds": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p,
                                                     ctypes.POINTER(Opts), ctypes.POINTER(Stats)]),
+    "dpf_kernel_table": (ctypes.c_int, [_p, _p, _i32, _p, _i32]),
     "dpf_profile_kernel": (ctypes.c_int, [_p, _i32, _i32, _i32, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_inspect_last_error": (ctypes.c_char_p, []),
     "dpf_inspect_create": (ctypes.c_int, [_i64, _i64, _p, _p, _i64, _p, _i32, _i32,

@@ -237,6 +237,29 @@ struct Cheb {
   }
 };
 
+// Kernel classes of the timing table (dpf_kernel_table).
+enum KCat {
+  KC_CHEB, KC_RESIDUAL, KC_RESTRICT, KC_PROLONG, KC_PROLONG_COARSE, KC_COARSE_SOLVE,
+  KC_RZ_DOT, KC_CG_START, KC_CG_PQ, KC_CG_XR, KC_FINISH, KC_OUTER_RESIDUAL, KC_REFINE_VEC,
+  KC_ASSEMBLE, KC_PSMOOTH, KC_RGATHER, KC_PROD_AP, KC_PROD_RAP, KC_MIRROR, KC_ROWDIAG,
+  KC_DENSE_INVERSE, KC_KMAP, KC_GRADIENT, KC_SCATTER, KC_NCAT
+};
+static const char* const kcat_name[KC_NCAT] = {
+  "cheb_sweep", "residual", "restrict", "prolong", "prolong_coarse_solve", "coarse_solve",
+  "rz_dot", "cg_start", "cg_pq", "cg_xr", "finish", "outer_residual", "refine_vector",
+  "assemble", "psmooth", "rgather", "galerkin_ap", "galerkin_rap", "mirror_copy", "rowdiag",
+  "dense_inverse", "k_map", "gradient", "scatter_gather"};
+
+struct KRec {
+  int cat, level, w, iter, phase;
+  double bytes;
+  hipEvent_t a, b;
+};
+struct KAcc {
+  int64_t launches = 0;
+  double ms = 0, bytes = 0;
+};
+
 }  // namespace dpf
 
 using namespace dpf;
@@ -299,12 +322,17 @@ struct dpf_topology {
   bool tuned[3] = {false, false, false};
   double tune_ms[3][7] = {};  // autotuner trials: 5 degree pairs, then 2 intervals
   int tune_iters[3][7] = {};
-  // optional event timing of every finest-level smoothing sweep (opts->flags)
-  bool timing = false;
+  // optional event timing (opts->flags): every finest-level smoothing sweep
+  // (sweep timing) or every launch (kernel timing), per kernel class, level
+  // and value size
+  bool timing = false, ktiming = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   double sweep_ms = 0;
   int sweep_n = 0;
+  int kiter = -1, kphase = 0;  // Krylov iteration and phase of the launches being queued
+  std::vector<KRec> kpend;
+  std::map<std::tuple<int, int, int>, KAcc> ktable;
   ~dpf_topology() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
   }
@@ -356,6 +384,93 @@ void set_err(const std::string& m) { g_err = m; }
 hipStream_t pick_stream(dpf_topology* t, const dpf_opts* o) {
   if (o && o->memory == DPF_DEVICE && o->stream) return (hipStream_t)o->stream;
   return t->own_stream;
+}
+
+// Event pair around one timed launch.
+hipEvent_t timing_event(dpf_topology* t) {
+  if (t->evused == t->evpool.size()) {
+    hipEvent_t e;
+    HIPTRY(hipEventCreate(&e));
+    t->evpool.push_back(e);
+  }
+  return t->evpool[t->evused++];
+}
+
+// A launch of kernel class `cat` on AMG level `level` (-1: none) with values
+// of w bytes, moving `bytes` algorithmic bytes.  With kernel timing on
+// (DPF_FLAG_KERNEL_TIMING; or sweep timing, for finest-level sweeps), the
+// launch carries a start and a stop event that the runtime stamps at the
+// dispatch's own begin and end (the timestamps a kernel trace reports), so
+// the timing adds no packets between kernels; harvest_timing adds them up.
+template <typename F, typename... Args>
+void klaunch(dpf_topology* t, int cat, int level, int w, double bytes, F kernel, dim3 grid,
+             dim3 block, size_t shmem, hipStream_t s, Args... args) {
+  if (t->ktiming || (t->timing && cat == KC_CHEB && level == 0)) {
+    hipEvent_t a = timing_event(t);
+    hipEvent_t b = timing_event(t);
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, args...);
+    t->kpend.push_back(KRec{cat, level, w, t->kiter, t->kphase, bytes, a, b});
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  }
+}
+
+// Adds the recorded durations (finest-level sweeps to the sweep total, every
+// launch to the kernel table).  its >= 0 (end of a Krylov solve with its
+// iterations): launches the early-exit flag turned into no-ops are left out,
+// i.e. every launch of a later iteration and the V-cycle of iteration its
+// (launches outside the Krylov loop are tagged iteration -1).
+void harvest_timing(dpf_topology* t, int its = -1) {
+  if (!t->kpend.empty()) HIPTRY(hipEventSynchronize(t->kpend.back().b));
+  for (const KRec& r : t->kpend) {
+    if (its >= 0 && r.iter >= 0 && (r.iter > its || (r.iter == its && r.phase == 1))) continue;
+    float ms = 0;
+    HIPTRY(hipEventElapsedTime(&ms, r.a, r.b));
+    if (t->timing && r.cat == KC_CHEB && r.level == 0) {
+      t->sweep_ms += ms;
+      t->sweep_n++;
+    }
+    if (t->ktiming) {
+      KAcc& e = t->ktable[std::make_tuple(r.cat, r.level, r.w)];
+      e.launches++;
+      e.ms += ms;
+      e.bytes += r.bytes;
+    }
+  }
+  t->kpend.clear();
+  t->evused = 0;
+}
+
+// Sizes for the algorithmic byte counts of level l: rows, nonzeros of the
+// level operator and the bytes of its stored columns (the fine operator's
+// offset slices store none), nonzeros of the prolongator and rows of the
+// next level.
+struct LvlSize {
+  double rows, nnz, cols, pnnz, nc;
+};
+LvlSize lvl_size(const dpf_topology* t, int l) {
+  const LevelHost& H = t->H.levels[l];
+  LvlSize z;
+  z.rows = (double)H.A.nrows;
+  z.nnz = (double)H.A.nnz;
+  z.cols = 4.0 * (l == 0 ? (double)t->fine_explicit_nnz : (double)H.A.nnz);
+  z.pnnz = H.agg.empty() ? 0.0 : (double)H.P.nnz;
+  z.nc = H.agg.empty() ? 0.0 : (double)H.R.nrows;
+  return z;
+}
+
+// Kernel timing of one C-ABI call (DPF_FLAG_KERNEL_TIMING): on from its
+// start, harvested into the topology's table once its stream has drained.
+void ktiming_begin(dpf_topology* t, const dpf_opts* o) {
+  t->kpend.clear();
+  t->evused = 0;
+  t->kiter = -1;
+  t->kphase = 0;
+  t->ktiming = (o->flags & DPF_FLAG_KERNEL_TIMING) != 0;
+}
+void ktiming_end(dpf_topology* t) {
+  if (t->ktiming) harvest_timing(t);
+  t->ktiming = false;
 }
 
 template <typename T>
@@ -420,7 +535,15 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
     v32 = t->h32.lev[0].val.as<float>();
     i32 = t->h32.lev[0].invd.as<float>();
   }
-  hipLaunchKernelGGL(k_assemble, dim3(nblocks(t->H.nf)), dim3(BLK), 0, s, (int64_t)t->H.nf,
+  // incidence (edge, other, slot) and the K, aol gathers per entry; per row
+  // its maps, Q and diagonal slot; the values (FP64, and FP32 if wanted), the
+  // l1 diagonals and b written
+  const LvlSize z0 = lvl_size(t, 0);
+  const double ninc = (double)t->H.inc_ptr.back();
+  const double ab = ninc * 26.0 + z0.rows * 22.0 + z0.nnz * (want32 ? 12.0 : 8.0) +
+                    z0.rows * ((want32 ? 12.0 : 8.0) + (want_b ? 8.0 : 0.0));
+  klaunch(t, KC_ASSEMBLE, 0, 8, ab, k_assemble, dim3(nblocks(t->H.nf)), dim3(BLK), 0, s,
+          (int64_t)t->H.nf,
                      t->f2n.as<int32_t>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
                      t->inc_other.as<int32_t>(), t->inc_slot.as<uint16_t>(),
                      L0.diag_slot.as<uint16_t>(), t->Ke.as<double>(), t->aol.as<double>(),
@@ -442,37 +565,44 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
     SellD<T> P = d.P.view<T>(h.lev[l].pval.template as<T>());
     T* apv = h.lev[l].apval.template as<T>();
     T* cval = level_val(t, h, l + 1);
-    hipLaunchKernelGGL(k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
-                       (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(),
-                       d.p_self.as<uint16_t>(), P);
-    hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries,
-                       d.rs_src.as<int32_t>(), (const T*)P.val,
-                       h.lev[l].rval.template as<T>());
-    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s,
-                       d.ap_nnz, d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr,
-                       d.ap_a.as<int32_t>(), d.ap_p.as<int32_t>(), (const T*)A.val,
-                       (const T*)P.val, apv);
-    hipLaunchKernelGGL(k_prod_gather<T>, dim3(nblocks(d.n_rap << d.rap_lg)), dim3(BLK), 0,
-                       s, d.n_rap, d.rap_lg, d.rap_off.as<int64_t>(),
-                       d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(), d.rap_ap.as<int32_t>(),
-                       (const T*)P.val, (const T*)apv, cval);
+    const double w = sizeof(T);
+    const LevelHost& LH = t->H.levels[l];
+    const LvlSize z = lvl_size(t, l), zn = lvl_size(t, l + 1);
+    const double nap = (double)LH.ap_off.back(), nrap = (double)LH.rap_off.back();
+    klaunch(t, KC_PSMOOTH, l, (int)w, z.nnz * (w + 2.0) + z.rows * (w + 2.0) + z.pnnz * w,
+            k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
+            (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(), d.p_self.as<uint16_t>(), P);
+    klaunch(t, KC_RGATHER, l, (int)w, z.pnnz * (4.0 + 2.0 * w), k_rgather<T>,
+            dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries, d.rs_src.as<int32_t>(),
+            (const T*)P.val, h.lev[l].rval.template as<T>());
+    klaunch(t, KC_PROD_AP, l, (int)w, nap * (8.0 + 2.0 * w) + (double)d.ap_nnz * (8.0 + w),
+            k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s, d.ap_nnz,
+            d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr, d.ap_a.as<int32_t>(),
+            d.ap_p.as<int32_t>(), (const T*)A.val, (const T*)P.val, apv);
+    klaunch(t, KC_PROD_RAP, l, (int)w, nrap * (8.0 + 2.0 * w) + (double)d.n_rap * (12.0 + w),
+            k_prod_gather<T>, dim3(nblocks(d.n_rap << d.rap_lg)), dim3(BLK), 0, s, d.n_rap,
+            d.rap_lg, d.rap_off.as<int64_t>(), d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(),
+            d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
     if (d.n_mirror > 0)
-      hipLaunchKernelGGL(k_mirror_copy<T>, dim3(nblocks(d.n_mirror)), dim3(BLK), 0, s,
-                         d.n_mirror, d.rap_mdst.as<int32_t>(), d.rap_msrc.as<int32_t>(), cval);
-    hipLaunchKernelGGL(k_rowdiag<T>, dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s,
-                       dn.A.view<T>(cval), dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
+      klaunch(t, KC_MIRROR, l + 1, (int)w, (double)d.n_mirror * (8.0 + 2.0 * w),
+              k_mirror_copy<T>, dim3(nblocks(d.n_mirror)), dim3(BLK), 0, s, d.n_mirror,
+              d.rap_mdst.as<int32_t>(), d.rap_msrc.as<int32_t>(), cval);
+    klaunch(t, KC_ROWDIAG, l + 1, (int)w, zn.nnz * w + zn.rows * (2.0 + w), k_rowdiag<T>,
+            dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, dn.A.view<T>(cval),
+            dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
     if (dn.as_lg > 0)
-      hipLaunchKernelGGL(k_rgather<T>, dim3(nblocks(dn.As.entries)), dim3(BLK), 0, s,
-                         dn.As.entries, dn.as_src.as<int32_t>(), (const T*)cval,
-                         h.lev[l + 1].asval.template as<T>());
+      klaunch(t, KC_RGATHER, l + 1, (int)w, zn.nnz * (4.0 + 2.0 * w), k_rgather<T>,
+              dim3(nblocks(dn.As.entries)), dim3(BLK), 0, s, dn.As.entries,
+              dn.as_src.as<int32_t>(), (const T*)cval, h.lev[l + 1].asval.template as<T>());
   }
   if (t->H.coarse_dense) {
     LevelDev& dc = *t->lev[L - 1];
     int ncd = (int)dc.A.nrows;
     size_t lds = ncd <= DENSE_LDS_MAX ? (size_t)ncd * ncd * sizeof(double) : 0;
-    hipLaunchKernelGGL(k_dense_inverse<T>, dim3(1), dim3(1024), lds, s,
-                       dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
-                       h.W.template as<double>(), h.status.template as<int>());
+    klaunch(t, KC_DENSE_INVERSE, L - 1, (int)sizeof(T), 2.0 * ncd * ncd * 8.0,
+            k_dense_inverse<T>, dim3(1), dim3(1024), lds, s,
+            dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
+            h.W.template as<double>(), h.status.template as<int>());
   }
   HIPTRY(hipGetLastError());
 }
@@ -482,7 +612,7 @@ auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
   return level == 0 ? &k_cheb<T, true> : &k_cheb<T, false>;
 }
 
-// Finishes the grid sum the previous kernel (nparts blocks) left in t->partial.
+// The member table of a batch (null for a single system).
 const int2* bm_ptr(dpf_topology* t) { return t->nm > 1 ? t->bm.as<int2>() : nullptr; }
 
 // Member maps of the fine grid (logical blocks of BLK rows).  A single system
@@ -509,15 +639,6 @@ void setup_members(dpf_topology* t) {
   upload(t->bm, bm);
 }
 
-void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol = 0,
-            double abstol = 0, int maxiter = 0, int abs_target = 0, int seq = 0,
-            const double* partial = nullptr) {
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1024), 0, s, mode, t->nm, (int)nparts,
-                     partial ? partial : (const double*)t->partial.as<double>(), bm_ptr(t),
-                     (const int*)t->mblk.as<int>(), t->scal.as<SolveScalars>(),
-                     t->gdone.as<int>(), reltol, abstol, maxiter, abs_target, seq, t->hm_dev,
-                     t->hm_outer_dev);
-}
 
 int mirror_load(const int* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
 double mirror_load(const double* p) {
@@ -544,41 +665,23 @@ void wait_published(dpf_topology* t, hipStream_t s, const int* seqp, int want) {
   }
 }
 
-// Event pair around one finest-level sweep (only when timing is on).
-hipEvent_t timing_event(dpf_topology* t) {
-  if (t->evused == t->evpool.size()) {
-    hipEvent_t e;
-    HIPTRY(hipEventCreate(&e));
-    t->evpool.push_back(e);
-  }
-  return t->evpool[t->evused++];
+// Finishes the grid sum the previous kernel (nparts blocks) left in t->partial.
+void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double reltol = 0,
+            double abstol = 0, int maxiter = 0, int abs_target = 0, int seq = 0,
+            const double* partial = nullptr) {
+  klaunch(t, KC_FINISH, -1, 8, 16.0 * (double)nparts + 256.0, k_finish, dim3(1), dim3(1024), 0,
+          s, mode, t->nm, (int)nparts,
+          partial ? partial : (const double*)t->partial.as<double>(), bm_ptr(t),
+          (const int*)t->mblk.as<int>(), t->scal.as<SolveScalars>(), t->gdone.as<int>(), reltol,
+          abstol, maxiter, abs_target, seq, t->hm_dev, t->hm_outer_dev);
 }
 
-// A finest-level smoothing sweep.  With sweep timing on, the launch carries
-// a start and a stop event that the runtime stamps at the kernel's own begin
-// and end (the dispatch's timestamps, as a kernel trace reports them), so
-// the timing adds no event packets between kernels.
-template <typename F, typename... Args>
-void sweep_launch(dpf_topology* t, int l, F kernel, dim3 grid, hipStream_t s, Args... args) {
-  if (t->timing && l == 0) {
-    hipEvent_t a = timing_event(t);
-    hipEvent_t b = timing_event(t);
-    hipExtLaunchKernelGGL(kernel, grid, dim3(BLK), 0, s, a, b, 0, args...);
-  } else {
-    hipLaunchKernelGGL(kernel, grid, dim3(BLK), 0, s, args...);
-  }
-}
-
-// Adds the recorded sweep durations (the stream has been synchronised).
-void harvest_timing(dpf_topology* t) {
-  if (t->evused >= 2) HIPTRY(hipEventSynchronize(t->evpool[t->evused - 1]));
-  for (size_t i = 0; i + 1 < t->evused; i += 2) {
-    float ms = 0;
-    HIPTRY(hipEventElapsedTime(&ms, t->evpool[i], t->evpool[i + 1]));
-    t->sweep_ms += ms;
-    t->sweep_n++;
-  }
-  t->evused = 0;
+// Algorithmic bytes of a Chebyshev sweep on level l (values of w bytes):
+// the operator, reads of x, b and the l1 diagonal, the write of x', and d
+// read (c1 != 0) and written (when a later sweep reads it).
+double cheb_bytes(const dpf_topology* t, int l, double w, bool din, bool dout) {
+  LvlSize z = lvl_size(t, l);
+  return z.nnz * w + z.cols + z.rows * w * (4.0 + (din ? 1.0 : 0.0) + (dout ? 1.0 : 0.0));
 }
 
 // One V-cycle on levels l..: input b at level l (for l = 0 the CG residual),
@@ -597,6 +700,8 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   double* part = t->partial.as<double>();
   unsigned g = nblocks(d.A.nrows);
   int m = (int)ch.c1.size();
+  const int w = (int)sizeof(T);
+  const LvlSize z = lvl_size(t, l);
   // the smoothing sweeps and the residual use the row-split copy on coarse levels
   int lg = d.as_lg;
   SellD<T> Asm = lg > 0 ? d.As.view<T>(n.asval.template as<T>()) : A;
@@ -607,12 +712,14 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   if (l == L - 1) {
     if (t->H.coarse_dense) {
       int nc = (int)d.A.nrows;
-      hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), nc * sizeof(double), s, nc,
-                         (const double*)h.W.template as<double>(), bl, cur, (const int*)done);
+      klaunch(t, KC_COARSE_SOLVE, l, w, (double)nc * nc * 8.0 + 2.0 * nc * w,
+              k_coarse_solve<T>, dim3(1), dim3(256), nc * sizeof(double), s, nc,
+              (const double*)h.W.template as<double>(), bl, cur, (const int*)done);
       // a one-level hierarchy: the CG scalar r.z still has to be formed
       if (fuse_dot_at_fine) {
-        hipLaunchKernelGGL(k_rz_dot<T>, dim3(g), dim3(BLK), 0, s, d.A.nrows, bl, (const T*)cur,
-                           (const int*)done, bm_ptr(t), part);
+        klaunch(t, KC_RZ_DOT, l, w, 2.0 * z.rows * w + 16.0 * g, k_rz_dot<T>, dim3(g),
+                dim3(BLK), 0, s, d.A.nrows, bl, (const T*)cur, (const int*)done, bm_ptr(t),
+                part);
         finish(t, s, FIN_RZ, g);
       }
     } else {
@@ -621,9 +728,11 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       c2.make(2 * m, 1.0, ch.ratio);
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
-        sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
-                     (const T*)dd, k == 2 * m - 1 ? (T*)nullptr : dd, invd, (T)c2.c1[k],
-                     (T)c2.c2[k], (const int*)done, fuse, bm_ptr(t), part);
+        bool wd = k != 2 * m - 1;
+        klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd), cheb_kernel<T>(l), dim3(gs),
+                dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
+                wd ? dd : (T*)nullptr, invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse,
+                bm_ptr(t), part);
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
       }
@@ -634,43 +743,49 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   // pre-smoothing (step 0 done by the producer of b)
   for (int k = 1; k < m; k++) {
     // the last pre-smoothing sweep's d is dead: the post-smoothing restarts at c1 = 0
-    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
-                 (const T*)dd, k == m - 1 ? (T*)nullptr : dd, invd, (T)ch.c1[k], (T)ch.c2[k],
-                 (const int*)done, 0, bm_ptr(t), part);
+    bool wd = k != m - 1;
+    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd), cheb_kernel<T>(l), dim3(gs),
+            dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
+            wd ? dd : (T*)nullptr, invd, (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0,
+            bm_ptr(t), part);
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
-  hipLaunchKernelGGL(k_residual<T>, dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, rl,
-                     (const int*)done);
+  klaunch(t, KC_RESIDUAL, l, w, z.nnz * w + z.cols + 3.0 * z.rows * w, k_residual<T>,
+          dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, rl, (const int*)done);
   LevelDev& dn = *t->lev[l + 1];
   LevelNum<T>& nn = h.lev[l + 1];
   bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
   SellD<T> Rs = d.Rs.view<T>(n.rval.template as<T>());
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg,
-                     dn.A.nrows, (const T*)rl, nn.b.template as<T>(),
-                     (const T*)level_invd(t, h, l + 1),
-                     nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0,
-                     coarse_next ? 0 : 1, (const int*)done);
+  klaunch(t, KC_RESTRICT, l, w,
+          z.pnnz * (w + 4.0) + z.rows * w + z.nc * w * (coarse_next ? 1.0 : 4.0),
+          k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg, dn.A.nrows,
+          (const T*)rl, nn.b.template as<T>(), (const T*)level_invd(t, h, l + 1),
+          nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0, coarse_next ? 0 : 1,
+          (const int*)done);
   SellD<T> P = d.P.view<T>(n.pval.template as<T>());
   if (coarse_next && dn.A.nrows <= COARSE_FOLD_MAX) {
     // the dense coarsest solve is folded into the prolongation
-    hipLaunchKernelGGL(k_prolong_coarse<T>, dim3(g), dim3(BLK), 0, s, P, (int)dn.A.nrows,
-                       (const double*)h.W.template as<double>(), (const T*)nn.b.template as<T>(),
-                       cur, (const int*)done);
+    klaunch(t, KC_PROLONG_COARSE, l, w,
+            z.pnnz * (w + 4.0) + z.nc * z.nc * 8.0 + z.nc * w + 2.0 * z.rows * w,
+            k_prolong_coarse<T>, dim3(g), dim3(BLK), 0, s, P, (int)dn.A.nrows,
+            (const double*)h.W.template as<double>(), (const T*)nn.b.template as<T>(), cur,
+            (const int*)done);
   } else {
     T* xc = nullptr;
     vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
               nn.xs.template as<T>(), &xc, chs, done, false);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur,
-                       (const int*)done);
+    klaunch(t, KC_PROLONG, l, w, z.pnnz * (w + 4.0) + z.nc * w + 2.0 * z.rows * w,
+            k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur, (const int*)done);
   }
   for (int k = 0; k < m; k++) {
     double c1 = k == 0 ? 0.0 : ch.c1[k];
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
-    sweep_launch(t, l, cheb_kernel<T>(l), dim3(gs), s, Asm, lg, nr, (const T*)cur, oth, bl,
-                 (const T*)dd, k == m - 1 ? (T*)nullptr : dd, invd, (T)c1, (T)c2,
-                 (const int*)done, fuse, bm_ptr(t), part);
+    bool wd = k != m - 1;
+    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, k != 0, wd), cheb_kernel<T>(l), dim3(gs),
+            dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
+            wd ? dd : (T*)nullptr, invd, (T)c1, (T)c2, (const int*)done, fuse, bm_ptr(t), part);
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
   }
@@ -719,28 +834,42 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   TP* z0 = n0.x.template as<TP>();
   TP* zs = n0.xs.template as<TP>();
   int seq = ++t->seq;
-  hipLaunchKernelGGL((k_cg_start<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, b, r, x, rp, invd, dd,
-                     z0, (TP)ch.c0, bm_ptr(t), part);
+  // algorithmic bytes of the CG kernels (Krylov values wk, V-cycle values wp)
+  const double wk = sizeof(TK), wp = sizeof(TP), rows = (double)nf;
+  const LvlSize z0s = lvl_size(t, 0);
+  const double vec_p = (split ? wp : 0.0) + (dd ? wp : 0.0) + 2.0 * wp;  // rp, d0, invd, z0
+  const double b_start = rows * (3.0 * wk + vec_p) + 16.0 * g;
+  const double b_pq = z0s.nnz * wk + z0s.cols + rows * (wp + 3.0 * wk) + 16.0 * g;
+  const double b_xr = rows * (6.0 * wk + vec_p) + 16.0 * g;
+  t->kiter = 0;
+  t->kphase = 0;
+  klaunch(t, KC_CG_START, 0, (int)wk, b_start, k_cg_start<TK, TP>, dim3(g), dim3(BLK), 0, s, nf,
+          b, r, x, rp, invd, dd, z0, (TP)ch.c0, bm_ptr(t), part);
   finish(t, s, FIN_START, g, reltol, abstol, maxiter, abs_target, seq);
   TP* z = nullptr;
+  t->kphase = 1;
   vcycle<TP>(t, hp, s, 0, vb, z0, zs, &z, chs, done, true);
   int par = 0;
-  // one iteration's launches; the V-cycle's output buffer is the same on
-  // every call, so z does not change from one iteration to the next
-  auto body = [&](int pr) {
-    hipLaunchKernelGGL((k_cg_pq<TK, TP>), dim3(g), dim3(BLK), 0, s, A, (const TP*)z,
-                       (const TK*)pb[pr], pb[pr ^ 1], q, (const int*)done,
-                       (const SolveScalars*)sc, bm_ptr(t), part);
+  // one iteration's launches (iteration it, for the timing records); the
+  // V-cycle's output buffer is the same on every call, so z does not change
+  // from one iteration to the next
+  auto body = [&](int pr, int it) {
+    t->kiter = it;
+    t->kphase = 0;
+    klaunch(t, KC_CG_PQ, 0, (int)wk, b_pq, k_cg_pq<TK, TP>, dim3(g), dim3(BLK), 0, s, A,
+            (const TP*)z, (const TK*)pb[pr], pb[pr ^ 1], q, (const int*)done,
+            (const SolveScalars*)sc, bm_ptr(t), part);
     finish(t, s, FIN_PQ, g);
-    hipLaunchKernelGGL((k_cg_xr<TK, TP>), dim3(g), dim3(BLK), 0, s, nf, x, r,
-                       (const TK*)pb[pr ^ 1], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0,
-                       (const int*)done, (const SolveScalars*)sc, bm_ptr(t), part);
+    klaunch(t, KC_CG_XR, 0, (int)wk, b_xr, k_cg_xr<TK, TP>, dim3(g), dim3(BLK), 0, s, nf, x, r,
+            (const TK*)pb[pr ^ 1], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0, (const int*)done,
+            (const SolveScalars*)sc, bm_ptr(t), part);
     finish(t, s, FIN_XR, g);
     TP* zz = nullptr;
+    t->kphase = 1;
     vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
     if (zz != z) throw Err{DPF_EHIP, "V-cycle output buffer moved between iterations"};
   };
-  bool graphs = t->use_graphs && !t->timing;
+  bool graphs = t->use_graphs && !t->timing && !t->ktiming;
   std::string gkey;
   if (graphs) {
     // everything the captured launches take as arguments and can differ
@@ -769,7 +898,7 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
     HIPTRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t gr = nullptr;
     try {
-      body(pr);
+      body(pr, 0);
     } catch (...) {
       (void)hipStreamEndCapture(s, &gr);
       if (gr) (void)hipGraphDestroy(gr);
@@ -808,7 +937,7 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
       if (graphs)
         HIPTRY(hipGraphLaunch(graph_for(par), s));
       else
-        body(par);
+        body(par, launched + 1);
       par ^= 1;
       launched++;
       HIPTRY(hipGetLastError());
@@ -825,7 +954,9 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
     }
     __builtin_ia32_pause();
   }
-  harvest_timing(t);
+  t->kiter = -1;
+  t->kphase = 0;
+  harvest_timing(t, res.iters);
   return res;
 }
 
@@ -1208,6 +1339,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int64_t nf = t->H.nf;
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
   t->use_graphs = (o->flags & DPF_FLAG_GRAPH) != 0;
+  const LvlSize z0s = lvl_size(t, 0);
   t->sweep_ms = 0;
   t->sweep_n = 0;
   int pidx = o->precision;
@@ -1239,17 +1371,21 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   // which is first replaced by its A-norm-optimal multiple.  Returns the
   // number of members whose residual is above their target.
   auto residual = [&](int first, bool warm = false) -> int {
-    hipLaunchKernelGGL(k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
+    klaunch(t, KC_OUTER_RESIDUAL, 0, 8,
+            first ? 16.0 * z0s.rows + 16.0 * g
+                  : z0s.nnz * 8.0 + z0s.cols + 24.0 * z0s.rows + 16.0 * g,
+            k_outer_residual, dim3(g), dim3(BLK), 0, s, A64,
                        first ? (const double*)nullptr : (const double*)t->x64.as<double>(), b,
                        t->r64.as<double>(), (const SolveScalars*)sc, bm_ptr(t), part);
     if (warm) {
       double* part2 = t->partial2.as<double>();
-      hipLaunchKernelGGL(k_warm_dots, dim3(g), dim3(BLK), 0, s, nf,
+      klaunch(t, KC_REFINE_VEC, 0, 8, 24.0 * z0s.rows, k_warm_dots, dim3(g), dim3(BLK), 0, s, nf,
                          (const double*)t->x64.as<double>(), b, (const double*)t->r64.as<double>(),
                          (const SolveScalars*)sc, bm_ptr(t), part, part2);
       finish(t, s, FIN_WXB, g);
       finish(t, s, FIN_WXR, g, 0, 0, 0, 0, 0, part2);
-      hipLaunchKernelGGL(k_warm_apply, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(), b,
+      klaunch(t, KC_REFINE_VEC, 0, 8, 40.0 * z0s.rows, k_warm_apply, dim3(g), dim3(BLK), 0, s, nf,
+              t->x64.as<double>(), b,
                          t->r64.as<double>(), (const SolveScalars*)sc, bm_ptr(t), part);
     }
     int oseq = ++t->seq;
@@ -1267,15 +1403,18 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   // and keeps the FP64 norms of b and of the residuals clear of underflow
   // and overflow (an adjoint that decays over many time steps reaches
   // |b| ~ 1e-200 long before it reaches zero)
-  hipLaunchKernelGGL(k_amax, dim3(g), dim3(BLK), 0, s, nf, b, bm_ptr(t), part);
+  klaunch(t, KC_REFINE_VEC, 0, 8, 8.0 * z0s.rows, k_amax, dim3(g), dim3(BLK), 0, s, nf, b,
+          bm_ptr(t), part);
   finish(t, s, FIN_AMAX, g);
-  hipLaunchKernelGGL(k_zero_f64, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>());
+  klaunch(t, KC_REFINE_VEC, 0, 8, 8.0 * z0s.rows, k_zero_f64, dim3(g), dim3(BLK), 0, s, nf,
+          t->x64.as<double>());
   int active = residual(1);
   std::vector<double> bnorm(t->nm);
   for (int m = 0; m < t->nm; m++) bnorm[m] = mirror_load(&t->hm_outer_host[m]);
   bool finite = all_finite();
   if (finite && x0) {
-    hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(BLK), 0, s, nf, x0, t->x64.as<double>(),
+    klaunch(t, KC_REFINE_VEC, 0, 8, 16.0 * z0s.rows, k_scale_copy, dim3(g), dim3(BLK), 0, s, nf,
+            x0, t->x64.as<double>(),
                        (const SolveScalars*)sc, bm_ptr(t), 0);
     active = residual(0, true);
     finite = all_finite();
@@ -1320,7 +1459,8 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     }
     SolveResult r;
     if (o->precision == DPF_MIXED)
-      hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(BLK), 0, s, nf,
+      klaunch(t, KC_REFINE_VEC, 0, 8, 12.0 * z0s.rows, k_scale_to_f32, dim3(g), dim3(BLK), 0, s,
+              nf,
                          (const double*)t->r64.as<double>(), t->b32.as<float>(),
                          (const SolveScalars*)sc, bm_ptr(t));
     if (tune) {
@@ -1333,8 +1473,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       // the whole refinement's worth of convergence, where a cheaper V-cycle
       // that converges more slowly loses what it seems to win over the first
       // few digits.
-      bool graphs_on = t->use_graphs;
+      bool graphs_on = t->use_graphs, ktiming_on = t->ktiming, timing_on = t->timing;
       t->use_graphs = false;  // trials use launches: a capture would distort their times
+      t->ktiming = t->timing = false;  // nor do the trials enter the timing records
       static const int cand[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
       static const double ratios[] = {30.0, 8.0};
       double trial_tol = std::max(o->inner_reltol * o->inner_reltol, 1e-12);
@@ -1363,15 +1504,19 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       for (int k = 0; k < 2; k++) trial(m0, mc, ratios[k], 5 + k);
       t->tuned[pidx] = true;
       t->use_graphs = graphs_on;
+      t->ktiming = ktiming_on;
+      t->timing = timing_on;
       tune = false;
       chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
     }
     r = krylov(chs, o->inner_reltol);
     if (o->precision != DPF_MIXED) {
-      hipLaunchKernelGGL(k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf, 1.0,
+      klaunch(t, KC_REFINE_VEC, 0, 8, 24.0 * z0s.rows, k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf,
+              1.0,
                          (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
     } else {
-      hipLaunchKernelGGL(k_refine, dim3(g), dim3(BLK), 0, s, nf, t->x64.as<double>(),
+      klaunch(t, KC_REFINE_VEC, 0, 8, 20.0 * z0s.rows, k_refine, dim3(g), dim3(BLK), 0, s, nf,
+              t->x64.as<double>(),
                          (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc, bm_ptr(t));
     }
     total_it += r.iters;
@@ -1396,7 +1541,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   }
   if (!(relres == relres)) relres = INFINITY;
 
-  hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(BLK), 0, s, nf,
+  klaunch(t, KC_REFINE_VEC, 0, 8, 16.0 * z0s.rows, k_scale_copy, dim3(g), dim3(BLK), 0, s, nf,
                      (const double*)t->x64.as<double>(), xout, (const SolveScalars*)sc, bm_ptr(t),
                      1);
   if (st) {
@@ -1523,16 +1668,19 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
       t->stage_e.alloc(t->H.ne * sizeof(double));
       out = t->stage_e.as<double>();
     }
-    if (geometric)
-      hipLaunchKernelGGL(k_geo_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
-                         t->edges.as<int2>(), lk, out);
-    else {
-      DBuf kb(t->pool.get());
+    ktiming_begin(t, o);
+    const double nn = (double)t->H.n, ne = (double)t->H.ne;
+    DBuf kb(t->pool.get());
+    if (geometric) {
+      klaunch(t, KC_KMAP, -1, 8, ne * 16.0 + nn * 8.0, k_geo_forward, dim3(nblocks(t->H.ne)),
+              dim3(BLK), 0, s, t->H.ne, t->edges.as<int2>(), lk, out);
+    } else {
       kb.alloc(t->H.n * sizeof(double));
-      hipLaunchKernelGGL(k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, lk,
-                         kb.as<double>());
-      hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(t->H.ne)), dim3(BLK), 0, s, t->H.ne,
-                         t->edges.as<int2>(), (const double*)kb.as<double>(), out);
+      klaunch(t, KC_KMAP, -1, 8, nn * 16.0, k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s,
+              t->H.n, lk, kb.as<double>());
+      klaunch(t, KC_KMAP, -1, 8, ne * 16.0 + nn * 8.0, k_hm_forward, dim3(nblocks(t->H.ne)),
+              dim3(BLK), 0, s, t->H.ne, t->edges.as<int2>(), (const double*)kb.as<double>(),
+              out);
     }
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
@@ -1541,6 +1689,7 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
     // when this call returns, and a later call on another stream may take it
     // while k_hm_forward would still be reading it
     HIPTRY(hipStreamSynchronize(s));
+    ktiming_end(t);
     return DPF_OK;
   });
 }
@@ -1566,23 +1715,27 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
       t->stage_n2.alloc(t->H.n * sizeof(double));
       out = t->stage_n2.as<double>();
     }
-    if (geometric)
-      hipLaunchKernelGGL(k_geo_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
-                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(), ke, dk, out);
-    else {
-      DBuf kb(t->pool.get());
+    ktiming_begin(t, o);
+    const double nn = (double)t->H.n, ninc = (double)t->H.inc_ptr.back();
+    DBuf kb(t->pool.get());
+    if (geometric) {
+      klaunch(t, KC_KMAP, -1, 8, ninc * 4.0 + ninc * 16.0 + nn * 16.0, k_geo_pullback,
+              dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, t->inc_ptr.as<int64_t>(),
+              t->inc_edge.as<int32_t>(), ke, dk, out);
+    } else {
       kb.alloc(t->H.n * sizeof(double));
-      hipLaunchKernelGGL(k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, lk,
-                         kb.as<double>());
-      hipLaunchKernelGGL(k_hm_pullback, dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n,
-                         t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
-                         t->inc_other.as<int32_t>(), (const double*)kb.as<double>(), ke, dk,
-                         out);
+      klaunch(t, KC_KMAP, -1, 8, nn * 16.0, k_exp, dim3(nblocks(t->H.n)), dim3(BLK), 0, s,
+              t->H.n, lk, kb.as<double>());
+      // per incidence entry: edge, other cell, K_e and dK_e; per cell k, dlogK
+      klaunch(t, KC_KMAP, -1, 8, ninc * 24.0 + nn * 24.0, k_hm_pullback, dim3(nblocks(t->H.n)),
+              dim3(BLK), 0, s, t->H.n, t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+              t->inc_other.as<int32_t>(), (const double*)kb.as<double>(), ke, dk, out);
     }
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(dlogK, out, t->H.n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
+    ktiming_end(t);
     return DPF_OK;
   });
 }
@@ -1639,6 +1792,7 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
       HIPTRY(hipMemcpyAsync(t->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
     }
+    ktiming_begin(t, o);
     HIPTRY(hipEventRecord(t->ev_setup[0], s));
     prepare(t, s, o->precision, t->Q.as<double>(), true);
     HIPTRY(hipEventRecord(t->ev_setup[1], s));
@@ -1659,8 +1813,8 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       t->stage_n2.alloc(n * sizeof(double));
       hdev = t->stage_n2.as<double>();
     }
-    hipLaunchKernelGGL(k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
-                       (const double*)hfree, t->dh.as<double>(), hdev);
+    klaunch(t, KC_SCATTER, -1, 8, (double)n * 20.0, k_scatter_h, dim3(nblocks(n)), dim3(BLK), 0,
+            s, n, t->n2f.as<int32_t>(), (const double*)hfree, t->dh.as<double>(), hdev);
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(h_out, hdev, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1682,6 +1836,7 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       stats->setup_ms = setup;
       stats->solve_ms = stats->total_ms - setup;
     }
+    ktiming_end(t);
     if (status == DPF_ENOCONV) set_err("AMG-PCG did not reach the tolerance within maxiter");
     if (status == DPF_EBREAKDOWN) set_err("AMG-PCG breakdown (non-SPD operator or NaN input)");
     if (state_out && status == DPF_OK) *state_out = st.release();
@@ -1709,6 +1864,7 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
     hipStream_t s = pick_stream(t, o);
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     // the hierarchy must hold this state's conductivities in the requested precision
+    ktiming_begin(t, o);
     bool fresh = (t->version == st->version) &&
                  ((o->precision != DPF_FP64) ? t->h32.version == st->version
                                               : t->h64.version == st->version);
@@ -1722,8 +1878,8 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
     st->lam.alloc(nf * sizeof(double));
     t->stage_f3.alloc(nf * sizeof(double));
     double* rhs = t->stage_f3.as<double>();
-    hipLaunchKernelGGL(k_gather_free, dim3(nblocks(nf)), dim3(BLK), 0, s, nf,
-                       t->f2n.as<int32_t>(), dl, rhs);
+    klaunch(t, KC_SCATTER, -1, 8, (double)nf * 20.0, k_gather_free, dim3(nblocks(nf)), dim3(BLK),
+            0, s, nf, t->f2n.as<int32_t>(), dl, rhs);
     double t1 = now_ms();
     int status = solve_system(t, s, rhs, st->lam.as<double>(), o, stats);
     double t2 = now_ms();
@@ -1737,13 +1893,16 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
       if (ddh) { bd.alloc(n * sizeof(double)); dd = bd.as<double>(); }
       if (dQs) { bq.alloc(n * sizeof(double)); dq = bq.as<double>(); }
     }
+    // per edge: the pair, aol, the free maps of its cells and the write;
+    // per free cell its head and multiplier
     if (dK)
-      hipLaunchKernelGGL(k_edge_grad, dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(),
-                         t->n2f.as<int32_t>(), t->aol.as<double>(),
-                         (const double*)st->hfree.as<double>(), (const double*)st->dh.as<double>(),
-                         lam, 1.0, 0, dK);
+      klaunch(t, KC_GRADIENT, -1, 8, (double)ne * 32.0 + (double)nf * 16.0, k_edge_grad,
+              dim3(nblocks(ne)), dim3(BLK), 0, s, ne, t->edges.as<int2>(), t->n2f.as<int32_t>(),
+              t->aol.as<double>(), (const double*)st->hfree.as<double>(),
+              (const double*)st->dh.as<double>(), lam, 1.0, 0, dK);
     if (dd || dq)
-      hipLaunchKernelGGL(k_ddh_dq, dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
+      klaunch(t, KC_GRADIENT, -1, 8, (double)n * 28.0 + (double)nf * 8.0, k_ddh_dq,
+              dim3(nblocks(n)), dim3(BLK), 0, s, n, t->n2f.as<int32_t>(),
                          t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
                          t->inc_other.as<int32_t>(), (const double*)st->Ke.as<double>(),
                          t->aol.as<double>(), lam, dl, dd, dq);
@@ -1759,10 +1918,37 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
       stats->solve_ms = t2 - t1;
       stats->total_ms = now_ms() - t0;
     }
+    ktiming_end(t);
     if (status == DPF_ENOCONV) set_err("adjoint AMG-PCG did not reach the tolerance within maxiter");
     if (status == DPF_EBREAKDOWN) set_err("adjoint AMG-PCG breakdown");
     return status;
   });
+}
+
+int dpf_kernel_table(dpf_topology* t, dpf_kernel_record* out, int32_t cap, int32_t* count,
+                     int32_t reset) {
+  if (!t || !count || cap < 0 || (cap > 0 && !out)) {
+    set_err("invalid argument");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  int32_t k = 0;
+  for (const auto& kv : t->ktable) {
+    if (k < cap) {
+      dpf_kernel_record& r = out[k];
+      std::memset(&r, 0, sizeof(r));
+      std::snprintf(r.kernel, sizeof(r.kernel), "%s", kcat_name[std::get<0>(kv.first)]);
+      r.level = std::get<1>(kv.first);
+      r.value_bytes = std::get<2>(kv.first);
+      r.launches = kv.second.launches;
+      r.ms = kv.second.ms;
+      r.bytes = kv.second.bytes;
+    }
+    k++;
+  }
+  *count = k;
+  if (reset) t->ktable.clear();
+  return DPF_OK;
 }
 
 int dpf_state_destroy(dpf_state* st) {

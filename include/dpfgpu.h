@@ -47,9 +47,13 @@ enum { DPF_HOST = 0, DPF_DEVICE = 1 };
  * V-cycle: ~20 launches) as one hipGraph, captured on the topology's first
  * iteration with the same precision and smoother and kept until it is
  * destroyed; the kernels and their order are those of the launched form, so
- * results are bitwise equal.  Ignored together with sweep timing and during
- * the autotuner's trials.  The stream must not be the legacy NULL stream. */
-enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2 };
+ * results are bitwise equal.  Ignored together with sweep or kernel timing
+ * and during the autotuner's trials.  The stream must not be the legacy NULL
+ * stream.  DPF_FLAG_KERNEL_TIMING times every launch of dpf_steady_forward,
+ * dpf_steady_pullback and the logK maps the same way and adds the durations
+ * and algorithmic bytes to the topology's table (dpf_kernel_table); launches
+ * that the converged solve turned into no-ops are left out. */
+enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4 };
 
 typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
@@ -381,6 +385,25 @@ int dpf_transient_pullback_seeds(dpf_state *state, int32_t nseed, const int32_t 
  *   dJ_du_final: nfree;  dKs: nedges;  dQs: n or NULL. */
 int dpf_transient_pullback(dpf_state *state, const double *dJ_du_final, double *dKs,
                            double *dQs, const dpf_opts *opts, dpf_stats *stats);
+
+/* The kernel table of calls made with DPF_FLAG_KERNEL_TIMING (no counterpart
+ * in the reference): one record per kernel class, AMG level (-1 where none
+ * applies) and value size (4 or 8 bytes), with the launches, their summed
+ * duration (ms, from dispatch timestamps) and summed algorithmic bytes (the
+ * bytes a launch must move: operator values and the columns it stores,
+ * vectors read and written once each, index maps of the gathers).
+ * *count = records in the table; at most cap are written to out.  reset != 0
+ * empties the table afterwards. */
+typedef struct {
+  char kernel[32];
+  int32_t level;
+  int32_t value_bytes;
+  int64_t launches;
+  double ms;
+  double bytes;
+} dpf_kernel_record;
+int dpf_kernel_table(dpf_topology *topo, dpf_kernel_record *out, int32_t cap, int32_t *count,
+                     int32_t reset);
 
 /* Kernel timing for roofline reporting (no counterpart in the reference).
  * Launches one kernel of the current hierarchy `reps` times back to back
