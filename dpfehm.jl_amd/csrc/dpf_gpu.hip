@@ -149,8 +149,8 @@ static T* upload(DBuf& d, const std::vector<T>& v) {
 
 struct SellDev {
   int64_t nrows = 0, ncols = 0, entries = 0;
-  DBuf sptr, col, rowlen, soff;
-  bool offsets = false;
+  DBuf sptr, col, rowlen, soff, col16, cbase;
+  bool offsets = false, compact = false;
   void up(const SellHost& h) {
     nrows = h.nrows;
     ncols = h.ncols;
@@ -158,6 +158,44 @@ struct SellDev {
     upload(sptr, h.sptr);
     upload(col, h.col);
     upload(rowlen, h.rowlen);
+  }
+  // Compact columns (16-bit offsets from each slice's smallest column, the
+  // padding slots pointing at that column) when every slice's columns span
+  // at most 65535; 32-bit columns otherwise.  For operators whose columns
+  // only the row sums read (split level operators, prolongators,
+  // restrictions).
+  void up_compact(const SellHost& h) {
+    const int64_t ns = h.nslices();
+    std::vector<int32_t> base(ns, 0);
+    std::vector<uint16_t> c16(h.entries(), 0);
+    int bad = 0;
+    for (int64_t sl = 0; sl < ns; sl++) {
+      int64_t r0 = sl * SELL_C, r1 = std::min<int64_t>(h.nrows, r0 + SELL_C);
+      int32_t lo = INT32_MAX, hi = INT32_MIN;
+      for (int64_t r = r0; r < r1; r++)
+        for (int k = 0; k < h.rowlen[r]; k++) {
+          int32_t c = h.at(r, k);
+          lo = std::min(lo, c);
+          hi = std::max(hi, c);
+        }
+      if (lo > hi) lo = hi = 0;  // a slice without entries
+      if ((int64_t)hi - lo > 65535) bad = 1;
+      base[sl] = lo;
+      for (int64_t r = r0; r < r1; r++)
+        for (int k = 0; k < h.rowlen[r]; k++) c16[h.idx(r, k)] = (uint16_t)(h.at(r, k) - lo);
+    }
+    if (bad) {
+      up(h);
+      return;
+    }
+    nrows = h.nrows;
+    ncols = h.ncols;
+    entries = h.entries();
+    upload(sptr, h.sptr);
+    upload(rowlen, h.rowlen);
+    upload(col16, c16);
+    upload(cbase, base);
+    compact = true;
   }
   // the fine operator in its offset-slice layout (rowlen is not uploaded:
   // a row's entries are not the first slots of an offset slice)
@@ -173,7 +211,9 @@ struct SellDev {
   template <typename T>
   SellD<T> view(T* val) const {
     return SellD<T>{nrows, sptr.as<int64_t>(), col.as<int32_t>(), val,
-                    offsets ? soff.as<int32_t>() : nullptr};
+                    offsets ? soff.as<int32_t>() : nullptr,
+                    compact ? col16.as<uint16_t>() : nullptr,
+                    compact ? cbase.as<int32_t>() : nullptr};
   }
 };
 
@@ -442,19 +482,25 @@ void harvest_timing(dpf_topology* t, int its = -1) {
 }
 
 // Sizes for the algorithmic byte counts of level l: rows, nonzeros of the
-// level operator and the bytes of its stored columns (the fine operator's
-// offset slices store none), nonzeros of the prolongator and rows of the
-// next level.
+// level operator and the bytes of the columns the smoothing sweeps read (the
+// fine operator's offset slices store none; compact columns take 2 bytes),
+// nonzeros of the prolongator, the bytes of one P or R column, and rows of
+// the next level.
 struct LvlSize {
-  double rows, nnz, cols, pnnz, nc;
+  double rows, nnz, cols, pnnz, pcol, nc;
 };
 LvlSize lvl_size(const dpf_topology* t, int l) {
   const LevelHost& H = t->H.levels[l];
+  const LevelDev& D = *t->lev[l];
   LvlSize z;
   z.rows = (double)H.A.nrows;
   z.nnz = (double)H.A.nnz;
-  z.cols = 4.0 * (l == 0 ? (double)t->fine_explicit_nnz : (double)H.A.nnz);
+  if (l == 0)
+    z.cols = 4.0 * (double)t->fine_explicit_nnz;
+  else
+    z.cols = (D.as_lg > 0 && D.As.compact ? 2.0 : 4.0) * (double)H.A.nnz;
   z.pnnz = H.agg.empty() ? 0.0 : (double)H.P.nnz;
+  z.pcol = D.transfer && D.P.compact ? 2.0 : 4.0;
   z.nc = H.agg.empty() ? 0.0 : (double)H.R.nrows;
   return z;
 }
@@ -758,7 +804,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
   SellD<T> Rs = d.Rs.view<T>(n.rval.template as<T>());
   klaunch(t, KC_RESTRICT, l, w,
-          z.pnnz * (w + 4.0) + z.rows * w + z.nc * w * (coarse_next ? 1.0 : 4.0),
+          z.pnnz * (w + (d.Rs.compact ? 2.0 : 4.0)) + z.rows * w + z.nc * w * (coarse_next ? 1.0 : 4.0),
           k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg, dn.A.nrows,
           (const T*)rl, nn.b.template as<T>(), (const T*)level_invd(t, h, l + 1),
           nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0, coarse_next ? 0 : 1,
@@ -767,7 +813,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   if (coarse_next && dn.A.nrows <= COARSE_FOLD_MAX) {
     // the dense coarsest solve is folded into the prolongation
     klaunch(t, KC_PROLONG_COARSE, l, w,
-            z.pnnz * (w + 4.0) + z.nc * z.nc * 8.0 + z.nc * w + 2.0 * z.rows * w,
+            z.pnnz * (w + z.pcol) + z.nc * z.nc * 8.0 + z.nc * w + 2.0 * z.rows * w,
             k_prolong_coarse<T>, dim3(g), dim3(BLK), 0, s, P, (int)dn.A.nrows,
             (const double*)h.W.template as<double>(), (const T*)nn.b.template as<T>(), cur,
             (const int*)done);
@@ -775,7 +821,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     T* xc = nullptr;
     vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
               nn.xs.template as<T>(), &xc, chs, done, false);
-    klaunch(t, KC_PROLONG, l, w, z.pnnz * (w + 4.0) + z.nc * w + 2.0 * z.rows * w,
+    klaunch(t, KC_PROLONG, l, w, z.pnnz * (w + z.pcol) + z.nc * w + 2.0 * z.rows * w,
             k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur, (const int*)done);
   }
   for (int k = 0; k < m; k++) {
@@ -1062,15 +1108,15 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
       upload(d->diag_slot, fl ? F.diag_slot : lh.diag_slot);
       d->as_lg = lh.as_lg;
       if (lh.as_lg > 0) {
-        d->As.up(lh.As);
+        d->As.up_compact(lh.As);
         upload(d->as_src, lh.as_src);
         maxrows = std::max(maxrows, lh.As.nrows);
       }
       maxrows = std::max(maxrows, lh.A.nrows);
       if (!lh.agg.empty()) {
         d->transfer = true;
-        d->P.up(lh.P);
-        d->Rs.up(lh.Rs);
+        d->P.up_compact(lh.P);
+        d->Rs.up_compact(lh.Rs);
         d->rs_lg = lh.rs_lg;
         d->nc = lh.R.nrows;
         upload(d->p_slot, fl ? F.p_slot : lh.p_slot);

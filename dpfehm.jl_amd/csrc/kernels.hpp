@@ -71,6 +71,9 @@ struct SolveScalars {
 // A SELL-64 operator on the device.  soff (fine operator only, else null):
 // per slice 9 ints, [0] = width of an offset slice (0: explicit columns),
 // [1..8] the offsets o_k, so that slot k of row r has column r + o_k.
+// col16 / cbase (compact columns, else null): the column of slot e of slice s
+// is cbase[s] + col16[e] (every column of a slice within 65535 of its
+// smallest; padding slots point at that smallest column), in place of col.
 template <typename T>
 struct SellD {
   int64_t nrows;
@@ -78,6 +81,8 @@ struct SellD {
   const int32_t* col;
   T* val;
   const int32_t* soff = nullptr;
+  const uint16_t* col16 = nullptr;
+  const int32_t* cbase = nullptr;
 };
 
 // The offsets of row r's slice if it is an offset slice, else null.  Every
@@ -473,6 +478,7 @@ struct SellRow {
   int w;
   int offs;  // nonzero: an offset slice, slot k of row r has column r + o[k]
   int32_t o[8];
+  int32_t cb;  // compact columns: the slice's base column
 };
 
 template <typename T>
@@ -483,6 +489,7 @@ __device__ __forceinline__ SellRow sell_row_at(const SellD<T>& A, int64_t r) {
   R.base = p0 + (r & 63);
   R.w = (int)((A.sptr[s + 1] - p0) >> 6);
   R.offs = 0;
+  R.cb = A.cbase ? A.cbase[s] : 0;
 #pragma unroll
   for (int u = 0; u < 8; u++) R.o[u] = 0;
   if (A.soff) {
@@ -924,6 +931,25 @@ __device__ __forceinline__ T dot_group(const SellD<T>& A, int64_t e0,
   return s;
 }
 
+// The same with compact columns: cb + 16-bit offsets.
+template <int N, typename T, typename TX>
+__device__ __forceinline__ T dot_group16(const SellD<T>& A, int64_t e0, int32_t cb,
+                                         const TX* __restrict__ x, T s) {
+  T v[N];
+  int32_t c[N];
+  TX xv[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) {
+    v[u] = A.val[e0 + (int64_t)u * 64];
+    c[u] = cb + (int32_t)A.col16[e0 + (int64_t)u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; u++) xv[u] = x[c[u]];
+#pragma unroll
+  for (int u = 0; u < N; u++) s += v[u] * (T)xv[u];
+  return s;
+}
+
 // The same over an offset slice's N slots: columns r + o[u], no column loads.
 template <int N, typename T, typename TX>
 __device__ __forceinline__ T dot_group_off(const SellD<T>& A, int64_t e0, int32_t r,
@@ -989,6 +1015,22 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, const SellRow& R, int64
     }
   }
   int k0 = 0;
+  if (A.col16) {
+    const int32_t cb = R.cb;
+    for (; k0 + 8 <= w; k0 += 8) s = dot_group16<8>(A, base + (int64_t)k0 * 64, cb, x, s);
+    const int64_t e0 = base + (int64_t)k0 * 64;
+    switch (w - k0) {
+      case 1: s = dot_group16<1>(A, e0, cb, x, s); break;
+      case 2: s = dot_group16<2>(A, e0, cb, x, s); break;
+      case 3: s = dot_group16<3>(A, e0, cb, x, s); break;
+      case 4: s = dot_group16<4>(A, e0, cb, x, s); break;
+      case 5: s = dot_group16<5>(A, e0, cb, x, s); break;
+      case 6: s = dot_group16<6>(A, e0, cb, x, s); break;
+      case 7: s = dot_group16<7>(A, e0, cb, x, s); break;
+      default: break;
+    }
+    return s;
+  }
   for (; k0 + 8 <= w; k0 += 8) s = dot_group<8>(A, base + (int64_t)k0 * 64, x, s);
   const int64_t e0 = base + (int64_t)k0 * 64;
   switch (w - k0) {
