@@ -65,6 +65,9 @@ def main():
                          "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
                          "run's configuration under a profiler, whose overheads change the "
                          "autotuner's timings)")
+    ap.add_argument("--fp32-storage", action="store_true",
+                    help="steady2m: the FP32 V-cycle reads FP32 operator values instead of "
+                         "bf16 (DPF_FLAG_FP32_STORAGE)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -126,9 +129,10 @@ def main():
 
     def opts(precision, timing=False):
         o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph,
-                       "inner_reltol": args.inner_reltol}, memory=_lib.DPF_DEVICE, stream=sptr)
+                       "inner_reltol": args.inner_reltol, "fp32_storage": args.fp32_storage},
+                      memory=_lib.DPF_DEVICE, stream=sptr)
         if timing:
-            o.flags = _lib.DPF_FLAG_KERNEL_TIMING
+            o.flags = (o.flags | _lib.DPF_FLAG_KERNEL_TIMING) & ~_lib.DPF_FLAG_GRAPH
         return o
 
     def p(t):

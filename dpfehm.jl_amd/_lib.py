@@ -15,6 +15,7 @@ DPF_HOST, DPF_DEVICE = 0, 1
 DPF_FLAG_SWEEP_TIMING = 1
 DPF_FLAG_GRAPH = 2
 DPF_FLAG_KERNEL_TIMING = 4
+DPF_FLAG_FP32_STORAGE = 8
 
 _ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOCONV", 4: "EBREAKDOWN", 5: "ESINGULAR", 6: "ENOMEM"}
 

@@ -20,7 +20,8 @@ recomputes it in every call.
 
 Keyword arguments: ``reltol`` (default 1e-10), ``abstol``, ``maxiter``,
 ``precision`` ("fp64", "mixed" or "mixed_pc"), ``inner_reltol``, ``max_outer``,
-``cheb_degree``, ``graph``.  ``linear_solver`` selects amg_solver (the
+``cheb_degree``, ``graph``, ``fp32_storage`` (the FP32 V-cycle reads FP32
+instead of bf16 operator values).  ``linear_solver`` selects amg_solver (the
 default: AMG-PCG to ``reltol``) or cholesky_solver (the same AMG-PCG to a
 relative residual of 1e-14, a direct solve's accuracy; tolerance keywords
 ignored), as a function or by name; the reference's rrule always uses AMG
@@ -84,7 +85,8 @@ def cholesky_solver(A=None, b=None, **kwargs):
 
 def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
     """dpf_opts from keyword arguments; ``graph=True`` sets DPF_FLAG_GRAPH
-    (each PCG iteration replayed as one captured hipGraph); ``linear_solver``
+    (each PCG iteration replayed as one captured hipGraph), ``fp32_storage=True``
+    DPF_FLAG_FP32_STORAGE; ``linear_solver``
     is amg_solver (default) or cholesky_solver, as a function or by name."""
     kw = dict(kwargs)
     ls = kw.pop("linear_solver", None)
@@ -94,12 +96,15 @@ def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
     if name == "cholesky_solver":
         kw["reltol"], kw["abstol"] = 1e-14, 0.0
     graph = bool(kw.pop("graph", False))
+    fp32_storage = bool(kw.pop("fp32_storage", False))
     bad = set(kw) - _OPT_KEYS
     if bad:
         raise TypeError(f"unexpected keyword argument(s): {sorted(bad)}")
     o = _lib.default_opts(**kw)
     if graph:
         o.flags |= _lib.DPF_FLAG_GRAPH
+    if fp32_storage:
+        o.flags |= _lib.DPF_FLAG_FP32_STORAGE
     o.memory = memory
     o.stream = stream
     return o

@@ -237,6 +237,9 @@ template <typename T>
 struct LevelNum {
   DBuf val, invd, pval, rval, apval, asval;
   DBuf x, xs, b, r, d;
+  // FP32 hierarchy: bf16 copies of the transfer operators and of the coarse
+  // split operator, and that operator's compensated FP32 diagonal
+  DBuf p16, r16, a16, as16, dg;
 };
 
 template <typename T>
@@ -366,6 +369,8 @@ struct dpf_topology {
   // (sweep timing) or every launch (kernel timing), per kernel class, level
   // and value size
   bool timing = false, ktiming = false;
+  // the FP32 V-cycle reads the bf16 copies (off with DPF_FLAG_FP32_STORAGE)
+  bool vbf16 = true;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   double sweep_ms = 0;
@@ -599,6 +604,38 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
   HIPTRY(hipGetLastError());
 }
 
+// bf16 copies the FP32 V-cycle reads: every prolongator and restriction,
+// and every coarse level's split operator (with its compensated diagonal).
+void make_bf16(dpf_topology* t, Hier<float>& h, hipStream_t s) {
+  int L = t->levels();
+  for (int l = 0; l < L; l++) {
+    LevelDev& d = *t->lev[l];
+    LevelNum<float>& n = h.lev[l];
+    if (d.transfer) {
+      n.p16.alloc(d.P.entries * sizeof(uint16_t));
+      n.r16.alloc(d.Rs.entries * sizeof(uint16_t));
+      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.P.entries, k_to_bf16, dim3(nblocks(d.P.entries)),
+              dim3(BLK), 0, s, d.P.entries, (const float*)n.pval.as<float>(),
+              n.p16.as<uint16_t>());
+      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.Rs.entries, k_to_bf16, dim3(nblocks(d.Rs.entries)),
+              dim3(BLK), 0, s, d.Rs.entries, (const float*)n.rval.as<float>(),
+              n.r16.as<uint16_t>());
+    }
+    if (l > 0 && d.as_lg > 0) {
+      n.a16.alloc(d.A.entries * sizeof(uint16_t));
+      n.as16.alloc(d.As.entries * sizeof(uint16_t));
+      n.dg.alloc(d.A.nrows * sizeof(float));
+      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.A.entries + 6.0 * d.A.nrows, k_level_bf16,
+              dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, d.A.view<float>(level_val(t, h, l)),
+              d.diag_slot.as<uint16_t>(), n.a16.as<uint16_t>(), n.dg.as<float>());
+      klaunch(t, KC_RGATHER, l, 4, 8.0 * d.As.entries, k_gather_bf16,
+              dim3(nblocks(d.As.entries)), dim3(BLK), 0, s, d.As.entries,
+              d.as_src.as<int32_t>(), (const uint16_t*)n.a16.as<uint16_t>(),
+              n.as16.as<uint16_t>());
+    }
+  }
+}
+
 // Numeric refresh of levels 1.. and the coarsest inverse, from level 0 values.
 template <typename T>
 void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
@@ -650,6 +687,7 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
             dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
             h.W.template as<double>(), h.status.template as<int>());
   }
+  if constexpr (std::is_same<T, float>::value) make_bf16(t, h, s);
   HIPTRY(hipGetLastError());
 }
 
@@ -725,9 +763,11 @@ void finish(dpf_topology* t, hipStream_t s, int mode, unsigned nparts, double re
 // Algorithmic bytes of a Chebyshev sweep on level l (values of w bytes):
 // the operator, reads of x, b and the l1 diagonal, the write of x', and d
 // read (c1 != 0) and written (when a later sweep reads it).
-double cheb_bytes(const dpf_topology* t, int l, double w, bool din, bool dout) {
+// (With bf16 values, 2 bytes per value and the compensated diagonal read.)
+double cheb_bytes(const dpf_topology* t, int l, double w, bool din, bool dout, bool b16) {
   LvlSize z = lvl_size(t, l);
-  return z.nnz * w + z.cols + z.rows * w * (4.0 + (din ? 1.0 : 0.0) + (dout ? 1.0 : 0.0));
+  return z.nnz * (b16 ? 2.0 : w) + z.cols +
+         z.rows * w * (4.0 + (din ? 1.0 : 0.0) + (dout ? 1.0 : 0.0) + (b16 ? 1.0 : 0.0));
 }
 
 // One V-cycle on levels l..: input b at level l (for l = 0 the CG residual),
@@ -748,9 +788,19 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   int m = (int)ch.c1.size();
   const int w = (int)sizeof(T);
   const LvlSize z = lvl_size(t, l);
-  // the smoothing sweeps and the residual use the row-split copy on coarse levels
+  // the smoothing sweeps and the residual use the row-split copy on coarse
+  // levels; the FP32 V-cycle reads it, and the transfer operators, in bf16
   int lg = d.as_lg;
   SellD<T> Asm = lg > 0 ? d.As.view<T>(n.asval.template as<T>()) : A;
+  constexpr bool is32 = std::is_same<T, float>::value;
+  const bool b16 = is32 && t->vbf16;
+  const bool a16 = b16 && l > 0 && lg > 0;
+  const T* dg = nullptr;
+  if (a16) {
+    Asm.v16 = n.as16.template as<uint16_t>();
+    dg = n.dg.template as<T>();
+  }
+  const double wv = a16 ? 2.0 : (double)w, wt = b16 ? 2.0 : (double)w;
   unsigned gs = nblocks(Asm.nrows);
   int64_t nr = d.A.nrows;
   T* cur = xl;
@@ -775,10 +825,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
       for (int k = 1; k < 2 * m; k++) {
         int fuse = (fuse_dot_at_fine && k == 2 * m - 1) ? 1 : 0;
         bool wd = k != 2 * m - 1;
-        klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd), cheb_kernel<T>(l), dim3(gs),
-                dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
-                wd ? dd : (T*)nullptr, invd, (T)c2.c1[k], (T)c2.c2[k], (const int*)done, fuse,
-                bm_ptr(t), part);
+        klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd, a16), cheb_kernel<T>(l),
+                dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
+                wd ? dd : (T*)nullptr, invd, dg, (T)c2.c1[k], (T)c2.c2[k], (const int*)done,
+                fuse, bm_ptr(t), part);
         if (fuse) finish(t, s, FIN_RZ, g);
         std::swap(cur, oth);
       }
@@ -790,30 +840,34 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   for (int k = 1; k < m; k++) {
     // the last pre-smoothing sweep's d is dead: the post-smoothing restarts at c1 = 0
     bool wd = k != m - 1;
-    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd), cheb_kernel<T>(l), dim3(gs),
+    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, true, wd, a16), cheb_kernel<T>(l), dim3(gs),
             dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
-            wd ? dd : (T*)nullptr, invd, (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0,
+            wd ? dd : (T*)nullptr, invd, dg, (T)ch.c1[k], (T)ch.c2[k], (const int*)done, 0,
             bm_ptr(t), part);
     std::swap(cur, oth);
   }
   T* rl = n.r.template as<T>();
-  klaunch(t, KC_RESIDUAL, l, w, z.nnz * w + z.cols + 3.0 * z.rows * w, k_residual<T>,
-          dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, rl, (const int*)done);
+  klaunch(t, KC_RESIDUAL, l, w, z.nnz * wv + z.cols + (a16 ? 4.0 : 3.0) * z.rows * w,
+          k_residual<T>, dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, dg, rl,
+          (const int*)done);
   LevelDev& dn = *t->lev[l + 1];
   LevelNum<T>& nn = h.lev[l + 1];
   bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
   SellD<T> Rs = d.Rs.view<T>(n.rval.template as<T>());
+  if (b16) Rs.v16 = n.r16.template as<uint16_t>();
   klaunch(t, KC_RESTRICT, l, w,
-          z.pnnz * (w + (d.Rs.compact ? 2.0 : 4.0)) + z.rows * w + z.nc * w * (coarse_next ? 1.0 : 4.0),
+          z.pnnz * (wt + (d.Rs.compact ? 2.0 : 4.0)) + z.rows * w +
+              z.nc * w * (coarse_next ? 1.0 : 4.0),
           k_restrict<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK), 0, s, Rs, d.rs_lg, dn.A.nrows,
           (const T*)rl, nn.b.template as<T>(), (const T*)level_invd(t, h, l + 1),
           nn.d.template as<T>(), nn.x.template as<T>(), (T)chs[l + 1].c0, coarse_next ? 0 : 1,
           (const int*)done);
   SellD<T> P = d.P.view<T>(n.pval.template as<T>());
+  if (b16) P.v16 = n.p16.template as<uint16_t>();
   if (coarse_next && dn.A.nrows <= COARSE_FOLD_MAX) {
     // the dense coarsest solve is folded into the prolongation
     klaunch(t, KC_PROLONG_COARSE, l, w,
-            z.pnnz * (w + z.pcol) + z.nc * z.nc * 8.0 + z.nc * w + 2.0 * z.rows * w,
+            z.pnnz * (wt + z.pcol) + z.nc * z.nc * 8.0 + z.nc * w + 2.0 * z.rows * w,
             k_prolong_coarse<T>, dim3(g), dim3(BLK), 0, s, P, (int)dn.A.nrows,
             (const double*)h.W.template as<double>(), (const T*)nn.b.template as<T>(), cur,
             (const int*)done);
@@ -821,7 +875,7 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     T* xc = nullptr;
     vcycle<T>(t, h, s, l + 1, nn.b.template as<T>(), nn.x.template as<T>(),
               nn.xs.template as<T>(), &xc, chs, done, false);
-    klaunch(t, KC_PROLONG, l, w, z.pnnz * (w + z.pcol) + z.nc * w + 2.0 * z.rows * w,
+    klaunch(t, KC_PROLONG, l, w, z.pnnz * (wt + z.pcol) + z.nc * w + 2.0 * z.rows * w,
             k_prolong<T>, dim3(g), dim3(BLK), 0, s, P, (const T*)xc, cur, (const int*)done);
   }
   for (int k = 0; k < m; k++) {
@@ -829,9 +883,10 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     double c2 = k == 0 ? ch.c0 : ch.c2[k];
     int fuse = (fuse_dot_at_fine && k == m - 1) ? 1 : 0;
     bool wd = k != m - 1;
-    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, k != 0, wd), cheb_kernel<T>(l), dim3(gs),
-            dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
-            wd ? dd : (T*)nullptr, invd, (T)c1, (T)c2, (const int*)done, fuse, bm_ptr(t), part);
+    klaunch(t, KC_CHEB, l, w, cheb_bytes(t, l, w, k != 0, wd, a16), cheb_kernel<T>(l),
+            dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, oth, bl, (const T*)dd,
+            wd ? dd : (T*)nullptr, invd, dg, (T)c1, (T)c2, (const int*)done, fuse, bm_ptr(t),
+            part);
     if (fuse) finish(t, s, FIN_RZ, g);
     std::swap(cur, oth);
   }
@@ -921,8 +976,8 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
     // everything the captured launches take as arguments and can differ
     // between calls: types, the smoother's coefficients, the buffers
     char buf[160];
-    std::snprintf(buf, sizeof(buf), "%zu/%zu/%p/%p/%p/%p/%p/", sizeof(TK), sizeof(TP), (void*)z,
-                  (const void*)vb, (void*)dd, (void*)A.val, (void*)x);
+    std::snprintf(buf, sizeof(buf), "%zu/%zu/%d/%p/%p/%p/%p/%p/", sizeof(TK), sizeof(TP),
+                  (int)t->vbf16, (void*)z, (const void*)vb, (void*)dd, (void*)A.val, (void*)x);
     gkey = buf;
     auto num = [&](double v) {  // exact (hexadecimal) text of a coefficient
       std::snprintf(buf, sizeof(buf), "%a,", v);
@@ -1385,6 +1440,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int64_t nf = t->H.nf;
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
   t->use_graphs = (o->flags & DPF_FLAG_GRAPH) != 0;
+  t->vbf16 = (o->flags & DPF_FLAG_FP32_STORAGE) == 0;
   const LvlSize z0s = lvl_size(t, 0);
   t->sweep_ms = 0;
   t->sweep_n = 0;
@@ -2039,8 +2095,8 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
       // the fused r.z partials
       hipLaunchKernelGGL((k_cheb<T, true>), dim3(g), dim3(BLK), 0, s, A, 0, nf, (const T*)xa, xb,
                          (const T*)h.cr.template as<T>(), (const T*)nullptr, (T*)nullptr,
-                         (const T*)level_invd(t, h, 0), (T)0, (T)ch.c0, nodone, 1, bm_ptr(t),
-                         part);
+                         (const T*)level_invd(t, h, 0), (const T*)nullptr, (T)0, (T)ch.c0, nodone,
+                         1, bm_ptr(t), part);
     } else if (kernel == 1) {
       hipLaunchKernelGGL((k_cg_pq<T, T>), dim3(g), dim3(BLK), 0, s, A, (const T*)xa,
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
@@ -2049,7 +2105,8 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
       hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
                          d0.A.view<double>(t->A64.as<double>()), 0, nf,
                          (const double*)t->x64.as<double>(),
-                         (const double*)t->b64.as<double>(), t->r64.as<double>(), nodone);
+                         (const double*)t->b64.as<double>(), (const double*)nullptr,
+                         t->r64.as<double>(), nodone);
     }
   };
   launch(0);  // warm

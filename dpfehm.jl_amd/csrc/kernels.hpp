@@ -83,7 +83,21 @@ struct SellD {
   const int32_t* soff = nullptr;
   const uint16_t* col16 = nullptr;
   const int32_t* cbase = nullptr;
+  // bf16 values (the upper 16 bits of the FP32 value), read in place of val
+  // when set (the FP32 V-cycle's transfer and coarse split operators)
+  const uint16_t* v16 = nullptr;
 };
+
+// bf16 <-> FP32: a bf16 value is the upper half of an FP32 one; rounding to
+// nearest, ties to even (finite values)
+__device__ __forceinline__ float bf16_to_f32(uint16_t u) {
+  return __uint_as_float((uint32_t)u << 16);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
 
 // The offsets of row r's slice if it is an offset slice, else null.  Every
 // lane of a wave is in the same slice, so the lookup is a scalar load.
@@ -723,6 +737,50 @@ __global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
   rval[e] = s >= 0 ? pval[s] : (T)0;
 }
 
+// The bf16 copy of a level operator (one row per lane): off-diagonal values
+// rounded to bf16, the diagonal slot 0, and dg_i = a_ii + sum_j (a_ij -
+// bf16(a_ij)) over the off-diagonals, so that the copy's rows (with dg as
+// their diagonal) sum exactly as the operator's do: the operator's near-null
+// space, and with it positive definiteness, survive the rounding.
+__global__ void k_level_bf16(SellD<float> A, const uint16_t* __restrict__ diag_slot,
+                             uint16_t* __restrict__ a16, float* __restrict__ dg) {
+  int64_t i = logical_row();
+  if (i >= A.nrows) return;
+  int64_t base;
+  int w;
+  sell_row(A, i, base, w);
+  const int ds = diag_slot[i];
+  float err = 0.0f, dv = 0.0f;
+  for (int k = 0; k < w; k++) {
+    int64_t e = base + (int64_t)k * 64;
+    float v = A.val[e];
+    if (k == ds) {
+      dv = v;
+      a16[e] = 0;
+    } else {
+      uint16_t u = f32_to_bf16(v);
+      a16[e] = u;
+      err += v - bf16_to_f32(u);
+    }
+  }
+  dg[i] = dv + err;
+}
+
+// out[e] = a16[src[e]] (0 where src < 0): the row-split copy of a bf16 operator
+__global__ void k_gather_bf16(int64_t nent, const int32_t* __restrict__ src,
+                              const uint16_t* __restrict__ a16, uint16_t* __restrict__ out) {
+  int64_t e = logical_row();
+  if (e >= nent) return;
+  int32_t sidx = src[e];
+  out[e] = sidx >= 0 ? a16[sidx] : (uint16_t)0;
+}
+
+// out = bf16(in)
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ in, uint16_t* __restrict__ out) {
+  int64_t e = logical_row();
+  if (e < n) out[e] = f32_to_bf16(in[e]);
+}
+
 // Galerkin products grouped by destination: list k holds the contributions
 // of destination dest[k] (k itself when dest is null):
 //   out[dest[k]] = sum_{q in [off[k], off[k+1])} X[s1[q]] * Y[s2[q]].
@@ -913,16 +971,23 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
 // back to back.  (Entry loads guarded by per-entry conditions make the
 // compiler wait for each load at the join of its branch, which serializes
 // the group.)
-template <int N, typename T, typename TX>
-__device__ __forceinline__ T dot_group(const SellD<T>& A, int64_t e0,
+template <int N, bool C16, bool B16, typename T, typename TX>
+__device__ __forceinline__ T dot_group(const SellD<T>& A, int64_t e0, int32_t cb,
                                        const TX* __restrict__ x, T s) {
   T v[N];
   int32_t c[N];
   TX xv[N];
 #pragma unroll
   for (int u = 0; u < N; u++) {
-    v[u] = A.val[e0 + (int64_t)u * 64];
-    c[u] = A.col[e0 + (int64_t)u * 64];
+    const int64_t e = e0 + (int64_t)u * 64;
+    if constexpr (B16)
+      v[u] = (T)bf16_to_f32(A.v16[e]);
+    else
+      v[u] = A.val[e];
+    if constexpr (C16)
+      c[u] = cb + (int32_t)A.col16[e];  // compact: the slice's base + 16-bit offset
+    else
+      c[u] = A.col[e];
   }
 #pragma unroll
   for (int u = 0; u < N; u++) xv[u] = x[c[u]];
@@ -931,22 +996,26 @@ __device__ __forceinline__ T dot_group(const SellD<T>& A, int64_t e0,
   return s;
 }
 
-// The same with compact columns: cb + 16-bit offsets.
-template <int N, typename T, typename TX>
-__device__ __forceinline__ T dot_group16(const SellD<T>& A, int64_t e0, int32_t cb,
-                                         const TX* __restrict__ x, T s) {
-  T v[N];
-  int32_t c[N];
-  TX xv[N];
-#pragma unroll
-  for (int u = 0; u < N; u++) {
-    v[u] = A.val[e0 + (int64_t)u * 64];
-    c[u] = cb + (int32_t)A.col16[e0 + (int64_t)u * 64];
+// The row sum over explicit columns: groups of 8, then one group of the
+// remaining 1..7 entries.
+template <bool C16, bool B16, typename T, typename TX>
+__device__ __forceinline__ T explicit_dot(const SellD<T>& A, int64_t base, int w, int32_t cb,
+                                          const TX* __restrict__ x) {
+  T s = (T)0;
+  int k0 = 0;
+  for (; k0 + 8 <= w; k0 += 8)
+    s = dot_group<8, C16, B16>(A, base + (int64_t)k0 * 64, cb, x, s);
+  const int64_t e0 = base + (int64_t)k0 * 64;
+  switch (w - k0) {
+    case 1: s = dot_group<1, C16, B16>(A, e0, cb, x, s); break;
+    case 2: s = dot_group<2, C16, B16>(A, e0, cb, x, s); break;
+    case 3: s = dot_group<3, C16, B16>(A, e0, cb, x, s); break;
+    case 4: s = dot_group<4, C16, B16>(A, e0, cb, x, s); break;
+    case 5: s = dot_group<5, C16, B16>(A, e0, cb, x, s); break;
+    case 6: s = dot_group<6, C16, B16>(A, e0, cb, x, s); break;
+    case 7: s = dot_group<7, C16, B16>(A, e0, cb, x, s); break;
+    default: break;
   }
-#pragma unroll
-  for (int u = 0; u < N; u++) xv[u] = x[c[u]];
-#pragma unroll
-  for (int u = 0; u < N; u++) s += v[u] * (T)xv[u];
   return s;
 }
 
@@ -993,6 +1062,8 @@ __device__ __forceinline__ T dot_group_zp(const SellD<T>& A, int64_t e0, int32_t
 // remaining 1..7 (the width is the same for every lane of a slice, so the
 // group sizes are uniform): a lane keeps up to 8 independent gather chains in
 // flight, where one chain at a time would leave the sweeps latency-bound.
+// Columns come from col, or compact from cbase + col16; values from val, or
+// from v16 (bf16).
 // In an offset slice (fine operator) the columns come from the slice record
 // already in registers, so the row's value and x loads all issue at once.
 template <typename T, typename TX>
@@ -1014,36 +1085,11 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, const SellRow& R, int64
       default: return dot_group_off<8>(A, base, rr, R.o, x, s);
     }
   }
-  int k0 = 0;
-  if (A.col16) {
-    const int32_t cb = R.cb;
-    for (; k0 + 8 <= w; k0 += 8) s = dot_group16<8>(A, base + (int64_t)k0 * 64, cb, x, s);
-    const int64_t e0 = base + (int64_t)k0 * 64;
-    switch (w - k0) {
-      case 1: s = dot_group16<1>(A, e0, cb, x, s); break;
-      case 2: s = dot_group16<2>(A, e0, cb, x, s); break;
-      case 3: s = dot_group16<3>(A, e0, cb, x, s); break;
-      case 4: s = dot_group16<4>(A, e0, cb, x, s); break;
-      case 5: s = dot_group16<5>(A, e0, cb, x, s); break;
-      case 6: s = dot_group16<6>(A, e0, cb, x, s); break;
-      case 7: s = dot_group16<7>(A, e0, cb, x, s); break;
-      default: break;
-    }
-    return s;
-  }
-  for (; k0 + 8 <= w; k0 += 8) s = dot_group<8>(A, base + (int64_t)k0 * 64, x, s);
-  const int64_t e0 = base + (int64_t)k0 * 64;
-  switch (w - k0) {
-    case 1: s = dot_group<1>(A, e0, x, s); break;
-    case 2: s = dot_group<2>(A, e0, x, s); break;
-    case 3: s = dot_group<3>(A, e0, x, s); break;
-    case 4: s = dot_group<4>(A, e0, x, s); break;
-    case 5: s = dot_group<5>(A, e0, x, s); break;
-    case 6: s = dot_group<6>(A, e0, x, s); break;
-    case 7: s = dot_group<7>(A, e0, x, s); break;
-    default: break;
-  }
-  return s;
+  if (A.col16)
+    return A.v16 ? explicit_dot<true, true>(A, base, w, R.cb, x)
+                 : explicit_dot<true, false>(A, base, w, R.cb, x);
+  return A.v16 ? explicit_dot<false, true>(A, base, w, R.cb, x)
+               : explicit_dot<false, false>(A, base, w, R.cb, x);
 }
 
 // The same row sum for a caller that has only the row's start and width.
@@ -1125,11 +1171,14 @@ __device__ __forceinline__ T split_row_dot(const SellD<T>& A, const SellRow& R, 
 // With fuse_dot, also writes the block partials of b . xout (the fine-level
 // sweep that completes z = M^-1 r, where b is the CG residual).
 // FINE marks the finest level's instantiation (a separate symbol in profiles).
+// dg (null: none): the diagonal of a bf16 operator, kept in FP32 outside its
+// values (whose diagonal slots hold 0); the row's own lane adds dg_r x_r.
 template <typename T, bool FINE>
 __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
                        T* __restrict__ xout, const T* __restrict__ b, const T* din, T* dout,
-                       const T* __restrict__ invd, T c1, T c2, const int* __restrict__ done,
-                       int fuse_dot, const int2* __restrict__ bm, double* __restrict__ partial) {
+                       const T* __restrict__ invd, const T* __restrict__ dg, T c1, T c2,
+                       const int* __restrict__ done, int fuse_dot,
+                       const int2* __restrict__ bm, double* __restrict__ partial) {
   const int dn = *done;
   int64_t sub = logical_row();
   int64_t r = sub >> lg;
@@ -1139,8 +1188,10 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && r < n;
   T br = own ? b[r] : (T)0, ir = own ? invd[r] : (T)0, xr = own ? xin[r] : (T)0;
   T dr = own && c1 != (T)0 ? din[r] : (T)0;
+  T gr = own && dg ? dg[r] : (T)0;
   if (dn || R.w < 0) return;
   T ax = split_row_dot(A, R, sub, lg, xin);
+  if (dg) ax += gr * xr;
   double local = 0.0;
   if (own) {
     T dnew = c2 * ir * (br - ax);
@@ -1160,17 +1211,18 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
 // r = b - A x  (rows of A split over 2^lg lanes)
 template <typename T>
 __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ x,
-                           const T* __restrict__ b, T* __restrict__ r,
-                           const int* __restrict__ done) {
+                           const T* __restrict__ b, const T* __restrict__ dg,
+                           T* __restrict__ r, const int* __restrict__ done) {
   const int dn = *done;
   int64_t sub = logical_row();
   int64_t i = sub >> lg;
   const SellRow R = split_row_at(A, sub);
   const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n;
   T bi = own ? b[i] : (T)0;  // loaded with the operator, not after it
+  T gx = own && dg ? dg[i] * x[i] : (T)0;
   if (dn || R.w < 0) return;
   T ax = split_row_dot(A, R, sub, lg, x);
-  if (own) r[i] = bi - ax;
+  if (own) r[i] = bi - (ax + gx);
 }
 
 // bc = R rf with every row of R split over G = 2^lg consecutive lanes

@@ -52,8 +52,13 @@ enum { DPF_HOST = 0, DPF_DEVICE = 1 };
  * stream.  DPF_FLAG_KERNEL_TIMING times every launch of dpf_steady_forward,
  * dpf_steady_pullback and the logK maps the same way and adds the durations
  * and algorithmic bytes to the topology's table (dpf_kernel_table); launches
- * that the converged solve turned into no-ops are left out. */
-enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4 };
+ * that the converged solve turned into no-ops are left out.
+ * The FP32 V-cycle (DPF_MIXED, DPF_MIXED_PC) computes in FP32 and reads the
+ * prolongators, restrictions and coarse-level operators as bf16 values (each
+ * coarse row's rounding error folded into an FP32 diagonal, so row sums are
+ * kept); DPF_FLAG_FP32_STORAGE makes it read FP32 values instead. */
+enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4,
+       DPF_FLAG_FP32_STORAGE = 8 };
 
 typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
