@@ -544,6 +544,19 @@ void ensure_hier(dpf_topology* t, Hier<T>& h) {
       n.apval.alloc(d.ap_nnz * sizeof(T));
     }
     if (d.as_lg > 0) n.asval.alloc(d.As.entries * sizeof(T));
+    if (std::is_same<T, float>::value) {
+      // the bf16 copies the FP32 V-cycle reads: transfer operators and the
+      // coarse levels' split operators
+      if (d.transfer) {
+        n.p16.alloc(d.P.entries * sizeof(uint16_t));
+        n.r16.alloc(d.Rs.entries * sizeof(uint16_t));
+      }
+      if (l > 0 && d.as_lg > 0) {
+        n.a16.alloc(d.A.entries * sizeof(uint16_t));
+        n.dg.alloc(nr * sizeof(T));
+      }
+      if (l > 0 && d.as_lg > 0) n.as16.alloc(d.As.entries * sizeof(uint16_t));
+    }
     n.x.alloc(nr * sizeof(T));
     n.xs.alloc(nr * sizeof(T));
     n.d.alloc(nr * sizeof(T));
@@ -581,6 +594,9 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
   LevelDev& L0 = *t->lev[0];
   float* v32 = nullptr;
   float* i32 = nullptr;
+  // the finest level has no bf16 copy (a16, dg stay null)
+  uint16_t* a16 = nullptr;
+  float* dg = nullptr;
   if (want32) {
     ensure_hier(t, t->h32);
     v32 = t->h32.lev[0].val.as<float>();
@@ -600,40 +616,8 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
                      L0.diag_slot.as<uint16_t>(), t->Ke.as<double>(), t->aol.as<double>(),
                      t->dh.as<double>(), Q, tscale, svec,
                      L0.A.view<double>(t->A64.as<double>()), v32, t->invd64.as<double>(), i32,
-                     want_b ? t->b64.as<double>() : (double*)nullptr);
+                     want_b ? t->b64.as<double>() : (double*)nullptr, a16, dg);
   HIPTRY(hipGetLastError());
-}
-
-// bf16 copies the FP32 V-cycle reads: every prolongator and restriction,
-// and every coarse level's split operator (with its compensated diagonal).
-void make_bf16(dpf_topology* t, Hier<float>& h, hipStream_t s) {
-  int L = t->levels();
-  for (int l = 0; l < L; l++) {
-    LevelDev& d = *t->lev[l];
-    LevelNum<float>& n = h.lev[l];
-    if (d.transfer) {
-      n.p16.alloc(d.P.entries * sizeof(uint16_t));
-      n.r16.alloc(d.Rs.entries * sizeof(uint16_t));
-      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.P.entries, k_to_bf16, dim3(nblocks(d.P.entries)),
-              dim3(BLK), 0, s, d.P.entries, (const float*)n.pval.as<float>(),
-              n.p16.as<uint16_t>());
-      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.Rs.entries, k_to_bf16, dim3(nblocks(d.Rs.entries)),
-              dim3(BLK), 0, s, d.Rs.entries, (const float*)n.rval.as<float>(),
-              n.r16.as<uint16_t>());
-    }
-    if (l > 0 && d.as_lg > 0) {
-      n.a16.alloc(d.A.entries * sizeof(uint16_t));
-      n.as16.alloc(d.As.entries * sizeof(uint16_t));
-      n.dg.alloc(d.A.nrows * sizeof(float));
-      klaunch(t, KC_RGATHER, l, 4, 6.0 * d.A.entries + 6.0 * d.A.nrows, k_level_bf16,
-              dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, d.A.view<float>(level_val(t, h, l)),
-              d.diag_slot.as<uint16_t>(), n.a16.as<uint16_t>(), n.dg.as<float>());
-      klaunch(t, KC_RGATHER, l, 4, 8.0 * d.As.entries, k_gather_bf16,
-              dim3(nblocks(d.As.entries)), dim3(BLK), 0, s, d.As.entries,
-              d.as_src.as<int32_t>(), (const uint16_t*)n.a16.as<uint16_t>(),
-              n.as16.as<uint16_t>());
-    }
-  }
 }
 
 // Numeric refresh of levels 1.. and the coarsest inverse, from level 0 values.
@@ -649,15 +633,19 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
     T* apv = h.lev[l].apval.template as<T>();
     T* cval = level_val(t, h, l + 1);
     const double w = sizeof(T);
+    const double b16w = std::is_same<T, float>::value ? 1.0 : 0.0;  // bf16 copies written
     const LevelHost& LH = t->H.levels[l];
     const LvlSize z = lvl_size(t, l), zn = lvl_size(t, l + 1);
     const double nap = (double)LH.ap_off.back(), nrap = (double)LH.rap_off.back();
-    klaunch(t, KC_PSMOOTH, l, (int)w, z.nnz * (w + 2.0) + z.rows * (w + 2.0) + z.pnnz * w,
+    klaunch(t, KC_PSMOOTH, l, (int)w,
+            z.nnz * (w + 2.0) + z.rows * (w + 2.0) + z.pnnz * (w + 2.0 * b16w),
             k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
-            (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(), d.p_self.as<uint16_t>(), P);
-    klaunch(t, KC_RGATHER, l, (int)w, z.pnnz * (4.0 + 2.0 * w), k_rgather<T>,
+            (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(), d.p_self.as<uint16_t>(), P,
+            h.lev[l].p16.template as<uint16_t>());
+    klaunch(t, KC_RGATHER, l, (int)w, z.pnnz * (4.0 + 2.0 * w + 2.0 * b16w), k_rgather<T>,
             dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries, d.rs_src.as<int32_t>(),
-            (const T*)P.val, h.lev[l].rval.template as<T>());
+            (const T*)P.val, h.lev[l].rval.template as<T>(), (const uint16_t*)nullptr,
+            h.lev[l].r16.template as<uint16_t>());
     klaunch(t, KC_PROD_AP, l, (int)w, nap * (8.0 + 2.0 * w) + (double)d.ap_nnz * (8.0 + w),
             k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s, d.ap_nnz,
             d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr, d.ap_a.as<int32_t>(),
@@ -670,13 +658,17 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
       klaunch(t, KC_MIRROR, l + 1, (int)w, (double)d.n_mirror * (8.0 + 2.0 * w),
               k_mirror_copy<T>, dim3(nblocks(d.n_mirror)), dim3(BLK), 0, s, d.n_mirror,
               d.rap_mdst.as<int32_t>(), d.rap_msrc.as<int32_t>(), cval);
-    klaunch(t, KC_ROWDIAG, l + 1, (int)w, zn.nnz * w + zn.rows * (2.0 + w), k_rowdiag<T>,
+    klaunch(t, KC_ROWDIAG, l + 1, (int)w,
+            zn.nnz * (w + 2.0 * b16w) + zn.rows * (2.0 + w + w * b16w), k_rowdiag<T>,
             dim3(nblocks(dn.A.nrows)), dim3(BLK), 0, s, dn.A.view<T>(cval),
-            dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1));
+            dn.diag_slot.as<uint16_t>(), level_invd(t, h, l + 1),
+            h.lev[l + 1].a16.template as<uint16_t>(), h.lev[l + 1].dg.template as<T>());
     if (dn.as_lg > 0)
-      klaunch(t, KC_RGATHER, l + 1, (int)w, zn.nnz * (4.0 + 2.0 * w), k_rgather<T>,
+      klaunch(t, KC_RGATHER, l + 1, (int)w, zn.nnz * (4.0 + 2.0 * w + 4.0 * b16w), k_rgather<T>,
               dim3(nblocks(dn.As.entries)), dim3(BLK), 0, s, dn.As.entries,
-              dn.as_src.as<int32_t>(), (const T*)cval, h.lev[l + 1].asval.template as<T>());
+              dn.as_src.as<int32_t>(), (const T*)cval, h.lev[l + 1].asval.template as<T>(),
+              (const uint16_t*)h.lev[l + 1].a16.template as<uint16_t>(),
+              h.lev[l + 1].as16.template as<uint16_t>());
   }
   if (t->H.coarse_dense) {
     LevelDev& dc = *t->lev[L - 1];
@@ -687,7 +679,6 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
             dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
             h.W.template as<double>(), h.status.template as<int>());
   }
-  if constexpr (std::is_same<T, float>::value) make_bf16(t, h, s);
   HIPTRY(hipGetLastError());
 }
 
@@ -794,6 +785,9 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   SellD<T> Asm = lg > 0 ? d.As.view<T>(n.asval.template as<T>()) : A;
   constexpr bool is32 = std::is_same<T, float>::value;
   const bool b16 = is32 && t->vbf16;
+  // (not the finest level: its FP32 values, which the CG product reads in the
+  // same iteration, stay in the Infinity Cache between the kernels that read
+  // them, where a bf16 copy would add its own traffic)
   const bool a16 = b16 && l > 0 && lg > 0;
   const T* dg = nullptr;
   if (a16) {

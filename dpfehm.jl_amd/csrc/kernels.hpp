@@ -582,7 +582,9 @@ __global__ void k_hm_pullback(int64_t n, const int64_t* __restrict__ inc_ptr,
 
 // Row j of  M = tscale * dr/dh (+ diag(svec) if given), and optionally
 // b_j = Q_i + sum over Dirichlet neighbours of K_e aol_e dh_o, for free cell
-// i = f2n[j].  Values are written in FP64 and, if val32 is given, also in FP32.
+// i = f2n[j].  Values are written in FP64 and, if val32 is given, also in FP32
+// and, if a16 is given, in the bf16 form of k_rowdiag (diagonal slot 0, the
+// FP32 diagonal plus the off-diagonals' rounding error in dg32).
 // invd = 1 / (m_jj + sum_k |m_jk|) (l1-Jacobi diagonal).
 __global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __restrict__ f2n,
                            const int64_t* __restrict__ inc_ptr,
@@ -594,7 +596,8 @@ __global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __r
                            const double* __restrict__ dh, const double* __restrict__ Q,
                            double tscale, const double* __restrict__ svec, SellD<double> A,
                            float* __restrict__ val32, double* __restrict__ invd64,
-                           float* __restrict__ invd32, double* __restrict__ b) {
+                           float* __restrict__ invd32, double* __restrict__ b,
+                           uint16_t* __restrict__ a16, float* __restrict__ dg32) {
   // rows up to AW slots accumulate in LDS ([slot][lane]) and are written once;
   // wider rows (unstructured meshes) accumulate in place
   constexpr int AW = 8;
@@ -643,11 +646,19 @@ __global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __r
   double inv = 1.0 / l1;
   if (invd64) invd64[j] = inv;
   if (invd32) invd32[j] = (float)inv;
+  float err = 0.0f;  // bf16 copy (with val32): rounding error of the off-diagonals
   for (int k = 0; k < w; k++) {
     double v = slot(k);
-    if (lds) A.val[base + (int64_t)k * 64] = v;
-    if (val32) val32[base + (int64_t)k * 64] = (float)v;
+    int64_t e = base + (int64_t)k * 64;
+    if (lds) A.val[e] = v;
+    if (val32) val32[e] = (float)v;
+    if (a16) {
+      uint16_t bv = k == ds ? (uint16_t)0 : f32_to_bf16((float)v);
+      a16[e] = bv;
+      if (k != ds) err += (float)v - bf16_to_f32(bv);
+    }
   }
+  if (a16) dg32[j] = (float)dv + err;
   if (b) b[j] = bj;
 }
 
@@ -681,10 +692,18 @@ __global__ void k_residuals(int64_t nf, const int32_t* __restrict__ f2n,
 
 // P(i,:) = e_agg(i) - w invd_i sum_k a_ik e_agg(k), w = 4/3 (l1 diagonal,
 // whose spectral radius bound is 1).
+// p16 (may be null): P's values also in bf16.
+template <typename T>
+__device__ __forceinline__ void put_val(T* val, uint16_t* v16, int64_t e, T v) {
+  val[e] = v;
+  if (v16) v16[e] = f32_to_bf16((float)v);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict__ invd,
                                                  const uint16_t* __restrict__ p_slot,
-                                                 const uint16_t* __restrict__ p_self, SellD<T> P) {
+                                                 const uint16_t* __restrict__ p_self, SellD<T> P,
+                                                 uint16_t* __restrict__ p16) {
   // the lane's P row accumulates in LDS ([slot][lane]: no bank conflicts)
   // instead of through read-modify-writes of global memory
   constexpr int PW = 16;
@@ -715,7 +734,7 @@ __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict
         if (sl[u] != NO_SLOT_D) acc[sl[u]][tid] -= f * v[u];
     }
     acc[p_self[i]][tid] += (T)1;
-    for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = acc[k][tid];
+    for (int k = 0; k < pw; k++) put_val(P.val, p16, pb + (int64_t)k * 64, acc[k][tid]);
   } else {
     for (int k = 0; k < pw; k++) P.val[pb + (int64_t)k * 64] = (T)0;
     for (int k = 0; k < aw; k++) {
@@ -724,61 +743,26 @@ __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict
       if (sl != NO_SLOT_D) P.val[pb + (int64_t)sl * 64] -= f * A.val[e];
     }
     P.val[pb + (int64_t)p_self[i] * 64] += (T)1;
+    if (p16)
+      for (int k = 0; k < pw; k++)
+        p16[pb + (int64_t)k * 64] = f32_to_bf16((float)P.val[pb + (int64_t)k * 64]);
   }
 }
 
-// R = P': every R entry copies the P entry it mirrors.
+// R = P': every R entry copies the P entry it mirrors (also: a row-split
+// copy of an operator).  out16 (may be null): the bf16 copy too, gathered
+// from src16 where given (a bf16 operator whose diagonal lives apart), else
+// rounded from the value.
 template <typename T>
 __global__ void k_rgather(int64_t nent, const int32_t* __restrict__ r_src,
-                          const T* __restrict__ pval, T* __restrict__ rval) {
+                          const T* __restrict__ pval, T* __restrict__ rval,
+                          const uint16_t* __restrict__ src16, uint16_t* __restrict__ out16) {
   int64_t e = logical_row();
   if (e >= nent) return;
   int32_t s = r_src[e];
-  rval[e] = s >= 0 ? pval[s] : (T)0;
-}
-
-// The bf16 copy of a level operator (one row per lane): off-diagonal values
-// rounded to bf16, the diagonal slot 0, and dg_i = a_ii + sum_j (a_ij -
-// bf16(a_ij)) over the off-diagonals, so that the copy's rows (with dg as
-// their diagonal) sum exactly as the operator's do: the operator's near-null
-// space, and with it positive definiteness, survive the rounding.
-__global__ void k_level_bf16(SellD<float> A, const uint16_t* __restrict__ diag_slot,
-                             uint16_t* __restrict__ a16, float* __restrict__ dg) {
-  int64_t i = logical_row();
-  if (i >= A.nrows) return;
-  int64_t base;
-  int w;
-  sell_row(A, i, base, w);
-  const int ds = diag_slot[i];
-  float err = 0.0f, dv = 0.0f;
-  for (int k = 0; k < w; k++) {
-    int64_t e = base + (int64_t)k * 64;
-    float v = A.val[e];
-    if (k == ds) {
-      dv = v;
-      a16[e] = 0;
-    } else {
-      uint16_t u = f32_to_bf16(v);
-      a16[e] = u;
-      err += v - bf16_to_f32(u);
-    }
-  }
-  dg[i] = dv + err;
-}
-
-// out[e] = a16[src[e]] (0 where src < 0): the row-split copy of a bf16 operator
-__global__ void k_gather_bf16(int64_t nent, const int32_t* __restrict__ src,
-                              const uint16_t* __restrict__ a16, uint16_t* __restrict__ out) {
-  int64_t e = logical_row();
-  if (e >= nent) return;
-  int32_t sidx = src[e];
-  out[e] = sidx >= 0 ? a16[sidx] : (uint16_t)0;
-}
-
-// out = bf16(in)
-__global__ void k_to_bf16(int64_t n, const float* __restrict__ in, uint16_t* __restrict__ out) {
-  int64_t e = logical_row();
-  if (e < n) out[e] = f32_to_bf16(in[e]);
+  T v = s >= 0 ? pval[s] : (T)0;
+  rval[e] = v;
+  if (out16) out16[e] = src16 ? (s >= 0 ? src16[s] : (uint16_t)0) : f32_to_bf16((float)v);
 }
 
 // Galerkin products grouped by destination: list k holds the contributions
@@ -837,27 +821,47 @@ __global__ void k_mirror_copy(int64_t nm, const int32_t* __restrict__ dst,
   if (m < nm) val[dst[m]] = val[src[m]];
 }
 
-// invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator
+// invd_i = 1 / (a_ii + sum_{j != i} |a_ij|) for a level operator.  With a16
+// (FP32 only), also the operator's bf16 copy: off-diagonal values rounded to
+// bf16, the diagonal slot 0, and dg_i = a_ii + sum_j (a_ij - bf16(a_ij)) over
+// the off-diagonals, so that the copy's rows (with dg as their diagonal) sum
+// exactly as the operator's do: the operator's near-null space, and with it
+// positive definiteness, survive the rounding.
 template <typename T>
 __global__ void k_rowdiag(SellD<T> A, const uint16_t* __restrict__ diag_slot,
-                          T* __restrict__ invd) {
+                          T* __restrict__ invd, uint16_t* __restrict__ a16, T* __restrict__ dg) {
   int64_t i = logical_row();
   if (i >= A.nrows) return;
   int64_t base;
   int w;
   sell_row(A, i, base, w);
   int ds = diag_slot[i];
-  T l1 = A.val[base + (int64_t)ds * 64];
+  T dv = A.val[base + (int64_t)ds * 64];
+  T l1 = dv;
+  float err = 0.0f;
   constexpr int U = 8;
   for (int k0 = 0; k0 < w; k0 += U) {
     T v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) v[u] = k0 + u < w ? A.val[base + (int64_t)(k0 + u) * 64] : (T)0;
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (k0 + u < w && k0 + u != ds) l1 += fabs(v[u]);
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= w) continue;
+      int64_t e = base + (int64_t)(k0 + u) * 64;
+      if (k0 + u != ds) {
+        l1 += fabs(v[u]);
+        if (a16) {
+          uint16_t b = f32_to_bf16((float)v[u]);
+          a16[e] = b;
+          err += (float)v[u] - bf16_to_f32(b);
+        }
+      } else if (a16) {
+        a16[e] = 0;
+      }
+    }
   }
   invd[i] = (T)1 / l1;
+  if (a16) dg[i] = (T)((float)dv + err);
 }
 
 // In-place Gauss-Jordan inverse of the (SPD) coarsest operator, in FP64, by
@@ -1020,19 +1024,39 @@ __device__ __forceinline__ T explicit_dot(const SellD<T>& A, int64_t base, int w
 }
 
 // The same over an offset slice's N slots: columns r + o[u], no column loads.
-template <int N, typename T, typename TX>
+template <int N, bool B16, typename T, typename TX>
 __device__ __forceinline__ T dot_group_off(const SellD<T>& A, int64_t e0, int32_t r,
                                            const int32_t* o, const TX* __restrict__ x, T s) {
   T v[N];
   TX xv[N];
 #pragma unroll
   for (int u = 0; u < N; u++) {
-    v[u] = A.val[e0 + (int64_t)u * 64];
+    const int64_t e = e0 + (int64_t)u * 64;
+    if constexpr (B16)
+      v[u] = (T)bf16_to_f32(A.v16[e]);
+    else
+      v[u] = A.val[e];
     xv[u] = x[r + o[u]];
   }
 #pragma unroll
   for (int u = 0; u < N; u++) s += v[u] * (T)xv[u];
   return s;
+}
+
+template <bool B16, typename T, typename TX>
+__device__ __forceinline__ T offset_dot(const SellD<T>& A, int64_t base, int w, int32_t r,
+                                        const int32_t* o, const TX* __restrict__ x) {
+  const T s = (T)0;
+  switch (w) {
+    case 1: return dot_group_off<1, B16>(A, base, r, o, x, s);
+    case 2: return dot_group_off<2, B16>(A, base, r, o, x, s);
+    case 3: return dot_group_off<3, B16>(A, base, r, o, x, s);
+    case 4: return dot_group_off<4, B16>(A, base, r, o, x, s);
+    case 5: return dot_group_off<5, B16>(A, base, r, o, x, s);
+    case 6: return dot_group_off<6, B16>(A, base, r, o, x, s);
+    case 7: return dot_group_off<7, B16>(A, base, r, o, x, s);
+    default: return dot_group_off<8, B16>(A, base, r, o, x, s);
+  }
 }
 
 // The p-forming row sum sum_u A[e_u] (z[c_u] + beta p[c_u]) over a group.
@@ -1071,20 +1095,9 @@ __device__ __forceinline__ T sell_dot(const SellD<T>& A, const SellRow& R, int64
                                       const TX* __restrict__ x) {
   const int64_t base = R.base;
   const int w = R.w;
-  T s = (T)0;
-  if (R.offs) {
-    const int32_t rr = (int32_t)r;
-    switch (w) {
-      case 1: return dot_group_off<1>(A, base, rr, R.o, x, s);
-      case 2: return dot_group_off<2>(A, base, rr, R.o, x, s);
-      case 3: return dot_group_off<3>(A, base, rr, R.o, x, s);
-      case 4: return dot_group_off<4>(A, base, rr, R.o, x, s);
-      case 5: return dot_group_off<5>(A, base, rr, R.o, x, s);
-      case 6: return dot_group_off<6>(A, base, rr, R.o, x, s);
-      case 7: return dot_group_off<7>(A, base, rr, R.o, x, s);
-      default: return dot_group_off<8>(A, base, rr, R.o, x, s);
-    }
-  }
+  if (R.offs)
+    return A.v16 ? offset_dot<true>(A, base, w, (int32_t)r, R.o, x)
+                 : offset_dot<false>(A, base, w, (int32_t)r, R.o, x);
   if (A.col16)
     return A.v16 ? explicit_dot<true, true>(A, base, w, R.cb, x)
                  : explicit_dot<true, false>(A, base, w, R.cb, x);
