@@ -244,7 +244,7 @@ def main():
     cpu = None
     if args.cpu_baseline == "auto":
         cpu = cpu_baseline(w, fields, h_obs_d.cpu().numpy(), args.cpu_seconds)
-    roof, evalroof, table = roofline_from_tables(ktab, results, args.steps)
+    roof, evalroof, tables = roofline_from_tables(ktab, results, args.steps)
 
     out = {
         "metric": "forward+adjoint evals/s @2M cells, mixed vs FP64 GPU, <=1e-8 rel head err",
@@ -286,7 +286,8 @@ def main():
         "hipgraph_iterations": bool(args.graph),
         "roofline": roof,
         "roofline_eval": evalroof,
-        "kernel_table_mixed": table,
+        "kernel_table_mixed": tables["mixed"],
+        "kernel_table_fp64": tables["fp64"],
         "kernels_back_to_back": {f"{pr}/{k}": {"us": round(v[0] * 1e3, 2),
                                            "GBps": round(v[1] / v[0] / 1e6, 1)}
                              for (pr, k), v in prof.items()},
@@ -789,13 +790,20 @@ def roofline_from_tables(ktab, results, steps):
                           "frac": round(b_eval / t_eval / 1e9 / peak, 4),
                           "kernel_busy_s": round(k_eval, 6),
                           "achieved_over_kernel_busy": round(b_eval / k_eval / 1e9, 1)}
-    table = []
-    for r in mix[:16]:
-        table.append({"kernel": kernel_label(r), "launches_per_eval": round(r["launches"] / steps, 1),
-                      "us_per_eval": round(1e3 * r["ms"] / steps, 1),
-                      "us_per_launch": round(1e3 * r["ms"] / r["launches"], 2),
-                      "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1),
-                      "frac": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9 / peak, 3)})
+    tables = {}
+    for prec in ("mixed", "fp64"):
+        rows = []
+        for r in sorted(ktab[prec], key=lambda r: -r["ms"])[:20]:
+            rows.append({"kernel": kernel_label(r),
+                         "launches_per_eval": round(r["launches"] / steps, 1),
+                         "us_per_eval": round(1e3 * r["ms"] / steps, 1),
+                         "us_per_launch": round(1e3 * r["ms"] / r["launches"], 2),
+                         "bytes_per_launch": round(r["bytes"] / r["launches"]),
+                         "traffic_per_launch": traffic_per_launch(r),
+                         "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1),
+                         "frac": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9 / peak, 3)})
+        tables[prec] = rows
+    table = tables["mixed"]
     for r in table[:12]:
         log(f"[bench] {r['kernel']:55s} {r['launches_per_eval']:6.1f}/eval "
             f"{r['us_per_eval']:8.1f} us/eval {r['us_per_launch']:7.2f} us {r['GBps']:7.1f} GB/s")
@@ -803,7 +811,7 @@ def roofline_from_tables(ktab, results, steps):
         log(f"[bench] {prec}: B_eval {e['B_eval_bytes'] / 1e9:.2f} GB, t_eval {1e3 * e['t_eval_s']:.2f} ms "
             f"-> {e['achieved']:.0f} GB/s ({e['frac']:.3f} of peak); kernel busy "
             f"{1e3 * e['kernel_busy_s']:.2f} ms")
-    return roof, evalroof, table
+    return roof, evalroof, tables
 
 
 def traffic_per_launch(rec):
