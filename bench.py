@@ -65,6 +65,12 @@ def main():
                          "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
                          "run's configuration under a profiler, whose overheads change the "
                          "autotuner's timings)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="steady2m: use the library's fixed default smoother and layout "
+                         "(no DPF_FLAG_AUTOTUNE: no lanes-per-row or degree trials)")
+    ap.add_argument("--lanes", default=None, metavar="LG1,LG2,...",
+                    help="steady2m: pin the lanes-per-row exponent of each split coarse "
+                         "level (in level order) instead of autotuning it")
     ap.add_argument("--fp32-storage", action="store_true",
                     help="steady2m: the FP32 V-cycle reads FP32 operator values instead of "
                          "bf16 (DPF_FLAG_FP32_STORAGE)")
@@ -106,6 +112,10 @@ def main():
         f, c, r = args.tuning.split(",")
         for prec in ("fp64", "mixed", "mixed_pc"):
             topo.set_tuning(prec, int(f), int(c), float(r))
+    if args.lanes:
+        split = [l for l, g in enumerate(topo.lanes()[0]) if g > 0]
+        for l, g in zip(split, args.lanes.split(",")):
+            topo.set_lanes(l, int(g))
     topo_s = time.time() - t0
     log(f"[bench] topology: nfree={topo.nfree} nnz={topo.nnz} levels={topo.levels} "
         f"coarse={topo.coarse_rows} opcx={topo.operator_complexity:.2f} "
@@ -129,7 +139,8 @@ def main():
 
     def opts(precision, timing=False):
         o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph,
-                       "inner_reltol": args.inner_reltol, "fp32_storage": args.fp32_storage},
+                       "inner_reltol": args.inner_reltol, "fp32_storage": args.fp32_storage,
+                       "autotune": not args.no_autotune},
                       memory=_lib.DPF_DEVICE, stream=sptr)
         if timing:
             o.flags = (o.flags | _lib.DPF_FLAG_KERNEL_TIMING) & ~_lib.DPF_FLAG_GRAPH
@@ -237,7 +248,10 @@ def main():
                 f"{by.value / 1e6:.1f} MB -> {by.value / ms.value / 1e6:.0f} GB/s")
 
     tuned = topo.tuned_degrees()
-    log(f"[bench] tuned Chebyshev degrees (finest, coarse): {tuned}; layout {topo.layout()}")
+    lanes, lane_trials = topo.lanes()
+    log(f"[bench] Chebyshev degrees (finest, coarse, ratio): {tuned}; layout {topo.layout()}; "
+        f"lanes per row by level {[1 << g if g else 0 for g in lanes]}; lane trials (us) "
+        f"{ {f'{l}/{1 << k}': round(v, 2) for (l, k), v in lane_trials.items()} }")
     value = args.steps / results["mixed"]["seconds"]
     fp64_rate = args.steps / results["fp64"]["seconds"]
 
@@ -283,6 +297,9 @@ def main():
             k: round(results[k]["stats"][-1][0]["setup_ms"], 3) for k in ("fp64", "mixed")},
         "cheb_tuned_finest_coarse_ratio": tuned,
         "cheb_pinned": bool(args.tuning),
+        "autotuned": not args.no_autotune,
+        "lanes_per_row_by_level": [1 << g if g else 0 for g in lanes],
+        "lane_trials_us": {f"{l}/{1 << k}": round(v, 2) for (l, k), v in lane_trials.items()},
         "hipgraph_iterations": bool(args.graph),
         "roofline": roof,
         "roofline_eval": evalroof,
@@ -342,7 +359,8 @@ def bench_ensemble(args, P, api, _lib, torch):
 
     res = {}
     for precision in ("fp64", "mixed", "mixed_pc"):
-        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph},
+        o = api._opts({"precision": precision, "reltol": 1e-10, "graph": args.graph,
+                       "autotune": not args.no_autotune},
                       memory=_lib.DPF_DEVICE, stream=sptr)
         for _ in range(args.warmup):
             evaluate(o)

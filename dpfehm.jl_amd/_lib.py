@@ -16,6 +16,7 @@ DPF_FLAG_SWEEP_TIMING = 1
 DPF_FLAG_GRAPH = 2
 DPF_FLAG_KERNEL_TIMING = 4
 DPF_FLAG_FP32_STORAGE = 8
+DPF_FLAG_AUTOTUNE = 16
 
 _ERRNAMES = {1: "EINVAL", 2: "EHIP", 3: "ENOCONV", 4: "EBREAKDOWN", 5: "ESINGULAR", 6: "ENOMEM"}
 
@@ -28,6 +29,12 @@ class DpfError(RuntimeError):
 
 class NotConverged(DpfError):
     pass
+
+
+class SingularSystem(DpfError):
+    """DPF_ESINGULAR: a free cell without edges, or (steady solves) a
+    connected component with no Dirichlet cell, through the edges of nonzero
+    conductance."""
 
 
 class Opts(ctypes.Structure):
@@ -90,6 +97,8 @@ SIGNATURES = {
     "dpf_topology_info": (ctypes.c_int, [_p, _p, _i32]),
     "dpf_topology_tuning": (ctypes.c_int, [_p, _i32, _p, _p, _i32]),
     "dpf_topology_set_tuning": (ctypes.c_int, [_p, _i32, _i32, _i32, ctypes.c_double]),
+    "dpf_topology_lanes": (ctypes.c_int, [_p, _p, _p, _i32]),
+    "dpf_topology_set_lanes": (ctypes.c_int, [_p, _i32, _i32]),
     "dpf_twophase_tpfa": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "dpf_twophase_upstream": (ctypes.c_int, [_p, _p, _p, _p, ctypes.c_double, _p, _p, _p, _p,
                                              _p]),
@@ -132,6 +141,7 @@ SIGNATURES = {
     "dpf_inspect_levels": (_i32, [_p]),
     "dpf_inspect_size": (_i64, [_p, _i32, ctypes.c_char_p]),
     "dpf_inspect_get": (ctypes.c_int, [_p, _i32, ctypes.c_char_p, _p, _i64]),
+    "dpf_inspect_check_steady": (ctypes.c_int, [_p, _p]),
 }
 
 _lib = None
@@ -156,6 +166,8 @@ def check(code):
         msg = lib().dpf_last_error().decode(errors="replace")
         if code == DPF_ENOCONV:
             raise NotConverged(code, msg)
+        if code == DPF_ESINGULAR:
+            raise SingularSystem(code, msg)
         raise DpfError(code, msg)
 
 

@@ -21,12 +21,17 @@ recomputes it in every call.
 Keyword arguments: ``reltol`` (default 1e-10), ``abstol``, ``maxiter``,
 ``precision`` ("fp64", "mixed" or "mixed_pc"), ``inner_reltol``, ``max_outer``,
 ``cheb_degree``, ``graph``, ``fp32_storage`` (the FP32 V-cycle reads FP32
-instead of bf16 operator values).  ``linear_solver`` selects amg_solver (the
+instead of bf16 operator values), ``autotune`` (the topology's first solve in
+a precision times smoother degrees and coarse-level lanes per row and keeps
+the fastest; without it the defaults or pinned values apply, and results do
+not depend on timing).  ``linear_solver`` selects amg_solver (the
 default: AMG-PCG to ``reltol``) or cholesky_solver (the same AMG-PCG to a
 relative residual of 1e-14, a direct solve's accuracy; tolerance keywords
 ignored), as a function or by name; the reference's rrule always uses AMG
 (src/groundwater.jl:90), and here the rrule honours the choice.  Unknown
-keywords raise TypeError.  Non-convergence raises ``NotConverged`` instead of returning
+keywords raise TypeError.  A steady system with a component that no edge of nonzero
+conductance connects to a Dirichlet cell (e.g. ``dirichletnodes=[]``) raises
+``SingularSystem``.  Non-convergence raises ``NotConverged`` instead of returning
 silently.
 """
 import ctypes
@@ -35,7 +40,7 @@ import threading
 import numpy as np
 
 from . import _lib
-from ._lib import DpfError, NotConverged  # noqa: F401
+from ._lib import DpfError, NotConverged, SingularSystem  # noqa: F401
 
 try:
     import xxhash
@@ -86,7 +91,7 @@ def cholesky_solver(A=None, b=None, **kwargs):
 def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
     """dpf_opts from keyword arguments; ``graph=True`` sets DPF_FLAG_GRAPH
     (each PCG iteration replayed as one captured hipGraph), ``fp32_storage=True``
-    DPF_FLAG_FP32_STORAGE; ``linear_solver``
+    DPF_FLAG_FP32_STORAGE, ``autotune=True`` DPF_FLAG_AUTOTUNE; ``linear_solver``
     is amg_solver (default) or cholesky_solver, as a function or by name."""
     kw = dict(kwargs)
     ls = kw.pop("linear_solver", None)
@@ -97,6 +102,7 @@ def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
         kw["reltol"], kw["abstol"] = 1e-14, 0.0
     graph = bool(kw.pop("graph", False))
     fp32_storage = bool(kw.pop("fp32_storage", False))
+    autotune = bool(kw.pop("autotune", False))
     bad = set(kw) - _OPT_KEYS
     if bad:
         raise TypeError(f"unexpected keyword argument(s): {sorted(bad)}")
@@ -105,6 +111,8 @@ def _opts(kwargs, memory=_lib.DPF_HOST, stream=None):
         o.flags |= _lib.DPF_FLAG_GRAPH
     if fp32_storage:
         o.flags |= _lib.DPF_FLAG_FP32_STORAGE
+    if autotune:
+        o.flags |= _lib.DPF_FLAG_AUTOTUNE
     o.memory = memory
     o.stream = stream
     return o
@@ -156,17 +164,28 @@ class Topology:
         return {"fine_offset_slices": int(self.info()[12])}
 
     def tuned_degrees(self):
-        """{precision: (finest degree, coarse degree, interval ratio)} for the
-        precisions tuned so far."""
+        """{precision: (finest degree, coarse degree, interval ratio)} in use
+        (defaults, pinned or autotuned)."""
         i = self.info()
-        out = {}
-        if i[8]:
-            out["fp64"] = (int(i[8]), int(i[9]), int(i[15]))
-        if i[10]:
-            out["mixed"] = (int(i[10]), int(i[11]), int(i[16]))
-        if i[13]:
-            out["mixed_pc"] = (int(i[13]), int(i[14]), int(i[17]))
-        return out
+        return {"fp64": (int(i[8]), int(i[9]), int(i[15])),
+                "mixed": (int(i[10]), int(i[11]), int(i[16])),
+                "mixed_pc": (int(i[13]), int(i[14]), int(i[17]))}
+
+    def lanes(self):
+        """(lanes exponent per level, {(level, exponent): sweep us}): each
+        level's split copy has 2^lg lanes a row (0: no split copy); the
+        autotuner's sweep timings where it ran."""
+        L = self.levels
+        lg = np.zeros(L, dtype=np.int32)
+        us = np.zeros(7 * L)
+        _lib.check(_lib.lib().dpf_topology_lanes(self._h, _ptr(lg), _ptr(us), L))
+        trials = {(l, k): float(us[7 * l + k]) for l in range(L) for k in range(7)
+                  if us[7 * l + k] > 0}
+        return [int(v) for v in lg], trials
+
+    def set_lanes(self, level, lg):
+        """Pin level ``level``'s split copy at 2^lg lanes a row."""
+        _lib.check(_lib.lib().dpf_topology_set_lanes(self._h, int(level), int(lg)))
 
     def tuning(self, precision="fp64"):
         """The autotuner's trials for ``precision``: [(ms, iterations)] for

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -155,6 +156,7 @@ struct SellDev {
     nrows = h.nrows;
     ncols = h.ncols;
     entries = h.entries();
+    compact = false;
     upload(sptr, h.sptr);
     upload(col, h.col);
     upload(rowlen, h.rowlen);
@@ -358,11 +360,17 @@ struct dpf_topology {
   // Chebyshev interval [1/ratio, 1] of D_l1^-1 A: default, and tuned per precision
   double cheb_ratio = 15.0;
   double ratio_tuned[3] = {15.0, 15.0, 15.0};
-  // Chebyshev degrees of the finest / coarser levels per precision (FP64,
-  // FP32), default 2/2 until tuned, and the tuner's record
-  // (index = precision: FP64, mixed refinement, FP64 Krylov with FP32 V-cycle)
-  int deg_fine[3] = {2, 2, 2}, deg_coarse[3] = {2, 2, 2};
+  // Chebyshev degrees of the finest / coarser levels per precision (index =
+  // precision: FP64, mixed refinement, FP64 Krylov with FP32 V-cycle): the
+  // fixed default 1/2 unless autotuned (DPF_FLAG_AUTOTUNE) or pinned, and
+  // the tuner's record
+  int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {2, 2, 2};
   bool tuned[3] = {false, false, false};
+  // lanes per row of the coarse levels' split copies: the autotuner's sweep
+  // times per level and lane exponent 0..6 (0 = not timed), and whether the
+  // layout was tuned or pinned (either fixes it for the topology's life)
+  std::vector<std::array<double, 7>> lane_us;
+  bool lanes_fixed = false;
   double tune_ms[3][7] = {};  // autotuner trials: 5 degree pairs, then 2 intervals
   int tune_iters[3][7] = {};
   // optional event timing (opts->flags): every finest-level smoothing sweep
@@ -616,7 +624,8 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
                      L0.diag_slot.as<uint16_t>(), t->Ke.as<double>(), t->aol.as<double>(),
                      t->dh.as<double>(), Q, tscale, svec,
                      L0.A.view<double>(t->A64.as<double>()), v32, t->invd64.as<double>(), i32,
-                     want_b ? t->b64.as<double>() : (double*)nullptr, a16, dg);
+                     want_b ? t->b64.as<double>() : (double*)nullptr, a16, dg,
+                     &t->hm_dev->cut);
   HIPTRY(hipGetLastError());
 }
 
@@ -716,6 +725,7 @@ void setup_members(dpf_topology* t) {
 
 
 int mirror_load(const int* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+void mirror_store(int* p, int v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
 double mirror_load(const double* p) {
   double v;
   __atomic_load(p, &v, __ATOMIC_ACQUIRE);
@@ -885,6 +895,115 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
     std::swap(cur, oth);
   }
   *xout = cur;
+}
+
+// Refills level l's split-copy values in hierarchy h from the level's
+// operator, as the numeric refresh does (after the copy was re-laid).
+template <typename T>
+void refill_split(dpf_topology* t, Hier<T>& h, int l, hipStream_t s) {
+  if (!h.allocated) return;
+  LevelDev& d = *t->lev[l];
+  LevelNum<T>& n = h.lev[l];
+  constexpr bool is32 = std::is_same<T, float>::value;
+  n.asval.alloc(d.As.entries * sizeof(T));
+  if (is32) n.as16.alloc(d.As.entries * sizeof(uint16_t));
+  const double w = sizeof(T);
+  const LvlSize z = lvl_size(t, l);
+  klaunch(t, KC_RGATHER, l, (int)w, z.nnz * (4.0 + 2.0 * w + (is32 ? 4.0 : 0.0)), k_rgather<T>,
+          dim3(nblocks(d.As.entries)), dim3(BLK), 0, s, d.As.entries, d.as_src.as<int32_t>(),
+          (const T*)level_val(t, h, l), n.asval.template as<T>(),
+          (const uint16_t*)n.a16.template as<uint16_t>(), n.as16.template as<uint16_t>());
+}
+
+// Re-lays level l's split copy with 2^lg lanes a row (1 <= lg <= 6): host
+// split, compact upload, values refilled in every allocated hierarchy.
+// Captured iteration graphs hold the old buffers and are dropped.
+void set_level_lanes(dpf_topology* t, int l, int lg, hipStream_t s) {
+  LevelHost& lh = t->H.levels[l];
+  LevelDev& d = *t->lev[l];
+  if (d.as_lg == lg) return;
+  // work queued by earlier calls, on any stream, may still read the old copy
+  HIPTRY(hipDeviceSynchronize());
+  int got = 0;
+  split_rows(lh.A, got, lh.As, lh.as_src, lg);
+  lh.as_lg = got;
+  d.As.up_compact(lh.As);
+  upload(d.as_src, lh.as_src);
+  d.as_lg = got;
+  refill_split(t, t->h64, l, s);
+  refill_split(t, t->h32, l, s);
+  for (auto& kv : t->iter_graphs) (void)hipGraphExecDestroy(kv.second);
+  t->iter_graphs.clear();
+}
+
+// Mean time in microseconds of one level-l smoothing sweep in the V-cycle's
+// configuration for precision T (d read, not written), over reps launches
+// back to back after one untimed launch, on zeroed level vectors.
+template <typename T>
+double time_level_sweep(dpf_topology* t, Hier<T>& h, int l, hipStream_t s, int reps) {
+  LevelDev& d = *t->lev[l];
+  LevelNum<T>& n = h.lev[l];
+  constexpr bool is32 = std::is_same<T, float>::value;
+  const bool a16 = is32 && t->vbf16 && d.as_lg > 0;
+  SellD<T> Asm = d.As.view<T>(n.asval.template as<T>());
+  const T* dg = nullptr;
+  if (a16) {
+    Asm.v16 = n.as16.template as<uint16_t>();
+    dg = n.dg.template as<T>();
+  }
+  const int64_t nr = d.A.nrows;
+  for (DBuf* v : {&n.x, &n.xs, &n.b, &n.d}) HIPTRY(hipMemsetAsync(v->p, 0, nr * sizeof(T), s));
+  const unsigned gs = nblocks(Asm.nrows);
+  auto kern = &k_cheb<T, false>;
+  auto launch = [&]() {
+    hipLaunchKernelGGL(kern, dim3(gs), dim3(BLK), 0, s, Asm, d.as_lg, nr,
+                       (const T*)n.x.template as<T>(), n.xs.template as<T>(),
+                       (const T*)n.b.template as<T>(), (const T*)n.d.template as<T>(),
+                       (T*)nullptr, (const T*)level_invd(t, h, l), dg, (T)0.5, (T)0.5,
+                       (const int*)t->zero_flag.as<int>(), 0, bm_ptr(t),
+                       t->partial.as<double>());
+  };
+  hipEvent_t e0, e1;
+  HIPTRY(hipEventCreate(&e0));
+  HIPTRY(hipEventCreate(&e1));
+  launch();
+  HIPTRY(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; k++) launch();
+  HIPTRY(hipEventRecord(e1, s));
+  HIPTRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPTRY(hipGetLastError());
+  return 1e3 * (double)ms / reps;
+}
+
+// Lanes-per-row autotuner: for every coarse level with a split copy, time its
+// smoothing sweep with 2, 4, ..., 32 lanes a row (on the hierarchy h the
+// V-cycle runs in, after its numeric refresh) and keep the fastest layout.
+template <typename T>
+void tune_lanes(dpf_topology* t, Hier<T>& h, hipStream_t s) {
+  const int L = t->levels();
+  t->lane_us.assign(L, std::array<double, 7>{});
+  for (int l = 1; l < L; l++) {
+    if (t->lev[l]->as_lg <= 0) continue;
+    if (l == L - 1 && t->H.coarse_dense) continue;  // solved densely, never swept
+    int best = t->lev[l]->as_lg;
+    double best_us = 1e300;
+    for (int lg = 1; lg <= 5; lg++) {
+      set_level_lanes(t, l, lg, s);
+      double us = time_level_sweep(t, h, l, s, 20);
+      t->lane_us[l][lg] = us;
+      if (us < best_us) {
+        best_us = us;
+        best = lg;
+      }
+    }
+    set_level_lanes(t, l, best, s);
+  }
+  HIPTRY(hipStreamSynchronize(s));
+  t->lanes_fixed = true;
 }
 
 struct SolveResult {
@@ -1373,6 +1492,45 @@ int dpf_topology_set_tuning(dpf_topology* t, int32_t precision, int32_t deg_fine
   return DPF_OK;
 }
 
+int dpf_topology_lanes(const dpf_topology* t, int32_t* lg, double* us, int32_t nlevels) {
+  if (!t || !lg) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  const int L = t->levels();
+  for (int l = 0; l < nlevels; l++) {
+    lg[l] = l < L ? t->lev[l]->as_lg : 0;
+    if (us)
+      for (int k = 0; k < 7; k++)
+        us[7 * l + k] = (l < L && l < (int)t->lane_us.size()) ? t->lane_us[l][k] : 0.0;
+  }
+  return DPF_OK;
+}
+
+int dpf_topology_set_lanes(dpf_topology* t, int32_t level, int32_t lg) {
+  if (!t) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  const int L = t->levels();
+  if (level < 1 || level >= L || t->lev[level]->as_lg <= 0 || lg < 1 || lg > 6) {
+    set_err("need a level that has a split copy (1 <= level < levels, lanes exponent > 0) "
+            "and a lanes exponent in 1..6");
+    return DPF_EINVAL;
+  }
+  try {
+    HIPTRY(hipSetDevice(t->device));
+    set_level_lanes(t, level, lg, t->own_stream);
+    HIPTRY(hipStreamSynchronize(t->own_stream));
+  } catch (const Err& e) {
+    set_err(e.msg);
+    return e.code;
+  }
+  t->lanes_fixed = true;
+  return DPF_OK;
+}
+
 int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   if (!t || !info) {
     set_err("NULL argument");
@@ -1389,16 +1547,16 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
                   H.levels.back().A.nrows,
                   (int64_t)(1000.0 * (double)nnz_all / (double)H.levels[0].A.nnz),
                   (int64_t)(H.setup_ms * 1000.0),
-                  t->tuned[0] ? t->deg_fine[0] : 0,
-                  t->tuned[0] ? t->deg_coarse[0] : 0,
-                  t->tuned[1] ? t->deg_fine[1] : 0,
-                  t->tuned[1] ? t->deg_coarse[1] : 0,
+                  t->deg_fine[0],
+                  t->deg_coarse[0],
+                  t->deg_fine[1],
+                  t->deg_coarse[1],
                   t->fine_offset_slices,
-                  t->tuned[2] ? t->deg_fine[2] : 0,
-                  t->tuned[2] ? t->deg_coarse[2] : 0,
-                  t->tuned[0] ? (int64_t)t->ratio_tuned[0] : 0,
-                  t->tuned[1] ? (int64_t)t->ratio_tuned[1] : 0,
-                  t->tuned[2] ? (int64_t)t->ratio_tuned[2] : 0};
+                  t->deg_fine[2],
+                  t->deg_coarse[2],
+                  (int64_t)t->ratio_tuned[0],
+                  (int64_t)t->ratio_tuned[1],
+                  (int64_t)t->ratio_tuned[2]};
   for (int i = 0; i < ninfo && i < 18; i++) info[i] = v[i];
   return DPF_OK;
 }
@@ -1430,7 +1588,8 @@ double now_ms() {
 // Solve A x = b64 (device, FP64, nf) with the current hierarchy; x -> xout (device FP64).
 // Returns 0 / DPF_ENOCONV / DPF_EBREAKDOWN and fills stats.
 int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
-                 const dpf_opts* o, dpf_stats* st, const double* x0 = nullptr) {
+                 const dpf_opts* o, dpf_stats* st, const double* x0 = nullptr,
+                 bool steady_check = false) {
   int64_t nf = t->H.nf;
   t->timing = (o->flags & DPF_FLAG_SWEEP_TIMING) != 0;
   t->use_graphs = (o->flags & DPF_FLAG_GRAPH) != 0;
@@ -1446,8 +1605,11 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     return v;
   };
   // smoothers: explicit degree (every level, default interval), or the
-  // degrees and interval tuned on this topology's first solve in this precision
-  bool tune = o->cheb_degree == 0 && !t->tuned[pidx];
+  // topology's degrees and interval for this precision: the fixed default,
+  // pinned ones, or with DPF_FLAG_AUTOTUNE the ones tuned on its first such
+  // solve in this precision
+  const bool autotune = o->cheb_degree == 0 && (o->flags & DPF_FLAG_AUTOTUNE) != 0;
+  bool tune = autotune && !t->tuned[pidx];
   std::vector<Cheb> chs =
       o->cheb_degree > 0
           ? make_chs(o->cheb_degree, o->cheb_degree, t->cheb_ratio)
@@ -1505,6 +1667,14 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   klaunch(t, KC_REFINE_VEC, 0, 8, 8.0 * z0s.rows, k_zero_f64, dim3(g), dim3(BLK), 0, s, nf,
           t->x64.as<double>());
   int active = residual(1);
+  // a steady system whose assembly met a zero conductance: singular if that
+  // cut leaves a component without a Dirichlet cell (host check on K_e)
+  if (steady_check && mirror_load(&t->hm_host->cut)) {
+    std::vector<double> K(t->H.ne);
+    HIPTRY(hipMemcpy(K.data(), t->Ke.p, t->H.ne * sizeof(double), hipMemcpyDeviceToHost));
+    std::string e = floating_component(t->H, K.data());
+    if (!e.empty()) throw Err{DPF_ESINGULAR, e};
+  }
   std::vector<double> bnorm(t->nm);
   for (int m = 0; m < t->nm; m++) bnorm[m] = mirror_load(&t->hm_outer_host[m]);
   bool finite = all_finite();
@@ -1536,6 +1706,19 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     ensure_hier(t, t->h32);
     ensure_hier(t, t->h64);  // its fine-level Krylov vectors
   }
+  // autotuner, part 1: lanes per row of the coarse split copies (once per
+  // topology, as the layout serves every precision; unless pinned), timed in
+  // the precision the V-cycle runs in on the freshly refreshed hierarchy
+  if (autotune && !t->lanes_fixed) {
+    bool ktiming_on = t->ktiming, timing_on = t->timing;
+    t->ktiming = t->timing = false;
+    if (o->precision == DPF_FP64)
+      tune_lanes(t, t->h64, s);
+    else
+      tune_lanes(t, t->h32, s);
+    t->ktiming = ktiming_on;
+    t->timing = timing_on;
+  }
   // one Krylov solve of the current residual with the given smoothers (the
   // FP32 inner solve of DPF_MIXED to relative tolerance inner)
   auto krylov = [&](const std::vector<Cheb>& cc, double inner) -> SolveResult {
@@ -1563,7 +1746,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
       // very system at the default interval, then two other intervals at the
       // best pair, and keep the fastest; the result is fixed for the life of
-      // the topology, so its solves stay bitwise reproducible.  An untimed
+      // the topology, so its later solves are bitwise reproducible (another
+      // process may time differently: it repeats a tuned configuration by
+      // pinning it, dpf_topology_set_tuning and dpf_topology_set_lanes).  An untimed
       // solve first brings the kernels' code and the clocks up, and the FP32
       // trials of DPF_MIXED solve to inner_reltol^2 rather than inner_reltol:
       // the whole refinement's worth of convergence, where a cheaper V-cycle
@@ -1888,7 +2073,11 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       HIPTRY(hipMemcpyAsync(t->dh.p, dirichleths, n * sizeof(double), hipMemcpyHostToDevice, s));
       HIPTRY(hipMemcpyAsync(t->Q.p, Qs, n * sizeof(double), hipMemcpyHostToDevice, s));
     }
+    // a component without a Dirichlet cell (e.g. dirichletnodes empty)
+    // makes the steady operator singular
+    if (!t->H.floating.empty()) throw Err{DPF_ESINGULAR, t->H.floating};
     ktiming_begin(t, o);
+    mirror_store(&t->hm_host->cut, 0);
     HIPTRY(hipEventRecord(t->ev_setup[0], s));
     prepare(t, s, o->precision, t->Q.as<double>(), true);
     HIPTRY(hipEventRecord(t->ev_setup[1], s));
@@ -1903,7 +2092,7 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
       tmp.alloc(nf * sizeof(double));
       hfree = tmp.as<double>();
     }
-    int status = solve_system(t, s, t->b64.as<double>(), hfree, o, stats);
+    int status = solve_system(t, s, t->b64.as<double>(), hfree, o, stats, nullptr, true);
     double* hdev = h_out;
     if (o->memory == DPF_HOST) {
       t->stage_n2.alloc(n * sizeof(double));

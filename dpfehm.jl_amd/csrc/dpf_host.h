@@ -97,6 +97,10 @@ struct TopologyHost {
   std::vector<LevelHost> levels;    // levels[0].A has the fine pattern
   bool coarse_dense = false;        // coarsest level solved by a dense inverse
   double setup_ms = 0;              // host time of the symbolic setup
+  // non-empty when some connected component of the mesh holds no Dirichlet
+  // cell (e.g. dirichletnodes is empty): the steady operator is singular
+  // there, while a transient one (storage on the diagonal) is not
+  std::string floating;
 };
 
 // GPU layout of the fine operator.  A 64-row slice whose nonzeros all sit at
@@ -127,6 +131,12 @@ std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t
 // aggregate and each owns a contiguous row range on every level.
 std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max,
                             const std::vector<int64_t>& member_row0 = {});
-void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src);
+void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src,
+                int force_lg = -1);
+// "" when every connected component of the mesh -- through the edges with
+// conductance Ks[e] aol[e] != 0 (aol[e] != 0 when Ks is null) -- holds a Dirichlet
+// cell, else a message naming the lowest-numbered free cell of the first
+// component without one and the component's size.
+std::string floating_component(const TopologyHost& T, const double* Ks);
 
 }  // namespace dpf

@@ -141,6 +141,7 @@ std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t
     if (T.inc_ptr[i + 1] == T.inc_ptr[i])
       return "free cell " + std::to_string(i + 1) + " has no edges: the system is singular";
   }
+  T.floating = floating_component(T, nullptr);
   T.levels.assign(1, LevelHost());
   LevelHost& L0 = T.levels[0];
   L0.A.build(nf, nf, rp, ci, true);
@@ -542,13 +543,54 @@ std::string build_transfer(LevelHost& L, LevelHost& next, int64_t nc) {
 
 }  // namespace
 
+std::string floating_component(const TopologyHost& T, const double* Ks) {
+  const int64_t n = T.n;
+  std::vector<int32_t> up(n);
+  for (int64_t i = 0; i < n; i++) up[i] = (int32_t)i;
+  auto find = [&](int32_t a) {
+    while (up[a] != a) {
+      up[a] = up[up[a]];
+      a = up[a];
+    }
+    return a;
+  };
+  for (int64_t e = 0; e < T.ne; e++) {
+    // a zero conductance K_e aol_e connects nothing
+    if (T.aol[e] * (Ks ? Ks[e] : 1.0) == 0.0) continue;
+    int32_t a = find(T.edges[2 * e]), b = find(T.edges[2 * e + 1]);
+    if (a != b) up[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<uint8_t> anchored(n, 0);
+  std::vector<int64_t> size(n, 0);
+  for (int64_t i = 0; i < n; i++) {
+    int32_t r = find((int32_t)i);
+    size[r]++;
+    if (!T.isfree[i]) anchored[r] = 1;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    int32_t r = find((int32_t)i);
+    if (!anchored[r])
+      return "free cell " + std::to_string(i + 1) + " lies in a component of " +
+             std::to_string(size[r]) + " cells that " +
+             (Ks ? "no edge with K_e aol_e != 0 connects to a Dirichlet cell"
+                 : "holds no Dirichlet cell") +
+             ": the steady system is singular";
+  }
+  return "";
+}
+
 // Row-split copy of a SELL matrix: sub-row r*G + g holds the entries
-// g, g + G, ... of row r, G = 2^lg chosen so that a lane sees ~8 entries.
-void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src) {
+// g, g + G, ... of row r, G = 2^lg: force_lg when it is given (0..6), else
+// chosen so that a lane sees ~8 entries.
+void split_rows(const SellHost& A, int& lg, SellHost& S, std::vector<int32_t>& src,
+                int force_lg) {
   double avg = A.nrows > 0 ? (double)A.nnz / (double)A.nrows : 1.0;
   lg = 0;
   // ~8 entries a lane: one batch of eight loads in flight per lane
-  while ((1 << lg) * 8 < avg && lg < 6) lg++;
+  if (force_lg >= 0)
+    lg = std::min(force_lg, 6);
+  else
+    while ((1 << lg) * 8 < avg && lg < 6) lg++;
   int G = 1 << lg;
   int64_t ns = A.nrows * G;
   std::vector<int64_t> rp(ns + 1, 0);

@@ -52,6 +52,18 @@ void dpf_inspect_destroy(dpf_inspect* h) { delete h; }
 
 int32_t dpf_inspect_levels(const dpf_inspect* h) { return (int32_t)h->H.levels.size(); }
 
+// The steady solve's singularity check: 0, or 5 (DPF_ESINGULAR) with the
+// message in dpf_inspect_last_error when a connected component -- through
+// every edge, or through the edges with Ks[e] != 0 when Ks is given -- holds
+// no Dirichlet cell.
+int dpf_inspect_check_steady(const dpf_inspect* h, const double* Ks) {
+  std::string e = h->H.floating;
+  if (e.empty() && Ks) e = floating_component(h->H, Ks);
+  if (e.empty()) return 0;
+  g_ierr = e;
+  return 5;
+}
+
 }  // extern "C"
 
 namespace {

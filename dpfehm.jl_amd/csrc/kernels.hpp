@@ -33,7 +33,7 @@ struct HostMirror {
   int done;       // 0 while any member runs; else 3 if any broke down, 2 if any hit the cap, 1
   int outer_seq;  // sequence number of the refinement residuals
   int active;     // members whose refinement residual is above their target
-  int pad;
+  int cut;        // set by the assembly when an edge's conductance K_e aol_e is 0
   double rr;
 };
 
@@ -597,7 +597,8 @@ __global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __r
                            double tscale, const double* __restrict__ svec, SellD<double> A,
                            float* __restrict__ val32, double* __restrict__ invd64,
                            float* __restrict__ invd32, double* __restrict__ b,
-                           uint16_t* __restrict__ a16, float* __restrict__ dg32) {
+                           uint16_t* __restrict__ a16, float* __restrict__ dg32,
+                           int* __restrict__ cut) {
   // rows up to AW slots accumulate in LDS ([slot][lane]) and are written once;
   // wider rows (unstructured meshes) accumulate in place
   constexpr int AW = 8;
@@ -630,6 +631,9 @@ __global__ void __launch_bounds__(BLK) k_assemble(int64_t nf, const int32_t* __r
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (p0 + u >= p1) break;
+      // a zero conductance may cut a component off every Dirichlet cell:
+      // flagged for the steady solve's host check
+      if (t[u] == 0.0 && cut) mput(cut, 1);
       dsum += t[u];
       if (sl[u] != NO_SLOT_D)
         slot(sl[u]) -= tscale * t[u];

@@ -31,7 +31,10 @@ enum {
   DPF_EHIP = 2,        /* HIP runtime failure */
   DPF_ENOCONV = 3,     /* iteration cap reached before the tolerance */
   DPF_EBREAKDOWN = 4,  /* CG breakdown: matrix or preconditioner not SPD, or NaN */
-  DPF_ESINGULAR = 5,   /* structurally singular system (a free cell without edges) */
+  DPF_ESINGULAR = 5,   /* singular system: a free cell without edges, or (steady
+                          solves) a connected component that no edge of nonzero
+                          conductance K_e aol_e joins to a Dirichlet cell, e.g.
+                          dirichletnodes empty */
   DPF_ENOMEM = 6
 };
 /* DPF_FP64: FP64 AMG-PCG.  DPF_MIXED: FP32 AMG-PCG inner solves inside FP64
@@ -56,9 +59,16 @@ enum { DPF_HOST = 0, DPF_DEVICE = 1 };
  * The FP32 V-cycle (DPF_MIXED, DPF_MIXED_PC) computes in FP32 and reads the
  * prolongators, restrictions and coarse-level operators as bf16 values (each
  * coarse row's rounding error folded into an FP32 diagonal, so row sums are
- * kept); DPF_FLAG_FP32_STORAGE makes it read FP32 values instead. */
+ * kept); DPF_FLAG_FP32_STORAGE makes it read FP32 values instead.
+ * DPF_FLAG_AUTOTUNE (with cheb_degree 0): the topology's first solve in a
+ * precision times the lanes per row of the coarse split copies (once per
+ * topology) and the Chebyshev degrees and interval, and keeps the fastest
+ * for the topology's life (dpf_topology_lanes, dpf_topology_tuning report
+ * the trials).  Without it the smoother and layout are fixed defaults
+ * (degrees 1 and 2, ratio 15, ~8 entries a lane) or what was pinned, so
+ * results do not depend on timing. */
 enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4,
-       DPF_FLAG_FP32_STORAGE = 8 };
+       DPF_FLAG_FP32_STORAGE = 8, DPF_FLAG_AUTOTUNE = 16 };
 
 typedef struct {
   double reltol;       /* stop when ||b - A h|| <= max(reltol ||b||, abstol), FP64 residual */
@@ -69,9 +79,9 @@ typedef struct {
                           whose residual is within 10/inner_reltol of the target solves
                           to half the target instead */
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
-  int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = autotuned:
-                          the first solve on a topology (per precision) times
-                          several (finest, coarse) degree pairs and keeps the fastest */
+  int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = the
+                          topology's degrees: default (1 finest, 2 coarser),
+                          pinned, or autotuned with DPF_FLAG_AUTOTUNE */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
   int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
   void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = topology stream */
@@ -95,7 +105,8 @@ typedef struct {
 const char *dpf_last_error(void);
 
 /* Defaults: reltol 1e-10, abstol 0, maxiter 2000, DPF_FP64, inner_reltol 1e-5,
- * max_outer 20, cheb_degree 0 (autotuned), DPF_HOST, stream NULL, flags 0. */
+ * max_outer 20, cheb_degree 0 (the topology's degrees), DPF_HOST, stream NULL,
+ * flags 0. */
 void dpf_opts_init(dpf_opts *opts);
 
 /* Topology cache.  Replaces the per-call work the reference repeats inside
@@ -133,20 +144,21 @@ int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers)
 
 /* info[0..17] = n, nedges, nfree, nnz(A), levels, coarse rows,
  * operator-complexity*1000, host symbolic setup time in microseconds, the
- * tuned Chebyshev degrees (finest, coarse) for DPF_FP64 and for DPF_MIXED
- * (0 = not tuned yet), the fine operator's 64-row slices stored as column
- * offsets (no column indices), the tuned degrees for DPF_MIXED_PC, then the
- * tuned Chebyshev interval ratio r (interval [lmax/r, lmax]) for DPF_FP64,
- * DPF_MIXED and DPF_MIXED_PC. */
+ * Chebyshev degrees (finest, coarse) in use for DPF_FP64 and for DPF_MIXED
+ * (the default 1 and 2 unless tuned or pinned), the fine operator's 64-row
+ * slices stored as column offsets (no column indices), the degrees for
+ * DPF_MIXED_PC, then the Chebyshev interval ratio r (interval [lmax/r, lmax];
+ * default 15) for DPF_FP64, DPF_MIXED and DPF_MIXED_PC. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* The Chebyshev autotuner's trials for one precision (DPF_FP64, DPF_MIXED or
- * DPF_MIXED_PC), zeros before its first solve: ms[k] / iters[k] for k < min(n,
- * 7) = wall time and PCG iterations of trial k, which solves the first
- * refinement step's system with (finest, coarse) degrees (1,1), (1,2), (2,1),
- * (2,2), (3,2) at interval ratio 15 (k = 0..4), then with the best of these
- * at ratios 30 and 8 (k = 5, 6); -1 ms marks a trial that did not converge.
- * No counterpart in the reference (its AMG smoother is fixed). */
+ * DPF_MIXED_PC), zeros unless a solve with DPF_FLAG_AUTOTUNE tuned it:
+ * ms[k] / iters[k] for k < min(n, 7) = wall time and PCG iterations of trial
+ * k, which solves the first refinement step's system with (finest, coarse)
+ * degrees (1,1), (1,2), (2,1), (2,2), (3,2) at interval ratio 15 (k = 0..4),
+ * then with the best of these at ratios 30 and 8 (k = 5, 6); -1 ms marks a
+ * trial that did not converge.  No counterpart in the reference (its AMG
+ * smoother is fixed). */
 int dpf_topology_tuning(const dpf_topology *topo, int32_t precision, double *ms,
                         int32_t *iters, int32_t n);
 
@@ -157,6 +169,23 @@ int dpf_topology_tuning(const dpf_topology *topo, int32_t precision, double *ms,
  * configuration under a profiler.  No counterpart in the reference. */
 int dpf_topology_set_tuning(dpf_topology *topo, int32_t precision, int32_t deg_fine,
                             int32_t deg_coarse, double ratio);
+
+/* Lanes per row of each level's row-split copy, the layout the coarse
+ * levels' smoothing sweeps and residuals read (2^lg[l] lanes share row r;
+ * lg[l] = 0: level l has no split copy), for levels 0..nlevels-1; us (may be
+ * NULL, else 7*nlevels) gets the autotuner's sweep time in microseconds of
+ * level l with 2^k lanes a row at us[7*l + k], 0 where not timed.  The
+ * default layout gives each lane about 8 entries; a solve with
+ * DPF_FLAG_AUTOTUNE times 2..32 lanes a row on every split level first and
+ * keeps the fastest.  No counterpart in the reference. */
+int dpf_topology_lanes(const dpf_topology *topo, int32_t *lg, double *us, int32_t nlevels);
+
+/* Fixes level's split copy at 2^lg lanes a row (1 <= lg <= 6; level >= 1 must
+ * have a split copy), in place of the autotuner (which then leaves the
+ * layout alone).  The row sums' order follows the layout, so two processes
+ * reproduce each other's results bit for bit when they pin the same lanes
+ * and smoother.  No counterpart in the reference. */
+int dpf_topology_set_lanes(dpf_topology *topo, int32_t level, int32_t lg);
 
 /* groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes,
  * dirichleths, Qs, specificstorage, volumes), src/groundwater.jl:1-38.

@@ -33,6 +33,11 @@ int dpf_inspect_create(int64_t n, int64_t nedges, const int64_t *neighbors,
                        dpf_inspect **out);
 void dpf_inspect_destroy(dpf_inspect *h);
 int32_t dpf_inspect_levels(const dpf_inspect *h);
+/* The check dpf_steady_forward makes before it solves: 0, or DPF_ESINGULAR
+ * (5) with the message in dpf_inspect_last_error() when some connected
+ * component of the mesh holds no Dirichlet cell, through every edge or, with
+ * Ks (nedges) given, through the edges with Ks[e] != 0. */
+int dpf_inspect_check_steady(const dpf_inspect *h, const double *Ks);
 int64_t dpf_inspect_size(const dpf_inspect *h, int32_t level, const char *name);
 int dpf_inspect_get(const dpf_inspect *h, int32_t level, const char *name, void *out,
                     int64_t capacity);

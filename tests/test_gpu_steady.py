@@ -501,7 +501,7 @@ def test_pinned_tuning(pkg, api):
         topo.close()
     assert np.array_equal(hs[0], hs[1])
     h_auto = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,
-                                         w.dirichleths, w.Qs, reltol=1e-12)
+                                         w.dirichleths, w.Qs, reltol=1e-12, autotune=True)
     assert rel(hs[0], h_auto) < 1e-10
 
 
@@ -569,3 +569,38 @@ def test_user_stream_call_then_topology_stream_call(pkg, api, oracle):
         hip.hipFree(d_lk)
         hip.hipFree(d_Ke)
         hip.hipStreamDestroy(stream)
+
+
+def test_steady_singular_systems_raise(pkg, api):
+    """A steady solve on a mesh without Dirichlet cells, or with a K_e = 0
+    cut that floats a layer of cells, raises SingularSystem (DPF_ESINGULAR)
+    naming the component; the same topology then solves the anchored system,
+    and a transient step (storage on the diagonal) needs no Dirichlet cell."""
+    _say("singular systems")
+    nx, ny, nz = 6, 5, 4
+    idx = np.arange(nx * ny * nz).reshape(nz, ny, nx) + 1
+    nb = np.concatenate([np.stack([idx[:, :, :-1].ravel(), idx[:, :, 1:].ravel()], 1),
+                         np.stack([idx[:, :-1, :].ravel(), idx[:, 1:, :].ravel()], 1),
+                         np.stack([idx[:-1, :, :].ravel(), idx[1:, :, :].ravel()], 1)])
+    n, ne = idx.size, nb.shape[0]
+    aol = np.ones(ne)
+    K = np.full(ne, 1e-5)
+    dh = np.zeros(n)
+    Q = np.zeros(n)
+    Q[0] = 1e-4
+    with pytest.raises(api.SingularSystem, match="no Dirichlet cell"):
+        api.groundwater_steadystate(K, nb, aol, np.zeros(0, dtype=np.int64), dh, Q)
+    top = idx[-1].ravel()
+    z0 = idx[0].ravel()
+    cut = np.isin(nb[:, 0], z0) & ~np.isin(nb[:, 1], z0)
+    Kc = K.copy()
+    Kc[cut] = 0.0
+    for precision in ("fp64", "mixed"):
+        with pytest.raises(api.SingularSystem, match="30 cells"):
+            api.groundwater_steadystate(Kc, nb, aol, top, dh, Q, precision=precision)
+    h = api.groundwater_steadystate(K, nb, aol, top, dh, Q, reltol=1e-12)
+    assert np.all(np.isfinite(h)) and h[0] > 0
+    u = api.groundwater_transient(K, nb, aol, np.zeros(0, dtype=np.int64), dh, Q,
+                                  np.full(n, 1e-5), np.ones(n), np.zeros(n), 3, 10.0,
+                                  np.ones(4), reltol=1e-12)
+    assert np.all(np.isfinite(u)) and u[0] > 0

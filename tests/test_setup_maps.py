@@ -4,6 +4,7 @@ replayed in numpy through the same maps and compared with independent sparse
 algebra (scipy) -- the fine Jacobian against the oracle, P against
 (I - 4/3 D_l1^-1 A) T, R against P', A*P and R*A*P against scipy products."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -23,10 +24,31 @@ DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.in
       "rap_msrc": np.int32}
 
 
+_INSPECT = None
+
+
+def inspect_lib():
+    """The host-setup entry points: from libdpfgpu.so, or from the CPU-only
+    library DPF_INSPECT_LIB names (the AddressSanitizer build)."""
+    global _INSPECT
+    if _INSPECT is None:
+        from dpfehm_jl_amd import _lib
+        path = os.environ.get("DPF_INSPECT_LIB")
+        if not path:
+            _INSPECT = _lib.lib()
+        else:
+            L = ctypes.CDLL(path)
+            for name, (res, args) in _lib.SIGNATURES.items():
+                if name.startswith("dpf_inspect_"):
+                    f = getattr(L, name)
+                    f.restype, f.argtypes = res, args
+            _INSPECT = L
+    return _INSPECT
+
+
 class Inspect:
     def __init__(self, n, nb, aol, dn, max_levels=25, coarse_max=16):
-        from dpfehm_jl_amd import _lib
-        self.L = _lib.lib()
+        self.L = inspect_lib()
         nb = np.ascontiguousarray(nb, dtype=np.int64)
         aol = np.ascontiguousarray(aol[:nb.shape[0]], dtype=np.float64)
         dn = np.ascontiguousarray(dn, dtype=np.int64)
@@ -217,8 +239,7 @@ def test_topology_maps(pkg, oracle):
 
 
 def test_topology_rejects_bad_input(pkg):
-    from dpfehm_jl_amd import _lib
-    L = _lib.lib()
+    L = inspect_lib()
     nb = np.array([[1, 2], [2, 5]], dtype=np.int64)
     aol = np.ones(2)
     dn = np.array([1], dtype=np.int64)
@@ -352,3 +373,85 @@ def test_fine_offset_layout(pkg, oracle, shape, dirichlet):
     holes = np.ones(Fs[-1], dtype=bool)
     holes[emap[real_host]] = False
     assert (Fp[holes] == NO_SLOT).all()
+
+
+def _box(nx, ny, nz):
+    """Edges of an nx x ny x nz box (1-based), x fastest."""
+    idx = np.arange(nx * ny * nz).reshape(nz, ny, nx) + 1
+    pairs = [np.stack([idx[:, :, :-1].ravel(), idx[:, :, 1:].ravel()], 1),
+             np.stack([idx[:, :-1, :].ravel(), idx[:, 1:, :].ravel()], 1),
+             np.stack([idx[:-1, :, :].ravel(), idx[1:, :, :].ravel()], 1)]
+    return idx, np.concatenate(pairs).astype(np.int64)
+
+
+def test_steady_singular_systems_are_named(pkg):
+    """The check dpf_steady_forward makes before solving (DPF_ESINGULAR):
+    dirichletnodes empty (a mesh without any Dirichlet cell) and a K_e = 0
+    edge cut that leaves a slab of cells without a Dirichlet cell are
+    rejected with a message naming a cell of the floating component and its
+    size; the same mesh with K_e > 0 everywhere, or a cut that leaves every
+    part anchored, passes."""
+    idx, nb = _box(6, 5, 4)
+    n = idx.size
+    top = idx[-1].ravel()
+    aol = np.ones(nb.shape[0])
+    # no Dirichlet cell at all
+    I = Inspect(n, nb, aol, np.zeros(0, dtype=np.int64))
+    assert I.L.dpf_inspect_check_steady(I.h, None) == 5
+    msg = I.L.dpf_inspect_last_error().decode()
+    assert "no Dirichlet cell" in msg and f"{n} cells" in msg and "singular" in msg
+    I.close()
+    I = Inspect(n, nb, aol, top)
+    K = np.ones(nb.shape[0])
+    assert I.L.dpf_inspect_check_steady(I.h, None) == 0
+    assert I.L.dpf_inspect_check_steady(I.h, K.ctypes.data) == 0
+    # cut every vertical edge between layers 1 and 2 (z index 0 -> 1): the
+    # bottom layer (30 cells) loses its path to the top
+    z0 = idx[0].ravel()
+    cut = np.isin(nb[:, 0], z0) & ~np.isin(nb[:, 1], z0)
+    assert cut.sum() == 30
+    Kc = K.copy()
+    Kc[cut] = 0.0
+    assert I.L.dpf_inspect_check_steady(I.h, Kc.ctypes.data) == 5
+    msg = I.L.dpf_inspect_last_error().decode()
+    assert "free cell 1 " in msg and "30 cells" in msg and "K_e" in msg
+    # zero areasoverlengths cut the same way
+    a2 = aol.copy()
+    a2[cut] = 0.0
+    I2 = Inspect(n, nb, a2, top)
+    assert I2.L.dpf_inspect_check_steady(I2.h, None) == 5
+    I2.close()
+    # a cut that leaves both sides anchored: fine
+    Kh = K.copy()
+    Kh[cut & (np.arange(nb.shape[0]) % 2 == 0)] = 0.0
+    assert I.L.dpf_inspect_check_steady(I.h, Kh.ctypes.data) == 0
+    I.close()
+
+
+ASAN_SOURCES = ("host_setup.cpp", "inspect.cpp")
+
+
+@pytest.mark.skipif(bool(os.environ.get("DPF_INSPECT_LIB")), reason="already the ASan run")
+def test_host_setup_under_address_sanitizer(tmp_path):
+    """host_setup.cpp and inspect.cpp built with -fsanitize=address (host
+    code only, g++), and this file's tests run against that build in a child
+    process: any heap overflow or use-after-free in the symbolic setup fails
+    the run."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    csrc = os.path.join(root, "dpfehm.jl_amd", "csrc")
+    so = str(tmp_path / "libdpf_inspect_asan.so")
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+           "-fsanitize=address", "-fno-omit-frame-pointer", "-I", os.path.join(root, "include")]
+    cmd += [os.path.join(csrc, f) for f in ASAN_SOURCES] + ["-o", so]
+    subprocess.run(cmd, check=True, capture_output=True)
+    asan = subprocess.run(["g++", "-print-file-name=libasan.so"], check=True,
+                          capture_output=True, text=True).stdout.strip()
+    env = dict(os.environ, DPF_INSPECT_LIB=so, LD_PRELOAD=asan,
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
+                        os.path.abspath(__file__)], env=env, capture_output=True, text=True,
+                       cwd=root, timeout=900)
+    assert r.returncode == 0, (r.stdout[-4000:], r.stderr[-4000:])
+    assert "passed" in r.stdout
