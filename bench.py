@@ -43,7 +43,7 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["steady2m", "ensemble", "transient", "twophase",
-                                           "richards", "transport"],
+                                           "richards", "transport", "barrier"],
                     default="steady2m",
                     help="steady2m (default, the headline line); ensemble: 96 packed "
                          "realizations on 32x64^3 + 32x96x96x24 + 32x128x128x16 grids; "
@@ -98,6 +98,8 @@ def main():
         return bench_richards(args, P, api, _lib, torch)
     if args.workload == "transport":
         return bench_transport(args, P, api, _lib, torch)
+    if args.workload == "barrier":
+        return bench_barrier(args, _lib)
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda:0")
@@ -682,6 +684,31 @@ def bench_richards(args, P, api, _lib, torch):
            "fp32_vcycle_bicgstab_fwd_adj": [a32.iterations, b32.iterations],
            "fp32_vcycle_rel_err_head_grad": [rel(psi32, psi64), rel(dK32, dK64)]}
     print(json.dumps(res), flush=True)
+
+
+def bench_barrier(args, _lib):
+    """Cost of one grid-wide barrier inside a cooperative launch
+    (dpf_profile_grid_barrier), the alternative to a kernel boundary between
+    dependent phases of a persistent kernel: workgroups taking part on every
+    XCD or on one, 1000 rounds of two barriers each."""
+    L = _lib.lib()
+    rows = []
+    for one_xcd in (0, 1):
+        for nparts in (8, 32, 64, 128, 256):
+            us = ctypes.c_double()
+            errs = ctypes.c_int32()
+            code = L.dpf_profile_grid_barrier(nparts, one_xcd, 1000, ctypes.byref(us),
+                                              ctypes.byref(errs))
+            if code != 0:
+                log(f"[bench] barrier nparts {nparts} one_xcd {one_xcd}: "
+                    f"{L.dpf_last_error().decode()}")
+                continue
+            rows.append({"workgroups": nparts, "one_xcd": bool(one_xcd),
+                         "us_per_barrier": round(us.value, 3), "errors": errs.value})
+            log(f"[bench] barrier: {nparts} workgroups {'on one XCD' if one_xcd else 'on all XCDs'}:"
+                f" {us.value:.3f} us per barrier, errors {errs.value}")
+    print(json.dumps({"metric": "grid barrier cost", "unit": "us", "higher_is_better": False,
+                      "rows": rows}), flush=True)
 
 
 def bench_transport(args, P, api, _lib, torch):

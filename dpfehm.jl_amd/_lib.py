@@ -134,6 +134,7 @@ SIGNATURES = {
                                                     ctypes.POINTER(Opts), ctypes.POINTER(Stats)]),
     "dpf_kernel_table": (ctypes.c_int, [_p, _p, _i32, _p, _i32]),
     "dpf_profile_kernel": (ctypes.c_int, [_p, _i32, _i32, _i32, _p, _p, ctypes.POINTER(Opts)]),
+    "dpf_profile_grid_barrier": (ctypes.c_int, [_i32, _i32, _i32, _p, _p]),
     "dpf_inspect_last_error": (ctypes.c_char_p, []),
     "dpf_inspect_create": (ctypes.c_int, [_i64, _i64, _p, _p, _i64, _p, _i32, _i32,
                                           ctypes.POINTER(_p)]),

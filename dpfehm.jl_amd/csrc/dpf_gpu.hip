@@ -2313,6 +2313,130 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
 
 }  // namespace
 
+namespace {
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident):
+// thread 0 of each taking-part workgroup arrives on *count; the last arrival
+// resets it and moves *gen on, the others wait for *gen to move.  Agent-scope
+// fences on both sides make each workgroup's earlier writes visible to every
+// other's later reads, across the XCDs' separate L2 caches.  Each wait is
+// bounded (spin_limit polls): one that runs out sets *err and the barrier
+// returns false in every thread of its workgroup, so no wave waits forever.
+__device__ bool grid_barrier(unsigned* count, unsigned* gen, unsigned nparts, unsigned& epoch,
+                             int spin_limit, int* err) {
+  __shared__ int ok_sh;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned g = epoch;
+    __threadfence();
+    unsigned prev = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == nparts - 1) {
+      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (++spins > spin_limit) {
+          ok = 0;
+          __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+    epoch = g + 1u;
+    ok_sh = ok;
+  }
+  __syncthreads();
+  return ok_sh != 0;
+}
+
+// Barrier cost probe: rounds of (write own word, barrier, read the next
+// workgroup's word and check it, barrier).  stride 8: only the workgroups
+// with blockIdx % 8 == 0 take part -- all on one XCD when the dispatcher
+// deals workgroups to the 8 XCDs in turn; stride 1: every workgroup.
+__global__ void __launch_bounds__(256) k_barrier_probe(unsigned* count, unsigned* gen, int rounds,
+                                                       int stride, int* err, int* data) {
+  if (blockIdx.x % stride != 0) return;
+  const unsigned me = blockIdx.x / stride, np = gridDim.x / stride;
+  unsigned epoch = 0;
+  for (int k = 0; k < rounds; k++) {
+    if (threadIdx.x == 0) data[me] = k * 4096 + (int)me;
+    if (!grid_barrier(count, gen, np, epoch, 1 << 17, err)) return;
+    if (threadIdx.x == 0) {
+      const unsigned o = (me + 1) % np;
+      if (data[o] != k * 4096 + (int)o)
+        __hip_atomic_fetch_or(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!grid_barrier(count, gen, np, epoch, 1 << 17, err)) return;
+  }
+}
+
+}  // namespace
+
+extern "C" int dpf_profile_grid_barrier(int32_t nparts, int32_t one_xcd, int32_t rounds,
+                                        double* us, int32_t* errors) {
+  if (!us || !errors || nparts < 1 || nparts > 4096 || rounds < 1 || rounds > 100000) {
+    set_err("need 1 <= nparts <= 4096, 1 <= rounds <= 100000 and output pointers");
+    return DPF_EINVAL;
+  }
+  try {
+    int dev = 0, coop = 0;
+    HIPTRY(hipGetDevice(&dev));
+    HIPTRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) throw Err{DPF_EINVAL, "the device does not support cooperative launches"};
+    const int stride = one_xcd ? 8 : 1;
+    DBuf words;
+    words.alloc((size_t)(4 + nparts) * sizeof(int));
+    int* base = words.as<int>();
+    unsigned* count = (unsigned*)base;
+    unsigned* gen = (unsigned*)base + 1;
+    int* err = base + 2;
+    int* data = base + 4;
+    hipStream_t s = nullptr;
+    HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&](int nr) -> hipError_t {
+      HIPTRY(hipMemsetAsync(base, 0, (size_t)(4 + nparts) * sizeof(int), s));
+      void* args[] = {&count, &gen, &nr, (void*)&stride, &err, &data};
+      return hipLaunchCooperativeKernel((const void*)k_barrier_probe,
+                                        dim3((unsigned)(nparts * stride)), dim3(256), args, 0, s);
+    };
+    hipError_t le = run(std::min(rounds, 10));
+    float ms = 0.f;
+    if (le == hipSuccess) {
+      HIPTRY(hipEventCreate(&e0));
+      HIPTRY(hipEventCreate(&e1));
+      HIPTRY(hipMemsetAsync(base, 0, (size_t)(4 + nparts) * sizeof(int), s));
+      void* args[] = {&count, &gen, &rounds, (void*)&stride, &err, &data};
+      HIPTRY(hipEventRecord(e0, s));
+      le = hipLaunchCooperativeKernel((const void*)k_barrier_probe,
+                                      dim3((unsigned)(nparts * stride)), dim3(256), args, 0, s);
+      HIPTRY(hipEventRecord(e1, s));
+      HIPTRY(hipEventSynchronize(e1));
+      HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+    }
+    int herr = 0;
+    if (le == hipSuccess) HIPTRY(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    HIPTRY(hipStreamSynchronize(s));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (le != hipSuccess) {
+      (void)hipGetLastError();
+      throw Err{DPF_EINVAL, std::string("cooperative launch refused: ") + hipGetErrorString(le)};
+    }
+    *us = 1e3 * (double)ms / (2.0 * rounds);
+    *errors = herr;
+  } catch (const Err& e) {
+    set_err(e.msg);
+    return e.code;
+  }
+  return DPF_OK;
+}
+
 extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t kernel,
                                   int32_t reps, double* avg_ms, double* bytes,
                                   const dpf_opts* opts) {

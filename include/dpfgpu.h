@@ -459,6 +459,16 @@ int dpf_kernel_table(dpf_topology *topo, dpf_kernel_record *out, int32_t cap, in
 int dpf_profile_kernel(dpf_topology *topo, int32_t precision, int32_t kernel, int32_t reps,
                        double *avg_ms, double *bytes, const dpf_opts *opts);
 
+/* Cost of a grid-wide barrier inside one cooperative launch (no counterpart
+ * in the reference): nparts workgroups -- on every XCD, or with one_xcd only
+ * those the dispatcher places on one XCD -- run `rounds` rounds of (write a
+ * word, barrier, read a neighbour's word, barrier); us = mean time per
+ * barrier, errors = 0, or bit 1 when a bounded wait ran out, bit 2 when a
+ * read missed the neighbour's write.  The measurement behind the choice of
+ * kernel boundaries over a persistent V-cycle kernel. */
+int dpf_profile_grid_barrier(int32_t nparts, int32_t one_xcd, int32_t rounds, double *us,
+                             int32_t *errors);
+
 #ifdef __cplusplus
 }
 #endif
