@@ -114,6 +114,7 @@ SIGNATURES = {
     "dpf_transport_pullback": (ctypes.c_int, [_p] * 9),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
+    "dpf_hm_forward_noalloc": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_geo_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_geo_pullback": (ctypes.c_int, [_p, _p, _p, _p, ctypes.POINTER(Opts)]),

@@ -1202,6 +1202,35 @@ namespace {
 
 // Shared by dpf_topology_create and dpf_batch_topology_create.  members:
 // cell offsets of the packed members (empty for a single system).
+// Staging of the allocation-free entry point dpf_hm_forward_noalloc: exp(logK)
+// per cell, DPF_NOALLOC_MAX_CELLS doubles (32 MiB) in every GPU's copy of the
+// code object, held until the process ends.  Host code never passes this
+// name to a kernel (on the host it is only a stand-in for the symbol): it
+// uses the device address hipGetSymbolAddress returns for the current
+// device, kept per device in g_scratch.
+static __device__ __align__(256) double scratch[DPF_NOALLOC_MAX_CELLS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The address of `scratch` on device dev, looked up once per device with dev
+// current (the caller makes it current).
+double* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw Err{DPF_EINVAL, "device index out of range"};
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw Err{DPF_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch[dev] = (double*)ptr;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
 int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
                 const double* areasoverlengths, int64_t ndirichlet,
                 const int64_t* dirichletnodes, const std::vector<int64_t>& cell0,
@@ -1237,6 +1266,8 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
       return DPF_EINVAL;
     }
     HIPTRY(hipGetDevice(&t->device));
+    // the allocation-free entry point's staging address on this device
+    (void)scratch_on(t->device);
     HIPTRY(hipStreamCreateWithFlags(&t->own_stream, hipStreamNonBlocking));
     for (auto& ev : t->ev_setup) HIPTRY(hipEventCreate(&ev));
     // room for a few forward states (K_e, dh, Q, heads, multipliers) and the
@@ -2036,6 +2067,41 @@ int dpf_hm_pullback(dpf_topology* t, const double* logK, const double* Ke, const
 
 int dpf_geo_forward(dpf_topology* t, const double* logK, double* Ke, const dpf_opts* opts) {
   return kmap_forward(t, true, logK, Ke, opts);
+}
+
+int dpf_hm_forward_noalloc(dpf_topology* t, const double* logK, double* Ke,
+                           const dpf_opts* opts) {
+  if (!t || !logK || !Ke || !opts) {
+    set_err("NULL argument");
+    return DPF_EINVAL;
+  }
+  if (opts->memory != DPF_DEVICE) {
+    set_err("dpf_hm_forward_noalloc takes device memory only (opts->memory = DPF_DEVICE)");
+    return DPF_EINVAL;
+  }
+  if (t->H.n > DPF_NOALLOC_MAX_CELLS) {
+    set_err(std::to_string(t->H.n) + " cells exceed the allocation-free staging of " +
+            std::to_string(DPF_NOALLOC_MAX_CELLS) + " cells (use dpf_hm_forward)");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  try {
+    // the address was looked up when the topology was created on its device
+    double* kb = g_scratch[t->device];
+    if (!kb) throw Err{DPF_EHIP, "no staging address for the topology's device"};
+    hipStream_t s = pick_stream(t, opts);
+    const int64_t n = t->H.n, ne = t->H.ne;
+    // the same two kernels as dpf_hm_forward, with the staging in `scratch`;
+    // k_exp writes every entry k_hm_forward reads, so nothing is cleared
+    hipLaunchKernelGGL(k_exp, dim3(nblocks(n)), dim3(BLK), 0, s, n, logK, kb);
+    hipLaunchKernelGGL(k_hm_forward, dim3(nblocks(ne)), dim3(BLK), 0, s, ne,
+                       (const int2*)t->edges.as<int2>(), (const double*)kb, Ke);
+    HIPTRY(hipGetLastError());
+  } catch (const Err& e) {
+    set_err(e.msg);
+    return e.code;
+  }
+  return DPF_OK;
 }
 
 int dpf_geo_pullback(dpf_topology* t, const double* Ke, const double* dKe, double* dlogK,

@@ -201,6 +201,23 @@ int dpf_residuals(dpf_topology *topo, const double *hfree, const double *Ks,
  * map of examples/gw_steadystate_3d/inputdeck.jl:51), as a cell gather.
  *   logK: n;  Ke, dKe: nedges;  dlogK: n. */
 int dpf_hm_forward(dpf_topology *topo, const double *logK, double *Ke, const dpf_opts *opts);
+
+/* Allocation-free dpf_hm_forward.  No counterpart in the reference: for
+ * callers that cannot allocate (code under hipGraph stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * results as dpf_hm_forward (bit for bit), but its exp(logK) staging lives in
+ * a fixed device array of DPF_NOALLOC_MAX_CELLS doubles (32 MiB per GPU, held
+ * until the process ends) instead of a pooled buffer, and the call neither
+ * allocates, frees nor synchronises: it queues two kernels on opts->stream
+ * (or the topology's stream) and returns.  Device memory only
+ * (opts->memory = DPF_DEVICE, else DPF_EINVAL); a topology of more than
+ * DPF_NOALLOC_MAX_CELLS cells is refused with DPF_EINVAL.  Every call on one
+ * GPU shares that GPU's staging array: calls that may overlap on one GPU
+ * (different streams, or different host threads) race and must be ordered
+ * by the caller. */
+#define DPF_NOALLOC_MAX_CELLS 4194304
+int dpf_hm_forward_noalloc(dpf_topology *topo, const double *logK, double *Ke,
+                           const dpf_opts *opts);
 int dpf_hm_pullback(dpf_topology *topo, const double *logK, const double *Ke,
                     const double *dKe, double *dlogK, const dpf_opts *opts);
 
