@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libdpfgpu.so")
 DPF_OK, DPF_EINVAL, DPF_EHIP, DPF_ENOCONV, DPF_EBREAKDOWN, DPF_ESINGULAR, DPF_ENOMEM = range(7)
 DPF_FP64, DPF_MIXED, DPF_MIXED_PC = 0, 1, 2
 DPF_HOST, DPF_DEVICE = 0, 1
+DPF_JAC_H, DPF_JAC_KS, DPF_JAC_DIRICHLETHS, DPF_JAC_QS = 0, 1, 2, 3
 DPF_FLAG_SWEEP_TIMING = 1
 DPF_FLAG_GRAPH = 2
 DPF_FLAG_KERNEL_TIMING = 4
@@ -113,6 +114,9 @@ SIGNATURES = {
                                                        _i32, _p, _p, _p, _p, _p]),
     "dpf_transport_pullback": (ctypes.c_int, [_p] * 9),
     "dpf_residuals": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),
+    "dpf_jacobian_pattern": (ctypes.c_int, [_p, _i32, _p, _p, _p]),
+    "dpf_jacobian_values": (ctypes.c_int, [_p, _i32, _p, _p, _p, _p, _p, _p,
+                                           ctypes.POINTER(Opts)]),
     "dpf_hm_forward": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_forward_noalloc": (ctypes.c_int, [_p, _p, _p, ctypes.POINTER(Opts)]),
     "dpf_hm_pullback": (ctypes.c_int, [_p, _p, _p, _p, _p, ctypes.POINTER(Opts)]),

@@ -340,6 +340,63 @@ def groundwater_residuals(h, Ks, neighbors, areasoverlengths, dirichletnodes, di
     return r
 
 
+def _jacobian(which, hfree, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+              specificstorage, volumes):
+    import scipy.sparse as sp
+    topo = get_topology(len(Qs), neighbors, areasoverlengths, dirichletnodes)
+    L = _lib.lib()
+    nnz = ctypes.c_int64()
+    _lib.check(L.dpf_jacobian_pattern(topo.handle, which, ctypes.byref(nnz), None, None))
+    rp = np.empty(topo.nfree + 1, dtype=np.int64)
+    cols = np.empty(max(nnz.value, 1), dtype=np.int32)
+    _lib.check(L.dpf_jacobian_pattern(topo.handle, which, ctypes.byref(nnz), _ptr(rp),
+                                      _ptr(cols)))
+    vals = np.empty(max(nnz.value, 1))
+    arrs = [None if a is None else _f64(a) for a in (hfree, Ks, dirichleths, specificstorage,
+                                                    volumes)]
+    o = _opts({})
+    _lib.check(L.dpf_jacobian_values(topo.handle, which,
+                                     *[None if a is None else _ptr(a) for a in arrs],
+                                     _ptr(vals), ctypes.byref(o)))
+    ncol = {_lib.DPF_JAC_H: topo.nfree, _lib.DPF_JAC_KS: topo.ne}.get(which, topo.n)
+    m = nnz.value
+    return sp.csr_matrix((vals[:m], cols[:m], rp), shape=(topo.nfree, ncol))
+
+
+def groundwater_h(h, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                  specificstorage, volumes):
+    """d groundwater_residuals / dh (free cells x free cells) as a scipy CSR
+    matrix, the generated groundwater_h of src/groundwater.jl:1 (f_gw_u in
+    examples/gw_transient_3d/inputdeck.jl:82-85).  ``h`` is not read (the
+    residual is affine in h)."""
+    return _jacobian(_lib.DPF_JAC_H, None, Ks, neighbors, areasoverlengths, dirichletnodes,
+                     dirichleths, Qs, specificstorage, volumes)
+
+
+def groundwater_Ks(h, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                   specificstorage, volumes):
+    """d groundwater_residuals / dKs (free cells x edges), scipy CSR: the
+    generated groundwater_Ks (f_gw_p's Ks block, inputdeck.jl:86-91)."""
+    return _jacobian(_lib.DPF_JAC_KS, h, None, neighbors, areasoverlengths, dirichletnodes,
+                     dirichleths, Qs, specificstorage, volumes)
+
+
+def groundwater_dirichleths(h, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths,
+                            Qs, specificstorage, volumes):
+    """d groundwater_residuals / d dirichleths (free cells x cells), scipy
+    CSR: the generated groundwater_dirichleths (f_gw_p's dirichleths block)."""
+    return _jacobian(_lib.DPF_JAC_DIRICHLETHS, None, Ks, neighbors, areasoverlengths,
+                     dirichletnodes, dirichleths, Qs, specificstorage, volumes)
+
+
+def groundwater_Qs(h, Ks, neighbors, areasoverlengths, dirichletnodes, dirichleths, Qs,
+                   specificstorage, volumes):
+    """d groundwater_residuals / dQs (free cells x cells), scipy CSR: the
+    generated groundwater_Qs."""
+    return _jacobian(_lib.DPF_JAC_QS, None, None, neighbors, areasoverlengths, dirichletnodes,
+                     dirichleths, Qs, specificstorage, volumes)
+
+
 def hm(x, y):
     """DPFEHM.hm (src/richards.jl:50-52), elementwise on host arrays."""
     return 2.0 / (1.0 / np.asarray(x) + 1.0 / np.asarray(y))

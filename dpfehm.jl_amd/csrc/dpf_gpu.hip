@@ -356,6 +356,14 @@ struct dpf_topology {
   // two-phase reverse mode: per incidence entry the same edge's entry in the
   // other cell's list, per edge its two entries (built on first use)
   DBuf tp_mirror, tp_epos;
+  // Jacobian export (built on first use): per kind (h, Ks, dirichleths, Qs)
+  // the CSR pattern on the host and its row pointers on the device, per
+  // incidence the value index (h, Ks, dirichleths), the diagonals' indices
+  bool jac_maps = false, jac_dev = false;
+  std::vector<int64_t> jac_rp[4];
+  std::vector<int32_t> jac_cols[4];
+  std::vector<int32_t> jac_pos[3], jac_dpos;
+  DBuf jac_rp_d[4], jac_pos_d[3], jac_dpos_d;
   bool tp_maps = false;
   // Chebyshev interval [1/ratio, 1] of D_l1^-1 A: default, and tuned per precision
   double cheb_ratio = 15.0;
@@ -1952,6 +1960,136 @@ int dpf_residuals(dpf_topology* t, const double* hfree, const double* Ks,
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(r_out, r, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    return DPF_OK;
+  });
+}
+
+// CSR patterns of the four Jacobians and the incidence -> value maps
+// (host), built once per topology.
+static void build_jac_maps(dpf_topology* t) {
+  if (t->jac_maps) return;
+  const TopologyHost& H = t->H;
+  const LevelHost& L0 = H.levels[0];
+  const int64_t nf = H.nf, ninc = H.inc_ptr.back();
+  for (int k = 0; k < 4; k++) {
+    t->jac_rp[k].assign(nf + 1, 0);
+    t->jac_cols[k].clear();
+  }
+  for (int k = 0; k < 3; k++) t->jac_pos[k].assign(ninc, -1);
+  t->jac_dpos.assign(nf, 0);
+  std::vector<int32_t> dn;
+  for (int64_t j = 0; j < nf; j++) {
+    const int32_t i = H.f2n[j];
+    const int64_t p0 = H.inc_ptr[i], p1 = H.inc_ptr[i + 1];
+    // h: the fine pattern's row (columns ascending)
+    const int64_t b0 = (int64_t)t->jac_cols[0].size();
+    for (int k = 0; k < L0.A.rowlen[j]; k++) t->jac_cols[0].push_back(L0.A.at(j, k));
+    t->jac_dpos[j] = (int32_t)(b0 + L0.diag_slot[j]);
+    for (int64_t p = p0; p < p1; p++)
+      if (H.inc_slot[p] != NO_SLOT) t->jac_pos[0][p] = (int32_t)(b0 + H.inc_slot[p]);
+    t->jac_rp[0][j + 1] = (int64_t)t->jac_cols[0].size();
+    // Ks: one entry per incident edge, in edge order
+    const int64_t b1 = (int64_t)t->jac_cols[1].size();
+    for (int64_t p = p0; p < p1; p++) {
+      t->jac_pos[1][p] = (int32_t)(b1 + (p - p0));
+      t->jac_cols[1].push_back(H.inc_edge[p]);
+    }
+    t->jac_rp[1][j + 1] = (int64_t)t->jac_cols[1].size();
+    // dirichleths: the Dirichlet neighbours, ascending, each once
+    dn.clear();
+    for (int64_t p = p0; p < p1; p++)
+      if (!H.isfree[H.inc_other[p]]) dn.push_back(H.inc_other[p]);
+    std::sort(dn.begin(), dn.end());
+    dn.erase(std::unique(dn.begin(), dn.end()), dn.end());
+    const int64_t b2 = (int64_t)t->jac_cols[2].size();
+    for (int64_t p = p0; p < p1; p++) {
+      int32_t o = H.inc_other[p];
+      if (!H.isfree[o])
+        t->jac_pos[2][p] = (int32_t)(b2 + (std::lower_bound(dn.begin(), dn.end(), o) - dn.begin()));
+    }
+    t->jac_cols[2].insert(t->jac_cols[2].end(), dn.begin(), dn.end());
+    t->jac_rp[2][j + 1] = (int64_t)t->jac_cols[2].size();
+    // Qs: the row's own cell
+    t->jac_cols[3].push_back(i);
+    t->jac_rp[3][j + 1] = j + 1;
+  }
+  t->jac_maps = true;
+}
+
+int dpf_jacobian_pattern(dpf_topology* t, int32_t which, int64_t* nnz, int64_t* rowptr,
+                         int32_t* cols) {
+  if (!t || !nnz || which < DPF_JAC_H || which > DPF_JAC_QS) {
+    set_err("need a topology, a Jacobian kind (DPF_JAC_*) and nnz");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  try {
+    build_jac_maps(t);
+  } catch (const std::bad_alloc&) {
+    set_err("host allocation failed");
+    return DPF_ENOMEM;
+  }
+  const auto& rp = t->jac_rp[which];
+  const auto& cc = t->jac_cols[which];
+  *nnz = (int64_t)cc.size();
+  if (rowptr) std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int64_t));
+  if (cols && !cc.empty()) std::memcpy(cols, cc.data(), cc.size() * sizeof(int32_t));
+  return DPF_OK;
+}
+
+int dpf_jacobian_values(dpf_topology* t, int32_t which, const double* hfree, const double* Ks,
+                        const double* dirichleths, const double* Ss, const double* V,
+                        double* vals, const dpf_opts* opts) {
+  if (!t || !Ss || !V || !vals || which < DPF_JAC_H || which > DPF_JAC_QS ||
+      ((which == DPF_JAC_H || which == DPF_JAC_DIRICHLETHS) && !Ks) ||
+      (which == DPF_JAC_KS && (!hfree || !dirichleths))) {
+    set_err("NULL argument or unknown Jacobian kind (DPF_JAC_H and DPF_JAC_DIRICHLETHS read "
+            "Ks; DPF_JAC_KS reads hfree and dirichleths; every kind reads Ss and V)");
+    return DPF_EINVAL;
+  }
+  dpf_opts dflt;
+  dpf_opts_init(&dflt);
+  const dpf_opts* o = opts ? opts : &dflt;
+  if (check_opts(o)) {
+    set_err("invalid options");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  return guarded([&]() {
+    build_jac_maps(t);
+    hipStream_t s = pick_stream(t, o);
+    if (!t->jac_dev) {
+      for (int k = 0; k < 4; k++) upload(t->jac_rp_d[k], t->jac_rp[k]);
+      for (int k = 0; k < 3; k++) upload(t->jac_pos_d[k], t->jac_pos[k]);
+      upload(t->jac_dpos_d, t->jac_dpos);
+      t->jac_dev = true;
+    }
+    const int64_t n = t->H.n, nf = t->H.nf, ne = t->H.ne;
+    const int64_t nnz = (int64_t)t->jac_cols[which].size();
+    const double* K = which == DPF_JAC_KS || which == DPF_JAC_QS
+                          ? nullptr
+                          : stage_in(t, o, t->stage_e, Ks, ne, s);
+    const double* h = which == DPF_JAC_KS ? stage_in(t, o, t->stage_f, hfree, nf, s) : nullptr;
+    const double* dh = which == DPF_JAC_KS ? stage_in(t, o, t->stage_n, dirichleths, n, s)
+                                           : nullptr;
+    DBuf ssb, vb, outb;
+    const double* ss = stage_in(t, o, ssb, Ss, n, s);
+    const double* vv = stage_in(t, o, vb, V, n, s);
+    double* out = vals;
+    if (o->memory == DPF_HOST) {
+      outb.alloc(std::max<int64_t>(nnz, 1) * sizeof(double));
+      out = outb.as<double>();
+    }
+    hipLaunchKernelGGL(k_jacobian, dim3(nblocks(nf)), dim3(BLK), 0, s, (int)which, nf,
+                       t->f2n.as<int32_t>(), t->inc_ptr.as<int64_t>(), t->inc_edge.as<int32_t>(),
+                       t->inc_other.as<int32_t>(), t->n2f.as<int32_t>(),
+                       t->jac_rp_d[which].as<int64_t>(),
+                       which < 3 ? t->jac_pos_d[which].as<int32_t>() : (int32_t*)nullptr,
+                       t->jac_dpos_d.as<int32_t>(), K, t->aol.as<double>(), h, dh, ss, vv, out);
+    HIPTRY(hipGetLastError());
+    if (o->memory == DPF_HOST && nnz > 0)
+      HIPTRY(hipMemcpyAsync(vals, out, nnz * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
     return DPF_OK;
   });

@@ -692,6 +692,53 @@ __global__ void k_residuals(int64_t nf, const int32_t* __restrict__ f2n,
   r[j] = acc;
 }
 
+// Jacobians of groundwater_residuals over the free rows (the matrices the
+// generated groundwater_h / _Ks / _dirichleths / _Qs of src/groundwater.jl:1
+// return), values in the CSR order of their patterns: columns ascending,
+// contributions to one entry summed in incidence (edge) order.  pos[p]: the
+// value index of incidence p of the row's cell, -1 where the incidence adds
+// no entry to this Jacobian; dpos[j]: row j's diagonal (kind 0).  s = Ss V:
+//   kind 0 (h)           diagonal += K aol / s;  [free other] -= K aol / s
+//   kind 1 (Ks)          [edge] = (h_i - h~_o) aol / s
+//   kind 2 (dirichleths) [Dirichlet other] -= K aol / s
+//   kind 3 (Qs)          [cell] = -1 / s
+__global__ void k_jacobian(int kind, int64_t nf, const int32_t* __restrict__ f2n,
+                           const int64_t* __restrict__ inc_ptr,
+                           const int32_t* __restrict__ inc_edge,
+                           const int32_t* __restrict__ inc_other,
+                           const int32_t* __restrict__ n2f, const int64_t* __restrict__ rowptr,
+                           const int32_t* __restrict__ pos, const int32_t* __restrict__ dpos,
+                           const double* __restrict__ Ke, const double* __restrict__ aol,
+                           const double* __restrict__ h, const double* __restrict__ dh,
+                           const double* __restrict__ Ss, const double* __restrict__ V,
+                           double* __restrict__ vals) {
+  int64_t j = logical_row();
+  if (j >= nf) return;
+  int32_t i = f2n[j];
+  double s = Ss[i] * V[i];
+  if (kind == 3) {
+    vals[j] = -1.0 / s;
+    return;
+  }
+  if (kind != 1)
+    for (int64_t q = rowptr[j]; q < rowptr[j + 1]; q++) vals[q] = 0.0;
+  const double hi = kind == 1 ? h[j] : 0.0;
+  double diag = 0.0;
+  for (int64_t p = inc_ptr[i]; p < inc_ptr[i + 1]; p++) {
+    int32_t e = inc_edge[p], q = pos[p];
+    if (kind == 1) {
+      int32_t o = inc_other[p], fo = n2f[o];
+      double ho = fo >= 0 ? h[fo] : dh[o];
+      vals[q] = (hi - ho) * aol[e] / s;
+    } else {
+      double t = Ke[e] * aol[e] / s;
+      if (kind == 0) diag += t;
+      if (q >= 0) vals[q] -= t;
+    }
+  }
+  if (kind == 0) vals[dpos[j]] += diag;
+}
+
 // ------------------------------------------------------------------ AMG numeric setup
 
 // P(i,:) = e_agg(i) - w invd_i sum_k a_ik e_agg(k), w = 4/3 (l1 diagonal,

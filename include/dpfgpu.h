@@ -195,6 +195,29 @@ int dpf_residuals(dpf_topology *topo, const double *hfree, const double *Ks,
                   const double *dirichleths, const double *Qs, const double *specificstorage,
                   const double *volumes, double *r_out, const dpf_opts *opts);
 
+/* Jacobians of groundwater_residuals as sparse matrices over the free cells:
+ * the groundwater_h, groundwater_Ks, groundwater_dirichleths and
+ * groundwater_Qs that the @NonlinearEquations.equations macro generates
+ * (src/groundwater.jl:1 for groundwater_residuals at :1-38), the f_gw_u /
+ * f_gw_p blocks of examples/gw_transient_3d/inputdeck.jl:82-91.  CSR, rows =
+ * free cells in order, columns 0-based and ascending, contributions to one
+ * entry summed (as sparse() sums duplicates):
+ *   DPF_JAC_H            nfree x nfree   dr_j/dh_k       (reads Ks, Ss, V)
+ *   DPF_JAC_KS           nfree x nedges  dr_j/dKs_e      (reads hfree, dirichleths, Ss, V)
+ *   DPF_JAC_DIRICHLETHS  nfree x n       dr_j/ddirichleths_i (reads Ks, Ss, V)
+ *   DPF_JAC_QS           nfree x n       dr_j/dQs_i      (reads Ss, V)
+ * The pattern is structural (an entry per edge term, even where K_e = 0);
+ * self-edges (a == b), whose terms vanish, add none.
+ * dpf_jacobian_pattern: *nnz, and where not NULL rowptr (nfree + 1) and cols
+ * (nnz), host arrays.  dpf_jacobian_values: the nnz values in pattern order
+ * (arrays in opts->memory, like dpf_residuals). */
+enum { DPF_JAC_H = 0, DPF_JAC_KS = 1, DPF_JAC_DIRICHLETHS = 2, DPF_JAC_QS = 3 };
+int dpf_jacobian_pattern(dpf_topology *topo, int32_t which, int64_t *nnz, int64_t *rowptr,
+                         int32_t *cols);
+int dpf_jacobian_values(dpf_topology *topo, int32_t which, const double *hfree,
+                        const double *Ks, const double *dirichleths, const double *Ss,
+                        const double *V, double *vals, const dpf_opts *opts);
+
 /* Edge conductivity from cell log-conductivity, K_e = hm(exp(logK_a),
  * exp(logK_b)) with DPFEHM.hm (src/richards.jl:50-52), and its pullback
  * dlogK = (dK_e/dlogK)' dK (the scatter Zygote performs for the user-side
