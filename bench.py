@@ -225,6 +225,31 @@ def main():
     log(f"[bench] mixed vs fp64: head rel err {head_err:.2e}, gradient rel err {grad_err:.2e}; "
         f"mixed_pc vs fp64: {pc_err[0]:.2e}, {pc_err[1]:.2e}")
 
+    # the same evaluation through the host-buffer interface (numpy arrays in
+    # and out: logK, K_e, heads, seed and gradient cross PCIe), mixed
+    # precision, for the PCIe-inclusive rate; never the headline value
+    def host_eval(k):
+        lk = fields[k % len(fields)]
+        K = api.logKs2Ks_harmonic(lk, w.neighbors, w.dirichletnodes, w.areasoverlengths)
+        h, pb = api.rrule_groundwater_steadystate(K, w.neighbors, w.areasoverlengths,
+                                                  w.dirichletnodes, w.dirichleths, w.Qs,
+                                                  reltol=1e-10, precision="mixed",
+                                                  autotune=args.autotune)
+        delta = np.zeros(w.n)
+        delta[w.obs - 1] = 2.0 * (h[w.obs - 1] - h_obs_host)
+        dK = pb(delta)[1]
+        pb.state.close()
+        return api.logKs2Ks_harmonic_pullback(lk, K, dK, w.neighbors, w.dirichletnodes,
+                                              w.areasoverlengths)
+    h_obs_host = h_obs_d.cpu().numpy()
+    host_eval(0)
+    t0 = time.perf_counter()
+    nh = 3
+    for k in range(nh):
+        host_eval(k)
+    host_rate = nh / (time.perf_counter() - t0)
+    log(f"[bench] mixed through host buffers (PCIe included): {host_rate:.2f} evals/s")
+
     # kernel table: the same evaluations again with every launch timed by a
     # dispatch event pair on the stream it runs on, per kernel class, AMG
     # level and value size, with each launch's algorithmic bytes
@@ -289,6 +314,7 @@ def main():
         "fp64_evals_per_s": round(fp64_rate, 3),
         "mixed_over_fp64": round(value / fp64_rate, 3),
         "mixed_pc_evals_per_s": round(args.steps / results["mixed_pc"]["seconds"], 3),
+        "host_buffers_mixed_evals_per_s": round(host_rate, 3),
         "mixed_pc_rel_err_head_grad": pc_err,
         "head_rel_err_mixed_vs_fp64": head_err,
         "grad_rel_err_mixed_vs_fp64": grad_err,
