@@ -1558,11 +1558,17 @@ int dpf_topology_set_lanes(dpf_topology* t, int32_t level, int32_t lg) {
             "and a lanes exponent in 1..6");
     return DPF_EINVAL;
   }
+  int prev = -1;
   try {
+    // the topology's device is made current for the re-layout and the
+    // caller's current device restored afterwards
+    HIPTRY(hipGetDevice(&prev));
     HIPTRY(hipSetDevice(t->device));
     set_level_lanes(t, level, lg, t->own_stream);
     HIPTRY(hipStreamSynchronize(t->own_stream));
+    HIPTRY(hipSetDevice(prev));
   } catch (const Err& e) {
+    if (prev >= 0) (void)hipSetDevice(prev);
     set_err(e.msg);
     return e.code;
   }
