@@ -1075,7 +1075,8 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   // one iteration's launches (iteration it, for the timing records); the
   // V-cycle's output buffer is the same on every call, so z does not change
   // from one iteration to the next
-  auto body = [&](int pr, int it) {
+  // (in two parts: the CG update up to the convergence test, then the V-cycle)
+  auto body_a = [&](int pr, int it) {
     t->kiter = it;
     t->kphase = 0;
     klaunch(t, KC_CG_PQ, 0, (int)wk, b_pq, k_cg_pq<TK, TP>, dim3(g), dim3(BLK), 0, s, A,
@@ -1086,10 +1087,16 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
             (const TK*)pb[pr ^ 1], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0, (const int*)done,
             (const SolveScalars*)sc, bm_ptr(t), part);
     finish(t, s, FIN_XR, g);
+  };
+  auto body_b = [&]() {
     TP* zz = nullptr;
     t->kphase = 1;
     vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
     if (zz != z) throw Err{DPF_EHIP, "V-cycle output buffer moved between iterations"};
+  };
+  auto body = [&](int pr, int it) {
+    body_a(pr, it);
+    body_b();
   };
   bool graphs = t->use_graphs && !t->timing && !t->ktiming;
   std::string gkey;
@@ -1143,6 +1150,15 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   int launched = 0;
   double t0 = now_wall_ms();
   unsigned spins = 0;
+  // Convergence forecast (single systems, launched iterations): from rr of
+  // the last two published iterations; an iteration forecast to reach the
+  // stop threshold is queued without its V-cycle, which follows only once its
+  // convergence test has come back negative.  A converged solve then leaves
+  // no V-cycle of no-op launches behind; a wrong forecast costs one host
+  // round trip.
+  int seen = -1;
+  double rr_a = -1.0, rr_b = -1.0;
+  bool half = false;  // the last queued iteration's V-cycle is not queued
   while (true) {
     int its = 0, dn = 0;
     if (mirror_load(&m->seq) == seq) {
@@ -1155,11 +1171,31 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
       res.rr = mirror_load(&m->rr);
       break;
     }
-    if (launched < maxiter && launched <= its) {
+    if (its != seen && its > 0) {
+      seen = its;
+      rr_a = rr_b;
+      rr_b = mirror_load(&m->rr);
+    }
+    if (half) {
+      if (its >= launched) {  // its test came back and the solve goes on
+        body_b();
+        half = false;
+        HIPTRY(hipGetLastError());
+        continue;
+      }
+    } else if (launched < maxiter && launched <= its) {
+      bool forecast = false;
+      if (!graphs && t->nm == 1 && rr_a > 0.0 && rr_b > 0.0 && rr_b < rr_a) {
+        const double stop = mirror_load(&m->rr_stop);
+        forecast = stop > 0.0 && rr_b * (rr_b / rr_a) <= stop;
+      }
       if (graphs)
         HIPTRY(hipGraphLaunch(graph_for(par), s));
+      else if (forecast)
+        body_a(par, launched + 1);
       else
         body(par, launched + 1);
+      half = forecast;
       par ^= 1;
       launched++;
       HIPTRY(hipGetLastError());

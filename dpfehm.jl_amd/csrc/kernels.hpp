@@ -35,6 +35,7 @@ struct HostMirror {
   int active;     // members whose refinement residual is above their target
   int cut;        // set by the assembly when an edge's conductance K_e aol_e is 0
   double rr;
+  double rr_stop;  // a single system's stop threshold on rr (FIN_START), -1 for a batch
 };
 
 __device__ __forceinline__ void mput(int* p, int v) {
@@ -468,7 +469,10 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
     mput(&hm->iters, maxit);
     mput(&hm->done, dn);
   }
-  if (mode == FIN_START) mpublish(&hm->seq, seq);
+  if (mode == FIN_START) {
+    mput(&hm->rr_stop, nm == 1 ? sc[0].rr_stop : -1.0);
+    mpublish(&hm->seq, seq);
+  }
 }
 
 // Start and width of row r's SELL entries.  Every caller runs the 64 rows of
