@@ -1410,6 +1410,9 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     t->A64.alloc(t->lev[0]->A.entries * sizeof(double));
     t->invd64.alloc(H.nf * sizeof(double));
     t->b64.alloc(H.nf * sizeof(double));
+    // pooled: a forward that keeps a state hands these buffers to the state
+    // and takes fresh ones from the pool
+    for (DBuf* b : {&t->Ke, &t->dh, &t->Q}) b->pool = t->pool.get();
     t->Ke.alloc(H.ne * sizeof(double));
     t->dh.alloc(H.n * sizeof(double));
     t->Q.alloc(H.n * sizeof(double));
@@ -2352,12 +2355,17 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
     if (st) {
       st->version = t->version;
       st->precision = o->precision;
-      st->Ke.alloc(ne * sizeof(double));
-      st->dh.alloc(n * sizeof(double));
-      st->Q.alloc(n * sizeof(double));
-      copy_dev(st->Ke.p, t->Ke.p, ne * sizeof(double), s);
-      copy_dev(st->dh.p, t->dh.p, n * sizeof(double), s);
-      copy_dev(st->Q.p, t->Q.p, n * sizeof(double), s);
+      // the state takes this call's K_e, dirichleths and Q buffers as they
+      // are (no copies; the kernels queued on them keep valid memory) and the
+      // topology takes fresh ones from the shared pool for its next call
+      for (auto pr : {std::make_pair(&st->Ke, &t->Ke), std::make_pair(&st->dh, &t->dh),
+                      std::make_pair(&st->Q, &t->Q)}) {
+        std::swap(pr.first->p, pr.second->p);
+        std::swap(pr.first->bytes, pr.second->bytes);
+      }
+      t->Ke.alloc(ne * sizeof(double));
+      t->dh.alloc(n * sizeof(double));
+      t->Q.alloc(n * sizeof(double));
     }
     HIPTRY(hipStreamSynchronize(s));
     if (stats) {
