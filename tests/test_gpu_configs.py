@@ -113,7 +113,8 @@ def test_c3_2m_sigma_sweep(pkg, api, sigma):
 def test_c5a_ensemble_96_members(pkg, api, oracle):
     """C5a: 96 realizations (32 x 64^3, 32 x 96x96x24, 32 x 128x128x16; 23.9 M
     cells) packed into one batch topology, forward + adjoint in mixed
-    precision with hipGraph-replayed iterations."""
+    precision with hipGraph-replayed iterations and the autotuner (lanes per
+    row of the coarse levels, smoother degrees) on its first solve."""
     _say("c5a build")
     ms = pkg.workloads.ensemble_members()
     topo = api.BatchTopology([(m["n"], m["nb"], m["aol"], m["dn"]) for m in ms])
@@ -124,8 +125,9 @@ def test_c5a_ensemble_96_members(pkg, api, oracle):
     Q = np.concatenate([m["Q"] for m in ms])
     st = {}
     h, pb = api.rrule_batch_steadystate(topo, Ke, dh, Q, reltol=1e-10, precision="mixed",
-                                        graph=True, stats=st)
-    _say("c5a forward", st)
+                                        graph=True, autotune=True, stats=st)
+    _say("c5a forward", st, "lanes", topo.lanes()[0], "degrees", topo.tuned_degrees()["mixed"])
+    assert topo.lanes()[1], "the autotuner timed no lanes-per-row layout"
     relres = topo.member_relres()
     assert np.all(relres <= 1e-10), relres.max()
     delta = np.zeros(topo.n)
