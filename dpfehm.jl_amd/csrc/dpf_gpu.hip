@@ -316,6 +316,8 @@ struct dpf_topology {
   int device = 0;
   std::shared_ptr<BufPool> pool;       // reused state and temporary buffers
   hipEvent_t ev_setup[2] = {nullptr, nullptr};  // GPU time of a call's numeric setup
+  // ordering of DPF_DEVICE calls with a NULL stream against the NULL stream
+  hipEvent_t ev_null[2] = {nullptr, nullptr};
   // topology arrays on the device
   DBuf edges, aol, n2f, f2n, inc_ptr, inc_edge, inc_other, inc_slot;
   std::vector<std::unique_ptr<LevelDev>> lev;
@@ -442,10 +444,38 @@ namespace {
 
 void set_err(const std::string& m) { g_err = m; }
 
-hipStream_t pick_stream(dpf_topology* t, const dpf_opts* o) {
-  if (o && o->memory == DPF_DEVICE && o->stream) return (hipStream_t)o->stream;
-  return t->own_stream;
-}
+// The stream a call queues its work on.  DPF_DEVICE with a stream: that
+// stream.  Otherwise the topology's own (non-blocking) stream; in DPF_DEVICE
+// mode with a NULL stream the call is then ordered as if it ran on the legacy
+// NULL stream: its work starts after everything the NULL stream holds when
+// the call begins (the caller's device inputs may still be being written
+// there), and work the caller queues on the NULL stream after the call
+// returns starts after the call's work -- two event waits, no host
+// synchronisation.  Construct it with the topology's lock held.
+struct CallStream {
+  dpf_topology* t;
+  hipStream_t s;
+  bool order_back = false;
+  CallStream(dpf_topology* t_, const dpf_opts* o) : t(t_) {
+    if (o && o->memory == DPF_DEVICE && o->stream) {
+      s = (hipStream_t)o->stream;
+      return;
+    }
+    s = t->own_stream;
+    if (o && o->memory == DPF_DEVICE) {
+      HIPTRY(hipEventRecord(t->ev_null[0], nullptr));
+      HIPTRY(hipStreamWaitEvent(s, t->ev_null[0], 0));
+      order_back = true;
+    }
+  }
+  ~CallStream() {
+    if (!order_back) return;
+    if (hipEventRecord(t->ev_null[1], s) == hipSuccess)
+      (void)hipStreamWaitEvent(nullptr, t->ev_null[1], 0);
+  }
+  CallStream(const CallStream&) = delete;
+  CallStream& operator=(const CallStream&) = delete;
+};
 
 // Event pair around one timed launch.
 hipEvent_t timing_event(dpf_topology* t) {
@@ -1314,6 +1344,7 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     (void)scratch_on(t->device);
     HIPTRY(hipStreamCreateWithFlags(&t->own_stream, hipStreamNonBlocking));
     for (auto& ev : t->ev_setup) HIPTRY(hipEventCreate(&ev));
+    for (auto& ev : t->ev_null) HIPTRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     // room for a few forward states (K_e, dh, Q, heads, multipliers) and the
     // per-call temporaries of the host-memory interface
     size_t state_bytes = (size_t)(t->H.ne + 3 * t->H.n + 2 * t->H.nf) * sizeof(double);
@@ -1525,6 +1556,8 @@ int dpf_topology_destroy(dpf_topology* t) {
     std::lock_guard<std::mutex> g(t->mu);
     if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
     for (auto& ev : t->ev_setup)
+      if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : t->ev_null)
       if (ev) (void)hipEventDestroy(ev);
     if (t->scal_host) (void)hipHostFree(t->scal_host);
     if (t->hm_host) (void)hipHostFree(t->hm_host);
@@ -1984,7 +2017,8 @@ int dpf_residuals(dpf_topology* t, const double* hfree, const double* Ks,
   }
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, nf = t->H.nf, ne = t->H.ne;
     const double* dh = stage_in(t, o, t->stage_n, dirichleths, n, s);
     const double* Q = stage_in(t, o, t->stage_n2, Qs, n, s);
@@ -2103,7 +2137,8 @@ int dpf_jacobian_values(dpf_topology* t, int32_t which, const double* hfree, con
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     build_jac_maps(t);
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     if (!t->jac_dev) {
       for (int k = 0; k < 4; k++) upload(t->jac_rp_d[k], t->jac_rp[k]);
       for (int k = 0; k < 3; k++) upload(t->jac_pos_d[k], t->jac_pos[k]);
@@ -2156,7 +2191,8 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     const double* lk = stage_in(t, o, t->stage_n, logK, t->H.n, s);
     double* out = Ke;
     if (o->memory == DPF_HOST) {
@@ -2200,7 +2236,8 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     const double* lk = geometric ? nullptr : stage_in(t, o, t->stage_n, logK, t->H.n, s);
     const double* ke = stage_in(t, o, t->stage_e, Ke, t->H.ne, s);
     DBuf dkb(t->pool.get());
@@ -2272,7 +2309,8 @@ int dpf_hm_forward_noalloc(dpf_topology* t, const double* logK, double* Ke,
     // the address was looked up when the topology was created on its device
     double* kb = g_scratch[t->device];
     if (!kb) throw Err{DPF_EHIP, "no staging address for the topology's device"};
-    hipStream_t s = pick_stream(t, opts);
+    CallStream cs(t, opts);
+    hipStream_t s = cs.s;
     const int64_t n = t->H.n, ne = t->H.ne;
     // the same two kernels as dpf_hm_forward, with the staging in `scratch`;
     // k_exp writes every entry k_hm_forward reads, so nothing is cleared
@@ -2310,7 +2348,8 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     // inputs -> topology-owned device buffers
     if (o->memory == DPF_DEVICE) {
@@ -2400,7 +2439,8 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     // the hierarchy must hold this state's conductivities in the requested precision
     ktiming_begin(t, o);
@@ -2703,7 +2743,8 @@ extern "C" int dpf_profile_kernel(dpf_topology* t, int32_t precision, int32_t ke
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     if (precision != DPF_FP64) {
       if (!t->h32.allocated || t->h32.version == 0) throw Err{DPF_EINVAL, "no FP32 solve yet"};
       *avg_ms = profile_one<float>(t, t->h32, s, kernel, reps, bytes);
@@ -2806,7 +2847,8 @@ int transient_forward_impl(dpf_topology* t, const double* Ks, const double* diri
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     auto st = std::make_unique<dpf_state>(t);
     st->kind = 1;
@@ -2956,7 +2998,8 @@ extern "C" int dpf_transient_pullback_seeds(dpf_state* st, int32_t nseed,
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     BufPool* bp = t->pool.get();
     // the seeds on the device, and per step the seed row (null: no seed)
@@ -3248,7 +3291,8 @@ int dpf_twophase_tpfa(dpf_topology* t, const double* K, const double* dirichleth
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne;
     DBuf kb(t->pool.get()), pb(t->pool.get()), vb(t->pool.get());
     const double* Kd = dev_in(t, o, kb, K, n, s);
@@ -3294,7 +3338,8 @@ int dpf_twophase_upstream(dpf_topology* t, double* S, const double* fluid, const
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne;
     DBuf sb(t->pool.get()), qb(t->pool.get()), pb(t->pool.get()), vb(t->pool.get()),
         wb(t->pool.get());
@@ -3341,7 +3386,8 @@ int dpf_solvetwophase(dpf_topology* t, const double* h0, const double* S0, const
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne;
     bool host = o->memory == DPF_HOST;
     auto out = [&](double* dst, const double* src, int64_t count) {
@@ -3492,7 +3538,8 @@ extern "C" int dpf_solvetwophase_pullback(dpf_state* st, const double* dP_final,
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     int64_t ninc = t->H.inc_ptr.back();
     int nt = st->nsteps;
@@ -3885,7 +3932,8 @@ extern "C" int dpf_richards_steady(dpf_topology* t, const double* psi0, const do
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     bool fp32 = o->precision != DPF_FP64;  // FP32 V-cycle preconditioner
     t->timing = false;
@@ -4013,7 +4061,8 @@ extern "C" int dpf_richards_pullback(dpf_state* st, const double* delta, double*
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double t0 = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     bool fp32 = o->precision != DPF_FP64;  // FP32 V-cycle preconditioner
     bool host = o->memory == DPF_HOST;
@@ -4108,7 +4157,8 @@ extern "C" int dpf_richards_residuals(dpf_topology* t, const double* psi_free, c
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne, nf = t->H.nf;
     BufPool* pl = t->pool.get();
     DBuf b1(pl), b2(pl), b3(pl), b4(pl), b5(pl), b6(pl), b7(pl), b8(pl), b9(pl), F(pl), sv(pl),
@@ -4233,7 +4283,8 @@ extern "C" int dpf_transport_residuals(dpf_topology* t, const double* u, const d
   const dpf_opts* o = opts ? opts : &dflt;
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n;
     BufPool* pl = t->pool.get();
     TrBufs b(pl);
@@ -4281,7 +4332,8 @@ extern "C" int dpf_transport_steps(dpf_topology* t, const double* u0, const doub
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double tstart = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n;
     bool host = o->memory == DPF_HOST;
     BufPool* pl = t->pool.get();
@@ -4364,7 +4416,8 @@ extern "C" int dpf_transport_pullback(dpf_state* st, const double* dJ_du_final, 
   std::lock_guard<std::mutex> g(t->mu);
   return guarded([&]() {
     double tstart = now_ms();
-    hipStream_t s = pick_stream(t, o);
+    CallStream cs(t, o);
+    hipStream_t s = cs.s;
     int64_t n = t->H.n, ne = t->H.ne;
     bool host = o->memory == DPF_HOST;
     BufPool* pl = t->pool.get();

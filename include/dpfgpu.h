@@ -9,9 +9,11 @@
  *
  * Array arguments of compute calls are host pointers when opts->memory is
  * DPF_HOST (the library stages them through device buffers) and device
- * pointers when it is DPF_DEVICE (work is queued on opts->stream, or on the
- * topology's own stream when that is NULL).  Calls on one topology are
- * serialised by the library.
+ * pointers when it is DPF_DEVICE (work is queued on opts->stream; when that
+ * is NULL, on the topology's own stream, ordered after the work already on
+ * the legacy NULL stream when the call starts and ahead of the NULL-stream
+ * work queued after it returns, as if the call ran on the NULL stream).
+ * Calls on one topology are serialised by the library.
  */
 #ifndef DPFGPU_H
 #define DPFGPU_H
@@ -84,7 +86,7 @@ typedef struct {
                           pinned, or autotuned with DPF_FLAG_AUTOTUNE */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
   int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
-  void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = topology stream */
+  void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = NULL-stream order */
 } dpf_opts;
 
 typedef struct {

@@ -571,6 +571,87 @@ def test_user_stream_call_then_topology_stream_call(pkg, api, oracle):
         hip.hipStreamDestroy(stream)
 
 
+def test_null_stream_device_calls_follow_null_stream_order(pkg, api, oracle):
+    """Device-memory calls with a NULL stream run in the order of the legacy
+    NULL stream: a pullback whose seed the caller writes on the NULL stream
+    behind ~2 GB of copies reads the written seed, not the zeros before it;
+    and NULL-stream work queued right after an allocation-free logK map (which
+    does not synchronise) sees its output.  Device memory and copies come from
+    the HIP runtime the library itself links."""
+    import ctypes
+
+    from dpfehm_jl_amd import _lib
+    _say("null stream order")
+    L = _lib.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+
+    def ok(code):
+        assert code == 0, code
+
+    w = pkg.workloads.steady_2m(ns=(64, 64, 32), maxs=(32.0, 32.0, 16.0))
+    topo = api.get_topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    n, ne = w.n, topo.ne
+    K = oracle.hm_forward(w.logK, w.neighbors)
+    h_ref, pb_ref = oracle.steadystate_rrule(K, w.neighbors, w.areasoverlengths,
+                                             w.dirichletnodes, w.dirichleths, w.Qs, reltol=1e-14)
+    delta = np.zeros(n)
+    delta[w.obs - 1] = 2.0 * h_ref[w.obs - 1]
+    dK_ref = pb_ref(delta)[1]
+    big = 1 << 30
+    bufs = {}
+    try:
+        for name, nb in (("Ke", 8 * ne), ("dh", 8 * n), ("Q", 8 * n), ("h", 8 * n),
+                         ("delta", 8 * n), ("delta_src", 8 * n), ("dK", 8 * ne),
+                         ("lk", 8 * n), ("a", big), ("b", big)):
+            bufs[name] = vp()
+            ok(hip.hipMalloc(ctypes.byref(bufs[name]), sz(nb)))
+        for name, arr in (("Ke", K), ("dh", w.dirichleths), ("Q", w.Qs), ("delta_src", delta),
+                          ("lk", w.logK)):
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            ok(hip.hipMemcpy(bufs[name], a.ctypes.data_as(vp), sz(a.nbytes), 1))
+        ok(hip.hipMemset(bufs["delta"], 0, sz(8 * n)))
+        ok(hip.hipMemset(bufs["a"], 0, sz(big)))
+        ok(hip.hipDeviceSynchronize())
+        for precision in ("fp64", "mixed"):
+            o = api._opts({"precision": precision, "reltol": 1e-12}, memory=_lib.DPF_DEVICE,
+                          stream=None)
+            st = _lib.Stats()
+            sh = vp()
+            _lib.check(L.dpf_steady_forward(topo.handle, bufs["Ke"], bufs["dh"], bufs["Q"],
+                                            bufs["h"], ctypes.byref(o), ctypes.byref(st),
+                                            ctypes.byref(sh)))
+            try:
+                ok(hip.hipMemset(bufs["delta"], 0, sz(8 * n)))
+                ok(hip.hipDeviceSynchronize())
+                # the seed is written on the NULL stream behind 2 GiB of copies
+                for _ in range(2):
+                    ok(hip.hipMemcpyAsync(bufs["b"], bufs["a"], sz(big), 3, vp()))
+                ok(hip.hipMemcpyAsync(bufs["delta"], bufs["delta_src"], sz(8 * n), 3, vp()))
+                st2 = _lib.Stats()
+                _lib.check(L.dpf_steady_pullback(sh, bufs["delta"], bufs["dK"], None, None,
+                                                 ctypes.byref(o), ctypes.byref(st2)))
+            finally:
+                L.dpf_state_destroy(sh)
+            dK = np.empty(ne)
+            ok(hip.hipMemcpy(dK.ctypes.data_as(vp), bufs["dK"], sz(8 * ne), 2))
+            assert st2.iterations > 0
+            assert rel(dK, dK_ref) < (1e-8 if precision == "fp64" else 1e-6), precision
+        # NULL-stream work queued right after a call that returns without
+        # synchronising
+        o = api._opts({}, memory=_lib.DPF_DEVICE, stream=None)
+        Ke = np.empty(ne)
+        ok(hip.hipMemset(bufs["Ke"], 0, sz(8 * ne)))
+        ok(hip.hipDeviceSynchronize())
+        _lib.check(L.dpf_hm_forward_noalloc(topo.handle, bufs["lk"], bufs["Ke"], ctypes.byref(o)))
+        ok(hip.hipMemcpyAsync(Ke.ctypes.data_as(vp), bufs["Ke"], sz(8 * ne), 2, vp()))
+        ok(hip.hipStreamSynchronize(vp()))
+        assert rel(Ke, K) < 1e-14
+    finally:
+        for b in bufs.values():
+            hip.hipFree(b)
+
+
 def test_steady_singular_systems_raise(pkg, api):
     """A steady solve on a mesh without Dirichlet cells, or with a K_e = 0
     cut that floats a layer of cells, raises SingularSystem (DPF_ESINGULAR)
