@@ -824,7 +824,8 @@ KERNEL_SYMBOL = {"cheb_sweep": "k_cheb", "residual": "k_residual", "restrict": "
                  "prolong": "k_prolong", "prolong_coarse_solve": "k_prolong_coarse",
                  "cg_pq": "k_cg_pq", "cg_xr": "k_cg_xr", "finish": "k_finish",
                  "outer_residual": "k_outer_residual", "galerkin_ap": "k_prod_gather",
-                 "galerkin_rap": "k_prod_gather", "assemble": "k_assemble"}
+                 "galerkin_rap": "k_prod_gather", "galerkin_ap_rows": "k_ap_rows",
+                 "galerkin_rap_rows": "k_rap_rows", "assemble": "k_assemble"}
 
 
 def kernel_label(r):

@@ -232,6 +232,11 @@ struct LevelDev {
   int64_t n_rap = 0, n_mirror = 0;  // Galerkin lists (upper entries), mirrored entries
   int ap_lg = 0, rap_lg = 0;
   int64_t ap_nnz = 0;
+  // row-wise Galerkin products (GalerkinPlan): A*P and/or R*(A*P) by rows
+  bool ap_rows = false, rap_rows = false;
+  int apr_lg = 0, ap_w = 0, rap_w = 0;  // lanes per fine row of the row-wise A*P
+  int64_t ap_prods = 0, rap_prods = 0;
+  DBuf ap_rp, apc_ptr, apc, rcd_ptr, rcd, rnp;
 };
 
 // Numeric values and work vectors of a hierarchy in precision T.
@@ -287,13 +292,14 @@ enum KCat {
   KC_CHEB, KC_RESIDUAL, KC_RESTRICT, KC_PROLONG, KC_PROLONG_COARSE, KC_COARSE_SOLVE,
   KC_RZ_DOT, KC_CG_START, KC_CG_PQ, KC_CG_XR, KC_FINISH, KC_OUTER_RESIDUAL, KC_REFINE_VEC,
   KC_ASSEMBLE, KC_PSMOOTH, KC_RGATHER, KC_PROD_AP, KC_PROD_RAP, KC_MIRROR, KC_ROWDIAG,
-  KC_DENSE_INVERSE, KC_KMAP, KC_GRADIENT, KC_SCATTER, KC_NCAT
+  KC_DENSE_INVERSE, KC_KMAP, KC_GRADIENT, KC_SCATTER, KC_AP_ROWS, KC_RAP_ROWS, KC_NCAT
 };
 static const char* const kcat_name[KC_NCAT] = {
   "cheb_sweep", "residual", "restrict", "prolong", "prolong_coarse_solve", "coarse_solve",
   "rz_dot", "cg_start", "cg_pq", "cg_xr", "finish", "outer_residual", "refine_vector",
   "assemble", "psmooth", "rgather", "galerkin_ap", "galerkin_rap", "mirror_copy", "rowdiag",
-  "dense_inverse", "k_map", "gradient", "scatter_gather"};
+  "dense_inverse", "k_map", "gradient", "scatter_gather", "galerkin_ap_rows",
+  "galerkin_rap_rows"};
 
 struct KRec {
   int cat, level, w, iter, phase;
@@ -693,14 +699,42 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
             dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries, d.rs_src.as<int32_t>(),
             (const T*)P.val, h.lev[l].rval.template as<T>(), (const uint16_t*)nullptr,
             h.lev[l].r16.template as<uint16_t>());
-    klaunch(t, KC_PROD_AP, l, (int)w, nap * (8.0 + 2.0 * w) + (double)d.ap_nnz * (8.0 + w),
-            k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s, d.ap_nnz,
-            d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr, d.ap_a.as<int32_t>(),
-            d.ap_p.as<int32_t>(), (const T*)A.val, (const T*)P.val, apv);
-    klaunch(t, KC_PROD_RAP, l, (int)w, nrap * (8.0 + 2.0 * w) + (double)d.n_rap * (12.0 + w),
-            k_prod_gather<T>, dim3(nblocks(d.n_rap << d.rap_lg)), dim3(BLK), 0, s, d.n_rap,
-            d.rap_lg, d.rap_off.as<int64_t>(), d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(),
-            d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
+    if (d.ap_rows) {
+      // one fine row per lane: the row's A entries and, per entry, the P row
+      // it selects (value, start, length), a code byte and a P value per
+      // product, the row-wise AP entries written
+      klaunch(t, KC_AP_ROWS, l, (int)w,
+              z.nnz * (w + 12.0) + (double)d.ap_prods * (1.0 + w) + (double)d.ap_nnz * w,
+              k_ap_rows<T>, dim3(nblocks(d.A.nrows << d.apr_lg)), dim3(BLK),
+              (size_t)d.ap_w * BLK * sizeof(T), s, A,
+              d.A.offsets ? (const int32_t*)nullptr : d.A.rowlen.as<int32_t>(), d.apr_lg,
+              d.ap_w, P, d.P.rowlen.as<int32_t>(), d.apc_ptr.as<int64_t>(),
+              d.apc.as<uint8_t>(), d.ap_rp.as<int64_t>(), apv);
+    } else {
+      klaunch(t, KC_PROD_AP, l, (int)w, nap * (8.0 + 2.0 * w) + (double)d.ap_nnz * (8.0 + w),
+              k_prod_gather<T>, dim3(nblocks(d.ap_nnz << d.ap_lg)), dim3(BLK), 0, s, d.ap_nnz,
+              d.ap_lg, d.ap_off.as<int64_t>(), (const int32_t*)nullptr, d.ap_a.as<int32_t>(),
+              d.ap_p.as<int32_t>(), (const T*)A.val, (const T*)P.val, apv);
+    }
+    if (d.rap_rows) {
+      // coarse rows over Rs's lanes: Rs's entries (column, P-value map, P
+      // value, AP row start), a product code and an AP value per product, the
+      // upper entries written
+      klaunch(t, KC_RAP_ROWS, l, (int)w,
+              (double)d.Rs.entries * (14.0 + w) + (double)d.rap_prods * (2.0 + w) +
+                  (double)d.n_rap * w,
+              k_rap_rows<T>, dim3(nblocks(d.Rs.nrows)), dim3(BLK),
+              (size_t)d.rap_w * BLK * sizeof(T), s, d.Rs.view<T>(nullptr),
+              d.Rs.rowlen.as<int32_t>(), d.rs_lg, d.rap_w, dn.A.nrows, d.rs_src.as<int32_t>(),
+              d.rcd_ptr.as<int64_t>(), d.rcd.as<uint16_t>(), d.rnp.as<uint16_t>(),
+              (const T*)P.val, d.ap_rp.as<int64_t>(), (const T*)apv, dn.A.sptr.as<int64_t>(),
+              dn.A.rowlen.as<int32_t>(), dn.diag_slot.as<uint16_t>(), cval);
+    } else {
+      klaunch(t, KC_PROD_RAP, l, (int)w, nrap * (8.0 + 2.0 * w) + (double)d.n_rap * (12.0 + w),
+              k_prod_gather<T>, dim3(nblocks(d.n_rap << d.rap_lg)), dim3(BLK), 0, s, d.n_rap,
+              d.rap_lg, d.rap_off.as<int64_t>(), d.rap_dest.as<int32_t>(), d.rap_p.as<int32_t>(),
+              d.rap_ap.as<int32_t>(), (const T*)P.val, (const T*)apv, cval);
+    }
     if (d.n_mirror > 0)
       klaunch(t, KC_MIRROR, l + 1, (int)w, (double)d.n_mirror * (8.0 + 2.0 * w),
               k_mirror_copy<T>, dim3(nblocks(d.n_mirror)), dim3(BLK), 0, s, d.n_mirror,
@@ -1408,8 +1442,29 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
         d->ap_lg = lh.ap_lg;
         d->rap_lg = lh.rap_lg;
         upload(d->rap_p, lh.rap_p);
-        upload(d->rap_ap, lh.rap_ap);
         d->ap_nnz = lh.ap_rp.back();
+        GalerkinPlan G;
+        std::string ge = build_galerkin_plan(H, (int)l, fl ? &F : nullptr, G);
+        if (!ge.empty()) throw Err{DPF_EINVAL, ge};
+        d->ap_rows = G.ap_rows;
+        d->rap_rows = G.rap_rows;
+        d->apr_lg = G.ap_lg;
+        d->ap_w = G.ap_w;
+        d->rap_w = G.rap_w;
+        if (G.ap_rows) {
+          d->ap_prods = (int64_t)G.apc.size();
+          upload(d->ap_rp, lh.ap_rp);
+          upload(d->apc_ptr, G.apc_ptr);
+          upload(d->apc, G.apc);
+        }
+        if (G.rap_rows) {
+          d->rap_prods = (int64_t)G.rcd.size();
+          upload(d->rcd_ptr, G.rcd_ptr);
+          upload(d->rcd, G.rcd);
+          upload(d->rnp, G.rnp);
+        } else {
+          upload(d->rap_ap, lh.rap_ap);
+        }
         maxrows = std::max(maxrows, lh.Rs.nrows);
       }
       t->lev.push_back(std::move(d));

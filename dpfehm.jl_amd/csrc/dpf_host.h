@@ -123,6 +123,43 @@ struct FineLayout {
 };
 void build_fine_layout(const TopologyHost& T, FineLayout& F);
 
+// Row-wise Galerkin products of one level with transfer operators (the GPU
+// kernels k_ap_rows and k_rap_rows), an alternative to the product lists of
+// LevelHost that reads each operator row once instead of gathering two
+// values through two 32-bit indices per product.
+//  * A*P: each fine row i is spread over G = 2^ap_lg lanes (sub-row i G + g);
+//    lane g walks the row's A slots s = g, g + G, ... in the order of the GPU
+//    layout (the fine offset layout on level 0: every slot of an offset slice,
+//    the absent ones too; else the row's entries) and, for the slot's column
+//    j, the entries t < rowlen(j) of P's row j, adding each product into AP
+//    entry apc[.] of the row -- 255 for the products of a slot the row does
+//    not hold (dropped).  apc is a SELL layout over the sub-rows (apc_ptr;
+//    product q of sub-row r at apc_ptr[r/64] + 64 q + r%64).  The G lanes'
+//    partial rows are then summed in a fixed pattern and the row's entries
+//    written in A*P's CSR order (ap_rp, ap_col).
+//  * The next level's entries on and above the diagonal: each coarse row is
+//    spread over the lanes of R's row-split copy (Rs, 2^rs_lg lanes a row; at
+//    most 8 entries a lane).  A lane's products AP(i_t, k) P(i_t, I), over its
+//    entries t (fine row i_t) and the AP entries k of row i_t with column >=
+//    I, are coded t | k << 3 | d << 8 (d: the destination among the coarse
+//    row's entries from its diagonal on) and sorted by d, then t, so that
+//    every destination's products form one run that the lane sums in a
+//    register, in ascending fine row (rcd, a SELL layout over Rs's sub-rows,
+//    rcd_ptr; rnp: products per sub-row).  The lanes' partial rows are then
+//    summed in a fixed pattern.
+struct GalerkinPlan {
+  bool ap_rows = false, rap_rows = false;
+  int ap_lg = 0;            // log2 lanes per fine row of the row-wise A*P
+  int ap_w = 0, rap_w = 0;  // widest AP row, widest upper part of a coarse row
+  std::vector<int64_t> apc_ptr, rcd_ptr;
+  std::vector<uint8_t> apc;
+  std::vector<uint16_t> rcd, rnp;
+};
+// fine: level 0's GPU layout (null: the level's operator is in the host
+// layout, walked through its row lengths).  Returns "" or an error message.
+std::string build_galerkin_plan(const TopologyHost& T, int level, const FineLayout* fine,
+                                GalerkinPlan& G);
+
 // Both return "" on success or an error message.
 std::string build_topology(TopologyHost& T, int64_t n, int64_t ne, const int64_t* nb,
                            const double* aol, int64_t nd, const int64_t* dn);

@@ -709,6 +709,146 @@ void build_fine_layout(const TopologyHost& T, FineLayout& F) {
   for (int64_t q = 0; q < (int64_t)L.ap_a.size(); q++) F.ap_a[q] = (int32_t)emap[L.ap_a[q]];
 }
 
+std::string build_galerkin_plan(const TopologyHost& T, int level, const FineLayout* F,
+                                GalerkinPlan& G) {
+  G = GalerkinPlan();
+  if (level < 0 || level + 1 >= (int)T.levels.size()) return "";
+  const LevelHost& L = T.levels[level];
+  if (L.agg.empty() || L.ap_rp.empty()) return "";
+  const SellHost& A = L.A;
+  const SellHost& P = L.P;
+  const SellHost& R = L.R;
+  const SellHost& Rs = L.Rs;
+  const LevelHost& Nx = T.levels[level + 1];
+  const SellHost& C = Nx.A;
+  const int64_t n = A.nrows, nc = R.nrows;
+  if (F && (int64_t)F->sptr.size() != A.nslices() + 1) return "fine layout does not match level 0";
+  int apw = 0;
+  for (int64_t i = 0; i < n; i++) apw = std::max(apw, (int)(L.ap_rp[i + 1] - L.ap_rp[i]));
+  // row-wise A*P: enough rows to fill the GPU, and rows short enough for the
+  // per-lane accumulators in LDS
+  if (n < 8192 || apw > 32) return "";
+  G.ap_rows = true;
+  G.ap_w = apw;
+  // lanes per fine row: about 8 A slots a lane (level 0's offset slices: one)
+  int lg = 0;
+  if (!F) {
+    double avg = n > 0 ? (double)A.nnz / (double)n : 1.0;
+    while ((1 << lg) * 8 < avg && lg < 3) lg++;
+  }
+  G.ap_lg = lg;
+  const int GL = 1 << lg;
+  const int64_t nsub = n * GL, nss = (nsub + SELL_C - 1) / SELL_C;
+  // the A slots the GPU layout walks for sub-row (i, g): f(column, present)
+  auto walk = [&](int64_t i, int g, auto&& f) {
+    if (F) {
+      const int64_t sl = i / SELL_C;
+      const int lane = (int)(i % SELL_C);
+      const int32_t* so = &F->soff[9 * sl];
+      if (so[0]) {
+        for (int k = 0; k < so[0]; k++) {
+          int32_t c = (int32_t)i + so[1 + k];
+          bool present = false;
+          for (int m = 0; m < A.rowlen[i]; m++)
+            if (A.at(i, m) == c) present = true;
+          f(c, present);
+        }
+      } else {
+        int w = (int)((F->sptr[sl + 1] - F->sptr[sl]) / SELL_C);
+        for (int k = 0; k < w; k++) f(F->col[F->sptr[sl] + (int64_t)k * SELL_C + lane], k < A.rowlen[i]);
+      }
+    } else {
+      for (int k = g; k < A.rowlen[i]; k += GL) f(A.at(i, k), true);
+    }
+  };
+  std::vector<int32_t> nprod(nsub, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < nsub; r++) {
+    int c = 0;
+    walk(r / GL, (int)(r % GL), [&](int32_t j, bool) { c += P.rowlen[j]; });
+    nprod[r] = c;
+  }
+  G.apc_ptr.assign(nss + 1, 0);
+  for (int64_t sl = 0; sl < nss; sl++) {
+    int wc = 0;
+    for (int64_t r = sl * SELL_C; r < std::min<int64_t>(nsub, (sl + 1) * SELL_C); r++)
+      wc = std::max(wc, nprod[r]);
+    G.apc_ptr[sl + 1] = G.apc_ptr[sl] + (int64_t)wc * SELL_C;
+  }
+  G.apc.assign(G.apc_ptr[nss], 255);
+#pragma omp parallel
+  {
+    std::vector<int32_t> slot_of(P.ncols, -1);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t sl = 0; sl < nss; sl++)
+      for (int64_t r = sl * SELL_C; r < std::min<int64_t>(nsub, (sl + 1) * SELL_C); r++) {
+        const int64_t i = r / GL;
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++)
+          slot_of[L.ap_col[a]] = (int32_t)(a - L.ap_rp[i]);
+        int64_t q = 0;
+        const int64_t base = G.apc_ptr[sl] + r % SELL_C;
+        walk(i, (int)(r % GL), [&](int32_t j, bool present) {
+          for (int t = 0; t < P.rowlen[j]; t++, q++)
+            G.apc[base + q * SELL_C] = present ? (uint8_t)slot_of[P.at(j, t)] : (uint8_t)255;
+        });
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) slot_of[L.ap_col[a]] = -1;
+      }
+  }
+  // row-wise R*(A*P) over Rs's sub-rows: enough coarse rows, at most 8 entries
+  // a sub-row, short upper parts and AP rows (a product code holds the
+  // entry in 3 bits, the AP entry in 5 and the destination in 8)
+  int rw = 0;
+  for (int64_t I = 0; I < nc; I++) rw = std::max(rw, C.rowlen[I] - (int)Nx.diag_slot[I]);
+  int rsw = 0;
+  for (int64_t r = 0; r < Rs.nrows; r++) rsw = std::max(rsw, Rs.rowlen[r]);
+  if (nc < 8192 || rw > 32 || apw > 32 || rsw > 8) return "";
+  G.rap_rows = true;
+  G.rap_w = rw;
+  const int rG = 1 << L.rs_lg;
+  const int64_t rsub = Rs.nrows, rss = Rs.nslices();
+  // per sub-row its products, sorted by destination and, within one, by entry
+  // (ascending fine row): every destination's products form one run
+  std::vector<std::vector<uint16_t>> prods(rsub);
+#pragma omp parallel
+  {
+    std::vector<int32_t> slot_of(nc, -1);
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t r = 0; r < rsub; r++) {
+      const int64_t I = r / rG;
+      const int ds = Nx.diag_slot[I];
+      for (int k = ds; k < C.rowlen[I]; k++) slot_of[C.at(I, k)] = k - ds;
+      std::vector<uint16_t>& v = prods[r];
+      for (int t = 0; t < Rs.rowlen[r]; t++) {
+        const int32_t i = Rs.at(r, t);
+        for (int64_t a = L.ap_rp[i]; a < L.ap_rp[i + 1]; a++) {
+          if (L.ap_col[a] < I) continue;
+          const int k = (int)(a - L.ap_rp[i]);
+          v.push_back((uint16_t)(t | (k << 3) | (slot_of[L.ap_col[a]] << 8)));
+        }
+      }
+      std::stable_sort(v.begin(), v.end(), [](uint16_t x, uint16_t y) { return (x >> 8) < (y >> 8); });
+      for (int k = ds; k < C.rowlen[I]; k++) slot_of[C.at(I, k)] = -1;
+    }
+  }
+  G.rnp.assign(rsub, 0);
+  G.rcd_ptr.assign(rss + 1, 0);
+  for (int64_t sl = 0; sl < rss; sl++) {
+    size_t wc = 0;
+    for (int64_t r = sl * SELL_C; r < std::min<int64_t>(rsub, (sl + 1) * SELL_C); r++) {
+      wc = std::max(wc, prods[r].size());
+      if (prods[r].size() > 65535) return "row-wise R*(A*P): too many products in a lane";
+      G.rnp[r] = (uint16_t)prods[r].size();
+    }
+    G.rcd_ptr[sl + 1] = G.rcd_ptr[sl] + (int64_t)wc * SELL_C;
+  }
+  G.rcd.assign(G.rcd_ptr[rss], 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < rsub; r++)
+    for (size_t q = 0; q < prods[r].size(); q++)
+      G.rcd[G.rcd_ptr[r / SELL_C] + (int64_t)q * SELL_C + r % SELL_C] = prods[r][q];
+  return "";
+}
+
 std::string build_hierarchy(TopologyHost& T, int max_levels, int coarse_max, int dense_max,
                             const std::vector<int64_t>& member_row0) {
   auto t0 = std::chrono::steady_clock::now();

@@ -12,6 +12,7 @@ using namespace dpf;
 struct dpf_inspect {
   TopologyHost H;
   FineLayout F;  // the fine operator's GPU layout (hierarchies of two or more levels)
+  std::vector<GalerkinPlan> G;  // per level: the row-wise Galerkin plan the GPU uses
 };
 
 static thread_local std::string g_ierr;
@@ -39,6 +40,9 @@ int dpf_inspect_create(int64_t n, int64_t nedges, const int64_t* neighbors,
                                  dirichletnodes);
   if (e.empty()) e = build_hierarchy(h->H, max_levels, coarse_max, 512);
   if (e.empty() && h->H.levels.size() >= 2) build_fine_layout(h->H, h->F);
+  h->G.resize(h->H.levels.size());
+  for (size_t l = 0; e.empty() && l + 1 < h->H.levels.size(); l++)
+    e = build_galerkin_plan(h->H, (int)l, l == 0 ? &h->F : nullptr, h->G[l]);
   if (!e.empty()) {
     g_ierr = e;
     delete h;
@@ -121,6 +125,13 @@ int with_array(const dpf_inspect* h, int32_t level, const std::string& s, Fn&& f
   if (s == "rap_ap") return f(L.rap_ap);
   if (s == "rap_mdst") return f(L.rap_mdst);
   if (s == "rap_msrc") return f(L.rap_msrc);
+  // the level's row-wise Galerkin plan ("G." names; empty where it uses lists)
+  const GalerkinPlan& G = h->G[level];
+  if (s == "G.apc_ptr") return f(G.apc_ptr);
+  if (s == "G.apc") return f(G.apc);
+  if (s == "G.rcd_ptr") return f(G.rcd_ptr);
+  if (s == "G.rcd") return f(G.rcd);
+  if (s == "G.rnp") return f(G.rnp);
   return -1;
 }
 
@@ -130,12 +141,19 @@ extern "C" {
 
 // Sizes (element counts) of the arrays dpf_inspect_get can return; the
 // pseudo-names "rs_lg" / "as_lg" return log2 of the lanes per row of the
-// split restriction / split level operator.
+// split restriction / split level operator, "G.ap_rows" / "G.rap_rows" 1 when
+// the level computes A*P / R*(A*P) row-wise, "G.ap_lg" log2 of the lanes per
+// fine row of the row-wise A*P.
 int64_t dpf_inspect_size(const dpf_inspect* h, int32_t level, const char* name) {
   std::string s(name);
   if (s == "rs_lg" || s == "as_lg") {
     if (level < 0 || level >= (int32_t)h->H.levels.size()) return -1;
     return s == "rs_lg" ? h->H.levels[level].rs_lg : h->H.levels[level].as_lg;
+  }
+  if (s == "G.ap_rows" || s == "G.rap_rows" || s == "G.ap_lg") {
+    if (level < 0 || level >= (int32_t)h->G.size()) return -1;
+    const GalerkinPlan& G = h->G[level];
+    return s == "G.ap_rows" ? G.ap_rows : (s == "G.rap_rows" ? G.rap_rows : G.ap_lg);
   }
   int64_t out = -1;
   with_array(h, level, s, [&](const auto& v) {

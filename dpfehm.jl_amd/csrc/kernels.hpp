@@ -867,6 +867,171 @@ __global__ void k_prod_gather(int64_t nlists, int lg, const int64_t* __restrict_
   if ((sub & (G - 1)) == G - 1 && k < nlists) out[dest ? dest[k] : k] = acc;
 }
 
+// A*P by rows (a GalerkinPlan's row-wise form): fine row i is spread over
+// G = 2^lg lanes; lane g walks the row's A slots s = g, g + G, ... -- every
+// slot of an offset slice (G = 1), or those below arowlen[i] (null: the
+// slice width) -- and, for slot s with column j, the entries t < rowlen(j) of
+// P's row j, adding a_s p_jt into the AP entry its product code names (255:
+// the row holds no entry in that slot, the product is dropped).  The lanes'
+// partial rows accumulate in LDS ([entry][lane]: no bank conflicts whatever
+// the entries), are summed over the G lanes in a fixed pattern, and the
+// group's last lane writes the row's ap_w or fewer entries in CSR order
+// (ap_rp).  Dynamic LDS: ap_w * BLK * sizeof(T).
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_ap_rows(SellD<T> A, const int32_t* __restrict__ arowlen,
+                                                 int lg, int ap_w, SellD<T> P,
+                                                 const int32_t* __restrict__ prowlen,
+                                                 const int64_t* __restrict__ apc_ptr,
+                                                 const uint8_t* __restrict__ apc,
+                                                 const int64_t* __restrict__ ap_rp,
+                                                 T* __restrict__ apv) {
+  extern __shared__ unsigned char ap_lds[];
+  T* acc = (T*)ap_lds;
+  const int64_t sub = logical_row();
+  const int64_t i = sub >> lg;
+  const int G = 1 << lg, g = (int)(sub & (G - 1));
+  const bool valid = i < A.nrows;
+  const int tid = threadIdx.x;
+  for (int k = 0; k < ap_w; k++) acc[k * BLK + tid] = (T)0;
+  if (valid) {
+    const SellRow R = sell_row_at(A, i);
+    const int64_t cq = apc_ptr[sub >> 6] + (sub & 63);
+    int64_t q = 0;
+    // one A slot: the P row's entries, their codes and values in groups of 8
+    auto slot = [&](T a, int32_t j) {
+      const int64_t pb = P.sptr[j >> 6] + (j & 63);
+      const int pw = prowlen[j];
+      for (int t0 = 0; t0 < pw; t0 += 8) {
+        T pv[8];
+        uint8_t cd[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const bool ok = t0 + u < pw;
+          pv[u] = ok ? P.val[pb + (int64_t)(t0 + u) * 64] : (T)0;
+          cd[u] = ok ? apc[cq + (q + t0 + u) * 64] : (uint8_t)255;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (cd[u] != 255) acc[cd[u] * BLK + tid] += a * pv[u];
+      }
+      q += pw;
+    };
+    if (R.offs) {
+      // an offset slice (G = 1): slot s has column i + o[s] (unrolled: the
+      // offsets stay in registers)
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        if (s < R.w) slot(A.val[R.base + (int64_t)s * 64], (int32_t)i + R.o[s]);
+    } else {
+      const int wa = arowlen ? arowlen[i] : R.w;
+      for (int s = g; s < wa; s += G) {
+        const int64_t e = R.base + (int64_t)s * 64;
+        const int32_t j = A.col16 ? R.cb + (int32_t)A.col16[e] : A.col[e];
+        slot(A.val[e], j);
+      }
+    }
+  }
+  // every lane of a group takes part in the sums (groups lie within a wave)
+  const int64_t rb = valid ? ap_rp[i] : 0;
+  const int m = valid ? (int)(ap_rp[i + 1] - rb) : 0;
+  for (int k = 0; k < ap_w; k++) {
+    T v = group_sum(acc[k * BLK + tid], lg);
+    if (g == G - 1 && k < m) apv[rb + k] = v;
+  }
+}
+
+// The next level's entries on and above the diagonal by rows (a
+// GalerkinPlan's row-wise form), over R's row-split copy Rs: sub-row (I, g)
+// holds the entries t of R's row I with t = g mod 2^lg (at most 8), each with
+// fine row i_t and value P(i_t, I) = pval[rs_src].  The sub-row's products
+// AP(i_t, k) P(i_t, I), coded t | k << 3 | d << 8, come sorted by
+// destination d and then t, so each destination's products are one run summed
+// in a register (ascending fine row) and stored once into the lane's partial
+// row in LDS; the partial rows are summed over the lanes in a fixed pattern
+// and the group's last lane writes the upper entries.  Dynamic LDS:
+// rap_w * BLK * sizeof(T).
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_rap_rows(
+    SellD<T> Rs, const int32_t* __restrict__ rsrowlen, int lg, int rap_w, int64_t nc,
+    const int32_t* __restrict__ rs_src, const int64_t* __restrict__ rcd_ptr,
+    const uint16_t* __restrict__ rcd, const uint16_t* __restrict__ rnp,
+    const T* __restrict__ pval, const int64_t* __restrict__ ap_rp, const T* __restrict__ apv,
+    const int64_t* __restrict__ csptr, const int32_t* __restrict__ crowlen,
+    const uint16_t* __restrict__ cdiag, T* __restrict__ cval) {
+  extern __shared__ unsigned char rap_lds[];
+  T* acc = (T*)rap_lds;
+  const int64_t sub = logical_row();
+  const int64_t I = sub >> lg;
+  const int G = 1 << lg, g = (int)(sub & (G - 1));
+  const bool valid = I < nc;
+  const int tid = threadIdx.x;
+  for (int d = 0; d < rap_w; d++) acc[d * BLK + tid] = (T)0;
+  if (valid) {
+    const SellRow R = sell_row_at(Rs, sub);
+    const int rw = rsrowlen[sub];
+    const int np = rnp[sub];
+    const int64_t cq = rcd_ptr[sub >> 6] + (sub & 63);
+    // the sub-row's entries: fine row, P value, AP row start
+    int32_t src[8], fi[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int64_t e = R.base + (int64_t)t * 64;
+      const bool ok = t < rw;
+      fi[t] = ok ? (Rs.col16 ? R.cb + (int32_t)Rs.col16[e] : Rs.col[e]) : 0;
+      src[t] = ok ? rs_src[e] : 0;
+    }
+    T pv[8];
+    int64_t ab[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const bool ok = t < rw;
+      pv[t] = ok ? pval[src[t]] : (T)0;
+      ab[t] = ok ? ap_rp[fi[t]] : 0;
+    }
+    T run = (T)0;
+    int dcur = -1;
+    for (int q0 = 0; q0 < np; q0 += 8) {
+      uint16_t cd[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) cd[u] = q0 + u < np ? rcd[cq + (int64_t)(q0 + u) * 64] : (uint16_t)0;
+      T av[8], pu[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int t = cd[u] & 7;
+        int64_t a = ab[0];
+        T p = pv[0];
+#pragma unroll
+        for (int v = 1; v < 8; v++)
+          if (t == v) {
+            a = ab[v];
+            p = pv[v];
+          }
+        pu[u] = p;
+        av[u] = q0 + u < np ? apv[a + ((cd[u] >> 3) & 31)] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (q0 + u >= np) break;
+        const int d = cd[u] >> 8;
+        if (d != dcur) {
+          if (dcur >= 0) acc[dcur * BLK + tid] = run;
+          run = (T)0;
+          dcur = d;
+        }
+        run += pu[u] * av[u];
+      }
+    }
+    if (dcur >= 0) acc[dcur * BLK + tid] = run;
+  }
+  const int ds = valid ? cdiag[I] : 0;
+  const int up = valid ? crowlen[I] - ds : 0;
+  const int64_t cb = valid ? csptr[I >> 6] + (I & 63) : 0;
+  for (int d = 0; d < rap_w; d++) {
+    T v = group_sum(acc[d * BLK + tid], lg);
+    if (g == G - 1 && d < up) cval[cb + (int64_t)(ds + d) * 64] = v;
+  }
+}
+
 // val[dst[m]] = val[src[m]]: the entries below the diagonal of a symmetric
 // Galerkin operator from their mirrors above it (disjoint sets).
 template <typename T>

@@ -13,7 +13,11 @@
  *   As.rowlen, agg, r_src, rs_src, as_src, ap_col, ap_a, ap_p, rap_dest, rap_p, rap_ap: int32
  *   diag_slot, p_slot, p_self: uint16
  * dpf_inspect_size(h, level, "rs_lg" | "as_lg") returns log2 of the lanes per
- * row of the split restriction / split level operator (no array).
+ * row of the split restriction / split level operator (no array);
+ * "G.ap_rows" | "G.rap_rows" 1 when the level computes A*P / R*(A*P) by rows
+ * (dpf_host.h GalerkinPlan), "G.ap_lg" log2 of the lanes per fine row of the
+ * former; the plan's arrays are
+ *   G.apc_ptr, G.rcd_ptr: int64   G.apc: uint8   G.rcd, G.rnp: uint16
  */
 #ifndef DPFGPU_INSPECT_H
 #define DPFGPU_INSPECT_H

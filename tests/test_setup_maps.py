@@ -21,7 +21,9 @@ DT = {"n2f": np.int32, "f2n": np.int32, "inc_edge": np.int32, "inc_other": np.in
       "ap_off": np.int64, "ap_a": np.int32, "ap_p": np.int32, "rap_off": np.int64,
       "rap_p": np.int32, "rap_ap": np.int32, "As.sptr": np.int64, "As.col": np.int32,
       "As.rowlen": np.int32, "as_src": np.int32, "rap_dest": np.int32, "rap_mdst": np.int32,
-      "rap_msrc": np.int32}
+      "rap_msrc": np.int32, "F.sptr": np.int64, "F.col": np.int32, "F.soff": np.int32,
+      "G.apc_ptr": np.int64, "G.rcd_ptr": np.int64, "G.apc": np.uint8, "G.rcd": np.uint16,
+      "G.rnp": np.uint16}
 
 
 _INSPECT = None
@@ -455,3 +457,119 @@ def test_host_setup_under_address_sanitizer(tmp_path):
                        cwd=root, timeout=900)
     assert r.returncode == 0, (r.stdout[-4000:], r.stderr[-4000:])
     assert "passed" in r.stdout
+
+
+def _rows_of(counts):
+    """(owner, rank) for a flat list grouped by owner, counts[o] entries each."""
+    owner = np.repeat(np.arange(counts.size), counts)
+    rank = np.arange(owner.size) - np.repeat(np.cumsum(counts) - counts, counts)
+    return owner, rank
+
+
+def test_rowwise_galerkin_plans_match_sparse_algebra(pkg):
+    """The row-wise Galerkin plans (k_ap_rows / k_rap_rows) replayed in numpy
+    with random operator values: A*P from each fine row's lanes, which walk
+    the row's A slots in the GPU layout (the fine offset layout on level 0),
+    the selected P rows and the product codes; the next level's upper entries
+    from the lanes of R's row-split copy, each entry's run of AP entries and
+    the destination codes; against scipy's A @ P and P' A P."""
+    w = pkg.workloads.steady_2m(ns=(64, 48, 32), maxs=(32.0, 24.0, 16.0))
+    I = Inspect(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes, 25, 128)
+    rng = np.random.default_rng(5)
+    try:
+        size = lambda name, l: I.L.dpf_inspect_size(I.h, l, name.encode())
+        assert size("G.ap_rows", 0) == 1 and size("G.rap_rows", 0) == 1
+        assert size("G.ap_rows", 1) == 1 and size("G.ap_lg", 1) >= 1
+        for l in range(I.levels - 1):
+            if not size("G.ap_rows", l):
+                continue
+            As_, Ac_, Al = I.get("A.sptr", l), I.get("A.col", l), I.get("A.rowlen", l)
+            n = Al.size
+            Ps, Pc, Pl = I.get("P.sptr", l), I.get("P.col", l), I.get("P.rowlen", l)
+            nc = I.get("R.rowlen", l).size
+            A = sell_to_csr(As_, Ac_, Al, rng.normal(size=As_[-1]), n, n)
+            pv = rng.normal(size=Ps[-1])
+            P = sell_to_csr(Ps, Pc, Pl, pv, n, nc)
+            lg = size("G.ap_lg", l)
+            G = 1 << lg
+            # the A slots each sub-row (i, g) walks, in the GPU layout's order
+            if l == 0:
+                assert lg == 0
+                Fs, Fc, So = I.get("F.sptr"), I.get("F.col"), I.get("F.soff").reshape(-1, 9)
+                sl = np.arange(n) // 64
+                W = np.where(So[sl, 0] > 0, So[sl, 0], (Fs[sl + 1] - Fs[sl]) // 64)
+                owner, k = _rows_of(W)
+                s_ = owner // 64
+                offs = So[s_, 0] > 0
+                col = np.where(offs, owner + So[s_, 1 + np.minimum(k, 7)],
+                               Fc[np.minimum(Fs[s_] + 64 * k + owner % 64, Fc.size - 1)])
+                hostcols = [set(Ac_[As_[r // 64] + 64 * m + r % 64] for m in range(Al[r]))
+                            for r in range(n)]
+                present = np.where(offs, [c in hostcols[o] for o, c in zip(owner, col)],
+                                   k < Al[owner])
+                subrow = owner
+            else:
+                owner, k = _rows_of(Al)
+                col = Ac_[As_[owner // 64] + 64 * k + owner % 64]
+                present = np.ones(owner.size, bool)
+                subrow = owner * G + k % G
+                order = np.lexsort((k, subrow))  # each lane's slots in ascending order
+                owner, k, col, present, subrow = (owner[order], k[order], col[order],
+                                                  present[order], subrow[order])
+            aval = np.where(present, np.asarray(A[owner, col]).ravel(), 0.0)
+            # products: P row col's entries t < rowlen, numbered per sub-row in order
+            cnt = Pl[col]
+            pair, t = _rows_of(cnt)
+            sr = subrow[pair]
+            per = np.bincount(subrow, weights=cnt, minlength=n * G).astype(np.int64)
+            q = np.arange(pair.size) - np.repeat(np.cumsum(per) - per, per)
+            apc_ptr, apc = I.get("G.apc_ptr", l), I.get("G.apc", l)
+            code = apc[apc_ptr[sr // 64] + 64 * q + sr % 64]
+            assert np.all(code[~present[pair]] == 255) and np.all(code[present[pair]] < 255)
+            ok = code != 255
+            pvals = pv[Ps[col[pair] // 64] + 64 * t + col[pair] % 64]
+            ap_rp, ap_col = I.get("ap_rp", l), I.get("ap_col", l)
+            apw = int(np.diff(ap_rp).max())
+            APrw = np.zeros((n, apw))
+            np.add.at(APrw, (owner[pair][ok], code[ok].astype(np.int64)), (aval[pair] * pvals)[ok])
+            AP_ref = (A @ P).tocsr()
+            ar, ak = _rows_of(np.diff(ap_rp))
+            want = np.asarray(AP_ref[ar, ap_col]).ravel()
+            assert np.abs(APrw[ar, ak] - want).max() <= 1e-12 * np.abs(want).max()
+            assert np.all(APrw[np.arange(apw)[None, :] >= np.diff(ap_rp)[:, None]] == 0)
+            if not size("G.rap_rows", l):
+                continue
+            apv = APrw[ar, ak]  # CSR order
+            rlg = size("rs_lg", l)
+            Ss_, Sc, Sl, rsrc = (I.get("Rs.sptr", l), I.get("Rs.col", l), I.get("Rs.rowlen", l),
+                                 I.get("rs_src", l))
+            rcd_ptr, rcd, rnp = I.get("G.rcd_ptr", l), I.get("G.rcd", l), I.get("G.rnp", l)
+            Cs, Cc, Cl, Cd = (I.get("A.sptr", l + 1), I.get("A.col", l + 1),
+                              I.get("A.rowlen", l + 1), I.get("diag_slot", l + 1))
+            assert Sl.max() <= 8
+            sr, q = _rows_of(rnp.astype(np.int64))
+            cd = rcd[rcd_ptr[sr // 64] + 64 * q + sr % 64].astype(np.int64)
+            t_, k_, d_ = cd & 7, (cd >> 3) & 31, cd >> 8
+            # within a sub-row: sorted by destination, then by entry
+            key = sr * 4096 + d_ * 16 + t_
+            assert np.all(np.diff(key) > 0)
+            assert np.all(t_ < Sl[sr])
+            e = Ss_[sr // 64] + 64 * t_ + sr % 64
+            fi = Sc[e]
+            Ir = sr >> rlg
+            # exactly the AP entries of column >= I of every entry's fine row
+            assert np.all(ap_col[ap_rp[fi] + k_] >= Ir) and np.all(k_ < np.diff(ap_rp)[fi])
+            rr, rt = _rows_of(Sl)
+            fr = Sc[Ss_[rr // 64] + 64 * rt + rr % 64]
+            nup = np.array([np.sum(ap_col[ap_rp[f]:ap_rp[f + 1]] >= (r >> rlg)) for r, f in zip(rr, fr)])
+            assert nup.sum() == rnp.sum()
+            up = Cl - Cd.astype(np.int64)
+            Cup = np.zeros((nc, int(up.max())))
+            np.add.at(Cup, (Ir, d_), pv[rsrc[e]] * apv[ap_rp[fi] + k_])
+            Cref = (P.T @ AP_ref).tocsr()
+            uI, ud = _rows_of(up)
+            ccol = Cc[Cs[uI // 64] + 64 * (Cd[uI].astype(np.int64) + ud) + uI % 64]
+            want = np.asarray(Cref[uI, ccol]).ravel()
+            assert np.abs(Cup[uI, ud] - want).max() <= 1e-12 * np.abs(want).max()
+    finally:
+        I.close()
