@@ -333,7 +333,7 @@ struct dpf_topology {
   // inputs staged from the host / kept for the pullback
   DBuf Ke, dh, Q, stage_n, stage_n2, stage_e, stage_f, stage_f2, stage_f3;
   // refinement vectors
-  DBuf x64, r64, b32;
+  DBuf x64, r64;
   Hier<double> h64;
   Hier<float> h32;
   // reductions and solve scalars
@@ -1094,7 +1094,7 @@ struct SolveResult {
 template <typename TK, typename TP>
 SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, const TK* b,
                 double reltol, double abstol, int maxiter, const std::vector<Cheb>& chs,
-                int abs_target = 0) {
+                int abs_target = 0, const double* b64 = nullptr) {
   constexpr bool split = !std::is_same<TK, TP>::value;
   const Cheb& ch = chs[0];
   LevelDev& d0 = *t->lev[0];
@@ -1129,8 +1129,9 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   const double b_xr = rows * (6.0 * wk + vec_p) + 16.0 * g;
   t->kiter = 0;
   t->kphase = 0;
-  klaunch(t, KC_CG_START, 0, (int)wk, b_start, k_cg_start<TK, TP>, dim3(g), dim3(BLK), 0, s, nf,
-          b, r, x, rp, invd, dd, z0, (TP)ch.c0, bm_ptr(t), part);
+  klaunch(t, KC_CG_START, 0, (int)wk, b_start + (b64 ? rows * (8.0 - wk) : 0.0),
+          k_cg_start<TK, TP>, dim3(g), dim3(BLK), 0, s, nf, b64 ? (const TK*)nullptr : b, b64,
+          (const SolveScalars*)sc, r, x, rp, invd, dd, z0, (TP)ch.c0, bm_ptr(t), part);
   finish(t, s, FIN_START, g, reltol, abstol, maxiter, abs_target, seq);
   TP* z = nullptr;
   t->kphase = 1;
@@ -1795,6 +1796,15 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   int total_it = 0, outer = 0;
   t->x64.alloc(nf * sizeof(double));
   t->r64.alloc(nf * sizeof(double));
+  // the members' ||r64|| into the pinned mirror (first: also ||b|| and the
+  // refinement target), waited for; returns the members above their target
+  auto publish_outer = [&](int first) -> int {
+    int oseq = ++t->seq;
+    finish(t, s, FIN_OUTER, g, o->reltol, o->abstol, 0, first, oseq);
+    HIPTRY(hipGetLastError());
+    wait_published(t, s, &t->hm_host->outer_seq, oseq);
+    return mirror_load(&t->hm_host->active);
+  };
   // r64 = b - A x64 (b alone when first: x64 = 0); every member's ||r64||
   // lands in the pinned mirror.  first: also take it as the member's ||b||
   // and set its refinement target.  warm: x64 holds the initial guess x0,
@@ -1818,11 +1828,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
               t->x64.as<double>(), b,
                          t->r64.as<double>(), (const SolveScalars*)sc, bm_ptr(t), part);
     }
-    int oseq = ++t->seq;
-    finish(t, s, FIN_OUTER, g, o->reltol, o->abstol, 0, first, oseq);
-    HIPTRY(hipGetLastError());
-    wait_published(t, s, &t->hm_host->outer_seq, oseq);
-    return mirror_load(&t->hm_host->active);
+    return publish_outer(first);
   };
   auto all_finite = [&]() {
     for (int m = 0; m < t->nm; m++)
@@ -1870,10 +1876,7 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   // FP64 Krylov: one CG to the full tolerance; a restart only if rounding
   // left the true residual just above it
   int max_outer = o->precision == DPF_MIXED ? o->max_outer : 3;
-  if (o->precision == DPF_MIXED) {
-    ensure_hier(t, t->h32);
-    t->b32.alloc(nf * sizeof(float));
-  }
+  if (o->precision == DPF_MIXED) ensure_hier(t, t->h32);
   if (o->precision == DPF_MIXED_PC) {
     ensure_hier(t, t->h32);
     ensure_hier(t, t->h64);  // its fine-level Krylov vectors
@@ -1900,8 +1903,8 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     if (o->precision == DPF_MIXED_PC)
       return pcg<double, float>(t, t->h64, t->h32, s, t->r64.as<double>(), 0.0, 0.0, budget,
                                 cc, 1);
-    return pcg<float, float>(t, t->h32, t->h32, s, t->b32.as<float>(), inner, 0.0, budget,
-                             cc, 0);
+    return pcg<float, float>(t, t->h32, t->h32, s, (const float*)nullptr, inner, 0.0, budget,
+                             cc, 0, (const double*)t->r64.as<double>());
   };
   while (active > 0) {
     if (outer >= max_outer || budget <= 0) {
@@ -1909,11 +1912,6 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       break;
     }
     SolveResult r;
-    if (o->precision == DPF_MIXED)
-      klaunch(t, KC_REFINE_VEC, 0, 8, 12.0 * z0s.rows, k_scale_to_f32, dim3(g), dim3(BLK), 0, s,
-              nf,
-                         (const double*)t->r64.as<double>(), t->b32.as<float>(),
-                         (const SolveScalars*)sc, bm_ptr(t));
     if (tune) {
       // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
       // very system at the default interval, then two other intervals at the
@@ -1963,15 +1961,13 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
     }
     r = krylov(chs, o->inner_reltol);
-    if (o->precision != DPF_MIXED) {
+    if (o->precision != DPF_MIXED)
       klaunch(t, KC_REFINE_VEC, 0, 8, 24.0 * z0s.rows, k_axpy_f64, dim3(g), dim3(BLK), 0, s, nf,
-              1.0,
-                         (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
-    } else {
+              1.0, (const double*)t->h64.cx.as<double>(), t->x64.as<double>());
+    else
       klaunch(t, KC_REFINE_VEC, 0, 8, 20.0 * z0s.rows, k_refine, dim3(g), dim3(BLK), 0, s, nf,
-              t->x64.as<double>(),
-                         (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc, bm_ptr(t));
-    }
+              t->x64.as<double>(), (const float*)t->h32.cx.as<float>(), (const SolveScalars*)sc,
+              bm_ptr(t));
     total_it += r.iters;
     budget -= r.iters;
     outer++;

@@ -1598,18 +1598,29 @@ __global__ void k_rz_dot(int64_t n, const T* __restrict__ b, const T* __restrict
 // r = b, x = 0, partials of b.b, and the V-cycle's zero-guess first step
 // d0 = z0 = c0 D^-1 r.  The Krylov vectors are in TK and the V-cycle in TP;
 // when they differ, rp receives r rounded to TP, the V-cycle's input (rp is
-// null when TK == TP: the V-cycle reads r itself).
+// null when TK == TP: the V-cycle reads r itself).  With b64 (b null), the
+// right-hand side is the refinement residual b64 / outer_s of the members
+// above their refinement target, rounded to TK, and 0 for the others (whose
+// solve then ends at once) -- the FP32 inner solve's right-hand side.
 template <typename TK, typename TP>
-__global__ void k_cg_start(int64_t n, const TK* __restrict__ b, TK* __restrict__ r,
+__global__ void k_cg_start(int64_t n, const TK* __restrict__ b, const double* __restrict__ b64,
+                           const SolveScalars* __restrict__ sc, TK* __restrict__ r,
                            TK* __restrict__ x, TP* __restrict__ rp, const TP* __restrict__ invd,
                            TP* __restrict__ d0, TP* __restrict__ z0, TP c0,
                            const int2* __restrict__ bm, double* __restrict__ partial) {
   int64_t i = logical_row();
   bool second;
-  member_of(bm, i, second);
+  const int mi = member_of(bm, i, second);
   double local = 0.0;
   if (i < n) {
-    TK bv = b[i];
+    TK bv;
+    if (b64) {
+      const SolveScalars& m = sc[mi];
+      const double sv = m.outer_s;
+      bv = (sv > m.target && sv > 0.0) ? (TK)(b64[i] / sv) : (TK)0;
+    } else {
+      bv = b[i];
+    }
     r[i] = bv;
     x[i] = (TK)0;
     TP rv = (TP)bv;
@@ -1784,19 +1795,6 @@ __global__ void k_outer_residual(SellD<double> A, const double* __restrict__ x,
     local = rv * rv;
   }
   block_partial2(local, second, partial, bm != nullptr);
-}
-
-// inner right-hand side in FP32: b32 = r64 / s for members above their
-// refinement target, 0 for the others (whose inner solve then ends at once)
-__global__ void k_scale_to_f32(int64_t n, const double* __restrict__ r, float* __restrict__ b32,
-                               const SolveScalars* __restrict__ sc,
-                               const int2* __restrict__ bm) {
-  int64_t i = logical_row();
-  if (i >= n) return;
-  bool second;
-  const SolveScalars& m = sc[member_of(bm, i, second)];
-  double s = m.outer_s;
-  b32[i] = (s > m.target && s > 0.0) ? (float)(r[i] / s) : 0.0f;
 }
 
 // x64 += s * x32 for members above their refinement target
