@@ -768,6 +768,19 @@ auto cheb_kernel(int level) -> decltype(&k_cheb<T, true>) {
   return level == 0 ? &k_cheb<T, true> : &k_cheb<T, false>;
 }
 
+// The residual kernel of level l (the finest level's can fold a FIN_XR).
+template <typename T>
+auto residual_kernel(int level) -> decltype(&k_residual<T, true>) {
+  return level == 0 ? &k_residual<T, true> : &k_residual<T, false>;
+}
+
+// No finish folded into a kernel.
+FoldFin no_fold() {
+  FoldFin f{};
+  f.mode = -1;
+  return f;
+}
+
 // The member table of a batch (null for a single system).
 const int2* bm_ptr(dpf_topology* t) { return t->nm > 1 ? t->bm.as<int2>() : nullptr; }
 
@@ -848,7 +861,8 @@ double cheb_bytes(const dpf_topology* t, int l, double w, bool din, bool dout, b
 // buffer holding x_l on exit through *xout.
 template <typename T>
 void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* xl, T* xsl,
-            T** xout, const std::vector<Cheb>& chs, int* done, bool fuse_dot_at_fine) {
+            T** xout, const std::vector<Cheb>& chs, int* done, bool fuse_dot_at_fine,
+            const FoldFin* fold = nullptr) {
   const Cheb& ch = chs[l];
   int L = t->levels();
   LevelDev& d = *t->lev[l];
@@ -881,6 +895,11 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   int64_t nr = d.A.nrows;
   T* cur = xl;
   T* oth = xsl;
+  // a folded finish rides on the finest level's residual, its first kernel
+  // when the finest smoother has degree 1
+  if (fold && (l != 0 || l == L - 1 || m != 1))
+    throw Err{DPF_EHIP, "a finish can be folded only into the finest level's first kernel"};
+  const FoldFin nofold = no_fold();
   if (l == L - 1) {
     if (t->H.coarse_dense) {
       int nc = (int)d.A.nrows;
@@ -924,8 +943,8 @@ void vcycle(dpf_topology* t, Hier<T>& h, hipStream_t s, int l, const T* bl, T* x
   }
   T* rl = n.r.template as<T>();
   klaunch(t, KC_RESIDUAL, l, w, z.nnz * wv + z.cols + (a16 ? 4.0 : 3.0) * z.rows * w,
-          k_residual<T>, dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, dg, rl,
-          (const int*)done);
+          residual_kernel<T>(l), dim3(gs), dim3(BLK), 0, s, Asm, lg, nr, (const T*)cur, bl, dg,
+          rl, (const int*)done, fold ? *fold : nofold);
   LevelDev& dn = *t->lev[l + 1];
   LevelNum<T>& nn = h.lev[l + 1];
   bool coarse_next = (l + 1 == L - 1) && t->H.coarse_dense;
@@ -1140,8 +1159,23 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
   // one iteration's launches (iteration it, for the timing records); the
   // V-cycle's output buffer is the same on every call, so z does not change
   // from one iteration to the next
-  // (in two parts: the CG update up to the convergence test, then the V-cycle)
-  auto body_a = [&](int pr, int it) {
+  // (in two parts: the CG update up to the convergence test, then the V-cycle).
+  // A single system's convergence test (FIN_XR) rides on block 0 of the
+  // V-cycle's first kernel, the finest level's residual (finest smoother of
+  // degree 1), when the V-cycle follows the update in the same queue: one
+  // launch less an iteration.  A batch, and an update queued without its
+  // V-cycle (the forecast's), keep the k_finish launch.
+  const bool fold_xr = t->nm == 1 && t->levels() >= 2 && ch.c1.size() == 1;
+  FoldFin fxr = no_fold();
+  if (fold_xr) {
+    fxr.partial = part;
+    fxr.sc = sc;
+    fxr.gdone = done;
+    fxr.hm = t->hm_dev;
+    fxr.nblk = (int)g;
+    fxr.mode = FIN_XR;
+  }
+  auto body_a = [&](int pr, int it, bool xr_finish) {
     t->kiter = it;
     t->kphase = 0;
     klaunch(t, KC_CG_PQ, 0, (int)wk, b_pq, k_cg_pq<TK, TP>, dim3(g), dim3(BLK), 0, s, A,
@@ -1151,17 +1185,17 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
     klaunch(t, KC_CG_XR, 0, (int)wk, b_xr, k_cg_xr<TK, TP>, dim3(g), dim3(BLK), 0, s, nf, x, r,
             (const TK*)pb[pr ^ 1], (const TK*)q, rp, invd, dd, z0, (TP)ch.c0, (const int*)done,
             (const SolveScalars*)sc, bm_ptr(t), part);
-    finish(t, s, FIN_XR, g);
+    if (xr_finish) finish(t, s, FIN_XR, g);
   };
-  auto body_b = [&]() {
+  auto body_b = [&](bool folded) {
     TP* zz = nullptr;
     t->kphase = 1;
-    vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true);
+    vcycle<TP>(t, hp, s, 0, vb, z0, zs, &zz, chs, done, true, folded ? &fxr : nullptr);
     if (zz != z) throw Err{DPF_EHIP, "V-cycle output buffer moved between iterations"};
   };
   auto body = [&](int pr, int it) {
-    body_a(pr, it);
-    body_b();
+    body_a(pr, it, !fold_xr);
+    body_b(fold_xr);
   };
   bool graphs = t->use_graphs && !t->timing && !t->ktiming;
   std::string gkey;
@@ -1243,7 +1277,7 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
     }
     if (half) {
       if (its >= launched) {  // its test came back and the solve goes on
-        body_b();
+        body_b(false);
         half = false;
         HIPTRY(hipGetLastError());
         continue;
@@ -1257,7 +1291,7 @@ SolveResult pcg(dpf_topology* t, Hier<TK>& hk, Hier<TP>& hp, hipStream_t s, cons
       if (graphs)
         HIPTRY(hipGraphLaunch(graph_for(par), s));
       else if (forecast)
-        body_a(par, launched + 1);
+        body_a(par, launched + 1, true);
       else
         body(par, launched + 1);
       half = forecast;
@@ -2630,11 +2664,11 @@ double profile_one(dpf_topology* t, Hier<T>& h, hipStream_t s, int kernel, int r
                          (const T*)h.cp0.template as<T>(), h.cp1.template as<T>(),
                          h.cq.template as<T>(), nodone, (const SolveScalars*)sc, bm_ptr(t), part);
     } else {
-      hipLaunchKernelGGL(k_residual<double>, dim3(g), dim3(BLK), 0, s,
+      hipLaunchKernelGGL((k_residual<double, true>), dim3(g), dim3(BLK), 0, s,
                          d0.A.view<double>(t->A64.as<double>()), 0, nf,
                          (const double*)t->x64.as<double>(),
                          (const double*)t->b64.as<double>(), (const double*)nullptr,
-                         t->r64.as<double>(), nodone);
+                         t->r64.as<double>(), nodone, no_fold());
     }
   };
   launch(0);  // warm

@@ -342,6 +342,33 @@ __device__ void finish_member(int mode, double tot, SolveScalars& m, double relt
   }
 }
 
+// The end of a Krylov step's finish (every mode but FIN_OUTER): the early-exit
+// flag and the host mirror from the folded member states (flags: bit 0 a
+// member still running, bit 1 one capped, bit 2 one broken down).
+__device__ __forceinline__ void publish_solve(int mode, int flags, int maxit, double rr,
+                                              double rr_stop, int seq, int* gdone,
+                                              HostMirror* hm) {
+  int running = flags & 1, capped = flags & 2, broke = flags & 4;
+  int dn = running ? 0 : (broke ? 3 : (capped ? 2 : 1));
+  *gdone = dn;
+  // the host follows the iteration count (FIN_XR) and the end of the solve;
+  // the other steps of a running iteration leave the mirror as it is
+  if (mode == FIN_START || mode == FIN_XR || dn != 0) {
+    mput(&hm->rr, rr);
+    mput(&hm->iters, maxit);
+    mput(&hm->done, dn);
+  }
+  if (mode == FIN_START) {
+    mput(&hm->rr_stop, rr_stop);
+    mpublish(&hm->seq, seq);
+  }
+}
+
+// A single system's member flags, as k_finish folds them.
+__device__ __forceinline__ int member_flags(const SolveScalars& q) {
+  return (q.done == 0 && !q.conv ? 1 : 0) | (q.done == 2 ? 2 : 0) | (q.done == 3 ? 4 : 0);
+}
+
 // Sums the block partials of the previous kernel per member, in a fixed
 // order, applies finish_member, then folds the members' states into the
 // global early-exit flag *gdone and the host mirror.  One workgroup; with
@@ -369,7 +396,7 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
   int flags = 0, maxit = 0, active = 0;
   double rr = 0.0;
   auto fold = [&](const SolveScalars& q) {
-    flags |= (q.done == 0 && !q.conv ? 1 : 0) | (q.done == 2 ? 2 : 0) | (q.done == 3 ? 4 : 0);
+    flags |= member_flags(q);
     maxit = q.iters > maxit ? q.iters : maxit;
     rr = q.rr > rr ? q.rr : rr;
     active += q.outer_s > q.target ? 1 : 0;
@@ -454,25 +481,74 @@ __global__ void __launch_bounds__(1024) k_finish(int mode, int nm, int nblk,
       rr = sh_rr[k] > rr ? sh_rr[k] : rr;
     }
   }
-  int running = flags & 1, capped = flags & 2, broke = flags & 4;
   if (mode == FIN_OUTER) {
     mput(&hm->active, active);
     mpublish(&hm->outer_seq, seq);
     return;
   }
-  int dn = running ? 0 : (broke ? 3 : (capped ? 2 : 1));
-  *gdone = dn;
-  // the host follows the iteration count (FIN_XR) and the end of the solve;
-  // the other steps of a running iteration leave the mirror as it is
-  if (mode == FIN_START || mode == FIN_XR || dn != 0) {
-    mput(&hm->rr, rr);
-    mput(&hm->iters, maxit);
-    mput(&hm->done, dn);
+  publish_solve(mode, flags, maxit, rr, nm == 1 ? sc[0].rr_stop : -1.0, seq, gdone, hm);
+}
+
+// FIN_XR of a single system, done by block 0 of the next kernel in place of a
+// k_finish launch of its own: the step's results (rr, the iteration count,
+// convergence) are needed only by the host and by the early exit of later
+// kernels (which only saves work), not by that kernel's other blocks.
+// mode < 0: none.
+struct FoldFin {
+  const double* partial;
+  SolveScalars* sc;
+  int* gdone;
+  HostMirror* hm;
+  double reltol, abstol;
+  int nblk, mode, maxiter, abs_target, seq;
+};
+
+// The fold, called by every thread of block 0 (BLK threads) of a kernel that
+// has not taken the early exit: the partials are summed exactly as k_finish's
+// 1024 threads sum them (each thread here stands for four of them, the
+// 16 wave sums combined in the same order), so the results are bitwise
+// those of the k_finish launch it replaces.
+__device__ void fold_finish(const FoldFin& f) {
+  constexpr int VT = 1024, NW = VT / 64, J = VT / BLK;
+  static_assert(VT % BLK == 0 && J % 2 == 0, "four virtual threads a thread");
+  __shared__ double sh[NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // two virtual threads at a time: sixteen loads in flight
+#pragma unroll
+  for (int j = 0; j < J; j += 2) {
+    const int vt0 = threadIdx.x + j * BLK;  // virtual wave: vt >> 6 = wave + (BLK / 64) j
+    double s0 = 0.0, s1 = 0.0;
+    for (int b0 = vt0; b0 < f.nblk; b0 += 8 * VT) {
+      double v0[8], v1[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        int b = b0 + k * VT;
+        v0[k] = b < f.nblk ? f.partial[2 * b] : 0.0;
+        v1[k] = b + BLK < f.nblk ? f.partial[2 * (b + BLK)] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        s0 += v0[k];
+        s1 += v1[k];
+      }
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane == 63) {
+      sh[wave + (BLK / 64) * j] = s0;
+      sh[wave + (BLK / 64) * (j + 1)] = s1;
+    }
   }
-  if (mode == FIN_START) {
-    mput(&hm->rr_stop, nm == 1 ? sc[0].rr_stop : -1.0);
-    mpublish(&hm->seq, seq);
-  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tot = 0.0;
+  for (int k = 0; k < NW; k++) tot += sh[k];
+  SolveScalars q = f.sc[0];
+  finish_member(f.mode, tot, q, f.reltol, f.abstol, f.maxiter, f.abs_target, 0);
+  f.sc[0] = q;
+  // (k_finish's fold of one member: largest count and rr from 0)
+  publish_solve(f.mode, member_flags(q), q.iters > 0 ? q.iters : 0, q.rr > 0.0 ? q.rr : 0.0,
+                q.rr_stop, f.seq, f.gdone, f.hm);
 }
 
 // Start and width of row r's SELL entries.  Every caller runs the 64 rows of
@@ -1441,11 +1517,12 @@ __global__ void k_cheb(SellD<T> A, int lg, int64_t n, const T* __restrict__ xin,
   }
 }
 
-// r = b - A x  (rows of A split over 2^lg lanes)
-template <typename T>
+// r = b - A x  (rows of A split over 2^lg lanes).  FOLD: the finest level's
+// instantiation, whose block 0 also does the CG step's FIN_XR (ff.mode >= 0).
+template <typename T, bool FOLD>
 __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ x,
                            const T* __restrict__ b, const T* __restrict__ dg,
-                           T* __restrict__ r, const int* __restrict__ done) {
+                           T* __restrict__ r, const int* __restrict__ done, FoldFin ff) {
   const int dn = *done;
   int64_t sub = logical_row();
   int64_t i = sub >> lg;
@@ -1453,9 +1530,12 @@ __global__ void k_residual(SellD<T> A, int lg, int64_t n, const T* __restrict__ 
   const bool own = (sub & ((1 << lg) - 1)) == (1 << lg) - 1 && i < n;
   T bi = own ? b[i] : (T)0;  // loaded with the operator, not after it
   T gx = own && dg ? dg[i] * x[i] : (T)0;
-  if (dn || R.w < 0) return;
+  if (dn || R.w < 0) return;  // (R.w >= 0 always: every thread of block 0 reaches the fold)
   T ax = split_row_dot(A, R, sub, lg, x);
   if (own) r[i] = bi - (ax + gx);
+  if constexpr (FOLD) {
+    if (ff.mode >= 0 && blockIdx.x == 0) fold_finish(ff);
+  }
 }
 
 // bc = R rf with every row of R split over G = 2^lg consecutive lanes
