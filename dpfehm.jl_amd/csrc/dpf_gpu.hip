@@ -754,11 +754,18 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
   if (t->H.coarse_dense) {
     LevelDev& dc = *t->lev[L - 1];
     int ncd = (int)dc.A.nrows;
-    size_t lds = ncd <= DENSE_LDS_MAX ? (size_t)ncd * ncd * sizeof(double) : 0;
-    klaunch(t, KC_DENSE_INVERSE, L - 1, (int)sizeof(T), 2.0 * ncd * ncd * 8.0,
-            k_dense_inverse<T>, dim3(1), dim3(1024), lds, s,
-            dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
-            h.W.template as<double>(), h.status.template as<int>());
+    if (ncd <= 64) {
+      klaunch(t, KC_DENSE_INVERSE, L - 1, (int)sizeof(T), 2.0 * ncd * ncd * 8.0,
+              k_dense_inverse64<T>, dim3(1), dim3(256), 0, s,
+              dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
+              h.W.template as<double>(), h.status.template as<int>());
+    } else {
+      size_t lds = ncd <= DENSE_LDS_MAX ? (size_t)ncd * ncd * sizeof(double) : 0;
+      klaunch(t, KC_DENSE_INVERSE, L - 1, (int)sizeof(T), 2.0 * ncd * ncd * 8.0,
+              k_dense_inverse<T>, dim3(1), dim3(1024), lds, s,
+              dc.A.view<T>(level_val(t, h, L - 1)), dc.A.rowlen.as<int32_t>(),
+              h.W.template as<double>(), h.status.template as<int>());
+    }
   }
   HIPTRY(hipGetLastError());
 }

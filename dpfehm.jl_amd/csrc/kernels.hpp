@@ -1263,6 +1263,102 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(SellD<T> Ac, const int32
   if (tid == 0) status[0] = bad;
 }
 
+// The same inverse for a coarsest operator of at most 64 rows, by one
+// 256-thread workgroup: a 16 x 16 grid of threads, each holding a 4 x 4 tile
+// of the zero-padded 64 x 64 matrix in registers (the padding never mixes
+// with the matrix: its pivot-row and pivot-column entries are 0).  Per pivot
+// step the owners of row and column k publish them in LDS (double-buffered
+// by step parity: one barrier a step) and every thread updates its 16
+// positions with 4 + 4 pivot values, the same arithmetic per position as
+// k_dense_inverse.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_inverse64(SellD<T> Ac,
+                                                         const int32_t* __restrict__ rowlen,
+                                                         double* __restrict__ W, int* status) {
+  constexpr int N = 64;
+  const int nc = (int)Ac.nrows;  // <= N
+  const int tid = threadIdx.x;
+  __shared__ double Ad[N][N + 1];  // (+1: the column reads below do not conflict)
+  __shared__ double prow[2][N], pcol[2][N];
+  __shared__ int bad;
+  for (int t = tid; t < N * (N + 1); t += 256) (&Ad[0][0])[t] = 0.0;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  {
+    // row I = lane, entries k = wave, wave + 4, ...: eight independent loads
+    // a lane in flight (one slice: nc <= 64)
+    const int I = tid & 63;
+    const int len = I < nc ? rowlen[I] : 0;
+    const int64_t p0 = Ac.sptr[0] + I;
+    for (int k0 = tid >> 6; k0 < len; k0 += 32) {
+      int32_t c[8];
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int k = k0 + 4 * u;
+        const bool ok = k < len;
+        c[u] = ok ? Ac.col[p0 + (int64_t)k * 64] : 0;
+        v[u] = ok ? (double)Ac.val[p0 + (int64_t)k * 64] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (k0 + 4 * u < len) Ad[I][c[u]] = v[u];
+    }
+  }
+  __syncthreads();
+  const int ty = tid >> 4, tx = tid & 15;
+  double v[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) v[a][b] = Ad[4 * ty + a][4 * tx + b];
+  for (int k = 0; k < nc; k++) {
+    const int bb = k & 1, kt = k >> 2, kq = k & 3;
+    // publish row k and column k (static register indices: the step's
+    // position inside the tile is selected, not indexed)
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      if (ty == kt && a == kq)
+#pragma unroll
+        for (int b = 0; b < 4; b++) prow[bb][4 * tx + b] = v[a][b];
+      if (tx == kt && a == kq)
+#pragma unroll
+        for (int b = 0; b < 4; b++) pcol[bb][4 * ty + b] = v[b][a];
+    }
+    __syncthreads();
+    double piv = prow[bb][k];
+    if (!(piv > 0.0)) {
+      if (tid == 0) bad = 1;
+      piv = 1.0;
+    }
+    const double ip = 1.0 / piv;
+    double pc[4], pr[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      pc[a] = pcol[bb][4 * ty + a];
+      pr[a] = prow[bb][4 * tx + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const bool ri = 4 * ty + a == k, cj = 4 * tx + b == k;
+        const double upd = v[a][b] - pc[a] * pr[b] * ip;
+        const double scl = v[a][b] * (ri ? ip : -ip);
+        v[a][b] = (!ri && !cj) ? upd : ((ri && cj) ? ip : scl);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int i = 4 * ty + a, j = 4 * tx + b;
+      if (i < nc && j < nc) W[(int64_t)i * nc + j] = v[a][b];
+    }
+  __syncthreads();
+  if (tid == 0) status[0] = bad;
+}
+
 // ------------------------------------------------------------------ V-cycle
 
 // Groups of N consecutive entries of one SELL row (N known at compile time):

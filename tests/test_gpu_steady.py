@@ -505,6 +505,36 @@ def test_pinned_tuning(pkg, api):
     assert rel(hs[0], h_auto) < 1e-10
 
 
+@pytest.mark.parametrize("ns", [(6, 5, 3), (5, 4, 5)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed_pc"])
+def test_one_level_dense_inverse(pkg, api, oracle, ns, precision):
+    """A system of at most 128 free cells is its own coarsest level, so the
+    V-cycle is the dense inverse of the operator: FP64 CG converges in one
+    iteration, FP64 CG with the FP32 hierarchy (the inverse of the operator's
+    FP32 copy, applied to the residual rounded to FP32) in a few, and the heads
+    are the direct solve's.  60 free cells take the register-tile inverse (at
+    most 64 rows), 80 the one staged in LDS."""
+    _say("one-level dense inverse", ns, precision)
+    _, nb, aol, _ = pkg.grid.regulargrid3d((0.0, 0.0, 0.0), (5.0, 4.0, 2.0), ns)
+    n = int(np.prod(ns))
+    rng = np.random.default_rng(11)
+    K = np.exp(rng.normal(-2.0, 1.0, nb.shape[0]))
+    top = np.arange(n) % ns[2] == ns[2] - 1  # the z = max face (z runs fastest)
+    dn = np.flatnonzero(top) + 1
+    dh = np.zeros(n)
+    dh[top] = rng.normal(0.0, 1.0, dn.size)
+    Q = rng.normal(0.0, 1e-2, n)
+    Q[top] = 0.0
+    args = (K, nb, aol, dn, dh, Q)
+    h_ref = oracle.groundwater_steadystate(*args, reltol=1e-14)
+    st = {}
+    h = api.groundwater_steadystate(*args, reltol=1e-12, precision=precision, stats=st)
+    _say(st)
+    assert st["levels"] == 1 and st["converged"] == 1
+    assert st["iterations"] == 1 if precision == "fp64" else st["iterations"] <= 3
+    assert rel(h, h_ref) < 1e-12
+
+
 def test_linear_solver_choice(pkg, api, oracle):
     """linear_solver=cholesky_solver (src/groundwater.jl:56-60) solves to direct-solve
     accuracy (relative residual 1e-14, tolerance keywords ignored); amg_solver is the
