@@ -61,38 +61,90 @@ static double now_wall_ms() {
 // for reuse: hipFree synchronises the whole device and hipMalloc of large
 // buffers costs hundreds of microseconds, both per evaluation otherwise.
 // Shared (reference-counted) by a topology and its states, so that a state
-// may outlive its topology's handle.  Every call that uses a pooled buffer
-// synchronises its stream before returning, so a released buffer has no
-// work pending when it is handed out again.
+// may outlive its topology's handle.  Release is stream-ordered: a buffer
+// given back by a stream-ordered owner carries an event recorded on the
+// stream of its last use, and whoever takes it next waits for that event on
+// its own stream (a GPU-side wait), or on the host when it names no stream.
+// An owner that names no stream gives the buffer back with no work pending
+// (its call synchronised its stream).
 struct BufPool {
+  struct Entry {
+    void* p;
+    hipEvent_t ev;  // work that may still use p (null: none)
+  };
   std::mutex mu;
-  std::multimap<size_t, void*> free;
+  std::multimap<size_t, Entry> free;
+  std::vector<hipEvent_t> spare;  // events no entry holds
   size_t cached = 0, limit;
   explicit BufPool(size_t lim) : limit(lim) {}
   BufPool(const BufPool&) = delete;
   BufPool& operator=(const BufPool&) = delete;
   ~BufPool() {
-    for (auto& kv : free) (void)hipFree(kv.second);
+    for (auto& kv : free) {
+      if (kv.second.ev) {
+        (void)hipEventSynchronize(kv.second.ev);
+        (void)hipEventDestroy(kv.second.ev);
+      }
+      (void)hipFree(kv.second.p);
+    }
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
   }
-  // a cached buffer of at least b bytes (and at most 2b), or null
-  void* take(size_t b, size_t& got) {
-    std::lock_guard<std::mutex> g(mu);
-    auto it = free.lower_bound(b);
-    if (it == free.end() || it->first > 2 * b) return nullptr;
-    void* p = it->second;
-    got = it->first;
-    cached -= got;
-    free.erase(it);
-    return p;
+  // a cached buffer of at least b bytes (and at most 2b), or null; with
+  // ordered, stream s waits for the work that may still use it
+  void* take(size_t b, size_t& got, bool ordered, hipStream_t s) {
+    Entry e;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = free.lower_bound(b);
+      if (it == free.end() || it->first > 2 * b) return nullptr;
+      e = it->second;
+      got = it->first;
+      cached -= got;
+      free.erase(it);
+    }
+    if (e.ev) {
+      hipError_t r = ordered ? hipStreamWaitEvent(s, e.ev, 0) : hipEventSynchronize(e.ev);
+      if (r != hipSuccess) {
+        (void)hipEventSynchronize(e.ev);
+        (void)hipEventDestroy(e.ev);
+        give(e.p, got, false, nullptr);
+        throw Err{DPF_EHIP, std::string("pooled buffer wait: ") + hipGetErrorString(r)};
+      }
+      std::lock_guard<std::mutex> g(mu);
+      spare.push_back(e.ev);
+    }
+    return e.p;
   }
-  void give(void* p, size_t b) {
+  // with ordered, the buffer's last use was queued on stream s
+  void give(void* p, size_t b, bool ordered, hipStream_t s) {
+    hipEvent_t ev = nullptr;
+    if (ordered) {
+      {
+        std::lock_guard<std::mutex> g(mu);
+        if (!spare.empty()) {
+          ev = spare.back();
+          spare.pop_back();
+        }
+      }
+      if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+      if (!ev || hipEventRecord(ev, s) != hipSuccess) {
+        // no event to order by: wait for the stream here instead
+        if (ev) (void)hipEventDestroy(ev);
+        ev = nullptr;
+        (void)hipStreamSynchronize(s);
+      }
+    }
     {
       std::lock_guard<std::mutex> g(mu);
       if (cached + b <= limit) {
-        free.emplace(b, p);
+        free.emplace(b, Entry{p, ev});
         cached += b;
         return;
       }
+    }
+    if (ev) {
+      (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
     }
     (void)hipFree(p);
   }
@@ -102,15 +154,20 @@ struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
   BufPool* pool = nullptr;  // where the memory goes back to (null: hipFree)
+  // a pooled buffer used on one stream: taken after the work that still used
+  // it (waited for on that stream), given back ordered after its own
+  bool ordered = false;
+  hipStream_t stream = nullptr;
   DBuf() = default;
   explicit DBuf(BufPool* bp) : pool(bp) {}
+  DBuf(BufPool* bp, hipStream_t s) : pool(bp), ordered(true), stream(s) {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
     if (!p) return;
     if (pool)
-      pool->give(p, bytes);
+      pool->give(p, bytes, ordered, stream);
     else
       (void)hipFree(p);
     p = nullptr;
@@ -122,7 +179,7 @@ struct DBuf {
     if (b == 0) b = 8;
     if (pool) {
       size_t got = 0;
-      if ((p = pool->take(b, got))) {
+      if ((p = pool->take(b, got, ordered, stream))) {
         bytes = got;
         return;
       }
@@ -134,6 +191,24 @@ struct DBuf {
                                 hipGetErrorString(e)};
     }
     bytes = b;
+  }
+  // alloc(b) whose take from the pool waits on stream s (GPU side) for work
+  // that may still use the pooled buffer; the buffer's own release mode is
+  // kept
+  void alloc_on(size_t b, hipStream_t s) {
+    const bool o0 = ordered;
+    const hipStream_t s0 = stream;
+    ordered = true;
+    stream = s;
+    try {
+      alloc(b);
+    } catch (...) {
+      ordered = o0;
+      stream = s0;
+      throw;
+    }
+    ordered = o0;
+    stream = s0;
   }
   template <typename T>
   T* as() const {
@@ -324,6 +399,10 @@ struct dpf_topology {
   hipEvent_t ev_setup[2] = {nullptr, nullptr};  // GPU time of a call's numeric setup
   // ordering of DPF_DEVICE calls with a NULL stream against the NULL stream
   hipEvent_t ev_null[2] = {nullptr, nullptr};
+  // the end of the last call's work (on whatever stream it ran): every call
+  // starts after it, so the calls on a topology run in order on the GPU
+  hipEvent_t ev_call = nullptr;
+  bool call_seen = false;
   // topology arrays on the device
   DBuf edges, aol, n2f, f2n, inc_ptr, inc_edge, inc_other, inc_slot;
   std::vector<std::unique_ptr<LevelDev>> lev;
@@ -437,13 +516,27 @@ struct dpf_state {
   double tp_fluid[4] = {0, 0, 0, 0};
   int tp_zero_mean = 0;
   explicit dpf_state(dpf_topology* t);
+  std::vector<DBuf*> buffers();
+  // the buffers go back to the pool ordered after the work queued on s (not
+  // ordered: with no work pending)
+  void release_on(hipStream_t s, bool ordered = true);
 };
 
 dpf_state::dpf_state(dpf_topology* t) : topo(t), pool(t->pool), version(0), precision(0) {
-  for (DBuf* b : {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt, &tp_S, &tp_Ksum, &tp_P,
-                  &tp_Vn, &tp_K, &tp_pv, &r_z, &r_al, &r_N, &tr_u, &tr_v, &tr_D, &tr_ud,
-                  &tr_Q, &tr_V, &tr_c})
-    b->pool = pool.get();
+  for (DBuf* b : buffers()) b->pool = pool.get();
+}
+
+std::vector<DBuf*> dpf_state::buffers() {
+  return {&Ke, &dh, &hfree, &lam, &delta, &Q, &svec, &ckpt, &tp_S, &tp_Ksum, &tp_P,
+          &tp_Vn, &tp_K, &tp_pv, &r_z, &r_al, &r_N, &tr_u, &tr_v, &tr_D, &tr_ud,
+          &tr_Q, &tr_V, &tr_c};
+}
+
+void dpf_state::release_on(hipStream_t s, bool ordered) {
+  for (DBuf* b : buffers()) {
+    b->ordered = ordered;
+    b->stream = s;
+  }
 }
 
 namespace {
@@ -457,24 +550,35 @@ void set_err(const std::string& m) { g_err = m; }
 // the call begins (the caller's device inputs may still be being written
 // there), and work the caller queues on the NULL stream after the call
 // returns starts after the call's work -- two event waits, no host
-// synchronisation.  Construct it with the topology's lock held.
+// synchronisation.  Whatever the stream, the call's work starts after the
+// previous call's on the same topology (an event wait on the GPU), since
+// calls share the topology's buffers and a call in DPF_DEVICE mode may
+// return before its work is done -- except on a stream under graph capture,
+// whose caller orders the captured work itself.  Construct it with the
+// topology's lock held.
 struct CallStream {
   dpf_topology* t;
   hipStream_t s;
   bool order_back = false;
+  bool chain = true;
   CallStream(dpf_topology* t_, const dpf_opts* o) : t(t_) {
     if (o && o->memory == DPF_DEVICE && o->stream) {
       s = (hipStream_t)o->stream;
-      return;
+    } else {
+      s = t->own_stream;
+      if (o && o->memory == DPF_DEVICE) {
+        HIPTRY(hipEventRecord(t->ev_null[0], nullptr));
+        HIPTRY(hipStreamWaitEvent(s, t->ev_null[0], 0));
+        order_back = true;
+      }
     }
-    s = t->own_stream;
-    if (o && o->memory == DPF_DEVICE) {
-      HIPTRY(hipEventRecord(t->ev_null[0], nullptr));
-      HIPTRY(hipStreamWaitEvent(s, t->ev_null[0], 0));
-      order_back = true;
-    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone)
+      chain = false;
+    if (chain && t->call_seen) HIPTRY(hipStreamWaitEvent(s, t->ev_call, 0));
   }
   ~CallStream() {
+    if (chain && hipEventRecord(t->ev_call, s) == hipSuccess) t->call_seen = true;
     if (!order_back) return;
     if (hipEventRecord(t->ev_null[1], s) == hipSuccess)
       (void)hipStreamWaitEvent(nullptr, t->ev_null[1], 0);
@@ -1421,6 +1525,7 @@ int create_impl(int64_t n, int64_t nedges, const int64_t* neighbors,
     HIPTRY(hipStreamCreateWithFlags(&t->own_stream, hipStreamNonBlocking));
     for (auto& ev : t->ev_setup) HIPTRY(hipEventCreate(&ev));
     for (auto& ev : t->ev_null) HIPTRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPTRY(hipEventCreateWithFlags(&t->ev_call, hipEventDisableTiming));
     // room for a few forward states (K_e, dh, Q, heads, multipliers) and the
     // per-call temporaries of the host-memory interface
     size_t state_bytes = (size_t)(t->H.ne + 3 * t->H.n + 2 * t->H.nf) * sizeof(double);
@@ -1651,11 +1756,14 @@ int dpf_topology_destroy(dpf_topology* t) {
   if (!t) return DPF_OK;
   {
     std::lock_guard<std::mutex> g(t->mu);
+    // every call's work done, on whatever stream it ran
+    if (t->own_stream && t->call_seen) (void)hipStreamWaitEvent(t->own_stream, t->ev_call, 0);
     if (t->own_stream) (void)hipStreamSynchronize(t->own_stream);
     for (auto& ev : t->ev_setup)
       if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : t->ev_null)
       if (ev) (void)hipEventDestroy(ev);
+    if (t->ev_call) (void)hipEventDestroy(t->ev_call);
     if (t->scal_host) (void)hipHostFree(t->scal_host);
     if (t->hm_host) (void)hipHostFree(t->hm_host);
     if (t->hm_outer_host) (void)hipHostFree(t->hm_outer_host);
@@ -2293,7 +2401,7 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
     }
     ktiming_begin(t, o);
     const double nn = (double)t->H.n, ne = (double)t->H.ne;
-    DBuf kb(t->pool.get());
+    DBuf kb(t->pool.get(), s);  // back to the pool ordered after k_hm_forward
     if (geometric) {
       klaunch(t, KC_KMAP, -1, 8, ne * 16.0 + nn * 8.0, k_geo_forward, dim3(nblocks(t->H.ne)),
               dim3(BLK), 0, s, t->H.ne, t->edges.as<int2>(), lk, out);
@@ -2308,10 +2416,9 @@ int kmap_forward(dpf_topology* t, bool geometric, const double* logK, double* Ke
     HIPTRY(hipGetLastError());
     if (o->memory == DPF_HOST)
       HIPTRY(hipMemcpyAsync(Ke, out, t->H.ne * sizeof(double), hipMemcpyDeviceToHost, s));
-    // in every memory mode: the pooled exp(logK) buffer goes back to the pool
-    // when this call returns, and a later call on another stream may take it
-    // while k_hm_forward would still be reading it
-    HIPTRY(hipStreamSynchronize(s));
+    // host memory: K_e has reached the caller's array when the call returns;
+    // device memory: the call returns with its work queued
+    if (o->memory == DPF_HOST) HIPTRY(hipStreamSynchronize(s));
     ktiming_end(t);
     return DPF_OK;
   });
@@ -2332,7 +2439,7 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
     hipStream_t s = cs.s;
     const double* lk = geometric ? nullptr : stage_in(t, o, t->stage_n, logK, t->H.n, s);
     const double* ke = stage_in(t, o, t->stage_e, Ke, t->H.ne, s);
-    DBuf dkb(t->pool.get());
+    DBuf dkb(t->pool.get(), s);
     const double* dk = stage_in(t, o, dkb, dKe, t->H.ne, s);
     double* out = dlogK;
     if (o->memory == DPF_HOST) {
@@ -2341,7 +2448,7 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
     }
     ktiming_begin(t, o);
     const double nn = (double)t->H.n, ninc = (double)t->H.inc_ptr.back();
-    DBuf kb(t->pool.get());
+    DBuf kb(t->pool.get(), s);
     if (geometric) {
       klaunch(t, KC_KMAP, -1, 8, ninc * 4.0 + ninc * 16.0 + nn * 16.0, k_geo_pullback,
               dim3(nblocks(t->H.n)), dim3(BLK), 0, s, t->H.n, t->inc_ptr.as<int64_t>(),
@@ -2356,9 +2463,10 @@ int kmap_pullback(dpf_topology* t, bool geometric, const double* logK, const dou
               t->inc_other.as<int32_t>(), (const double*)kb.as<double>(), ke, dk, out);
     }
     HIPTRY(hipGetLastError());
-    if (o->memory == DPF_HOST)
+    if (o->memory == DPF_HOST) {
       HIPTRY(hipMemcpyAsync(dlogK, out, t->H.n * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPTRY(hipStreamSynchronize(s));
+      HIPTRY(hipStreamSynchronize(s));
+    }
     ktiming_end(t);
     return DPF_OK;
   });
@@ -2463,9 +2571,10 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
     HIPTRY(hipEventRecord(t->ev_setup[1], s));
     std::unique_ptr<dpf_state> st;
     double* hfree;
-    DBuf tmp(t->pool.get());
+    DBuf tmp(t->pool.get(), s);
     if (state_out) {
       st = std::make_unique<dpf_state>(t);
+      st->release_on(s);  // should the state be freed here: behind this call's work
       st->hfree.alloc(nf * sizeof(double));
       hfree = st->hfree.as<double>();
     } else {
@@ -2494,12 +2603,17 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
         std::swap(pr.first->p, pr.second->p);
         std::swap(pr.first->bytes, pr.second->bytes);
       }
-      t->Ke.alloc(ne * sizeof(double));
-      t->dh.alloc(n * sizeof(double));
-      t->Q.alloc(n * sizeof(double));
+      t->Ke.alloc_on(ne * sizeof(double), s);
+      t->dh.alloc_on(n * sizeof(double), s);
+      t->Q.alloc_on(n * sizeof(double), s);
     }
-    HIPTRY(hipStreamSynchronize(s));
+    // host memory: the heads have reached the caller's array when the call
+    // returns; device memory: the call returns with its work queued (the
+    // solve loop has already seen it converge), and the next call on the
+    // topology, or the state's release, is ordered after it on the GPU
+    if (o->memory == DPF_HOST) HIPTRY(hipStreamSynchronize(s));
     if (stats) {
+      // (the setup's events completed before the solve converged)
       float setup = 0.f;
       HIPTRY(hipEventElapsedTime(&setup, t->ev_setup[0], t->ev_setup[1]));
       stats->total_ms = now_ms() - t0;
@@ -2509,7 +2623,10 @@ int dpf_steady_forward(dpf_topology* t, const double* Ks, const double* dirichle
     ktiming_end(t);
     if (status == DPF_ENOCONV) set_err("AMG-PCG did not reach the tolerance within maxiter");
     if (status == DPF_EBREAKDOWN) set_err("AMG-PCG breakdown (non-SPD operator or NaN input)");
-    if (state_out && status == DPF_OK) *state_out = st.release();
+    if (state_out && status == DPF_OK) {
+      st->release_on(nullptr, false);  // dpf_state_destroy orders its release
+      *state_out = st.release();
+    }
     return status;
   });
 }
@@ -2546,7 +2663,7 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
       st->version = t->version;
     }
     const double* dl = stage_in(t, o, t->stage_n, delta, n, s);
-    st->lam.alloc(nf * sizeof(double));
+    st->lam.alloc_on(nf * sizeof(double), s);
     t->stage_f3.alloc(nf * sizeof(double));
     double* rhs = t->stage_f3.as<double>();
     klaunch(t, KC_SCATTER, -1, 8, (double)nf * 20.0, k_gather_free, dim3(nblocks(nf)), dim3(BLK),
@@ -2582,8 +2699,8 @@ int dpf_steady_pullback(dpf_state* st, const double* delta, double* dKs, double*
       if (dKs) HIPTRY(hipMemcpyAsync(dKs, dK, ne * sizeof(double), hipMemcpyDeviceToHost, s));
       if (ddh) HIPTRY(hipMemcpyAsync(ddh, dd, n * sizeof(double), hipMemcpyDeviceToHost, s));
       if (dQs) HIPTRY(hipMemcpyAsync(dQs, dq, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPTRY(hipStreamSynchronize(s));
     }
-    HIPTRY(hipStreamSynchronize(s));
     if (stats) {
       stats->setup_ms = t1 - t0;
       stats->solve_ms = t2 - t1;
@@ -2625,8 +2742,18 @@ int dpf_kernel_table(dpf_topology* t, dpf_kernel_record* out, int32_t cap, int32
 int dpf_state_destroy(dpf_state* st) {
   if (!st) return DPF_OK;
   if (st->topo) {
-    std::lock_guard<std::mutex> g(st->topo->mu);
-    (void)hipStreamSynchronize(st->topo->own_stream);
+    // the state's buffers go back to the pool ordered after every call made
+    // so far on its topology (a call in DPF_DEVICE mode may return before its
+    // work is done): on the topology's stream, behind the last call's end
+    dpf_topology* t = st->topo;
+    std::lock_guard<std::mutex> g(t->mu);
+    bool ordered = !t->call_seen || hipStreamWaitEvent(t->own_stream, t->ev_call, 0) == hipSuccess;
+    if (ordered)
+      st->release_on(t->own_stream);
+    else
+      (void)hipStreamSynchronize(t->own_stream);
+    delete st;
+    return DPF_OK;
   }
   delete st;
   return DPF_OK;

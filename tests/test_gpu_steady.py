@@ -601,6 +601,88 @@ def test_user_stream_call_then_topology_stream_call(pkg, api, oracle):
         hip.hipStreamDestroy(stream)
 
 
+def test_device_calls_on_two_streams_need_no_host_sync(pkg, api, oracle):
+    """Device-memory calls return with their work queued, and a topology's
+    calls run in order on the GPU whatever streams they name: logK map and
+    forward solve on stream A, the adjoint at once on stream B, the logK
+    pullback at once on A again (reading B's dK_e), with no host
+    synchronisation in between, on two alternating fields (a stale read of the
+    other field's buffers would show).  Results equal the oracle's."""
+    import ctypes
+
+    from dpfehm_jl_amd import _lib
+    _say("two streams, no host sync")
+    L = _lib.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+
+    def ok(code):
+        assert code == 0, code
+
+    w = pkg.workloads.steady_2m(ns=(64, 64, 32), maxs=(32.0, 32.0, 16.0))
+    topo = api.get_topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
+    n, ne = w.n, topo.ne
+    delta = np.zeros(n)
+    delta[w.obs - 1] = np.linspace(1.0, 2.0, w.obs.size)
+    fields = [np.ascontiguousarray(w.logK), np.ascontiguousarray(w.logK[::-1])]
+    refs = []
+    for lk in fields:
+        K = oracle.hm_forward(lk, w.neighbors)
+        h_ref, pb_ref = oracle.steadystate_rrule(K, w.neighbors, w.areasoverlengths,
+                                                 w.dirichletnodes, w.dirichleths, w.Qs,
+                                                 reltol=1e-14)
+        dK_ref = pb_ref(delta)[1]
+        refs.append((h_ref, dK_ref, oracle.hm_pullback(lk, w.neighbors, dK_ref)))
+    sA, sB = vp(), vp()
+    ok(hip.hipStreamCreateWithFlags(ctypes.byref(sA), 1))
+    ok(hip.hipStreamCreateWithFlags(ctypes.byref(sB), 1))
+    bufs = {}
+    try:
+        for name, nb in (("lk0", 8 * n), ("lk1", 8 * n), ("Ke", 8 * ne), ("dh", 8 * n),
+                         ("Q", 8 * n), ("h", 8 * n), ("delta", 8 * n), ("dK", 8 * ne),
+                         ("dl", 8 * n)):
+            bufs[name] = vp()
+            ok(hip.hipMalloc(ctypes.byref(bufs[name]), sz(nb)))
+        for name, a in (("lk0", fields[0]), ("lk1", fields[1]), ("dh", w.dirichleths),
+                        ("Q", w.Qs), ("delta", delta)):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            ok(hip.hipMemcpy(bufs[name], a.ctypes.data_as(vp), sz(a.nbytes), 1))
+        oA = api._opts({"precision": "fp64", "reltol": 1e-12}, memory=_lib.DPF_DEVICE, stream=sA)
+        oB = api._opts({"precision": "fp64", "reltol": 1e-12}, memory=_lib.DPF_DEVICE, stream=sB)
+        for it in range(4):
+            k = it % 2
+            lk = bufs[f"lk{k}"]
+            for name, nb, s in (("h", 8 * n, sA), ("dK", 8 * ne, sB), ("dl", 8 * n, sA)):
+                ok(hip.hipMemsetAsync(bufs[name], 0, sz(nb), s))
+            _lib.check(L.dpf_hm_forward(topo.handle, lk, bufs["Ke"], ctypes.byref(oA)))
+            st, sh = _lib.Stats(), vp()
+            _lib.check(L.dpf_steady_forward(topo.handle, bufs["Ke"], bufs["dh"], bufs["Q"],
+                                            bufs["h"], ctypes.byref(oA), ctypes.byref(st),
+                                            ctypes.byref(sh)))
+            st2 = _lib.Stats()
+            code = L.dpf_steady_pullback(sh, bufs["delta"], bufs["dK"], None, None,
+                                         ctypes.byref(oB), ctypes.byref(st2))
+            L.dpf_state_destroy(sh)
+            _lib.check(code)
+            _lib.check(L.dpf_hm_pullback(topo.handle, lk, bufs["Ke"], bufs["dK"], bufs["dl"],
+                                         ctypes.byref(oA)))
+            ok(hip.hipStreamSynchronize(sA))
+            ok(hip.hipStreamSynchronize(sB))
+            out = {}
+            for name, m in (("h", n), ("dK", ne), ("dl", n)):
+                out[name] = np.empty(m)
+                ok(hip.hipMemcpy(out[name].ctypes.data_as(vp), bufs[name], sz(8 * m), 2))
+            h_ref, dK_ref, dl_ref = refs[k]
+            assert rel(out["h"], h_ref) < 1e-10
+            assert rel(out["dK"], dK_ref) < 1e-8
+            assert rel(out["dl"], dl_ref) < 1e-8
+    finally:
+        for b in bufs.values():
+            hip.hipFree(b)
+        hip.hipStreamDestroy(sA)
+        hip.hipStreamDestroy(sB)
+
+
 def test_null_stream_device_calls_follow_null_stream_order(pkg, api, oracle):
     """Device-memory calls with a NULL stream run in the order of the legacy
     NULL stream: a pullback whose seed the caller writes on the NULL stream
