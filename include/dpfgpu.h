@@ -13,7 +13,13 @@
  * is NULL, on the topology's own stream, ordered after the work already on
  * the legacy NULL stream when the call starts and ahead of the NULL-stream
  * work queued after it returns, as if the call ran on the NULL stream).
- * Calls on one topology are serialised by the library.
+ * Calls on one topology are serialised by the library, and their work runs
+ * in call order on the GPU whatever streams they name.  With DPF_HOST the
+ * outputs are in place when a call returns; with DPF_DEVICE the steady
+ * forward and adjoint solves (which wait for their own convergence) and the
+ * logK maps return with the rest of their work queued, and the outputs are
+ * ready once the call's stream has reached it (synchronise it, or queue the
+ * consumer on it).
  */
 #ifndef DPFGPU_H
 #define DPFGPU_H
