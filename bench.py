@@ -60,9 +60,9 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="steady2m and ensemble: replay each PCG iteration as a captured "
                          "hipGraph (DPF_FLAG_GRAPH)")
-    ap.add_argument("--tuning", default=None, metavar="FINE,COARSE,RATIO",
+    ap.add_argument("--tuning", default=None, metavar="FINE,SECOND,DEEPER,RATIO",
                     help="steady2m: fix the Chebyshev degrees and interval ratio of every "
-                         "precision instead of autotuning (e.g. 1,2,15, to repeat a tuned "
+                         "precision instead of autotuning (e.g. 1,1,3,15, to repeat a tuned "
                          "run's configuration under a profiler, whose overheads change the "
                          "autotuner's timings)")
     ap.add_argument("--autotune", action="store_true",
@@ -113,9 +113,9 @@ def main():
     t0 = time.time()
     topo = api.Topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
     if args.tuning:
-        f, c, r = args.tuning.split(",")
+        f, c, d, r = args.tuning.split(",")
         for prec in ("fp64", "mixed", "mixed_pc"):
-            topo.set_tuning(prec, int(f), int(c), float(r))
+            topo.set_smoother(prec, int(f), int(c), int(d), float(r))
     if args.lanes:
         split = [l for l, g in enumerate(topo.lanes()[0]) if g > 0]
         for l, g in zip(split, args.lanes.split(",")):
@@ -278,7 +278,7 @@ def main():
 
     tuned = topo.tuned_degrees()
     lanes, lane_trials = topo.lanes()
-    log(f"[bench] Chebyshev degrees (finest, coarse, ratio): {tuned}; layout {topo.layout()}; "
+    log(f"[bench] Chebyshev degrees (finest, second, deeper, ratio): {tuned}; layout {topo.layout()}; "
         f"lanes per row by level {[1 << g if g else 0 for g in lanes]}; lane trials (us) "
         f"{ {f'{l}/{1 << k}': round(v, 2) for (l, k), v in lane_trials.items()} }")
     value = args.steps / results["mixed"]["seconds"]
@@ -325,7 +325,7 @@ def main():
         "topology_setup_s": round(topo_s, 3),
         "numeric_setup_ms_per_solve": {
             k: round(results[k]["stats"][-1][0]["setup_ms"], 3) for k in ("fp64", "mixed")},
-        "cheb_tuned_finest_coarse_ratio": tuned,
+        "cheb_degrees_finest_second_deeper_ratio": tuned,
         "cheb_pinned": bool(args.tuning),
         "autotuned": args.autotune,
         "lanes_per_row_by_level": [1 << g if g else 0 for g in lanes],

@@ -154,8 +154,8 @@ class Topology:
         self.lock = threading.Lock()
 
     def info(self):
-        info = np.zeros(18, dtype=np.int64)
-        _lib.lib().dpf_topology_info(self._h, _ptr(info), 18)
+        info = np.zeros(21, dtype=np.int64)
+        _lib.lib().dpf_topology_info(self._h, _ptr(info), 21)
         return info
 
     def layout(self):
@@ -164,12 +164,12 @@ class Topology:
         return {"fine_offset_slices": int(self.info()[12])}
 
     def tuned_degrees(self):
-        """{precision: (finest degree, coarse degree, interval ratio)} in use
-        (defaults, pinned or autotuned)."""
+        """{precision: (finest degree, second-level degree, deeper levels'
+        degree, interval ratio)} in use (defaults, pinned or autotuned)."""
         i = self.info()
-        return {"fp64": (int(i[8]), int(i[9]), int(i[15])),
-                "mixed": (int(i[10]), int(i[11]), int(i[16])),
-                "mixed_pc": (int(i[13]), int(i[14]), int(i[17]))}
+        return {"fp64": (int(i[8]), int(i[9]), int(i[18]), int(i[15])),
+                "mixed": (int(i[10]), int(i[11]), int(i[19]), int(i[16])),
+                "mixed_pc": (int(i[13]), int(i[14]), int(i[20]), int(i[17]))}
 
     def lanes(self):
         """(lanes exponent per level, {(level, exponent): sweep us}): each
@@ -189,8 +189,9 @@ class Topology:
 
     def tuning(self, precision="fp64"):
         """The autotuner's trials for ``precision``: [(ms, iterations)] for
-        the degree pairs (1,1), (1,2), (2,1), (2,2), (3,2) at interval ratio
-        15, then the best pair at ratios 30 and 8 (empty before tuning)."""
+        the degree triples (finest, second, deeper levels) (1,1,3), (1,1,2),
+        (1,1,4), (1,2,2), (1,2,3) at interval ratio 15, then the best triple at
+        ratios 30 and 8 (empty before tuning)."""
         ms = np.zeros(7)
         its = np.zeros(7, dtype=np.int32)
         _lib.check(_lib.lib().dpf_topology_tuning(self._h, _lib.precision_code(precision),
@@ -198,10 +199,17 @@ class Topology:
         return [(float(a), int(b)) for a, b in zip(ms, its)] if its.any() else []
 
     def set_tuning(self, precision, fine, coarse, ratio):
-        """Fix the Chebyshev degrees (finest level, coarser levels) and the
-        interval ratio of ``precision`` instead of autotuning them."""
+        """Fix the Chebyshev degrees (finest level, every coarser level) and
+        the interval ratio of ``precision`` instead of autotuning them."""
         _lib.check(_lib.lib().dpf_topology_set_tuning(self._h, _lib.precision_code(precision),
                                                       int(fine), int(coarse), float(ratio)))
+
+    def set_smoother(self, precision, fine, second, deeper, ratio):
+        """Fix the Chebyshev degrees (finest level, second level, every level
+        below it) and the interval ratio of ``precision``."""
+        _lib.check(_lib.lib().dpf_topology_set_smoother(
+            self._h, _lib.precision_code(precision), int(fine), int(second), int(deeper),
+            float(ratio)))
 
     @property
     def handle(self):

@@ -455,11 +455,15 @@ struct dpf_topology {
   // Chebyshev interval [1/ratio, 1] of D_l1^-1 A: default, and tuned per precision
   double cheb_ratio = 15.0;
   double ratio_tuned[3] = {15.0, 15.0, 15.0};
-  // Chebyshev degrees of the finest / coarser levels per precision (index =
-  // precision: FP64, mixed refinement, FP64 Krylov with FP32 V-cycle): the
-  // fixed default 1/2 unless autotuned (DPF_FLAG_AUTOTUNE) or pinned, and
-  // the tuner's record
-  int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {2, 2, 2};
+  // Chebyshev degrees of the finest level, the second level and every deeper
+  // level per precision (index = precision: FP64, mixed refinement, FP64
+  // Krylov with FP32 V-cycle): the fixed default 1/1/3 unless autotuned
+  // (DPF_FLAG_AUTOTUNE) or pinned, and the tuner's record.  The second level
+  // holds about half the finest level's nonzeros and its sweeps are
+  // bandwidth-bound, so it gets one sweep a V-cycle; the deeper levels are
+  // small enough that a sweep costs about a launch, and their stronger
+  // smoothing is what keeps the V-cycle's convergence.
+  int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {1, 1, 1}, deg_deep[3] = {3, 3, 3};
   bool tuned[3] = {false, false, false};
   // lanes per row of the coarse levels' split copies: the autotuner's sweep
   // times per level and lane exponent 0..6 (0 = not timed), and whether the
@@ -1789,16 +1793,18 @@ int dpf_topology_tuning(const dpf_topology* t, int32_t precision, double* ms, in
   return DPF_OK;
 }
 
-int dpf_topology_set_tuning(dpf_topology* t, int32_t precision, int32_t deg_fine,
-                            int32_t deg_coarse, double ratio) {
+int dpf_topology_set_smoother(dpf_topology* t, int32_t precision, int32_t deg_fine,
+                              int32_t deg_second, int32_t deg_deeper, double ratio) {
   if (!t || precision < DPF_FP64 || precision > DPF_MIXED_PC || deg_fine < 1 || deg_fine > 16 ||
-      deg_coarse < 1 || deg_coarse > 16 || !(ratio > 1.0) || !std::isfinite(ratio)) {
+      deg_second < 1 || deg_second > 16 || deg_deeper < 1 || deg_deeper > 16 ||
+      !(ratio > 1.0) || !std::isfinite(ratio)) {
     set_err("need a precision code, degrees in 1..16 and an interval ratio > 1");
     return DPF_EINVAL;
   }
   std::lock_guard<std::mutex> g(t->mu);
   t->deg_fine[precision] = deg_fine;
-  t->deg_coarse[precision] = deg_coarse;
+  t->deg_coarse[precision] = deg_second;
+  t->deg_deep[precision] = deg_deeper;
   t->ratio_tuned[precision] = ratio;
   for (int k = 0; k < 7; k++) {
     t->tune_ms[precision][k] = 0.0;
@@ -1806,6 +1812,11 @@ int dpf_topology_set_tuning(dpf_topology* t, int32_t precision, int32_t deg_fine
   }
   t->tuned[precision] = true;
   return DPF_OK;
+}
+
+int dpf_topology_set_tuning(dpf_topology* t, int32_t precision, int32_t deg_fine,
+                            int32_t deg_coarse, double ratio) {
+  return dpf_topology_set_smoother(t, precision, deg_fine, deg_coarse, deg_coarse, ratio);
 }
 
 int dpf_topology_lanes(const dpf_topology* t, int32_t* lg, double* us, int32_t nlevels) {
@@ -1861,7 +1872,7 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
   const TopologyHost& H = t->H;
   int64_t nnz_all = 0;
   for (auto& l : H.levels) nnz_all += l.A.nnz;
-  int64_t v[18] = {H.n,
+  int64_t v[21] = {H.n,
                   H.ne,
                   H.nf,
                   H.levels[0].A.nnz,
@@ -1878,8 +1889,11 @@ int dpf_topology_info(const dpf_topology* t, int64_t* info, int32_t ninfo) {
                   t->deg_coarse[2],
                   (int64_t)t->ratio_tuned[0],
                   (int64_t)t->ratio_tuned[1],
-                  (int64_t)t->ratio_tuned[2]};
-  for (int i = 0; i < ninfo && i < 18; i++) info[i] = v[i];
+                  (int64_t)t->ratio_tuned[2],
+                  t->deg_deep[0],
+                  t->deg_deep[1],
+                  t->deg_deep[2]};
+  for (int i = 0; i < ninfo && i < 21; i++) info[i] = v[i];
   return DPF_OK;
 }
 
@@ -1921,9 +1935,10 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   t->sweep_n = 0;
   int pidx = o->precision;
   int L = t->levels();
-  auto make_chs = [&](int m0, int mc, double ratio) {
+  // degree m0 on the finest level, m1 on the second, md on every deeper one
+  auto make_chs = [&](int m0, int m1, int md, double ratio) {
     std::vector<Cheb> v(L);
-    for (int l = 0; l < L; l++) v[l].make(l == 0 ? m0 : mc, 1.0, ratio);
+    for (int l = 0; l < L; l++) v[l].make(l == 0 ? m0 : l == 1 ? m1 : md, 1.0, ratio);
     return v;
   };
   // smoothers: explicit degree (every level, default interval), or the
@@ -1934,8 +1949,9 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
   bool tune = autotune && !t->tuned[pidx];
   std::vector<Cheb> chs =
       o->cheb_degree > 0
-          ? make_chs(o->cheb_degree, o->cheb_degree, t->cheb_ratio)
-          : make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
+          ? make_chs(o->cheb_degree, o->cheb_degree, o->cheb_degree, t->cheb_ratio)
+          : make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->deg_deep[pidx],
+                     t->ratio_tuned[pidx]);
   double* part = t->partial.as<double>();
   SolveScalars* sc = t->scal.as<SolveScalars>();
   LevelDev& d0 = *t->lev[0];
@@ -2062,12 +2078,13 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
     }
     SolveResult r;
     if (tune) {
-      // Autotuner: time each (finest, coarse) Chebyshev degree pair on this
-      // very system at the default interval, then two other intervals at the
-      // best pair, and keep the fastest; the result is fixed for the life of
-      // the topology, so its later solves are bitwise reproducible (another
-      // process may time differently: it repeats a tuned configuration by
-      // pinning it, dpf_topology_set_tuning and dpf_topology_set_lanes).  An untimed
+      // Autotuner: time each (finest, second, deeper levels) Chebyshev degree
+      // triple on this very system at the default interval, then two other
+      // intervals at the best triple, and keep the fastest; the result is fixed
+      // for the life of the topology, so its later solves are bitwise
+      // reproducible (another process may time differently: it repeats a tuned
+      // configuration by pinning it, dpf_topology_set_smoother and
+      // dpf_topology_set_lanes).  An untimed
       // solve first brings the kernels' code and the clocks up, and the FP32
       // trials of DPF_MIXED solve to inner_reltol^2 rather than inner_reltol:
       // the whole refinement's worth of convergence, where a cheaper V-cycle
@@ -2076,13 +2093,13 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       bool graphs_on = t->use_graphs, ktiming_on = t->ktiming, timing_on = t->timing;
       t->use_graphs = false;  // trials use launches: a capture would distort their times
       t->ktiming = t->timing = false;  // nor do the trials enter the timing records
-      static const int cand[][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
+      static const int cand[][3] = {{1, 1, 3}, {1, 1, 2}, {1, 1, 4}, {1, 2, 2}, {1, 2, 3}};
       static const double ratios[] = {30.0, 8.0};
       double trial_tol = std::max(o->inner_reltol * o->inner_reltol, 1e-12);
       double best = 1e300;
-      (void)krylov(make_chs(1, 2, t->cheb_ratio), trial_tol);
-      auto trial = [&](int m0, int mc, double ratio, int slot) {
-        std::vector<Cheb> cc = make_chs(m0, mc, ratio);
+      (void)krylov(make_chs(1, 1, 3, t->cheb_ratio), trial_tol);
+      auto trial = [&](int m0, int m1, int md, double ratio, int slot) {
+        std::vector<Cheb> cc = make_chs(m0, m1, md, ratio);
         HIPTRY(hipStreamSynchronize(s));
         double ta = now_ms();
         SolveResult tr = krylov(cc, trial_tol);
@@ -2095,19 +2112,21 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
         if (tr.status == 1 && el < best) {
           best = el;
           t->deg_fine[pidx] = m0;
-          t->deg_coarse[pidx] = mc;
+          t->deg_coarse[pidx] = m1;
+          t->deg_deep[pidx] = md;
           t->ratio_tuned[pidx] = ratio;
         }
       };
-      for (int c = 0; c < 5; c++) trial(cand[c][0], cand[c][1], t->cheb_ratio, c);
-      int m0 = t->deg_fine[pidx], mc = t->deg_coarse[pidx];
-      for (int k = 0; k < 2; k++) trial(m0, mc, ratios[k], 5 + k);
+      for (int c = 0; c < 5; c++) trial(cand[c][0], cand[c][1], cand[c][2], t->cheb_ratio, c);
+      int m0 = t->deg_fine[pidx], m1 = t->deg_coarse[pidx], md = t->deg_deep[pidx];
+      for (int k = 0; k < 2; k++) trial(m0, m1, md, ratios[k], 5 + k);
       t->tuned[pidx] = true;
       t->use_graphs = graphs_on;
       t->ktiming = ktiming_on;
       t->timing = timing_on;
       tune = false;
-      chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->ratio_tuned[pidx]);
+      chs = make_chs(t->deg_fine[pidx], t->deg_coarse[pidx], t->deg_deep[pidx],
+                     t->ratio_tuned[pidx]);
     }
     r = krylov(chs, o->inner_reltol);
     if (o->precision != DPF_MIXED)
@@ -4120,7 +4139,9 @@ void rich_matrices(dpf_topology* t, hipStream_t s, const RichIn& in, const doubl
 
 std::vector<Cheb> rich_chebs(dpf_topology* t) {
   std::vector<Cheb> v(t->levels());
-  for (int l = 0; l < t->levels(); l++) v[l].make(l == 0 ? 1 : 2, 1.0, 15.0);
+  // the steady solve's default smoother: degree 1 on the finest and the
+  // second level, 3 below
+  for (int l = 0; l < t->levels(); l++) v[l].make(l == 0 ? 1 : l == 1 ? 1 : 3, 1.0, 15.0);
   return v;
 }
 

@@ -73,7 +73,7 @@ enum { DPF_HOST = 0, DPF_DEVICE = 1 };
  * topology) and the Chebyshev degrees and interval, and keeps the fastest
  * for the topology's life (dpf_topology_lanes, dpf_topology_tuning report
  * the trials).  Without it the smoother and layout are fixed defaults
- * (degrees 1 and 2, ratio 15, ~8 entries a lane) or what was pinned, so
+ * (degrees 1, 1 and 3, ratio 15, ~8 entries a lane) or what was pinned, so
  * results do not depend on timing. */
 enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4,
        DPF_FLAG_FP32_STORAGE = 8, DPF_FLAG_AUTOTUNE = 16 };
@@ -88,8 +88,9 @@ typedef struct {
                           to half the target instead */
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
   int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = the
-                          topology's degrees: default (1 finest, 2 coarser),
-                          pinned, or autotuned with DPF_FLAG_AUTOTUNE */
+                          topology's degrees: default (1 on the finest and the
+                          second level, 3 below), pinned, or autotuned with
+                          DPF_FLAG_AUTOTUNE */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
   int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
   void *stream;        /* hipStream_t for DPF_DEVICE calls, NULL = NULL-stream order */
@@ -150,33 +151,42 @@ int dpf_batch_topology_create(int32_t nmembers, const int64_t *ncells, const int
  * single system). */
 int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers);
 
-/* info[0..17] = n, nedges, nfree, nnz(A), levels, coarse rows,
+/* info[0..20] = n, nedges, nfree, nnz(A), levels, coarse rows,
  * operator-complexity*1000, host symbolic setup time in microseconds, the
- * Chebyshev degrees (finest, coarse) in use for DPF_FP64 and for DPF_MIXED
- * (the default 1 and 2 unless tuned or pinned), the fine operator's 64-row
- * slices stored as column offsets (no column indices), the degrees for
- * DPF_MIXED_PC, then the Chebyshev interval ratio r (interval [lmax/r, lmax];
- * default 15) for DPF_FP64, DPF_MIXED and DPF_MIXED_PC. */
+ * Chebyshev degrees (finest level, second level) in use for DPF_FP64 and for
+ * DPF_MIXED (the default 1 and 1 unless tuned or pinned), the fine operator's
+ * 64-row slices stored as column offsets (no column indices), the degrees
+ * for DPF_MIXED_PC, then the Chebyshev interval ratio r (interval
+ * [lmax/r, lmax]; default 15) for DPF_FP64, DPF_MIXED and DPF_MIXED_PC, then
+ * the degree of the levels below the second (default 3) for the same three. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* The Chebyshev autotuner's trials for one precision (DPF_FP64, DPF_MIXED or
  * DPF_MIXED_PC), zeros unless a solve with DPF_FLAG_AUTOTUNE tuned it:
  * ms[k] / iters[k] for k < min(n, 7) = wall time and PCG iterations of trial
- * k, which solves the first refinement step's system with (finest, coarse)
- * degrees (1,1), (1,2), (2,1), (2,2), (3,2) at interval ratio 15 (k = 0..4),
- * then with the best of these at ratios 30 and 8 (k = 5, 6); -1 ms marks a
- * trial that did not converge.  No counterpart in the reference (its AMG
- * smoother is fixed). */
+ * k, which solves the first refinement step's system with (finest, second,
+ * deeper levels) degrees (1,1,3), (1,1,2), (1,1,4), (1,2,2), (1,2,3) at
+ * interval ratio 15 (k = 0..4), then with the best of these at ratios 30 and
+ * 8 (k = 5, 6); -1 ms marks a trial that did not converge.  No counterpart in
+ * the reference (its AMG smoother is fixed). */
 int dpf_topology_tuning(const dpf_topology *topo, int32_t precision, double *ms,
                         int32_t *iters, int32_t n);
 
 /* Fixes the smoother of one precision on this topology, in place of the
  * autotuner (which then does not run for it): Chebyshev degree deg_fine on
- * the finest level and deg_coarse below (1..16), interval [lmax/ratio, lmax],
- * ratio > 1.  For runs that must not depend on timing, e.g. to repeat a tuned
- * configuration under a profiler.  No counterpart in the reference. */
+ * the finest level and deg_coarse on every level below (1..16), interval
+ * [lmax/ratio, lmax], ratio > 1.  For runs that must not depend on timing,
+ * e.g. to repeat a tuned configuration under a profiler.  No counterpart in
+ * the reference. */
 int dpf_topology_set_tuning(dpf_topology *topo, int32_t precision, int32_t deg_fine,
                             int32_t deg_coarse, double ratio);
+
+/* As dpf_topology_set_tuning with three degrees: deg_fine on the finest
+ * level, deg_second on the second level, deg_deeper on every level below it
+ * (each 1..16; the default smoother is 1, 1, 3 at ratio 15).  No counterpart
+ * in the reference. */
+int dpf_topology_set_smoother(dpf_topology *topo, int32_t precision, int32_t deg_fine,
+                              int32_t deg_second, int32_t deg_deeper, double ratio);
 
 /* Lanes per row of each level's row-split copy, the layout the coarse
  * levels' smoothing sweeps and residuals read (2^lg[l] lanes share row r;

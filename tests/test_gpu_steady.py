@@ -474,10 +474,11 @@ def test_uge_mesh_geometric_dirichlet_faces(pkg, api, oracle, precision, tmp_pat
 
 
 def test_pinned_tuning(pkg, api):
-    """dpf_topology_set_tuning fixes the smoother instead of the autotuner:
-    the topology reports the pinned degrees and no trials, two topologies
-    pinned alike give bitwise-equal heads, and the heads agree with an
-    autotuned solve to the solve tolerance."""
+    """dpf_topology_set_tuning / _set_smoother fix the smoother instead of
+    the autotuner: the topology reports the pinned degrees and no trials, two
+    topologies pinned alike give bitwise-equal heads, set_tuning's coarse
+    degree covers every coarse level, and the heads agree with an autotuned
+    solve to the solve tolerance."""
     import ctypes
     _say("pinned tuning")
     from dpfehm_jl_amd import _lib
@@ -492,12 +493,17 @@ def test_pinned_tuning(pkg, api):
         _lib.check(_lib.lib().dpf_steady_forward(
             topo.handle, api._ptr(api._f64(K)), api._ptr(api._f64(w.dirichleths)),
             api._ptr(api._f64(w.Qs)), api._ptr(h), ctypes.byref(o), None, None))
-        assert topo.tuned_degrees()["fp64"] == (2, 1, 8) and topo.tuning("fp64") == []
+        assert topo.tuned_degrees()["fp64"] == (2, 1, 1, 8) and topo.tuning("fp64") == []
+        assert topo.tuned_degrees()["mixed"] == (1, 1, 3, 15)
+        topo.set_smoother("mixed", 2, 3, 4, 10.0)
+        assert topo.tuned_degrees()["mixed"] == (2, 3, 4, 10)
         hs.append(h)
         with pytest.raises(_lib.DpfError):
             topo.set_tuning("fp64", 0, 1, 8.0)
         with pytest.raises(_lib.DpfError):
             topo.set_tuning("mixed", 1, 2, 1.0)
+        with pytest.raises(_lib.DpfError):
+            topo.set_smoother("mixed", 1, 1, 17, 15.0)
         topo.close()
     assert np.array_equal(hs[0], hs[1])
     h_auto = api.groundwater_steadystate(K, w.neighbors, w.areasoverlengths, w.dirichletnodes,

@@ -5,7 +5,7 @@
   and gradients.
 * DPF_FLAG_AUTOTUNE times the coarse levels' lanes per row and the smoother
   degrees on the topology's first solve; a second process that pins the
-  tuned configuration (dpf_topology_set_tuning, dpf_topology_set_lanes)
+  tuned configuration (dpf_topology_set_smoother, dpf_topology_set_lanes)
   reproduces the tuned process's results bit for bit.
 * Every lanes-per-row layout gives the same heads to the solve tolerance.
 
@@ -37,8 +37,8 @@ print(json.dumps({"case": cfg["case"]}), flush=True)
 w = P.workloads.steady_2m(ns=tuple(cfg["ns"]), maxs=tuple(cfg["maxs"]))
 K = api.logKs2Ks_harmonic(w.logK, w.neighbors, w.dirichletnodes, w.areasoverlengths)
 topo = api.get_topology(w.n, w.neighbors, w.areasoverlengths, w.dirichletnodes)
-for prec, (f, c, r) in cfg.get("pin", {}).items():
-    topo.set_tuning(prec, f, c, r)
+for prec, (f, c, d, r) in cfg.get("pin", {}).items():
+    topo.set_smoother(prec, f, c, d, r)
 for level, lg in cfg.get("lanes", []):
     topo.set_lanes(level, lg)
 out = {"case": cfg["case"]}
@@ -76,7 +76,7 @@ def test_default_bitwise_across_processes():
     a = _child(cfg)
     b = _child(cfg)
     assert a["lane_trials"] == 0 and b["lane_trials"] == 0
-    assert a["degrees"]["fp64"] == [1, 2, 15] and a["degrees"]["mixed"] == [1, 2, 15]
+    assert a["degrees"]["fp64"] == [1, 1, 3, 15] and a["degrees"]["mixed"] == [1, 1, 3, 15]
     for prec in ("fp64", "mixed"):
         assert a[prec] == b[prec], prec
 
@@ -86,9 +86,9 @@ def test_autotuned_configuration_pinned_in_another_process():
     2 pins what it chose and reproduces its results bit for bit."""
     tuned = _child(dict(CFG, case="autotune", precisions=["mixed"], autotune=True))
     assert tuned["lane_trials"] > 0
-    f, c, r = tuned["degrees"]["mixed"]
+    f, c, d, r = tuned["degrees"]["mixed"]
     pinned = _child(dict(CFG, case="pinned", precisions=["mixed"],
-                         pin={"mixed": [f, c, float(r)]}, lanes=tuned["lanes"]))
+                         pin={"mixed": [f, c, d, float(r)]}, lanes=tuned["lanes"]))
     assert pinned["lanes"] == tuned["lanes"]
     assert pinned["mixed"] == tuned["mixed"]
 
