@@ -821,8 +821,9 @@ __global__ void k_jacobian(int kind, int64_t nf, const int32_t* __restrict__ f2n
 
 // ------------------------------------------------------------------ AMG numeric setup
 
-// P(i,:) = e_agg(i) - w invd_i sum_k a_ik e_agg(k), w = 4/3 (l1 diagonal,
-// whose spectral radius bound is 1).
+// P(i,:) = e_agg(i) - omega invd_i sum_k a_ik e_agg(k) (invd: the l1 diagonal,
+// whose spectral radius bound is 1; omega = 4 / (3 rho) with the caller's
+// estimate rho of rho(D_l1^-1 A)).
 // p16 (may be null): P's values also in bf16.
 template <typename T>
 __device__ __forceinline__ void put_val(T* val, uint16_t* v16, int64_t e, T v) {
@@ -834,7 +835,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict__ invd,
                                                  const uint16_t* __restrict__ p_slot,
                                                  const uint16_t* __restrict__ p_self, SellD<T> P,
-                                                 uint16_t* __restrict__ p16) {
+                                                 uint16_t* __restrict__ p16, double omega) {
   // the lane's P row accumulates in LDS ([slot][lane]: no bank conflicts)
   // instead of through read-modify-writes of global memory
   constexpr int PW = 16;
@@ -845,7 +846,7 @@ __global__ void __launch_bounds__(BLK) k_psmooth(SellD<T> A, const T* __restrict
   int pw, aw;
   sell_row(P, i, pb, pw);
   sell_row(A, i, ab, aw);
-  T f = (T)(4.0 / 3.0) * invd[i];
+  T f = (T)omega * invd[i];
   int tid = threadIdx.x;
   if (pw <= PW) {
     for (int k = 0; k < pw; k++) acc[k][tid] = (T)0;

@@ -17,7 +17,8 @@
 //                                                         (src/groundwater.jl:93-101)
 //   dJ/dlogK_c = sum_{e at c} dK_e K_e k_other / (k_c + k_other)
 // Same method as the GPU library: aggregation of the matrix graph (every
-// coupling strong, three-phase), P = (I - 4/3 D_l1^-1 A) T, R = P', Galerkin
+// coupling strong, three-phase), P = (I - omega D_l1^-1 A) T (omega 1.5 on
+// the finest level, 2 below), R = P', Galerkin
 // coarse operators, l1-Jacobi Chebyshev smoothing (degree 1 on the finest
 // and the second level, 3 below, interval [1/15, 1]), dense Cholesky of the
 // coarsest level.
@@ -322,7 +323,9 @@ int refresh(Handle& H) {
     for (I i = 0; i < n; i++) {
       double* pr = L.P.v.data() + L.P.rp[i];
       for (I q = L.P.rp[i]; q < L.P.rp[i + 1]; q++) pr[q - L.P.rp[i]] = 0.0;
-      double f = (4.0 / 3.0) * L.invd[i];
+      // omega = 4 / (3 rho) with the estimates rho = 0.89 (finest level) and
+      // 0.67 (Galerkin levels) of rho(D_l1^-1 A), as the GPU library uses
+      double f = (l == 0 ? 1.5 : 2.0) * L.invd[i];
       for (I p = L.A.rp[i]; p < L.A.rp[i + 1]; p++) pr[L.pslot[p]] -= f * L.A.v[p];
       pr[L.pself[i]] += 1.0;
     }
