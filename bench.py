@@ -62,7 +62,7 @@ def main():
                          "hipGraph (DPF_FLAG_GRAPH)")
     ap.add_argument("--tuning", default=None, metavar="FINE,SECOND,DEEPER,RATIO",
                     help="steady2m: fix the Chebyshev degrees and interval ratio of every "
-                         "precision instead of autotuning (e.g. 1,1,3,15, to repeat a tuned "
+                         "precision instead of autotuning (e.g. 1,1,2,15, to repeat a tuned "
                          "run's configuration under a profiler, whose overheads change the "
                          "autotuner's timings)")
     ap.add_argument("--autotune", action="store_true",

@@ -189,7 +189,7 @@ class Topology:
 
     def tuning(self, precision="fp64"):
         """The autotuner's trials for ``precision``: [(ms, iterations)] for
-        the degree triples (finest, second, deeper levels) (1,1,3), (1,1,2),
+        the degree triples (finest, second, deeper levels) (1,1,2), (1,1,3),
         (1,1,4), (1,2,2), (1,2,3) at interval ratio 15, then the best triple at
         ratios 30 and 8 (empty before tuning)."""
         ms = np.zeros(7)

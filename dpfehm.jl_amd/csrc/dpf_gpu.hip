@@ -457,13 +457,13 @@ struct dpf_topology {
   double ratio_tuned[3] = {15.0, 15.0, 15.0};
   // Chebyshev degrees of the finest level, the second level and every deeper
   // level per precision (index = precision: FP64, mixed refinement, FP64
-  // Krylov with FP32 V-cycle): the fixed default 1/1/3 unless autotuned
+  // Krylov with FP32 V-cycle): the fixed default 1/1/2 unless autotuned
   // (DPF_FLAG_AUTOTUNE) or pinned, and the tuner's record.  The second level
   // holds about half the finest level's nonzeros and its sweeps are
   // bandwidth-bound, so it gets one sweep a V-cycle; the deeper levels are
   // small enough that a sweep costs about a launch, and their stronger
   // smoothing is what keeps the V-cycle's convergence.
-  int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {1, 1, 1}, deg_deep[3] = {3, 3, 3};
+  int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {1, 1, 1}, deg_deep[3] = {2, 2, 2};
   bool tuned[3] = {false, false, false};
   // lanes per row of the coarse levels' split copies: the autotuner's sweep
   // times per level and lane exponent 0..6 (0 = not timed), and whether the
@@ -2101,11 +2101,11 @@ int solve_system(dpf_topology* t, hipStream_t s, const double* b, double* xout,
       bool graphs_on = t->use_graphs, ktiming_on = t->ktiming, timing_on = t->timing;
       t->use_graphs = false;  // trials use launches: a capture would distort their times
       t->ktiming = t->timing = false;  // nor do the trials enter the timing records
-      static const int cand[][3] = {{1, 1, 3}, {1, 1, 2}, {1, 1, 4}, {1, 2, 2}, {1, 2, 3}};
+      static const int cand[][3] = {{1, 1, 2}, {1, 1, 3}, {1, 1, 4}, {1, 2, 2}, {1, 2, 3}};
       static const double ratios[] = {30.0, 8.0};
       double trial_tol = std::max(o->inner_reltol * o->inner_reltol, 1e-12);
       double best = 1e300;
-      (void)krylov(make_chs(1, 1, 3, t->cheb_ratio), trial_tol);
+      (void)krylov(make_chs(1, 1, 2, t->cheb_ratio), trial_tol);
       auto trial = [&](int m0, int m1, int md, double ratio, int slot) {
         std::vector<Cheb> cc = make_chs(m0, m1, md, ratio);
         HIPTRY(hipStreamSynchronize(s));
@@ -4147,9 +4147,11 @@ void rich_matrices(dpf_topology* t, hipStream_t s, const RichIn& in, const doubl
 
 std::vector<Cheb> rich_chebs(dpf_topology* t) {
   std::vector<Cheb> v(t->levels());
-  // the steady solve's default smoother: degree 1 on the finest and the
-  // second level, 3 below
-  for (int l = 0; l < t->levels(); l++) v[l].make(l == 0 ? 1 : l == 1 ? 1 : 3, 1.0, 15.0);
+  // degree 1 on the finest and the second level, 3 below (the BiCGStab
+  // iteration of the Newton steps gains more from the deeper levels'
+  // smoothing than the steady CG does: 33 + 14 against 36 + 15 iterations
+  // with degree 2 below at 2M cells)
+  for (int l = 0; l < t->levels(); l++) v[l].make(l <= 1 ? 1 : 3, 1.0, 15.0);
   return v;
 }
 

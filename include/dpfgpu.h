@@ -73,7 +73,7 @@ enum { DPF_HOST = 0, DPF_DEVICE = 1 };
  * topology) and the Chebyshev degrees and interval, and keeps the fastest
  * for the topology's life (dpf_topology_lanes, dpf_topology_tuning report
  * the trials).  Without it the smoother and layout are fixed defaults
- * (degrees 1, 1 and 3, ratio 15, ~8 entries a lane) or what was pinned, so
+ * (degrees 1, 1 and 2, ratio 15, ~8 entries a lane) or what was pinned, so
  * results do not depend on timing. */
 enum { DPF_FLAG_SWEEP_TIMING = 1, DPF_FLAG_GRAPH = 2, DPF_FLAG_KERNEL_TIMING = 4,
        DPF_FLAG_FP32_STORAGE = 8, DPF_FLAG_AUTOTUNE = 16 };
@@ -89,7 +89,7 @@ typedef struct {
   int32_t max_outer;   /* DPF_MIXED: refinement step cap */
   int32_t cheb_degree; /* Chebyshev smoother degree on every level; 0 = the
                           topology's degrees: default (1 on the finest and the
-                          second level, 3 below), pinned, or autotuned with
+                          second level, 2 below), pinned, or autotuned with
                           DPF_FLAG_AUTOTUNE */
   int32_t memory;      /* DPF_HOST or DPF_DEVICE */
   int32_t flags;       /* DPF_FLAG_* bits, 0 by default */
@@ -158,14 +158,14 @@ int dpf_member_stats(const dpf_topology *topo, double *relres, int32_t nmembers)
  * 64-row slices stored as column offsets (no column indices), the degrees
  * for DPF_MIXED_PC, then the Chebyshev interval ratio r (interval
  * [lmax/r, lmax]; default 15) for DPF_FP64, DPF_MIXED and DPF_MIXED_PC, then
- * the degree of the levels below the second (default 3) for the same three. */
+ * the degree of the levels below the second (default 2) for the same three. */
 int dpf_topology_info(const dpf_topology *topo, int64_t *info, int32_t ninfo);
 
 /* The Chebyshev autotuner's trials for one precision (DPF_FP64, DPF_MIXED or
  * DPF_MIXED_PC), zeros unless a solve with DPF_FLAG_AUTOTUNE tuned it:
  * ms[k] / iters[k] for k < min(n, 7) = wall time and PCG iterations of trial
  * k, which solves the first refinement step's system with (finest, second,
- * deeper levels) degrees (1,1,3), (1,1,2), (1,1,4), (1,2,2), (1,2,3) at
+ * deeper levels) degrees (1,1,2), (1,1,3), (1,1,4), (1,2,2), (1,2,3) at
  * interval ratio 15 (k = 0..4), then with the best of these at ratios 30 and
  * 8 (k = 5, 6); -1 ms marks a trial that did not converge.  No counterpart in
  * the reference (its AMG smoother is fixed). */
@@ -183,7 +183,7 @@ int dpf_topology_set_tuning(dpf_topology *topo, int32_t precision, int32_t deg_f
 
 /* As dpf_topology_set_tuning with three degrees: deg_fine on the finest
  * level, deg_second on the second level, deg_deeper on every level below it
- * (each 1..16; the default smoother is 1, 1, 3 at ratio 15).  No counterpart
+ * (each 1..16; the default smoother is 1, 1, 2 at ratio 15).  No counterpart
  * in the reference. */
 int dpf_topology_set_smoother(dpf_topology *topo, int32_t precision, int32_t deg_fine,
                               int32_t deg_second, int32_t deg_deeper, double ratio);

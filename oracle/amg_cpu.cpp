@@ -20,7 +20,7 @@
 // coupling strong, three-phase), P = (I - omega D_l1^-1 A) T (omega 1.5 on
 // the finest level, 2 below), R = P', Galerkin
 // coarse operators, l1-Jacobi Chebyshev smoothing (degree 1 on the finest
-// and the second level, 3 below, interval [1/15, 1]), dense Cholesky of the
+// and the second level, 2 below, interval [1/15, 1]), dense Cholesky of the
 // coarsest level.
 // The conductivity-independent part (aggregates, every sparsity pattern) is
 // built once per mesh; each evaluation redoes all numeric work.  OpenMP over
@@ -557,7 +557,7 @@ int acpu_eval(void* h, const double* logK, const double* dh, const double* Q,
   }
   if (refresh(*H)) return 1;
   std::vector<Cheb> chs(H->lev.size());
-  for (size_t l = 0; l < chs.size(); l++) chs[l].make(l <= 1 ? 1 : 3, 15.0);
+  for (size_t l = 0; l < chs.size(); l++) chs[l].make(l <= 1 ? 1 : 2, 15.0);
   double t1 = now_ms();
   std::vector<double> hf(nf), hfull(n);
   int it1 = pcg(*H, chs, b.data(), hf.data(), reltol, 5000);

@@ -494,7 +494,7 @@ def test_pinned_tuning(pkg, api):
             topo.handle, api._ptr(api._f64(K)), api._ptr(api._f64(w.dirichleths)),
             api._ptr(api._f64(w.Qs)), api._ptr(h), ctypes.byref(o), None, None))
         assert topo.tuned_degrees()["fp64"] == (2, 1, 1, 8) and topo.tuning("fp64") == []
-        assert topo.tuned_degrees()["mixed"] == (1, 1, 3, 15)
+        assert topo.tuned_degrees()["mixed"] == (1, 1, 2, 15)
         topo.set_smoother("mixed", 2, 3, 4, 10.0)
         assert topo.tuned_degrees()["mixed"] == (2, 3, 4, 10)
         hs.append(h)

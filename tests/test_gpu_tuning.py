@@ -76,7 +76,7 @@ def test_default_bitwise_across_processes():
     a = _child(cfg)
     b = _child(cfg)
     assert a["lane_trials"] == 0 and b["lane_trials"] == 0
-    assert a["degrees"]["fp64"] == [1, 1, 3, 15] and a["degrees"]["mixed"] == [1, 1, 3, 15]
+    assert a["degrees"]["fp64"] == [1, 1, 2, 15] and a["degrees"]["mixed"] == [1, 1, 2, 15]
     for prec in ("fp64", "mixed"):
         assert a[prec] == b[prec], prec
 
