@@ -69,7 +69,11 @@ def test_c1_dlogK_100k(pkg, api, oracle, precision, tol_h):
 @pytest.mark.parametrize("sigma", [1.0, 2.0, 3.0])
 def test_c3_2m_sigma_sweep(pkg, api, sigma):
     """C3 (and C4's objective): the 2M-cell problem with logK of standard
-    deviation sigma, FP64 and mixed precision, each to relres 1e-10."""
+    deviation sigma, FP64 and mixed precision, each to relres 1e-10.  The
+    default hierarchy (smoother degrees 1 / 1 / 2, prolongator damping 1.5 /
+    2) converges in 18-20 PCG iterations a solve here; the bound (21 FP64, 22
+    mixed, summed over refinement steps) fails on a weaker hierarchy, e.g. the
+    damping 4/3 with degree 2 on the second level (21-23 / 23)."""
     _say("c3 sigma", sigma)
     w = pkg.workloads.steady_2m(sigma=sigma)
     args = (w.neighbors, w.areasoverlengths, w.dirichletnodes, w.dirichleths, w.Qs)
@@ -102,6 +106,8 @@ def test_c3_2m_sigma_sweep(pkg, api, sigma):
         pb.state.close()
         res[precision] = (h, g)
         _say("c3", sigma, precision, st["iterations"], st2["iterations"], st["relres"], indep)
+        cap = 21 if precision == "fp64" else 22
+        assert st["iterations"] <= cap and st2["iterations"] <= cap, (st, st2)
     eh = rel(res["mixed"][0], res["fp64"][0])
     eg = rel(res["mixed"][1], res["fp64"][1])
     _say("c3", sigma, "mixed vs fp64", eh, eg)
