@@ -99,6 +99,7 @@ SIGNATURES = {
     "dpf_topology_tuning": (ctypes.c_int, [_p, _i32, _p, _p, _i32]),
     "dpf_topology_set_tuning": (ctypes.c_int, [_p, _i32, _i32, _i32, ctypes.c_double]),
     "dpf_topology_set_smoother": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, ctypes.c_double]),
+    "dpf_topology_set_damping": (ctypes.c_int, [_p, ctypes.c_double, ctypes.c_double]),
     "dpf_topology_lanes": (ctypes.c_int, [_p, _p, _p, _i32]),
     "dpf_topology_set_lanes": (ctypes.c_int, [_p, _i32, _i32]),
     "dpf_twophase_tpfa": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _i32, _p, _p]),

@@ -204,6 +204,12 @@ class Topology:
         _lib.check(_lib.lib().dpf_topology_set_tuning(self._h, _lib.precision_code(precision),
                                                       int(fine), int(coarse), float(ratio)))
 
+    def set_damping(self, fine, coarse):
+        """Damping of the prolongator smoothing on the finest level and on
+        the coarser levels (default 1.5 and 2; 4/3 on both can suit fields
+        that are rough from cell to cell)."""
+        _lib.check(_lib.lib().dpf_topology_set_damping(self._h, float(fine), float(coarse)))
+
     def set_smoother(self, precision, fine, second, deeper, ratio):
         """Fix the Chebyshev degrees (finest level, second level, every level
         below it) and the interval ratio of ``precision``."""

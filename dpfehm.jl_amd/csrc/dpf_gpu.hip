@@ -464,6 +464,15 @@ struct dpf_topology {
   // small enough that a sweep costs about a launch, and their stronger
   // smoothing is what keeps the V-cycle's convergence.
   int deg_fine[3] = {1, 1, 1}, deg_coarse[3] = {1, 1, 1}, deg_deep[3] = {2, 2, 2};
+  // Damping of the prolongator smoothing P = (I - omega D_l1^-1 A) T on the
+  // finest level and on the Galerkin levels: omega = 4 / (3 rho) with a fixed
+  // estimate rho of rho(D_l1^-1 A) instead of its bound 1 (rho = 0.89, omega
+  // 1.5; rho = 0.67, omega 2, where the l1 diagonal overestimates the
+  // spectrum more).  No eigenvalue estimate runs at set-up.  On spatially
+  // correlated conductivity fields these take fewer PCG iterations than 4/3
+  // (2M cells: 18-19 against 20-22); on fields rough from cell to cell, 4/3
+  // can take fewer, and dpf_topology_set_damping sets other values.
+  double omega_fine = 1.5, omega_coarse = 2.0;
   bool tuned[3] = {false, false, false};
   // lanes per row of the coarse levels' split copies: the autotuner's sweep
   // times per level and lane exponent 0..6 (0 = not timed), and whether the
@@ -781,14 +790,6 @@ void assemble(dpf_topology* t, hipStream_t s, bool want32, const double* Q, bool
   HIPTRY(hipGetLastError());
 }
 
-// Damping of the prolongator smoothing P = (I - omega D_l1^-1 A) T, omega =
-// 4 / (3 rho) with a fixed estimate rho of rho(D_l1^-1 A) instead of its bound
-// 1: rho = 0.89 on the finest level (omega 1.5) and 0.67 on the Galerkin
-// levels (omega 2), whose l1 diagonal overestimates the spectrum more.  No
-// eigenvalue estimate runs at set-up; on the 2M-cell, 100k-cell and ensemble
-// workloads these take the PCG iterations from 20-22 to 18-19 against 4/3.
-constexpr double kOmegaFine = 1.5, kOmegaCoarse = 2.0;
-
 // Numeric refresh of levels 1.. and the coarsest inverse, from level 0 values.
 template <typename T>
 void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
@@ -810,7 +811,7 @@ void refresh(dpf_topology* t, Hier<T>& h, hipStream_t s) {
             z.nnz * (w + 2.0) + z.rows * (w + 2.0) + z.pnnz * (w + 2.0 * b16w),
             k_psmooth<T>, dim3(nblocks(d.A.nrows)), dim3(BLK), 0, s, A,
             (const T*)level_invd(t, h, l), d.p_slot.as<uint16_t>(), d.p_self.as<uint16_t>(), P,
-            h.lev[l].p16.template as<uint16_t>(), l == 0 ? kOmegaFine : kOmegaCoarse);
+            h.lev[l].p16.template as<uint16_t>(), l == 0 ? t->omega_fine : t->omega_coarse);
     klaunch(t, KC_RGATHER, l, (int)w, z.pnnz * (4.0 + 2.0 * w + 2.0 * b16w), k_rgather<T>,
             dim3(nblocks(d.Rs.entries)), dim3(BLK), 0, s, d.Rs.entries, d.rs_src.as<int32_t>(),
             (const T*)P.val, h.lev[l].rval.template as<T>(), (const uint16_t*)nullptr,
@@ -1819,6 +1820,20 @@ int dpf_topology_set_smoother(dpf_topology* t, int32_t precision, int32_t deg_fi
     t->tune_iters[precision][k] = 0;
   }
   t->tuned[precision] = true;
+  return DPF_OK;
+}
+
+int dpf_topology_set_damping(dpf_topology* t, double omega_fine, double omega_coarse) {
+  if (!t || !(omega_fine > 0.0 && omega_fine < 4.0) || !(omega_coarse > 0.0 && omega_coarse < 4.0)) {
+    set_err("need a topology and damping factors in (0, 4)");
+    return DPF_EINVAL;
+  }
+  std::lock_guard<std::mutex> g(t->mu);
+  t->omega_fine = omega_fine;
+  t->omega_coarse = omega_coarse;
+  // the hierarchies refreshed so far used the old damping: kept states
+  // refresh again before their adjoint solves
+  t->version++;
   return DPF_OK;
 }
 

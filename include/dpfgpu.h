@@ -188,6 +188,15 @@ int dpf_topology_set_tuning(dpf_topology *topo, int32_t precision, int32_t deg_f
 int dpf_topology_set_smoother(dpf_topology *topo, int32_t precision, int32_t deg_fine,
                               int32_t deg_second, int32_t deg_deeper, double ratio);
 
+/* Damping of the AMG prolongator smoothing P = (I - omega D_l1^-1 A) T on
+ * the finest level (omega_fine) and on every coarser level (omega_coarse),
+ * each in (0, 4); default 1.5 and 2, which suit spatially correlated
+ * conductivity fields.  Fields that are rough from cell to cell can converge
+ * in fewer iterations with 4/3 on both.  Applies from the next numeric
+ * refresh; states kept by earlier forward solves refresh again before their
+ * adjoint solves.  No counterpart in the reference. */
+int dpf_topology_set_damping(dpf_topology *topo, double omega_fine, double omega_coarse);
+
 /* Lanes per row of each level's row-split copy, the layout the coarse
  * levels' smoothing sweeps and residuals read (2^lg[l] lanes share row r;
  * lg[l] = 0: level l has no split copy), for levels 0..nlevels-1; us (may be
